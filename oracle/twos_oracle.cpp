@@ -1,0 +1,781 @@
+// CPU oracle for the TWoS estimator -- TEST INFRASTRUCTURE ONLY.
+//
+// This file restates the reference estimator in plain C++17 + OpenMP, binary64,
+// no contraction.  It is the checker for the HIP library, never part of it:
+// only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+// Reference citations are paths under the hertjo/walking-into-holes tree.
+//
+// Parity of each part is pinned by tests (see twos_oracle.h for the list).
+
+#include "twos_oracle.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <random>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+struct V3 {
+  double x = 0.0, y = 0.0, z = 0.0;
+};
+inline V3 operator+(const V3 &a, const V3 &b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator-(const V3 &a, const V3 &b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator*(const V3 &a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline double dot(const V3 &a, const V3 &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline double norm(const V3 &a) { return std::sqrt(dot(a, a)); }
+inline V3 load3(const double *p) { return {p[0], p[1], p[2]}; }
+inline double comp(const V3 &a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+
+constexpr double kInf = std::numeric_limits<double>::infinity();
+
+// splitmix64 finalizer: benchmarks/compiled_bunny/src/wih_twos.cpp:529-534.
+uint64_t splitmix64(uint64_t x) {
+  x += 0x9e3779b97f4a7c15ULL;
+  x = (x ^ (x >> 30U)) * 0xbf58476d1ce4e5b9ULL;
+  x = (x ^ (x >> 27U)) * 0x94d049bb133111ebULL;
+  return x ^ (x >> 31U);
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy
+// as 1, 2, 3", SC'11): ten rounds of two 32x32->64 multiplies with a Weyl key
+// schedule.  Multipliers and Weyl increments are the published constants.
+void philox4x32_10(const uint32_t in[4], const uint32_t key_in[2], uint32_t out[4]) {
+  uint32_t c0 = in[0], c1 = in[1], c2 = in[2], c3 = in[3];
+  uint32_t k0 = key_in[0], k1 = key_in[1];
+  for (int round = 0; round < 10; ++round) {
+    const uint64_t p0 = uint64_t(0xD2511F53u) * c0;
+    const uint64_t p1 = uint64_t(0xCD9E8D57u) * c2;
+    const uint32_t hi0 = uint32_t(p0 >> 32), lo0 = uint32_t(p0);
+    const uint32_t hi1 = uint32_t(p1 >> 32), lo1 = uint32_t(p1);
+    c0 = hi1 ^ c1 ^ k0;
+    c1 = lo1;
+    c2 = hi0 ^ c3 ^ k1;
+    c3 = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// Stream identity of one (seed, site, side): a 64-bit key for Philox.
+void philox_key(uint64_t seed, int64_t site, int32_t side, uint32_t key[2]) {
+  const uint64_t k = splitmix64(splitmix64(seed) + (uint64_t(site) << 1 | uint64_t(side & 1)));
+  key[0] = uint32_t(k);
+  key[1] = uint32_t(k >> 32);
+}
+
+inline double u53(uint32_t lo, uint32_t hi) {
+  return double(((uint64_t(hi) << 32) | lo) >> 11) * 0x1.0p-53;
+}
+
+constexpr double kTwoPi = 6.283185307179586;  // == 2 * acos(-1.0) in binary64
+
+// Direction for draw block `step` of walk `walk` under `key`: one Philox block
+// per step, u1 from words (0,1), u2 from words (2,3).  The same z / phi / radial
+// construction as random_unit_vector (wih_twos.cpp:536-542).
+V3 philox_direction(const uint32_t key[2], uint32_t walk, uint32_t step) {
+  const uint32_t ctr[4] = {step, walk, 0u, 0u};
+  uint32_t w[4];
+  philox4x32_10(ctr, key, w);
+  const double z = 1.0 - 2.0 * u53(w[0], w[1]);
+  const double phi = kTwoPi * u53(w[2], w[3]);
+  const double radial = std::sqrt(std::max(0.0, 1.0 - z * z));
+  return {radial * std::cos(phi), radial * std::sin(phi), z};
+}
+
+// random_unit_vector: wih_twos.cpp:536-542 (libstdc++ uniform_real_distribution).
+V3 random_unit_vector(std::mt19937_64 &rng) {
+  std::uniform_real_distribution<double> uniform(0.0, 1.0);
+  const double z = 1.0 - 2.0 * uniform(rng);
+  const double phi = 2.0 * std::acos(-1.0) * uniform(rng);
+  const double radial = std::sqrt(std::max(0.0, 1.0 - z * z));
+  return {radial * std::cos(phi), radial * std::sin(phi), z};
+}
+
+// Ericson's closest point on a triangle: wih_twos.cpp:247-289.
+V3 closest_point_triangle(const V3 &p, const V3 &a, const V3 &b, const V3 &c) {
+  const V3 ab = b - a;
+  const V3 ac = c - a;
+  const V3 ap = p - a;
+  const double d1 = dot(ab, ap);
+  const double d2 = dot(ac, ap);
+  if (d1 <= 0.0 && d2 <= 0.0) return a;
+  const V3 bp = p - b;
+  const double d3 = dot(ab, bp);
+  const double d4 = dot(ac, bp);
+  if (d3 >= 0.0 && d4 <= d3) return b;
+  const double vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+    const double v = d1 / (d1 - d3);
+    return a + ab * v;
+  }
+  const V3 cp = p - c;
+  const double d5 = dot(ab, cp);
+  const double d6 = dot(ac, cp);
+  if (d6 >= 0.0 && d5 <= d6) return c;
+  const double vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+    const double w = d2 / (d2 - d6);
+    return a + ac * w;
+  }
+  const double va = d3 * d6 - d5 * d4;
+  if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+    const V3 bc = c - b;
+    const double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    return b + bc * w;
+  }
+  const double denom = 1.0 / (va + vb + vc);
+  const double v = vb * denom;
+  const double w = vc * denom;
+  return a + ab * v + ac * w;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Triangle mesh with the oracle's own BVH (median split, <=4 triangles/leaf).
+// The closest-point query returns the exact minimum over all triangles of
+// |p - closest_point_triangle|, ties broken by the lowest triangle index, so it
+// equals a brute-force scan bit for bit.  The reference's Embree query
+// (wih_twos.cpp:339-359, :416-430) only prunes and computes the same binary64
+// distance in its callback.
+struct og_mesh {
+  struct Node {
+    double lo[3], hi[3];
+    int32_t left, right;  // internal: children; leaf: left = first, right = -count
+  };
+  std::vector<V3> v;
+  std::vector<std::array<uint32_t, 3>> f;
+  std::vector<uint32_t> order;
+  std::vector<Node> nodes;
+
+  void bounds_of(uint32_t t, double lo[3], double hi[3]) const {
+    for (int k = 0; k < 3; ++k) { lo[k] = kInf; hi[k] = -kInf; }
+    for (int j = 0; j < 3; ++j) {
+      const V3 &q = v[f[t][j]];
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(lo[k], comp(q, k));
+        hi[k] = std::max(hi[k], comp(q, k));
+      }
+    }
+  }
+
+  int32_t build(uint32_t begin, uint32_t end) {
+    Node node;
+    for (int k = 0; k < 3; ++k) { node.lo[k] = kInf; node.hi[k] = -kInf; }
+    double clo[3] = {kInf, kInf, kInf}, chi[3] = {-kInf, -kInf, -kInf};
+    for (uint32_t i = begin; i < end; ++i) {
+      double lo[3], hi[3];
+      bounds_of(order[i], lo, hi);
+      for (int k = 0; k < 3; ++k) {
+        node.lo[k] = std::min(node.lo[k], lo[k]);
+        node.hi[k] = std::max(node.hi[k], hi[k]);
+        const double c = 0.5 * (lo[k] + hi[k]);
+        clo[k] = std::min(clo[k], c);
+        chi[k] = std::max(chi[k], c);
+      }
+    }
+    const int32_t index = int32_t(nodes.size());
+    nodes.push_back(node);
+    if (end - begin <= 4) {
+      nodes[index].left = int32_t(begin);
+      nodes[index].right = -int32_t(end - begin);
+      return index;
+    }
+    int axis = 0;
+    for (int k = 1; k < 3; ++k)
+      if (chi[k] - clo[k] > chi[axis] - clo[axis]) axis = k;
+    const uint32_t mid = begin + (end - begin) / 2;
+    std::nth_element(order.begin() + begin, order.begin() + mid, order.begin() + end,
+                     [&](uint32_t a, uint32_t b) {
+                       double la[3], ha[3], lb[3], hb[3];
+                       bounds_of(a, la, ha);
+                       bounds_of(b, lb, hb);
+                       const double ca = la[axis] + ha[axis], cb = lb[axis] + hb[axis];
+                       return ca < cb || (ca == cb && a < b);
+                     });
+    const int32_t l = build(begin, mid);
+    const int32_t r = build(mid, end);
+    nodes[index].left = l;
+    nodes[index].right = r;
+    return index;
+  }
+
+  static double box_d2(const Node &n, const V3 &p) {
+    double s = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      const double x = comp(p, k);
+      const double d = x < n.lo[k] ? n.lo[k] - x : (x > n.hi[k] ? x - n.hi[k] : 0.0);
+      s += d * d;
+    }
+    return s;
+  }
+
+  void test_triangle(uint32_t t, const V3 &p, double &best, V3 &best_pt, int64_t &best_tri) const {
+    const V3 q = closest_point_triangle(p, v[f[t][0]], v[f[t][1]], v[f[t][2]]);
+    const V3 d = p - q;
+    const double d2 = dot(d, d);
+    if (d2 < best || (d2 == best && int64_t(t) < best_tri)) {
+      best = d2;
+      best_pt = q;
+      best_tri = t;
+    }
+  }
+
+  double closest(const V3 &p, V3 &out, int64_t &tri, bool brute, int64_t *nv, int64_t *nt) const {
+    double best = kInf;
+    V3 best_pt;
+    int64_t best_tri = std::numeric_limits<int64_t>::max();
+    int64_t visited = 0, tested = 0;
+    if (brute) {
+      for (uint32_t t = 0; t < f.size(); ++t) test_triangle(t, p, best, best_pt, best_tri);
+      tested = int64_t(f.size());
+    } else {
+      int32_t stack[128];
+      int sp = 0;
+      stack[sp++] = 0;
+      while (sp) {
+        const Node &n = nodes[size_t(stack[--sp])];
+        ++visited;
+        // Relative slack keeps rounding in the box bound from pruning a triangle
+        // whose computed distance ties the current best.
+        if (box_d2(n, p) * (1.0 - 1e-12) > best) continue;
+        if (n.right < 0) {
+          for (int32_t i = 0; i < -n.right; ++i) {
+            ++tested;
+            test_triangle(order[size_t(n.left + i)], p, best, best_pt, best_tri);
+          }
+          continue;
+        }
+        const double dl = box_d2(nodes[size_t(n.left)], p);
+        const double dr = box_d2(nodes[size_t(n.right)], p);
+        if (dl <= dr) {
+          stack[sp++] = n.right;
+          stack[sp++] = n.left;
+        } else {
+          stack[sp++] = n.left;
+          stack[sp++] = n.right;
+        }
+      }
+    }
+    if (nv) *nv += visited;
+    if (nt) *nt += tested;
+    out = best_pt;
+    tri = best_tri;
+    return std::sqrt(best);
+  }
+
+  // Closest ray hit with t in [tnear, tfar] (Moller-Trumbore in binary64).
+  bool ray_hit(const V3 &o, const V3 &d, double tnear, double tfar, double &t_hit) const {
+    double best = tfar;
+    bool found = false;
+    int32_t stack[128];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+      const Node &n = nodes[size_t(stack[--sp])];
+      double t0 = tnear, t1 = best;
+      bool miss = false;
+      for (int k = 0; k < 3 && !miss; ++k) {
+        const double ok = comp(o, k), dk = comp(d, k);
+        if (dk == 0.0) {
+          if (ok < n.lo[k] || ok > n.hi[k]) miss = true;
+          continue;
+        }
+        double ta = (n.lo[k] - ok) / dk, tb = (n.hi[k] - ok) / dk;
+        if (ta > tb) std::swap(ta, tb);
+        t0 = std::max(t0, ta - 1e-12);
+        t1 = std::min(t1, tb + 1e-12);
+        if (t0 > t1) miss = true;
+      }
+      if (miss) continue;
+      if (n.right < 0) {
+        for (int32_t i = 0; i < -n.right; ++i) {
+          const auto &tri = f[order[size_t(n.left + i)]];
+          const V3 a = v[tri[0]], e1 = v[tri[1]] - a, e2 = v[tri[2]] - a;
+          const V3 pv{d.y * e2.z - d.z * e2.y, d.z * e2.x - d.x * e2.z, d.x * e2.y - d.y * e2.x};
+          const double det = dot(e1, pv);
+          if (det == 0.0) continue;
+          const double inv = 1.0 / det;
+          const V3 tv = o - a;
+          const double uu = dot(tv, pv) * inv;
+          if (uu < 0.0 || uu > 1.0) continue;
+          const V3 qv{tv.y * e1.z - tv.z * e1.y, tv.z * e1.x - tv.x * e1.z, tv.x * e1.y - tv.y * e1.x};
+          const double vv = dot(d, qv) * inv;
+          if (vv < 0.0 || uu + vv > 1.0) continue;
+          const double t = dot(e2, qv) * inv;
+          if (t >= tnear && t <= best) {
+            best = t;
+            found = true;
+          }
+        }
+        continue;
+      }
+      stack[sp++] = n.left;
+      stack[sp++] = n.right;
+    }
+    t_hit = best;
+    return found;
+  }
+
+  // Ray parity along one fixed irrational direction, stepping from hit to hit:
+  // EmbreeMesh::inside, wih_twos.cpp:361-394 (tnear 1e-6, tfar 4, step 2e-6,
+  // at most 128 crossings).  Returns -1 when the crossing limit is exceeded.
+  int inside(const V3 &p) const {
+    V3 direction{1.0, 0.3713906763541037, 0.1570796326794897};
+    direction = direction * (1.0 / norm(direction));
+    V3 origin = p;
+    unsigned hits = 0;
+    for (; hits < 128; ++hits) {
+      double t;
+      if (!ray_hit(origin, direction, 1.0e-6, 4.0, t)) break;
+      origin = origin + direction * (t + 2.0e-6);
+    }
+    if (hits == 128) return -1;
+    return int(hits % 2U == 1U);
+  }
+};
+
+namespace {
+
+// Closest boundary point of the domain and its boundary class.
+// Mesh + cavities follows domain_distance / closest_boundary
+// (wih_twos.cpp:585-593, :602-626): the mesh first, then each cavity wins only
+// when strictly closer.  Analytic containers and holes are the build's data
+// model (sphere, box and axis-aligned cylinder SDFs), evaluated the same way.
+double closest_boundary(const og_geom &g, const V3 &p, V3 &point, int32_t &cls) {
+  double best;
+  if (g.container == OG_MESH) {
+    int64_t tri;
+    best = g.mesh->closest(p, point, tri, false, nullptr, nullptr);
+    // The retained reference outputs (walk-traces.csv, field-*.csv) were written
+    // by a build whose distance was the float query radius that
+    // point_query_callback stores (wih_twos.cpp:426): replaying them bit for
+    // bit needs the binary32-rounded distance.
+    if (g.mesh_radius_f32) best = double(float(best));
+    cls = 0;
+  } else if (g.container == OG_BALL) {
+    const V3 c{g.ball[0], g.ball[1], g.ball[2]};
+    const V3 d = p - c;
+    const double len = norm(d);
+    best = g.ball[3] - len;
+    point = len > 0.0 ? c + d * (g.ball[3] / len) : V3{c.x + g.ball[3], c.y, c.z};
+    cls = 0;
+  } else {
+    best = kInf;
+    for (int face = 0; face < 6; ++face) {
+      const int k = face >> 1;
+      const double x = comp(p, k);
+      const double dist = (face & 1) ? g.box_hi[k] - x : x - g.box_lo[k];
+      if (dist < best) {
+        best = dist;
+        point = p;
+        const double plane = (face & 1) ? g.box_hi[k] : g.box_lo[k];
+        if (k == 0) point.x = plane; else if (k == 1) point.y = plane; else point.z = plane;
+        cls = face;
+      }
+    }
+  }
+  for (int32_t i = 0; i < g.n_spheres; ++i) {
+    const double *s = g.spheres + 4 * i;
+    const V3 c{s[0], s[1], s[2]};
+    const V3 d = p - c;
+    const double len = norm(d);
+    const double dist = len - s[3];
+    if (dist < best && len > 0.0) {
+      best = dist;
+      point = c + d * (s[3] / len);
+      cls = 6;
+    }
+  }
+  for (int32_t i = 0; i < g.n_cylinders; ++i) {
+    const double *s = g.cylinders + 4 * i;
+    const int axis = int(s[0]);
+    const int ka = axis == 0 ? 1 : 0, kb = axis == 2 ? 1 : 2;
+    const double da = comp(p, ka) - s[1], db = comp(p, kb) - s[2];
+    const double len = std::sqrt(da * da + db * db);
+    const double dist = len - s[3];
+    if (dist < best && len > 0.0) {
+      best = dist;
+      point = p;
+      const double pa = s[1] + da * (s[3] / len), pb = s[2] + db * (s[3] / len);
+      if (ka == 0) point.x = pa; else point.y = pa;
+      if (kb == 1) point.y = pb; else point.z = pb;
+      cls = 6;
+    }
+  }
+  for (int32_t i = 0; i < g.n_cavities; ++i) {
+    const V3 c = load3(g.cavities + 3 * i);
+    const V3 radial = p - c;
+    const double len = norm(radial);
+    const double dist = len - g.cavity_radius;
+    if (dist < best && len > 0.0) {
+      best = dist;
+      point = c + radial * (g.cavity_radius / len);
+      cls = 7;
+    }
+  }
+  return best;
+}
+
+double payoff(const og_side &s, const V3 &y, int32_t cls) {
+  double value = s.cls_const[cls & 7];
+  if (s.cls_poly_mask >> (cls & 7) & 1U) {
+    const double *c = s.poly;
+    value += c[0] + c[1] * y.x + c[2] * y.y + c[3] * y.z + c[4] * y.x * y.x + c[5] * y.y * y.y +
+             c[6] * y.z * y.z + c[7] * y.x * y.y + c[8] * y.x * y.z + c[9] * y.y * y.z;
+  }
+  return value;
+}
+
+// Source of the Green-ball term: the reference's affine source
+// f = 1 + slope * (y + 0.5) (wih_twos.cpp:546-552) is the case s0 = 1,
+// sy = slope, oy = 0.5, every other coefficient zero, and evaluates to the
+// same binary64 value.
+double source(const og_side &s, const V3 &x) {
+  const double *c = s.src;
+  return ((c[0] + c[1] * (x.x + c[4])) + c[2] * (x.y + c[5])) + c[3] * (x.z + c[6]);
+}
+
+struct WalkOut {
+  double a = 0.0, b = 0.0;
+  int32_t steps = 0;
+  bool cap = false;
+};
+
+// One Walk on Spheres path: wih_twos.cpp:635-664.  Each step queries the
+// domain distance r, stops when !(r > eps), adds the Green-ball source
+// (r^2/6) f(x) at the ball centre, and moves to x + r * direction.  A path
+// that takes max_steps steps is capped.  The build adds the Dirichlet payoff
+// g(closest boundary point) at termination (the reference's terminal value
+// is zero; its 2-D harmonic-data experiment projects to the boundary the same
+// way, experiments/run_controlled_experiments.py:115-147).  `two` accumulates
+// a second side's source and payoff along the same path (study.cpp:91-113).
+template <class Dir>
+WalkOut walk(const og_geom &g, const og_side &s0, const og_side *s1, V3 x, double eps,
+             int32_t max_steps, Dir &&next_direction, double *trace, int32_t max_rows,
+             int32_t *rows) {
+  WalkOut out;
+  int32_t row = 0;
+  for (int32_t step = 0; step < max_steps; ++step) {
+    V3 y;
+    int32_t cls;
+    const double radius = closest_boundary(g, x, y, cls);
+    if (trace && row < max_rows) {
+      trace[4 * row + 0] = x.x; trace[4 * row + 1] = x.y;
+      trace[4 * row + 2] = x.z; trace[4 * row + 3] = radius;
+      ++row;
+    }
+    if (!(radius > eps)) {
+      out.a += payoff(s0, y, cls);
+      if (s1) out.b += payoff(*s1, y, cls);
+      if (rows) *rows = row;
+      return out;
+    }
+    if (s0.source_kind == 1) out.a += source(s0, x) * radius * radius / 6.0;
+    if (s1 && s1->source_kind == 1) out.b += source(*s1, x) * radius * radius / 6.0;
+    x = x + next_direction(step) * radius;
+    ++out.steps;
+  }
+  if (trace && row < max_rows) {
+    V3 y;
+    int32_t cls;
+    const double radius = closest_boundary(g, x, y, cls);
+    trace[4 * row + 0] = x.x; trace[4 * row + 1] = x.y;
+    trace[4 * row + 2] = x.z; trace[4 * row + 3] = radius;
+    ++row;
+  }
+  if (rows) *rows = row;
+  out.cap = true;
+  return out;
+}
+
+// Per-walk key of evaluate_site: wih_twos.cpp:681-685.
+uint64_t site_key(uint64_t seed, int32_t repeat, int64_t site, int64_t sample) {
+  return splitmix64(seed + 0x2d358dccaa6c78a5ULL) ^
+         splitmix64(uint64_t(repeat) + 0x8bb84b93962eacc9ULL) ^
+         splitmix64(uint64_t(site) + 0x4b33a62ed433d4a3ULL) ^
+         splitmix64(uint64_t(sample) + 0x4d5a2da51de1aa47ULL);
+}
+
+// One sample (a, b) of walk `w` at site `s`.
+WalkOut sample(const og_geom &g, const og_side sides[2], const og_params &p, const V3 &z,
+               int64_t s, int64_t w) {
+  if (p.rng == OG_RNG_PHILOX) {
+    uint32_t k0[2], k1[2];
+    philox_key(p.seed, s, 0, k0);
+    auto dir0 = [&](int32_t step) { return philox_direction(k0, uint32_t(w), uint32_t(step)); };
+    if (p.combine == OG_SQUARE) {
+      WalkOut o = walk(g, sides[0], nullptr, z, p.eps, p.max_steps, dir0, nullptr, 0, nullptr);
+      o.b = o.a;
+      return o;
+    }
+    if (p.combine == OG_PAIRED)
+      return walk(g, sides[0], &sides[1], z, p.eps, p.max_steps, dir0, nullptr, 0, nullptr);
+    philox_key(p.seed, s, 1, k1);
+    auto dir1 = [&](int32_t step) { return philox_direction(k1, uint32_t(w), uint32_t(step)); };
+    WalkOut a = walk(g, sides[0], nullptr, z, p.eps, p.max_steps, dir0, nullptr, 0, nullptr);
+    WalkOut b = walk(g, sides[1], nullptr, z, p.eps, p.max_steps, dir1, nullptr, 0, nullptr);
+    a.b = b.a;
+    a.steps += b.steps;
+    a.cap = a.cap || b.cap;
+    return a;
+  }
+  std::mt19937_64 rng;
+  if (p.rng == OG_RNG_MT_SITE) {
+    // walk(): std::mt19937_64 rng(splitmix64(seed)) with evaluate_site's key.
+    rng.seed(splitmix64(site_key(p.seed, p.repeat, s, w)));
+  } else {
+    // paired_site: rng(splitmix64(seed)^splitmix64(site+91347)^splitmix64(n+571)),
+    // benchmarks/submission_study/study.cpp:89.
+    rng.seed(splitmix64(p.seed) ^ splitmix64(uint64_t(s) + 91347) ^ splitmix64(uint64_t(w) + 571));
+  }
+  auto dir = [&](int32_t) { return random_unit_vector(rng); };
+  if (p.combine == OG_PAIRED)
+    return walk(g, sides[0], &sides[1], z, p.eps, p.max_steps, dir, nullptr, 0, nullptr);
+  WalkOut o = walk(g, sides[0], nullptr, z, p.eps, p.max_steps, dir, nullptr, 0, nullptr);
+  o.b = o.a;
+  return o;
+}
+
+// Cross-index combine and leave-one-out jackknife over explicit samples,
+// evaluated in the reference's order: wih_twos.cpp:687-715 (a == b) and
+// study.cpp:114-119 (paired).  out = {u, v, score, score_se, same, n}.
+void combine(const double *a, const double *b, int64_t count, bool square, double out[6]) {
+  double sa = 0.0, sb = 0.0, sab = 0.0;
+  for (int64_t i = 0; i < count; ++i) sa += a[i];
+  if (square) {
+    for (int64_t i = 0; i < count; ++i) sab += a[i] * a[i];
+    sb = sa;
+  } else {
+    for (int64_t i = 0; i < count; ++i) sb += b[i];
+    for (int64_t i = 0; i < count; ++i) sab += a[i] * b[i];
+  }
+  const double n = double(count);
+  out[0] = sa / n;
+  out[1] = sb / n;
+  out[2] = (sa * sb - sab) / (n * (n - 1.0));
+  out[4] = sab / n;
+  out[5] = n;
+  if (count >= 4) {
+    std::vector<double> leave(static_cast<size_t>(count));
+    for (int64_t i = 0; i < count; ++i) {
+      const double bi = square ? a[i] : b[i];
+      const double la = sa - a[i];
+      const double lb = sb - bi;
+      const double lab = sab - a[i] * bi;
+      leave[size_t(i)] = (la * lb - lab) / ((n - 1.0) * (n - 2.0));
+    }
+    double mean = 0.0;
+    for (double x : leave) mean += x;
+    mean /= n;
+    double spread = 0.0;
+    for (double x : leave) spread += (x - mean) * (x - mean);
+    out[3] = std::sqrt((n - 1.0) / n * spread);
+  } else {
+    out[3] = std::numeric_limits<double>::quiet_NaN();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+uint64_t og_splitmix64(uint64_t x) { return splitmix64(x); }
+
+void og_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  philox4x32_10(ctr, key, out);
+}
+
+void og_philox_key(uint64_t seed, int64_t site, int32_t side, uint32_t key[2]) {
+  philox_key(seed, site, side, key);
+}
+
+void og_mt19937_64(uint64_t seed, uint64_t *out, int32_t n) {
+  std::mt19937_64 rng(seed);
+  for (int32_t i = 0; i < n; ++i) out[i] = rng();
+}
+
+void og_mt_uniforms(uint64_t seed, double *out, int32_t n) {
+  std::mt19937_64 rng(seed);
+  std::uniform_real_distribution<double> uniform(0.0, 1.0);
+  for (int32_t i = 0; i < n; ++i) out[i] = uniform(rng);
+}
+
+og_mesh *og_mesh_create(const double *vertices, int64_t nv, const uint32_t *faces, int64_t nf) {
+  if (nv <= 0 || nf <= 0) return nullptr;
+  auto *m = new og_mesh;
+  m->v.resize(size_t(nv));
+  for (int64_t i = 0; i < nv; ++i) m->v[size_t(i)] = load3(vertices + 3 * i);
+  m->f.resize(size_t(nf));
+  for (int64_t i = 0; i < nf; ++i) {
+    for (int k = 0; k < 3; ++k) {
+      if (faces[3 * i + k] >= uint64_t(nv)) {
+        delete m;
+        return nullptr;
+      }
+      m->f[size_t(i)][size_t(k)] = faces[3 * i + k];
+    }
+  }
+  m->order.resize(size_t(nf));
+  for (int64_t i = 0; i < nf; ++i) m->order[size_t(i)] = uint32_t(i);
+  m->nodes.reserve(size_t(nf));
+  m->build(0, uint32_t(nf));
+  return m;
+}
+
+void og_mesh_destroy(og_mesh *mesh) { delete mesh; }
+
+double og_mesh_closest(const og_mesh *mesh, const double p[3], double out_point[3], int64_t *out_tri,
+                       int32_t brute_force, int64_t *nodes_visited, int64_t *tris_tested) {
+  V3 q;
+  int64_t tri;
+  const double d = mesh->closest(load3(p), q, tri, brute_force != 0, nodes_visited, tris_tested);
+  if (out_point) { out_point[0] = q.x; out_point[1] = q.y; out_point[2] = q.z; }
+  if (out_tri) *out_tri = tri;
+  return d;
+}
+
+void og_closest_point_triangle(const double p[3], const double a[3], const double b[3],
+                               const double c[3], double out[3]) {
+  const V3 q = closest_point_triangle(load3(p), load3(a), load3(b), load3(c));
+  out[0] = q.x; out[1] = q.y; out[2] = q.z;
+}
+
+int32_t og_mesh_inside(const og_mesh *mesh, const double p[3]) { return mesh->inside(load3(p)); }
+
+int32_t og_mesh_inside_many(const og_mesh *mesh, const double *p, int64_t n, int32_t *out,
+                            int32_t threads) {
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 64)
+#endif
+  for (int64_t i = 0; i < n; ++i) out[i] = mesh->inside(load3(p + 3 * i));
+  return 0;
+}
+
+double og_closest_boundary(const og_geom *g, const double p[3], double out_point[3], int32_t *out_cls) {
+  V3 q;
+  int32_t cls = 0;
+  const double d = closest_boundary(*g, load3(p), q, cls);
+  if (out_point) { out_point[0] = q.x; out_point[1] = q.y; out_point[2] = q.z; }
+  if (out_cls) *out_cls = cls;
+  return d;
+}
+
+void og_combine(const double *a, const double *b, int64_t n, int32_t square, double out[6]) {
+  combine(a, b, n, square != 0, out);
+}
+
+// Exact leave-one-out jackknife from eight binary64 power sums
+// {Sa, Sb, Sab, Saa, Sbb, Saab, Sabb, Saabb}.  The leave-one-out estimate is
+// (C - Sa b_i - Sb a_i + 2 a_i b_i) / ((n-1)(n-2)), C = Sa Sb - Sab, which is
+// linear in (a_i, b_i, a_i b_i); its spread needs only those second moments.
+void og_combine_powersums(const double s[8], int64_t count, double out[6]) {
+  const double n = double(count);
+  out[0] = s[0] / n;
+  out[1] = s[1] / n;
+  out[2] = (s[0] * s[1] - s[2]) / (n * (n - 1.0));
+  out[4] = s[2] / n;
+  out[5] = n;
+  if (count < 4) {
+    out[3] = std::numeric_limits<double>::quiet_NaN();
+    return;
+  }
+  const double sa = s[0], sb = s[1];
+  const double z2 = sa * sa * s[4] + sb * sb * s[3] + 4.0 * s[7] + 2.0 * sa * sb * s[2] -
+                    4.0 * sa * s[6] - 4.0 * sb * s[5];
+  const double z1 = 2.0 * (s[2] - sa * sb);
+  const double d = (n - 1.0) * (n - 2.0);
+  const double spread = std::max(0.0, z2 - z1 * z1 / n) / (d * d);
+  out[3] = std::sqrt((n - 1.0) / n * spread);
+}
+
+int32_t og_estimate(const og_geom *g, const og_side sides[2], const og_params *prm,
+                    const double *sites, int64_t n, og_site *out, double *dump_a, double *dump_b,
+                    int32_t *dump_steps, int64_t dump_stride) {
+  const og_params p = *prm;
+  if (p.walks < 2 || p.max_steps < 1 || !(p.eps > 0.0) || n < 0) return 2;
+  if (p.rng != OG_RNG_PHILOX && p.combine == OG_CROSS) return 2;
+  if (p.rng == OG_RNG_MT_KEY) return 2;
+  const int32_t walks_max = std::max(p.walks, p.walks_max);
+  int32_t status = 0;
+#ifdef _OPENMP
+  if (p.threads > 0) omp_set_num_threads(p.threads);
+#pragma omp parallel for schedule(dynamic, 4)
+#endif
+  for (int64_t s = 0; s < n; ++s) {
+    const V3 z = load3(sites + 3 * s);
+    std::vector<double> a, b;
+    a.reserve(size_t(walks_max));
+    b.reserve(size_t(walks_max));
+    og_site r{};
+    int64_t steps = 0, caps = 0;
+    int64_t done = 0, target = p.walks;
+    double c[6];
+    while (true) {
+      for (int64_t w = done; w < target; ++w) {
+        const WalkOut o = sample(*g, sides, p, z, s, w);
+        a.push_back(o.a);
+        b.push_back(o.b);
+        steps += o.steps;
+        caps += o.cap ? 1 : 0;
+        if (dump_a && w < dump_stride) {
+          dump_a[s * dump_stride + w] = o.a;
+          dump_b[s * dump_stride + w] = o.b;
+          dump_steps[s * dump_stride + w] = o.steps | (o.cap ? int32_t(1u << 30) : 0);
+        }
+      }
+      done = target;
+      combine(a.data(), b.data(), done, p.combine == OG_SQUARE, c);
+      const bool met = std::isfinite(c[3]) && p.ci_z * c[3] <= p.ci_rel * std::fabs(c[2]);
+      if (walks_max <= p.walks || met) {
+        r.status = 0;
+        break;
+      }
+      if (done >= walks_max) {
+        r.status = 1;
+        break;
+      }
+      target = std::min<int64_t>(2 * done, walks_max);
+    }
+    r.u = c[0]; r.v = c[1]; r.score = c[2]; r.score_se = c[3];
+    r.td = -c[2]; r.same = c[4];
+    r.walks = done; r.steps = steps; r.caps = caps;
+    out[s] = r;
+  }
+  for (int64_t s = 0; s < n; ++s)
+    if (out[s].caps) status = 3;
+  return status;
+}
+
+int32_t og_trace(const og_geom *g, const og_side *side, const og_params *prm, const double start[3],
+                 uint64_t key, int64_t site, int32_t w, double *rows, int32_t max_rows) {
+  int32_t count = 0;
+  const og_params &p = *prm;
+  if (p.rng == OG_RNG_PHILOX) {
+    uint32_t k[2];
+    philox_key(p.seed, site, 0, k);
+    auto dir = [&](int32_t step) { return philox_direction(k, uint32_t(w), uint32_t(step)); };
+    walk(*g, *side, nullptr, load3(start), p.eps, p.max_steps, dir, rows, max_rows, &count);
+    return count;
+  }
+  std::mt19937_64 rng;
+  if (p.rng == OG_RNG_MT_KEY) rng.seed(splitmix64(key));
+  else if (p.rng == OG_RNG_MT_SITE) rng.seed(splitmix64(site_key(p.seed, p.repeat, site, w)));
+  else rng.seed(splitmix64(p.seed) ^ splitmix64(uint64_t(site) + 91347) ^ splitmix64(uint64_t(w) + 571));
+  auto dir = [&](int32_t) { return random_unit_vector(rng); };
+  walk(*g, *side, nullptr, load3(start), p.eps, p.max_steps, dir, rows, max_rows, &count);
+  return count;
+}
+
+}  // extern "C"

@@ -1,0 +1,118 @@
+#!/usr/bin/env python3
+"""Write the reference's own retained data as compact test fixtures.
+
+Reads data files (meshes, site lists, field outputs, walk traces, summaries)
+that the reference repository keeps under benchmarks/compiled_bunny/output/ and
+stores them as small .npz / .json fixtures beside this script.  Nothing here
+imports or runs reference code: the files are parsed as text.
+
+Run once in a container where the reference tree is mounted:
+    python tests/golden/make_fixtures.py [/root/reference]
+"""
+
+from __future__ import annotations
+
+import csv
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+
+
+def read_ascii_ply(path: Path) -> tuple[np.ndarray, np.ndarray]:
+    """ASCII triangle PLY, read the way load_ascii_ply reads it
+    (benchmarks/compiled_bunny/src/wih_twos.cpp:437-493): the first three
+    numbers of each vertex line and `3 a b c` face lines."""
+    lines = path.read_text().splitlines()
+    nv = nf = 0
+    start = None
+    for i, line in enumerate(lines):
+        words = line.split()
+        if words[:2] == ["element", "vertex"]:
+            nv = int(words[2])
+        elif words[:2] == ["element", "face"]:
+            nf = int(words[2])
+        elif words[:1] == ["end_header"]:
+            start = i + 1
+            break
+    assert start is not None and nv > 0 and nf > 0
+    vertices = np.array([[float(w) for w in lines[start + i].split()[:3]] for i in range(nv)])
+    faces = np.array([[int(w) for w in lines[start + nv + i].split()[1:4]] for i in range(nf)],
+                     dtype=np.uint32)
+    return vertices, faces
+
+
+def read_field(path: Path) -> dict[str, np.ndarray]:
+    with path.open() as handle:
+        rows = list(csv.DictReader(handle))
+    return {
+        "u": np.array([float(r["u"]) for r in rows]),
+        "score": np.array([float(r["score"]) for r in rows]),
+        "score_se": np.array([float(r["score_se"]) for r in rows]),
+        "steps": np.array([int(r["steps"]) for r in rows], dtype=np.int64),
+        "limit_hits": np.array([int(r["limit_hits"]) for r in rows], dtype=np.int64),
+        "xyz": np.array([[float(r["x"]), float(r["y"]), float(r["z"])] for r in rows]),
+    }
+
+
+def main() -> None:
+    root = Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    out = root / "benchmarks" / "compiled_bunny" / "output"
+    # Config.seed of the bunny study (benchmarks/stanford_bunny/run_bunny_benchmark.py:33).
+    base_seed = 20260912
+
+    for faces in (1000, 69674):
+        v, f = read_ascii_ply(out / f"bunny-{faces:05d}-faces.ply")
+        np.savez_compressed(HERE / f"bunny_{faces}.npz", vertices=v, faces=f)
+
+    points = np.loadtxt(out / "evaluation-points.csv", delimiter=",")
+    np.save(HERE / "bunny_sites.npy", points)
+
+    trace = np.loadtxt(out / "walk-traces.csv", delimiter=",", skiprows=1)
+    np.save(HERE / "bunny_walk_traces.npy", trace)
+
+    fields = {}
+    # field-reference.csv: 2048 walks, seed Config.seed + 720, repeat 0
+    # (benchmarks/compiled_bunny/run_compiled_benchmark.py:670-675).
+    fields["reference"] = (out / "field-reference.csv", 2048, base_seed + 720, 69674)
+    # field-twos-NNNN.csv: seed Config.seed + 1000 + walks (:686-692).
+    for walks in (4, 8, 16, 32, 64, 128, 256, 384, 512):
+        fields[f"twos_{walks}"] = (out / f"field-twos-{walks:04d}.csv", walks,
+                                   base_seed + 1000 + walks, 69674)
+    # field-bunny-NNNNN-faces.csv: 32 walks, seed Config.seed + 1800 (:760-766).
+    fields["faces_1000"] = (out / "field-bunny-01000-faces.csv", 32, base_seed + 1800, 1000)
+    fields["faces_69674"] = (out / "field-bunny-69674-faces.csv", 32, base_seed + 1800, 69674)
+
+    index = {}
+    for name, (path, walks, seed, nfaces) in fields.items():
+        data = read_field(path)
+        assert np.array_equal(data["xyz"], points), name
+        del data["xyz"]
+        np.savez_compressed(HERE / f"field_{name}.npz", **data)
+        index[name] = {"walks": walks, "seed": seed, "faces": nfaces, "repeat": 0,
+                       "epsilon": 5e-4, "max_steps": 512, "source_y_slope": 0.65,
+                       "source": str(path.relative_to(root))}
+
+    summary = json.loads((out / "reference-summary.json").read_text())
+    meta = {
+        "fields": index,
+        # walk-traces.csv: 12 traced sites, site = i * 384 / 12, seed
+        # Config.seed + 720 + 900000 + site (wih_twos.cpp:1725-1744).
+        "traces": {"sites": 12, "seed": base_seed + 720, "key_offset": 900000, "faces": 69674,
+                   "epsilon": 5e-4, "max_steps": 512},
+        "reference_summary": {k: summary[k] for k in ("mean_steps", "paths_per_second",
+                                                      "distance_queries_per_second",
+                                                      "runtime_seconds_median", "threads")},
+        # Recorded Embree-vs-libigl start-point distance agreement
+        # (benchmarks/compiled_bunny/output/summary.json).
+        "embree_libigl_max_error": json.loads((out / "summary.json").read_text())[
+            "maximum_embree_libigl_distance_error"],
+    }
+    (HERE / "bunny_meta.json").write_text(json.dumps(meta, indent=1) + "\n")
+
+
+if __name__ == "__main__":
+    main()
