@@ -1,0 +1,218 @@
+/*
+ * twos.h -- C-ABI of the MI355X Topological Walk on Spheres (TWoS) estimator.
+ *
+ * One call estimates, for every candidate site z, the Dirichlet-cavity
+ * topological derivative D_T(z) = -(u(z) - h) p(z) by Walk on Spheres, with the
+ * leave-diagonal-out cross-index combine and a jackknife standard error.  It is
+ * the in-process replacement of the reference's compiled backend, which is a
+ * process-level plugin (flags + ASCII PLY/CSV in, CSV/JSON out):
+ *
+ *   twos_estimate     replaces  `wih_twos --mode twos` (benchmarks/compiled_bunny/
+ *                               src/wih_twos.cpp:1799-1864: evaluate_site :674-717
+ *                               over an OpenMP site loop :1828-1836) and the paired
+ *                               estimator of `wih_study` (benchmarks/submission_study/
+ *                               study.cpp:80-120, :334-344)
+ *   twos_mesh_create  replaces  EmbreeMesh(path) (wih_twos.cpp:298-325); the caller
+ *                               parses the PLY (load_ascii_ply :437-493)
+ *   twos_geometry_create / twos_geometry_union_sphere
+ *                     replace   --cavities/--cavity-radius (wih_twos.cpp:585-626) and
+ *                               add the analytic sphere/box/cylinder domains
+ *   twos_inside       replaces  EmbreeMesh::inside (wih_twos.cpp:361-394)
+ *   twos_closest      replaces  EmbreeMesh::closest_point / domain_distance
+ *                               (wih_twos.cpp:339-359, :585-593)
+ *   twos_trace        replaces  write_trace (wih_twos.cpp:1725-1744)
+ *   twos_server_*     replace   the `--serve` loop (study.cpp:358-381), one
+ *                               request at a time there, continuous batching here
+ *
+ * Conventions: plain pointers to caller-owned host memory; results are written
+ * before return.  Status codes mirror the reference's exits: usage errors
+ * (exit 2) -> TWOS_ERR_INVALID_ARGUMENT; "one or more paths hit --max-steps"
+ * (:1846-1848) -> TWOS_ERR_STEP_CAP with results still written; a failed
+ * closest-point query or ray parity overflow (:355-357, :390-392) ->
+ * TWOS_ERR_GEOMETRY.  Nothing throws across the ABI; twos_last_error() gives
+ * the message of the calling thread's last failure.
+ */
+#ifndef TWOS_H
+#define TWOS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TWOS_OK 0
+#define TWOS_ERR_INVALID_ARGUMENT 2
+#define TWOS_ERR_STEP_CAP 3
+#define TWOS_ERR_GEOMETRY 4
+#define TWOS_ERR_ADMISSION 5
+#define TWOS_ERR_DEVICE 6
+#define TWOS_ERR_NOT_FOUND 7
+
+/* Domain containers. */
+#define TWOS_BALL 0
+#define TWOS_BOX 1
+#define TWOS_MESH 2
+
+/* Combine: SQUARE is evaluate_site's compliance score u^2 (a = b = one path,
+ * wih_twos.cpp:687-715); PAIRED accumulates both sides on one path
+ * (study.cpp:91-117); CROSS runs independent primal and adjoint streams. */
+#define TWOS_SQUARE 0
+#define TWOS_PAIRED 1
+#define TWOS_CROSS 2
+
+/* Streams: PHILOX keyed by (seed, site, side) with counter (step, walk);
+ * MT_SITE / MT_PAIRED / MT_KEY replay the reference's mt19937_64 layouts
+ * (wih_twos.cpp:681-685, study.cpp:89, wih_twos.cpp:1737), binary64 only. */
+#define TWOS_RNG_PHILOX 0
+#define TWOS_RNG_MT_SITE 1
+#define TWOS_RNG_MT_PAIRED 2
+#define TWOS_RNG_MT_KEY 3
+
+/* Schedules: PLAIN launches one kernel per site and round, one thread per
+ * walk; PERSISTENT runs every site of every admitted request in one
+ * persistent wavefront launch per round with walk regeneration. */
+#define TWOS_PLAIN 0
+#define TWOS_PERSISTENT 1
+
+typedef struct twos_ctx twos_ctx;
+typedef int32_t twos_status;
+
+/* Boundary data and source of one side: the Dirichlet payoff at the closest
+ * boundary point y of class c (0-5 container faces; ball and mesh use 0,
+ * box faces are -x,+x,-y,+y,-z,+z; 6 analytic holes; 7 cavities) is
+ *   cls_const[c] + (bit c of cls_poly_mask) * poly(y),
+ * poly(y) = c0 + cx x + cy y + cz z + cxx x^2 + cyy y^2 + czz z^2 + cxy xy + cxz xz + cyz yz;
+ * source_kind 1 adds (r^2/6) f(x) per step with the affine
+ * f = ((s0 + sx (x+ox)) + sy (y+oy)) + sz (z+oz) (the reference's
+ * 1 + slope (y + 0.5) is s0 = 1, sy = slope, oy = 0.5; wih_twos.cpp:546-552). */
+typedef struct {
+  double poly[10];
+  double cls_const[8];
+  uint32_t cls_poly_mask;
+  int32_t source_kind;
+  double src[7];
+} twos_side;
+
+typedef struct {
+  int32_t container;
+  int32_t n_spheres;       /* holes: rows (cx, cy, cz, r) */
+  int32_t n_cylinders;     /* holes: rows (axis 0|1|2, ca, cb, r), infinite, axis-aligned */
+  int32_t n_cavities;      /* cavities: rows (cx, cy, cz) of radius cavity_radius */
+  int32_t mesh;            /* mesh id for TWOS_MESH */
+  int32_t mesh_radius_f32; /* 1: binary32-rounded mesh distance (the reference's recorded outputs) */
+  double ball[4];
+  double box_lo[3];
+  double box_hi[3];
+  double cavity_radius;
+  const double *spheres;
+  const double *cylinders;
+  const double *cavities;
+} twos_geometry_desc;
+
+typedef struct {
+  int32_t combine;
+  int32_t rng;
+  int32_t precision;    /* 32 or 64: arithmetic of walk positions and distances */
+  int32_t schedule;
+  uint64_t seed;
+  int32_t repeat;       /* the reference's repeat index in MT_SITE keys */
+  int32_t max_steps;
+  double eps;
+  int32_t walks;        /* fixed budget per side, or the first rung */
+  int32_t walks_max;    /* > walks: double until ci_z * se <= ci_rel * |score| */
+  double ci_rel;
+  double ci_z;
+  int32_t use_graph;    /* capture each round's launches in a hipGraph */
+  int32_t block;        /* threads per block of the walk kernel, 0 = tuned */
+  int32_t waves_per_simd; /* persistent occupancy, 0 = tuned */
+  int32_t site_offset;  /* global index of sites[0] in stream keys */
+  int32_t flags;        /* TWOS_FLAG_* */
+  int32_t pad;
+} twos_params;
+
+/* Count BVH nodes fetched and triangles tested (slower; for roofline figures). */
+#define TWOS_FLAG_COUNT_TRAVERSAL 1
+
+typedef struct {
+  double u;         /* mean of side-0 samples */
+  double v;         /* mean of side-1 samples (= u for SQUARE) */
+  double score;     /* (Sa Sb - Sab) / (N (N-1)): the reference's score column */
+  double score_se;  /* leave-one-out jackknife SE of score; NaN if N < 4 */
+  double td;        /* -score */
+  double same;      /* Sab / N: same-index product (study.cpp same_index) */
+  int64_t walks;    /* N per side */
+  int64_t steps;    /* sphere steps over all of this site's paths */
+  int64_t caps;     /* paths that hit max_steps */
+  int32_t status;   /* 0 done, 1 budget exhausted before the CI target */
+  int32_t pad;
+} twos_site_result;
+
+typedef struct {
+  double seconds;           /* wall time of the estimate, host clock */
+  double walk_kernel_ms;    /* summed duration of the walk kernel launches (HIP events) */
+  double reduce_kernel_ms;
+  int64_t walk_launches;
+  int64_t steps;            /* all sphere steps of all paths */
+  int64_t paths;
+  int64_t nodes_visited;    /* BVH nodes fetched (when counting is on) */
+  int64_t tris_tested;
+  int32_t rounds;
+  int32_t block;
+  int32_t waves_per_simd;
+  int32_t grid;
+} twos_stats;
+
+const char *twos_version(void);
+const char *twos_last_error(void);
+
+twos_status twos_create(int32_t device, uint64_t hbm_budget_bytes, twos_ctx **out);
+void twos_destroy(twos_ctx *ctx);
+
+twos_status twos_mesh_create(twos_ctx *ctx, const double *vertices, int64_t nv, const uint32_t *faces,
+                             int64_t nf, int32_t *mesh_id, double *build_ms);
+twos_status twos_mesh_info(twos_ctx *ctx, int32_t mesh_id, int32_t *depth, int64_t *nodes, int64_t *bytes);
+twos_status twos_mesh_release(twos_ctx *ctx, int32_t mesh_id);
+
+twos_status twos_geometry_create(twos_ctx *ctx, const twos_geometry_desc *desc, int32_t *geom_id);
+twos_status twos_geometry_union_sphere(twos_ctx *ctx, int32_t base, const double center[3], double radius,
+                                       int32_t *geom_id);
+twos_status twos_geometry_release(twos_ctx *ctx, int32_t geom_id);
+
+twos_status twos_estimate(twos_ctx *ctx, int32_t geom_id, const twos_side sides[2], const twos_params *params,
+                          const double *sites, int64_t n, twos_site_result *out, twos_stats *stats);
+
+/* As twos_estimate, also writing each path's samples a, b (binary64) and step
+ * count (bit 30 set when capped) as [n][stride] arrays, walks >= stride dropped. */
+twos_status twos_estimate_dump(twos_ctx *ctx, int32_t geom_id, const twos_side sides[2],
+                               const twos_params *params, const double *sites, int64_t n,
+                               twos_site_result *out, twos_stats *stats, double *dump_a, double *dump_b,
+                               int32_t *dump_steps, int64_t stride);
+
+twos_status twos_inside(twos_ctx *ctx, int32_t mesh_id, const double *points, int64_t n, int32_t *out);
+twos_status twos_closest(twos_ctx *ctx, int32_t geom_id, int32_t precision, const double *points, int64_t n,
+                         double *distance, double *closest, int32_t *cls);
+twos_status twos_trace(twos_ctx *ctx, int32_t geom_id, const twos_side *side, const twos_params *params,
+                       const double start[3], int64_t site, int32_t walk, double *rows, int32_t max_rows,
+                       int32_t *n_rows);
+
+/* Continuous-batching server: requests are admitted between rounds as device
+ * memory allows, all admitted sites share each persistent launch, and each
+ * result depends only on (seed, site, side, walk), alone or in the mix. */
+typedef struct twos_server twos_server;
+twos_status twos_server_create(twos_ctx *ctx, int64_t max_sites, int64_t max_walks_per_round,
+                               twos_server **out);
+void twos_server_destroy(twos_server *srv);
+twos_status twos_server_submit(twos_server *srv, int32_t geom_id, const twos_side sides[2],
+                               const twos_params *params, const double *sites, int64_t n,
+                               twos_site_result *out, int64_t *request_id);
+/* Runs one round over every admitted request; *n_done receives how many
+ * requests completed in this round, their ids written to done_ids. */
+twos_status twos_server_step(twos_server *srv, int64_t *done_ids, int32_t max_done, int32_t *n_done);
+twos_status twos_server_pending(twos_server *srv, int64_t *queued, int64_t *active);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,462 @@
+// Device-side building blocks of the TWoS hot path for gfx950 (CDNA4, wave64).
+//
+// Everything a walk touches per step lives here: counter-based Philox streams,
+// the reference's mt19937_64 replay stream, direction sampling, analytic SDF
+// unions, the LBVH closest-point traversal and Ericson's closest point on a
+// triangle.  Templates are instantiated twice: binary32 walks (contraction on,
+// twos_walk_f32.hip) and binary64 walks (contraction off, twos_walk_f64.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "twos_types.h"
+
+namespace twos {
+
+template <typename R>
+struct V3 {
+  R x, y, z;
+};
+template <typename R>
+__device__ __forceinline__ V3<R> operator+(V3<R> a, V3<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <typename R>
+__device__ __forceinline__ V3<R> operator-(V3<R> a, V3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <typename R>
+__device__ __forceinline__ V3<R> operator*(V3<R> a, R s) { return {a.x * s, a.y * s, a.z * s}; }
+template <typename R>
+__device__ __forceinline__ R dot(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+template <typename R> struct Inf;
+template <> struct Inf<float> { static __device__ __forceinline__ float v() { return __builtin_huge_valf(); } };
+template <> struct Inf<double> { static __device__ __forceinline__ double v() { return __builtin_huge_val(); } };
+
+// ---------------------------------------------------------------------------
+// Streams
+
+// splitmix64 finalizer (benchmarks/compiled_bunny/src/wih_twos.cpp:529-534).
+__host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  x += 0x9e3779b97f4a7c15ULL;
+  x = (x ^ (x >> 30U)) * 0xbf58476d1ce4e5b9ULL;
+  x = (x ^ (x >> 27U)) * 0x94d049bb133111ebULL;
+  return x ^ (x >> 31U);
+}
+
+// Philox4x32-10 (Salmon et al., SC'11): two 32x32 multiplies per round, Weyl
+// key schedule, ten rounds.  Counter = (step, walk, 0, 0).
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k) {
+#pragma unroll
+  for (int round = 0; round < 10; ++round) {
+    const uint32_t lo0 = 0xD2511F53u * c.x, hi0 = __umulhi(0xD2511F53u, c.x);
+    const uint32_t lo1 = 0xCD9E8D57u * c.z, hi1 = __umulhi(0xCD9E8D57u, c.z);
+    c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+    k.x += 0x9E3779B9u;
+    k.y += 0xBB67AE85u;
+  }
+  return c;
+}
+
+// Stream identity of (seed, site, side), computed on the host per site slot.
+inline uint2 philox_key_host(uint64_t seed, int64_t site, int32_t side) {
+  const uint64_t k = splitmix64(splitmix64(seed) + ((uint64_t(site) << 1) | uint64_t(side & 1)));
+  return make_uint2(uint32_t(k), uint32_t(k >> 32));
+}
+
+constexpr double kTwoPi = 6.283185307179586;  // 2 * acos(-1.0) in binary64
+
+// Unit direction from one Philox block.  binary64: 53-bit uniforms from word
+// pairs (0,1) and (2,3).  binary32: the top 24 bits of the same two numbers,
+// so a binary32 walk and its binary64 rerun follow the same directions to
+// within 2^-24.  z = 1 - 2 u1, phi = 2 pi u2 (wih_twos.cpp:536-542).
+__device__ __forceinline__ V3<double> philox_direction(uint4 w, double) {
+  const double u1 = double(((uint64_t(w.y) << 32) | w.x) >> 11) * 0x1.0p-53;
+  const double u2 = double(((uint64_t(w.w) << 32) | w.z) >> 11) * 0x1.0p-53;
+  const double z = 1.0 - 2.0 * u1;
+  const double phi = kTwoPi * u2;
+  const double radial = sqrt(fmax(0.0, 1.0 - z * z));
+  double s, c;
+  sincos(phi, &s, &c);
+  return {radial * c, radial * s, z};
+}
+
+__device__ __forceinline__ V3<float> philox_direction(uint4 w, float) {
+  const float u1 = float(w.y >> 8) * 0x1.0p-24f;
+  const float u2 = float(w.w >> 8) * 0x1.0p-24f;
+  const float z = 1.0f - 2.0f * u1;
+  const float radial = sqrtf(fmaxf(0.0f, (1.0f - z) * (1.0f + z)));
+  float s, c;
+  sincospif(2.0f * u2, &s, &c);
+  return {radial * c, radial * s, z};
+}
+
+// mt19937_64 (Matsumoto-Nishimura; the C++ standard's parameters w=64, n=312,
+// m=156, r=31, a=0xB5026F5AA96619E9, tempering u=29 d=0x5555555555555555
+// s=17 b=0x71D67FFFEDA60000 t=37 c=0xFFF7EEE000000000 l=43, f=6364136223846793005).
+// Used only to replay the reference's own streams in binary64.
+struct Mt64 {
+  uint64_t mt[312];
+  int idx;
+  __device__ void seed(uint64_t s) {
+    mt[0] = s;
+    for (int i = 1; i < 312; ++i) mt[i] = 6364136223846793005ULL * (mt[i - 1] ^ (mt[i - 1] >> 62)) + uint64_t(i);
+    idx = 312;
+  }
+  __device__ void twist() {
+    for (int i = 0; i < 312; ++i) {
+      const uint64_t x = (mt[i] & 0xFFFFFFFF80000000ULL) | (mt[i + 1 < 312 ? i + 1 : 0] & 0x7FFFFFFFULL);
+      uint64_t xa = x >> 1;
+      if (x & 1ULL) xa ^= 0xB5026F5AA96619E9ULL;
+      const int j = i + 156 < 312 ? i + 156 : i + 156 - 312;
+      mt[i] = mt[j] ^ xa;
+    }
+    idx = 0;
+  }
+  __device__ uint64_t next() {
+    if (idx >= 312) twist();
+    uint64_t x = mt[idx++];
+    x ^= (x >> 29) & 0x5555555555555555ULL;
+    x ^= (x << 17) & 0x71D67FFFEDA60000ULL;
+    x ^= (x << 37) & 0xFFF7EEE000000000ULL;
+    x ^= (x >> 43);
+    return x;
+  }
+  // uniform_real_distribution<double>(0,1) over a 64-bit engine (libstdc++
+  // generate_canonical with one draw): x / 2^64, clamped below 1.
+  __device__ double uniform() {
+    double u = double(next()) * 0x1.0p-64;
+    if (u >= 1.0) u = 0x1.fffffffffffffp-1;
+    return u;
+  }
+  // random_unit_vector (wih_twos.cpp:536-542).
+  __device__ V3<double> direction() {
+    const double z = 1.0 - 2.0 * uniform();
+    const double phi = kTwoPi * uniform();
+    const double radial = sqrt(fmax(0.0, 1.0 - z * z));
+    return {radial * cos(phi), radial * sin(phi), z};
+  }
+};
+
+// Per-walk mt19937_64 seed of the reference's key layouts.
+__device__ __forceinline__ uint64_t mt_walk_seed(int rng, uint64_t seed, int32_t repeat, int64_t site,
+                                                  int64_t walk) {
+  if (rng == kRngMtSite) {
+    // evaluate_site key (wih_twos.cpp:681-685), seeded through walk() (:638).
+    const uint64_t key = splitmix64(seed + 0x2d358dccaa6c78a5ULL) ^
+                         splitmix64(uint64_t(repeat) + 0x8bb84b93962eacc9ULL) ^
+                         splitmix64(uint64_t(site) + 0x4b33a62ed433d4a3ULL) ^
+                         splitmix64(uint64_t(walk) + 0x4d5a2da51de1aa47ULL);
+    return splitmix64(key);
+  }
+  if (rng == kRngMtPaired) {
+    // paired_site (benchmarks/submission_study/study.cpp:89).
+    return splitmix64(seed) ^ splitmix64(uint64_t(site) + 91347) ^ splitmix64(uint64_t(walk) + 571);
+  }
+  // Explicit key (write_trace: seed + 900000 + site, wih_twos.cpp:1737).
+  return splitmix64(seed + uint64_t(site));
+}
+
+// ---------------------------------------------------------------------------
+// Closest point on a triangle: Ericson, Real-Time Collision Detection 5.1.5
+// (the reference's closest_point_triangle, wih_twos.cpp:247-289).
+template <typename R>
+__device__ __forceinline__ V3<R> closest_point_triangle(V3<R> p, V3<R> a, V3<R> b, V3<R> c) {
+  const V3<R> ab = b - a;
+  const V3<R> ac = c - a;
+  const V3<R> ap = p - a;
+  const R d1 = dot(ab, ap);
+  const R d2 = dot(ac, ap);
+  if (d1 <= R(0) && d2 <= R(0)) return a;
+  const V3<R> bp = p - b;
+  const R d3 = dot(ab, bp);
+  const R d4 = dot(ac, bp);
+  if (d3 >= R(0) && d4 <= d3) return b;
+  const R vc = d1 * d4 - d3 * d2;
+  if (vc <= R(0) && d1 >= R(0) && d3 <= R(0)) {
+    const R v = d1 / (d1 - d3);
+    return a + ab * v;
+  }
+  const V3<R> cp = p - c;
+  const R d5 = dot(ab, cp);
+  const R d6 = dot(ac, cp);
+  if (d6 >= R(0) && d5 <= d6) return c;
+  const R vb = d5 * d2 - d1 * d6;
+  if (vb <= R(0) && d2 >= R(0) && d6 <= R(0)) {
+    const R w = d2 / (d2 - d6);
+    return a + ac * w;
+  }
+  const R va = d3 * d6 - d5 * d4;
+  if (va <= R(0) && (d4 - d3) >= R(0) && (d5 - d6) >= R(0)) {
+    const V3<R> bc = c - b;
+    const R w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    return b + bc * w;
+  }
+  const R denom = R(1) / (va + vb + vc);
+  const R v = vb * denom;
+  const R w = vc * denom;
+  return a + ab * v + ac * w;
+}
+
+// ---------------------------------------------------------------------------
+// LBVH closest-point traversal.
+//
+// Node i (internal, 0 <= i < n-1) is 64 bytes, four float4:
+//   q0 = (L.lo.x, L.lo.y, L.lo.z, L.hi.x)
+//   q1 = (L.hi.y, L.hi.z, R.lo.x, R.lo.y)
+//   q2 = (R.lo.z, R.hi.x, R.hi.y, R.hi.z)
+//   q3 = (child L, child R, parent, -)   as int bits; a child >= 0 is an
+//        internal node, a child < 0 is leaf ~k (sorted triangle k).
+// One 64-byte fetch yields both child boxes, so a visit costs one line.
+// Triangles are stored in sorted order: binary32 as 9 floats, binary64 as 9
+// doubles, plus the original triangle index (tie-break for exact queries).
+
+struct MeshView {
+  const float4 *nodes;
+  const float *tf;
+  const double *td;
+  const int32_t *tid;
+  int32_t n_tris;
+};
+
+template <typename R>
+__device__ __forceinline__ R box_d2(V3<R> p, R lx, R ly, R lz, R hx, R hy, R hz) {
+  const R dx = fmax(fmax(lx - p.x, p.x - hx), R(0));
+  const R dy = fmax(fmax(ly - p.y, p.y - hy), R(0));
+  const R dz = fmax(fmax(lz - p.z, p.z - hz), R(0));
+  return dx * dx + dy * dy + dz * dz;
+}
+
+template <typename R>
+__device__ __forceinline__ void load_tri(const MeshView &m, int32_t k, V3<R> &a, V3<R> &b, V3<R> &c);
+
+template <>
+__device__ __forceinline__ void load_tri<float>(const MeshView &m, int32_t k, V3<float> &a, V3<float> &b,
+                                                V3<float> &c) {
+  const float *t = m.tf + 9 * size_t(k);
+  a = {t[0], t[1], t[2]};
+  b = {t[3], t[4], t[5]};
+  c = {t[6], t[7], t[8]};
+}
+
+template <>
+__device__ __forceinline__ void load_tri<double>(const MeshView &m, int32_t k, V3<double> &a, V3<double> &b,
+                                                 V3<double> &c) {
+  const double *t = m.td + 9 * size_t(k);
+  a = {t[0], t[1], t[2]};
+  b = {t[3], t[4], t[5]};
+  c = {t[6], t[7], t[8]};
+}
+
+struct TraversalCount {
+  uint32_t nodes;
+  uint32_t tris;
+};
+
+// Exact closest point on the mesh.  binary64: distances in binary64 against
+// outward-rounded binary32 boxes, ties broken by the original triangle index,
+// so the result equals a brute-force scan.  binary32: the same traversal in
+// binary32.  Depth of the stack is bounded by the tree depth, which the
+// builder checks against kStackDepth.
+constexpr int kStackDepth = 64;
+
+template <typename R, bool kCount>
+__device__ __forceinline__ R mesh_closest(const MeshView &m, V3<R> p, V3<R> &best_pt, TraversalCount *cnt) {
+  R best = Inf<R>::v();
+  int32_t best_id = 0x7fffffff;
+  if (m.n_tris == 1) {
+    V3<R> a, b, c;
+    load_tri<R>(m, 0, a, b, c);
+    best_pt = closest_point_triangle(p, a, b, c);
+    const V3<R> d = p - best_pt;
+    if (kCount) cnt->tris += 1;
+    return dot(d, d);
+  }
+  int32_t stack[kStackDepth];
+  int sp = 0;
+  int32_t node = 0;
+  // A tree over n triangles has 2n - 1 nodes and a traversal visits each at
+  // most once; the bound only guards against a corrupt tree.
+  int32_t visits = 2 * m.n_tris + 8;
+  // Pruning slack: binary64 keeps a relative margin so rounding in the box
+  // bound never prunes a triangle that ties the current best.
+  const R slack = sizeof(R) == 8 ? R(1.0 - 1e-12) : R(1);
+  while (true) {
+    if (node >= 0) {
+      const float4 q0 = m.nodes[4 * node + 0];
+      const float4 q1 = m.nodes[4 * node + 1];
+      const float4 q2 = m.nodes[4 * node + 2];
+      const float4 q3 = m.nodes[4 * node + 3];
+      if (kCount) cnt->nodes += 1;
+      const R dl = box_d2<R>(p, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y) * slack;
+      const R dr = box_d2<R>(p, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w) * slack;
+      const int32_t cl = __float_as_int(q3.x), cr = __float_as_int(q3.y);
+      const bool hl = sizeof(R) == 8 ? dl <= best : dl < best;
+      const bool hr = sizeof(R) == 8 ? dr <= best : dr < best;
+      if (--visits < 0) break;
+      if (hl && hr) {
+        const bool left_first = dl <= dr;
+        if (sp < kStackDepth) stack[sp++] = left_first ? cr : cl;
+        node = left_first ? cl : cr;
+        continue;
+      }
+      if (hl) { node = cl; continue; }
+      if (hr) { node = cr; continue; }
+    } else {
+      const int32_t k = ~node;
+      V3<R> a, b, c;
+      load_tri<R>(m, k, a, b, c);
+      if (kCount) cnt->tris += 1;
+      const V3<R> q = closest_point_triangle(p, a, b, c);
+      const V3<R> d = p - q;
+      const R d2 = dot(d, d);
+      if (sizeof(R) == 8) {
+        const int32_t id = m.tid[k];
+        if (d2 < best || (d2 == best && id < best_id)) {
+          best = d2;
+          best_pt = q;
+          best_id = id;
+        }
+      } else if (d2 < best) {
+        best = d2;
+        best_pt = q;
+      }
+    }
+    if (sp == 0) break;
+    node = stack[--sp];
+  }
+  return best;
+}
+
+// ---------------------------------------------------------------------------
+// Domain: container (ball, box or mesh) minus sphere / cylinder holes minus
+// reference-style cavities.  Returns the distance to the boundary, the
+// closest boundary point and its class (0-5 container faces, 6 holes,
+// 7 cavities).  The mesh is first and later candidates win only when strictly
+// closer, as domain_distance / closest_boundary order them
+// (wih_twos.cpp:585-593, :602-626).
+
+template <typename R>
+struct Vec4 { R x, y, z, w; };
+
+template <typename R>
+__device__ __forceinline__ Vec4<R> load4(const DevGeom &g, int kind, int i);
+
+template <>
+__device__ __forceinline__ Vec4<float> load4<float>(const DevGeom &g, int kind, int i) {
+  const float4 v = kind == 0 ? g.spheres_f[i] : (kind == 1 ? g.cylinders_f[i] : g.cavities_f[i]);
+  return {v.x, v.y, v.z, v.w};
+}
+
+template <>
+__device__ __forceinline__ Vec4<double> load4<double>(const DevGeom &g, int kind, int i) {
+  const double *b = kind == 0 ? g.spheres_d : (kind == 1 ? g.cylinders_d : g.cavities_d);
+  return {b[4 * i + 0], b[4 * i + 1], b[4 * i + 2], b[4 * i + 3]};
+}
+
+template <typename R>
+__device__ __forceinline__ R comp(V3<R> v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+template <typename R, int kKind, bool kCount>
+__device__ __forceinline__ R closest_boundary(const DevGeom &g, V3<R> p, V3<R> &point, int &cls,
+                                              TraversalCount *cnt) {
+  R best;
+  if (kKind == kGeomMesh || (kKind == kGeomGeneric && g.container == kMesh)) {
+    MeshView m{g.nodes, g.tris_f, g.tris_d, g.tri_ids, g.n_tris};
+    const R d2 = mesh_closest<R, kCount>(m, p, point, cnt);
+    best = sqrt(d2);
+    if (sizeof(R) == 8 && g.mesh_radius_f32) best = R(float(best));
+    cls = 0;
+  } else if (g.container == kBall) {
+    const V3<R> c{R(g.ball[0]), R(g.ball[1]), R(g.ball[2])};
+    const R rad = R(g.ball[3]);
+    const V3<R> d = p - c;
+    const R len = sqrt(dot(d, d));
+    best = rad - len;
+    point = len > R(0) ? c + d * (rad / len) : V3<R>{c.x + rad, c.y, c.z};
+    cls = 0;
+  } else {
+    best = Inf<R>::v();
+    cls = 0;
+#pragma unroll
+    for (int face = 0; face < 6; ++face) {
+      const int k = face >> 1;
+      const R x = comp(p, k);
+      const R dist = (face & 1) ? R(g.box_hi[k]) - x : x - R(g.box_lo[k]);
+      if (dist < best) {
+        best = dist;
+        cls = face;
+      }
+    }
+    point = p;
+    const R plane = (cls & 1) ? R(g.box_hi[cls >> 1]) : R(g.box_lo[cls >> 1]);
+    if ((cls >> 1) == 0) point.x = plane; else if ((cls >> 1) == 1) point.y = plane; else point.z = plane;
+  }
+  if (kKind != kGeomMesh) {
+    for (int i = 0; i < g.n_spheres; ++i) {
+      const Vec4<R> s = load4<R>(g, 0, i);
+      const V3<R> c{s.x, s.y, s.z};
+      const V3<R> d = p - c;
+      const R len = sqrt(dot(d, d));
+      const R dist = len - s.w;
+      if (dist < best && len > R(0)) {
+        best = dist;
+        point = c + d * (s.w / len);
+        cls = 6;
+      }
+    }
+    for (int i = 0; i < g.n_cylinders; ++i) {
+      const Vec4<R> s = load4<R>(g, 1, i);
+      const int axis = int(s.x);
+      const int ka = axis == 0 ? 1 : 0, kb = axis == 2 ? 1 : 2;
+      const R da = comp(p, ka) - s.y, db = comp(p, kb) - s.z;
+      const R len = sqrt(da * da + db * db);
+      const R dist = len - s.w;
+      if (dist < best && len > R(0)) {
+        best = dist;
+        point = p;
+        const R pa = s.y + da * (s.w / len), pb = s.z + db * (s.w / len);
+        if (ka == 0) point.x = pa; else point.y = pa;
+        if (kb == 1) point.y = pb; else point.z = pb;
+        cls = 6;
+      }
+    }
+  }
+  for (int i = 0; i < g.n_cavities; ++i) {
+    const Vec4<R> s = load4<R>(g, 2, i);
+    const V3<R> c{s.x, s.y, s.z};
+    const V3<R> d = p - c;
+    const R len = sqrt(dot(d, d));
+    const R cr = R(g.cavity_radius);
+    const R dist = len - cr;
+    if (dist < best && len > R(0)) {
+      best = dist;
+      point = c + d * (cr / len);
+      cls = 7;
+    }
+  }
+  return best;
+}
+
+// ---------------------------------------------------------------------------
+// Boundary data g(y, class) and volume source f(x) of one side.
+template <typename R>
+__device__ __forceinline__ R side_payoff(const DevSide &s, V3<R> y, int cls) {
+  R value = R(s.cls_const[cls & 7]);
+  if ((s.cls_poly_mask >> (cls & 7)) & 1U) {
+    const double *c = s.poly;
+    value += R(c[0]) + R(c[1]) * y.x + R(c[2]) * y.y + R(c[3]) * y.z + R(c[4]) * y.x * y.x +
+             R(c[5]) * y.y * y.y + R(c[6]) * y.z * y.z + R(c[7]) * y.x * y.y + R(c[8]) * y.x * y.z +
+             R(c[9]) * y.y * y.z;
+  }
+  return value;
+}
+
+// f = ((s0 + sx (x+ox)) + sy (y+oy)) + sz (z+oz); the reference's affine
+// source 1 + slope (y + 0.5) (wih_twos.cpp:546-552) is one instance and gives
+// the same binary64 value in the contraction-free binary64 build.
+template <typename R>
+__device__ __forceinline__ R side_source(const DevSide &s, V3<R> x) {
+  const double *c = s.src;
+  return ((R(c[0]) + R(c[1]) * (x.x + R(c[4]))) + R(c[2]) * (x.y + R(c[5]))) + R(c[3]) * (x.z + R(c[6]));
+}
+
+}  // namespace twos

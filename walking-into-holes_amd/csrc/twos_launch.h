@@ -1,0 +1,39 @@
+// Host-side launch entry points of the gfx950 kernels (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "twos_types.h"
+
+namespace twos {
+
+struct Tables;
+
+struct MeshBuffers {
+  float4 *nodes = nullptr;
+  float *tris_f = nullptr;
+  double *tris_d = nullptr;
+  int32_t *tri_ids = nullptr;
+  int32_t n_tris = 0;
+};
+
+hipError_t lbvh_build(const double *d_vtx, const uint32_t *d_faces, int32_t n, MeshBuffers &mb, int32_t *depth,
+                      hipStream_t stream);
+hipError_t launch_inside(const MeshBuffers &mb, const double *d_pts, int64_t n, int32_t *d_out, hipStream_t stream);
+hipError_t launch_closest(const DevGeom *d_geom, int precision, const double *d_pts, int64_t n, double *d_dist,
+                          double *d_cp, int32_t *d_cls, hipStream_t stream);
+
+// geom_kind: kGeomGeneric / kGeomMesh / kGeomAnalytic.
+hipError_t launch_walk_simple_f32(int geom_kind, const Tables &tb, const WalkParams &wp, int block,
+                                  hipStream_t stream);
+hipError_t launch_walk_persistent_f32(int geom_kind, bool count, const Tables &tb, const WalkParams &wp, int grid,
+                                      int block, hipStream_t stream);
+hipError_t launch_walk_simple_f64(int geom_kind, bool mt, const Tables &tb, const WalkParams &wp, int block,
+                                  hipStream_t stream);
+hipError_t launch_walk_persistent_f64(int geom_kind, bool count, const Tables &tb, const WalkParams &wp, int grid,
+                                      int block, hipStream_t stream);
+hipError_t launch_reduce(const Tables &tb, const WalkParams &wp, int grid, hipStream_t stream);
+hipError_t launch_zero(unsigned long long *p, int n, hipStream_t stream);
+
+}  // namespace twos

@@ -1,0 +1,522 @@
+// GPU LBVH over a triangle mesh, plus the mesh point/ray queries that are not
+// part of the walk itself (closest-point batches, ray-parity inside tests).
+//
+// Build (Karras, "Maximizing parallelism in the construction of BVHs, octrees
+// and k-d trees", HPG 2012):
+//   1. per-triangle binary64 bounds, centroids, outward-rounded binary32 boxes
+//   2. centroid bounds (block partials, one finishing block)
+//   3. 30-bit Morton codes of the centroids
+//   4. LSD radix sort of (code, triangle) pairs: 8-bit digits, one item per
+//      thread, per-tile digit counts from wave ballots, one scan, stable scatter
+//   5. hierarchy: each internal node finds its key range and split in parallel
+//   6. bottom-up refit: the second child to arrive at a node writes its box
+//      into the node's parent
+// The nodes store both child boxes (64 B per node, see twos_device.h), and
+// triangles are reordered into leaf order in binary32 and binary64.
+
+#include <algorithm>
+#include <vector>
+
+#include "twos_device.h"
+#include "twos_launch.h"
+
+namespace twos {
+
+namespace {
+
+constexpr int kSortBlock = 256;
+
+__device__ __forceinline__ float round_down(double v) {
+  float f = float(v);
+  if (double(f) > v) f = nextafterf(f, -__builtin_huge_valf());
+  return f;
+}
+
+__device__ __forceinline__ float round_up(double v) {
+  float f = float(v);
+  if (double(f) < v) f = nextafterf(f, __builtin_huge_valf());
+  return f;
+}
+
+__global__ void tri_prep(const double *vtx, const uint32_t *faces, int32_t n, double *centroid, float *box) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double lo[3], hi[3];
+  for (int k = 0; k < 3; ++k) { lo[k] = __builtin_huge_val(); hi[k] = -__builtin_huge_val(); }
+  for (int c = 0; c < 3; ++c) {
+    const double *p = vtx + 3 * size_t(faces[3 * size_t(i) + c]);
+    for (int k = 0; k < 3; ++k) { lo[k] = fmin(lo[k], p[k]); hi[k] = fmax(hi[k], p[k]); }
+  }
+  for (int k = 0; k < 3; ++k) {
+    centroid[3 * size_t(i) + k] = 0.5 * (lo[k] + hi[k]);
+    box[6 * size_t(i) + k] = round_down(lo[k]);
+    box[6 * size_t(i) + 3 + k] = round_up(hi[k]);
+  }
+}
+
+// Centroid bounds: every block reduces a grid-stride share to a partial; the
+// last block to finish (atomic ticket) folds the partials.
+__global__ __launch_bounds__(256) void centroid_bounds(const double *centroid, int32_t n, double *partial,
+                                                       unsigned int *ticket, double *out) {
+  __shared__ double red[256][6];
+  double v[6] = {__builtin_huge_val(), __builtin_huge_val(), __builtin_huge_val(),
+                 -__builtin_huge_val(), -__builtin_huge_val(), -__builtin_huge_val()};
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (int k = 0; k < 3; ++k) {
+      v[k] = fmin(v[k], centroid[3 * size_t(i) + k]);
+      v[3 + k] = fmax(v[3 + k], centroid[3 * size_t(i) + k]);
+    }
+  }
+  for (int k = 0; k < 6; ++k) red[threadIdx.x][k] = v[k];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int k = 0; k < 6; ++k)
+        red[threadIdx.x][k] = k < 3 ? fmin(red[threadIdx.x][k], red[threadIdx.x + s][k])
+                                    : fmax(red[threadIdx.x][k], red[threadIdx.x + s][k]);
+    __syncthreads();
+  }
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < 6; ++k) partial[6 * blockIdx.x + k] = red[0][k];
+    __threadfence();
+    last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (last && threadIdx.x == 0) {
+    __threadfence();
+    double r[6] = {__builtin_huge_val(), __builtin_huge_val(), __builtin_huge_val(),
+                   -__builtin_huge_val(), -__builtin_huge_val(), -__builtin_huge_val()};
+    for (unsigned b = 0; b < gridDim.x; ++b)
+      for (int k = 0; k < 6; ++k) {
+        const double x = __hip_atomic_load(partial + 6 * b + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        r[k] = k < 3 ? fmin(r[k], x) : fmax(r[k], x);
+      }
+    for (int k = 0; k < 6; ++k) out[k] = r[k];
+  }
+}
+
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+__global__ void morton(const double *centroid, const double *bounds, int32_t n, uint32_t *keys, uint32_t *vals) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t q[3];
+  for (int k = 0; k < 3; ++k) {
+    const double ext = bounds[3 + k] - bounds[k];
+    const double t = ext > 0.0 ? (centroid[3 * size_t(i) + k] - bounds[k]) / ext : 0.5;
+    q[k] = uint32_t(fmin(fmax(t * 1024.0, 0.0), 1023.0));
+  }
+  keys[i] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+  vals[i] = uint32_t(i);
+}
+
+// Digit counts of one 256-item tile: lanes holding the same digit are found
+// with eight ballots; the lowest lane of each group records the group size.
+__device__ __forceinline__ void tile_rank(uint32_t digit, bool valid, uint32_t (*wave_hist)[256], int *rank_in_wave) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t match = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const uint64_t bits = __ballot(valid && ((digit >> b) & 1u));
+    match &= ((digit >> b) & 1u) ? bits : ~bits;
+  }
+  if (valid) {
+    *rank_in_wave = __popcll(match & ((1ull << lane) - 1ull));
+    if (*rank_in_wave == 0) wave_hist[wave][digit] = uint32_t(__popcll(match));
+  }
+}
+
+__global__ __launch_bounds__(kSortBlock) void radix_hist(const uint32_t *keys, int32_t n, int shift,
+                                                         uint32_t *hist, int32_t n_tiles) {
+  __shared__ uint32_t wave_hist[kSortBlock / 64][256];
+  for (int i = threadIdx.x; i < (kSortBlock / 64) * 256; i += kSortBlock) (&wave_hist[0][0])[i] = 0;
+  __syncthreads();
+  const int32_t i = blockIdx.x * kSortBlock + threadIdx.x;
+  const bool valid = i < n;
+  const uint32_t digit = valid ? (keys[i] >> shift) & 0xFFu : 0u;
+  int rank = 0;
+  tile_rank(digit, valid, wave_hist, &rank);
+  __syncthreads();
+  for (int d = threadIdx.x; d < 256; d += kSortBlock) {
+    uint32_t c = 0;
+    for (int w = 0; w < kSortBlock / 64; ++w) c += wave_hist[w][d];
+    hist[size_t(d) * n_tiles + blockIdx.x] = c;
+  }
+}
+
+// Exclusive scan of the digit-major count matrix in one block.
+__global__ __launch_bounds__(1024) void scan_exclusive(uint32_t *data, int32_t n) {
+  __shared__ uint32_t part[1024];
+  const int32_t per = (n + 1023) / 1024;
+  const int32_t begin = threadIdx.x * per, end = min(begin + per, n);
+  uint32_t sum = 0;
+  for (int32_t i = begin; i < end; ++i) sum += data[i];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+  for (int32_t i = begin; i < end; ++i) {
+    const uint32_t v = data[i];
+    data[i] = run;
+    run += v;
+  }
+}
+
+__global__ __launch_bounds__(kSortBlock) void radix_scatter(const uint32_t *keys, const uint32_t *vals, int32_t n,
+                                                            int shift, const uint32_t *offsets, int32_t n_tiles,
+                                                            uint32_t *keys_out, uint32_t *vals_out) {
+  __shared__ uint32_t wave_hist[kSortBlock / 64][256];
+  for (int i = threadIdx.x; i < (kSortBlock / 64) * 256; i += kSortBlock) (&wave_hist[0][0])[i] = 0;
+  __syncthreads();
+  const int32_t i = blockIdx.x * kSortBlock + threadIdx.x;
+  const bool valid = i < n;
+  const uint32_t key = valid ? keys[i] : 0u;
+  const uint32_t digit = (key >> shift) & 0xFFu;
+  int rank = 0;
+  tile_rank(digit, valid, wave_hist, &rank);
+  __syncthreads();
+  if (!valid) return;
+  const int wave = threadIdx.x >> 6;
+  uint32_t before = 0;
+  for (int w = 0; w < wave; ++w) before += wave_hist[w][digit];
+  const uint32_t dst = offsets[size_t(digit) * n_tiles + blockIdx.x] + before + uint32_t(rank);
+  keys_out[dst] = key;
+  vals_out[dst] = vals[i];
+}
+
+__device__ __forceinline__ int delta(const uint32_t *codes, int32_t n, int32_t i, int32_t j) {
+  if (j < 0 || j >= n) return -1;
+  const uint32_t a = codes[i], b = codes[j];
+  if (a == b) return 32 + __clz(uint32_t(i) ^ uint32_t(j));
+  return __clz(a ^ b);
+}
+
+__global__ void karras(const uint32_t *codes, int32_t n, float4 *nodes, int32_t *leaf_parent) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  const int d = (delta(codes, n, i, i + 1) - delta(codes, n, i, i - 1)) >= 0 ? 1 : -1;
+  const int dmin = delta(codes, n, i, i - d);
+  int32_t lmax = 2;
+  while (delta(codes, n, i, i + lmax * d) > dmin) lmax <<= 1;
+  int32_t l = 0;
+  for (int32_t t = lmax >> 1; t >= 1; t >>= 1)
+    if (delta(codes, n, i, i + (l + t) * d) > dmin) l += t;
+  const int32_t j = i + l * d;
+  const int dnode = delta(codes, n, i, j);
+  int32_t s = 0, step = l;
+  do {
+    step = (step + 1) >> 1;
+    if (delta(codes, n, i, i + (s + step) * d) > dnode) s += step;
+  } while (step > 1);
+  const int32_t gamma = i + s * d + min(d, 0);
+  const int32_t lo = min(i, j), hi = max(i, j);
+  const int32_t left = lo == gamma ? ~gamma : gamma;
+  const int32_t right = hi == gamma + 1 ? ~(gamma + 1) : gamma + 1;
+  float4 q3;
+  q3.x = __int_as_float(left);
+  q3.y = __int_as_float(right);
+  q3.z = __int_as_float(-1);
+  q3.w = 0.0f;
+  // The parent field is filled by the children's owners below; keep it in a
+  // separate pass so no node is written by two threads in one store.
+  nodes[4 * size_t(i) + 3] = q3;
+  if (left < 0) leaf_parent[~left] = i;
+  if (right < 0) leaf_parent[~right] = i;
+}
+
+__global__ void link_parents(float4 *nodes, int32_t n_internal) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_internal) return;
+  const float4 q3 = nodes[4 * size_t(i) + 3];
+  const int32_t l = __float_as_int(q3.x), r = __float_as_int(q3.y);
+  float *pz = reinterpret_cast<float *>(nodes);
+  if (l >= 0) pz[16 * size_t(l) + 14] = __int_as_float(i);
+  if (r >= 0) pz[16 * size_t(r) + 14] = __int_as_float(i);
+}
+
+// Writes box (lo, hi) into slot `which` (0 left, 1 right) of node p.
+__device__ __forceinline__ void store_child_box(float4 *nodes, int32_t p, int which, const float *b) {
+  float *f = reinterpret_cast<float *>(nodes + 4 * size_t(p));
+  const int base = which ? 6 : 0;
+  for (int k = 0; k < 6; ++k) __hip_atomic_store(f + base + k, b[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void refit(float4 *nodes, const int32_t *leaf_parent, const uint32_t *order, const float *tri_box,
+                      int32_t n, unsigned int *flags) {
+  const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  float b[6];
+  const uint32_t t = order[k];
+  for (int c = 0; c < 6; ++c) b[c] = tri_box[6 * size_t(t) + c];
+  int32_t child = ~k;
+  int32_t p = leaf_parent[k];
+  int guard = 0;
+  while (p >= 0 && guard++ < 4096) {
+    float *f = reinterpret_cast<float *>(nodes + 4 * size_t(p));
+    const int32_t left = __float_as_int(__hip_atomic_load(f + 12, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    const int which = left == child ? 0 : 1;
+    store_child_box(nodes, p, which, b);
+    __threadfence();
+    const unsigned prev = __hip_atomic_fetch_add(flags + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (prev == 0) return;  // the sibling has not arrived; it will continue
+    __threadfence();
+    float o[12];
+    for (int c = 0; c < 12; ++c) o[c] = __hip_atomic_load(f + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int c = 0; c < 3; ++c) {
+      b[c] = fminf(o[c], o[6 + c]);
+      b[3 + c] = fmaxf(o[3 + c], o[9 + c]);
+    }
+    child = p;
+    p = __float_as_int(__hip_atomic_load(f + 14, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  }
+}
+
+__global__ void reorder_tris(const double *vtx, const uint32_t *faces, const uint32_t *order, int32_t n, float *tf,
+                             double *td, int32_t *tid) {
+  const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t t = order[k];
+  for (int c = 0; c < 3; ++c) {
+    const double *p = vtx + 3 * size_t(faces[3 * size_t(t) + c]);
+    for (int a = 0; a < 3; ++a) {
+      td[9 * size_t(k) + 3 * c + a] = p[a];
+      tf[9 * size_t(k) + 3 * c + a] = float(p[a]);
+    }
+  }
+  tid[k] = int32_t(t);
+}
+
+__global__ void leaf_depth(const float4 *nodes, const int32_t *leaf_parent, int32_t n, int32_t *max_depth) {
+  const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int32_t depth = 1;
+  int32_t p = leaf_parent[k];
+  while (p >= 0 && depth < 4096) {
+    ++depth;
+    p = __float_as_int(nodes[4 * size_t(p) + 3].z);
+  }
+  atomicMax(max_depth, depth);
+}
+
+// Ray parity inside test: closest hit along a fixed irrational direction,
+// step 2e-6 past it, count crossings up to 128 (EmbreeMesh::inside,
+// wih_twos.cpp:361-394).  Binary64 Moller-Trumbore against outward-rounded
+// binary32 boxes.
+__device__ bool ray_hit(const MeshView &m, V3<double> o, V3<double> d, double tnear, double tfar, double &t_out) {
+  double best = tfar;
+  bool found = false;
+  int32_t stack[kStackDepth];
+  int sp = 0;
+  int32_t node = 0;
+  if (m.n_tris == 1) node = ~0;
+  int32_t visits = 2 * m.n_tris + 8;
+  while (--visits >= 0) {
+    if (node >= 0) {
+      const float4 q0 = m.nodes[4 * node + 0], q1 = m.nodes[4 * node + 1], q2 = m.nodes[4 * node + 2],
+                   q3 = m.nodes[4 * node + 3];
+      const float bx[2][6] = {{q0.x, q0.y, q0.z, q0.w, q1.x, q1.y}, {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w}};
+      const int32_t ch[2] = {__float_as_int(q3.x), __float_as_int(q3.y)};
+      bool hit[2];
+      for (int c = 0; c < 2; ++c) {
+        double t0 = tnear, t1 = best;
+        bool miss = false;
+        for (int k = 0; k < 3 && !miss; ++k) {
+          const double ok = comp(o, k), dk = comp(d, k);
+          const double lo = bx[c][k], hi = bx[c][3 + k];
+          if (dk == 0.0) {
+            if (ok < lo || ok > hi) miss = true;
+            continue;
+          }
+          double ta = (lo - ok) / dk, tb = (hi - ok) / dk;
+          if (ta > tb) { const double s = ta; ta = tb; tb = s; }
+          t0 = fmax(t0, ta - 1e-12);
+          t1 = fmin(t1, tb + 1e-12);
+          if (t0 > t1) miss = true;
+        }
+        hit[c] = !miss;
+      }
+      if (hit[0] && hit[1]) {
+        if (sp < kStackDepth) stack[sp++] = ch[1];
+        node = ch[0];
+        continue;
+      }
+      if (hit[0]) { node = ch[0]; continue; }
+      if (hit[1]) { node = ch[1]; continue; }
+    } else {
+      V3<double> a, b, c;
+      load_tri<double>(m, ~node, a, b, c);
+      const V3<double> e1 = b - a, e2 = c - a;
+      const V3<double> pv{d.y * e2.z - d.z * e2.y, d.z * e2.x - d.x * e2.z, d.x * e2.y - d.y * e2.x};
+      const double det = dot(e1, pv);
+      if (det != 0.0) {
+        const double inv = 1.0 / det;
+        const V3<double> tv = o - a;
+        const double uu = dot(tv, pv) * inv;
+        if (uu >= 0.0 && uu <= 1.0) {
+          const V3<double> qv{tv.y * e1.z - tv.z * e1.y, tv.z * e1.x - tv.x * e1.z, tv.x * e1.y - tv.y * e1.x};
+          const double vv = dot(d, qv) * inv;
+          if (vv >= 0.0 && uu + vv <= 1.0) {
+            const double t = dot(e2, qv) * inv;
+            if (t >= tnear && t <= best) {
+              best = t;
+              found = true;
+            }
+          }
+        }
+      }
+    }
+    if (sp == 0) break;
+    node = stack[--sp];
+  }
+  t_out = best;
+  return found;
+}
+
+__global__ void inside_kernel(MeshView m, const double *pts, int64_t n, int32_t *out) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  V3<double> dir{1.0, 0.3713906763541037, 0.1570796326794897};
+  const double len = sqrt(dot(dir, dir));
+  dir = dir * (1.0 / len);
+  V3<double> o{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+  unsigned hits = 0;
+  for (; hits < 128; ++hits) {
+    double t;
+    if (!ray_hit(m, o, dir, 1.0e-6, 4.0, t)) break;
+    o = o + dir * (t + 2.0e-6);
+  }
+  out[i] = hits == 128 ? -1 : int32_t(hits & 1u);
+}
+
+template <typename R>
+__global__ void closest_kernel(const DevGeom *g, const double *pts, int64_t n, double *dist, double *cp, int32_t *cls) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const V3<R> p{R(pts[3 * i]), R(pts[3 * i + 1]), R(pts[3 * i + 2])};
+  V3<R> q;
+  int c = 0;
+  const R d = closest_boundary<R, kGeomGeneric, false>(*g, p, q, c, nullptr);
+  dist[i] = double(d);
+  cp[3 * i] = double(q.x);
+  cp[3 * i + 1] = double(q.y);
+  cp[3 * i + 2] = double(q.z);
+  cls[i] = c;
+}
+
+#define TWOS_CHECK(x)                          \
+  do {                                         \
+    const hipError_t e_ = (x);                 \
+    if (e_ != hipSuccess) return e_;           \
+  } while (0)
+
+inline unsigned blocks_for(int64_t n, int b) { return unsigned((n + b - 1) / b); }
+
+}  // namespace
+
+hipError_t lbvh_build(const double *d_vtx, const uint32_t *d_faces, int32_t n, MeshBuffers &mb, int32_t *depth,
+                      hipStream_t stream) {
+  const int32_t n_tiles = int32_t(blocks_for(n, kSortBlock));
+  double *centroid = nullptr, *bounds = nullptr, *partial = nullptr;
+  float *tri_box = nullptr;
+  uint32_t *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr}, *hist = nullptr;
+  int32_t *leaf_parent = nullptr, *d_depth = nullptr;
+  unsigned int *flags = nullptr, *ticket = nullptr;
+  const int red_blocks = 64;
+  hipError_t err = hipSuccess;
+  auto cleanup = [&]() {
+    (void)hipFree(centroid); (void)hipFree(bounds); (void)hipFree(partial); (void)hipFree(tri_box);
+    (void)hipFree(keys[0]); (void)hipFree(keys[1]); (void)hipFree(vals[0]); (void)hipFree(vals[1]); (void)hipFree(hist);
+    (void)hipFree(leaf_parent); (void)hipFree(d_depth); (void)hipFree(flags); (void)hipFree(ticket);
+  };
+#define TRY(x)                    \
+  do {                            \
+    err = (x);                    \
+    if (err != hipSuccess) {      \
+      cleanup();                  \
+      return err;                 \
+    }                             \
+  } while (0)
+  TRY(hipMalloc(&centroid, sizeof(double) * 3 * size_t(n)));
+  TRY(hipMalloc(&bounds, sizeof(double) * 6));
+  TRY(hipMalloc(&partial, sizeof(double) * 6 * red_blocks));
+  TRY(hipMalloc(&tri_box, sizeof(float) * 6 * size_t(n)));
+  for (int b = 0; b < 2; ++b) {
+    TRY(hipMalloc(&keys[b], sizeof(uint32_t) * size_t(n)));
+    TRY(hipMalloc(&vals[b], sizeof(uint32_t) * size_t(n)));
+  }
+  TRY(hipMalloc(&hist, sizeof(uint32_t) * 256 * size_t(n_tiles)));
+  TRY(hipMalloc(&leaf_parent, sizeof(int32_t) * size_t(n)));
+  TRY(hipMalloc(&d_depth, sizeof(int32_t)));
+  TRY(hipMalloc(&flags, sizeof(unsigned int) * size_t(std::max(n - 1, 1))));
+  TRY(hipMalloc(&ticket, sizeof(unsigned int)));
+  TRY(hipMalloc(&mb.nodes, sizeof(float4) * 4 * size_t(std::max(n - 1, 1))));
+  TRY(hipMalloc(&mb.tris_f, sizeof(float) * 9 * size_t(n)));
+  TRY(hipMalloc(&mb.tris_d, sizeof(double) * 9 * size_t(n)));
+  TRY(hipMalloc(&mb.tri_ids, sizeof(int32_t) * size_t(n)));
+  mb.n_tris = n;
+
+  hipLaunchKernelGGL(tri_prep, dim3(blocks_for(n, 256)), dim3(256), 0, stream, d_vtx, d_faces, n, centroid, tri_box);
+  TRY(hipMemsetAsync(ticket, 0, sizeof(unsigned int), stream));
+  hipLaunchKernelGGL(centroid_bounds, dim3(red_blocks), dim3(256), 0, stream, centroid, n, partial, ticket, bounds);
+  hipLaunchKernelGGL(morton, dim3(blocks_for(n, 256)), dim3(256), 0, stream, centroid, bounds, n, keys[0], vals[0]);
+  int cur = 0;
+  for (int shift = 0; shift < 32; shift += 8) {
+    hipLaunchKernelGGL(radix_hist, dim3(n_tiles), dim3(kSortBlock), 0, stream, keys[cur], n, shift, hist, n_tiles);
+    hipLaunchKernelGGL(scan_exclusive, dim3(1), dim3(1024), 0, stream, hist, 256 * n_tiles);
+    hipLaunchKernelGGL(radix_scatter, dim3(n_tiles), dim3(kSortBlock), 0, stream, keys[cur], vals[cur], n, shift,
+                       hist, n_tiles, keys[cur ^ 1], vals[cur ^ 1]);
+    cur ^= 1;
+  }
+  TRY(hipGetLastError());
+  TRY(hipMemsetAsync(leaf_parent, 0xff, sizeof(int32_t) * size_t(n), stream));
+  TRY(hipMemsetAsync(mb.nodes, 0, sizeof(float4) * 4 * size_t(std::max(n - 1, 1)), stream));
+  if (n > 1) {
+    hipLaunchKernelGGL(karras, dim3(blocks_for(n - 1, 256)), dim3(256), 0, stream, keys[cur], n, mb.nodes,
+                       leaf_parent);
+    hipLaunchKernelGGL(link_parents, dim3(blocks_for(n - 1, 256)), dim3(256), 0, stream, mb.nodes, n - 1);
+    TRY(hipMemsetAsync(flags, 0, sizeof(unsigned int) * size_t(n - 1), stream));
+    hipLaunchKernelGGL(refit, dim3(blocks_for(n, 256)), dim3(256), 0, stream, mb.nodes, leaf_parent, vals[cur],
+                       tri_box, n, flags);
+  }
+  hipLaunchKernelGGL(reorder_tris, dim3(blocks_for(n, 256)), dim3(256), 0, stream, d_vtx, d_faces, vals[cur], n,
+                     mb.tris_f, mb.tris_d, mb.tri_ids);
+  TRY(hipMemsetAsync(d_depth, 0, sizeof(int32_t), stream));
+  hipLaunchKernelGGL(leaf_depth, dim3(blocks_for(n, 256)), dim3(256), 0, stream, mb.nodes, leaf_parent, n, d_depth);
+  TRY(hipGetLastError());
+  TRY(hipMemcpyAsync(depth, d_depth, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+  TRY(hipStreamSynchronize(stream));
+  cleanup();
+#undef TRY
+  return hipSuccess;
+}
+
+hipError_t launch_inside(const MeshBuffers &mb, const double *d_pts, int64_t n, int32_t *d_out, hipStream_t stream) {
+  MeshView m{mb.nodes, mb.tris_f, mb.tris_d, mb.tri_ids, mb.n_tris};
+  hipLaunchKernelGGL(inside_kernel, dim3(blocks_for(n, 128)), dim3(128), 0, stream, m, d_pts, n, d_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_closest(const DevGeom *d_geom, int precision, const double *d_pts, int64_t n, double *d_dist,
+                          double *d_cp, int32_t *d_cls, hipStream_t stream) {
+  if (precision == 64)
+    hipLaunchKernelGGL(closest_kernel<double>, dim3(blocks_for(n, 128)), dim3(128), 0, stream, d_geom, d_pts, n,
+                       d_dist, d_cp, d_cls);
+  else
+    hipLaunchKernelGGL(closest_kernel<float>, dim3(blocks_for(n, 128)), dim3(128), 0, stream, d_geom, d_pts, n,
+                       d_dist, d_cp, d_cls);
+  return hipGetLastError();
+}
+
+}  // namespace twos

@@ -1,0 +1,1286 @@
+// Host runtime and C-ABI of the TWoS estimator (include/twos.h).
+//
+// A context owns one device, one stream, the resident meshes and geometries,
+// and a round engine.  The engine is a continuous-batching scheduler: sites
+// of admitted requests occupy slots with resident binary64 accumulators; each
+// round gives every unfinished site its next rung of the doubling ladder,
+// runs all of them in one persistent launch (or one launch per site for the
+// plain schedule), folds the samples into the accumulators and reads back one
+// compact result per site.  Requests wait in FIFO order until enough slots
+// are free.  twos_estimate is one request on the context's own engine.
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "twos_device.h"
+#include "twos_launch.h"
+#include "twos_walk.h"
+
+namespace twos {
+namespace {
+
+thread_local std::string g_error;
+
+twos_status fail(twos_status code, const std::string &msg) {
+  g_error = msg;
+  return code;
+}
+
+#define HIP_OK(x)                                                                             \
+  do {                                                                                        \
+    const hipError_t e_ = (x);                                                                \
+    if (e_ != hipSuccess) return fail(TWOS_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(T **p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return hipMalloc(reinterpret_cast<void **>(p), sizeof(T) * count);
+}
+
+template <typename T>
+void dfree(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+constexpr int kMaxClasses = 8;
+
+}  // namespace
+
+struct MeshEntry {
+  MeshBuffers mb;
+  int32_t depth = 0;
+  int64_t bytes = 0;
+  double build_ms = 0.0;
+  int32_t users = 0;
+};
+
+struct GeomEntry {
+  twos_geometry_desc desc{};
+  std::vector<double> spheres, cylinders, cavities;
+  DevGeom dev{};
+  std::vector<void *> owned;
+  int32_t kind = kGeomGeneric;
+};
+
+struct Engine;
+
+}  // namespace twos
+
+struct twos_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipDeviceProp_t prop{};
+  uint64_t budget = 0;
+  std::mutex mu;
+  std::map<int32_t, twos::MeshEntry> meshes;
+  std::map<int32_t, twos::GeomEntry> geoms;
+  int32_t next_mesh = 0, next_geom = 0;
+  twos::DevGeom *d_geoms = nullptr;
+  int32_t d_geoms_cap = 0;
+  std::unique_ptr<twos::Engine> engine;
+};
+
+namespace twos {
+
+// ---------------------------------------------------------------------------
+// Round engine
+
+struct Request {
+  int64_t id = 0;
+  int32_t geom = 0;
+  int32_t problem = -1;
+  twos_params prm{};
+  DevSide sides[2];
+  std::vector<double> sites;
+  int64_t n = 0;
+  twos_site_result *out = nullptr;
+  int64_t slot0 = -1;
+  std::vector<int32_t> done;    // walks done per site (per side)
+  std::vector<uint8_t> state;   // 0 needs walks, 1 finished
+  int64_t remaining = 0;
+  int32_t cls = 0;              // combine class
+  bool admitted = false;
+};
+
+struct Engine {
+  twos_ctx *ctx = nullptr;
+  int64_t cap_sites = 0, cap_walks = 0;
+  int32_t cap_problems = 0;
+  bool growable = false;
+
+  SlotArrays s{};
+  DevSide *d_sides = nullptr;
+  double *d_val = nullptr;
+  int32_t *d_steps = nullptr;
+  unsigned long long *d_counters = nullptr;  // [kMaxClasses] tickets, then nodes, tris
+  // Round input block (one H2D copy) and output block (one D2H copy).
+  char *d_in = nullptr, *h_in = nullptr, *d_out = nullptr, *h_out = nullptr;
+  // One-site work list of the plain schedule: 64 B in, 96 B out.
+  char *d_plain = nullptr, *h_plain = nullptr;
+  size_t in_bytes = 0, out_bytes = 0;
+  size_t off_slot = 0, off_base = 0, off_prefix = 0, off_nact = 0, off_total = 0;
+  size_t off_compact = 0, off_decision = 0;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+
+  std::deque<Request *> queue;
+  std::vector<Request *> active;
+  std::map<int64_t, int64_t> free_slots;  // start -> length
+  std::vector<int32_t> free_problems;
+  int64_t next_id = 1;
+
+  // hipGraph of a single-class persistent round.
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int64_t graph_key = -1;
+
+  twos_stats stats{};
+
+  ~Engine() { release(); }
+
+  void release() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+    graph_key = -1;
+    dfree(s.pos); dfree(s.geom); dfree(s.problem); dfree(s.keys); dfree(s.site); dfree(s.seed);
+    dfree(s.repeat); dfree(s.walks_max); dfree(s.max_steps); dfree(s.eps); dfree(s.ci_rel); dfree(s.ci_z);
+    dfree(s.acc); dfree(s.acc_walks); dfree(s.acc_steps); dfree(s.acc_caps); dfree(s.result);
+    dfree(d_sides); dfree(d_val); dfree(d_steps); dfree(d_counters); dfree(d_in); dfree(d_out); dfree(d_plain);
+    if (h_in) (void)hipHostFree(h_in);
+    if (h_out) (void)hipHostFree(h_out);
+    if (h_plain) (void)hipHostFree(h_plain);
+    h_in = h_out = h_plain = nullptr;
+    for (auto &e : ev)
+      if (e) { (void)hipEventDestroy(e); e = nullptr; }
+  }
+
+  twos_status allocate(int64_t sites, int64_t walks, int32_t problems) {
+    release();
+    cap_sites = std::max<int64_t>(sites, 1);
+    cap_walks = std::max<int64_t>(walks, 64);
+    cap_problems = std::max(problems, 1);
+    const size_t C = size_t(cap_sites);
+    HIP_OK(dalloc(&s.pos, 3 * C)); HIP_OK(dalloc(&s.geom, C)); HIP_OK(dalloc(&s.problem, C));
+    HIP_OK(dalloc(&s.keys, 2 * C)); HIP_OK(dalloc(&s.site, C)); HIP_OK(dalloc(&s.seed, C));
+    HIP_OK(dalloc(&s.repeat, C)); HIP_OK(dalloc(&s.walks_max, C)); HIP_OK(dalloc(&s.max_steps, C));
+    HIP_OK(dalloc(&s.eps, C)); HIP_OK(dalloc(&s.ci_rel, C)); HIP_OK(dalloc(&s.ci_z, C));
+    HIP_OK(dalloc(&s.acc, 8 * C)); HIP_OK(dalloc(&s.acc_walks, C)); HIP_OK(dalloc(&s.acc_steps, C));
+    HIP_OK(dalloc(&s.acc_caps, C)); HIP_OK(dalloc(&s.result, C));
+    HIP_OK(dalloc(&d_sides, 2 * size_t(cap_problems)));
+    HIP_OK(dalloc(&d_val, 2 * size_t(cap_walks)));
+    HIP_OK(dalloc(&d_steps, 2 * size_t(cap_walks)));
+    HIP_OK(dalloc(&d_counters, kMaxClasses + 2));
+    auto align = [](size_t v) { return (v + 255) & ~size_t(255); };
+    off_slot = 0;
+    off_base = align(off_slot + 4 * C);
+    off_prefix = align(off_base + 4 * C);
+    off_nact = align(off_prefix + 8 * (C + kMaxClasses));
+    off_total = align(off_nact + 4 * kMaxClasses);
+    in_bytes = align(off_total + 8 * kMaxClasses);
+    off_compact = 0;
+    off_decision = align(sizeof(twos_site_result) * C);
+    out_bytes = align(off_decision + 4 * C);
+    HIP_OK(dalloc(&d_in, in_bytes));
+    HIP_OK(dalloc(&d_out, out_bytes));
+    HIP_OK(hipHostMalloc(reinterpret_cast<void **>(&h_in), in_bytes, hipHostMallocDefault));
+    HIP_OK(hipHostMalloc(reinterpret_cast<void **>(&h_out), out_bytes, hipHostMallocDefault));
+    HIP_OK(dalloc(&d_plain, 256));
+    HIP_OK(hipHostMalloc(reinterpret_cast<void **>(&h_plain), 256, hipHostMallocDefault));
+    std::memset(h_plain, 0, 256);
+    std::memset(h_in, 0, in_bytes);
+    for (auto &e : ev) HIP_OK(hipEventCreate(&e));
+    free_slots.clear();
+    free_slots[0] = cap_sites;
+    free_problems.clear();
+    for (int32_t p = cap_problems - 1; p >= 0; --p) free_problems.push_back(p);
+    return TWOS_OK;
+  }
+
+  bool idle() const { return queue.empty() && active.empty(); }
+
+  int64_t take_slots(int64_t n) {
+    for (auto it = free_slots.begin(); it != free_slots.end(); ++it) {
+      if (it->second >= n) {
+        const int64_t start = it->first, len = it->second;
+        free_slots.erase(it);
+        if (len > n) free_slots[start + n] = len - n;
+        return start;
+      }
+    }
+    return -1;
+  }
+
+  void give_slots(int64_t start, int64_t n) {
+    auto it = free_slots.emplace(start, n).first;
+    auto next = std::next(it);
+    if (next != free_slots.end() && it->first + it->second == next->first) {
+      it->second += next->second;
+      free_slots.erase(next);
+    }
+    if (it != free_slots.begin()) {
+      auto prev = std::prev(it);
+      if (prev->first + prev->second == it->first) {
+        prev->second += it->second;
+        free_slots.erase(it);
+      }
+    }
+  }
+
+  Tables plain_tables() const {
+    Tables tb{};
+    tb.geoms = ctx->d_geoms;
+    tb.sides = d_sides;
+    tb.slots = s;
+    tb.round.act_slot = reinterpret_cast<int32_t *>(d_plain + 0);
+    tb.round.walk_base = reinterpret_cast<int32_t *>(d_plain + 4);
+    tb.round.act_prefix = reinterpret_cast<int64_t *>(d_plain + 8);
+    tb.round.n_active = reinterpret_cast<int32_t *>(d_plain + 24);
+    tb.round.total_jobs = reinterpret_cast<int64_t *>(d_plain + 32);
+    tb.round.counter = d_counters;
+    tb.round.compact = reinterpret_cast<twos_site_result *>(d_plain + 64);
+    tb.round.decision = reinterpret_cast<int32_t *>(d_plain + 64 + sizeof(twos_site_result));
+    return tb;
+  }
+
+  Tables tables(int cls, int64_t e0) const {
+    Tables tb{};
+    tb.geoms = ctx->d_geoms;
+    tb.sides = d_sides;
+    tb.slots = s;
+    tb.round.act_slot = reinterpret_cast<int32_t *>(d_in + off_slot) + e0;
+    tb.round.walk_base = reinterpret_cast<int32_t *>(d_in + off_base) + e0;
+    tb.round.act_prefix = reinterpret_cast<int64_t *>(d_in + off_prefix) + e0 + cls;
+    tb.round.n_active = reinterpret_cast<int32_t *>(d_in + off_nact) + cls;
+    tb.round.total_jobs = reinterpret_cast<int64_t *>(d_in + off_total) + cls;
+    tb.round.counter = d_counters + cls;
+    tb.round.compact = reinterpret_cast<twos_site_result *>(d_out + off_compact) + e0;
+    tb.round.decision = reinterpret_cast<int32_t *>(d_out + off_decision) + e0;
+    return tb;
+  }
+
+  // Upload one admitted request's per-site slot data and clear its
+  // accumulators.
+  twos_status admit(Request *r) {
+    const int64_t n = r->n, s0 = r->slot0;
+    hipStream_t st = ctx->stream;
+    std::vector<int32_t> gi(n, r->geom), pi(n, r->problem), rep(n, r->prm.repeat), wmax(n), msteps(n, r->prm.max_steps);
+    std::vector<uint2> keys(2 * n);
+    std::vector<int64_t> site(n);
+    std::vector<uint64_t> seed(n, r->prm.seed);
+    std::vector<double> eps(n, r->prm.eps), rel(n), z(n, r->prm.ci_z);
+    const bool adaptive = r->prm.walks_max > r->prm.walks;
+    for (int64_t i = 0; i < n; ++i) {
+      const int64_t gsite = int64_t(r->prm.site_offset) + i;
+      site[i] = gsite;
+      keys[2 * i] = philox_key_host(r->prm.seed, gsite, 0);
+      keys[2 * i + 1] = philox_key_host(r->prm.seed, gsite, 1);
+      wmax[i] = adaptive ? r->prm.walks_max : r->prm.walks;
+      rel[i] = adaptive ? r->prm.ci_rel : -1.0;
+    }
+    HIP_OK(hipMemcpyAsync(s.pos + 3 * s0, r->sites.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(s.geom + s0, gi.data(), 4 * n, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(s.problem + s0, pi.data(), 4 * n, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(s.keys + 2 * s0, keys.data(), sizeof(uint2) * 2 * n, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(s.site + s0, site.data(), 8 * n, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(s.seed + s0, seed.data(), 8 * n, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(s.repeat + s0, rep.data(), 4 * n, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(s.walks_max + s0, wmax.data(), 4 * n, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(s.max_steps + s0, msteps.data(), 4 * n, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(s.eps + s0, eps.data(), 8 * n, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(s.ci_rel + s0, rel.data(), 8 * n, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(s.ci_z + s0, z.data(), 8 * n, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemsetAsync(s.acc + 8 * s0, 0, sizeof(double) * 8 * n, st));
+    HIP_OK(hipMemsetAsync(s.acc_walks + s0, 0, 8 * n, st));
+    HIP_OK(hipMemsetAsync(s.acc_steps + s0, 0, 8 * n, st));
+    HIP_OK(hipMemsetAsync(s.acc_caps + s0, 0, 8 * n, st));
+    HIP_OK(hipMemcpyAsync(d_sides + 2 * r->problem, r->sides, sizeof(DevSide) * 2, hipMemcpyHostToDevice, st));
+    // The host vectors above die at return: finish the copies first.
+    HIP_OK(hipStreamSynchronize(st));
+    r->done.assign(n, 0);
+    r->state.assign(n, 0);
+    r->remaining = n;
+    r->admitted = true;
+    return TWOS_OK;
+  }
+
+  twos_status admit_queued() {
+    while (!queue.empty()) {
+      Request *r = queue.front();
+      if (free_problems.empty()) break;
+      const int64_t slot0 = take_slots(r->n);
+      if (slot0 < 0) break;
+      r->slot0 = slot0;
+      r->problem = free_problems.back();
+      free_problems.pop_back();
+      queue.pop_front();
+      const twos_status st = admit(r);
+      if (st != TWOS_OK) return st;
+      active.push_back(r);
+    }
+    return TWOS_OK;
+  }
+
+  int geom_kind_of(int32_t geom) const {
+    auto it = ctx->geoms.find(geom);
+    return it == ctx->geoms.end() ? int(kGeomGeneric) : it->second.kind;
+  }
+
+  int32_t next_inc(const Request *r, int64_t i) const {
+    const int32_t d = r->done[i];
+    const int32_t wmax = r->prm.walks_max > r->prm.walks ? r->prm.walks_max : r->prm.walks;
+    if (d == 0) return r->prm.walks;
+    return std::min<int32_t>(d, wmax - d);
+  }
+
+  int persistent_block() const { return 256; }
+
+  int persistent_grid(const twos_params &p) const {
+    const int block = p.block > 0 ? p.block : persistent_block();
+    const int wps = p.waves_per_simd > 0 ? p.waves_per_simd : 8;
+    const int waves = ctx->prop.multiProcessorCount * 4 * wps;
+    return std::max(1, waves * 64 / block);
+  }
+
+  twos_status launch_walks(const Request *r, int cls, int64_t e0, int64_t total_walks, int kind) {
+    const twos_params &p = r->prm;
+    Tables tb = tables(cls, e0);
+    WalkParams wp{};
+    wp.combine = p.combine;
+    wp.rng = p.rng;
+    wp.jobs_per_walk = p.combine == kCross ? 2 : 1;
+    wp.job_begin = 0;
+    wp.job_end = total_walks * wp.jobs_per_walk;
+    wp.val = d_val;
+    wp.steps = d_steps;
+    wp.counters = d_counters + kMaxClasses;
+    hipStream_t st = ctx->stream;
+    const bool count = (p.flags & TWOS_FLAG_COUNT_TRAVERSAL) != 0;
+    const int block = p.block > 0 ? p.block : 256;
+    hipError_t e;
+    if (p.rng != kRngPhilox) {
+      e = launch_walk_simple_f64(kind, true, tb, wp, 64, st);
+    } else if (p.schedule == TWOS_PLAIN) {
+      e = p.precision == 64 ? launch_walk_simple_f64(kind, false, tb, wp, block, st)
+                            : launch_walk_simple_f32(kind, tb, wp, block, st);
+    } else {
+      HIP_OK(launch_zero(tb.round.counter, 1, st));
+      const int grid = persistent_grid(p);
+      stats.grid = grid;
+      stats.block = block;
+      stats.waves_per_simd = p.waves_per_simd > 0 ? p.waves_per_simd : 8;
+      e = p.precision == 64 ? launch_walk_persistent_f64(kind, count, tb, wp, grid, block, st)
+                            : launch_walk_persistent_f32(kind, count, tb, wp, grid, block, st);
+    }
+    if (e != hipSuccess) return fail(TWOS_ERR_DEVICE, std::string("walk launch: ") + hipGetErrorString(e));
+    stats.walk_launches += 1;
+    return TWOS_OK;
+  }
+
+  // One round over all admitted requests.  Completed request ids go to done.
+  twos_status step(std::vector<int64_t> &done, double *dump_a = nullptr, double *dump_b = nullptr,
+                   int32_t *dump_steps = nullptr, int64_t dump_stride = 0) {
+    twos_status st = admit_queued();
+    if (st != TWOS_OK) return st;
+    if (active.empty()) {
+      if (!queue.empty()) return fail(TWOS_ERR_ADMISSION, "request does not fit the engine's slots");
+      return TWOS_OK;
+    }
+    // Work list: each unfinished site gets its next rung; classes group
+    // requests that share precision and combine mode.
+    struct Entry { Request *r; int64_t i; int32_t inc; };
+    std::vector<std::vector<Entry>> by_class;
+    std::vector<int64_t> class_key;
+    // Sites that do not fit this round's walk budget wait for the next one;
+    // a site always receives its whole rung in one round.
+    int64_t budget = cap_walks;
+    bool full = false;
+    for (Request *r : active) {
+      if (full) break;
+      const int64_t key = int64_t(r->prm.precision) * 16 + r->prm.combine * 4 + (r->prm.rng != kRngPhilox ? 1 : 0);
+      int c = -1;
+      for (size_t k = 0; k < class_key.size(); ++k)
+        if (class_key[k] == key) c = int(k);
+      for (int64_t i = 0; i < r->n; ++i) {
+        if (r->state[i]) continue;
+        const int32_t inc = next_inc(r, i);
+        if (inc > budget) { full = true; break; }
+        if (c < 0) {
+          if (int(class_key.size()) == kMaxClasses) { full = true; break; }
+          class_key.push_back(key);
+          by_class.emplace_back();
+          c = int(class_key.size()) - 1;
+        }
+        by_class[c].push_back({r, i, inc});
+        budget -= inc;
+      }
+    }
+    if (by_class.empty()) return fail(TWOS_ERR_ADMISSION, "walk budget of one round below one site's rung");
+    auto *a_slot = reinterpret_cast<int32_t *>(h_in + off_slot);
+    auto *a_base = reinterpret_cast<int32_t *>(h_in + off_base);
+    auto *a_prefix = reinterpret_cast<int64_t *>(h_in + off_prefix);
+    auto *a_nact = reinterpret_cast<int32_t *>(h_in + off_nact);
+    auto *a_total = reinterpret_cast<int64_t *>(h_in + off_total);
+    std::vector<int64_t> e0s(by_class.size()), totals(by_class.size());
+    int64_t e = 0;
+    for (size_t c = 0; c < by_class.size(); ++c) {
+      e0s[c] = e;
+      int64_t run = 0;
+      for (size_t k = 0; k < by_class[c].size(); ++k) {
+        const Entry &en = by_class[c][k];
+        a_slot[e + k] = int32_t(en.r->slot0 + en.i);
+        a_base[e + k] = en.r->done[en.i];
+        a_prefix[e + c + k] = run;
+        run += en.inc;
+      }
+      a_prefix[e + c + by_class[c].size()] = run;
+      a_nact[c] = int32_t(by_class[c].size());
+      a_total[c] = run * (by_class[c].front().r->prm.combine == kCross ? 2 : 1);
+      totals[c] = run;
+      e += int64_t(by_class[c].size());
+    }
+    const int64_t n_entries = e;
+    hipStream_t strm = ctx->stream;
+    const bool single = by_class.size() == 1;
+    Request *r0 = by_class[0].front().r;
+    const bool plain = r0->prm.schedule == TWOS_PLAIN;
+    const bool graphable = single && !plain && r0->prm.use_graph && r0->prm.rng == kRngPhilox && !dump_a &&
+                           !(r0->prm.flags & TWOS_FLAG_COUNT_TRAVERSAL);
+    int kind0 = geom_kind_of(r0->geom);
+    for (const Entry &en : by_class[0])
+      if (geom_kind_of(en.r->geom) != kind0) kind0 = kGeomGeneric;
+
+    if (plain) {
+      // One launch per site: a one-entry work list, the site's walks one
+      // thread per path, one fold, read back, synchronise.
+      const Tables tb = plain_tables();
+      for (size_t c = 0; c < by_class.size(); ++c) {
+        for (size_t k = 0; k < by_class[c].size(); ++k) {
+          const Entry &en = by_class[c][k];
+          const int jpw = en.r->prm.combine == kCross ? 2 : 1;
+          *reinterpret_cast<int32_t *>(h_plain + 0) = int32_t(en.r->slot0 + en.i);
+          *reinterpret_cast<int32_t *>(h_plain + 4) = en.r->done[en.i];
+          *reinterpret_cast<int64_t *>(h_plain + 8) = 0;
+          *reinterpret_cast<int64_t *>(h_plain + 16) = en.inc;
+          *reinterpret_cast<int32_t *>(h_plain + 24) = 1;
+          *reinterpret_cast<int64_t *>(h_plain + 32) = int64_t(en.inc) * jpw;
+          HIP_OK(hipMemcpyAsync(d_plain, h_plain, 64, hipMemcpyHostToDevice, strm));
+          HIP_OK(hipEventRecord(ev[0], strm));
+          WalkParams wp{};
+          wp.combine = en.r->prm.combine;
+          wp.rng = en.r->prm.rng;
+          wp.jobs_per_walk = jpw;
+          wp.job_begin = 0;
+          wp.job_end = int64_t(en.inc) * jpw;
+          wp.val = d_val;
+          wp.steps = d_steps;
+          const int block = en.r->prm.block > 0 ? en.r->prm.block : 256;
+          const int kind = geom_kind_of(en.r->geom);
+          const hipError_t le = en.r->prm.rng != kRngPhilox ? launch_walk_simple_f64(kind, true, tb, wp, 64, strm)
+                                : en.r->prm.precision == 64 ? launch_walk_simple_f64(kind, false, tb, wp, block, strm)
+                                                            : launch_walk_simple_f32(kind, tb, wp, block, strm);
+          if (le != hipSuccess) return fail(TWOS_ERR_DEVICE, std::string("walk launch: ") + hipGetErrorString(le));
+          stats.walk_launches += 1;
+          HIP_OK(hipEventRecord(ev[1], strm));
+          if (dump_a) {
+            HIP_OK(hipStreamSynchronize(strm));
+            const std::vector<Entry> one{en};
+            st = dump_round(one, dump_a, dump_b, dump_steps, dump_stride);
+            if (st != TWOS_OK) return st;
+          }
+          HIP_OK(launch_reduce(tb, wp, 1, strm));
+          HIP_OK(hipEventRecord(ev[2], strm));
+          HIP_OK(hipMemcpyAsync(h_plain + 64, d_plain + 64, sizeof(twos_site_result) + 4, hipMemcpyDeviceToHost, strm));
+          HIP_OK(hipStreamSynchronize(strm));
+          float ms = 0.f;
+          HIP_OK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+          stats.walk_kernel_ms += ms;
+          HIP_OK(hipEventElapsedTime(&ms, ev[1], ev[2]));
+          stats.reduce_kernel_ms += ms;
+          twos_site_result res;
+          int32_t dec;
+          std::memcpy(&res, h_plain + 64, sizeof(res));
+          std::memcpy(&dec, h_plain + 64 + sizeof(twos_site_result), 4);
+          finish_entry(en.r, en.i, en.inc, res, dec);
+        }
+      }
+    } else if (graphable) {
+      const int64_t key = int64_t(r0->prm.precision) * 1000003 + r0->prm.combine * 1009 + kind0 * 7 +
+                          persistent_grid(r0->prm) * 100000007LL + (r0->prm.block > 0 ? r0->prm.block : 256);
+      if (graph_key != key) {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+        HIP_OK(hipStreamBeginCapture(strm, hipStreamCaptureModeThreadLocal));
+        hipError_t ce = hipMemcpyAsync(d_in, h_in, in_bytes, hipMemcpyHostToDevice, strm);
+        twos_status ls = ce == hipSuccess ? launch_walks(r0, 0, 0, 0, kind0) : TWOS_ERR_DEVICE;
+        Tables tb = tables(0, 0);
+        WalkParams wp{};
+        wp.val = d_val;
+        wp.steps = d_steps;
+        if (ls == TWOS_OK) ce = launch_reduce(tb, wp, int(cap_sites), strm);
+        if (ls == TWOS_OK && ce == hipSuccess)
+          ce = hipMemcpyAsync(h_out, d_out, out_bytes, hipMemcpyDeviceToHost, strm);
+        hipGraph_t g = nullptr;
+        const hipError_t ee = hipStreamEndCapture(strm, &g);
+        if (ls != TWOS_OK) return ls;
+        HIP_OK(ce);
+        HIP_OK(ee);
+        graph = g;
+        HIP_OK(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+        graph_key = key;
+        stats.walk_launches -= 1;  // counted once per replay below
+      }
+      HIP_OK(hipEventRecord(ev[0], strm));
+      HIP_OK(hipGraphLaunch(graph_exec, strm));
+      HIP_OK(hipEventRecord(ev[1], strm));
+      HIP_OK(hipStreamSynchronize(strm));
+      stats.walk_launches += 1;
+      float ms = 0.f;
+      HIP_OK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+      stats.walk_kernel_ms += ms;
+      const auto *res = reinterpret_cast<twos_site_result *>(h_out + off_compact);
+      const auto *dec = reinterpret_cast<int32_t *>(h_out + off_decision);
+      for (size_t k = 0; k < by_class[0].size(); ++k)
+        finish_entry(by_class[0][k].r, by_class[0][k].i, by_class[0][k].inc, res[k], dec[k]);
+    } else {
+      HIP_OK(hipMemcpyAsync(d_in, h_in, in_bytes, hipMemcpyHostToDevice, strm));
+      for (size_t c = 0; c < by_class.size(); ++c) {
+        Request *rc = by_class[c].front().r;
+        int kind = geom_kind_of(rc->geom);
+        for (const Entry &en : by_class[c])
+          if (geom_kind_of(en.r->geom) != kind) kind = kGeomGeneric;
+        if (rc->prm.flags & TWOS_FLAG_COUNT_TRAVERSAL) HIP_OK(launch_zero(d_counters + kMaxClasses, 2, strm));
+        HIP_OK(hipEventRecord(ev[0], strm));
+        st = launch_walks(rc, int(c), e0s[c], totals[c], kind);
+        if (st != TWOS_OK) return st;
+        HIP_OK(hipEventRecord(ev[1], strm));
+        if (dump_a) {
+          HIP_OK(hipStreamSynchronize(strm));
+          st = dump_round(by_class[c], dump_a, dump_b, dump_steps, dump_stride);
+          if (st != TWOS_OK) return st;
+        }
+        Tables tb = tables(int(c), e0s[c]);
+        WalkParams wp{};
+        wp.val = d_val;
+        wp.steps = d_steps;
+        HIP_OK(launch_reduce(tb, wp, int(by_class[c].size()), strm));
+        HIP_OK(hipEventRecord(ev[2], strm));
+        HIP_OK(hipStreamSynchronize(strm));
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        stats.walk_kernel_ms += ms;
+        HIP_OK(hipEventElapsedTime(&ms, ev[1], ev[2]));
+        stats.reduce_kernel_ms += ms;
+        if (rc->prm.flags & TWOS_FLAG_COUNT_TRAVERSAL) {
+          unsigned long long cnt[2];
+          HIP_OK(hipMemcpy(cnt, d_counters + kMaxClasses, sizeof(cnt), hipMemcpyDeviceToHost));
+          stats.nodes_visited += int64_t(cnt[0]);
+          stats.tris_tested += int64_t(cnt[1]);
+        }
+      }
+      HIP_OK(hipMemcpyAsync(h_out + off_compact, d_out + off_compact, sizeof(twos_site_result) * n_entries,
+                            hipMemcpyDeviceToHost, strm));
+      HIP_OK(hipMemcpyAsync(h_out + off_decision, d_out + off_decision, 4 * n_entries, hipMemcpyDeviceToHost,
+                            strm));
+      HIP_OK(hipStreamSynchronize(strm));
+      const auto *res = reinterpret_cast<twos_site_result *>(h_out + off_compact);
+      const auto *dec = reinterpret_cast<int32_t *>(h_out + off_decision);
+      for (size_t c = 0; c < by_class.size(); ++c)
+        for (size_t k = 0; k < by_class[c].size(); ++k)
+          finish_entry(by_class[c][k].r, by_class[c][k].i, by_class[c][k].inc, res[e0s[c] + k],
+                       dec[e0s[c] + k]);
+    }
+    stats.rounds += 1;
+    // Retire completed requests.
+    for (size_t k = 0; k < active.size();) {
+      Request *r = active[k];
+      if (r->remaining == 0) {
+        give_slots(r->slot0, r->n);
+        free_problems.push_back(r->problem);
+        done.push_back(r->id);
+        active.erase(active.begin() + k);
+      } else {
+        ++k;
+      }
+    }
+    return TWOS_OK;
+  }
+
+  void finish_entry(Request *r, int64_t i, int32_t inc, const twos_site_result &res, int32_t decision) {
+    r->done[i] += inc;
+    if (decision != 1) {
+      r->state[i] = 1;
+      r->remaining -= 1;
+      if (r->out) r->out[i] = res;
+    }
+  }
+
+  template <typename E>
+  twos_status dump_round(const std::vector<E> &entries, double *da, double *db, int32_t *ds, int64_t stride) {
+    int64_t total = 0;
+    for (const E &en : entries) total += en.inc;
+    std::vector<double> val(2 * size_t(total));
+    std::vector<int32_t> stp(2 * size_t(total));
+    HIP_OK(hipMemcpy(val.data(), d_val, sizeof(double) * val.size(), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(stp.data(), d_steps, sizeof(int32_t) * stp.size(), hipMemcpyDeviceToHost));
+    int64_t t = 0;
+    for (const E &en : entries) {
+      for (int32_t w = 0; w < en.inc; ++w, ++t) {
+        const int64_t walk = en.r->done[en.i] + w;
+        if (walk >= stride) continue;
+        const int64_t at = en.i * stride + walk;
+        da[at] = val[2 * t];
+        db[at] = val[2 * t + 1];
+        const int32_t s0 = stp[2 * t], s1 = stp[2 * t + 1];
+        ds[at] = ((s0 & 0x3fffffff) + (s1 & 0x3fffffff)) | ((s0 | s1) & (1 << 30));
+      }
+    }
+    return TWOS_OK;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Validation shared by the estimate and server entry points.
+
+twos_status check_params(const twos_params *p, int64_t n) {
+  if (!p) return fail(TWOS_ERR_INVALID_ARGUMENT, "params is null");
+  if (n < 0) return fail(TWOS_ERR_INVALID_ARGUMENT, "negative site count");
+  if (p->walks < 2) return fail(TWOS_ERR_INVALID_ARGUMENT, "walks must be at least 2");
+  if (p->max_steps < 1) return fail(TWOS_ERR_INVALID_ARGUMENT, "max_steps must be at least 1");
+  if (!(p->eps > 0.0)) return fail(TWOS_ERR_INVALID_ARGUMENT, "eps must be positive");
+  if (p->combine < 0 || p->combine > 2) return fail(TWOS_ERR_INVALID_ARGUMENT, "unknown combine mode");
+  if (p->rng < 0 || p->rng > 3) return fail(TWOS_ERR_INVALID_ARGUMENT, "unknown stream mode");
+  if (p->precision != 32 && p->precision != 64) return fail(TWOS_ERR_INVALID_ARGUMENT, "precision must be 32 or 64");
+  if (p->rng != kRngPhilox && p->precision != 64)
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "mt19937_64 replay streams run in binary64 only");
+  if (p->rng != kRngPhilox && p->combine == kCross)
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "mt19937_64 replay streams have no independent adjoint stream");
+  if (p->rng == kRngMtKey) return fail(TWOS_ERR_INVALID_ARGUMENT, "MT_KEY streams are for twos_trace");
+  if (p->schedule != TWOS_PLAIN && p->schedule != TWOS_PERSISTENT)
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "unknown schedule");
+  if (p->walks_max > p->walks && !(p->ci_rel >= 0.0))
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "adaptive budgets need ci_rel >= 0");
+  if (p->block != 0 && (p->block % 64 != 0 || p->block > 256 || p->block < 64))
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "block must be 64, 128, 192 or 256");
+  if (p->waves_per_simd < 0 || p->waves_per_simd > 16)
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "waves_per_simd must be in [0, 16]");
+  return TWOS_OK;
+}
+
+Request *make_request(twos_ctx *ctx, int32_t geom, const twos_side sides[2], const twos_params *p,
+                      const double *sites, int64_t n, twos_site_result *out, twos_status *st) {
+  *st = TWOS_OK;
+  if (ctx->geoms.find(geom) == ctx->geoms.end()) {
+    *st = fail(TWOS_ERR_NOT_FOUND, "unknown geometry id");
+    return nullptr;
+  }
+  if (!sides) {
+    *st = fail(TWOS_ERR_INVALID_ARGUMENT, "sides is null");
+    return nullptr;
+  }
+  if (n > 0 && (!sites || !out)) {
+    *st = fail(TWOS_ERR_INVALID_ARGUMENT, "sites and out must be non-null");
+    return nullptr;
+  }
+  for (int64_t i = 0; i < 3 * n; ++i) {
+    if (!std::isfinite(sites[i])) {
+      *st = fail(TWOS_ERR_INVALID_ARGUMENT, "non-finite site coordinate");
+      return nullptr;
+    }
+  }
+  auto *r = new Request;
+  r->geom = geom;
+  r->prm = *p;
+  r->sides[0] = sides[0];
+  r->sides[1] = sides[1];
+  r->sites.assign(sites, sites + 3 * n);
+  r->n = n;
+  r->out = out;
+  return r;
+}
+
+// Context-level helpers.
+twos_status upload_geom_table(twos_ctx *ctx) {
+  int32_t need = ctx->next_geom;
+  if (need > ctx->d_geoms_cap) {
+    int32_t cap = std::max(16, ctx->d_geoms_cap * 2);
+    while (cap < need) cap *= 2;
+    DevGeom *p = nullptr;
+    HIP_OK(dalloc(&p, size_t(cap)));
+    dfree(ctx->d_geoms);
+    ctx->d_geoms = p;
+    ctx->d_geoms_cap = cap;
+    if (ctx->engine) ctx->engine->graph_key = -1;  // tables() pointer changed
+  }
+  std::vector<DevGeom> table(size_t(std::max(need, 1)));
+  for (auto &kv : ctx->geoms) table[size_t(kv.first)] = kv.second.dev;
+  HIP_OK(hipMemcpy(ctx->d_geoms, table.data(), sizeof(DevGeom) * table.size(), hipMemcpyHostToDevice));
+  return TWOS_OK;
+}
+
+template <typename T>
+twos_status upload_owned(GeomEntry &g, const T *host, size_t count, T **dev) {
+  *dev = nullptr;
+  if (count == 0) return TWOS_OK;
+  HIP_OK(dalloc(dev, count));
+  g.owned.push_back(*dev);
+  HIP_OK(hipMemcpy(*dev, host, sizeof(T) * count, hipMemcpyHostToDevice));
+  return TWOS_OK;
+}
+
+twos_status build_geom(twos_ctx *ctx, GeomEntry &g) {
+  const twos_geometry_desc &d = g.desc;
+  DevGeom &v = g.dev;
+  v = DevGeom{};
+  v.container = d.container;
+  v.n_spheres = d.n_spheres;
+  v.n_cylinders = d.n_cylinders;
+  v.n_cavities = d.n_cavities;
+  v.mesh_radius_f32 = d.mesh_radius_f32;
+  for (int k = 0; k < 4; ++k) v.ball[k] = d.ball[k];
+  for (int k = 0; k < 3; ++k) { v.box_lo[k] = d.box_lo[k]; v.box_hi[k] = d.box_hi[k]; }
+  v.cavity_radius = d.cavity_radius;
+  std::vector<float4> sf(g.spheres.size() / 4), cf(g.cylinders.size() / 4), vf(g.cavities.size() / 3);
+  std::vector<double> cav4(4 * vf.size());
+  for (size_t i = 0; i < sf.size(); ++i)
+    sf[i] = make_float4(float(g.spheres[4 * i]), float(g.spheres[4 * i + 1]), float(g.spheres[4 * i + 2]),
+                        float(g.spheres[4 * i + 3]));
+  for (size_t i = 0; i < cf.size(); ++i)
+    cf[i] = make_float4(float(g.cylinders[4 * i]), float(g.cylinders[4 * i + 1]), float(g.cylinders[4 * i + 2]),
+                        float(g.cylinders[4 * i + 3]));
+  for (size_t i = 0; i < vf.size(); ++i) {
+    vf[i] = make_float4(float(g.cavities[3 * i]), float(g.cavities[3 * i + 1]), float(g.cavities[3 * i + 2]), 0.f);
+    for (int k = 0; k < 3; ++k) cav4[4 * i + k] = g.cavities[3 * i + k];
+    cav4[4 * i + 3] = 0.0;
+  }
+  twos_status st;
+  float4 *p4 = nullptr;
+  double *pd = nullptr;
+  if ((st = upload_owned(g, sf.data(), sf.size(), &p4)) != TWOS_OK) return st;
+  v.spheres_f = p4;
+  if ((st = upload_owned(g, cf.data(), cf.size(), &p4)) != TWOS_OK) return st;
+  v.cylinders_f = p4;
+  if ((st = upload_owned(g, vf.data(), vf.size(), &p4)) != TWOS_OK) return st;
+  v.cavities_f = p4;
+  if ((st = upload_owned(g, g.spheres.data(), g.spheres.size(), &pd)) != TWOS_OK) return st;
+  v.spheres_d = pd;
+  if ((st = upload_owned(g, g.cylinders.data(), g.cylinders.size(), &pd)) != TWOS_OK) return st;
+  v.cylinders_d = pd;
+  if ((st = upload_owned(g, cav4.data(), cav4.size(), &pd)) != TWOS_OK) return st;
+  v.cavities_d = pd;
+  if (d.container == kMesh) {
+    auto it = ctx->meshes.find(d.mesh);
+    if (it == ctx->meshes.end()) return fail(TWOS_ERR_NOT_FOUND, "unknown mesh id");
+    const MeshBuffers &mb = it->second.mb;
+    v.nodes = mb.nodes;
+    v.tris_f = mb.tris_f;
+    v.tris_d = mb.tris_d;
+    v.tri_ids = mb.tri_ids;
+    v.n_tris = mb.n_tris;
+    it->second.users += 1;
+  }
+  g.kind = d.container == kMesh ? (g.spheres.empty() && g.cylinders.empty() ? int(kGeomMesh) : int(kGeomGeneric))
+                                : int(kGeomAnalytic);
+  return TWOS_OK;
+}
+
+void free_geom(twos_ctx *ctx, GeomEntry &g) {
+  for (void *p : g.owned) (void)hipFree(p);
+  g.owned.clear();
+  if (g.desc.container == kMesh) {
+    auto it = ctx->meshes.find(g.desc.mesh);
+    if (it != ctx->meshes.end()) it->second.users -= 1;
+  }
+}
+
+twos_status ensure_engine(twos_ctx *ctx, const twos_params *p, int64_t n) {
+  const int64_t walks_max = std::max(p->walks, p->walks_max);
+  const int64_t want_sites = std::min<int64_t>(std::max<int64_t>(n, 1), 1 << 20);
+  const int64_t limit_walks = int64_t(ctx->budget / 2 / 24);
+  int64_t want_walks = std::min<int64_t>(want_sites * walks_max, limit_walks);
+  want_walks = std::max<int64_t>(want_walks, walks_max);
+  Engine *e = ctx->engine.get();
+  if (e && e->cap_sites >= want_sites && e->cap_walks >= want_walks) return TWOS_OK;
+  auto ne = std::make_unique<Engine>();
+  ne->ctx = ctx;
+  const int64_t sites = std::max<int64_t>(want_sites, e ? e->cap_sites : 0);
+  const int64_t walks = std::max<int64_t>(want_walks, e ? e->cap_walks : 0);
+  ctx->engine.reset();
+  const twos_status st = ne->allocate(sites, walks, 64);
+  if (st != TWOS_OK) return st;
+  ctx->engine = std::move(ne);
+  return TWOS_OK;
+}
+
+twos_status run_estimate(twos_ctx *ctx, int32_t geom, const twos_side sides[2], const twos_params *p,
+                         const double *sites, int64_t n, twos_site_result *out, twos_stats *stats, double *da,
+                         double *db, int32_t *ds, int64_t stride) {
+  twos_status st = check_params(p, n);
+  if (st != TWOS_OK) return st;
+  if (n == 0) {
+    if (stats) *stats = twos_stats{};
+    return TWOS_OK;
+  }
+  const double t0 = now_s();
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIP_OK(hipSetDevice(ctx->device));
+  st = ensure_engine(ctx, p, n);
+  if (st != TWOS_OK) return st;
+  Engine *e = ctx->engine.get();
+  if (!e->idle()) return fail(TWOS_ERR_INVALID_ARGUMENT, "context engine busy");
+  e->stats = twos_stats{};
+  // Chunk the sites so each chunk fits the engine's slots; the global site
+  // index (site_offset + i) keeps every stream identical to one big call.
+  const int64_t chunk = e->cap_sites;
+  bool capped = false;
+  for (int64_t b = 0; b < n; b += chunk) {
+    const int64_t m = std::min(chunk, n - b);
+    twos_params q = *p;
+    q.site_offset = p->site_offset + int32_t(b);
+    Request *r = make_request(ctx, geom, sides, &q, sites + 3 * b, m, out + b, &st);
+    if (!r) return st;
+    std::unique_ptr<Request> hold(r);
+    r->id = e->next_id++;
+    e->queue.push_back(r);
+    std::vector<int64_t> done;
+    while (!e->idle()) {
+      st = e->step(done, da ? da + b * stride : nullptr, db ? db + b * stride : nullptr, ds ? ds + b * stride : nullptr,
+                   stride);
+      if (st != TWOS_OK) {
+        e->queue.clear();
+        e->active.clear();
+        (void)e->allocate(e->cap_sites, e->cap_walks, e->cap_problems);
+        return st;
+      }
+    }
+    for (int64_t i = 0; i < m; ++i) capped = capped || out[b + i].caps > 0;
+  }
+  e->stats.seconds = now_s() - t0;
+  e->stats.paths = 0;
+  int64_t steps = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    steps += out[i].steps;
+    e->stats.paths += out[i].walks * (p->combine == kCross ? 2 : 1);
+  }
+  e->stats.steps = steps;
+  if (stats) *stats = e->stats;
+  if (capped) return fail(TWOS_ERR_STEP_CAP, "one or more paths hit --max-steps");
+  return TWOS_OK;
+}
+
+}  // namespace twos
+
+struct twos_server {
+  twos_ctx *ctx = nullptr;
+  std::unique_ptr<twos::Engine> engine;
+  std::map<int64_t, std::unique_ptr<twos::Request>> requests;
+};
+
+using namespace twos;
+
+extern "C" {
+
+const char *twos_version(void) { return "twos-mi355x 0.1 (gfx950)"; }
+
+const char *twos_last_error(void) { return g_error.c_str(); }
+
+twos_status twos_create(int32_t device, uint64_t hbm_budget_bytes, twos_ctx **out) {
+  if (!out) return fail(TWOS_ERR_INVALID_ARGUMENT, "out is null");
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) return fail(TWOS_ERR_DEVICE, "no HIP device available");
+  if (device < 0 || device >= count) return fail(TWOS_ERR_INVALID_ARGUMENT, "device index out of range");
+  auto ctx = std::make_unique<twos_ctx>();
+  ctx->device = device;
+  HIP_OK(hipSetDevice(device));
+  HIP_OK(hipGetDeviceProperties(&ctx->prop, device));
+  HIP_OK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  size_t free_b = 0, total_b = 0;
+  HIP_OK(hipMemGetInfo(&free_b, &total_b));
+  ctx->budget = hbm_budget_bytes ? hbm_budget_bytes : uint64_t(free_b / 4);
+  *out = ctx.release();
+  return TWOS_OK;
+}
+
+void twos_destroy(twos_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->engine.reset();
+  for (auto &kv : ctx->geoms) free_geom(ctx, kv.second);
+  for (auto &kv : ctx->meshes) {
+    MeshBuffers &mb = kv.second.mb;
+    (void)hipFree(mb.nodes); (void)hipFree(mb.tris_f); (void)hipFree(mb.tris_d); (void)hipFree(mb.tri_ids);
+  }
+  (void)hipFree(ctx->d_geoms);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+twos_status twos_mesh_create(twos_ctx *ctx, const double *vertices, int64_t nv, const uint32_t *faces, int64_t nf,
+                             int32_t *mesh_id, double *build_ms) {
+  if (!ctx || !vertices || !faces || !mesh_id) return fail(TWOS_ERR_INVALID_ARGUMENT, "null argument");
+  if (nv <= 0 || nf <= 0) return fail(TWOS_ERR_INVALID_ARGUMENT, "expected a nonempty triangle mesh");
+  if (nf >= (int64_t(1) << 30)) return fail(TWOS_ERR_INVALID_ARGUMENT, "too many triangles");
+  for (int64_t i = 0; i < 3 * nf; ++i)
+    if (faces[i] >= uint64_t(nv)) return fail(TWOS_ERR_INVALID_ARGUMENT, "expected valid triangular faces");
+  for (int64_t i = 0; i < 3 * nv; ++i)
+    if (!std::isfinite(vertices[i])) return fail(TWOS_ERR_INVALID_ARGUMENT, "non-finite vertex");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIP_OK(hipSetDevice(ctx->device));
+  double *d_v = nullptr;
+  uint32_t *d_f = nullptr;
+  HIP_OK(dalloc(&d_v, size_t(3 * nv)));
+  HIP_OK(dalloc(&d_f, size_t(3 * nf)));
+  HIP_OK(hipMemcpy(d_v, vertices, sizeof(double) * 3 * nv, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d_f, faces, sizeof(uint32_t) * 3 * nf, hipMemcpyHostToDevice));
+  MeshEntry m;
+  hipEvent_t a, b;
+  HIP_OK(hipEventCreate(&a));
+  HIP_OK(hipEventCreate(&b));
+  HIP_OK(hipEventRecord(a, ctx->stream));
+  const hipError_t e = lbvh_build(d_v, d_f, int32_t(nf), m.mb, &m.depth, ctx->stream);
+  (void)hipEventRecord(b, ctx->stream);
+  (void)hipStreamSynchronize(ctx->stream);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, a, b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  (void)hipFree(d_v);
+  (void)hipFree(d_f);
+  if (e != hipSuccess) return fail(TWOS_ERR_DEVICE, std::string("LBVH build: ") + hipGetErrorString(e));
+  m.build_ms = ms;
+  m.bytes = int64_t(nf > 1 ? nf - 1 : 1) * 64 + nf * (36 + 72 + 4);
+  if (m.depth >= kStackDepth) {
+    (void)hipFree(m.mb.nodes); (void)hipFree(m.mb.tris_f); (void)hipFree(m.mb.tris_d); (void)hipFree(m.mb.tri_ids);
+    return fail(TWOS_ERR_GEOMETRY, "LBVH deeper than the traversal stack");
+  }
+  const int32_t id = ctx->next_mesh++;
+  ctx->meshes[id] = m;
+  *mesh_id = id;
+  if (build_ms) *build_ms = ms;
+  return TWOS_OK;
+}
+
+twos_status twos_mesh_info(twos_ctx *ctx, int32_t mesh_id, int32_t *depth, int64_t *nodes, int64_t *bytes) {
+  if (!ctx) return fail(TWOS_ERR_INVALID_ARGUMENT, "null context");
+  auto it = ctx->meshes.find(mesh_id);
+  if (it == ctx->meshes.end()) return fail(TWOS_ERR_NOT_FOUND, "unknown mesh id");
+  if (depth) *depth = it->second.depth;
+  if (nodes) *nodes = std::max<int64_t>(it->second.mb.n_tris - 1, 1);
+  if (bytes) *bytes = it->second.bytes;
+  return TWOS_OK;
+}
+
+twos_status twos_mesh_release(twos_ctx *ctx, int32_t mesh_id) {
+  if (!ctx) return fail(TWOS_ERR_INVALID_ARGUMENT, "null context");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  auto it = ctx->meshes.find(mesh_id);
+  if (it == ctx->meshes.end()) return fail(TWOS_ERR_NOT_FOUND, "unknown mesh id");
+  if (it->second.users > 0) return fail(TWOS_ERR_INVALID_ARGUMENT, "mesh still used by a geometry");
+  MeshBuffers &mb = it->second.mb;
+  (void)hipFree(mb.nodes); (void)hipFree(mb.tris_f); (void)hipFree(mb.tris_d); (void)hipFree(mb.tri_ids);
+  ctx->meshes.erase(it);
+  return TWOS_OK;
+}
+
+twos_status twos_geometry_create(twos_ctx *ctx, const twos_geometry_desc *desc, int32_t *geom_id) {
+  if (!ctx || !desc || !geom_id) return fail(TWOS_ERR_INVALID_ARGUMENT, "null argument");
+  const twos_geometry_desc &d = *desc;
+  if (d.container < 0 || d.container > 2) return fail(TWOS_ERR_INVALID_ARGUMENT, "unknown container");
+  if (d.n_spheres < 0 || d.n_cylinders < 0 || d.n_cavities < 0)
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "negative primitive count");
+  if ((d.n_spheres && !d.spheres) || (d.n_cylinders && !d.cylinders) || (d.n_cavities && !d.cavities))
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "missing primitive rows");
+  if (d.n_cavities && !(d.cavity_radius > 0.0))
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "cavities need a positive cavity_radius");
+  if (d.container == kBall && !(d.ball[3] > 0.0)) return fail(TWOS_ERR_INVALID_ARGUMENT, "ball radius must be positive");
+  if (d.container == kBox)
+    for (int k = 0; k < 3; ++k)
+      if (!(d.box_hi[k] > d.box_lo[k])) return fail(TWOS_ERR_INVALID_ARGUMENT, "empty box");
+  for (int i = 0; i < d.n_cylinders; ++i) {
+    const double axis = d.cylinders[4 * i];
+    if (axis != 0.0 && axis != 1.0 && axis != 2.0) return fail(TWOS_ERR_INVALID_ARGUMENT, "cylinder axis must be 0, 1 or 2");
+  }
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIP_OK(hipSetDevice(ctx->device));
+  GeomEntry g;
+  g.desc = d;
+  g.spheres.assign(d.spheres, d.spheres + 4 * size_t(d.n_spheres));
+  g.cylinders.assign(d.cylinders, d.cylinders + 4 * size_t(d.n_cylinders));
+  g.cavities.assign(d.cavities, d.cavities + 3 * size_t(d.n_cavities));
+  g.desc.spheres = g.desc.cylinders = g.desc.cavities = nullptr;
+  twos_status st = build_geom(ctx, g);
+  if (st != TWOS_OK) {
+    free_geom(ctx, g);
+    return st;
+  }
+  const int32_t id = ctx->next_geom++;
+  ctx->geoms[id] = std::move(g);
+  st = upload_geom_table(ctx);
+  if (st != TWOS_OK) return st;
+  *geom_id = id;
+  return TWOS_OK;
+}
+
+twos_status twos_geometry_union_sphere(twos_ctx *ctx, int32_t base, const double center[3], double radius,
+                                       int32_t *geom_id) {
+  if (!ctx || !center || !geom_id) return fail(TWOS_ERR_INVALID_ARGUMENT, "null argument");
+  if (!(radius > 0.0)) return fail(TWOS_ERR_INVALID_ARGUMENT, "radius must be positive");
+  twos_geometry_desc d;
+  std::vector<double> sp, cy, cv;
+  {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    auto it = ctx->geoms.find(base);
+    if (it == ctx->geoms.end()) return fail(TWOS_ERR_NOT_FOUND, "unknown geometry id");
+    d = it->second.desc;
+    sp = it->second.spheres;
+    cy = it->second.cylinders;
+    cv = it->second.cavities;
+  }
+  // A cavity of another radius than the base's cavities becomes an analytic
+  // sphere hole (class 6 data); a base without cavities takes it as its cavity.
+  if (d.n_cavities == 0 || d.cavity_radius == radius) {
+    cv.insert(cv.end(), center, center + 3);
+    d.n_cavities += 1;
+    d.cavity_radius = radius;
+  } else {
+    sp.insert(sp.end(), {center[0], center[1], center[2], radius});
+    d.n_spheres += 1;
+  }
+  d.spheres = sp.data();
+  d.cylinders = cy.data();
+  d.cavities = cv.data();
+  return twos_geometry_create(ctx, &d, geom_id);
+}
+
+twos_status twos_geometry_release(twos_ctx *ctx, int32_t geom_id) {
+  if (!ctx) return fail(TWOS_ERR_INVALID_ARGUMENT, "null context");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  auto it = ctx->geoms.find(geom_id);
+  if (it == ctx->geoms.end()) return fail(TWOS_ERR_NOT_FOUND, "unknown geometry id");
+  if (ctx->engine && !ctx->engine->idle()) return fail(TWOS_ERR_INVALID_ARGUMENT, "engine busy");
+  free_geom(ctx, it->second);
+  ctx->geoms.erase(it);
+  return upload_geom_table(ctx);
+}
+
+twos_status twos_estimate(twos_ctx *ctx, int32_t geom_id, const twos_side sides[2], const twos_params *params,
+                          const double *sites, int64_t n, twos_site_result *out, twos_stats *stats) {
+  if (!ctx) return fail(TWOS_ERR_INVALID_ARGUMENT, "null context");
+  return run_estimate(ctx, geom_id, sides, params, sites, n, out, stats, nullptr, nullptr, nullptr, 0);
+}
+
+twos_status twos_estimate_dump(twos_ctx *ctx, int32_t geom_id, const twos_side sides[2], const twos_params *params,
+                               const double *sites, int64_t n, twos_site_result *out, twos_stats *stats,
+                               double *dump_a, double *dump_b, int32_t *dump_steps, int64_t stride) {
+  if (!ctx) return fail(TWOS_ERR_INVALID_ARGUMENT, "null context");
+  if (!dump_a || !dump_b || !dump_steps || stride <= 0) return fail(TWOS_ERR_INVALID_ARGUMENT, "dump buffers");
+  if (params && params->use_graph) return fail(TWOS_ERR_INVALID_ARGUMENT, "dumps run without graphs");
+  return run_estimate(ctx, geom_id, sides, params, sites, n, out, stats, dump_a, dump_b, dump_steps, stride);
+}
+
+twos_status twos_inside(twos_ctx *ctx, int32_t mesh_id, const double *points, int64_t n, int32_t *out) {
+  if (!ctx || (n > 0 && (!points || !out))) return fail(TWOS_ERR_INVALID_ARGUMENT, "null argument");
+  if (n <= 0) return TWOS_OK;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  auto it = ctx->meshes.find(mesh_id);
+  if (it == ctx->meshes.end()) return fail(TWOS_ERR_NOT_FOUND, "unknown mesh id");
+  HIP_OK(hipSetDevice(ctx->device));
+  double *d_p = nullptr;
+  int32_t *d_o = nullptr;
+  HIP_OK(dalloc(&d_p, size_t(3 * n)));
+  HIP_OK(dalloc(&d_o, size_t(n)));
+  hipError_t e = hipMemcpyAsync(d_p, points, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = launch_inside(it->second.mb, d_p, n, d_o, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d_o, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d_p);
+  (void)hipFree(d_o);
+  if (e != hipSuccess) return fail(TWOS_ERR_DEVICE, std::string("inside: ") + hipGetErrorString(e));
+  for (int64_t i = 0; i < n; ++i)
+    if (out[i] < 0) return fail(TWOS_ERR_GEOMETRY, "ray-parity traversal exceeded 128 intersections");
+  return TWOS_OK;
+}
+
+twos_status twos_closest(twos_ctx *ctx, int32_t geom_id, int32_t precision, const double *points, int64_t n,
+                         double *distance, double *closest, int32_t *cls) {
+  if (!ctx || (n > 0 && (!points || !distance || !closest || !cls)))
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "null argument");
+  if (precision != 32 && precision != 64) return fail(TWOS_ERR_INVALID_ARGUMENT, "precision must be 32 or 64");
+  if (n <= 0) return TWOS_OK;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (ctx->geoms.find(geom_id) == ctx->geoms.end()) return fail(TWOS_ERR_NOT_FOUND, "unknown geometry id");
+  HIP_OK(hipSetDevice(ctx->device));
+  double *d_p = nullptr, *d_d = nullptr, *d_c = nullptr;
+  int32_t *d_k = nullptr;
+  HIP_OK(dalloc(&d_p, size_t(3 * n)));
+  HIP_OK(dalloc(&d_d, size_t(n)));
+  HIP_OK(dalloc(&d_c, size_t(3 * n)));
+  HIP_OK(dalloc(&d_k, size_t(n)));
+  hipError_t e = hipMemcpyAsync(d_p, points, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = launch_closest(ctx->d_geoms + geom_id, precision, d_p, n, d_d, d_c, d_k, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(distance, d_d, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(closest, d_c, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(cls, d_k, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d_p); (void)hipFree(d_d); (void)hipFree(d_c); (void)hipFree(d_k);
+  if (e != hipSuccess) return fail(TWOS_ERR_DEVICE, std::string("closest: ") + hipGetErrorString(e));
+  return TWOS_OK;
+}
+
+twos_status twos_trace(twos_ctx *ctx, int32_t geom_id, const twos_side *side, const twos_params *params,
+                       const double start[3], int64_t site, int32_t walk, double *rows, int32_t max_rows,
+                       int32_t *n_rows) {
+  if (!ctx || !side || !params || !start || !rows || !n_rows || max_rows < 1)
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "null argument");
+  const twos_params &p = *params;
+  if (p.max_steps < 1 || !(p.eps > 0.0)) return fail(TWOS_ERR_INVALID_ARGUMENT, "invalid numeric option");
+  if (p.rng != kRngPhilox && p.precision != 64) return fail(TWOS_ERR_INVALID_ARGUMENT, "mt streams need binary64");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (ctx->geoms.find(geom_id) == ctx->geoms.end()) return fail(TWOS_ERR_NOT_FOUND, "unknown geometry id");
+  HIP_OK(hipSetDevice(ctx->device));
+  Engine e;
+  e.ctx = ctx;
+  twos_status st = e.allocate(1, 64, 1);
+  if (st != TWOS_OK) return st;
+  Request r;
+  r.geom = geom_id;
+  r.prm = p;
+  r.prm.walks = 2;
+  r.prm.walks_max = 0;
+  r.prm.site_offset = int32_t(site);
+  r.sides[0] = *side;
+  r.sides[1] = *side;
+  r.sites.assign(start, start + 3);
+  r.n = 1;
+  r.slot0 = 0;
+  r.problem = 0;
+  st = e.admit(&r);
+  if (st != TWOS_OK) return st;
+  auto *a_slot = reinterpret_cast<int32_t *>(e.h_in + e.off_slot);
+  auto *a_base = reinterpret_cast<int32_t *>(e.h_in + e.off_base);
+  auto *a_prefix = reinterpret_cast<int64_t *>(e.h_in + e.off_prefix);
+  auto *a_nact = reinterpret_cast<int32_t *>(e.h_in + e.off_nact);
+  auto *a_total = reinterpret_cast<int64_t *>(e.h_in + e.off_total);
+  a_slot[0] = 0;
+  a_base[0] = walk;
+  a_prefix[0] = 0;
+  a_prefix[1] = 1;
+  a_nact[0] = 1;
+  a_total[0] = 1;
+  HIP_OK(hipMemcpy(e.d_in, e.h_in, e.in_bytes, hipMemcpyHostToDevice));
+  double *d_rows = nullptr;
+  HIP_OK(dalloc(&d_rows, 4 * size_t(max_rows)));
+  Tables tb = e.tables(0, 0);
+  WalkParams wp{};
+  wp.combine = kSquare;
+  wp.rng = p.rng;
+  wp.jobs_per_walk = 1;
+  wp.job_begin = 0;
+  wp.job_end = 1;
+  wp.val = e.d_val;
+  wp.steps = e.d_steps;
+  wp.trace = d_rows;
+  wp.trace_rows = max_rows;
+  hipError_t he = p.precision == 64
+                      ? launch_walk_simple_f64(kGeomGeneric, p.rng != kRngPhilox, tb, wp, 64, ctx->stream)
+                      : launch_walk_simple_f32(kGeomGeneric, tb, wp, 64, ctx->stream);
+  int32_t steps = 0;
+  if (he == hipSuccess) he = hipMemcpyAsync(rows, d_rows, sizeof(double) * 4 * max_rows, hipMemcpyDeviceToHost, ctx->stream);
+  if (he == hipSuccess) he = hipMemcpyAsync(&steps, e.d_steps, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream);
+  if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d_rows);
+  if (he != hipSuccess) return fail(TWOS_ERR_DEVICE, std::string("trace: ") + hipGetErrorString(he));
+  *n_rows = std::min((steps & 0x3fffffff) + 1, max_rows);
+  return TWOS_OK;
+}
+
+twos_status twos_server_create(twos_ctx *ctx, int64_t max_sites, int64_t max_walks_per_round, twos_server **out) {
+  if (!ctx || !out || max_sites < 1 || max_walks_per_round < 64)
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "invalid server capacity");
+  const uint64_t need = uint64_t(max_walks_per_round) * 24 + uint64_t(max_sites) * 256;
+  if (need > ctx->budget) return fail(TWOS_ERR_ADMISSION, "server capacity exceeds the context's HBM budget");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIP_OK(hipSetDevice(ctx->device));
+  auto srv = std::make_unique<twos_server>();
+  srv->ctx = ctx;
+  srv->engine = std::make_unique<Engine>();
+  srv->engine->ctx = ctx;
+  const twos_status st = srv->engine->allocate(max_sites, max_walks_per_round, 1024);
+  if (st != TWOS_OK) return st;
+  *out = srv.release();
+  return TWOS_OK;
+}
+
+void twos_server_destroy(twos_server *srv) {
+  if (!srv) return;
+  std::lock_guard<std::mutex> lock(srv->ctx->mu);
+  (void)hipStreamSynchronize(srv->ctx->stream);
+  srv->engine.reset();
+  delete srv;
+}
+
+twos_status twos_server_submit(twos_server *srv, int32_t geom_id, const twos_side sides[2], const twos_params *params,
+                               const double *sites, int64_t n, twos_site_result *out, int64_t *request_id) {
+  if (!srv || !request_id) return fail(TWOS_ERR_INVALID_ARGUMENT, "null argument");
+  twos_status st = check_params(params, n);
+  if (st != TWOS_OK) return st;
+  if (params->schedule != TWOS_PERSISTENT) return fail(TWOS_ERR_INVALID_ARGUMENT, "the server runs the persistent schedule");
+  if (n < 1) return fail(TWOS_ERR_INVALID_ARGUMENT, "a request needs at least one site");
+  std::lock_guard<std::mutex> lock(srv->ctx->mu);
+  Engine *e = srv->engine.get();
+  if (n > e->cap_sites) return fail(TWOS_ERR_ADMISSION, "request has more sites than the server's slots");
+  if (std::max(params->walks, params->walks_max) > e->cap_walks)
+    return fail(TWOS_ERR_ADMISSION, "request walk budget exceeds one round's capacity");
+  Request *r = make_request(srv->ctx, geom_id, sides, params, sites, n, out, &st);
+  if (!r) return st;
+  r->id = e->next_id++;
+  srv->requests[r->id].reset(r);
+  e->queue.push_back(r);
+  *request_id = r->id;
+  return TWOS_OK;
+}
+
+twos_status twos_server_step(twos_server *srv, int64_t *done_ids, int32_t max_done, int32_t *n_done) {
+  if (!srv || !n_done) return fail(TWOS_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> lock(srv->ctx->mu);
+  HIP_OK(hipSetDevice(srv->ctx->device));
+  std::vector<int64_t> done;
+  const twos_status st = srv->engine->step(done);
+  *n_done = 0;
+  for (int64_t id : done) {
+    if (done_ids && *n_done < max_done) done_ids[*n_done] = id;
+    *n_done += 1;
+    srv->requests.erase(id);
+  }
+  return st;
+}
+
+twos_status twos_server_pending(twos_server *srv, int64_t *queued, int64_t *active) {
+  if (!srv) return fail(TWOS_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> lock(srv->ctx->mu);
+  if (queued) *queued = int64_t(srv->engine->queue.size());
+  if (active) *active = int64_t(srv->engine->active.size());
+  return TWOS_OK;
+}
+
+}  // extern "C"

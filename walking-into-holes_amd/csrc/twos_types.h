@@ -1,0 +1,87 @@
+// Internal data layouts shared by the host runtime and the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/twos.h"
+
+namespace twos {
+
+enum { kBall = TWOS_BALL, kBox = TWOS_BOX, kMesh = TWOS_MESH };
+enum { kSquare = TWOS_SQUARE, kPaired = TWOS_PAIRED, kCross = TWOS_CROSS };
+enum { kRngPhilox = TWOS_RNG_PHILOX, kRngMtSite = TWOS_RNG_MT_SITE, kRngMtPaired = TWOS_RNG_MT_PAIRED,
+       kRngMtKey = TWOS_RNG_MT_KEY };
+// Kernel specialisations by geometry: any container (runtime switch), mesh
+// container only (no analytic holes), analytic container only.
+enum { kGeomGeneric = 0, kGeomMesh = 1, kGeomAnalytic = 2 };
+
+using DevSide = twos_side;
+
+// One resident geometry as the kernels see it.  Analytic rows are kept in
+// binary32 (float4) and binary64 (4 doubles, cavities padded) copies.
+struct DevGeom {
+  int32_t container, n_spheres, n_cylinders, n_cavities;
+  int32_t mesh_radius_f32, n_tris;
+  int32_t pad[2];
+  double ball[4];
+  double box_lo[3];
+  double box_hi[3];
+  double cavity_radius;
+  const float4 *spheres_f, *cylinders_f, *cavities_f;
+  const double *spheres_d, *cylinders_d, *cavities_d;
+  const float4 *nodes;
+  const float *tris_f;
+  const double *tris_d;
+  const int32_t *tri_ids;
+};
+
+// Per-site-slot state resident on the device (capacity = server max_sites).
+struct SlotArrays {
+  double *pos;        // [C][3]
+  int32_t *geom;      // index into the geometry table
+  int32_t *problem;   // index into the side table: sides 2*problem, 2*problem+1
+  uint2 *keys;        // [C][2] Philox keys of sides 0 and 1
+  int64_t *site;      // site index used by the mt19937_64 key layouts
+  uint64_t *seed;     // request seed (mt19937_64 layouts)
+  int32_t *repeat;    // repeat index (MT_SITE)
+  int32_t *walks_max;
+  int32_t *max_steps;
+  double *eps;
+  double *ci_rel;
+  double *ci_z;
+  double *acc;        // [C][8] binary64 power sums
+  int64_t *acc_walks;
+  int64_t *acc_steps;
+  int64_t *acc_caps;
+  twos_site_result *result;  // [C]
+};
+
+// One round's work list, filled by the host before the walk launch.
+struct RoundArrays {
+  int32_t *act_slot;     // [n_active]
+  int64_t *act_prefix;   // [n_active + 1], walk units
+  int32_t *walk_base;    // [n_active]: first walk index of this round
+  int32_t *n_active;     // device scalar
+  int64_t *total_jobs;   // device scalar
+  unsigned long long *counter;  // persistent ticket counter
+  twos_site_result *compact;    // [n_active] results gathered for the host
+  int32_t *decision;            // [n_active] 0 done, 1 more walks, 2 budget exhausted
+};
+
+struct WalkParams {
+  int32_t combine;
+  int32_t rng;
+  int32_t jobs_per_walk;  // 2 for CROSS, 1 otherwise
+  int32_t pad0;
+  int64_t job_begin;      // simple kernel: first job of this launch
+  int64_t job_end;
+  double *val;            // [total walks][2] binary64 samples (a, b)
+  int32_t *steps;         // [total walks][2] step counts, bit 30 = capped
+  double *trace;          // optional rows (x, y, z, r) for the trace entry point
+  int32_t trace_rows;
+  int32_t pad;
+  unsigned long long *counters;  // nodes, tris (when counting)
+};
+
+}  // namespace twos

@@ -1,0 +1,346 @@
+// Walk on Spheres kernels (templates; instantiated per precision in
+// twos_walk_f32.hip and twos_walk_f64.hip).
+//
+// Work model.  A round hands the kernels a list of active site slots with a
+// walk count each (RoundArrays).  Walk t of the round belongs to slot
+// act_slot[k] where act_prefix[k] <= t < act_prefix[k+1]; its global walk
+// index is walk_base[k] + (t - act_prefix[k]).  A job is one path: walk t
+// side s (CROSS runs two independent paths per walk, SQUARE and PAIRED one).
+// Every job writes its binary64 sample(s) to val[2t + 0/1] and its step count
+// to steps[2t + s]; the reduction then folds each site's samples in walk
+// order, so a result depends only on (seed, site, side, walk), never on which
+// lane, wave or launch ran the path.
+#pragma once
+
+#include "twos_device.h"
+
+namespace twos {
+
+struct Tables {
+  const DevGeom *geoms;
+  const DevSide *sides;
+  SlotArrays slots;
+  RoundArrays round;
+};
+
+__device__ __forceinline__ int32_t find_site(const int64_t *prefix, int32_t n, int64_t t) {
+  // Largest k with prefix[k] <= t (prefix[0] = 0, prefix[n] = total).
+  int32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (prefix[mid] <= t) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+template <typename R>
+struct Job {
+  V3<R> x;
+  R acc0, acc1;
+  int32_t steps;
+  int32_t side;
+  int32_t walk;
+  int32_t slot;
+  int32_t max_steps;
+  R eps;
+  int64_t t;
+  uint2 key;
+  const DevGeom *g;
+  const DevSide *s0;
+  const DevSide *s1;
+};
+
+template <typename R>
+__device__ __forceinline__ void job_init(Job<R> &j, const Tables &tb, const WalkParams &wp, int64_t u) {
+  const int32_t n_active = *tb.round.n_active;
+  const int64_t t = u / wp.jobs_per_walk;
+  j.side = int32_t(u - t * wp.jobs_per_walk);
+  const int32_t k = find_site(tb.round.act_prefix, n_active, t);
+  const int32_t slot = tb.round.act_slot[k];
+  j.slot = slot;
+  j.t = t;
+  j.walk = tb.round.walk_base[k] + int32_t(t - tb.round.act_prefix[k]);
+  const double *p = tb.slots.pos + 3 * size_t(slot);
+  j.x = {R(p[0]), R(p[1]), R(p[2])};
+  j.acc0 = R(0);
+  j.acc1 = R(0);
+  j.steps = 0;
+  j.max_steps = tb.slots.max_steps[slot];
+  j.eps = R(tb.slots.eps[slot]);
+  j.key = tb.slots.keys[2 * size_t(slot) + (wp.combine == kCross ? j.side : 0)];
+  j.g = tb.geoms + tb.slots.geom[slot];
+  const int32_t prob = tb.slots.problem[slot];
+  j.s0 = tb.sides + 2 * prob + (wp.combine == kCross ? j.side : 0);
+  j.s1 = tb.sides + 2 * prob + 1;
+}
+
+template <typename R>
+__device__ __forceinline__ void job_store(const Job<R> &j, const WalkParams &wp, bool capped) {
+  const int32_t st = j.steps | (capped ? (1 << 30) : 0);
+  if (wp.combine == kCross) {
+    wp.val[2 * j.t + j.side] = double(j.acc0);
+    wp.steps[2 * j.t + j.side] = st;
+  } else {
+    wp.val[2 * j.t + 0] = double(j.acc0);
+    wp.val[2 * j.t + 1] = double(wp.combine == kPaired ? j.acc1 : j.acc0);
+    wp.steps[2 * j.t + 0] = st;
+    wp.steps[2 * j.t + 1] = 0;
+  }
+}
+
+// One step of a path.  Returns 0 while the path continues, 1 when it ended in
+// the eps-shell (payoff added), 2 when it was capped at max_steps.  The cap
+// check precedes the query, so a capped path makes max_steps queries and a
+// terminated one steps + 1, as walk() does (wih_twos.cpp:644-663).
+template <typename R, int GK, bool kCount>
+__device__ __forceinline__ int job_step(Job<R> &j, const WalkParams &wp, TraversalCount *cnt, R *radius_out) {
+  if (j.steps >= j.max_steps) return 2;
+  V3<R> bp;
+  int cls;
+  const R r = closest_boundary<R, GK, kCount>(*j.g, j.x, bp, cls, cnt);
+  if (radius_out) *radius_out = r;
+  const DevSide &s0 = *j.s0;
+  if (!(r > j.eps)) {
+    j.acc0 += side_payoff<R>(s0, bp, cls);
+    if (wp.combine == kPaired) j.acc1 += side_payoff<R>(*j.s1, bp, cls);
+    return 1;
+  }
+  if (s0.source_kind == 1) j.acc0 += side_source<R>(s0, j.x) * r * r / R(6);
+  if (wp.combine == kPaired && j.s1->source_kind == 1) j.acc1 += side_source<R>(*j.s1, j.x) * r * r / R(6);
+  return 0;
+}
+
+template <typename R>
+__device__ __forceinline__ V3<R> philox_dir(const Job<R> &j) {
+  const uint4 w = philox4x32_10(make_uint4(uint32_t(j.steps), uint32_t(j.walk), 0u, 0u), j.key);
+  return philox_direction(w, R(0));
+}
+
+// ---------------------------------------------------------------------------
+// Simple kernel: one thread per job, the whole path in one go.  Used by the
+// plain schedule (one launch per site and round), by the mt19937_64 replay
+// streams and by the trace entry point.
+template <typename R, int GK, bool kMt>
+__global__ __launch_bounds__(256) void walk_simple(Tables tb, WalkParams wp) {
+  const int64_t u = wp.job_begin + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (u >= wp.job_end) return;
+  Job<R> j;
+  job_init(j, tb, wp, u);
+  Mt64 mt;
+  if constexpr (kMt) {
+    const int32_t slot = j.slot;
+    mt.seed(mt_walk_seed(wp.rng, tb.slots.seed[slot], tb.slots.repeat[slot], tb.slots.site[slot], j.walk));
+  }
+  int row = 0;
+  while (true) {
+    R r = R(0);
+    const int state = job_step<R, GK, false>(j, wp, nullptr, &r);
+    if (wp.trace && row < wp.trace_rows && state != 2) {
+      double *o = wp.trace + 4 * row++;
+      o[0] = double(j.x.x); o[1] = double(j.x.y); o[2] = double(j.x.z); o[3] = double(r);
+    }
+    if (state == 1) { job_store(j, wp, false); break; }
+    if (state == 2) {
+      if (wp.trace && row < wp.trace_rows) {
+        V3<R> bp;
+        int cls;
+        const R rr = closest_boundary<R, GK, false>(*j.g, j.x, bp, cls, nullptr);
+        double *o = wp.trace + 4 * row++;
+        o[0] = double(j.x.x); o[1] = double(j.x.y); o[2] = double(j.x.z); o[3] = double(rr);
+      }
+      job_store(j, wp, true);
+      break;
+    }
+    V3<R> d;
+    if constexpr (kMt) {
+      const V3<double> dd = mt.direction();
+      d = {R(dd.x), R(dd.y), R(dd.z)};
+    } else {
+      d = philox_dir(j);
+    }
+    j.x = j.x + d * r;
+    ++j.steps;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent wavefront kernel with walk regeneration (Philox streams).
+//
+// Each wave keeps a chunk of kChunk consecutive job tickets taken from the
+// round's global counter with a single atomic.  Whenever lanes finish a path
+// they take the next tickets of the chunk in lane order; when the chunk runs
+// dry one lane fetches the next.  So lanes stay busy until the round's last
+// tickets are handed out, and consecutive lanes start at the same site.  The
+// loop ends for a wave when the counter is exhausted and none of its lanes
+// holds a path: every wave reaches that state because every path ends within
+// max_steps steps.
+constexpr int kChunk = 64;
+
+template <typename R, int GK, bool kCount>
+__global__ __launch_bounds__(256) void walk_persistent(Tables tb, WalkParams wp) {
+  const int lane = int(threadIdx.x & 63);
+  const int64_t total = *tb.round.total_jobs;
+  Job<R> j;
+  bool active = false;
+  int64_t chunk_cur = 0, chunk_end = 0;  // wave-uniform
+  bool drained = false;                  // wave-uniform
+  TraversalCount cnt{0, 0};
+  while (true) {
+    uint64_t need = __ballot(!active);
+    while (need != 0 && !drained) {
+      if (chunk_cur >= chunk_end) {
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(tb.round.counter, (unsigned long long)kChunk);
+        base = __shfl(base, 0);
+        if (int64_t(base) >= total) {
+          drained = true;
+          break;
+        }
+        chunk_cur = int64_t(base);
+        chunk_end = min(int64_t(base) + kChunk, total);
+      }
+      const int64_t avail = chunk_end - chunk_cur;
+      const int cnt_need = __popcll(need);
+      const int take = int(min<int64_t>(avail, cnt_need));
+      const int rank = __popcll(need & ((1ull << lane) - 1ull));
+      if (!active && rank < take) {
+        job_init(j, tb, wp, chunk_cur + rank);
+        active = true;
+      }
+      chunk_cur += take;
+      need = __ballot(!active);
+    }
+    if (!__any(active)) break;
+    if (active) {
+      R r;
+      const int state = job_step<R, GK, kCount>(j, wp, &cnt, &r);
+      if (state == 0) {
+        const V3<R> d = philox_dir(j);
+        j.x = j.x + d * r;
+        ++j.steps;
+      } else {
+        job_store(j, wp, state == 2);
+        active = false;
+      }
+    }
+  }
+  if (kCount) {
+    atomicAdd(wp.counters + 0, (unsigned long long)cnt.nodes);
+    atomicAdd(wp.counters + 1, (unsigned long long)cnt.tris);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per-site reduction and combine (binary64).  One block per active site folds
+// the round's samples into eight power sums {Sa, Sb, Sab, Saa, Sbb, Saab,
+// Sabb, Saabb} -- each thread a fixed stride of walk indices, then a wave
+// shuffle and an LDS tree -- adds them to the site's resident accumulators
+// and recomputes the estimate:
+//   score = (Sa Sb - Sab) / (N (N-1))                (wih_twos.cpp:693;
+//                                                      study.cpp:117)
+//   jackknife SE over leave-one-out estimates        (wih_twos.cpp:698-712)
+// from the power sums: the leave-one-out estimate (C - Sa b_i - Sb a_i +
+// 2 a_i b_i) / ((N-1)(N-2)) is linear in (a_i, b_i, a_i b_i).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+__device__ __forceinline__ long long wave_sum_ll(long long v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+__device__ __forceinline__ void site_estimate(const double *s, int64_t count, twos_site_result &r) {
+  const double n = double(count);
+  r.u = s[0] / n;
+  r.v = s[1] / n;
+  r.score = (s[0] * s[1] - s[2]) / (n * (n - 1.0));
+  r.td = -r.score;
+  r.same = s[2] / n;
+  if (count < 4) {
+    r.score_se = __builtin_nan("");
+    return;
+  }
+  const double sa = s[0], sb = s[1];
+  const double z2 = sa * sa * s[4] + sb * sb * s[3] + 4.0 * s[7] + 2.0 * sa * sb * s[2] - 4.0 * sa * s[6] -
+                    4.0 * sb * s[5];
+  const double z1 = 2.0 * (s[2] - sa * sb);
+  const double d = (n - 1.0) * (n - 2.0);
+  const double spread = fmax(0.0, z2 - z1 * z1 / n) / (d * d);
+  r.score_se = sqrt((n - 1.0) / n * spread);
+}
+
+template <int kBlock>
+__global__ __launch_bounds__(kBlock) void reduce_sites(Tables tb, WalkParams wp) {
+  const int32_t k = blockIdx.x;
+  if (k >= *tb.round.n_active) return;
+  const int32_t slot = tb.round.act_slot[k];
+  const int64_t t0 = tb.round.act_prefix[k], t1 = tb.round.act_prefix[k + 1];
+  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  long long steps = 0, caps = 0;
+  for (int64_t t = t0 + threadIdx.x; t < t1; t += kBlock) {
+    const double a = wp.val[2 * t], b = wp.val[2 * t + 1];
+    const double ab = a * b;
+    s[0] += a; s[1] += b; s[2] += ab; s[3] += a * a; s[4] += b * b;
+    s[5] += ab * a; s[6] += ab * b; s[7] += ab * ab;
+    const int32_t st0 = wp.steps[2 * t], st1 = wp.steps[2 * t + 1];
+    steps += (st0 & 0x3fffffff) + (st1 & 0x3fffffff);
+    caps += ((st0 >> 30) & 1) + ((st1 >> 30) & 1);
+  }
+  __shared__ double red[kBlock / 64][8];
+  __shared__ long long redi[kBlock / 64][2];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) s[i] = wave_sum(s[i]);
+  steps = wave_sum_ll(steps);
+  caps = wave_sum_ll(caps);
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) red[wave][i] = s[i];
+    redi[wave][0] = steps;
+    redi[wave][1] = caps;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot[8];
+    long long tsteps = 0, tcaps = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) tot[i] = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) tot[i] += red[w][i];
+      tsteps += redi[w][0];
+      tcaps += redi[w][1];
+    }
+    double *acc = tb.slots.acc + 8 * size_t(slot);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) acc[i] += tot[i];
+    const int64_t walks = tb.slots.acc_walks[slot] + (t1 - t0);
+    tb.slots.acc_walks[slot] = walks;
+    tb.slots.acc_steps[slot] += tsteps;
+    tb.slots.acc_caps[slot] += tcaps;
+    twos_site_result r;
+    site_estimate(acc, walks, r);
+    r.walks = walks;
+    r.steps = tb.slots.acc_steps[slot];
+    r.caps = tb.slots.acc_caps[slot];
+    const double rel = tb.slots.ci_rel[slot];
+    int32_t decision;
+    if (!(rel >= 0.0)) {
+      decision = 0;  // fixed budget
+    } else {
+      const bool met = isfinite(r.score_se) && tb.slots.ci_z[slot] * r.score_se <= rel * fabs(r.score);
+      decision = met ? 0 : (walks >= tb.slots.walks_max[slot] ? 2 : 1);
+    }
+    r.status = decision == 2 ? 1 : 0;
+    r.pad = 0;
+    tb.slots.result[slot] = r;
+    tb.round.compact[k] = r;
+    tb.round.decision[k] = decision;
+  }
+}
+
+}  // namespace twos

@@ -1,0 +1,74 @@
+// binary64 walks (built with -ffp-contract=off so each operation rounds once,
+// as the reference build's do), the mt19937_64 replay streams, and the
+// binary64 per-site reduction.
+#include "twos_launch.h"
+#include "twos_walk.h"
+
+namespace twos {
+
+namespace {
+inline unsigned blocks_for(int64_t n, int b) { return unsigned((n + b - 1) / b); }
+constexpr int kReduceBlock = 256;
+}  // namespace
+
+hipError_t launch_walk_simple_f64(int geom_kind, bool mt, const Tables &tb, const WalkParams &wp, int block,
+                                  hipStream_t stream) {
+  const int64_t n = wp.job_end - wp.job_begin;
+  if (n <= 0) return hipSuccess;
+  const dim3 grid(blocks_for(n, block)), blk(block);
+  if (mt) {
+    hipLaunchKernelGGL((walk_simple<double, kGeomGeneric, true>), grid, blk, 0, stream, tb, wp);
+    return hipGetLastError();
+  }
+  switch (geom_kind) {
+    case kGeomMesh:
+      hipLaunchKernelGGL((walk_simple<double, kGeomMesh, false>), grid, blk, 0, stream, tb, wp);
+      break;
+    case kGeomAnalytic:
+      hipLaunchKernelGGL((walk_simple<double, kGeomAnalytic, false>), grid, blk, 0, stream, tb, wp);
+      break;
+    default:
+      hipLaunchKernelGGL((walk_simple<double, kGeomGeneric, false>), grid, blk, 0, stream, tb, wp);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_walk_persistent_f64(int geom_kind, bool count, const Tables &tb, const WalkParams &wp, int grid,
+                                      int block, hipStream_t stream) {
+  const dim3 g(grid), b(block);
+  if (count) {
+    hipLaunchKernelGGL((walk_persistent<double, kGeomGeneric, true>), g, b, 0, stream, tb, wp);
+    return hipGetLastError();
+  }
+  switch (geom_kind) {
+    case kGeomMesh:
+      hipLaunchKernelGGL((walk_persistent<double, kGeomMesh, false>), g, b, 0, stream, tb, wp);
+      break;
+    case kGeomAnalytic:
+      hipLaunchKernelGGL((walk_persistent<double, kGeomAnalytic, false>), g, b, 0, stream, tb, wp);
+      break;
+    default:
+      hipLaunchKernelGGL((walk_persistent<double, kGeomGeneric, false>), g, b, 0, stream, tb, wp);
+  }
+  return hipGetLastError();
+}
+
+namespace {
+__global__ void zero_u64(unsigned long long *p, int n) {
+  if (int(threadIdx.x) < n) p[threadIdx.x] = 0ull;
+}
+}  // namespace
+
+// Clears ticket counters with a kernel (graph-safe: no memset nodes).
+hipError_t launch_zero(unsigned long long *p, int n, hipStream_t stream) {
+  hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, stream, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce(const Tables &tb, const WalkParams &wp, int grid, hipStream_t stream) {
+  if (grid <= 0) return hipSuccess;
+  hipLaunchKernelGGL((reduce_sites<kReduceBlock>), dim3(grid), dim3(kReduceBlock), 0, stream, tb, wp);
+  return hipGetLastError();
+}
+
+}  // namespace twos

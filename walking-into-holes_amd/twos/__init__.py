@@ -1,0 +1,404 @@
+"""MI355X Topological Walk on Spheres (TWoS) estimator.
+
+Python host over the gfx950 HIP library (include/twos.h via ctypes).  The
+estimator call keeps the reference backend's meaning: geometry, candidate
+sites, a walk budget and a seed in; per site the state mean u, the
+cross-index score (Sa Sb - Sab)/(N(N-1)), its jackknife standard error and the
+step counts out (benchmarks/compiled_bunny/src/wih_twos.cpp:674-717,
+benchmarks/submission_study/study.cpp:80-120).  The topological derivative is
+td = -score.
+
+    ctx = twos.Context()
+    mesh = ctx.mesh(vertices, faces)
+    geom = ctx.geometry(twos.MESH, mesh=mesh)
+    res = ctx.estimate(geom, sites, [primal, adjoint], combine=twos.CROSS,
+                       walks=256, walks_max=16384, ci_rel=0.02)
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _capi
+from ._capi import (BALL, BOX, CROSS, MESH, PAIRED, PERSISTENT, PLAIN, RNG_MT_KEY, RNG_MT_PAIRED,
+                    RNG_MT_SITE, RNG_PHILOX, SQUARE, FLAG_COUNT_TRAVERSAL)
+
+__all__ = [
+    "Context", "Mesh", "Side", "Result", "Server", "TwosError", "InvalidArgument", "StepCapError",
+    "GeometryError", "AdmissionError", "DeviceError", "NotFoundError", "side", "reference_affine_side",
+    "BALL", "BOX", "MESH", "SQUARE", "PAIRED", "CROSS", "RNG_PHILOX", "RNG_MT_SITE", "RNG_MT_PAIRED",
+    "RNG_MT_KEY", "PLAIN", "PERSISTENT", "FLAG_COUNT_TRAVERSAL", "version",
+]
+
+
+class TwosError(RuntimeError):
+    code = -1
+
+    def __init__(self, message: str, code: int | None = None, result=None):
+        super().__init__(message)
+        if code is not None:
+            self.code = code
+        self.result = result
+
+
+class InvalidArgument(TwosError, ValueError):
+    code = _capi.ERR_INVALID_ARGUMENT
+
+
+class StepCapError(TwosError):
+    """One or more paths hit max_steps (the reference's fatal
+    "one or more paths hit --max-steps", wih_twos.cpp:1846-1848).  The
+    estimate is attached as `.result`."""
+    code = _capi.ERR_STEP_CAP
+
+
+class GeometryError(TwosError):
+    code = _capi.ERR_GEOMETRY
+
+
+class AdmissionError(TwosError):
+    code = _capi.ERR_ADMISSION
+
+
+class DeviceError(TwosError):
+    code = _capi.ERR_DEVICE
+
+
+class NotFoundError(TwosError, KeyError):
+    code = _capi.ERR_NOT_FOUND
+
+
+_ERRORS = {c.code: c for c in (InvalidArgument, StepCapError, GeometryError, AdmissionError, DeviceError,
+                               NotFoundError)}
+
+
+def _lib():
+    return _capi.load()
+
+
+def _check(status: int, result=None) -> None:
+    if status != _capi.OK:
+        msg = _lib().twos_last_error().decode(errors="replace")
+        raise _ERRORS.get(status, TwosError)(msg, status, result)
+
+
+def version() -> str:
+    return _lib().twos_version().decode()
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a: np.ndarray, t):
+    return a.ctypes.data_as(t)
+
+
+Side = _capi.Side
+
+
+def side(poly=None, cls_const=None, poly_classes=(), source=None) -> Side:
+    """Boundary data and source of one side.  The Dirichlet value at the closest
+    boundary point y of class c is cls_const[c] + [c in poly_classes] * poly(y),
+    poly = (c0, cx, cy, cz, cxx, cyy, czz, cxy, cxz, cyz).  Classes: 0-5 container
+    faces (ball and mesh use 0; box faces -x,+x,-y,+y,-z,+z), 6 holes, 7 cavities.
+    source = (s0, sx, sy, sz, ox, oy, oz) adds (r^2/6) f(x) per step with
+    f = ((s0 + sx (x+ox)) + sy (y+oy)) + sz (z+oz)."""
+    s = Side()
+    poly = list(poly) if poly is not None else [0.0] * 10
+    if len(poly) != 10:
+        raise InvalidArgument("poly needs 10 coefficients")
+    s.poly = (C.c_double * 10)(*poly)
+    cc = list(cls_const) if cls_const is not None else [0.0] * 8
+    if len(cc) != 8:
+        raise InvalidArgument("cls_const needs 8 entries")
+    s.cls_const = (C.c_double * 8)(*cc)
+    mask = 0
+    for c in poly_classes:
+        if not 0 <= c < 8:
+            raise InvalidArgument("boundary class out of range")
+        mask |= 1 << c
+    s.cls_poly_mask = mask
+    if source is not None:
+        if len(source) != 7:
+            raise InvalidArgument("source needs (s0, sx, sy, sz, ox, oy, oz)")
+        s.source_kind = 1
+        s.src = (C.c_double * 7)(*source)
+    return s
+
+
+def reference_affine_side(slope: float = 0.65) -> Side:
+    """The reference's compliance problem: zero Dirichlet data, source
+    f = 1 + slope * (y + 0.5) (wih_twos.cpp:546-552, --source-y-slope)."""
+    return side(source=(1.0, 0.0, slope, 0.0, 0.0, 0.5, 0.0))
+
+
+@dataclass
+class Mesh:
+    id: int
+    n_vertices: int
+    n_faces: int
+    build_ms: float
+    depth: int
+    bvh_bytes: int
+
+
+@dataclass
+class Result:
+    u: np.ndarray
+    v: np.ndarray
+    score: np.ndarray
+    score_se: np.ndarray
+    td: np.ndarray
+    same: np.ndarray
+    walks: np.ndarray
+    steps: np.ndarray
+    caps: np.ndarray
+    status: np.ndarray
+    stats: dict = field(default_factory=dict)
+    a: np.ndarray | None = None
+    b: np.ndarray | None = None
+    walk_steps: np.ndarray | None = None
+
+    @property
+    def limit_hits(self) -> np.ndarray:
+        return self.caps
+
+
+def _result(out, n, stats: dict) -> Result:
+    dt = np.dtype([(f[0], np.float64 if f[1] is C.c_double else (np.int64 if f[1] is C.c_int64 else np.int32))
+                   for f in _capi.SiteResult._fields_])
+    assert dt.itemsize == C.sizeof(_capi.SiteResult)
+    rec = np.frombuffer(out, dtype=dt, count=n).copy() if n else np.zeros(0, dtype=dt)
+    return Result(u=rec["u"].copy(), v=rec["v"].copy(), score=rec["score"].copy(),
+                  score_se=rec["score_se"].copy(), td=rec["td"].copy(), same=rec["same"].copy(),
+                  walks=rec["walks"].copy(), steps=rec["steps"].copy(), caps=rec["caps"].copy(),
+                  status=rec["status"].copy(), stats=stats)
+
+
+def _stats(st: _capi.Stats) -> dict:
+    return {f[0]: getattr(st, f[0]) for f in _capi.Stats._fields_}
+
+
+def _params(combine=SQUARE, rng=RNG_PHILOX, precision=32, schedule=PERSISTENT, seed=0, repeat=0,
+            max_steps=512, eps=5e-4, walks=16, walks_max=0, ci_rel=0.02, ci_z=1.96, use_graph=False,
+            block=0, waves_per_simd=0, site_offset=0, flags=0) -> _capi.Params:
+    return _capi.Params(combine=combine, rng=rng, precision=precision, schedule=schedule, seed=seed,
+                        repeat=repeat, max_steps=max_steps, eps=eps, walks=walks, walks_max=walks_max,
+                        ci_rel=ci_rel, ci_z=ci_z, use_graph=int(use_graph), block=block,
+                        waves_per_simd=waves_per_simd, site_offset=site_offset, flags=flags)
+
+
+class Context:
+    """One MI355X (HIP device), its stream, resident meshes/geometries and the
+    round engine."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        h = C.c_void_p()
+        _check(_lib().twos_create(device, hbm_budget, C.byref(h)))
+        self._h = h
+        self._geoms: dict[int, dict] = {}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib().twos_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- geometry -----------------------------------------------------------
+    def mesh(self, vertices, faces) -> Mesh:
+        """Upload a triangle mesh and build its LBVH on the GPU (the
+        EmbreeMesh constructor's role, wih_twos.cpp:298-325)."""
+        v = _f64(vertices).reshape(-1, 3)
+        f = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 3)
+        mid = C.c_int32()
+        ms = C.c_double()
+        _check(_lib().twos_mesh_create(self._h, _ptr(v, _capi._dp), len(v), _ptr(f, _capi._u32p), len(f),
+                                       C.byref(mid), C.byref(ms)))
+        depth, nodes, nbytes = C.c_int32(), C.c_int64(), C.c_int64()
+        _check(_lib().twos_mesh_info(self._h, mid.value, C.byref(depth), C.byref(nodes), C.byref(nbytes)))
+        return Mesh(mid.value, len(v), len(f), ms.value, depth.value, nbytes.value)
+
+    def geometry(self, container: int, *, mesh: Mesh | int | None = None, ball=(0.0, 0.0, 0.0, 1.0),
+                 box_lo=(0.0, 0.0, 0.0), box_hi=(1.0, 1.0, 1.0), spheres=None, cylinders=None, cavities=None,
+                 cavity_radius: float = 0.0, mesh_radius_f32: bool = False) -> int:
+        """Domain = container (ball, box or mesh interior) minus sphere and
+        axis-aligned cylinder holes minus cavities of one radius (the
+        reference's --cavities/--cavity-radius, wih_twos.cpp:585-626)."""
+        d = _capi.GeometryDesc()
+        d.container = container
+        d.mesh = (mesh.id if isinstance(mesh, Mesh) else int(mesh)) if mesh is not None else -1
+        d.mesh_radius_f32 = int(mesh_radius_f32)
+        d.ball = (C.c_double * 4)(*ball)
+        d.box_lo = (C.c_double * 3)(*box_lo)
+        d.box_hi = (C.c_double * 3)(*box_hi)
+        d.cavity_radius = cavity_radius
+        keep = []
+        for name, cnt, width in (("spheres", "n_spheres", 4), ("cylinders", "n_cylinders", 4),
+                                 ("cavities", "n_cavities", 3)):
+            arr = {"spheres": spheres, "cylinders": cylinders, "cavities": cavities}[name]
+            if arr is not None and len(arr):
+                a = _f64(arr).reshape(-1, width)
+                keep.append(a)
+                setattr(d, name, _ptr(a, _capi._dp))
+                setattr(d, cnt, len(a))
+        gid = C.c_int32()
+        _check(_lib().twos_geometry_create(self._h, C.byref(d), C.byref(gid)))
+        return gid.value
+
+    def union_sphere(self, geom: int, center, radius: float) -> int:
+        """Derived geometry with one more spherical cavity (fresh-seed
+        insertion validation)."""
+        c = _f64(center)
+        gid = C.c_int32()
+        _check(_lib().twos_geometry_union_sphere(self._h, geom, _ptr(c, _capi._dp), radius, C.byref(gid)))
+        return gid.value
+
+    def release_geometry(self, geom: int) -> None:
+        _check(_lib().twos_geometry_release(self._h, geom))
+
+    def release_mesh(self, mesh: Mesh | int) -> None:
+        _check(_lib().twos_mesh_release(self._h, mesh.id if isinstance(mesh, Mesh) else int(mesh)))
+
+    # -- queries ------------------------------------------------------------
+    def inside(self, mesh: Mesh | int, points) -> np.ndarray:
+        """Ray-parity membership (EmbreeMesh::inside, wih_twos.cpp:361-394)."""
+        p = _f64(points).reshape(-1, 3)
+        out = np.zeros(len(p), dtype=np.int32)
+        mid = mesh.id if isinstance(mesh, Mesh) else int(mesh)
+        _check(_lib().twos_inside(self._h, mid, _ptr(p, _capi._dp), len(p), _ptr(out, _capi._i32p)))
+        return out.astype(bool)
+
+    def closest(self, geom: int, points, precision: int = 64):
+        """Distance to the domain boundary, closest point and boundary class."""
+        p = _f64(points).reshape(-1, 3)
+        n = len(p)
+        d = np.zeros(n)
+        q = np.zeros((n, 3))
+        c = np.zeros(n, dtype=np.int32)
+        _check(_lib().twos_closest(self._h, geom, precision, _ptr(p, _capi._dp), n, _ptr(d, _capi._dp),
+                                   _ptr(q, _capi._dp), _ptr(c, _capi._i32p)))
+        return d, q, c
+
+    def trace(self, geom: int, start, side_: Side, *, rng=RNG_MT_KEY, seed=0, site=0, walk=0, repeat=0,
+              eps=5e-4, max_steps=512, precision=64) -> np.ndarray:
+        """Rows (x, y, z, radius) of one path (write_trace,
+        wih_twos.cpp:1725-1744).  With RNG_MT_KEY the stream is
+        mt19937_64(splitmix64(seed + site))."""
+        s = _f64(start)
+        rows = np.zeros((max_steps + 2, 4))
+        n = C.c_int32()
+        prm = _params(rng=rng, precision=precision, seed=seed, repeat=repeat, eps=eps, max_steps=max_steps,
+                      walks=2, schedule=PLAIN)
+        _check(_lib().twos_trace(self._h, geom, C.byref(side_), C.byref(prm), _ptr(s, _capi._dp), site, walk,
+                                 _ptr(rows, _capi._dp), len(rows), C.byref(n)))
+        return rows[:n.value].copy()
+
+    # -- the estimator ------------------------------------------------------
+    def estimate(self, geom: int, sites, sides, *, combine=SQUARE, rng=RNG_PHILOX, precision=32,
+                 schedule=PERSISTENT, seed=0, repeat=0, walks=16, walks_max=0, ci_rel=0.02, ci_z=1.96,
+                 eps=5e-4, max_steps=512, use_graph=False, block=0, waves_per_simd=0, site_offset=0,
+                 count_traversal=False, dump: bool = False, raise_on_cap: bool = True) -> Result:
+        """Per-site TWoS estimate.  walks_max > walks doubles each site's walks
+        until ci_z * score_se <= ci_rel * |score| or walks_max is reached."""
+        p = _f64(sites).reshape(-1, 3)
+        n = len(p)
+        sd = (Side * 2)(sides[0], sides[1] if len(sides) > 1 else sides[0])
+        prm = _params(combine=combine, rng=rng, precision=precision, schedule=schedule, seed=seed,
+                      repeat=repeat, max_steps=max_steps, eps=eps, walks=walks, walks_max=walks_max,
+                      ci_rel=ci_rel, ci_z=ci_z, use_graph=use_graph, block=block,
+                      waves_per_simd=waves_per_simd, site_offset=site_offset,
+                      flags=FLAG_COUNT_TRAVERSAL if count_traversal else 0)
+        out = (_capi.SiteResult * max(n, 1))()
+        st = _capi.Stats()
+        if dump:
+            stride = max(walks, walks_max)
+            da = np.zeros((n, stride))
+            db = np.zeros((n, stride))
+            ds = np.zeros((n, stride), dtype=np.int32)
+            status = _lib().twos_estimate_dump(self._h, geom, sd, C.byref(prm), _ptr(p, _capi._dp), n, out,
+                                               C.byref(st), _ptr(da, _capi._dp), _ptr(db, _capi._dp),
+                                               _ptr(ds, _capi._i32p), stride)
+        else:
+            status = _lib().twos_estimate(self._h, geom, sd, C.byref(prm), _ptr(p, _capi._dp), n, out,
+                                          C.byref(st))
+        res = _result(out, n, _stats(st))
+        if dump:
+            res.a, res.b, res.walk_steps = da, db, ds
+        if status == _capi.ERR_STEP_CAP and not raise_on_cap:
+            return res
+        _check(status, res)
+        return res
+
+    def server(self, max_sites: int, max_walks_per_round: int) -> "Server":
+        return Server(self, max_sites, max_walks_per_round)
+
+
+class Server:
+    """Continuous-batching request loop over one context (twos_server_*).
+    Requests are admitted between rounds as slots free up; all admitted sites
+    share each persistent launch."""
+
+    def __init__(self, ctx: Context, max_sites: int, max_walks_per_round: int):
+        h = C.c_void_p()
+        _check(_lib().twos_server_create(ctx._h, max_sites, max_walks_per_round, C.byref(h)))
+        self._h = h
+        self.ctx = ctx
+        self._pending: dict[int, tuple] = {}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib().twos_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, geom: int, sites, sides, *, combine=CROSS, precision=32, seed=0, walks=256,
+               walks_max=0, ci_rel=0.02, ci_z=1.96, eps=1e-4, max_steps=4096, use_graph=False,
+               waves_per_simd=0, site_offset=0) -> int:
+        p = _f64(sites).reshape(-1, 3)
+        n = len(p)
+        sd = (Side * 2)(sides[0], sides[1] if len(sides) > 1 else sides[0])
+        prm = _params(combine=combine, precision=precision, schedule=PERSISTENT, seed=seed, walks=walks,
+                      walks_max=walks_max, ci_rel=ci_rel, ci_z=ci_z, eps=eps, max_steps=max_steps,
+                      use_graph=use_graph, waves_per_simd=waves_per_simd, site_offset=site_offset)
+        out = (_capi.SiteResult * n)()
+        rid = C.c_int64()
+        _check(_lib().twos_server_submit(self._h, geom, sd, C.byref(prm), _ptr(p, _capi._dp), n, out,
+                                         C.byref(rid)))
+        self._pending[rid.value] = (out, n, p, sd)
+        return rid.value
+
+    def step(self) -> dict[int, Result]:
+        """Run one round; returns the requests that completed in it."""
+        ids = (C.c_int64 * 4096)()
+        nd = C.c_int32()
+        _check(_lib().twos_server_step(self._h, ids, 4096, C.byref(nd)))
+        done = {}
+        for k in range(min(nd.value, 4096)):
+            out, n, _, _ = self._pending.pop(ids[k])
+            done[ids[k]] = _result(out, n, {})
+        return done
+
+    def pending(self) -> tuple[int, int]:
+        q, a = C.c_int64(), C.c_int64()
+        _check(_lib().twos_server_pending(self._h, C.byref(q), C.byref(a)))
+        return q.value, a.value
