@@ -1,0 +1,85 @@
+"""The C-ABI library on the CPU: it loads, exports every entry point that
+include/twos.h declares, and its Python binding covers exactly that set.
+No compute is called (there is no GPU here)."""
+
+import ctypes
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "twos.h"
+
+
+def declared() -> set[str]:
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    return set(re.findall(r"\b(twos_[a-z_]+)\s*\(", text))
+
+
+def test_header_declares_the_boundary():
+    names = declared()
+    for required in ("twos_create", "twos_destroy", "twos_estimate", "twos_mesh_create",
+                     "twos_geometry_create", "twos_inside", "twos_last_error", "twos_server_submit"):
+        assert required in names
+
+
+def test_library_exports_every_declared_symbol():
+    from twos import _capi
+    lib = _capi.load()
+    for name in declared():
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", str(_capi.LIB_PATH)], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r" T (twos_[a-z_]+)$", out, flags=re.M))
+    assert declared() <= exported
+
+
+def test_binding_matches_header():
+    from twos import _capi
+    assert set(_capi.SIGNATURES) == declared()
+
+
+def test_struct_layouts_match_header_sizes():
+    from twos import _capi
+    # Sizes follow from the header's field lists on LP64.
+    assert ctypes.sizeof(_capi.Side) == 10 * 8 + 8 * 8 + 4 + 4 + 7 * 8
+    assert ctypes.sizeof(_capi.SiteResult) == 6 * 8 + 3 * 8 + 2 * 4
+    assert ctypes.sizeof(_capi.Params) == 4 * 4 + 8 + 2 * 4 + 8 + 2 * 4 + 2 * 8 + 6 * 4
+    assert ctypes.sizeof(_capi.GeometryDesc) == 6 * 4 + 4 * 8 + 3 * 8 + 3 * 8 + 8 + 3 * 8
+
+
+def test_version_and_no_device_error_without_gpu():
+    import twos
+    assert "gfx950" in twos.version()
+    try:
+        ctx = twos.Context()
+    except twos.DeviceError as e:
+        assert "device" in str(e).lower()
+    else:  # on a GPU box the context simply opens
+        ctx.close()
+
+
+def test_side_validation():
+    import twos
+    with pytest.raises(twos.InvalidArgument):
+        twos.side(poly=[1, 2, 3])
+    with pytest.raises(twos.InvalidArgument):
+        twos.side(poly_classes=(9,))
+    s = twos.reference_affine_side(0.65)
+    assert s.source_kind == 1 and list(s.src) == [1.0, 0.0, 0.65, 0.0, 0.0, 0.5, 0.0]
+
+
+def test_workloads_are_deterministic_and_sized():
+    from twos import workloads as W
+    a, b = W.sh_icosphere(n_sites=512), W.sh_icosphere(n_sites=512)
+    np.testing.assert_array_equal(a.sites, b.sites)
+    assert a.faces.shape == (81920, 3) and a.vertices.shape == (40962, 3)
+    assert a.vertices.min() >= 0.0 and a.vertices.max() <= 1.0 + 1e-12
+    c = W.cooling_block()
+    assert 2500 < len(c.sites) < 4096
+    assert len(c.spheres) == 12 and len(c.cylinders) == 2
+    u = W.unit_ball()
+    assert u.sites.shape == (64, 3) and np.linalg.norm(u.sites, axis=1).max() <= 0.8
