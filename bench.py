@@ -34,7 +34,7 @@ sys.path.insert(0, str(ROOT / "walking-into-holes_amd"))
 HBM_PEAK_GBS = 8000.0
 FP64_VALU_PEAK = 78.6e12
 FP32_VALU_PEAK = 157.3e12
-NODE_BYTES = 64   # both child boxes + child links per internal node visit
+NODE_BYTES = 128  # one 4-wide node: four child boxes + four child links
 TRI_BYTES = 36    # nine binary32 coordinates per triangle test
 
 

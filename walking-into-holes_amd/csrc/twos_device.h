@@ -210,7 +210,8 @@ __device__ __forceinline__ V3<R> closest_point_triangle(V3<R> p, V3<R> a, V3<R> 
 // doubles, plus the original triangle index (tie-break for exact queries).
 
 struct MeshView {
-  const float4 *nodes;
+  const float4 *nodes;   // binary LBVH (ray queries)
+  const float4 *nodes4;  // 4-wide BVH (closest-point queries)
   const float *tf;
   const double *td;
   const int32_t *tid;
@@ -251,78 +252,115 @@ struct TraversalCount {
   uint32_t tris;
 };
 
-// Exact closest point on the mesh.  binary64: distances in binary64 against
-// outward-rounded binary32 boxes, ties broken by the original triangle index,
-// so the result equals a brute-force scan.  binary32: the same traversal in
-// binary32.  Depth of the stack is bounded by the tree depth, which the
-// builder checks against kStackDepth.
-constexpr int kStackDepth = 64;
+// 4-wide node, 128 bytes, eight float4: the four children's bounds as
+// lo.x, lo.y, lo.z, hi.x, hi.y, hi.z rows (one child per lane of the float4),
+// then the four child links (>= 0 node, ~k leaf = sorted triangle k,
+// kEmptyChild unused slot whose box is empty), then padding.
+constexpr int32_t kEmptyChild = int32_t(0x80000000u);
+constexpr int kNode4Quads = 8;
 
+// Traversal stack depth: a 4-wide node pushes at most three siblings, and
+// the builder rejects trees whose 4-wide depth would overflow it.
+constexpr int kStackDepth = 48;
+
+// Exact closest point on the mesh over the 4-wide BVH.  The query first tests
+// `hint` (the previous step's closest triangle: a walk moves by less than
+// twice its distance, so this bound is tight), then visits children
+// nearest-first, testing leaves as soon as their box is within the bound and
+// re-checking each stacked subtree's box distance when it is popped.
+// binary64: binary64 distances against outward-rounded binary32 boxes, ties
+// broken by the original triangle index, so the result equals a brute-force
+// scan.  binary32: the same traversal in binary32.
 template <typename R, bool kCount>
-__device__ __forceinline__ R mesh_closest(const MeshView &m, V3<R> p, V3<R> &best_pt, TraversalCount *cnt) {
+__device__ __forceinline__ R mesh_closest(const MeshView &m, V3<R> p, V3<R> &best_pt, int32_t &hint,
+                                          TraversalCount *cnt) {
+  constexpr bool kExact = sizeof(R) == 8;
   R best = Inf<R>::v();
   int32_t best_id = 0x7fffffff;
-  if (m.n_tris == 1) {
+  int32_t best_k = -1;
+  auto test = [&](int32_t k) {
     V3<R> a, b, c;
-    load_tri<R>(m, 0, a, b, c);
-    best_pt = closest_point_triangle(p, a, b, c);
-    const V3<R> d = p - best_pt;
+    load_tri<R>(m, k, a, b, c);
     if (kCount) cnt->tris += 1;
-    return dot(d, d);
-  }
-  int32_t stack[kStackDepth];
-  int sp = 0;
-  int32_t node = 0;
-  // A tree over n triangles has 2n - 1 nodes and a traversal visits each at
-  // most once; the bound only guards against a corrupt tree.
-  int32_t visits = 2 * m.n_tris + 8;
-  // Pruning slack: binary64 keeps a relative margin so rounding in the box
-  // bound never prunes a triangle that ties the current best.
-  const R slack = sizeof(R) == 8 ? R(1.0 - 1e-12) : R(1);
-  while (true) {
-    if (node >= 0) {
-      const float4 q0 = m.nodes[4 * node + 0];
-      const float4 q1 = m.nodes[4 * node + 1];
-      const float4 q2 = m.nodes[4 * node + 2];
-      const float4 q3 = m.nodes[4 * node + 3];
-      if (kCount) cnt->nodes += 1;
-      const R dl = box_d2<R>(p, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y) * slack;
-      const R dr = box_d2<R>(p, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w) * slack;
-      const int32_t cl = __float_as_int(q3.x), cr = __float_as_int(q3.y);
-      const bool hl = sizeof(R) == 8 ? dl <= best : dl < best;
-      const bool hr = sizeof(R) == 8 ? dr <= best : dr < best;
-      if (--visits < 0) break;
-      if (hl && hr) {
-        const bool left_first = dl <= dr;
-        if (sp < kStackDepth) stack[sp++] = left_first ? cr : cl;
-        node = left_first ? cl : cr;
-        continue;
-      }
-      if (hl) { node = cl; continue; }
-      if (hr) { node = cr; continue; }
-    } else {
-      const int32_t k = ~node;
-      V3<R> a, b, c;
-      load_tri<R>(m, k, a, b, c);
-      if (kCount) cnt->tris += 1;
-      const V3<R> q = closest_point_triangle(p, a, b, c);
-      const V3<R> d = p - q;
-      const R d2 = dot(d, d);
-      if (sizeof(R) == 8) {
-        const int32_t id = m.tid[k];
-        if (d2 < best || (d2 == best && id < best_id)) {
-          best = d2;
-          best_pt = q;
-          best_id = id;
-        }
-      } else if (d2 < best) {
+    const V3<R> q = closest_point_triangle(p, a, b, c);
+    const V3<R> d = p - q;
+    const R d2 = dot(d, d);
+    if constexpr (kExact) {
+      const int32_t id = m.tid[k];
+      if (d2 < best || (d2 == best && id < best_id)) {
         best = d2;
         best_pt = q;
+        best_id = id;
+        best_k = k;
+      }
+    } else if (d2 < best) {
+      best = d2;
+      best_pt = q;
+      best_k = k;
+    }
+  };
+  if (hint >= 0 && hint < m.n_tris) test(hint);
+  if (m.n_tris == 1) {
+    if (best_k < 0) test(0);
+    hint = best_k;
+    return best;
+  }
+  // Relative slack keeps binary64 rounding in a box bound from pruning a
+  // triangle that ties the current best.
+  const R slack = kExact ? R(1.0 - 1e-12) : R(1);
+  auto within = [&](R d) { return kExact ? d * slack <= best : d < best; };
+  int32_t stk_node[kStackDepth];
+  R stk_d[kStackDepth];
+  int sp = 0;
+  int32_t node = 0;
+  int32_t visits = m.n_tris + 8;  // 4-wide nodes < triangles; guards a corrupt tree
+  while (--visits >= 0) {
+    const float4 *q = m.nodes4 + size_t(kNode4Quads) * node;
+    const float4 lx = q[0], ly = q[1], lz = q[2], hx = q[3], hy = q[4], hz = q[5];
+    const int4 ch = *reinterpret_cast<const int4 *>(q + 6);
+    if (kCount) cnt->nodes += 1;
+    R d[4] = {box_d2<R>(p, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x), box_d2<R>(p, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y),
+              box_d2<R>(p, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z), box_d2<R>(p, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w)};
+    int32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
+    // Leaves within the bound are tested at once (they tighten it).
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (c[i] < 0 && c[i] != kEmptyChild && within(d[i])) test(~c[i]);
+    // Internal children within the (tightened) bound, nearest first.
+    R key[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) key[i] = (c[i] >= 0 && within(d[i])) ? d[i] : Inf<R>::v();
+    auto cswap = [&](int i, int j) {
+      if (key[j] < key[i]) {
+        const R tk = key[i]; key[i] = key[j]; key[j] = tk;
+        const int32_t tc = c[i]; c[i] = c[j]; c[j] = tc;
+      }
+    };
+    cswap(0, 1); cswap(2, 3); cswap(0, 2); cswap(1, 3); cswap(1, 2);
+    const R inf = Inf<R>::v();
+    if (key[0] < inf) {
+#pragma unroll
+      for (int i = 3; i >= 1; --i)
+        if (key[i] < inf && sp < kStackDepth) {
+          stk_node[sp] = c[i];
+          stk_d[sp] = key[i];
+          ++sp;
+        }
+      node = c[0];
+      continue;
+    }
+    bool found = false;
+    while (sp > 0) {
+      --sp;
+      if (within(stk_d[sp])) {
+        node = stk_node[sp];
+        found = true;
+        break;
       }
     }
-    if (sp == 0) break;
-    node = stack[--sp];
+    if (!found) break;
   }
+  hint = best_k;
   return best;
 }
 
@@ -355,17 +393,12 @@ __device__ __forceinline__ Vec4<double> load4<double>(const DevGeom &g, int kind
 template <typename R>
 __device__ __forceinline__ R comp(V3<R> v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
 
-template <typename R, int kKind, bool kCount>
-__device__ __forceinline__ R closest_boundary(const DevGeom &g, V3<R> p, V3<R> &point, int &cls,
-                                              TraversalCount *cnt) {
+// Distance to an analytic container (ball or box interior), its closest
+// boundary point and class.
+template <typename R>
+__device__ __forceinline__ R container_analytic(const DevGeom &g, V3<R> p, V3<R> &point, int &cls) {
   R best;
-  if (kKind == kGeomMesh || (kKind == kGeomGeneric && g.container == kMesh)) {
-    MeshView m{g.nodes, g.tris_f, g.tris_d, g.tri_ids, g.n_tris};
-    const R d2 = mesh_closest<R, kCount>(m, p, point, cnt);
-    best = sqrt(d2);
-    if (sizeof(R) == 8 && g.mesh_radius_f32) best = R(float(best));
-    cls = 0;
-  } else if (g.container == kBall) {
+  if (g.container == kBall) {
     const V3<R> c{R(g.ball[0]), R(g.ball[1]), R(g.ball[2])};
     const R rad = R(g.ball[3]);
     const V3<R> d = p - c;
@@ -390,6 +423,13 @@ __device__ __forceinline__ R closest_boundary(const DevGeom &g, V3<R> p, V3<R> &
     const R plane = (cls & 1) ? R(g.box_hi[cls >> 1]) : R(g.box_lo[cls >> 1]);
     if ((cls >> 1) == 0) point.x = plane; else if ((cls >> 1) == 1) point.y = plane; else point.z = plane;
   }
+  return best;
+}
+
+// Holes (kKind != kGeomMesh) and cavities after the container: each wins
+// only when strictly closer than the best so far.
+template <typename R, int kKind>
+__device__ __forceinline__ R holes_and_cavities(const DevGeom &g, V3<R> p, R best, V3<R> &point, int &cls) {
   if (kKind != kGeomMesh) {
     for (int i = 0; i < g.n_spheres; ++i) {
       const Vec4<R> s = load4<R>(g, 0, i);
@@ -434,6 +474,159 @@ __device__ __forceinline__ R closest_boundary(const DevGeom &g, V3<R> p, V3<R> &
     }
   }
   return best;
+}
+
+// Mesh distance from a squared closest distance: binary64 walks replaying the
+// reference's recorded outputs round it to binary32 (the float query radius
+// of wih_twos.cpp:426).
+template <typename R>
+__device__ __forceinline__ R mesh_distance(const DevGeom &g, R d2) {
+  R best = sqrt(d2);
+  if (sizeof(R) == 8 && g.mesh_radius_f32) best = R(float(best));
+  return best;
+}
+
+template <typename R, int kKind, bool kCount>
+__device__ __forceinline__ R closest_boundary(const DevGeom &g, V3<R> p, V3<R> &point, int &cls, int32_t &hint,
+                                              TraversalCount *cnt) {
+  R best;
+  if (kKind == kGeomMesh || (kKind == kGeomGeneric && g.container == kMesh)) {
+    MeshView m{g.nodes, g.nodes4, g.tris_f, g.tris_d, g.tri_ids, g.n_tris};
+    best = mesh_distance<R>(g, mesh_closest<R, kCount>(m, p, point, hint, cnt));
+    cls = 0;
+  } else {
+    best = container_analytic<R>(g, p, point, cls);
+  }
+  return holes_and_cavities<R, kKind>(g, p, best, point, cls);
+}
+
+// ---------------------------------------------------------------------------
+// Resumable closest-point query for the wavefront kernel: the same traversal
+// as mesh_closest, one 4-wide node per call, so lanes whose query ends early
+// go on with their walk while others are still traversing.  The stack lives
+// in LDS (kLdsStack entries per lane, column-major so a wave's lanes hit
+// distinct banks) with a private overflow; an entry packs the 4-wide node
+// index (21 bits) with a lower bound of its squared box distance (the top
+// 11 bits of the binary32 pattern, truncated, so never above the bound).
+constexpr int kLdsStack = 24;
+constexpr int kOverflowStack = kStackDepth - kLdsStack;
+
+template <typename R>
+struct MeshQuery {
+  R best;
+  V3<R> pt;
+  int32_t best_id;
+  int32_t best_k;
+  int32_t node;
+  int32_t sp;
+};
+
+__device__ __forceinline__ uint32_t stack_pack(int32_t node, float lower) {
+  return (uint32_t(node) << 11) | ((__float_as_uint(lower) >> 20) & 0x7FFu);
+}
+__device__ __forceinline__ int32_t stack_node(uint32_t e) { return int32_t(e >> 11); }
+__device__ __forceinline__ float stack_lower(uint32_t e) { return __uint_as_float((e & 0x7FFu) << 20); }
+
+template <typename R>
+__device__ __forceinline__ float lower_f32(R d2);
+template <>
+__device__ __forceinline__ float lower_f32<float>(float d2) { return d2; }
+template <>
+__device__ __forceinline__ float lower_f32<double>(double d2) { return __double2float_rd(d2); }
+
+template <typename R, bool kCount>
+__device__ __forceinline__ void mq_test(const MeshView &m, V3<R> p, MeshQuery<R> &q, int32_t k, TraversalCount *cnt) {
+  V3<R> a, b, c;
+  load_tri<R>(m, k, a, b, c);
+  if (kCount) cnt->tris += 1;
+  const V3<R> cp = closest_point_triangle(p, a, b, c);
+  const V3<R> d = p - cp;
+  const R d2 = dot(d, d);
+  if constexpr (sizeof(R) == 8) {
+    const int32_t id = m.tid[k];
+    if (d2 < q.best || (d2 == q.best && id < q.best_id)) {
+      q.best = d2;
+      q.pt = cp;
+      q.best_id = id;
+      q.best_k = k;
+    }
+  } else if (d2 < q.best) {
+    q.best = d2;
+    q.pt = cp;
+    q.best_k = k;
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ bool mq_within(const MeshQuery<R> &q, R d) {
+  if constexpr (sizeof(R) == 8) return d * R(1.0 - 1e-12) <= q.best;
+  else return d < q.best;
+}
+
+// Starts a query; returns true when it is already complete.
+template <typename R, bool kCount>
+__device__ __forceinline__ bool mq_begin(const MeshView &m, V3<R> p, MeshQuery<R> &q, int32_t hint,
+                                         TraversalCount *cnt) {
+  q.best = Inf<R>::v();
+  q.best_id = 0x7fffffff;
+  q.best_k = -1;
+  q.node = 0;
+  q.sp = 0;
+  if (hint >= 0 && hint < m.n_tris) mq_test<R, kCount>(m, p, q, hint, cnt);
+  if (m.n_tris == 1) {
+    if (q.best_k < 0) mq_test<R, kCount>(m, p, q, 0, cnt);
+    return true;
+  }
+  return false;
+}
+
+// Visits q.node; returns true when the query is complete.
+template <typename R, bool kCount, int kBlock>
+__device__ __forceinline__ bool mq_visit(const MeshView &m, V3<R> p, MeshQuery<R> &q, uint32_t *lds,
+                                         uint32_t *overflow, TraversalCount *cnt) {
+  const float4 *nq = m.nodes4 + size_t(kNode4Quads) * q.node;
+  const float4 lx = nq[0], ly = nq[1], lz = nq[2], hx = nq[3], hy = nq[4], hz = nq[5];
+  const int4 ch = *reinterpret_cast<const int4 *>(nq + 6);
+  if (kCount) cnt->nodes += 1;
+  R d[4] = {box_d2<R>(p, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x), box_d2<R>(p, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y),
+            box_d2<R>(p, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z), box_d2<R>(p, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w)};
+  int32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (c[i] < 0 && c[i] != kEmptyChild && mq_within(q, d[i])) mq_test<R, kCount>(m, p, q, ~c[i], cnt);
+  R key[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) key[i] = (c[i] >= 0 && mq_within(q, d[i])) ? d[i] : Inf<R>::v();
+  auto cswap = [&](int i, int j) {
+    if (key[j] < key[i]) {
+      const R tk = key[i]; key[i] = key[j]; key[j] = tk;
+      const int32_t tc = c[i]; c[i] = c[j]; c[j] = tc;
+    }
+  };
+  cswap(0, 1); cswap(2, 3); cswap(0, 2); cswap(1, 3); cswap(1, 2);
+  const R inf = Inf<R>::v();
+  if (key[0] < inf) {
+#pragma unroll
+    for (int i = 3; i >= 1; --i) {
+      if (key[i] < inf && q.sp < kStackDepth) {
+        const uint32_t e = stack_pack(c[i], lower_f32<R>(key[i]));
+        if (q.sp < kLdsStack) lds[q.sp * kBlock] = e;
+        else overflow[q.sp - kLdsStack] = e;
+        ++q.sp;
+      }
+    }
+    q.node = c[0];
+    return false;
+  }
+  while (q.sp > 0) {
+    --q.sp;
+    const uint32_t e = q.sp < kLdsStack ? lds[q.sp * kBlock] : overflow[q.sp - kLdsStack];
+    if (mq_within(q, R(stack_lower(e)))) {
+      q.node = stack_node(e);
+      return false;
+    }
+  }
+  return true;
 }
 
 // ---------------------------------------------------------------------------

@@ -12,6 +12,7 @@ struct Tables;
 
 struct MeshBuffers {
   float4 *nodes = nullptr;
+  float4 *nodes4 = nullptr;
   float *tris_f = nullptr;
   double *tris_d = nullptr;
   int32_t *tri_ids = nullptr;

@@ -298,6 +298,60 @@ __global__ void reorder_tris(const double *vtx, const uint32_t *faces, const uin
   tid[k] = int32_t(t);
 }
 
+// 4-wide collapse: binary nodes at even depth become 4-wide nodes whose
+// children are the grandchildren through each internal child (a leaf child
+// stays a leaf).  flag[i] = 1 for even depth; an exclusive scan of the flags
+// numbers the 4-wide nodes, the root staying 0.
+__global__ void even_depth(const float4 *nodes, int32_t n_internal, uint32_t *flag, uint32_t *map) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_internal) return;
+  int32_t depth = 0;
+  int32_t p = __float_as_int(nodes[4 * size_t(i) + 3].z);
+  while (p >= 0 && depth < 4096) {
+    ++depth;
+    p = __float_as_int(nodes[4 * size_t(p) + 3].z);
+  }
+  flag[i] = (depth & 1) == 0 ? 1u : 0u;
+  map[i] = flag[i];
+}
+
+__global__ void collapse4(const float4 *nodes, const uint32_t *flag, const uint32_t *map, int32_t n_internal,
+                          float4 *out) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_internal || !flag[i]) return;
+  float lo[3][4], hi[3][4];
+  int32_t ch[4];
+  for (int s = 0; s < 4; ++s) {
+    for (int k = 0; k < 3; ++k) { lo[k][s] = __builtin_huge_valf(); hi[k][s] = -__builtin_huge_valf(); }
+    ch[s] = kEmptyChild;
+  }
+  int slot = 0;
+  const float *f = reinterpret_cast<const float *>(nodes + 4 * size_t(i));
+  for (int side = 0; side < 2; ++side) {
+    const int32_t c = __float_as_int(f[12 + side]);
+    if (c < 0) {
+      for (int k = 0; k < 3; ++k) { lo[k][slot] = f[6 * side + k]; hi[k][slot] = f[6 * side + 3 + k]; }
+      ch[slot++] = c;
+      continue;
+    }
+    const float *g = reinterpret_cast<const float *>(nodes + 4 * size_t(c));
+    for (int s2 = 0; s2 < 2; ++s2) {
+      const int32_t cc = __float_as_int(g[12 + s2]);
+      for (int k = 0; k < 3; ++k) { lo[k][slot] = g[6 * s2 + k]; hi[k][slot] = g[6 * s2 + 3 + k]; }
+      ch[slot++] = cc < 0 ? cc : int32_t(map[cc]);
+    }
+  }
+  float4 *o = out + size_t(kNode4Quads) * map[i];
+  o[0] = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+  o[1] = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+  o[2] = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+  o[3] = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+  o[4] = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+  o[5] = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+  o[6] = make_float4(__int_as_float(ch[0]), __int_as_float(ch[1]), __int_as_float(ch[2]), __int_as_float(ch[3]));
+  o[7] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 __global__ void leaf_depth(const float4 *nodes, const int32_t *leaf_parent, int32_t n, int32_t *max_depth) {
   const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
@@ -407,7 +461,8 @@ __global__ void closest_kernel(const DevGeom *g, const double *pts, int64_t n, d
   const V3<R> p{R(pts[3 * i]), R(pts[3 * i + 1]), R(pts[3 * i + 2])};
   V3<R> q;
   int c = 0;
-  const R d = closest_boundary<R, kGeomGeneric, false>(*g, p, q, c, nullptr);
+  int32_t hint = -1;
+  const R d = closest_boundary<R, kGeomGeneric, false>(*g, p, q, c, hint, nullptr);
   dist[i] = double(d);
   cp[3 * i] = double(q.x);
   cp[3 * i + 1] = double(q.y);
@@ -462,6 +517,7 @@ hipError_t lbvh_build(const double *d_vtx, const uint32_t *d_faces, int32_t n, M
   TRY(hipMalloc(&flags, sizeof(unsigned int) * size_t(std::max(n - 1, 1))));
   TRY(hipMalloc(&ticket, sizeof(unsigned int)));
   TRY(hipMalloc(&mb.nodes, sizeof(float4) * 4 * size_t(std::max(n - 1, 1))));
+  TRY(hipMalloc(&mb.nodes4, sizeof(float4) * kNode4Quads * size_t(std::max(n - 1, 1))));
   TRY(hipMalloc(&mb.tris_f, sizeof(float) * 9 * size_t(n)));
   TRY(hipMalloc(&mb.tris_d, sizeof(double) * 9 * size_t(n)));
   TRY(hipMalloc(&mb.tri_ids, sizeof(int32_t) * size_t(n)));
@@ -489,6 +545,12 @@ hipError_t lbvh_build(const double *d_vtx, const uint32_t *d_faces, int32_t n, M
     TRY(hipMemsetAsync(flags, 0, sizeof(unsigned int) * size_t(n - 1), stream));
     hipLaunchKernelGGL(refit, dim3(blocks_for(n, 256)), dim3(256), 0, stream, mb.nodes, leaf_parent, vals[cur],
                        tri_box, n, flags);
+    // flags doubles as the parity flags, keys[cur ^ 1] as the 4-wide numbering.
+    uint32_t *parity = flags, *map = keys[cur ^ 1];
+    hipLaunchKernelGGL(even_depth, dim3(blocks_for(n - 1, 256)), dim3(256), 0, stream, mb.nodes, n - 1, parity, map);
+    hipLaunchKernelGGL(scan_exclusive, dim3(1), dim3(1024), 0, stream, map, n - 1);
+    hipLaunchKernelGGL(collapse4, dim3(blocks_for(n - 1, 256)), dim3(256), 0, stream, mb.nodes, parity, map, n - 1,
+                       mb.nodes4);
   }
   hipLaunchKernelGGL(reorder_tris, dim3(blocks_for(n, 256)), dim3(256), 0, stream, d_vtx, d_faces, vals[cur], n,
                      mb.tris_f, mb.tris_d, mb.tri_ids);
@@ -503,7 +565,7 @@ hipError_t lbvh_build(const double *d_vtx, const uint32_t *d_faces, int32_t n, M
 }
 
 hipError_t launch_inside(const MeshBuffers &mb, const double *d_pts, int64_t n, int32_t *d_out, hipStream_t stream) {
-  MeshView m{mb.nodes, mb.tris_f, mb.tris_d, mb.tri_ids, mb.n_tris};
+  MeshView m{mb.nodes, mb.nodes4, mb.tris_f, mb.tris_d, mb.tri_ids, mb.n_tris};
   hipLaunchKernelGGL(inside_kernel, dim3(blocks_for(n, 128)), dim3(128), 0, stream, m, d_pts, n, d_out);
   return hipGetLastError();
 }

@@ -793,6 +793,7 @@ twos_status build_geom(twos_ctx *ctx, GeomEntry &g) {
     v.tris_f = mb.tris_f;
     v.tris_d = mb.tris_d;
     v.tri_ids = mb.tri_ids;
+    v.nodes4 = mb.nodes4;
     v.n_tris = mb.n_tris;
     it->second.users += 1;
   }
@@ -928,7 +929,7 @@ void twos_destroy(twos_ctx *ctx) {
   for (auto &kv : ctx->geoms) free_geom(ctx, kv.second);
   for (auto &kv : ctx->meshes) {
     MeshBuffers &mb = kv.second.mb;
-    (void)hipFree(mb.nodes); (void)hipFree(mb.tris_f); (void)hipFree(mb.tris_d); (void)hipFree(mb.tri_ids);
+    (void)hipFree(mb.nodes); (void)hipFree(mb.nodes4); (void)hipFree(mb.tris_f); (void)hipFree(mb.tris_d); (void)hipFree(mb.tri_ids);
   }
   (void)hipFree(ctx->d_geoms);
   (void)hipStreamDestroy(ctx->stream);
@@ -968,9 +969,10 @@ twos_status twos_mesh_create(twos_ctx *ctx, const double *vertices, int64_t nv, 
   (void)hipFree(d_f);
   if (e != hipSuccess) return fail(TWOS_ERR_DEVICE, std::string("LBVH build: ") + hipGetErrorString(e));
   m.build_ms = ms;
-  m.bytes = int64_t(nf > 1 ? nf - 1 : 1) * 64 + nf * (36 + 72 + 4);
-  if (m.depth >= kStackDepth) {
-    (void)hipFree(m.mb.nodes); (void)hipFree(m.mb.tris_f); (void)hipFree(m.mb.tris_d); (void)hipFree(m.mb.tri_ids);
+  m.bytes = int64_t(nf > 1 ? nf - 1 : 1) * (64 + 128) + nf * (36 + 72 + 4);
+  // The 4-wide traversal stacks at most three siblings per 4-wide level.
+  if (3 * ((m.depth - 1 + 1) / 2) >= kStackDepth) {
+    (void)hipFree(m.mb.nodes); (void)hipFree(m.mb.nodes4); (void)hipFree(m.mb.tris_f); (void)hipFree(m.mb.tris_d); (void)hipFree(m.mb.tri_ids);
     return fail(TWOS_ERR_GEOMETRY, "LBVH deeper than the traversal stack");
   }
   const int32_t id = ctx->next_mesh++;
@@ -997,7 +999,7 @@ twos_status twos_mesh_release(twos_ctx *ctx, int32_t mesh_id) {
   if (it == ctx->meshes.end()) return fail(TWOS_ERR_NOT_FOUND, "unknown mesh id");
   if (it->second.users > 0) return fail(TWOS_ERR_INVALID_ARGUMENT, "mesh still used by a geometry");
   MeshBuffers &mb = it->second.mb;
-  (void)hipFree(mb.nodes); (void)hipFree(mb.tris_f); (void)hipFree(mb.tris_d); (void)hipFree(mb.tri_ids);
+  (void)hipFree(mb.nodes); (void)hipFree(mb.nodes4); (void)hipFree(mb.tris_f); (void)hipFree(mb.tris_d); (void)hipFree(mb.tri_ids);
   ctx->meshes.erase(it);
   return TWOS_OK;
 }

@@ -31,6 +31,7 @@ struct DevGeom {
   const float4 *spheres_f, *cylinders_f, *cavities_f;
   const double *spheres_d, *cylinders_d, *cavities_d;
   const float4 *nodes;
+  const float4 *nodes4;
   const float *tris_f;
   const double *tris_d;
   const int32_t *tri_ids;

@@ -42,12 +42,14 @@ struct Job {
   int32_t walk;
   int32_t slot;
   int32_t max_steps;
+  int32_t hint;  // sorted index of the previous closest triangle, -1 before the first query
   R eps;
   int64_t t;
   uint2 key;
   const DevGeom *g;
   const DevSide *s0;
   const DevSide *s1;
+  MeshView mv;  // the geometry's mesh buffers, kept per lane (unused fields fold away)
 };
 
 template <typename R>
@@ -65,10 +67,12 @@ __device__ __forceinline__ void job_init(Job<R> &j, const Tables &tb, const Walk
   j.acc0 = R(0);
   j.acc1 = R(0);
   j.steps = 0;
+  j.hint = -1;
   j.max_steps = tb.slots.max_steps[slot];
   j.eps = R(tb.slots.eps[slot]);
   j.key = tb.slots.keys[2 * size_t(slot) + (wp.combine == kCross ? j.side : 0)];
   j.g = tb.geoms + tb.slots.geom[slot];
+  j.mv = MeshView{j.g->nodes, j.g->nodes4, j.g->tris_f, j.g->tris_d, j.g->tri_ids, j.g->n_tris};
   const int32_t prob = tb.slots.problem[slot];
   j.s0 = tb.sides + 2 * prob + (wp.combine == kCross ? j.side : 0);
   j.s1 = tb.sides + 2 * prob + 1;
@@ -97,7 +101,7 @@ __device__ __forceinline__ int job_step(Job<R> &j, const WalkParams &wp, Travers
   if (j.steps >= j.max_steps) return 2;
   V3<R> bp;
   int cls;
-  const R r = closest_boundary<R, GK, kCount>(*j.g, j.x, bp, cls, cnt);
+  const R r = closest_boundary<R, GK, kCount>(*j.g, j.x, bp, cls, j.hint, cnt);
   if (radius_out) *radius_out = r;
   const DevSide &s0 = *j.s0;
   if (!(r > j.eps)) {
@@ -144,7 +148,7 @@ __global__ __launch_bounds__(256) void walk_simple(Tables tb, WalkParams wp) {
       if (wp.trace && row < wp.trace_rows) {
         V3<R> bp;
         int cls;
-        const R rr = closest_boundary<R, GK, false>(*j.g, j.x, bp, cls, nullptr);
+        const R rr = closest_boundary<R, GK, false>(*j.g, j.x, bp, cls, j.hint, nullptr);
         double *o = wp.trace + 4 * row++;
         o[0] = double(j.x.x); o[1] = double(j.x.y); o[2] = double(j.x.z); o[3] = double(rr);
       }
@@ -166,22 +170,60 @@ __global__ __launch_bounds__(256) void walk_simple(Tables tb, WalkParams wp) {
 // ---------------------------------------------------------------------------
 // Persistent wavefront kernel with walk regeneration (Philox streams).
 //
-// Each wave keeps a chunk of kChunk consecutive job tickets taken from the
-// round's global counter with a single atomic.  Whenever lanes finish a path
-// they take the next tickets of the chunk in lane order; when the chunk runs
-// dry one lane fetches the next.  So lanes stay busy until the round's last
-// tickets are handed out, and consecutive lanes start at the same site.  The
-// loop ends for a wave when the counter is exhausted and none of its lanes
-// holds a path: every wave reaches that state because every path ends within
-// max_steps steps.
+// Each loop iteration advances every lane by one unit of work: one 4-wide
+// node of its current closest-point query, or -- when that query has just
+// ended -- the walk step it decides (payoff and retirement inside the
+// eps-shell, otherwise the Green-ball source, a Philox direction and the
+// move) and the start of the next query.  Lanes therefore never wait for the
+// slowest traversal of their wave.  A lane without a path takes the next job
+// ticket: each wave keeps a chunk of kChunk consecutive tickets taken from the
+// round's counter with one atomic and hands them out in lane order, so
+// consecutive lanes start at the same site.  A wave leaves the loop when the
+// counter is exhausted and none of its lanes holds a path; every wave gets
+// there because every path ends within max_steps steps.
 constexpr int kChunk = 64;
+constexpr int kWalkBlock = 256;
+
+// Applies a finished boundary query (distance r, closest point bp of class
+// cls) to the path: 0 moved on, 1 ended in the eps-shell, 2 capped.
+template <typename R>
+__device__ __forceinline__ int job_advance(Job<R> &j, const WalkParams &wp, R r, V3<R> bp, int cls) {
+  const DevSide &s0 = *j.s0;
+  if (!(r > j.eps)) {
+    j.acc0 += side_payoff<R>(s0, bp, cls);
+    if (wp.combine == kPaired) j.acc1 += side_payoff<R>(*j.s1, bp, cls);
+    return 1;
+  }
+  if (s0.source_kind == 1) j.acc0 += side_source<R>(s0, j.x) * r * r / R(6);
+  if (wp.combine == kPaired && j.s1->source_kind == 1) j.acc1 += side_source<R>(*j.s1, j.x) * r * r / R(6);
+  const V3<R> d = philox_dir(j);
+  j.x = j.x + d * r;
+  ++j.steps;
+  return j.steps >= j.max_steps ? 2 : 0;
+}
+
+template <typename R, int GK>
+__device__ __forceinline__ bool is_mesh(const Job<R> &j) {
+  return GK == kGeomMesh || (GK == kGeomGeneric && j.g->container == kMesh);
+}
+
+// Starts the query at the path's position; true while a traversal is pending.
+template <typename R, int GK, bool kCount>
+__device__ __forceinline__ bool query_start(Job<R> &j, MeshQuery<R> &q, TraversalCount *cnt) {
+  if (GK == kGeomAnalytic || !is_mesh<R, GK>(j)) return false;
+  return !mq_begin<R, kCount>(j.mv, j.x, q, j.hint, cnt);
+}
 
 template <typename R, int GK, bool kCount>
-__global__ __launch_bounds__(256) void walk_persistent(Tables tb, WalkParams wp) {
+__global__ __launch_bounds__(kWalkBlock) void walk_persistent(Tables tb, WalkParams wp) {
+  __shared__ uint32_t lds_stack[kLdsStack * kWalkBlock];
+  uint32_t *stack = lds_stack + threadIdx.x;
+  uint32_t overflow[kOverflowStack];
   const int lane = int(threadIdx.x & 63);
   const int64_t total = *tb.round.total_jobs;
   Job<R> j;
-  bool active = false;
+  MeshQuery<R> q;
+  bool active = false, in_query = false;
   int64_t chunk_cur = 0, chunk_end = 0;  // wave-uniform
   bool drained = false;                  // wave-uniform
   TraversalCount cnt{0, 0};
@@ -200,27 +242,39 @@ __global__ __launch_bounds__(256) void walk_persistent(Tables tb, WalkParams wp)
         chunk_end = min(int64_t(base) + kChunk, total);
       }
       const int64_t avail = chunk_end - chunk_cur;
-      const int cnt_need = __popcll(need);
-      const int take = int(min<int64_t>(avail, cnt_need));
+      const int take = int(min<int64_t>(avail, __popcll(need)));
       const int rank = __popcll(need & ((1ull << lane) - 1ull));
       if (!active && rank < take) {
         job_init(j, tb, wp, chunk_cur + rank);
         active = true;
+        in_query = query_start<R, GK, kCount>(j, q, &cnt);
       }
       chunk_cur += take;
       need = __ballot(!active);
     }
     if (!__any(active)) break;
     if (active) {
-      R r;
-      const int state = job_step<R, GK, kCount>(j, wp, &cnt, &r);
-      if (state == 0) {
-        const V3<R> d = philox_dir(j);
-        j.x = j.x + d * r;
-        ++j.steps;
-      } else {
-        job_store(j, wp, state == 2);
-        active = false;
+      if (in_query) in_query = !mq_visit<R, kCount, kWalkBlock>(j.mv, j.x, q, stack, overflow, &cnt);
+      if (!in_query) {
+        V3<R> bp;
+        int cls;
+        R r;
+        if (GK != kGeomAnalytic && is_mesh<R, GK>(j)) {
+          r = mesh_distance<R>(*j.g, q.best);
+          bp = q.pt;
+          cls = 0;
+          j.hint = q.best_k;
+        } else {
+          r = container_analytic<R>(*j.g, j.x, bp, cls);
+        }
+        r = holes_and_cavities<R, GK>(*j.g, j.x, r, bp, cls);
+        const int st = job_advance(j, wp, r, bp, cls);
+        if (st == 0) {
+          in_query = query_start<R, GK, kCount>(j, q, &cnt);
+        } else {
+          job_store(j, wp, st == 2);
+          active = false;
+        }
       }
     }
   }
