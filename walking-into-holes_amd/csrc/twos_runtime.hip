@@ -160,7 +160,7 @@ struct Engine {
     graph_key = -1;
     dfree(s.pos); dfree(s.geom); dfree(s.problem); dfree(s.keys); dfree(s.site); dfree(s.seed);
     dfree(s.repeat); dfree(s.walks_max); dfree(s.max_steps); dfree(s.eps); dfree(s.ci_rel); dfree(s.ci_z);
-    dfree(s.acc); dfree(s.acc_walks); dfree(s.acc_steps); dfree(s.acc_caps); dfree(s.result);
+    dfree(s.acc); dfree(s.acc_walks); dfree(s.acc_steps); dfree(s.acc_caps); dfree(s.result); dfree(s.hint0);
     dfree(d_sides); dfree(d_val); dfree(d_steps); dfree(d_counters); dfree(d_in); dfree(d_out); dfree(d_plain);
     if (h_in) (void)hipHostFree(h_in);
     if (h_out) (void)hipHostFree(h_out);
@@ -181,7 +181,7 @@ struct Engine {
     HIP_OK(dalloc(&s.repeat, C)); HIP_OK(dalloc(&s.walks_max, C)); HIP_OK(dalloc(&s.max_steps, C));
     HIP_OK(dalloc(&s.eps, C)); HIP_OK(dalloc(&s.ci_rel, C)); HIP_OK(dalloc(&s.ci_z, C));
     HIP_OK(dalloc(&s.acc, 8 * C)); HIP_OK(dalloc(&s.acc_walks, C)); HIP_OK(dalloc(&s.acc_steps, C));
-    HIP_OK(dalloc(&s.acc_caps, C)); HIP_OK(dalloc(&s.result, C));
+    HIP_OK(dalloc(&s.acc_caps, C)); HIP_OK(dalloc(&s.result, C)); HIP_OK(dalloc(&s.hint0, C));
     HIP_OK(dalloc(&d_sides, 2 * size_t(cap_problems)));
     HIP_OK(dalloc(&d_val, 2 * size_t(cap_walks)));
     HIP_OK(dalloc(&d_steps, 2 * size_t(cap_walks)));
@@ -310,6 +310,8 @@ struct Engine {
     HIP_OK(hipMemsetAsync(s.acc_steps + s0, 0, 8 * n, st));
     HIP_OK(hipMemsetAsync(s.acc_caps + s0, 0, 8 * n, st));
     HIP_OK(hipMemcpyAsync(d_sides + 2 * r->problem, r->sides, sizeof(DevSide) * 2, hipMemcpyHostToDevice, st));
+    // Each new site's first closest triangle seeds its paths' queries.
+    if (ctx->d_geoms) HIP_OK(launch_slot_hints(tables(0, 0), s0, n, st));
     // The host vectors above die at return: finish the copies first.
     HIP_OK(hipStreamSynchronize(st));
     r->done.assign(n, 0);
@@ -348,15 +350,6 @@ struct Engine {
     return std::min<int32_t>(d, wmax - d);
   }
 
-  int persistent_block() const { return 256; }
-
-  int persistent_grid(const twos_params &p) const {
-    const int block = p.block > 0 ? p.block : persistent_block();
-    const int wps = p.waves_per_simd > 0 ? p.waves_per_simd : 8;
-    const int waves = ctx->prop.multiProcessorCount * 4 * wps;
-    return std::max(1, waves * 64 / block);
-  }
-
   twos_status launch_walks(const Request *r, int cls, int64_t e0, int64_t total_walks, int kind) {
     const twos_params &p = r->prm;
     Tables tb = tables(cls, e0);
@@ -371,21 +364,24 @@ struct Engine {
     wp.counters = d_counters + kMaxClasses;
     hipStream_t st = ctx->stream;
     const bool count = (p.flags & TWOS_FLAG_COUNT_TRAVERSAL) != 0;
-    const int block = p.block > 0 ? p.block : 256;
     hipError_t e;
     if (p.rng != kRngPhilox) {
       e = launch_walk_simple_f64(kind, true, tb, wp, 64, st);
     } else if (p.schedule == TWOS_PLAIN) {
+      const int block = p.block > 0 ? p.block : 256;
       e = p.precision == 64 ? launch_walk_simple_f64(kind, false, tb, wp, block, st)
                             : launch_walk_simple_f32(kind, tb, wp, block, st);
     } else {
+      // The persistent kernel is written for 256-thread blocks (LDS stack).
       HIP_OK(launch_zero(tb.round.counter, 1, st));
-      const int grid = persistent_grid(p);
+      int grid = 0;
+      const int cus = ctx->prop.multiProcessorCount;
+      e = p.precision == 64
+              ? launch_walk_persistent_f64(kind, count, tb, wp, 256, cus, st, &grid)
+              : launch_walk_persistent_f32(kind, count, p.waves_per_simd, tb, wp, 256, cus, st, &grid);
       stats.grid = grid;
-      stats.block = block;
-      stats.waves_per_simd = p.waves_per_simd > 0 ? p.waves_per_simd : 8;
-      e = p.precision == 64 ? launch_walk_persistent_f64(kind, count, tb, wp, grid, block, st)
-                            : launch_walk_persistent_f32(kind, count, tb, wp, grid, block, st);
+      stats.block = 256;
+      stats.waves_per_simd = p.waves_per_simd;
     }
     if (e != hipSuccess) return fail(TWOS_ERR_DEVICE, std::string("walk launch: ") + hipGetErrorString(e));
     stats.walk_launches += 1;
@@ -521,7 +517,7 @@ struct Engine {
       }
     } else if (graphable) {
       const int64_t key = int64_t(r0->prm.precision) * 1000003 + r0->prm.combine * 1009 + kind0 * 7 +
-                          persistent_grid(r0->prm) * 100000007LL + (r0->prm.block > 0 ? r0->prm.block : 256);
+                          int64_t(r0->prm.waves_per_simd) * 100000007LL;
       if (graph_key != key) {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);

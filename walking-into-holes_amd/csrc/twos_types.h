@@ -47,6 +47,7 @@ struct SlotArrays {
   uint64_t *seed;     // request seed (mt19937_64 layouts)
   int32_t *repeat;    // repeat index (MT_SITE)
   int32_t *walks_max;
+  int32_t *hint0;     // sorted index of the closest triangle at the site (mesh), or -1
   int32_t *max_steps;
   double *eps;
   double *ci_rel;

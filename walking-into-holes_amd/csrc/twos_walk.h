@@ -67,7 +67,7 @@ __device__ __forceinline__ void job_init(Job<R> &j, const Tables &tb, const Walk
   j.acc0 = R(0);
   j.acc1 = R(0);
   j.steps = 0;
-  j.hint = -1;
+  j.hint = tb.slots.hint0[slot];
   j.max_steps = tb.slots.max_steps[slot];
   j.eps = R(tb.slots.eps[slot]);
   j.key = tb.slots.keys[2 * size_t(slot) + (wp.combine == kCross ? j.side : 0)];
@@ -214,8 +214,11 @@ __device__ __forceinline__ bool query_start(Job<R> &j, MeshQuery<R> &q, Traversa
   return !mq_begin<R, kCount>(j.mv, j.x, q, j.hint, cnt);
 }
 
-template <typename R, int GK, bool kCount>
-__global__ __launch_bounds__(kWalkBlock) void walk_persistent(Tables tb, WalkParams wp) {
+// kWaves is the occupancy the register allocator is asked to allow
+// (amdgpu_waves_per_eu); 1 leaves the choice to the compiler.
+template <typename R, int GK, bool kCount, int kWaves>
+__global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWaves))) void walk_persistent(
+    Tables tb, WalkParams wp) {
   __shared__ uint32_t lds_stack[kLdsStack * kWalkBlock];
   uint32_t *stack = lds_stack + threadIdx.x;
   uint32_t overflow[kOverflowStack];
@@ -282,6 +285,24 @@ __global__ __launch_bounds__(kWalkBlock) void walk_persistent(Tables tb, WalkPar
     atomicAdd(wp.counters + 0, (unsigned long long)cnt.nodes);
     atomicAdd(wp.counters + 1, (unsigned long long)cnt.tris);
   }
+}
+
+// Closest triangle at each new site: every path of a site starts with the
+// same query, so it is answered once at admission and seeds each path's hint.
+template <typename R>
+__global__ void slot_hints(Tables tb, int64_t first, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t slot = first + i;
+  const DevGeom &g = tb.geoms[tb.slots.geom[slot]];
+  int32_t hint = -1;
+  if (g.container == kMesh) {
+    const double *p = tb.slots.pos + 3 * slot;
+    const MeshView m{g.nodes, g.nodes4, g.tris_f, g.tris_d, g.tri_ids, g.n_tris};
+    V3<R> pt;
+    mesh_closest<R, false>(m, V3<R>{R(p[0]), R(p[1]), R(p[2])}, pt, hint, nullptr);
+  }
+  tb.slots.hint0[slot] = hint;
 }
 
 // ---------------------------------------------------------------------------

@@ -1,5 +1,7 @@
 // binary32 walks: positions, distance queries and Poisson-kernel weights in
 // binary32 (contraction on); samples leave the kernel as binary64.
+#include <algorithm>
+
 #include "twos_launch.h"
 #include "twos_walk.h"
 
@@ -27,23 +29,38 @@ hipError_t launch_walk_simple_f32(int geom_kind, const Tables &tb, const WalkPar
   return hipGetLastError();
 }
 
-hipError_t launch_walk_persistent_f32(int geom_kind, bool count, const Tables &tb, const WalkParams &wp, int grid,
-                                      int block, hipStream_t stream) {
-  const dim3 g(grid), b(block);
-  if (count) {
-    hipLaunchKernelGGL((walk_persistent<float, kGeomGeneric, true>), g, b, 0, stream, tb, wp);
-    return hipGetLastError();
-  }
-  switch (geom_kind) {
-    case kGeomMesh:
-      hipLaunchKernelGGL((walk_persistent<float, kGeomMesh, false>), g, b, 0, stream, tb, wp);
-      break;
-    case kGeomAnalytic:
-      hipLaunchKernelGGL((walk_persistent<float, kGeomAnalytic, false>), g, b, 0, stream, tb, wp);
-      break;
-    default:
-      hipLaunchKernelGGL((walk_persistent<float, kGeomGeneric, false>), g, b, 0, stream, tb, wp);
-  }
+namespace {
+template <int kWaves>
+hipError_t persistent_f32(int geom_kind, bool count, const Tables &tb, const WalkParams &wp, int block,
+                          int cus, hipStream_t stream, int *grid_out) {
+  void (*k)(Tables, WalkParams);
+  if (count) k = walk_persistent<float, kGeomGeneric, true, kWaves>;
+  else if (geom_kind == kGeomMesh) k = walk_persistent<float, kGeomMesh, false, kWaves>;
+  else if (geom_kind == kGeomAnalytic) k = walk_persistent<float, kGeomAnalytic, false, kWaves>;
+  else k = walk_persistent<float, kGeomGeneric, false, kWaves>;
+  int per_cu = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, 0);
+  if (e != hipSuccess) return e;
+  const int grid = std::max(1, per_cu) * cus;
+  *grid_out = grid;
+  hipLaunchKernelGGL(k, dim3(grid), dim3(block), 0, stream, tb, wp);
+  return hipGetLastError();
+}
+}  // namespace
+
+// waves: requested occupancy variant (6 or 8 cap the registers for that many
+// waves per SIMD; anything else uses the compiler's allocation).  The grid is
+// one resident wave set: blocks-per-CU from the occupancy query times CUs.
+hipError_t launch_walk_persistent_f32(int geom_kind, bool count, int waves, const Tables &tb, const WalkParams &wp,
+                                      int block, int cus, hipStream_t stream, int *grid_out) {
+  if (waves == 8) return persistent_f32<8>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
+  if (waves == 6) return persistent_f32<6>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
+  return persistent_f32<1>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
+}
+
+hipError_t launch_slot_hints(const Tables &tb, int64_t first, int64_t n, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(slot_hints<float>, dim3(blocks_for(n, 128)), dim3(128), 0, stream, tb, first, n);
   return hipGetLastError();
 }
 

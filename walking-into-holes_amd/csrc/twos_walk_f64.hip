@@ -1,6 +1,8 @@
 // binary64 walks (built with -ffp-contract=off so each operation rounds once,
 // as the reference build's do), the mt19937_64 replay streams, and the
 // binary64 per-site reduction.
+#include <algorithm>
+
 #include "twos_launch.h"
 #include "twos_walk.h"
 
@@ -33,23 +35,19 @@ hipError_t launch_walk_simple_f64(int geom_kind, bool mt, const Tables &tb, cons
   return hipGetLastError();
 }
 
-hipError_t launch_walk_persistent_f64(int geom_kind, bool count, const Tables &tb, const WalkParams &wp, int grid,
-                                      int block, hipStream_t stream) {
-  const dim3 g(grid), b(block);
-  if (count) {
-    hipLaunchKernelGGL((walk_persistent<double, kGeomGeneric, true>), g, b, 0, stream, tb, wp);
-    return hipGetLastError();
-  }
-  switch (geom_kind) {
-    case kGeomMesh:
-      hipLaunchKernelGGL((walk_persistent<double, kGeomMesh, false>), g, b, 0, stream, tb, wp);
-      break;
-    case kGeomAnalytic:
-      hipLaunchKernelGGL((walk_persistent<double, kGeomAnalytic, false>), g, b, 0, stream, tb, wp);
-      break;
-    default:
-      hipLaunchKernelGGL((walk_persistent<double, kGeomGeneric, false>), g, b, 0, stream, tb, wp);
-  }
+hipError_t launch_walk_persistent_f64(int geom_kind, bool count, const Tables &tb, const WalkParams &wp, int block,
+                                      int cus, hipStream_t stream, int *grid_out) {
+  void (*k)(Tables, WalkParams);
+  if (count) k = walk_persistent<double, kGeomGeneric, true, 1>;
+  else if (geom_kind == kGeomMesh) k = walk_persistent<double, kGeomMesh, false, 1>;
+  else if (geom_kind == kGeomAnalytic) k = walk_persistent<double, kGeomAnalytic, false, 1>;
+  else k = walk_persistent<double, kGeomGeneric, false, 1>;
+  int per_cu = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, 0);
+  if (e != hipSuccess) return e;
+  const int grid = std::max(1, per_cu) * cus;
+  *grid_out = grid;
+  hipLaunchKernelGGL(k, dim3(grid), dim3(block), 0, stream, tb, wp);
   return hipGetLastError();
 }
 
