@@ -51,6 +51,7 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--waves-per-simd", type=int, default=0)
+    ap.add_argument("--exact", action="store_true", help="exact closest-point radius at every mesh step")
     ap.add_argument("--sites", type=int, default=0, help="override the site count (profiling runs)")
     return ap.parse_args()
 
@@ -96,7 +97,8 @@ def main() -> int:
     sides = W.sides(w, twos.side)
     adaptive = w.walks_max > w.walks
     kw = dict(combine=twos.CROSS, precision=w.precision, seed=2026, walks=w.walks, walks_max=w.walks_max,
-              ci_rel=w.ci_rel, eps=w.eps, max_steps=w.max_steps, waves_per_simd=args.waves_per_simd)
+              ci_rel=w.ci_rel, eps=w.eps, max_steps=w.max_steps, waves_per_simd=args.waves_per_simd,
+              distance_grid=(w.container == twos.MESH and not args.exact))
 
     def run(schedule, pts, graph):
         return ctx.estimate(geom, pts, sides, schedule=schedule, use_graph=graph, **kw)

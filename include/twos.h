@@ -133,6 +133,11 @@ typedef struct {
 
 /* Count BVH nodes fetched and triangles tested (slower; for roofline figures). */
 #define TWOS_FLAG_COUNT_TRAVERSAL 1
+/* Mesh steps deep inside the domain take their radius from a conservative
+ * distance grid (a lower bound of the distance, so still a valid Walk on
+ * Spheres step) instead of an exact closest-point query; steps near the
+ * boundary stay exact.  Unbiased, but paths differ from the exact walk. */
+#define TWOS_FLAG_DISTANCE_GRID 2
 
 typedef struct {
   double u;         /* mean of side-0 samples */

@@ -1,0 +1,112 @@
+"""The three synthetic workloads on the GPU against closed forms and the
+CPU oracle (statistical parity: independent Philox paths are not shared
+across implementations when the radius rule differs, so agreement is in
+combined standard errors)."""
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+twos = pytest.importorskip("twos")
+from twos import workloads as W  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = twos.Context()
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def c3(ctx):
+    w = W.sh_icosphere(n_sites=512)
+    mesh = ctx.mesh(w.vertices, w.faces)
+    return w, mesh, ctx.geometry(twos.MESH, mesh=mesh)
+
+
+def _z(a, b):
+    return (a.td - b["td"]) / np.hypot(a.score_se, b["score_se"])
+
+
+def test_unit_ball_c1_against_closed_form(ctx):
+    w = W.unit_ball()
+    g = ctx.geometry(twos.BALL)
+    res = ctx.estimate(g, w.sites, W.sides(w, twos.side), combine=twos.CROSS, precision=64, seed=0,
+                       walks=1024, eps=w.eps, max_steps=8192)
+    z = (res.td - W.unit_ball_exact(w.sites)) / res.score_se
+    # 64 sites: at 3 SE the expected number of misses is 0.2; allow 2.
+    assert np.sum(np.abs(z) > 3) <= 2
+    assert np.max(np.abs(z)) < 5
+
+
+def test_c3_sites_inside_and_away_from_the_surface(ctx, c3):
+    w, mesh, g = c3
+    assert ctx.inside(mesh, w.sites).all()
+    d, _, _ = ctx.closest(g, w.sites, precision=64)
+    assert d.min() > 0.009
+
+
+def test_c3_exact_walks_against_oracle(ctx, c3):
+    w, mesh, g = c3
+    sites = w.sites[:24]
+    kw = dict(combine=twos.CROSS, seed=17, walks=2048, eps=w.eps, max_steps=w.max_steps)
+    gpu = ctx.estimate(g, sites, W.sides(w, twos.side), precision=32, **kw)
+    cpu = O.estimate(O.Geometry(O.MESH, mesh=O.Mesh(w.vertices, w.faces)), sites, W.sides(w, O.side),
+                     rng=O.RNG_PHILOX, threads=0, **{**kw, "seed": 18})
+    z = _z(gpu, cpu)
+    assert np.sum(np.abs(z) > 3) <= 1 and np.max(np.abs(z)) < 5
+
+
+def test_c3_distance_grid_against_exact(ctx, c3):
+    """The conservative-radius walk estimates the same quantity as the exact
+    walk (independent seeds)."""
+    w, mesh, g = c3
+    sites = w.sites[:128]
+    kw = dict(combine=twos.CROSS, precision=32, walks=4096, eps=w.eps, max_steps=w.max_steps)
+    exact = ctx.estimate(g, sites, W.sides(w, twos.side), seed=1, **kw)
+    grid = ctx.estimate(g, sites, W.sides(w, twos.side), seed=2, distance_grid=True, **kw)
+    z = (grid.td - exact.td) / np.hypot(grid.score_se, exact.score_se)
+    assert np.sum(np.abs(z) > 3) <= 2 and np.max(np.abs(z)) < 5
+    assert abs(np.mean(z)) < 0.35  # no systematic shift (128 sites: sd of the mean ~0.09)
+
+
+def test_c3_distance_grid_plain_equals_persistent(ctx, c3):
+    w, mesh, g = c3
+    kw = dict(combine=twos.CROSS, precision=32, seed=4, walks=64, walks_max=512, ci_rel=0.05, eps=w.eps,
+              max_steps=w.max_steps, distance_grid=True)
+    a = ctx.estimate(g, w.sites[:24], W.sides(w, twos.side), schedule=twos.PLAIN, **kw)
+    b = ctx.estimate(g, w.sites[:24], W.sides(w, twos.side), schedule=twos.PERSISTENT, **kw)
+    for k in ("score", "score_se", "walks", "steps"):
+        np.testing.assert_array_equal(getattr(a, k), getattr(b, k), err_msg=k)
+
+
+def test_cooling_block_c2_against_oracle(ctx):
+    w = W.cooling_block()
+    g = ctx.geometry(twos.BOX, box_lo=w.box_lo, box_hi=w.box_hi, spheres=w.spheres, cylinders=w.cylinders)
+    idx = np.linspace(0, len(w.sites) - 1, 32).astype(int)
+    sites = w.sites[idx]
+    kw = dict(combine=twos.CROSS, walks=2048, eps=w.eps, max_steps=w.max_steps)
+    gpu = ctx.estimate(g, sites, W.sides(w, twos.side), precision=32, seed=5, **kw)
+    geo = O.Geometry(O.BOX, box_lo=w.box_lo, box_hi=w.box_hi, spheres=w.spheres, cylinders=w.cylinders)
+    cpu = O.estimate(geo, sites, W.sides(w, O.side), rng=O.RNG_PHILOX, seed=6, **kw)
+    # u in [0, 1] is 1 on holes; v is the z = 1 face's harmonic measure.
+    assert np.all((gpu.u >= 0) & (gpu.u <= 1)) and np.all((gpu.v >= 0) & (gpu.v <= 1))
+    z = _z(gpu, cpu)
+    finite = np.isfinite(z)
+    assert np.sum(np.abs(z[finite]) > 3) <= 1
+
+
+def test_cooling_block_binary32_paths_shadow_binary64(ctx):
+    w = W.cooling_block()
+    g = ctx.geometry(twos.BOX, box_lo=w.box_lo, box_hi=w.box_hi, spheres=w.spheres, cylinders=w.cylinders)
+    sites = w.sites[::97]
+    kw = dict(combine=twos.CROSS, walks=1024, eps=w.eps, max_steps=w.max_steps, seed=8, dump=True)
+    a = ctx.estimate(g, sites, W.sides(w, twos.side), precision=32, **kw)
+    b = ctx.estimate(g, sites, W.sides(w, twos.side), precision=64, **kw)
+    # Same Philox streams: most binary32 paths end on the same boundary piece
+    # as their binary64 rerun.
+    assert np.mean(a.a == b.a) > 0.95 and np.mean(a.b == b.b) > 0.95

@@ -486,13 +486,42 @@ __device__ __forceinline__ R mesh_distance(const DevGeom &g, R d2) {
   return best;
 }
 
+// Conservative radius from the distance grid: with c the centre of the cell
+// holding p (clamped to the grid), the mesh distance is 1-Lipschitz, so
+// d(p) >= d(c) - |p - c|; a small margin covers binary32 rounding of d(c).
+template <typename R>
+__device__ __forceinline__ R grid_lower(const DevGeom &g, V3<R> p) {
+  const R h = R(g.grid_h);
+  int i = int(floor((p.x - R(g.grid_lo[0])) / h));
+  int k = int(floor((p.y - R(g.grid_lo[1])) / h));
+  int l = int(floor((p.z - R(g.grid_lo[2])) / h));
+  i = min(max(i, 0), g.grid_n[0] - 1);
+  k = min(max(k, 0), g.grid_n[1] - 1);
+  l = min(max(l, 0), g.grid_n[2] - 1);
+  const V3<R> c{R(g.grid_lo[0]) + (R(i) + R(0.5)) * h, R(g.grid_lo[1]) + (R(k) + R(0.5)) * h,
+                R(g.grid_lo[2]) + (R(l) + R(0.5)) * h};
+  const R dc = R(g.grid[(size_t(i) * g.grid_n[1] + k) * g.grid_n[2] + l]);
+  const V3<R> d = p - c;
+  return dc - sqrt(dot(d, d)) - R(2e-6) * (R(1) + dc);
+}
+
+// tau < inf: a mesh step whose grid lower bound is at least tau takes that
+// bound as its radius (any ball inside the domain is a valid Walk on Spheres
+// step) and skips the traversal; the point and class are then unused,
+// because such a step never ends the path (tau > eps).
 template <typename R, int kKind, bool kCount>
 __device__ __forceinline__ R closest_boundary(const DevGeom &g, V3<R> p, V3<R> &point, int &cls, int32_t &hint,
-                                              TraversalCount *cnt) {
+                                              TraversalCount *cnt, R tau = Inf<R>::v()) {
   R best;
   if (kKind == kGeomMesh || (kKind == kGeomGeneric && g.container == kMesh)) {
-    MeshView m{g.nodes, g.nodes4, g.tris_f, g.tris_d, g.tri_ids, g.n_tris};
-    best = mesh_distance<R>(g, mesh_closest<R, kCount>(m, p, point, hint, cnt));
+    const R lb = tau < Inf<R>::v() ? grid_lower<R>(g, p) : R(-1);
+    if (lb >= tau) {
+      best = lb;
+      point = p;
+    } else {
+      MeshView m{g.nodes, g.nodes4, g.tris_f, g.tris_d, g.tri_ids, g.n_tris};
+      best = mesh_distance<R>(g, mesh_closest<R, kCount>(m, p, point, hint, cnt));
+    }
     cls = 0;
   } else {
     best = container_analytic<R>(g, p, point, cls);
@@ -513,12 +542,13 @@ constexpr int kOverflowStack = kStackDepth - kLdsStack;
 
 template <typename R>
 struct MeshQuery {
-  R best;
+  R best;        // squared distance so far, or the grid radius when `grid`
   V3<R> pt;
   int32_t best_id;
   int32_t best_k;
   int32_t node;
   int32_t sp;
+  bool grid;
 };
 
 __device__ __forceinline__ uint32_t stack_pack(int32_t node, float lower) {
@@ -572,6 +602,7 @@ __device__ __forceinline__ bool mq_begin(const MeshView &m, V3<R> p, MeshQuery<R
   q.best_k = -1;
   q.node = 0;
   q.sp = 0;
+  q.grid = false;
   if (hint >= 0 && hint < m.n_tris) mq_test<R, kCount>(m, p, q, hint, cnt);
   if (m.n_tris == 1) {
     if (q.best_k < 0) mq_test<R, kCount>(m, p, q, 0, cnt);

@@ -15,6 +15,7 @@
 // triangles are reordered into leaf order in binary32 and binary64.
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "twos_device.h"
@@ -454,6 +455,19 @@ __global__ void inside_kernel(MeshView m, const double *pts, int64_t n, int32_t 
   out[i] = hits == 128 ? -1 : int32_t(hits & 1u);
 }
 
+// Distance at every cell centre of a uniform grid (binary32 exact query).
+__global__ void distance_grid(MeshView m, float lx, float ly, float lz, float h, int nx, int ny, int nz,
+                              float *out) {
+  const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t total = int64_t(nx) * ny * nz;
+  if (idx >= total) return;
+  const int l = int(idx % nz), k = int((idx / nz) % ny), i = int(idx / (int64_t(ny) * nz));
+  const V3<float> p{lx + (float(i) + 0.5f) * h, ly + (float(k) + 0.5f) * h, lz + (float(l) + 0.5f) * h};
+  int32_t hint = -1;
+  V3<float> pt;
+  out[idx] = sqrtf(mesh_closest<float, false>(m, p, pt, hint, nullptr));
+}
+
 template <typename R>
 __global__ void closest_kernel(const DevGeom *g, const double *pts, int64_t n, double *dist, double *cp, int32_t *cls) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -562,6 +576,27 @@ hipError_t lbvh_build(const double *d_vtx, const uint32_t *d_faces, int32_t n, M
   cleanup();
 #undef TRY
   return hipSuccess;
+}
+
+hipError_t build_distance_grid(MeshBuffers &mb, const double lo[3], const double hi[3], int cells,
+                               hipStream_t stream) {
+  double ext = 0.0;
+  for (int k = 0; k < 3; ++k) ext = std::max(ext, hi[k] - lo[k]);
+  const double pad = 1e-3 * ext + 1e-9;
+  const double h = (ext + 2 * pad) / cells;
+  int64_t total = 1;
+  for (int k = 0; k < 3; ++k) {
+    mb.grid_lo[k] = float(lo[k] - pad);
+    mb.grid_n[k] = std::max(1, int(std::ceil((hi[k] - lo[k] + 2 * pad) / h)));
+    total *= mb.grid_n[k];
+  }
+  mb.grid_h = float(h);
+  hipError_t e = hipMalloc(&mb.grid, sizeof(float) * size_t(total));
+  if (e != hipSuccess) return e;
+  MeshView m{mb.nodes, mb.nodes4, mb.tris_f, mb.tris_d, mb.tri_ids, mb.n_tris};
+  hipLaunchKernelGGL(distance_grid, dim3(blocks_for(total, 128)), dim3(128), 0, stream, m, mb.grid_lo[0],
+                     mb.grid_lo[1], mb.grid_lo[2], mb.grid_h, mb.grid_n[0], mb.grid_n[1], mb.grid_n[2], mb.grid);
+  return hipGetLastError();
 }
 
 hipError_t launch_inside(const MeshBuffers &mb, const double *d_pts, int64_t n, int32_t *d_out, hipStream_t stream) {

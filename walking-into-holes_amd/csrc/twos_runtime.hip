@@ -58,6 +58,10 @@ double now_s() {
 }
 
 constexpr int kMaxClasses = 8;
+// Distance grid: cells along the longest bounding-box axis, and the radius
+// threshold (in cell sizes) above which a mesh step takes the grid bound.
+constexpr int kGridCells = 128;
+constexpr float kGridKappa = 4.0f;
 
 }  // namespace
 
@@ -362,6 +366,7 @@ struct Engine {
     wp.val = d_val;
     wp.steps = d_steps;
     wp.counters = d_counters + kMaxClasses;
+    wp.grid_kappa = (p.flags & TWOS_FLAG_DISTANCE_GRID) ? kGridKappa : 0.f;
     hipStream_t st = ctx->stream;
     const bool count = (p.flags & TWOS_FLAG_COUNT_TRAVERSAL) != 0;
     hipError_t e;
@@ -485,6 +490,7 @@ struct Engine {
           wp.job_end = int64_t(en.inc) * jpw;
           wp.val = d_val;
           wp.steps = d_steps;
+          wp.grid_kappa = (en.r->prm.flags & TWOS_FLAG_DISTANCE_GRID) ? kGridKappa : 0.f;
           const int block = en.r->prm.block > 0 ? en.r->prm.block : 256;
           const int kind = geom_kind_of(en.r->geom);
           const hipError_t le = en.r->prm.rng != kRngPhilox ? launch_walk_simple_f64(kind, true, tb, wp, 64, strm)
@@ -790,6 +796,12 @@ twos_status build_geom(twos_ctx *ctx, GeomEntry &g) {
     v.tris_d = mb.tris_d;
     v.tri_ids = mb.tri_ids;
     v.nodes4 = mb.nodes4;
+    v.grid = mb.grid;
+    for (int k = 0; k < 3; ++k) {
+      v.grid_lo[k] = mb.grid_lo[k];
+      v.grid_n[k] = mb.grid_n[k];
+    }
+    v.grid_h = mb.grid_h;
     v.n_tris = mb.n_tris;
     it->second.users += 1;
   }
@@ -925,7 +937,7 @@ void twos_destroy(twos_ctx *ctx) {
   for (auto &kv : ctx->geoms) free_geom(ctx, kv.second);
   for (auto &kv : ctx->meshes) {
     MeshBuffers &mb = kv.second.mb;
-    (void)hipFree(mb.nodes); (void)hipFree(mb.nodes4); (void)hipFree(mb.tris_f); (void)hipFree(mb.tris_d); (void)hipFree(mb.tri_ids);
+    (void)hipFree(mb.nodes); (void)hipFree(mb.nodes4); (void)hipFree(mb.grid); (void)hipFree(mb.tris_f); (void)hipFree(mb.tris_d); (void)hipFree(mb.tri_ids);
   }
   (void)hipFree(ctx->d_geoms);
   (void)hipStreamDestroy(ctx->stream);
@@ -954,7 +966,16 @@ twos_status twos_mesh_create(twos_ctx *ctx, const double *vertices, int64_t nv, 
   HIP_OK(hipEventCreate(&a));
   HIP_OK(hipEventCreate(&b));
   HIP_OK(hipEventRecord(a, ctx->stream));
-  const hipError_t e = lbvh_build(d_v, d_f, int32_t(nf), m.mb, &m.depth, ctx->stream);
+  hipError_t e = lbvh_build(d_v, d_f, int32_t(nf), m.mb, &m.depth, ctx->stream);
+  if (e == hipSuccess) {
+    double lo[3] = {vertices[0], vertices[1], vertices[2]}, hi[3] = {vertices[0], vertices[1], vertices[2]};
+    for (int64_t i = 0; i < nv; ++i)
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(lo[k], vertices[3 * i + k]);
+        hi[k] = std::max(hi[k], vertices[3 * i + k]);
+      }
+    e = build_distance_grid(m.mb, lo, hi, kGridCells, ctx->stream);
+  }
   (void)hipEventRecord(b, ctx->stream);
   (void)hipStreamSynchronize(ctx->stream);
   float ms = 0.f;
@@ -968,7 +989,7 @@ twos_status twos_mesh_create(twos_ctx *ctx, const double *vertices, int64_t nv, 
   m.bytes = int64_t(nf > 1 ? nf - 1 : 1) * (64 + 128) + nf * (36 + 72 + 4);
   // The 4-wide traversal stacks at most three siblings per 4-wide level.
   if (3 * ((m.depth - 1 + 1) / 2) >= kStackDepth) {
-    (void)hipFree(m.mb.nodes); (void)hipFree(m.mb.nodes4); (void)hipFree(m.mb.tris_f); (void)hipFree(m.mb.tris_d); (void)hipFree(m.mb.tri_ids);
+    (void)hipFree(m.mb.nodes); (void)hipFree(m.mb.nodes4); (void)hipFree(m.mb.grid); (void)hipFree(m.mb.tris_f); (void)hipFree(m.mb.tris_d); (void)hipFree(m.mb.tri_ids);
     return fail(TWOS_ERR_GEOMETRY, "LBVH deeper than the traversal stack");
   }
   const int32_t id = ctx->next_mesh++;
@@ -995,7 +1016,7 @@ twos_status twos_mesh_release(twos_ctx *ctx, int32_t mesh_id) {
   if (it == ctx->meshes.end()) return fail(TWOS_ERR_NOT_FOUND, "unknown mesh id");
   if (it->second.users > 0) return fail(TWOS_ERR_INVALID_ARGUMENT, "mesh still used by a geometry");
   MeshBuffers &mb = it->second.mb;
-  (void)hipFree(mb.nodes); (void)hipFree(mb.nodes4); (void)hipFree(mb.tris_f); (void)hipFree(mb.tris_d); (void)hipFree(mb.tri_ids);
+  (void)hipFree(mb.nodes); (void)hipFree(mb.nodes4); (void)hipFree(mb.grid); (void)hipFree(mb.tris_f); (void)hipFree(mb.tris_d); (void)hipFree(mb.tri_ids);
   ctx->meshes.erase(it);
   return TWOS_OK;
 }

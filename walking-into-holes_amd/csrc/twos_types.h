@@ -35,6 +35,11 @@ struct DevGeom {
   const float *tris_f;
   const double *tris_d;
   const int32_t *tri_ids;
+  const float *grid;     // distance at cell centres (conservative-radius grid), or null
+  float grid_lo[3];
+  float grid_h;
+  int32_t grid_n[3];
+  int32_t grid_pad;
 };
 
 // Per-site-slot state resident on the device (capacity = server max_sites).
@@ -75,7 +80,7 @@ struct WalkParams {
   int32_t combine;
   int32_t rng;
   int32_t jobs_per_walk;  // 2 for CROSS, 1 otherwise
-  int32_t pad0;
+  float grid_kappa;       // > 0: mesh steps whose grid lower bound exceeds grid_kappa * cell size use it as radius
   int64_t job_begin;      // simple kernel: first job of this launch
   int64_t job_end;
   double *val;            // [total walks][2] binary64 samples (a, b)

@@ -42,6 +42,7 @@ struct Job {
   int32_t walk;
   int32_t slot;
   int32_t max_steps;
+  R tau;         // grid-radius threshold (infinite when the grid is off)
   int32_t hint;  // sorted index of the previous closest triangle, -1 before the first query
   R eps;
   int64_t t;
@@ -73,6 +74,7 @@ __device__ __forceinline__ void job_init(Job<R> &j, const Tables &tb, const Walk
   j.key = tb.slots.keys[2 * size_t(slot) + (wp.combine == kCross ? j.side : 0)];
   j.g = tb.geoms + tb.slots.geom[slot];
   j.mv = MeshView{j.g->nodes, j.g->nodes4, j.g->tris_f, j.g->tris_d, j.g->tri_ids, j.g->n_tris};
+  j.tau = (wp.grid_kappa > 0.f && j.g->grid) ? R(wp.grid_kappa) * R(j.g->grid_h) : Inf<R>::v();
   const int32_t prob = tb.slots.problem[slot];
   j.s0 = tb.sides + 2 * prob + (wp.combine == kCross ? j.side : 0);
   j.s1 = tb.sides + 2 * prob + 1;
@@ -101,7 +103,7 @@ __device__ __forceinline__ int job_step(Job<R> &j, const WalkParams &wp, Travers
   if (j.steps >= j.max_steps) return 2;
   V3<R> bp;
   int cls;
-  const R r = closest_boundary<R, GK, kCount>(*j.g, j.x, bp, cls, j.hint, cnt);
+  const R r = closest_boundary<R, GK, kCount>(*j.g, j.x, bp, cls, j.hint, cnt, j.tau);
   if (radius_out) *radius_out = r;
   const DevSide &s0 = *j.s0;
   if (!(r > j.eps)) {
@@ -148,7 +150,7 @@ __global__ __launch_bounds__(256) void walk_simple(Tables tb, WalkParams wp) {
       if (wp.trace && row < wp.trace_rows) {
         V3<R> bp;
         int cls;
-        const R rr = closest_boundary<R, GK, false>(*j.g, j.x, bp, cls, j.hint, nullptr);
+        const R rr = closest_boundary<R, GK, false>(*j.g, j.x, bp, cls, j.hint, nullptr, j.tau);
         double *o = wp.trace + 4 * row++;
         o[0] = double(j.x.x); o[1] = double(j.x.y); o[2] = double(j.x.z); o[3] = double(rr);
       }
@@ -211,6 +213,14 @@ __device__ __forceinline__ bool is_mesh(const Job<R> &j) {
 template <typename R, int GK, bool kCount>
 __device__ __forceinline__ bool query_start(Job<R> &j, MeshQuery<R> &q, TraversalCount *cnt) {
   if (GK == kGeomAnalytic || !is_mesh<R, GK>(j)) return false;
+  if (j.tau < Inf<R>::v()) {
+    const R lb = grid_lower<R>(*j.g, j.x);
+    if (lb >= j.tau) {
+      q.grid = true;
+      q.best = lb;
+      return false;
+    }
+  }
   return !mq_begin<R, kCount>(j.mv, j.x, q, j.hint, cnt);
 }
 
@@ -263,10 +273,15 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWav
         int cls;
         R r;
         if (GK != kGeomAnalytic && is_mesh<R, GK>(j)) {
-          r = mesh_distance<R>(*j.g, q.best);
-          bp = q.pt;
           cls = 0;
-          j.hint = q.best_k;
+          if (q.grid) {
+            r = q.best;
+            bp = j.x;
+          } else {
+            r = mesh_distance<R>(*j.g, q.best);
+            bp = q.pt;
+            j.hint = q.best_k;
+          }
         } else {
           r = container_analytic<R>(*j.g, j.x, bp, cls);
         }

@@ -24,13 +24,13 @@ import numpy as np
 
 from . import _capi
 from ._capi import (BALL, BOX, CROSS, MESH, PAIRED, PERSISTENT, PLAIN, RNG_MT_KEY, RNG_MT_PAIRED,
-                    RNG_MT_SITE, RNG_PHILOX, SQUARE, FLAG_COUNT_TRAVERSAL)
+                    RNG_MT_SITE, RNG_PHILOX, SQUARE, FLAG_COUNT_TRAVERSAL, FLAG_DISTANCE_GRID)
 
 __all__ = [
     "Context", "Mesh", "Side", "Result", "Server", "TwosError", "InvalidArgument", "StepCapError",
     "GeometryError", "AdmissionError", "DeviceError", "NotFoundError", "side", "reference_affine_side",
     "BALL", "BOX", "MESH", "SQUARE", "PAIRED", "CROSS", "RNG_PHILOX", "RNG_MT_SITE", "RNG_MT_PAIRED",
-    "RNG_MT_KEY", "PLAIN", "PERSISTENT", "FLAG_COUNT_TRAVERSAL", "version",
+    "RNG_MT_KEY", "PLAIN", "PERSISTENT", "FLAG_COUNT_TRAVERSAL", "FLAG_DISTANCE_GRID", "version",
 ]
 
 
@@ -312,9 +312,13 @@ class Context:
     def estimate(self, geom: int, sites, sides, *, combine=SQUARE, rng=RNG_PHILOX, precision=32,
                  schedule=PERSISTENT, seed=0, repeat=0, walks=16, walks_max=0, ci_rel=0.02, ci_z=1.96,
                  eps=5e-4, max_steps=512, use_graph=False, block=0, waves_per_simd=0, site_offset=0,
-                 count_traversal=False, dump: bool = False, raise_on_cap: bool = True) -> Result:
+                 count_traversal=False, distance_grid=False, dump: bool = False,
+                 raise_on_cap: bool = True) -> Result:
         """Per-site TWoS estimate.  walks_max > walks doubles each site's walks
-        until ci_z * score_se <= ci_rel * |score| or walks_max is reached."""
+        until ci_z * score_se <= ci_rel * |score| or walks_max is reached.
+        distance_grid: mesh steps far from the boundary take a conservative
+        radius from the mesh's distance grid (unbiased; paths differ from the
+        exact walk)."""
         p = _f64(sites).reshape(-1, 3)
         n = len(p)
         sd = (Side * 2)(sides[0], sides[1] if len(sides) > 1 else sides[0])
@@ -322,7 +326,8 @@ class Context:
                       repeat=repeat, max_steps=max_steps, eps=eps, walks=walks, walks_max=walks_max,
                       ci_rel=ci_rel, ci_z=ci_z, use_graph=use_graph, block=block,
                       waves_per_simd=waves_per_simd, site_offset=site_offset,
-                      flags=FLAG_COUNT_TRAVERSAL if count_traversal else 0)
+                      flags=(FLAG_COUNT_TRAVERSAL if count_traversal else 0)
+                      | (FLAG_DISTANCE_GRID if distance_grid else 0))
         out = (_capi.SiteResult * max(n, 1))()
         st = _capi.Stats()
         if dump:
@@ -373,13 +378,14 @@ class Server:
 
     def submit(self, geom: int, sites, sides, *, combine=CROSS, precision=32, seed=0, walks=256,
                walks_max=0, ci_rel=0.02, ci_z=1.96, eps=1e-4, max_steps=4096, use_graph=False,
-               waves_per_simd=0, site_offset=0) -> int:
+               waves_per_simd=0, site_offset=0, distance_grid=False) -> int:
         p = _f64(sites).reshape(-1, 3)
         n = len(p)
         sd = (Side * 2)(sides[0], sides[1] if len(sides) > 1 else sides[0])
         prm = _params(combine=combine, precision=precision, schedule=PERSISTENT, seed=seed, walks=walks,
                       walks_max=walks_max, ci_rel=ci_rel, ci_z=ci_z, eps=eps, max_steps=max_steps,
-                      use_graph=use_graph, waves_per_simd=waves_per_simd, site_offset=site_offset)
+                      use_graph=use_graph, waves_per_simd=waves_per_simd, site_offset=site_offset,
+                      flags=FLAG_DISTANCE_GRID if distance_grid else 0)
         out = (_capi.SiteResult * n)()
         rid = C.c_int64()
         _check(_lib().twos_server_submit(self._h, geom, sd, C.byref(prm), _ptr(p, _capi._dp), n, out,

@@ -26,6 +26,7 @@ SQUARE, PAIRED, CROSS = 0, 1, 2
 RNG_PHILOX, RNG_MT_SITE, RNG_MT_PAIRED, RNG_MT_KEY = 0, 1, 2, 3
 PLAIN, PERSISTENT = 0, 1
 FLAG_COUNT_TRAVERSAL = 1
+FLAG_DISTANCE_GRID = 2
 
 _dp = C.POINTER(C.c_double)
 _i32p = C.POINTER(C.c_int32)
