@@ -12,7 +12,7 @@ hipGraph).  The metric is TD sites/s: sites finished at the CI target per
 second of wall time.  Beside it: walk-steps/s, the plain one-launch-per-site
 GPU schedule on a seeded 512-site sample, the walk kernel against the HBM
 roofline, and the CPU oracle (same streams, OpenMP over all host cores) on a
-seeded 24-site sample.
+seeded 192-site sample.
 
     python bench.py [--gpus 1] [--steps 2] [--warmup 1] [--workload sh_icosphere|cooling_block|unit_ball]
 """
@@ -46,7 +46,7 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--workload", default="sh_icosphere",
                     choices=["sh_icosphere", "cooling_block", "unit_ball"])
     ap.add_argument("--plain-sites", type=int, default=512)
-    ap.add_argument("--cpu-sites", type=int, default=24)
+    ap.add_argument("--cpu-sites", type=int, default=192)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
@@ -142,8 +142,10 @@ def main() -> int:
         n_tris = cnt.stats["tris_tested"] / q_sub
         b_query = NODE_BYTES * n_nodes + TRI_BYTES * n_tris
         achieved = queries * b_query / (eager_ms * 1e-3) / 1e9
+        traffic = measured_traffic(w.name, n, not args.exact, launches)
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "algorithmic_bytes_per_launch": round(queries * b_query / launches),
                     "bytes_per_query": round(b_query, 1), "nodes_per_query": round(n_nodes, 2),
                     "tris_per_query": round(n_tris, 2), "walk_kernel_ms_per_launch": round(eager_ms / launches, 3),
                     "launches": launches}
@@ -199,6 +201,24 @@ def main() -> int:
     ctx.close()
     print(json.dumps(out))
     return 0
+
+
+def measured_traffic(workload: str, n_sites: int, grid: bool, launches: int):
+    """HBM bytes per walk-kernel launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE, each scaled by a copy kernel of known
+    size).  The estimate is deterministic, so the same workload repeats the
+    same launches; other configurations report null."""
+    path = ROOT / "profiles" / "walk_persistent_sh_icosphere_16384.md"
+    if workload != "sh_icosphere_81920" or n_sites != 16384 or not grid or not path.exists():
+        return None
+    fields = {}
+    for line in path.read_text().splitlines():
+        if ":" in line and line.split(":")[0] in ("traffic_bytes_per_launch", "traffic_launches"):
+            key, val = line.split(":", 1)
+            fields[key] = float(val)
+    if fields.get("traffic_launches") != launches:
+        return None
+    return int(fields["traffic_bytes_per_launch"])
 
 
 def cpu_baseline(w, sites, count: int) -> dict:
