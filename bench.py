@@ -44,7 +44,9 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="sh_icosphere",
-                    choices=["sh_icosphere", "cooling_block", "unit_ball"])
+                    choices=["sh_icosphere", "cooling_block", "unit_ball", "serve"])
+    ap.add_argument("--requests", type=int, default=300, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=6.0, help="serve: Poisson arrivals per second")
     ap.add_argument("--plain-sites", type=int, default=512)
     ap.add_argument("--cpu-sites", type=int, default=192)
     ap.add_argument("--no-cpu", action="store_true")
@@ -78,6 +80,8 @@ def main() -> int:
     import twos
     from twos import workloads as W
 
+    if args.workload == "serve":
+        return serve(args)
     w = build_workload(args.workload)
     sites = w.sites if args.sites <= 0 else w.sites[:args.sites]
     n = len(sites)
@@ -198,6 +202,36 @@ def main() -> int:
         out["x_plain"] = round(value / plain_rate, 2) if plain_rate > 0 else None
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, sites, args.cpu_sites)
+    ctx.close()
+    print(json.dumps(out))
+    return 0
+
+
+def serve(args) -> int:
+    """Configuration 4: a Poisson request stream over the three resident
+    geometries, continuous batching, p50/p99 latency."""
+    import twos
+    from twos import serving
+
+    ctx = twos.Context()
+    stream = serving.Stream(ctx, twos, n_requests=args.requests, rate=args.rate)
+    # Slots and one round's walk buffer sized to a slice of HBM (admission by memory).
+    srv = ctx.server(max_sites=65536, max_walks_per_round=1 << 26)
+    log(f"serving {args.requests} requests at {args.rate}/s")
+    res = serving.run(stream, srv, log=log)
+    out = {
+        "metric": "serving p50/p99 latency (TD queries + insertion validations)",
+        "value": round(res["p50_ms"], 2), "unit": "ms", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(res["wall_s"] * 1e3, 1), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 walks, f64 combine", "data": "synthetic",
+        "config": {"workload": f"serve: {args.requests} Poisson requests at {args.rate}/s over unit ball, cooling "
+                               "block and SH icosphere; sites log-uniform 1-8192, CI U[1%,5%], 20% validations",
+                   "max_sites": 65536, "max_walks_per_round": 1 << 26},
+        "p50_ms": round(res["p50_ms"], 2), "p99_ms": round(res["p99_ms"], 2), "mean_ms": round(res["mean_ms"], 2),
+        "td_sites_per_s": round(res["td_sites_per_s"], 2), "walk_steps_per_s": round(res["walk_steps_per_s"], 1),
+        "rounds": res["rounds"], "validations": res["validations"], "td_sites": res["td_sites"],
+    }
+    srv.close()
     ctx.close()
     print(json.dumps(out))
     return 0
