@@ -78,7 +78,9 @@ def test_admission_waits_for_slots(env):
 
 def test_insertion_validation_sees_the_cavity(env):
     ctx, ico, g_mesh, ball, g_ball = env
-    z = ico.sites[0]
+    # A site deep enough that every probe (within 2.5 r) starts inside the mesh.
+    d, _, _ = ctx.closest(g_mesh, ico.sites, precision=64)
+    z = ico.sites[int(np.argmax(d > 0.12))]
     r = 0.03
     g2 = ctx.union_sphere(g_mesh, z, r)
     rng = np.random.default_rng(0)

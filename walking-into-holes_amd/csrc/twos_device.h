@@ -299,6 +299,8 @@ __device__ __forceinline__ R mesh_closest(const MeshView &m, V3<R> p, V3<R> &bes
       best_k = k;
     }
   };
+  // A non-finite point would pass every box test (fmax drops NaN): no query.
+  if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z))) return best;
   if (hint >= 0 && hint < m.n_tris) test(hint);
   if (m.n_tris == 1) {
     if (best_k < 0) test(0);

@@ -105,6 +105,10 @@ __device__ __forceinline__ int job_step(Job<R> &j, const WalkParams &wp, Travers
   int cls;
   const R r = closest_boundary<R, GK, kCount>(*j.g, j.x, bp, cls, j.hint, cnt, j.tau);
   if (radius_out) *radius_out = r;
+  // A non-finite distance (a path that left a closed mesh and overflowed)
+  // ends the path as capped; the reference treats it as a failed query
+  // (wih_twos.cpp:355-357).
+  if (!isfinite(r)) return 2;
   const DevSide &s0 = *j.s0;
   if (!(r > j.eps)) {
     j.acc0 += side_payoff<R>(s0, bp, cls);
@@ -190,6 +194,7 @@ constexpr int kWalkBlock = 256;
 // cls) to the path: 0 moved on, 1 ended in the eps-shell, 2 capped.
 template <typename R>
 __device__ __forceinline__ int job_advance(Job<R> &j, const WalkParams &wp, R r, V3<R> bp, int cls) {
+  if (!isfinite(r)) return 2;  // as in job_step: a failed query ends the path capped
   const DevSide &s0 = *j.s0;
   if (!(r > j.eps)) {
     j.acc0 += side_payoff<R>(s0, bp, cls);
@@ -213,6 +218,11 @@ __device__ __forceinline__ bool is_mesh(const Job<R> &j) {
 template <typename R, int GK, bool kCount>
 __device__ __forceinline__ bool query_start(Job<R> &j, MeshQuery<R> &q, TraversalCount *cnt) {
   if (GK == kGeomAnalytic || !is_mesh<R, GK>(j)) return false;
+  if (!(isfinite(j.x.x) && isfinite(j.x.y) && isfinite(j.x.z))) {
+    q.grid = false;
+    q.best = Inf<R>::v();  // no query: the path ends capped
+    return false;
+  }
   if (j.tau < Inf<R>::v()) {
     const R lb = grid_lower<R>(*j.g, j.x);
     if (lb >= j.tau) {
