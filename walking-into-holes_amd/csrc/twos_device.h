@@ -155,6 +155,13 @@ __device__ __forceinline__ uint64_t mt_walk_seed(int rng, uint64_t seed, int32_t
   return splitmix64(seed + uint64_t(site));
 }
 
+// a / b.  binary32 walks take the hardware reciprocal (1 ulp) instead of the
+// correctly rounded IEEE sequence; binary64 divides exactly.
+template <typename R>
+__device__ __forceinline__ R ratio(R a, R b) { return a / b; }
+template <>
+__device__ __forceinline__ float ratio<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 // ---------------------------------------------------------------------------
 // Closest point on a triangle: Ericson, Real-Time Collision Detection 5.1.5
 // (the reference's closest_point_triangle, wih_twos.cpp:247-289).
@@ -172,7 +179,7 @@ __device__ __forceinline__ V3<R> closest_point_triangle(V3<R> p, V3<R> a, V3<R> 
   if (d3 >= R(0) && d4 <= d3) return b;
   const R vc = d1 * d4 - d3 * d2;
   if (vc <= R(0) && d1 >= R(0) && d3 <= R(0)) {
-    const R v = d1 / (d1 - d3);
+    const R v = ratio(d1, d1 - d3);
     return a + ab * v;
   }
   const V3<R> cp = p - c;
@@ -181,16 +188,16 @@ __device__ __forceinline__ V3<R> closest_point_triangle(V3<R> p, V3<R> a, V3<R> 
   if (d6 >= R(0) && d5 <= d6) return c;
   const R vb = d5 * d2 - d1 * d6;
   if (vb <= R(0) && d2 >= R(0) && d6 <= R(0)) {
-    const R w = d2 / (d2 - d6);
+    const R w = ratio(d2, d2 - d6);
     return a + ac * w;
   }
   const R va = d3 * d6 - d5 * d4;
   if (va <= R(0) && (d4 - d3) >= R(0) && (d5 - d6) >= R(0)) {
     const V3<R> bc = c - b;
-    const R w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    const R w = ratio(d4 - d3, (d4 - d3) + (d5 - d6));
     return b + bc * w;
   }
-  const R denom = R(1) / (va + vb + vc);
+  const R denom = ratio(R(1), va + vb + vc);
   const R v = vb * denom;
   const R w = vc * denom;
   return a + ab * v + ac * w;
@@ -493,10 +500,12 @@ __device__ __forceinline__ R mesh_distance(const DevGeom &g, R d2) {
 // d(p) >= d(c) - |p - c|; a small margin covers binary32 rounding of d(c).
 template <typename R>
 __device__ __forceinline__ R grid_lower(const DevGeom &g, V3<R> p) {
-  const R h = R(g.grid_h);
-  int i = int(floor((p.x - R(g.grid_lo[0])) / h));
-  int k = int(floor((p.y - R(g.grid_lo[1])) / h));
-  int l = int(floor((p.z - R(g.grid_lo[2])) / h));
+  // Any cell centre gives a valid bound, so the cell index may round either
+  // way at a cell face: multiply by 1/h instead of dividing.
+  const R h = R(g.grid_h), inv_h = R(g.grid_inv_h);
+  int i = int(floor((p.x - R(g.grid_lo[0])) * inv_h));
+  int k = int(floor((p.y - R(g.grid_lo[1])) * inv_h));
+  int l = int(floor((p.z - R(g.grid_lo[2])) * inv_h));
   i = min(max(i, 0), g.grid_n[0] - 1);
   k = min(max(k, 0), g.grid_n[1] - 1);
   l = min(max(l, 0), g.grid_n[2] - 1);
@@ -552,6 +561,10 @@ struct MeshQuery {
   int32_t sp;
   bool grid;
 };
+
+// Largest 4-wide node index an entry holds.  A tree over n triangles has at
+// most n - 1 internal nodes, so meshes of up to 2^21 triangles fit.
+constexpr int32_t kMaxStackIndex = (1 << 21) - 1;
 
 __device__ __forceinline__ uint32_t stack_pack(int32_t node, float lower) {
   return (uint32_t(node) << 11) | ((__float_as_uint(lower) >> 20) & 0x7FFu);

@@ -802,6 +802,7 @@ twos_status build_geom(twos_ctx *ctx, GeomEntry &g) {
       v.grid_n[k] = mb.grid_n[k];
     }
     v.grid_h = mb.grid_h;
+    v.grid_inv_h = mb.grid_h > 0.f ? 1.0f / mb.grid_h : 0.f;
     v.n_tris = mb.n_tris;
     it->second.users += 1;
   }
@@ -948,7 +949,9 @@ twos_status twos_mesh_create(twos_ctx *ctx, const double *vertices, int64_t nv, 
                              int32_t *mesh_id, double *build_ms) {
   if (!ctx || !vertices || !faces || !mesh_id) return fail(TWOS_ERR_INVALID_ARGUMENT, "null argument");
   if (nv <= 0 || nf <= 0) return fail(TWOS_ERR_INVALID_ARGUMENT, "expected a nonempty triangle mesh");
-  if (nf >= (int64_t(1) << 30)) return fail(TWOS_ERR_INVALID_ARGUMENT, "too many triangles");
+  // Traversal stack entries hold 21-bit 4-wide node indices.
+  if (nf > int64_t(kMaxStackIndex) + 1)
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "meshes are limited to 2^21 triangles");
   for (int64_t i = 0; i < 3 * nf; ++i)
     if (faces[i] >= uint64_t(nv)) return fail(TWOS_ERR_INVALID_ARGUMENT, "expected valid triangular faces");
   for (int64_t i = 0; i < 3 * nv; ++i)

@@ -39,7 +39,7 @@ struct DevGeom {
   float grid_lo[3];
   float grid_h;
   int32_t grid_n[3];
-  int32_t grid_pad;
+  float grid_inv_h;
 };
 
 // Per-site-slot state resident on the device (capacity = server max_sites).
