@@ -78,20 +78,38 @@ extern "C" {
 typedef struct twos_ctx twos_ctx;
 typedef int32_t twos_status;
 
+/* Source term types (source_kind 2), rows {type, cx, cy, cz, p0, p1, p2, w},
+ * t = max(0, 1 - |y - c|^2 / (p0 p0)):
+ *   TWOS_TERM_BUMP   w * (t t)             heat bump (benchmarks/submission_study/study.cpp:29-36)
+ *   TWOS_TERM_UNIT   (t t) / p1            unit-mass sensor bump, p1 = 32 pi p0^3 / 105 (study.cpp:44-51)
+ *   TWOS_TERM_GAUSS  w * exp(-((x-cx)/p0)^2 - ((y-cy)/p1)^2 - ((z-cz)/p2)^2)
+ *                                          housing chips (benchmarks/compiled_bunny/src/wih_twos.cpp:553-566) */
+#define TWOS_TERM_BUMP 0
+#define TWOS_TERM_UNIT 1
+#define TWOS_TERM_GAUSS 2
+#define TWOS_MAX_TERMS 8
+
 /* Boundary data and source of one side: the Dirichlet payoff at the closest
  * boundary point y of class c (0-5 container faces; ball and mesh use 0,
  * box faces are -x,+x,-y,+y,-z,+z; 6 analytic holes; 7 cavities) is
  *   cls_const[c] + (bit c of cls_poly_mask) * poly(y),
  * poly(y) = c0 + cx x + cy y + cz z + cxx x^2 + cyy y^2 + czz z^2 + cxy xy + cxz xz + cyz yz;
  * source_kind 1 adds (r^2/6) f(x) per step with the affine
- * f = ((s0 + sx (x+ox)) + sy (y+oy)) + sz (z+oz) (the reference's
- * 1 + slope (y + 0.5) is s0 = 1, sy = slope, oy = 0.5; wih_twos.cpp:546-552). */
+ * f = ((s0 + sx (x+ox)) + sy (y+oy)) + sz (z+oz) at the ball centre (the
+ * reference's 1 + slope (y + 0.5) is s0 = 1, sy = slope, oy = 0.5;
+ * wih_twos.cpp:546-552); source_kind 2 adds (r^2/6) f(a) with
+ * f = s0 + sum of the n_terms rows, at a Green-distributed point a of the
+ * ball (a = x + r t omega, t ~ Beta(2,2); wih_twos.cpp:648-653,
+ * study.cpp:106).  Both sides of a PAIRED path share one point a. */
 typedef struct {
   double poly[10];
   double cls_const[8];
   uint32_t cls_poly_mask;
   int32_t source_kind;
   double src[7];
+  int32_t n_terms;
+  int32_t pad;
+  double terms[TWOS_MAX_TERMS][8];
 } twos_side;
 
 typedef struct {

@@ -35,9 +35,14 @@ class Geom(C.Structure):
                 ("cylinders", _dp), ("cavities", _dp), ("mesh", C.c_void_p)]
 
 
+MAX_TERMS = 8
+TERM_BUMP, TERM_UNIT, TERM_GAUSS = 0, 1, 2
+
+
 class Side(C.Structure):
     _fields_ = [("poly", C.c_double * 10), ("cls_const", C.c_double * 8),
-                ("cls_poly_mask", C.c_uint32), ("source_kind", C.c_int32), ("src", C.c_double * 7)]
+                ("cls_poly_mask", C.c_uint32), ("source_kind", C.c_int32), ("src", C.c_double * 7),
+                ("n_terms", C.c_int32), ("pad", C.c_int32), ("terms", (C.c_double * 8) * MAX_TERMS)]
 
 
 class Params(C.Structure):
@@ -201,10 +206,23 @@ class Geometry:
         return g
 
 
-def side(poly=None, cls_const=None, poly_classes=(), source=None) -> Side:
+def side(poly=None, cls_const=None, poly_classes=(), source=None, terms=None, offset=0.0) -> Side:
     """Boundary data g = cls_const[class] + [class in poly_classes] * poly(y);
-    source = (s0, sx, sy, sz, ox, oy, oz) adds (r^2/6) f(x) per step."""
+    source = (s0, sx, sy, sz, ox, oy, oz) adds (r^2/6) f(x) per step at the ball
+    centre; terms = rows (type, cx, cy, cz, p0, p1, p2, w) add (r^2/6) f(a) with
+    f = offset + sum of terms at a Green-distributed point a."""
     s = Side()
+    if terms is not None:
+        if source is not None:
+            raise ValueError("a side has either an affine source or sampled terms")
+        terms = [list(map(float, t)) for t in terms]
+        if not 0 < len(terms) <= MAX_TERMS or any(len(t) != 8 for t in terms):
+            raise ValueError(f"expected 1..{MAX_TERMS} term rows of 8 numbers")
+        s.source_kind = 2
+        s.src[0] = offset
+        s.n_terms = len(terms)
+        for k, t in enumerate(terms):
+            s.terms[k] = (C.c_double * 8)(*t)
     s.poly = (C.c_double * 10)(*(list(poly) if poly is not None else [0.0] * 10))
     s.cls_const = (C.c_double * 8)(*(list(cls_const) if cls_const is not None else [0.0] * 8))
     mask = 0
@@ -220,6 +238,29 @@ def side(poly=None, cls_const=None, poly_classes=(), source=None) -> Side:
 def reference_affine_side(slope: float = 0.65) -> Side:
     """f = 1 + slope * (y + 0.5), zero terminal value (wih_twos.cpp:546-552)."""
     return side(source=(1.0, 0.0, slope, 0.0, 0.0, 0.5, 0.0))
+
+
+def study_terms(config_text: str):
+    """Heat and sensor source rows of a study configuration: first line the
+    sensor (cx cy cz R), then heat bumps (cx cy cz R A) (study.cpp:37-43).
+    The sensor's mass 32 pi R^3 / 105 is computed as study.cpp:50 does."""
+    vals = [float(v) for v in config_text.split()]
+    sensor, rest = vals[:4], vals[4:]
+    heat = [rest[i:i + 5] for i in range(0, len(rest) - len(rest) % 5, 5)]
+    if len(sensor) < 4 or not heat:
+        raise ValueError("expected a sensor line and at least one heat bump")
+    heat_rows = [(TERM_BUMP, h[0], h[1], h[2], h[3], 0.0, 0.0, h[4]) for h in heat]
+    mass = 32 * 3.141592653589793 * float(np.power(sensor[3], 3.0)) / 105
+    sensor_rows = [(TERM_UNIT, sensor[0], sensor[1], sensor[2], sensor[3], mass, 0.0, 0.0)]
+    return heat_rows, sensor_rows
+
+
+def housing_chip_terms():
+    """The built-in housing source 0.08 + 5.2 chip_a + 4.6 chip_b + 2.0 regulator
+    (wih_twos.cpp:553-566): (offset, rows)."""
+    return 0.08, [(TERM_GAUSS, -0.48, -0.18, -0.08, 0.24, 0.22, 0.16, 5.2),
+                  (TERM_GAUSS, 0.43, 0.19, -0.07, 0.21, 0.20, 0.15, 4.6),
+                  (TERM_GAUSS, 0.02, -0.31, -0.02, 0.30, 0.16, 0.19, 2.0)]
 
 
 def estimate(geometry: Geometry, sites, sides, *, combine=SQUARE, rng=RNG_PHILOX, seed=0,

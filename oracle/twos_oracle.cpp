@@ -82,8 +82,10 @@ constexpr double kTwoPi = 6.283185307179586;  // == 2 * acos(-1.0) in binary64
 // Direction for draw block `step` of walk `walk` under `key`: one Philox block
 // per step, u1 from words (0,1), u2 from words (2,3).  The same z / phi / radial
 // construction as random_unit_vector (wih_twos.cpp:536-542).
-V3 philox_direction(const uint32_t key[2], uint32_t walk, uint32_t step) {
-  const uint32_t ctr[4] = {step, walk, 0u, 0u};
+// Counter word 2 selects the draw within a step: 0 the move direction, 1 and
+// 2 the Green-ball point of a sampled source.
+V3 philox_direction(const uint32_t key[2], uint32_t walk, uint32_t step, uint32_t lane = 0u) {
+  const uint32_t ctr[4] = {step, walk, lane, 0u};
   uint32_t w[4];
   philox4x32_10(ctr, key, w);
   const double z = 1.0 - 2.0 * u53(w[0], w[1]);
@@ -100,6 +102,54 @@ V3 random_unit_vector(std::mt19937_64 &rng) {
   const double radial = std::sqrt(std::max(0.0, 1.0 - z * z));
   return {radial * std::cos(phi), radial * std::sin(phi), z};
 }
+
+// beta_22: Beta(2,2) by rejection from U(0,1) pairs, wih_twos.cpp:569-575.
+double beta_22(std::mt19937_64 &rng) {
+  std::uniform_real_distribution<double> uniform(0.0, 1.0);
+  while (true) {
+    const double value = uniform(rng);
+    if (uniform(rng) <= 4.0 * value * (1.0 - value)) return value;
+  }
+}
+
+// Beta(2,2) for the Philox streams: the median of three uniforms is Beta(2,2)
+// (order statistic 2 of 3).  Uniforms carry 24 bits so the binary32 and
+// binary64 walks draw the same value: words 0..2 of block (step, walk, 2, 0).
+double philox_beta22(const uint32_t key[2], uint32_t walk, uint32_t step) {
+  const uint32_t ctr[4] = {step, walk, 2u, 0u};
+  uint32_t w[4];
+  philox4x32_10(ctr, key, w);
+  const double a = double(w[0] >> 8) * 0x1.0p-24, b = double(w[1] >> 8) * 0x1.0p-24,
+               c = double(w[2] >> 8) * 0x1.0p-24;
+  return std::max(std::min(a, b), std::min(std::max(a, b), c));
+}
+
+// Per-path random draws.  dir(step): the move direction; green(step, r): the
+// offset r * t * omega of a Green-distributed point in the ball of radius r
+// (t ~ Beta(2,2), omega uniform on the sphere), the source point of
+// wih_twos.cpp:648-653 and study.cpp:106.
+struct PhiloxStream {
+  uint32_t key[2];
+  uint32_t walk;
+  V3 dir(int32_t step) const { return philox_direction(key, walk, uint32_t(step)); }
+  V3 green(int32_t step, double radius) const {
+    const V3 d = philox_direction(key, walk, uint32_t(step), 1u);
+    return d * (radius * philox_beta22(key, walk, uint32_t(step)));
+  }
+};
+
+// The reference draws the Green point as random_unit_vector(rng) *
+// (radius * beta_22(rng)); g++ evaluates the right operand, the Beta draw,
+// first.
+struct MtStream {
+  std::mt19937_64 rng;
+  V3 dir(int32_t) { return random_unit_vector(rng); }
+  V3 green(int32_t, double radius) {
+    const double t = beta_22(rng);
+    const V3 d = random_unit_vector(rng);
+    return d * (radius * t);
+  }
+};
 
 // Ericson's closest point on a triangle: wih_twos.cpp:247-289.
 V3 closest_point_triangle(const V3 &p, const V3 &a, const V3 &b, const V3 &c) {
@@ -446,6 +496,39 @@ double source(const og_side &s, const V3 &x) {
   return ((c[0] + c[1] * (x.x + c[4])) + c[2] * (x.y + c[5])) + c[3] * (x.z + c[6]);
 }
 
+// Sampled source of source_kind 2: f(y) = s0 + sum of the side's terms, summed
+// in order.  The heat bumps of study.cpp:29-36 (s0 = 0, BUMP rows), its
+// normalised sensor bump (:44-51, one UNIT row with p1 = 32 pi R^3 / 105) and
+// the housing chips of wih_twos.cpp:553-566 (s0 = 0.08, GAUSS rows) evaluate
+// to the reference's binary64 values.
+double field(const og_side &s, const V3 &y) {
+  double f = s.src[0];
+  for (int32_t k = 0; k < s.n_terms && k < OG_MAX_TERMS; ++k) {
+    const double *t = s.terms[k];
+    const int type = int(t[0]);
+    if (type == OG_TERM_GAUSS) {
+      const double a = (y.x - t[1]) / t[4], b = (y.y - t[2]) / t[5], c = (y.z - t[3]) / t[6];
+      f += t[7] * std::exp(-(a * a) - b * b - c * c);
+    } else {
+      const V3 d = y - V3{t[1], t[2], t[3]};
+      const double q = std::max(0.0, 1.0 - dot(d, d) / (t[4] * t[4]));
+      if (type == OG_TERM_UNIT) f += q * q / t[5];
+      else f += t[7] * (q * q);
+    }
+  }
+  return f;
+}
+
+// Green-ball source term of one step for one side: (r^2/6) f.  `paired_form`
+// follows paired_site's association r*r/6*f (study.cpp:106-108), otherwise
+// walk()'s f*r*r/6 (wih_twos.cpp:648-653).
+double green_term(const og_side &s, const V3 &x, const V3 &a, double radius, bool paired_form) {
+  if (s.source_kind == 1) return source(s, x) * radius * radius / 6.0;
+  if (s.source_kind == 2)
+    return paired_form ? radius * radius / 6.0 * field(s, a) : field(s, a) * radius * radius / 6.0;
+  return 0.0;
+}
+
 struct WalkOut {
   double a = 0.0, b = 0.0;
   int32_t steps = 0;
@@ -460,10 +543,11 @@ struct WalkOut {
 // is zero; its 2-D harmonic-data experiment projects to the boundary the same
 // way, experiments/run_controlled_experiments.py:115-147).  `two` accumulates
 // a second side's source and payoff along the same path (study.cpp:91-113).
-template <class Dir>
+template <class Stream>
 WalkOut walk(const og_geom &g, const og_side &s0, const og_side *s1, V3 x, double eps,
-             int32_t max_steps, Dir &&next_direction, double *trace, int32_t max_rows,
+             int32_t max_steps, Stream &stream, bool paired_form, double *trace, int32_t max_rows,
              int32_t *rows) {
+  const bool sampled = s0.source_kind == 2 || (s1 && s1->source_kind == 2);
   WalkOut out;
   int32_t row = 0;
   for (int32_t step = 0; step < max_steps; ++step) {
@@ -481,9 +565,11 @@ WalkOut walk(const og_geom &g, const og_side &s0, const og_side *s1, V3 x, doubl
       if (rows) *rows = row;
       return out;
     }
-    if (s0.source_kind == 1) out.a += source(s0, x) * radius * radius / 6.0;
-    if (s1 && s1->source_kind == 1) out.b += source(*s1, x) * radius * radius / 6.0;
-    x = x + next_direction(step) * radius;
+    // Both sides of a paired path share one Green point (study.cpp:104-106).
+    const V3 a = sampled ? x + stream.green(step, radius) : x;
+    if (s0.source_kind) out.a += green_term(s0, x, a, radius, paired_form);
+    if (s1 && s1->source_kind) out.b += green_term(*s1, x, a, radius, paired_form);
+    x = x + stream.dir(step) * radius;
     ++out.steps;
   }
   if (trace && row < max_rows) {
@@ -511,38 +597,37 @@ uint64_t site_key(uint64_t seed, int32_t repeat, int64_t site, int64_t sample) {
 WalkOut sample(const og_geom &g, const og_side sides[2], const og_params &p, const V3 &z,
                int64_t s, int64_t w) {
   if (p.rng == OG_RNG_PHILOX) {
-    uint32_t k0[2], k1[2];
-    philox_key(p.seed, s, 0, k0);
-    auto dir0 = [&](int32_t step) { return philox_direction(k0, uint32_t(w), uint32_t(step)); };
+    PhiloxStream st0{{0u, 0u}, uint32_t(w)};
+    philox_key(p.seed, s, 0, st0.key);
     if (p.combine == OG_SQUARE) {
-      WalkOut o = walk(g, sides[0], nullptr, z, p.eps, p.max_steps, dir0, nullptr, 0, nullptr);
+      WalkOut o = walk(g, sides[0], nullptr, z, p.eps, p.max_steps, st0, false, nullptr, 0, nullptr);
       o.b = o.a;
       return o;
     }
     if (p.combine == OG_PAIRED)
-      return walk(g, sides[0], &sides[1], z, p.eps, p.max_steps, dir0, nullptr, 0, nullptr);
-    philox_key(p.seed, s, 1, k1);
-    auto dir1 = [&](int32_t step) { return philox_direction(k1, uint32_t(w), uint32_t(step)); };
-    WalkOut a = walk(g, sides[0], nullptr, z, p.eps, p.max_steps, dir0, nullptr, 0, nullptr);
-    WalkOut b = walk(g, sides[1], nullptr, z, p.eps, p.max_steps, dir1, nullptr, 0, nullptr);
+      return walk(g, sides[0], &sides[1], z, p.eps, p.max_steps, st0, false, nullptr, 0, nullptr);
+    PhiloxStream st1{{0u, 0u}, uint32_t(w)};
+    philox_key(p.seed, s, 1, st1.key);
+    WalkOut a = walk(g, sides[0], nullptr, z, p.eps, p.max_steps, st0, false, nullptr, 0, nullptr);
+    WalkOut b = walk(g, sides[1], nullptr, z, p.eps, p.max_steps, st1, false, nullptr, 0, nullptr);
     a.b = b.a;
     a.steps += b.steps;
     a.cap = a.cap || b.cap;
     return a;
   }
-  std::mt19937_64 rng;
+  MtStream st;
   if (p.rng == OG_RNG_MT_SITE) {
     // walk(): std::mt19937_64 rng(splitmix64(seed)) with evaluate_site's key.
-    rng.seed(splitmix64(site_key(p.seed, p.repeat, s, w)));
+    st.rng.seed(splitmix64(site_key(p.seed, p.repeat, s, w)));
   } else {
     // paired_site: rng(splitmix64(seed)^splitmix64(site+91347)^splitmix64(n+571)),
     // benchmarks/submission_study/study.cpp:89.
-    rng.seed(splitmix64(p.seed) ^ splitmix64(uint64_t(s) + 91347) ^ splitmix64(uint64_t(w) + 571));
+    st.rng.seed(splitmix64(p.seed) ^ splitmix64(uint64_t(s) + 91347) ^ splitmix64(uint64_t(w) + 571));
   }
-  auto dir = [&](int32_t) { return random_unit_vector(rng); };
+  const bool paired_form = p.rng == OG_RNG_MT_PAIRED;
   if (p.combine == OG_PAIRED)
-    return walk(g, sides[0], &sides[1], z, p.eps, p.max_steps, dir, nullptr, 0, nullptr);
-  WalkOut o = walk(g, sides[0], nullptr, z, p.eps, p.max_steps, dir, nullptr, 0, nullptr);
+    return walk(g, sides[0], &sides[1], z, p.eps, p.max_steps, st, paired_form, nullptr, 0, nullptr);
+  WalkOut o = walk(g, sides[0], nullptr, z, p.eps, p.max_steps, st, paired_form, nullptr, 0, nullptr);
   o.b = o.a;
   return o;
 }
@@ -763,18 +848,17 @@ int32_t og_trace(const og_geom *g, const og_side *side, const og_params *prm, co
   int32_t count = 0;
   const og_params &p = *prm;
   if (p.rng == OG_RNG_PHILOX) {
-    uint32_t k[2];
-    philox_key(p.seed, site, 0, k);
-    auto dir = [&](int32_t step) { return philox_direction(k, uint32_t(w), uint32_t(step)); };
-    walk(*g, *side, nullptr, load3(start), p.eps, p.max_steps, dir, rows, max_rows, &count);
+    PhiloxStream st{{0u, 0u}, uint32_t(w)};
+    philox_key(p.seed, site, 0, st.key);
+    walk(*g, *side, nullptr, load3(start), p.eps, p.max_steps, st, false, rows, max_rows, &count);
     return count;
   }
-  std::mt19937_64 rng;
-  if (p.rng == OG_RNG_MT_KEY) rng.seed(splitmix64(key));
-  else if (p.rng == OG_RNG_MT_SITE) rng.seed(splitmix64(site_key(p.seed, p.repeat, site, w)));
-  else rng.seed(splitmix64(p.seed) ^ splitmix64(uint64_t(site) + 91347) ^ splitmix64(uint64_t(w) + 571));
-  auto dir = [&](int32_t) { return random_unit_vector(rng); };
-  walk(*g, *side, nullptr, load3(start), p.eps, p.max_steps, dir, rows, max_rows, &count);
+  MtStream st;
+  if (p.rng == OG_RNG_MT_KEY) st.rng.seed(splitmix64(key));
+  else if (p.rng == OG_RNG_MT_SITE) st.rng.seed(splitmix64(site_key(p.seed, p.repeat, site, w)));
+  else st.rng.seed(splitmix64(p.seed) ^ splitmix64(uint64_t(site) + 91347) ^ splitmix64(uint64_t(w) + 571));
+  walk(*g, *side, nullptr, load3(start), p.eps, p.max_steps, st, p.rng == OG_RNG_MT_PAIRED, rows, max_rows,
+       &count);
   return count;
 }
 

@@ -56,13 +56,26 @@ typedef struct {
   const og_mesh *mesh;
 } og_geom;
 
+/* Volume-source term types for source_kind 2, rows {type, cx, cy, cz, p0, p1, p2, w}
+ * with t = max(0, 1 - |y - c|^2 / (p0 p0)):
+ *   OG_TERM_BUMP     w * (t t)                       (benchmarks/submission_study/study.cpp:29-36)
+ *   OG_TERM_UNIT     (t t) / p1, p1 the bump's mass   (study.cpp:44-51)
+ *   OG_TERM_GAUSS    w * exp(-((x-cx)/p0)^2 - ((y-cy)/p1)^2 - ((z-cz)/p2)^2)
+ *                                                     (benchmarks/compiled_bunny/src/wih_twos.cpp:553-566) */
+enum { OG_TERM_BUMP = 0, OG_TERM_UNIT = 1, OG_TERM_GAUSS = 2 };
+#define OG_MAX_TERMS 8
+
 /* Boundary data and volume source of one side of the problem. */
 typedef struct {
   double poly[10];      /* c0 + cx x + cy y + cz z + cxx x^2 + cyy y^2 + czz z^2 + cxy xy + cxz xz + cyz yz */
   double cls_const[8];  /* per boundary class constant */
   uint32_t cls_poly_mask; /* bit c set: class c adds poly(y) */
-  int32_t source_kind;  /* 0 none, 1 affine f = ((s0 + sx (x+ox)) + sy (y+oy)) + sz (z+oz) */
+  int32_t source_kind;  /* 0 none; 1 affine f = ((s0 + sx (x+ox)) + sy (y+oy)) + sz (z+oz) at the
+                           ball centre; 2 f = s0 + sum of terms at a Green-distributed point */
   double src[7];        /* s0, sx, sy, sz, ox, oy, oz */
+  int32_t n_terms;
+  int32_t pad;
+  double terms[OG_MAX_TERMS][8];
 } og_side;
 
 typedef struct {

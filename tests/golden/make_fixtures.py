@@ -1,9 +1,10 @@
 #!/usr/bin/env python3
 """Write the reference's own retained data as compact test fixtures.
 
-Reads data files (meshes, site lists, field outputs, walk traces, summaries)
-that the reference repository keeps under benchmarks/compiled_bunny/output/ and
-stores them as small .npz / .json fixtures beside this script.  Nothing here
+Reads data files (meshes, site lists, field outputs, walk traces, summaries,
+request argument lists) that the reference repository keeps under
+benchmarks/compiled_bunny/output/ and benchmarks/submission_study/ and stores
+them as small .npz / .json fixtures beside this script.  Nothing here
 imports or runs reference code: the files are parsed as text.
 
 Run once in a container where the reference tree is mounted:
@@ -114,5 +115,64 @@ def main() -> None:
     (HERE / "bunny_meta.json").write_text(json.dumps(meta, indent=1) + "\n")
 
 
+def study_arguments(request_json: Path) -> dict[str, str]:
+    """Flag -> value of a retained wih_study command line (request.json)."""
+    argv = json.loads(request_json.read_text())
+    return {argv[i]: argv[i + 1] for i in range(1, len(argv) - 1) if argv[i].startswith("--")}
+
+
+def study_fixtures(root: Path) -> None:
+    """Paired and hot-only requests of the submission study on the branch mesh
+    (benchmarks/submission_study/**/raw/*/: request.json, points.txt,
+    holes.txt, field.csv).  Source configuration: the pilot and the
+    '-v1-remote-sources' validation used the archived first version
+    (pilot/source-config-v1/branch.txt, pilot/README.txt); the later runs the
+    current assets/branch.txt, which adds a sensor self-heating bump."""
+    study = root / "benchmarks" / "submission_study"
+    v, f = read_ascii_ply(study / "assets" / "branch.ply")
+    np.savez_compressed(HERE / "study_branch_mesh.npz", vertices=v, faces=f)
+    def walk_requests(pattern: str) -> list[Path]:
+        keep = []
+        for rq in sorted(study.glob(pattern)):
+            a = study_arguments(rq)
+            if a["--mode"] == "twos" and a.get("--diff", "0") == "0" and a.get("--fd", "0") == "0":
+                keep.append(rq)
+        return keep
+
+    picks = walk_requests("pilot/channels/branch/twos-1611/raw/round*-*/request.json")
+    picks += walk_requests("pilot/pipeline/branch/twos-1611/raw/round*-rescore/request.json")
+    for pattern in ("pilot/pipeline/branch/evaluation/1611/raw/*/request.json",
+                    "validation-local-cavity/raw/*/request.json", "validation/raw/*/request.json",
+                    "validation-v1-remote-sources/raw/*/request.json"):
+        picks += walk_requests(pattern)[:2]
+    arrays, requests = {}, []
+    for k, rq in enumerate(picks):
+        d = rq.parent
+        a = study_arguments(rq)
+        assert a["--mesh"].endswith("/branch.ply") and a["--config"].endswith("/branch.txt")
+        rel = str(d.relative_to(study))
+        points = np.loadtxt(d / "points.txt", ndmin=2).reshape(-1, 3)
+        holes_text = (d / "holes.txt").read_text() if (d / "holes.txt").exists() else ""
+        holes = np.array([float(x) for x in holes_text.split()]).reshape(-1, 3)
+        with (d / "field.csv").open() as handle:
+            rows = list(csv.DictReader(handle))
+        assert [int(r["site"]) for r in rows] == list(range(len(points)))
+        arrays[f"{k}_points"] = points
+        arrays[f"{k}_holes"] = holes
+        arrays[f"{k}_values"] = np.array([[float(r[c]) for c in ("u", "p", "score", "same_index")] for r in rows])
+        arrays[f"{k}_counts"] = np.array([[int(r["steps"]), int(r["caps"])] for r in rows], dtype=np.int64)
+        requests.append({"source": rel, "seed": int(a["--seed"]), "walks": int(a["--walks"]),
+                         "epsilon": float(a["--epsilon"]), "max_steps": int(a["--max-steps"]),
+                         "cavity_radius": float(a["--cavity-radius"]), "hot": a["--hot-only"] != "0",
+                         "config": "v1" if rel.startswith("pilot/") or "-v1-" in rel else "current"})
+    np.savez_compressed(HERE / "study_branch_requests.npz", **arrays)
+    meta = {"mesh": "benchmarks/submission_study/assets/branch.ply",
+            "configs": {"v1": (study / "pilot" / "source-config-v1" / "branch.txt").read_text(),
+                        "current": (study / "assets" / "branch.txt").read_text()},
+            "requests": requests}
+    (HERE / "study_branch_meta.json").write_text(json.dumps(meta, indent=1) + "\n")
+
+
 if __name__ == "__main__":
     main()
+    study_fixtures(Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference"))

@@ -42,13 +42,30 @@ def test_binding_matches_header():
     assert set(_capi.SIGNATURES) == declared()
 
 
-def test_struct_layouts_match_header_sizes():
+def test_struct_layouts_match_header_sizes(tmp_path):
+    """The ctypes mirrors have the size and field offsets the C compiler gives
+    the header's structs."""
+    import subprocess
     from twos import _capi
-    # Sizes follow from the header's field lists on LP64.
-    assert ctypes.sizeof(_capi.Side) == 10 * 8 + 8 * 8 + 4 + 4 + 7 * 8
-    assert ctypes.sizeof(_capi.SiteResult) == 6 * 8 + 3 * 8 + 2 * 4
-    assert ctypes.sizeof(_capi.Params) == 4 * 4 + 8 + 2 * 4 + 8 + 2 * 4 + 2 * 8 + 6 * 4
-    assert ctypes.sizeof(_capi.GeometryDesc) == 6 * 4 + 4 * 8 + 3 * 8 + 3 * 8 + 8 + 3 * 8
+    structs = {"twos_side": _capi.Side, "twos_site_result": _capi.SiteResult, "twos_params": _capi.Params,
+               "twos_geometry_desc": _capi.GeometryDesc, "twos_stats": _capi.Stats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{ROOT / "include" / "twos.h"}"',
+             'int main(void) {']
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, *_ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ['  return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    for line in filter(None, out):
+        cname, what, value = line.split()
+        py = structs[cname]
+        want = ctypes.sizeof(py) if what == "size" else getattr(py, what).offset
+        assert int(value) == want, (cname, what)
 
 
 def test_version_and_no_device_error_without_gpu():

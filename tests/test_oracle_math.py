@@ -214,6 +214,33 @@ def test_unit_ball_green_mass():
     assert abs(r["score"][0] - 1 / 36) < 3 * r["score_se"][0] + 2e-4
 
 
+@pytest.mark.parametrize("rng", [O.RNG_PHILOX, O.RNG_MT_SITE])
+def test_green_point_moments(rng):
+    """A walk from the centre of the unit ball with eps = 0.99 takes exactly one
+    step (r = 1, then it lands on the sphere), so its value is f(a) / 6 at one
+    Green point a = t omega with t ~ Beta(2,2): E|a|^2 = 3/10, E|a|^4 = 1/7,
+    E a = 0, E (a.c)^2 = |c|^2 / 10.  For the bump f = w (1 - |a - c|^2 / R^2)^2
+    with R large enough to cover the ball, E f = w (1 - 2 (3/10 + |c|^2) / R^2
+    + (1/7 + |c|^2 + |c|^4) / R^4).  A uniform t, a wrong radius scale or a
+    biased direction each miss this by far more than the sampling error."""
+    c = np.array([0.3, -0.2, 0.1])
+    w, R = 1.5, 2.0
+    bump = O.side(terms=[(O.TERM_BUMP, *c, R, 0.0, 0.0, w)])
+    n = 200000
+    r = O.estimate(_ball_geometry(), [[0.0, 0.0, 0.0]], [bump], combine=O.SQUARE, rng=rng, seed=4,
+                   walks=n, eps=0.99, max_steps=8, dump=True)
+    assert r["steps"][0] == n  # one step per walk
+    vals = r["a"][0]
+    c2 = float(c @ c)
+    expect = w * (1 - 2 * (0.3 + c2) / R**2 + (1 / 7 + c2 + c2**2) / R**4) / 6
+    se = vals.std() / math.sqrt(n)
+    assert abs(vals.mean() - expect) < 4 * se, (vals.mean(), expect, se)
+    # The uniform-t alternative (E|a|^2 = 1/3, E|a|^4 = 1/5, E (a.c)^2 = |c|^2 / 9)
+    # is excluded.
+    wrong = w * (1 - 2 * (1 / 3 + c2) / R**2 + (1 / 5 + 4 * c2 / 9 + 2 * c2 / 3 + c2**2) / R**4) / 6
+    assert abs(vals.mean() - wrong) > 10 * se
+
+
 def test_unit_ball_harmonic_td_c1():
     """Configuration 1: g_u = x^2 - y^2 + 0.5 z, g_v = x z + y are harmonic, so
     u = g_u, v = g_v and the Dirichlet-cavity TD is -g_u g_v

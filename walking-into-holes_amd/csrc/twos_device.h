@@ -127,6 +127,20 @@ struct Mt64 {
     if (u >= 1.0) u = 0x1.fffffffffffffp-1;
     return u;
   }
+  // beta_22: Beta(2,2) by rejection (wih_twos.cpp:569-575).
+  __device__ double beta22() {
+    while (true) {
+      const double value = uniform();
+      if (uniform() <= 4.0 * value * (1.0 - value)) return value;
+    }
+  }
+  // Green-ball offset random_unit_vector(rng) * (r * beta_22(rng)); g++
+  // evaluates the Beta draw (the right operand) first.
+  __device__ V3<double> green(double r) {
+    const double t = beta22();
+    const V3<double> d = direction();
+    return d * (r * t);
+  }
   // random_unit_vector (wih_twos.cpp:536-542).
   __device__ V3<double> direction() {
     const double z = 1.0 - 2.0 * uniform();
@@ -696,6 +710,56 @@ template <typename R>
 __device__ __forceinline__ R side_source(const DevSide &s, V3<R> x) {
   const double *c = s.src;
   return ((R(c[0]) + R(c[1]) * (x.x + R(c[4]))) + R(c[2]) * (x.y + R(c[5]))) + R(c[3]) * (x.z + R(c[6]));
+}
+
+// Sampled source of source_kind 2: f(a) = s0 + sum of the side's term rows in
+// order (heat bumps, the unit-mass sensor bump, housing Gaussians), each in
+// the reference's operation order (study.cpp:29-51, wih_twos.cpp:553-566).
+__device__ __forceinline__ float exp_r(float v) { return expf(v); }
+__device__ __forceinline__ double exp_r(double v) { return exp(v); }
+
+template <typename R>
+__device__ __forceinline__ R side_field(const DevSide &s, V3<R> a) {
+  R f = R(s.src[0]);
+  const int n = min(s.n_terms, TWOS_MAX_TERMS);
+  for (int k = 0; k < n; ++k) {
+    const double *t = s.terms[k];
+    const int type = int(t[0]);
+    if (type == TWOS_TERM_GAUSS) {
+      const R u = (a.x - R(t[1])) / R(t[4]), v = (a.y - R(t[2])) / R(t[5]), w = (a.z - R(t[3])) / R(t[6]);
+      f += R(t[7]) * exp_r(-(u * u) - v * v - w * w);
+    } else {
+      const V3<R> d = a - V3<R>{R(t[1]), R(t[2]), R(t[3])};
+      const R q = fmax(R(0), R(1) - dot(d, d) / (R(t[4]) * R(t[4])));
+      if (type == TWOS_TERM_UNIT) f += q * q / R(t[5]);
+      else f += R(t[7]) * (q * q);
+    }
+  }
+  return f;
+}
+
+// (r^2/6) f of one step for one side.  x: ball centre, a: Green point.
+// paired_form follows paired_site's r*r/6*f (study.cpp:106-108), otherwise
+// walk()'s f*r*r/6 (wih_twos.cpp:648-653).
+template <typename R>
+__device__ __forceinline__ R side_green_term(const DevSide &s, V3<R> x, V3<R> a, R r, bool paired_form) {
+  if (s.source_kind == 1) return side_source<R>(s, x) * r * r / R(6);
+  if (s.source_kind == 2) return paired_form ? r * r / R(6) * side_field<R>(s, a) : side_field<R>(s, a) * r * r / R(6);
+  return R(0);
+}
+
+// Green-ball offset r t omega from the Philox stream of (key, walk, step):
+// omega from block (step, walk, 1, 0) as the move direction is built; t the
+// median of three 24-bit uniforms from words 0..2 of block (step, walk, 2, 0)
+// (the median of three uniforms is Beta(2,2)); the same t in both precisions.
+template <typename R>
+__device__ __forceinline__ V3<R> philox_green(uint2 key, int32_t walk, int32_t step, R r) {
+  const uint4 w1 = philox4x32_10(make_uint4(uint32_t(step), uint32_t(walk), 1u, 0u), key);
+  const uint4 w2 = philox4x32_10(make_uint4(uint32_t(step), uint32_t(walk), 2u, 0u), key);
+  const V3<R> d = philox_direction(w1, R(0));
+  const R a = R(w2.x >> 8) * R(0x1.0p-24), b = R(w2.y >> 8) * R(0x1.0p-24), c = R(w2.z >> 8) * R(0x1.0p-24);
+  const R t = fmax(fmin(a, b), fmin(fmax(a, b), c));
+  return d * (r * t);
 }
 
 }  // namespace twos

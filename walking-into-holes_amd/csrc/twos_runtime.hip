@@ -686,6 +686,23 @@ twos_status check_params(const twos_params *p, int64_t n) {
   return TWOS_OK;
 }
 
+twos_status check_side(const twos_side &s) {
+  if (s.source_kind < 0 || s.source_kind > 2) return fail(TWOS_ERR_INVALID_ARGUMENT, "unknown source kind");
+  if (s.source_kind == 2) {
+    if (s.n_terms < 1 || s.n_terms > TWOS_MAX_TERMS)
+      return fail(TWOS_ERR_INVALID_ARGUMENT, "a sampled source needs 1..8 term rows");
+    for (int k = 0; k < s.n_terms; ++k) {
+      const double *t = s.terms[k];
+      if (t[0] != TWOS_TERM_BUMP && t[0] != TWOS_TERM_UNIT && t[0] != TWOS_TERM_GAUSS)
+        return fail(TWOS_ERR_INVALID_ARGUMENT, "unknown source term type");
+      if (!(t[4] > 0.0) || (t[0] == TWOS_TERM_UNIT && !(t[5] > 0.0)) ||
+          (t[0] == TWOS_TERM_GAUSS && !(t[5] > 0.0 && t[6] > 0.0)))
+        return fail(TWOS_ERR_INVALID_ARGUMENT, "source term widths must be positive");
+    }
+  }
+  return TWOS_OK;
+}
+
 Request *make_request(twos_ctx *ctx, int32_t geom, const twos_side sides[2], const twos_params *p,
                       const double *sites, int64_t n, twos_site_result *out, twos_status *st) {
   *st = TWOS_OK;
@@ -697,6 +714,8 @@ Request *make_request(twos_ctx *ctx, int32_t geom, const twos_side sides[2], con
     *st = fail(TWOS_ERR_INVALID_ARGUMENT, "sides is null");
     return nullptr;
   }
+  for (int k = 0; k < 2; ++k)
+    if ((*st = check_side(sides[k])) != TWOS_OK) return nullptr;
   if (n > 0 && (!sites || !out)) {
     *st = fail(TWOS_ERR_INVALID_ARGUMENT, "sites and out must be non-null");
     return nullptr;
@@ -1177,6 +1196,7 @@ twos_status twos_trace(twos_ctx *ctx, int32_t geom_id, const twos_side *side, co
   const twos_params &p = *params;
   if (p.max_steps < 1 || !(p.eps > 0.0)) return fail(TWOS_ERR_INVALID_ARGUMENT, "invalid numeric option");
   if (p.rng != kRngPhilox && p.precision != 64) return fail(TWOS_ERR_INVALID_ARGUMENT, "mt streams need binary64");
+  if (twos_status sst = check_side(*side)) return sst;
   std::lock_guard<std::mutex> lock(ctx->mu);
   if (ctx->geoms.find(geom_id) == ctx->geoms.end()) return fail(TWOS_ERR_NOT_FOUND, "unknown geometry id");
   HIP_OK(hipSetDevice(ctx->device));

@@ -98,8 +98,24 @@ __device__ __forceinline__ void job_store(const Job<R> &j, const WalkParams &wp,
 // the eps-shell (payoff added), 2 when it was capped at max_steps.  The cap
 // check precedes the query, so a capped path makes max_steps queries and a
 // terminated one steps + 1, as walk() does (wih_twos.cpp:644-663).
-template <typename R, int GK, bool kCount>
-__device__ __forceinline__ int job_step(Job<R> &j, const WalkParams &wp, TraversalCount *cnt, R *radius_out) {
+// The Green-ball source point of a sampled source (source_kind 2); both
+// sides of a paired path share it.
+template <typename R>
+__device__ __forceinline__ bool job_sampled(const Job<R> &j, const WalkParams &wp) {
+  return j.s0->source_kind == 2 || (wp.combine == kPaired && j.s1->source_kind == 2);
+}
+
+template <typename R>
+__device__ __forceinline__ void job_source(Job<R> &j, const WalkParams &wp, R r, V3<R> a) {
+  const bool paired_form = wp.rng == kRngMtPaired;
+  if (j.s0->source_kind) j.acc0 += side_green_term<R>(*j.s0, j.x, a, r, paired_form);
+  if (wp.combine == kPaired && j.s1->source_kind) j.acc1 += side_green_term<R>(*j.s1, j.x, a, r, paired_form);
+}
+
+// green(r) returns the Green-ball offset of this step from the path's stream.
+template <typename R, int GK, bool kCount, class Green>
+__device__ __forceinline__ int job_step(Job<R> &j, const WalkParams &wp, TraversalCount *cnt, R *radius_out,
+                                        Green &&green) {
   if (j.steps >= j.max_steps) return 2;
   V3<R> bp;
   int cls;
@@ -115,8 +131,7 @@ __device__ __forceinline__ int job_step(Job<R> &j, const WalkParams &wp, Travers
     if (wp.combine == kPaired) j.acc1 += side_payoff<R>(*j.s1, bp, cls);
     return 1;
   }
-  if (s0.source_kind == 1) j.acc0 += side_source<R>(s0, j.x) * r * r / R(6);
-  if (wp.combine == kPaired && j.s1->source_kind == 1) j.acc1 += side_source<R>(*j.s1, j.x) * r * r / R(6);
+  job_source(j, wp, r, job_sampled(j, wp) ? j.x + green(r) : j.x);
   return 0;
 }
 
@@ -144,7 +159,15 @@ __global__ __launch_bounds__(256) void walk_simple(Tables tb, WalkParams wp) {
   int row = 0;
   while (true) {
     R r = R(0);
-    const int state = job_step<R, GK, false>(j, wp, nullptr, &r);
+    auto green = [&](R rr) -> V3<R> {
+      if constexpr (kMt) {
+        const V3<double> o = mt.green(double(rr));
+        return {R(o.x), R(o.y), R(o.z)};
+      } else {
+        return philox_green<R>(j.key, j.walk, j.steps, rr);
+      }
+    };
+    const int state = job_step<R, GK, false>(j, wp, nullptr, &r, green);
     if (wp.trace && row < wp.trace_rows && state != 2) {
       double *o = wp.trace + 4 * row++;
       o[0] = double(j.x.x); o[1] = double(j.x.y); o[2] = double(j.x.z); o[3] = double(r);
@@ -201,8 +224,7 @@ __device__ __forceinline__ int job_advance(Job<R> &j, const WalkParams &wp, R r,
     if (wp.combine == kPaired) j.acc1 += side_payoff<R>(*j.s1, bp, cls);
     return 1;
   }
-  if (s0.source_kind == 1) j.acc0 += side_source<R>(s0, j.x) * r * r / R(6);
-  if (wp.combine == kPaired && j.s1->source_kind == 1) j.acc1 += side_source<R>(*j.s1, j.x) * r * r / R(6);
+  job_source(j, wp, r, job_sampled(j, wp) ? j.x + philox_green<R>(j.key, j.walk, j.steps, r) : j.x);
   const V3<R> d = philox_dir(j);
   j.x = j.x + d * r;
   ++j.steps;

@@ -24,13 +24,15 @@ import numpy as np
 
 from . import _capi
 from ._capi import (BALL, BOX, CROSS, MESH, PAIRED, PERSISTENT, PLAIN, RNG_MT_KEY, RNG_MT_PAIRED,
-                    RNG_MT_SITE, RNG_PHILOX, SQUARE, FLAG_COUNT_TRAVERSAL, FLAG_DISTANCE_GRID)
+                    RNG_MT_SITE, RNG_PHILOX, SQUARE, FLAG_COUNT_TRAVERSAL, FLAG_DISTANCE_GRID, MAX_TERMS,
+                    TERM_BUMP, TERM_GAUSS, TERM_UNIT)
 
 __all__ = [
     "Context", "Mesh", "Side", "Result", "Server", "TwosError", "InvalidArgument", "StepCapError",
     "GeometryError", "AdmissionError", "DeviceError", "NotFoundError", "side", "reference_affine_side",
     "BALL", "BOX", "MESH", "SQUARE", "PAIRED", "CROSS", "RNG_PHILOX", "RNG_MT_SITE", "RNG_MT_PAIRED",
     "RNG_MT_KEY", "PLAIN", "PERSISTENT", "FLAG_COUNT_TRAVERSAL", "FLAG_DISTANCE_GRID", "version",
+    "TERM_BUMP", "TERM_UNIT", "TERM_GAUSS", "MAX_TERMS", "study_terms", "housing_chip_terms",
 ]
 
 
@@ -100,14 +102,28 @@ def _ptr(a: np.ndarray, t):
 Side = _capi.Side
 
 
-def side(poly=None, cls_const=None, poly_classes=(), source=None) -> Side:
+def side(poly=None, cls_const=None, poly_classes=(), source=None, terms=None, offset=0.0) -> Side:
     """Boundary data and source of one side.  The Dirichlet value at the closest
     boundary point y of class c is cls_const[c] + [c in poly_classes] * poly(y),
     poly = (c0, cx, cy, cz, cxx, cyy, czz, cxy, cxz, cyz).  Classes: 0-5 container
     faces (ball and mesh use 0; box faces -x,+x,-y,+y,-z,+z), 6 holes, 7 cavities.
     source = (s0, sx, sy, sz, ox, oy, oz) adds (r^2/6) f(x) per step with
-    f = ((s0 + sx (x+ox)) + sy (y+oy)) + sz (z+oz)."""
+    f = ((s0 + sx (x+ox)) + sy (y+oy)) + sz (z+oz) at the ball centre.
+    terms = rows (type, cx, cy, cz, p0, p1, p2, w) (TERM_BUMP, TERM_UNIT,
+    TERM_GAUSS) add (r^2/6) f(a) with f = offset + sum of the rows, at a
+    Green-distributed point a of each step's ball (wih_twos.cpp:648-653)."""
     s = Side()
+    if terms is not None:
+        if source is not None:
+            raise InvalidArgument("a side has either an affine source or sampled terms")
+        rows = [[float(v) for v in t] for t in terms]
+        if not 0 < len(rows) <= MAX_TERMS or any(len(t) != 8 for t in rows):
+            raise InvalidArgument(f"terms needs 1..{MAX_TERMS} rows of 8 numbers")
+        s.source_kind = 2
+        s.src[0] = float(offset)
+        s.n_terms = len(rows)
+        for k, t in enumerate(rows):
+            s.terms[k] = (C.c_double * 8)(*t)
     poly = list(poly) if poly is not None else [0.0] * 10
     if len(poly) != 10:
         raise InvalidArgument("poly needs 10 coefficients")
@@ -134,6 +150,30 @@ def reference_affine_side(slope: float = 0.65) -> Side:
     """The reference's compliance problem: zero Dirichlet data, source
     f = 1 + slope * (y + 0.5) (wih_twos.cpp:546-552, --source-y-slope)."""
     return side(source=(1.0, 0.0, slope, 0.0, 0.0, 0.5, 0.0))
+
+
+def study_terms(config_text: str):
+    """(heat rows, sensor rows) of a study source configuration: the first
+    line is the sensor (cx cy cz R), then heat bumps (cx cy cz R A)
+    (benchmarks/submission_study/study.cpp:37-43).  The sensor's mass
+    32 pi R^3 / 105 is computed as study.cpp:50 computes it."""
+    vals = [float(v) for v in config_text.split()]
+    if len(vals) < 9 or (len(vals) - 4) % 5:
+        raise InvalidArgument("expected a sensor line and at least one heat bump")
+    sensor, rest = vals[:4], vals[4:]
+    heat = [(TERM_BUMP, *rest[i:i + 3], rest[i + 3], 0.0, 0.0, rest[i + 4]) for i in range(0, len(rest), 5)]
+    if len(heat) > MAX_TERMS:
+        raise InvalidArgument(f"at most {MAX_TERMS} heat bumps")
+    mass = 32 * 3.141592653589793 * float(np.power(sensor[3], 3.0)) / 105
+    return heat, [(TERM_UNIT, *sensor[:3], sensor[3], mass, 0.0, 0.0)]
+
+
+def housing_chip_terms():
+    """(offset, rows) of the built-in housing source
+    0.08 + 5.2 chip_a + 4.6 chip_b + 2.0 regulator (wih_twos.cpp:553-566)."""
+    return 0.08, [(TERM_GAUSS, -0.48, -0.18, -0.08, 0.24, 0.22, 0.16, 5.2),
+                  (TERM_GAUSS, 0.43, 0.19, -0.07, 0.21, 0.20, 0.15, 4.6),
+                  (TERM_GAUSS, 0.02, -0.31, -0.02, 0.30, 0.16, 0.19, 2.0)]
 
 
 @dataclass

@@ -27,6 +27,8 @@ RNG_PHILOX, RNG_MT_SITE, RNG_MT_PAIRED, RNG_MT_KEY = 0, 1, 2, 3
 PLAIN, PERSISTENT = 0, 1
 FLAG_COUNT_TRAVERSAL = 1
 FLAG_DISTANCE_GRID = 2
+TERM_BUMP, TERM_UNIT, TERM_GAUSS = 0, 1, 2
+MAX_TERMS = 8
 
 _dp = C.POINTER(C.c_double)
 _i32p = C.POINTER(C.c_int32)
@@ -36,7 +38,8 @@ _u32p = C.POINTER(C.c_uint32)
 
 class Side(C.Structure):
     _fields_ = [("poly", C.c_double * 10), ("cls_const", C.c_double * 8), ("cls_poly_mask", C.c_uint32),
-                ("source_kind", C.c_int32), ("src", C.c_double * 7)]
+                ("source_kind", C.c_int32), ("src", C.c_double * 7), ("n_terms", C.c_int32), ("pad", C.c_int32),
+                ("terms", (C.c_double * 8) * MAX_TERMS)]
 
 
 class GeometryDesc(C.Structure):
