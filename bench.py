@@ -52,7 +52,9 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--waves-per-simd", type=int, default=0)
+    ap.add_argument("--waves-per-simd", type=int, default=-1,
+                    help="persistent occupancy variant; default: the autotuner's choice")
+    ap.add_argument("--no-autotune", action="store_true", help="default launch shape instead of the tuner's")
     ap.add_argument("--exact", action="store_true", help="exact closest-point radius at every mesh step")
     ap.add_argument("--sites", type=int, default=0, help="override the site count (profiling runs)")
     return ap.parse_args()
@@ -100,12 +102,27 @@ def main() -> int:
         mesh = None
     sides = W.sides(w, twos.side)
     adaptive = w.walks_max > w.walks
+    grid_on = w.container == twos.MESH and not args.exact
     kw = dict(combine=twos.CROSS, precision=w.precision, seed=2026, walks=w.walks, walks_max=w.walks_max,
-              ci_rel=w.ci_rel, eps=w.eps, max_steps=w.max_steps, waves_per_simd=args.waves_per_simd,
-              distance_grid=(w.container == twos.MESH and not args.exact))
+              ci_rel=w.ci_rel, eps=w.eps, max_steps=w.max_steps, distance_grid=grid_on)
+    # Launch shape of the persistent path: the autotuner's choice (untimed,
+    # before warmup), unless fixed on the command line.
+    shape = {}
+    tuning = None
+    if args.waves_per_simd >= 0:
+        shape = {"waves_per_simd": args.waves_per_simd}
+    elif not args.no_autotune:
+        from twos import autotune
+        t0 = time.perf_counter()
+        tuning = autotune.autotune(ctx, geom, sites, sides, geom_sig=f"{w.name}", combine=twos.CROSS,
+                                   precision=w.precision, eps=w.eps, max_steps=w.max_steps,
+                                   distance_grid=grid_on, log=log)
+        shape = {k: v for k, v in tuning.choice.items() if grid_on or k != "grid_kappa"}
+        log(f"autotune {time.perf_counter() - t0:.1f} s: {tuning.choice}, "
+            f"{tuning.speedup:.3f}x the default shape per path")
 
     def run(schedule, pts, graph):
-        return ctx.estimate(geom, pts, sides, schedule=schedule, use_graph=graph, **kw)
+        return ctx.estimate(geom, pts, sides, schedule=schedule, use_graph=graph, **kw, **shape)
 
     def finished(res):
         if adaptive:
@@ -140,7 +157,7 @@ def main() -> int:
     if w.container == twos.MESH:
         sub = sites[:: max(1, n // 1024)]
         cnt = ctx.estimate(geom, sub, sides, schedule=twos.PERSISTENT, count_traversal=True,
-                           **{**kw, "walks": 512, "walks_max": 0})
+                           **{**kw, **shape, "walks": 512, "walks_max": 0})
         q_sub = cnt.stats["steps"] + cnt.stats["paths"]
         n_nodes = cnt.stats["nodes_visited"] / q_sub
         n_tris = cnt.stats["tris_tested"] / q_sub
@@ -182,8 +199,13 @@ def main() -> int:
         "sites_capped": capped,
         "mean_walks_per_side": round(float(np.mean(res.walks)), 1),
         "rounds": res.stats["rounds"],
+        "launch_shape": {"block": 256, "waves_per_simd": 0, "lds_stack": 24,
+                         **({"grid_kappa": 4.0} if grid_on else {}), **shape},
         "roofline": roofline,
     }
+    if tuning is not None:
+        out["autotune"] = {"choice": tuning.choice, "speedup_vs_default_shape": round(tuning.speedup, 4),
+                           "trials": len(tuning.trials)}
     if mesh is not None:
         out["lbvh_build_ms"] = round(mesh.build_ms, 3)
     if not args.no_plain:

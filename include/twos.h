@@ -142,10 +142,16 @@ typedef struct {
   double ci_rel;
   double ci_z;
   int32_t use_graph;    /* capture each round's launches in a hipGraph */
-  int32_t block;        /* threads per block of the walk kernel, 0 = tuned */
-  int32_t waves_per_simd; /* persistent occupancy, 0 = tuned */
+  int32_t block;        /* threads per block of the walk kernel (multiple of 64,
+                           at most 256; 0 = 256) */
+  int32_t waves_per_simd; /* persistent binary32 occupancy variant: 5, 6 or 8
+                           waves per SIMD, 0 = the compiler's allocation */
   int32_t site_offset;  /* global index of sites[0] in stream keys */
   int32_t flags;        /* TWOS_FLAG_* */
+  float grid_kappa;     /* distance grid: a step uses the grid bound when it is at
+                           least grid_kappa cell sizes (0 = 4) */
+  int32_t lds_stack;    /* binary32 persistent kernel: LDS traversal-stack entries
+                           per lane, 16, 24 or 32 (0 = 24) */
   int32_t pad;
 } twos_params;
 

@@ -173,6 +173,40 @@ def study_fixtures(root: Path) -> None:
     (HERE / "study_branch_meta.json").write_text(json.dumps(meta, indent=1) + "\n")
 
 
+def housing_fixtures(root: Path) -> None:
+    """Scout and validation fields of the engineering-housing design loop
+    (benchmarks/engineering_housing/output/field-rep-1-*.csv with their
+    points/cavities files): wih_twos --mode twos --source-mode housing,
+    eps 1e-3, max_steps 512, cavity radius 0.095, repeat 0.  Seeds follow
+    run_housing_benchmark.py:826-869: replicate 1 of "TWoS + validation" /
+    "TWoS equal-time" uses SEED + 100000, scout step k adds 1000 + k,
+    validation step k adds 2000 + k (SEED = 20260912, :28); walk counts are
+    the driver's defaults (:1116-1120)."""
+    housing = root / "benchmarks" / "engineering_housing"
+    v, f = read_ascii_ply(housing / "assets" / "stylized-finned-housing.ply")
+    np.savez_compressed(HERE / "housing_mesh.npz", vertices=v, faces=f)
+    base = 20260912 + 100000
+    picks = [("rep-1-twos-scout-01", base + 1001, 12), ("rep-1-twos-scout-03", base + 1003, 12),
+             ("rep-1-twos-validate-02", base + 2002, 192), ("rep-1-twos-validate-05", base + 2005, 192),
+             ("rep-1-twos-equal-scout-02", base + 1002, 96), ("rep-1-twos-equal-validate-04", base + 2004, 768)]
+    arrays, requests = {}, []
+    for k, (tag, seed, walks) in enumerate(picks):
+        data = read_field(housing / "output" / f"field-{tag}.csv")
+        cav_path = housing / "output" / f"cavities-{tag}.csv"
+        cav = np.loadtxt(cav_path, delimiter=",", ndmin=2) if cav_path.exists() else np.zeros((0, 3))
+        summary = json.loads((housing / "output" / f"summary-{tag}.json").read_text())
+        assert summary["walks_per_site"] == walks and summary["cavities"] == len(cav)
+        arrays[f"{k}_points"] = data["xyz"]
+        arrays[f"{k}_cavities"] = cav
+        arrays[f"{k}_values"] = np.c_[data["u"], data["score"], data["score_se"]]
+        arrays[f"{k}_counts"] = np.c_[data["steps"], data["limit_hits"]]
+        requests.append({"tag": tag, "seed": seed, "walks": walks, "epsilon": summary["epsilon"],
+                         "cavity_radius": 0.095, "max_steps": 512})
+    np.savez_compressed(HERE / "housing_requests.npz", **arrays)
+    (HERE / "housing_meta.json").write_text(json.dumps({"requests": requests}, indent=1) + "\n")
+
+
 if __name__ == "__main__":
     main()
     study_fixtures(Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference"))
+    housing_fixtures(Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference"))

@@ -73,3 +73,45 @@ def compare(req: StudyRequest, u, p, score, same, steps, caps, rtol: float) -> n
         err = np.abs(ours[:, c] - ref[:, c])
         assert np.all(err <= rtol * scale), (req.source, c, np.max(err / np.maximum(scale, 1e-300)))
     return np.all(ours == ref, axis=1)
+
+
+@dataclass
+class HousingRequest:
+    tag: str
+    points: np.ndarray
+    cavities: np.ndarray
+    values: np.ndarray   # [n, 3]: u, score, score_se
+    counts: np.ndarray   # [n, 2]: steps, limit_hits
+    seed: int
+    walks: int
+    epsilon: float
+    max_steps: int
+    cavity_radius: float
+
+
+def load_housing(golden: Path):
+    """Scout/validation fields of the engineering-housing design loop
+    (wih_twos --source-mode housing; tests/golden/housing_*)."""
+    meta = json.loads((golden / "housing_meta.json").read_text())
+    mesh = np.load(golden / "housing_mesh.npz")
+    arrays = np.load(golden / "housing_requests.npz")
+    requests = [HousingRequest(tag=r["tag"], points=arrays[f"{k}_points"], cavities=arrays[f"{k}_cavities"],
+                               values=arrays[f"{k}_values"], counts=arrays[f"{k}_counts"], seed=r["seed"],
+                               walks=r["walks"], epsilon=r["epsilon"], max_steps=r["max_steps"],
+                               cavity_radius=r["cavity_radius"])
+                for k, r in enumerate(meta["requests"])]
+    return mesh["vertices"], mesh["faces"], requests
+
+
+def compare_housing(req: HousingRequest, u, score, score_se, steps, caps, rtol: float) -> np.ndarray:
+    """Step and cap counts equal; u, score, score_se within rtol; returns the
+    per-site bitwise agreement of the three values."""
+    np.testing.assert_array_equal(steps, req.counts[:, 0])
+    np.testing.assert_array_equal(caps, req.counts[:, 1])
+    ours = np.c_[u, score, score_se]
+    for c in range(3):
+        ref = req.values[:, c]
+        scale = np.maximum(np.abs(ref), 1e-3 * np.max(np.abs(ref)))
+        err = np.abs(ours[:, c] - ref)
+        assert np.all(err <= rtol * scale), (req.tag, c, np.max(err / scale))
+    return np.all(ours == req.values, axis=1)

@@ -62,6 +62,34 @@ def test_same_alone_and_in_the_mix(env):
         np.testing.assert_array_equal(getattr(alone, k), getattr(direct, k), err_msg=k)
 
 
+def test_grid_choice_is_per_request_in_a_shared_launch(env):
+    """Requests of one class (binary32, CROSS) share each persistent launch;
+    the distance-grid choice and its threshold stay per request: an exact
+    request mixed with grid requests equals the exact request alone, and a
+    grid request with another threshold equals itself alone."""
+    ctx, ico, g_mesh, _, _ = env
+    kw = dict(combine=twos.CROSS, precision=32, walks=64, walks_max=512, ci_rel=0.04, eps=1e-4, max_steps=8192)
+    sides_m = W.sides(ico, twos.side)
+    exact_sites, grid_sites, other_sites = ico.sites[:64], ico.sites[64:160], ico.sites[160:224]
+
+    srv = ctx.server(4096, 1 << 20)
+    r_grid = srv.submit(g_mesh, grid_sites, sides_m, seed=31, distance_grid=True, **kw)
+    r_exact = srv.submit(g_mesh, exact_sites, sides_m, seed=32, **kw)
+    r_k2 = srv.submit(g_mesh, other_sites, sides_m, seed=33, distance_grid=True, grid_kappa=2.0, **kw)
+    mixed = _drain(srv)
+    srv.close()
+
+    exact = ctx.estimate(g_mesh, exact_sites, sides_m, seed=32, **kw)
+    grid = ctx.estimate(g_mesh, grid_sites, sides_m, seed=31, distance_grid=True, **kw)
+    k2 = ctx.estimate(g_mesh, other_sites, sides_m, seed=33, distance_grid=True, grid_kappa=2.0, **kw)
+    for rid, ref in ((r_exact, exact), (r_grid, grid), (r_k2, k2)):
+        for k in KEYS:
+            np.testing.assert_array_equal(getattr(mixed[rid], k), getattr(ref, k), err_msg=k)
+    # The threshold changes the paths (so it reached the kernel).
+    k4 = ctx.estimate(g_mesh, other_sites, sides_m, seed=33, distance_grid=True, **kw)
+    assert not np.array_equal(k4.steps, k2.steps)
+
+
 def test_admission_waits_for_slots(env):
     ctx, ico, g_mesh, ball, g_ball = env
     srv = ctx.server(100, 1 << 20)

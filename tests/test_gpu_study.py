@@ -114,3 +114,24 @@ def test_sampled_source_binary32_shadows_binary64(ctx):
     r64 = ctx.estimate(g, sites, [twos.side(terms=bump)], precision=64, **kw)
     se = r64.a.std(axis=1) / math.sqrt(r64.a.shape[1])
     assert np.max(np.abs(r32.u - r64.u) / se) < 0.5
+
+
+def test_housing_fields_on_gpu(ctx, golden):
+    """The engineering-housing scout/validation fields (evaluate_site with the
+    housing chip source at a Green point, binary32 query radius): binary64
+    walks on the reference's mt19937_64 streams reproduce every step count
+    and u / score / score_se to 1e-6 (glibc and device sin/cos/exp may differ
+    by an ulp, which a path carries forward)."""
+    v, f, requests = SF.load_housing(golden)
+    mesh = ctx.mesh(v, f)
+    off, chips = twos.housing_chip_terms()
+    for req in requests:
+        print("housing request", req.tag, len(req.points), "sites", req.walks, "walks", flush=True)
+        g = ctx.geometry(twos.MESH, mesh=mesh, cavities=req.cavities if len(req.cavities) else None,
+                         cavity_radius=req.cavity_radius, mesh_radius_f32=True)
+        r = ctx.estimate(g, req.points, [twos.side(terms=chips, offset=off)], combine=twos.SQUARE,
+                         rng=twos.RNG_MT_SITE, precision=64, seed=req.seed, walks=req.walks, eps=req.epsilon,
+                         max_steps=req.max_steps)
+        SF.compare_housing(req, r.u, r.score, r.score_se, r.steps, r.caps, rtol=1e-6)
+        ctx.release_geometry(g)
+    ctx.release_mesh(mesh)

@@ -61,14 +61,15 @@ def test_c3_exact_walks_against_oracle(ctx, c3):
     assert np.sum(np.abs(z) > 3) <= 1 and np.max(np.abs(z)) < 5
 
 
-def test_c3_distance_grid_against_exact(ctx, c3):
+@pytest.mark.parametrize("kappa", [1.0, 2.0, 4.0])
+def test_c3_distance_grid_against_exact(ctx, c3, kappa):
     """The conservative-radius walk estimates the same quantity as the exact
-    walk (independent seeds)."""
+    walk (independent seeds), at every threshold the autotuner may choose."""
     w, mesh, g = c3
     sites = w.sites[:128]
     kw = dict(combine=twos.CROSS, precision=32, walks=4096, eps=w.eps, max_steps=w.max_steps)
     exact = ctx.estimate(g, sites, W.sides(w, twos.side), seed=1, **kw)
-    grid = ctx.estimate(g, sites, W.sides(w, twos.side), seed=2, distance_grid=True, **kw)
+    grid = ctx.estimate(g, sites, W.sides(w, twos.side), seed=2, distance_grid=True, grid_kappa=kappa, **kw)
     z = (grid.td - exact.td) / np.hypot(grid.score_se, exact.score_se)
     assert np.sum(np.abs(z) > 3) <= 2 and np.max(np.abs(z)) < 5
     assert abs(np.mean(z)) < 0.35  # no systematic shift (128 sites: sd of the mean ~0.09)

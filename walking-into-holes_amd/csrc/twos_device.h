@@ -558,12 +558,12 @@ __device__ __forceinline__ R closest_boundary(const DevGeom &g, V3<R> p, V3<R> &
 // Resumable closest-point query for the wavefront kernel: the same traversal
 // as mesh_closest, one 4-wide node per call, so lanes whose query ends early
 // go on with their walk while others are still traversing.  The stack lives
-// in LDS (kLdsStack entries per lane, column-major so a wave's lanes hit
-// distinct banks) with a private overflow; an entry packs the 4-wide node
-// index (21 bits) with a lower bound of its squared box distance (the top
-// 11 bits of the binary32 pattern, truncated, so never above the bound).
-constexpr int kLdsStack = 24;
-constexpr int kOverflowStack = kStackDepth - kLdsStack;
+// in LDS (kLds entries per lane, column-major with the block's lane count
+// as stride so a wave's lanes hit distinct banks) with a private overflow of
+// kStackDepth - kLds entries; an entry packs the 4-wide node index (21 bits)
+// with a lower bound of its squared box distance (the top 11 bits of the
+// binary32 pattern, truncated, so never above the bound).  kLds is a kernel
+// variant (16, 24 or 32 entries) chosen by the autotuner.
 
 template <typename R>
 struct MeshQuery {
@@ -641,9 +641,9 @@ __device__ __forceinline__ bool mq_begin(const MeshView &m, V3<R> p, MeshQuery<R
 }
 
 // Visits q.node; returns true when the query is complete.
-template <typename R, bool kCount, int kBlock>
+template <typename R, bool kCount, int kLds>
 __device__ __forceinline__ bool mq_visit(const MeshView &m, V3<R> p, MeshQuery<R> &q, uint32_t *lds,
-                                         uint32_t *overflow, TraversalCount *cnt) {
+                                         uint32_t stride, uint32_t *overflow, TraversalCount *cnt) {
   const float4 *nq = m.nodes4 + size_t(kNode4Quads) * q.node;
   const float4 lx = nq[0], ly = nq[1], lz = nq[2], hx = nq[3], hy = nq[4], hz = nq[5];
   const int4 ch = *reinterpret_cast<const int4 *>(nq + 6);
@@ -670,8 +670,8 @@ __device__ __forceinline__ bool mq_visit(const MeshView &m, V3<R> p, MeshQuery<R
     for (int i = 3; i >= 1; --i) {
       if (key[i] < inf && q.sp < kStackDepth) {
         const uint32_t e = stack_pack(c[i], lower_f32<R>(key[i]));
-        if (q.sp < kLdsStack) lds[q.sp * kBlock] = e;
-        else overflow[q.sp - kLdsStack] = e;
+        if (q.sp < kLds) lds[q.sp * stride] = e;
+        else overflow[q.sp - kLds] = e;
         ++q.sp;
       }
     }
@@ -680,7 +680,7 @@ __device__ __forceinline__ bool mq_visit(const MeshView &m, V3<R> p, MeshQuery<R
   }
   while (q.sp > 0) {
     --q.sp;
-    const uint32_t e = q.sp < kLdsStack ? lds[q.sp * kBlock] : overflow[q.sp - kLdsStack];
+    const uint32_t e = q.sp < kLds ? lds[q.sp * stride] : overflow[q.sp - kLds];
     if (mq_within(q, R(stack_lower(e)))) {
       q.node = stack_node(e);
       return false;

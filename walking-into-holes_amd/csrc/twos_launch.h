@@ -34,8 +34,8 @@ hipError_t launch_closest(const DevGeom *d_geom, int precision, const double *d_
 // geom_kind: kGeomGeneric / kGeomMesh / kGeomAnalytic.
 hipError_t launch_walk_simple_f32(int geom_kind, const Tables &tb, const WalkParams &wp, int block,
                                   hipStream_t stream);
-hipError_t launch_walk_persistent_f32(int geom_kind, bool count, int waves, const Tables &tb, const WalkParams &wp,
-                                      int block, int cus, hipStream_t stream, int *grid_out);
+hipError_t launch_walk_persistent_f32(int geom_kind, bool count, int waves, int lds, const Tables &tb,
+                                      const WalkParams &wp, int block, int cus, hipStream_t stream, int *grid_out);
 hipError_t launch_slot_hints(const Tables &tb, int64_t first, int64_t n, hipStream_t stream);
 hipError_t launch_walk_simple_f64(int geom_kind, bool mt, const Tables &tb, const WalkParams &wp, int block,
                                   hipStream_t stream);

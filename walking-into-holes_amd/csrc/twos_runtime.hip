@@ -164,7 +164,7 @@ struct Engine {
     graph_key = -1;
     dfree(s.pos); dfree(s.geom); dfree(s.problem); dfree(s.keys); dfree(s.site); dfree(s.seed);
     dfree(s.repeat); dfree(s.walks_max); dfree(s.max_steps); dfree(s.eps); dfree(s.ci_rel); dfree(s.ci_z);
-    dfree(s.acc); dfree(s.acc_walks); dfree(s.acc_steps); dfree(s.acc_caps); dfree(s.result); dfree(s.hint0);
+    dfree(s.acc); dfree(s.acc_walks); dfree(s.acc_steps); dfree(s.acc_caps); dfree(s.result); dfree(s.hint0); dfree(s.grid_kappa);
     dfree(d_sides); dfree(d_val); dfree(d_steps); dfree(d_counters); dfree(d_in); dfree(d_out); dfree(d_plain);
     if (h_in) (void)hipHostFree(h_in);
     if (h_out) (void)hipHostFree(h_out);
@@ -186,6 +186,7 @@ struct Engine {
     HIP_OK(dalloc(&s.eps, C)); HIP_OK(dalloc(&s.ci_rel, C)); HIP_OK(dalloc(&s.ci_z, C));
     HIP_OK(dalloc(&s.acc, 8 * C)); HIP_OK(dalloc(&s.acc_walks, C)); HIP_OK(dalloc(&s.acc_steps, C));
     HIP_OK(dalloc(&s.acc_caps, C)); HIP_OK(dalloc(&s.result, C)); HIP_OK(dalloc(&s.hint0, C));
+    HIP_OK(dalloc(&s.grid_kappa, C));
     HIP_OK(dalloc(&d_sides, 2 * size_t(cap_problems)));
     HIP_OK(dalloc(&d_val, 2 * size_t(cap_walks)));
     HIP_OK(dalloc(&d_steps, 2 * size_t(cap_walks)));
@@ -288,6 +289,9 @@ struct Engine {
     std::vector<int64_t> site(n);
     std::vector<uint64_t> seed(n, r->prm.seed);
     std::vector<double> eps(n, r->prm.eps), rel(n), z(n, r->prm.ci_z);
+    const float kappa = (r->prm.flags & TWOS_FLAG_DISTANCE_GRID)
+                            ? (r->prm.grid_kappa > 0.f ? r->prm.grid_kappa : kGridKappa) : 0.f;
+    std::vector<float> kap(n, kappa);
     const bool adaptive = r->prm.walks_max > r->prm.walks;
     for (int64_t i = 0; i < n; ++i) {
       const int64_t gsite = int64_t(r->prm.site_offset) + i;
@@ -309,6 +313,7 @@ struct Engine {
     HIP_OK(hipMemcpyAsync(s.eps + s0, eps.data(), 8 * n, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(s.ci_rel + s0, rel.data(), 8 * n, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(s.ci_z + s0, z.data(), 8 * n, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(s.grid_kappa + s0, kap.data(), 4 * n, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemsetAsync(s.acc + 8 * s0, 0, sizeof(double) * 8 * n, st));
     HIP_OK(hipMemsetAsync(s.acc_walks + s0, 0, 8 * n, st));
     HIP_OK(hipMemsetAsync(s.acc_steps + s0, 0, 8 * n, st));
@@ -366,7 +371,6 @@ struct Engine {
     wp.val = d_val;
     wp.steps = d_steps;
     wp.counters = d_counters + kMaxClasses;
-    wp.grid_kappa = (p.flags & TWOS_FLAG_DISTANCE_GRID) ? kGridKappa : 0.f;
     hipStream_t st = ctx->stream;
     const bool count = (p.flags & TWOS_FLAG_COUNT_TRAVERSAL) != 0;
     hipError_t e;
@@ -377,15 +381,17 @@ struct Engine {
       e = p.precision == 64 ? launch_walk_simple_f64(kind, false, tb, wp, block, st)
                             : launch_walk_simple_f32(kind, tb, wp, block, st);
     } else {
-      // The persistent kernel is written for 256-thread blocks (LDS stack).
+      // Launch shape: block (64..256 lanes), occupancy variant and LDS stack
+      // depth (binary32 only); none of them changes a result.
       HIP_OK(launch_zero(tb.round.counter, 1, st));
       int grid = 0;
       const int cus = ctx->prop.multiProcessorCount;
+      const int block = p.block > 0 ? p.block : 256;
       e = p.precision == 64
-              ? launch_walk_persistent_f64(kind, count, tb, wp, 256, cus, st, &grid)
-              : launch_walk_persistent_f32(kind, count, p.waves_per_simd, tb, wp, 256, cus, st, &grid);
+              ? launch_walk_persistent_f64(kind, count, tb, wp, block, cus, st, &grid)
+              : launch_walk_persistent_f32(kind, count, p.waves_per_simd, p.lds_stack, tb, wp, block, cus, st, &grid);
       stats.grid = grid;
-      stats.block = 256;
+      stats.block = block;
       stats.waves_per_simd = p.waves_per_simd;
     }
     if (e != hipSuccess) return fail(TWOS_ERR_DEVICE, std::string("walk launch: ") + hipGetErrorString(e));
@@ -413,7 +419,12 @@ struct Engine {
     bool full = false;
     for (Request *r : active) {
       if (full) break;
-      const int64_t key = int64_t(r->prm.precision) * 16 + r->prm.combine * 4 + (r->prm.rng != kRngPhilox ? 1 : 0);
+      // Launch-wide choices split classes: precision, combine, the stream
+      // family and traversal counting.  Everything else a result depends on
+      // (eps, max_steps, the distance-grid threshold, sides, geometry) is
+      // per slot, so a site's result is the same alone and in any mix.
+      const int64_t key = int64_t(r->prm.precision) * 16 + r->prm.combine * 4 + (r->prm.rng != kRngPhilox ? 1 : 0) +
+                          ((r->prm.flags & TWOS_FLAG_COUNT_TRAVERSAL) ? 1024 : 0);
       int c = -1;
       for (size_t k = 0; k < class_key.size(); ++k)
         if (class_key[k] == key) c = int(k);
@@ -490,7 +501,6 @@ struct Engine {
           wp.job_end = int64_t(en.inc) * jpw;
           wp.val = d_val;
           wp.steps = d_steps;
-          wp.grid_kappa = (en.r->prm.flags & TWOS_FLAG_DISTANCE_GRID) ? kGridKappa : 0.f;
           const int block = en.r->prm.block > 0 ? en.r->prm.block : 256;
           const int kind = geom_kind_of(en.r->geom);
           const hipError_t le = en.r->prm.rng != kRngPhilox ? launch_walk_simple_f64(kind, true, tb, wp, 64, strm)
@@ -681,6 +691,10 @@ twos_status check_params(const twos_params *p, int64_t n) {
     return fail(TWOS_ERR_INVALID_ARGUMENT, "adaptive budgets need ci_rel >= 0");
   if (p->block != 0 && (p->block % 64 != 0 || p->block > 256 || p->block < 64))
     return fail(TWOS_ERR_INVALID_ARGUMENT, "block must be 64, 128, 192 or 256");
+  if (p->lds_stack != 0 && p->lds_stack != 16 && p->lds_stack != 24 && p->lds_stack != 32)
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "lds_stack must be 0, 16, 24 or 32");
+  if (!(p->grid_kappa >= 0.f && p->grid_kappa < 1e6f))
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "grid_kappa must be a non-negative number of cells");
   if (p->waves_per_simd < 0 || p->waves_per_simd > 16)
     return fail(TWOS_ERR_INVALID_ARGUMENT, "waves_per_simd must be in [0, 16]");
   return TWOS_OK;

@@ -53,6 +53,7 @@ struct SlotArrays {
   int32_t *repeat;    // repeat index (MT_SITE)
   int32_t *walks_max;
   int32_t *hint0;     // sorted index of the closest triangle at the site (mesh), or -1
+  float *grid_kappa;  // > 0: the request's distance-grid threshold in cell sizes; 0: exact radius
   int32_t *max_steps;
   double *eps;
   double *ci_rel;
@@ -80,7 +81,7 @@ struct WalkParams {
   int32_t combine;
   int32_t rng;
   int32_t jobs_per_walk;  // 2 for CROSS, 1 otherwise
-  float grid_kappa;       // > 0: mesh steps whose grid lower bound exceeds grid_kappa * cell size use it as radius
+  int32_t pad0;
   int64_t job_begin;      // simple kernel: first job of this launch
   int64_t job_end;
   double *val;            // [total walks][2] binary64 samples (a, b)

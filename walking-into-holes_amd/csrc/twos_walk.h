@@ -74,7 +74,10 @@ __device__ __forceinline__ void job_init(Job<R> &j, const Tables &tb, const Walk
   j.key = tb.slots.keys[2 * size_t(slot) + (wp.combine == kCross ? j.side : 0)];
   j.g = tb.geoms + tb.slots.geom[slot];
   j.mv = MeshView{j.g->nodes, j.g->nodes4, j.g->tris_f, j.g->tris_d, j.g->tri_ids, j.g->n_tris};
-  j.tau = (wp.grid_kappa > 0.f && j.g->grid) ? R(wp.grid_kappa) * R(j.g->grid_h) : Inf<R>::v();
+  // A grid step (radius = grid lower bound >= tau) must never end the path,
+  // so tau stays above 2 eps whatever the threshold.
+  const float kappa = tb.slots.grid_kappa[slot];
+  j.tau = (kappa > 0.f && j.g->grid) ? fmax(R(kappa) * R(j.g->grid_h), R(2) * j.eps) : Inf<R>::v();
   const int32_t prob = tb.slots.problem[slot];
   j.s0 = tb.sides + 2 * prob + (wp.combine == kCross ? j.side : 0);
   j.s1 = tb.sides + 2 * prob + 1;
@@ -211,7 +214,11 @@ __global__ __launch_bounds__(256) void walk_simple(Tables tb, WalkParams wp) {
 // counter is exhausted and none of its lanes holds a path; every wave gets
 // there because every path ends within max_steps steps.
 constexpr int kChunk = 64;
-constexpr int kWalkBlock = 256;
+constexpr int kWalkBlock = 256;  // largest block of the persistent kernel
+constexpr int kLdsDefault = 24;  // default LDS stack entries per lane
+
+// Dynamic LDS bytes of a persistent launch: the lanes' traversal stacks.
+constexpr size_t persistent_lds_bytes(int lds, int block) { return size_t(lds) * size_t(block) * 4u; }
 
 // Applies a finished boundary query (distance r, closest point bp of class
 // cls) to the path: 0 moved on, 1 ended in the eps-shell, 2 capped.
@@ -257,13 +264,16 @@ __device__ __forceinline__ bool query_start(Job<R> &j, MeshQuery<R> &q, Traversa
 }
 
 // kWaves is the occupancy the register allocator is asked to allow
-// (amdgpu_waves_per_eu); 1 leaves the choice to the compiler.
-template <typename R, int GK, bool kCount, int kWaves>
+// (amdgpu_waves_per_eu); 1 leaves the choice to the compiler.  kLds is the
+// per-lane LDS stack depth; the launch passes persistent_lds_bytes(kLds,
+// blockDim.x) of dynamic LDS.
+template <typename R, int GK, bool kCount, int kWaves, int kLds>
 __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWaves))) void walk_persistent(
     Tables tb, WalkParams wp) {
-  __shared__ uint32_t lds_stack[kLdsStack * kWalkBlock];
+  extern __shared__ uint32_t lds_stack[];
+  const uint32_t stride = blockDim.x;
   uint32_t *stack = lds_stack + threadIdx.x;
-  uint32_t overflow[kOverflowStack];
+  uint32_t overflow[kStackDepth - kLds];
   const int lane = int(threadIdx.x & 63);
   const int64_t total = *tb.round.total_jobs;
   Job<R> j;
@@ -299,7 +309,7 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWav
     }
     if (!__any(active)) break;
     if (active) {
-      if (in_query) in_query = !mq_visit<R, kCount, kWalkBlock>(j.mv, j.x, q, stack, overflow, &cnt);
+      if (in_query) in_query = !mq_visit<R, kCount, kLds>(j.mv, j.x, q, stack, stride, overflow, &cnt);
       if (!in_query) {
         V3<R> bp;
         int cls;

@@ -30,32 +30,47 @@ hipError_t launch_walk_simple_f32(int geom_kind, const Tables &tb, const WalkPar
 }
 
 namespace {
-template <int kWaves>
+template <int kWaves, int kLds>
 hipError_t persistent_f32(int geom_kind, bool count, const Tables &tb, const WalkParams &wp, int block,
                           int cus, hipStream_t stream, int *grid_out) {
   void (*k)(Tables, WalkParams);
-  if (count) k = walk_persistent<float, kGeomGeneric, true, kWaves>;
-  else if (geom_kind == kGeomMesh) k = walk_persistent<float, kGeomMesh, false, kWaves>;
-  else if (geom_kind == kGeomAnalytic) k = walk_persistent<float, kGeomAnalytic, false, kWaves>;
-  else k = walk_persistent<float, kGeomGeneric, false, kWaves>;
+  int lds = kLds;
+  if (count) k = walk_persistent<float, kGeomGeneric, true, kWaves, kLds>;
+  else if (geom_kind == kGeomMesh) k = walk_persistent<float, kGeomMesh, false, kWaves, kLds>;
+  else if (geom_kind == kGeomAnalytic) {
+    k = walk_persistent<float, kGeomAnalytic, false, kWaves, kLdsDefault>;  // no traversal stack in use
+    lds = 0;
+  } else k = walk_persistent<float, kGeomGeneric, false, kWaves, kLds>;
+  const size_t smem = persistent_lds_bytes(lds, block);
   int per_cu = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, 0);
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, smem);
   if (e != hipSuccess) return e;
   const int grid = std::max(1, per_cu) * cus;
   *grid_out = grid;
-  hipLaunchKernelGGL(k, dim3(grid), dim3(block), 0, stream, tb, wp);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(block), smem, stream, tb, wp);
   return hipGetLastError();
 }
 }  // namespace
 
-// waves: requested occupancy variant (6 or 8 cap the registers for that many
-// waves per SIMD; anything else uses the compiler's allocation).  The grid is
-// one resident wave set: blocks-per-CU from the occupancy query times CUs.
-hipError_t launch_walk_persistent_f32(int geom_kind, bool count, int waves, const Tables &tb, const WalkParams &wp,
-                                      int block, int cus, hipStream_t stream, int *grid_out) {
-  if (waves == 8) return persistent_f32<8>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
-  if (waves == 6) return persistent_f32<6>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
-  return persistent_f32<1>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
+// waves: occupancy variant (5, 6 or 8 cap the registers for that many waves
+// per SIMD; anything else uses the compiler's allocation, 4 waves at the
+// default 24-entry stack).  lds: per-lane LDS stack entries (16, 24 or 32;
+// 16 and 32 are built for the compiler's allocation and 5 waves).  The grid
+// is one resident wave set: blocks-per-CU from the occupancy query times CUs.
+hipError_t launch_walk_persistent_f32(int geom_kind, bool count, int waves, int lds, const Tables &tb,
+                                      const WalkParams &wp, int block, int cus, hipStream_t stream, int *grid_out) {
+  if (lds == 16) {
+    if (waves == 5) return persistent_f32<5, 16>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
+    return persistent_f32<1, 16>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
+  }
+  if (lds == 32) {
+    if (waves == 5) return persistent_f32<5, 32>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
+    return persistent_f32<1, 32>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
+  }
+  if (waves == 8) return persistent_f32<8, 24>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
+  if (waves == 6) return persistent_f32<6, 24>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
+  if (waves == 5) return persistent_f32<5, 24>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
+  return persistent_f32<1, 24>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
 }
 
 hipError_t launch_slot_hints(const Tables &tb, int64_t first, int64_t n, hipStream_t stream) {

@@ -38,16 +38,17 @@ hipError_t launch_walk_simple_f64(int geom_kind, bool mt, const Tables &tb, cons
 hipError_t launch_walk_persistent_f64(int geom_kind, bool count, const Tables &tb, const WalkParams &wp, int block,
                                       int cus, hipStream_t stream, int *grid_out) {
   void (*k)(Tables, WalkParams);
-  if (count) k = walk_persistent<double, kGeomGeneric, true, 1>;
-  else if (geom_kind == kGeomMesh) k = walk_persistent<double, kGeomMesh, false, 1>;
-  else if (geom_kind == kGeomAnalytic) k = walk_persistent<double, kGeomAnalytic, false, 1>;
-  else k = walk_persistent<double, kGeomGeneric, false, 1>;
+  if (count) k = walk_persistent<double, kGeomGeneric, true, 1, kLdsDefault>;
+  else if (geom_kind == kGeomMesh) k = walk_persistent<double, kGeomMesh, false, 1, kLdsDefault>;
+  else if (geom_kind == kGeomAnalytic) k = walk_persistent<double, kGeomAnalytic, false, 1, kLdsDefault>;
+  else k = walk_persistent<double, kGeomGeneric, false, 1, kLdsDefault>;
+  const size_t smem = geom_kind == kGeomAnalytic && !count ? 0 : persistent_lds_bytes(kLdsDefault, block);
   int per_cu = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, 0);
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, smem);
   if (e != hipSuccess) return e;
   const int grid = std::max(1, per_cu) * cus;
   *grid_out = grid;
-  hipLaunchKernelGGL(k, dim3(grid), dim3(block), 0, stream, tb, wp);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(block), smem, stream, tb, wp);
   return hipGetLastError();
 }
 

@@ -1,0 +1,152 @@
+"""Autotuner for the persistent walk kernel's launch choices.
+
+Axes (binary32 persistent kernel; binary64 tunes the block only):
+  block           64, 128 or 256 lanes per block
+  waves_per_simd  0 = the compiler's register allocation (4 waves per SIMD),
+                  5, 6 or 8 (registers capped for that occupancy; 6 and 8
+                  spill to scratch)
+  lds_stack       16, 24 or 32 LDS traversal-stack entries per lane (16 and
+                  32 exist for waves_per_simd 0 and 5)
+  grid_kappa      distance-grid threshold in cell sizes, 1, 1.5, 2, 3, 4 or 6
+                  (mesh geometries with the distance grid on)
+
+The launch shape never changes a result: every path is keyed by (seed, site,
+side, walk) and each site's samples are folded in walk order.  grid_kappa
+changes paths (a different mix of grid and exact radii) but not the
+estimator's expectation, so every axis is scored by the same objective: walk
+kernel time (HIP events) per path, on a fixed-budget sample of the caller's
+sites.
+
+Search: coordinate descent from the defaults (256, 0, 24, 4), one axis at a
+time, twice over, each candidate the best of `repeats` timed runs; a
+candidate replaces the current choice only when it is at least 1% faster
+(run-to-run noise of the kernel time is below that).  Results
+are cached in memory per (device, geometry signature, precision, combine,
+eps, distance grid) and, when `cache_path` is given, in a JSON file.
+"""
+
+from __future__ import annotations
+
+import json
+import math
+import os
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+from . import _capi
+
+DEFAULTS = {"block": 256, "waves_per_simd": 0, "lds_stack": 24, "grid_kappa": 4.0}
+AXES = {
+    "block": (64, 128, 256),
+    "waves_per_simd": (0, 5, 6, 8),
+    "lds_stack": (16, 24, 32),
+    "grid_kappa": (1.0, 1.5, 2.0, 3.0, 4.0, 6.0),
+}
+
+_memory: dict[str, "Tuning"] = {}
+
+
+@dataclass
+class Tuning:
+    choice: dict
+    ms_per_path: float
+    default_ms_per_path: float
+    trials: list = field(default_factory=list)  # (config, ms per path)
+
+    @property
+    def speedup(self) -> float:
+        return self.default_ms_per_path / self.ms_per_path
+
+    def as_dict(self) -> dict:
+        return {"choice": self.choice, "ms_per_path": self.ms_per_path,
+                "default_ms_per_path": self.default_ms_per_path, "speedup": self.speedup,
+                "trials": [[c, t] for c, t in self.trials]}
+
+
+def _valid(cfg: dict, precision: int) -> bool:
+    if precision == 64:
+        return cfg["waves_per_simd"] == 0 and cfg["lds_stack"] == 24
+    if cfg["lds_stack"] != 24 and cfg["waves_per_simd"] not in (0, 5):
+        return False
+    return True
+
+
+def _key(ctx, geom_sig: str, precision: int, combine: int, eps: float, grid: bool) -> str:
+    return f"dev{ctx.device}|{geom_sig}|p{precision}|c{combine}|e{eps:g}|g{int(grid)}"
+
+
+def autotune(ctx, geom: int, sites, sides, *, geom_sig: str, combine: int, precision: int = 32,
+             eps: float = 1e-4, max_steps: int = 8192, distance_grid: bool = False, sample: int = 4096,
+             walks: int = 128, seed: int = 12345, repeats: int = 2, cache_path: str | os.PathLike | None = None,
+             log=None) -> Tuning:
+    """Choose the launch shape for estimates like the one described.
+
+    geom_sig names the geometry for the cache (for example the mesh's triangle
+    count and kind); sites/sides are representative of the calls to tune for
+    (a sample of min(len(sites), sample) sites, evenly spaced, runs `walks`
+    walks per side with no adaptivity)."""
+    key = _key(ctx, geom_sig, precision, combine, eps, distance_grid)
+    if key in _memory:
+        return _memory[key]
+    if cache_path is not None and Path(cache_path).exists():
+        stored = json.loads(Path(cache_path).read_text())
+        if key in stored:
+            d = stored[key]
+            t = Tuning(d["choice"], d["ms_per_path"], d["default_ms_per_path"], [tuple(x) for x in d["trials"]])
+            _memory[key] = t
+            return t
+    pts = np.asarray(sites, dtype=np.float64).reshape(-1, 3)
+    stride = max(1, len(pts) // sample)
+    pts = pts[::stride][:sample]
+    axes = dict(AXES)
+    if not distance_grid:
+        axes.pop("grid_kappa")
+    if precision == 64:
+        axes = {"block": AXES["block"]}
+    measured: dict[tuple, float] = {}
+
+    def cost(cfg: dict) -> float:
+        k = tuple(sorted(cfg.items()))
+        if k in measured:
+            return measured[k]
+        best = math.inf
+        for _ in range(repeats):
+            r = ctx.estimate(geom, pts, sides, combine=combine, precision=precision, seed=seed, walks=walks,
+                             eps=eps, max_steps=max_steps, distance_grid=distance_grid,
+                             block=cfg["block"], waves_per_simd=cfg["waves_per_simd"], lds_stack=cfg["lds_stack"],
+                             grid_kappa=cfg["grid_kappa"] if distance_grid else 0.0, raise_on_cap=False)
+            best = min(best, r.stats["walk_kernel_ms"] / max(1, r.stats["paths"]))
+        measured[k] = best
+        if log:
+            log(f"[autotune] {cfg} -> {best * 1e6:.2f} ns/path")
+        return best
+
+    cfg = dict(DEFAULTS)
+    default_cost = cost(cfg)
+    for _ in range(2):
+        changed = False
+        for axis, values in axes.items():
+            for v in values:
+                trial = {**cfg, axis: v}
+                if trial == cfg or not _valid(trial, precision):
+                    continue
+                if cost(trial) < cost(cfg) * 0.99:
+                    cfg = trial
+                    changed = True
+        if not changed:
+            break
+    t = Tuning(cfg, cost(cfg), default_cost, [(dict(k), v) for k, v in measured.items()])
+    _memory[key] = t
+    if cache_path is not None:
+        path = Path(cache_path)
+        stored = json.loads(path.read_text()) if path.exists() else {}
+        stored[key] = t.as_dict()
+        path.parent.mkdir(parents=True, exist_ok=True)
+        path.write_text(json.dumps(stored, indent=1) + "\n")
+    return t
+
+
+def clear_cache() -> None:
+    _memory.clear()
