@@ -44,7 +44,7 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="sh_icosphere",
-                    choices=["sh_icosphere", "cooling_block", "unit_ball", "serve"])
+                    choices=["sh_icosphere", "cooling_block", "unit_ball", "serve", "copy"])
     ap.add_argument("--requests", type=int, default=300, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=6.0, help="serve: Poisson arrivals per second")
     ap.add_argument("--plain-sites", type=int, default=512)
@@ -55,6 +55,10 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--waves-per-simd", type=int, default=-1,
                     help="persistent occupancy variant; default: the autotuner's choice")
     ap.add_argument("--no-autotune", action="store_true", help="default launch shape instead of the tuner's")
+    ap.add_argument("--block", type=int, default=0, help="fixed launch shape (skips the autotuner)")
+    ap.add_argument("--lds-stack", type=int, default=0, help="fixed launch shape (skips the autotuner)")
+    ap.add_argument("--grid-kappa", type=float, default=0.0, help="fixed launch shape (skips the autotuner)")
+    ap.add_argument("--grid-cells", type=int, default=0, help="fixed distance-grid resolution (skips the autotuner)")
     ap.add_argument("--exact", action="store_true", help="exact closest-point radius at every mesh step")
     ap.add_argument("--sites", type=int, default=0, help="override the site count (profiling runs)")
     return ap.parse_args()
@@ -84,6 +88,8 @@ def main() -> int:
 
     if args.workload == "serve":
         return serve(args)
+    if args.workload == "copy":
+        return copy_probe()
     w = build_workload(args.workload)
     sites = w.sites if args.sites <= 0 else w.sites[:args.sites]
     n = len(sites)
@@ -109,15 +115,20 @@ def main() -> int:
     # before warmup), unless fixed on the command line.
     shape = {}
     tuning = None
-    if args.waves_per_simd >= 0:
-        shape = {"waves_per_simd": args.waves_per_simd}
+    fixed = {k: v for k, v in (("waves_per_simd", args.waves_per_simd), ("block", args.block),
+                               ("lds_stack", args.lds_stack), ("grid_kappa", args.grid_kappa))
+             if (v >= 0 if k == "waves_per_simd" else v > 0)}
+    if fixed or args.grid_cells > 0:
+        shape = fixed
+        if args.grid_cells > 0 and mesh is not None:
+            ctx.set_distance_grid(mesh, args.grid_cells)
     elif not args.no_autotune:
         from twos import autotune
         t0 = time.perf_counter()
         tuning = autotune.autotune(ctx, geom, sites, sides, geom_sig=f"{w.name}", combine=twos.CROSS,
                                    precision=w.precision, eps=w.eps, max_steps=w.max_steps,
-                                   distance_grid=grid_on, log=log)
-        shape = {k: v for k, v in tuning.choice.items() if grid_on or k != "grid_kappa"}
+                                   distance_grid=grid_on, mesh=mesh, log=log)
+        shape = {k: v for k, v in tuning.choice.items() if k != "grid_cells" and (grid_on or k != "grid_kappa")}
         log(f"autotune {time.perf_counter() - t0:.1f} s: {tuning.choice}, "
             f"{tuning.speedup:.3f}x the default shape per path")
 
@@ -200,7 +211,10 @@ def main() -> int:
         "mean_walks_per_side": round(float(np.mean(res.walks)), 1),
         "rounds": res.stats["rounds"],
         "launch_shape": {"block": 256, "waves_per_simd": 0, "lds_stack": 24,
-                         **({"grid_kappa": 4.0} if grid_on else {}), **shape},
+                         **({"grid_kappa": 4.0, "grid_cells": 128} if grid_on else {}), **shape,
+                         **({"grid_cells": tuning.choice["grid_cells"]}
+                            if tuning is not None and "grid_cells" in tuning.choice else {}),
+                         **({"grid_cells": args.grid_cells} if grid_on and args.grid_cells > 0 else {})},
         "roofline": roofline,
     }
     if tuning is not None:
@@ -256,6 +270,20 @@ def serve(args) -> int:
     srv.close()
     ctx.close()
     print(json.dumps(out))
+    return 0
+
+
+def copy_probe() -> int:
+    """A plain device-to-device copy of 256 MiB, three times: the reference
+    reading of rocprofv3's FETCH_SIZE / WRITE_SIZE for bytes of known count."""
+    import torch
+    n = 256 << 20
+    a = torch.ones(n, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n, "copies": 3}))
     return 0
 
 

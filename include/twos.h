@@ -201,6 +201,11 @@ void twos_destroy(twos_ctx *ctx);
 twos_status twos_mesh_create(twos_ctx *ctx, const double *vertices, int64_t nv, const uint32_t *faces,
                              int64_t nf, int32_t *mesh_id, double *build_ms);
 twos_status twos_mesh_info(twos_ctx *ctx, int32_t mesh_id, int32_t *depth, int64_t *nodes, int64_t *bytes);
+/* Rebuilds the mesh's conservative distance grid with `cells` cells along the
+ * longest axis of its bounding box (8..512; meshes start at 128) and updates
+ * every geometry on the mesh.  No counterpart in the reference (Embree has no
+ * distance grid); it is a tuning control for TWOS_FLAG_DISTANCE_GRID. */
+twos_status twos_mesh_set_distance_grid(twos_ctx *ctx, int32_t mesh_id, int32_t cells, double *build_ms);
 twos_status twos_mesh_release(twos_ctx *ctx, int32_t mesh_id);
 
 twos_status twos_geometry_create(twos_ctx *ctx, const twos_geometry_desc *desc, int32_t *geom_id);

@@ -14,10 +14,17 @@ class FakeContext:
     def __init__(self, cost):
         self.cost = cost
         self.calls = 0
+        self.cells = 128
+        self.cell_history = []
+
+    def set_distance_grid(self, mesh, cells):
+        self.cells = cells
+        self.cell_history.append(cells)
+        return 0.0
 
     def estimate(self, geom, pts, sides, *, block, waves_per_simd, lds_stack, grid_kappa, **kw):
         self.calls += 1
-        ms = self.cost(block, waves_per_simd, lds_stack, grid_kappa)
+        ms = self.cost(block, waves_per_simd, lds_stack, grid_kappa) * (1.0 + 0.1 * abs(self.cells - 192) / 64)
 
         class R:
             stats = {"walk_kernel_ms": ms * 1000.0, "paths": 1000}
@@ -43,7 +50,7 @@ def test_gains_below_one_percent_keep_the_default():
     autotune.clear_cache()
     t = autotune.autotune(FakeContext(lambda b, w, l, k: 1.0 - 0.005 * (b == 64)), 0, np.zeros((4, 3)), None,
                           geom_sig="fake5", combine=2, repeats=1)
-    assert t.choice == {**autotune.DEFAULTS}
+    assert t.choice == {k: autotune.DEFAULTS[k] for k in ("block", "waves_per_simd", "lds_stack", "grid_kappa")}
 
 
 def test_invalid_variants_are_never_run():
@@ -80,3 +87,15 @@ def test_memory_cache_skips_trials():
     n = ctx.calls
     autotune.autotune(ctx, 0, np.zeros((4, 3)), None, geom_sig="fake4", combine=2, repeats=1)
     assert ctx.calls == n
+
+
+def test_grid_resolution_is_tuned_and_left_applied():
+    autotune.clear_cache()
+    ctx = FakeContext(lambda b, w, l, k: 1.0)
+    t = autotune.autotune(ctx, 0, np.zeros((4, 3)), None, geom_sig="fake6", combine=2, distance_grid=True,
+                          mesh=1, repeats=1)
+    assert t.choice["grid_cells"] == 192 and ctx.cells == 192
+    ctx.set_distance_grid(1, 96)
+    autotune.autotune(ctx, 0, np.zeros((4, 3)), None, geom_sig="fake6", combine=2, distance_grid=True,
+                      mesh=1, repeats=1)
+    assert ctx.cells == 192  # a cached choice is applied again

@@ -22,13 +22,13 @@ def env():
     ctx = twos.Context()
     ico = W.sh_icosphere(n_sites=512)
     mesh = ctx.mesh(ico.vertices, ico.faces)
-    yield ctx, ico, ctx.geometry(twos.MESH, mesh=mesh)
+    yield ctx, ico, ctx.geometry(twos.MESH, mesh=mesh), mesh
     ctx.close()
 
 
 @pytest.mark.parametrize("grid", [False, True])
 def test_launch_shape_never_changes_results(env, grid):
-    ctx, ico, g = env
+    ctx, ico, g, _ = env
     kw = dict(combine=twos.CROSS, precision=32, seed=3, walks=64, walks_max=256, ci_rel=0.05, eps=1e-4,
               max_steps=8192, distance_grid=grid)
     sides = W.sides(ico, twos.side)
@@ -44,12 +44,13 @@ def test_launch_shape_never_changes_results(env, grid):
 
 
 def test_autotune_choice_is_valid_and_cached(env, tmp_path):
-    ctx, ico, g = env
+    ctx, ico, g, mesh = env
     autotune.clear_cache()
     path = tmp_path / "tune.json"
     sides = W.sides(ico, twos.side)
     t = autotune.autotune(ctx, g, ico.sites, sides, geom_sig="sh_icosphere_512", combine=twos.CROSS,
-                          precision=32, eps=1e-4, distance_grid=True, sample=512, walks=64, cache_path=path)
+                          precision=32, eps=1e-4, distance_grid=True, mesh=mesh, sample=512, walks=64,
+                          cache_path=path)
     for axis, values in autotune.AXES.items():
         assert t.choice[axis] in values
     assert t.ms_per_path <= t.default_ms_per_path
@@ -57,5 +58,5 @@ def test_autotune_choice_is_valid_and_cached(env, tmp_path):
     assert len(stored) == 1 and next(iter(stored.values()))["choice"] == t.choice
     autotune.clear_cache()
     again = autotune.autotune(ctx, g, ico.sites, sides, geom_sig="sh_icosphere_512", combine=twos.CROSS,
-                              precision=32, eps=1e-4, distance_grid=True, cache_path=path)
+                              precision=32, eps=1e-4, distance_grid=True, mesh=mesh, cache_path=path)
     assert again.choice == t.choice  # read back from the file, no new trials

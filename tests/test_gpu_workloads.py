@@ -61,7 +61,7 @@ def test_c3_exact_walks_against_oracle(ctx, c3):
     assert np.sum(np.abs(z) > 3) <= 1 and np.max(np.abs(z)) < 5
 
 
-@pytest.mark.parametrize("kappa", [1.0, 2.0, 4.0])
+@pytest.mark.parametrize("kappa", [0.5, 1.0, 2.0, 4.0])
 def test_c3_distance_grid_against_exact(ctx, c3, kappa):
     """The conservative-radius walk estimates the same quantity as the exact
     walk (independent seeds), at every threshold the autotuner may choose."""
@@ -73,6 +73,27 @@ def test_c3_distance_grid_against_exact(ctx, c3, kappa):
     z = (grid.td - exact.td) / np.hypot(grid.score_se, exact.score_se)
     assert np.sum(np.abs(z) > 3) <= 2 and np.max(np.abs(z)) < 5
     assert abs(np.mean(z)) < 0.35  # no systematic shift (128 sites: sd of the mean ~0.09)
+
+
+def test_c3_finer_distance_grid_against_exact(ctx, c3):
+    """A rebuilt 256-cell grid changes the paths and still estimates the
+    exact walk's quantity; the mesh is restored to 128 cells afterwards."""
+    w, mesh, g = c3
+    sites = w.sites[128:256]
+    kw = dict(combine=twos.CROSS, precision=32, walks=4096, eps=w.eps, max_steps=w.max_steps)
+    coarse = ctx.estimate(g, sites, W.sides(w, twos.side), seed=3, distance_grid=True, grid_kappa=2.0, **kw)
+    try:
+        ctx.set_distance_grid(mesh, 256)
+        fine = ctx.estimate(g, sites, W.sides(w, twos.side), seed=3, distance_grid=True, grid_kappa=2.0, **kw)
+    finally:
+        ctx.set_distance_grid(mesh, 128)
+    exact = ctx.estimate(g, sites, W.sides(w, twos.side), seed=4, **kw)
+    assert not np.array_equal(fine.steps, coarse.steps)
+    z = (fine.td - exact.td) / np.hypot(fine.score_se, exact.score_se)
+    assert np.sum(np.abs(z) > 3) <= 2 and np.max(np.abs(z)) < 5
+    assert abs(np.mean(z)) < 0.35
+    again = ctx.estimate(g, sites, W.sides(w, twos.side), seed=3, distance_grid=True, grid_kappa=2.0, **kw)
+    np.testing.assert_array_equal(again.steps, coarse.steps)
 
 
 def test_c3_distance_grid_plain_equals_persistent(ctx, c3):

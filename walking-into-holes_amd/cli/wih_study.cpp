@@ -1,0 +1,222 @@
+// wih_study: drop-in for the reference's paired-estimator executable
+// (benchmarks/submission_study/study.cpp:272-381) running on an MI355X
+// through the C-ABI.  One request = one command line: the wih_twos flags plus
+// --config (sensor line, then heat bumps), --hot-only, --fd, --diff and
+// --sensor-side.  Output site,u,p,score,same_index,steps,caps and a one-line
+// JSON summary, as the reference writes them.  `wih_study --serve` reads
+// length-prefixed argument lists from stdin (a line with the count n, then
+// n lines, the first being the program name) and answers each with
+// "DONE <code>", keeping the device context, the mesh's LBVH and its
+// geometries resident across requests; a session keeps one outer mesh.
+//
+// Mode twos only: the reference's multigrid solver (--mode mg), finite
+// differences (--fd 1) and Differential WoS (--diff 1) are other solvers; a
+// request asking for them fails with status 1.
+#include <chrono>
+#include <cmath>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "wih_io.h"
+#include "wih_options.h"
+
+namespace {
+
+struct Session {
+  std::unique_ptr<wih::Context> ctx;
+  std::string mesh_path;
+  int32_t mesh_id = -1;
+};
+
+// A request's geometry (outer mesh minus its cavities), released at the end
+// of the request; the mesh and its LBVH stay resident for the session.
+class Geometry {
+ public:
+  Geometry(Session &s, const std::vector<double> &holes, double radius) : ctx_(s.ctx->get()) {
+    twos_geometry_desc d{};
+    d.container = TWOS_MESH;
+    d.mesh = s.mesh_id;
+    d.n_cavities = int32_t(holes.size() / 3);
+    d.cavities = holes.empty() ? nullptr : holes.data();
+    d.cavity_radius = radius;
+    wih::check(twos_geometry_create(ctx_, &d, &id_), "geometry");
+  }
+  ~Geometry() {
+    if (id_ >= 0) twos_geometry_release(ctx_, id_);
+  }
+  Geometry(const Geometry &) = delete;
+  Geometry &operator=(const Geometry &) = delete;
+  int32_t id() const { return id_; }
+
+ private:
+  twos_ctx *ctx_;
+  int32_t id_ = -1;
+};
+
+// Sensor (cx cy cz R) then heat bumps (cx cy cz R A) (study.cpp:37-43): the
+// heat side sums A (1 - |a-c|^2/R^2)_+^2, the sensor side is the unit-mass
+// bump t^2 / (32 pi R^3 / 105) (:44-51).
+void load_config(const std::string &path, twos_side &heat, twos_side &sensor) {
+  std::ifstream f(path);
+  double c[4];
+  if (!(f >> c[0] >> c[1] >> c[2] >> c[3])) throw std::runtime_error("bad sensor config");
+  heat = twos_side{};
+  sensor = twos_side{};
+  heat.source_kind = 2;
+  double row[5];
+  while (f >> row[0] >> row[1] >> row[2] >> row[3] >> row[4]) {
+    if (heat.n_terms == TWOS_MAX_TERMS) throw std::runtime_error("at most 8 heat bumps");
+    double *t = heat.terms[heat.n_terms++];
+    t[0] = TWOS_TERM_BUMP;
+    t[1] = row[0]; t[2] = row[1]; t[3] = row[2]; t[4] = row[3]; t[7] = row[4];
+  }
+  if (heat.n_terms == 0) throw std::runtime_error("missing compact heat sources");
+  sensor.source_kind = 2;
+  sensor.n_terms = 1;
+  double *t = sensor.terms[0];
+  t[0] = TWOS_TERM_UNIT;
+  t[1] = c[0]; t[2] = c[1]; t[3] = c[2]; t[4] = c[3];
+  t[5] = 32 * 3.141592653589793 * std::pow(c[3], 3) / 105;
+}
+
+bool inside_cavity(const double *p, const std::vector<double> &holes, double radius) {
+  for (size_t k = 0; k + 2 < holes.size(); k += 3) {
+    const double dx = p[0] - holes[k], dy = p[1] - holes[k + 1], dz = p[2] - holes[k + 2];
+    if (std::sqrt(dx * dx + dy * dy + dz * dz) <= radius) return true;
+  }
+  return false;
+}
+
+int process_request(int argc, char **argv, Session &session) {
+  try {
+    bool hot = false, fd = false, diff = false;
+    std::string config;
+    std::vector<char *> args{argv[0]};
+    for (int i = 1; i < argc; ++i) {
+      const std::string key = argv[i];
+      if (key == "--config") {
+        if (i + 1 == argc) throw std::runtime_error("missing config");
+        config = argv[++i];
+      } else if (key == "--fd") {
+        if (i + 1 == argc) throw std::runtime_error("missing fd flag");
+        fd = std::stoi(argv[++i]) != 0;
+      } else if (key == "--diff") {
+        if (i + 1 == argc) throw std::runtime_error("missing diff flag");
+        diff = std::stoi(argv[++i]) != 0;
+      } else if (key == "--sensor-side") {
+        if (i + 1 == argc) throw std::runtime_error("missing sensor side");
+        const int side = std::stoi(argv[++i]);
+        if (side != 1 && side != -1) throw std::runtime_error("sensor side must be -1 or 1");
+      } else if (key == "--hot-only") {
+        if (i + 1 == argc) throw std::runtime_error("missing hot flag");
+        hot = std::stoi(argv[++i]) != 0;
+      } else {
+        args.push_back(argv[i]);
+      }
+    }
+    const wih::Options o = wih::parse_options(int(args.size()), args.data(), "wih_study");
+    twos_side heat, sensor;
+    load_config(config, heat, sensor);
+    if (o.mode != "twos" && o.mode != "mg") throw std::runtime_error("use twos or mg");
+    if (o.mode == "mg") throw std::runtime_error("mode mg is not provided by this build (twos only)");
+    if (fd) throw std::runtime_error("--fd 1 is not provided by this build");
+    if (diff) throw std::runtime_error("--diff 1 is not provided by this build");
+    const auto start = std::chrono::steady_clock::now();
+    if (!session.ctx) {
+      const wih::PlyMesh mesh = wih::load_ascii_ply(o.mesh_path);
+      session.ctx = std::make_unique<wih::Context>();
+      double ms = 0.0;
+      wih::check(twos_mesh_create(session.ctx->get(), mesh.vertices.data(), int64_t(mesh.vertex_count()),
+                                  mesh.faces.data(), int64_t(mesh.face_count()), &session.mesh_id, &ms),
+                 "mesh");
+      session.mesh_path = o.mesh_path;
+    }
+    if (session.mesh_path != o.mesh_path) throw std::runtime_error("a session must retain the same outer mesh");
+    const std::vector<double> points = wih::read_points(o.points_path);
+    const std::vector<double> holes = wih::read_optional_points(o.cavities_path);
+    const Geometry geometry(session, holes, o.cavity_radius);
+    const int32_t geom = geometry.id();
+    // Sites inside a cavity are skipped and report zeros (study.cpp:341-342).
+    const size_t n = points.size() / 3;
+    std::vector<size_t> live;
+    std::vector<double> live_pts;
+    for (size_t i = 0; i < n; ++i)
+      if (!inside_cavity(&points[3 * i], holes, o.cavity_radius)) {
+        live.push_back(i);
+        live_pts.insert(live_pts.end(), &points[3 * i], &points[3 * i] + 3);
+      }
+    std::vector<twos_site_result> res(live.size());
+    twos_stats stats{};
+    if (!live.empty()) {
+      const twos_side sides[2] = {heat, hot ? twos_side{} : sensor};
+      const twos_params p = wih::make_params(o, TWOS_PAIRED, TWOS_RNG_MT_PAIRED, 0);
+      const twos_status st = twos_estimate(session.ctx->get(), geom, sides, &p, live_pts.data(),
+                                           int64_t(live.size()), res.data(), &stats);
+      if (st != TWOS_OK && st != TWOS_ERR_STEP_CAP) wih::check(st, "estimate");
+    }
+    const double seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+    std::vector<twos_site_result> all(n);
+    for (size_t k = 0; k < live.size(); ++k) all[live[k]] = res[k];
+    uint64_t steps = 0, caps = 0, paths = 0;
+    std::ofstream f(o.output_path);
+    if (!f) throw std::runtime_error("cannot write field output: " + o.output_path);
+    f << "site,u,p,score,same_index,steps,caps\n" << std::setprecision(17);
+    for (size_t i = 0; i < n; ++i) {
+      const twos_site_result &r = all[i];
+      steps += uint64_t(r.steps);
+      caps += uint64_t(r.caps);
+      paths += uint64_t(r.walks);
+      // Mean-only mode reports the primal mean as its score (study.cpp:117).
+      const double score = hot ? r.u : r.score;
+      f << i << ',' << r.u << ',' << (hot ? 0.0 : r.v) << ',' << score << ',' << (hot ? 0.0 : r.same) << ','
+        << r.steps << ',' << r.caps << '\n';
+    }
+    f.close();
+    if (caps) throw std::runtime_error("walk step limit reached");
+    // One closest-point query per step plus the terminating one of each
+    // uncapped path.
+    const uint64_t queries = steps + paths - caps;
+    std::ofstream s(o.summary_path);
+    s << std::setprecision(17) << "{\"total_seconds\":" << seconds
+      << ",\"peak_rss_megabytes\":" << wih::peak_rss_megabytes() << ",\"distance_queries\":" << queries
+      << ",\"parity_rays\":0,\"enclosure_fallbacks\":0,\"relative_residual\":0,\"steps\":" << steps
+      << ",\"caps\":" << caps << "}\n";
+    return 0;
+  } catch (const std::exception &e) {
+    std::cerr << e.what() << '\n';
+    return 1;
+  }
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  Session session;
+  if (argc == 2 && std::string(argv[1]) == "--serve") {
+    std::string line;
+    while (std::getline(std::cin, line)) {
+      int n = 0;
+      try {
+        n = std::stoi(line);
+      } catch (...) {
+        return 2;
+      }
+      if (n <= 0 || n > 200) return 2;
+      std::vector<std::string> values(static_cast<size_t>(n));
+      for (auto &value : values)
+        if (!std::getline(std::cin, value)) return 2;
+      std::vector<char *> args;
+      for (auto &value : values) args.push_back(value.data());
+      const int code = process_request(n, args.data(), session);
+      std::cout << "DONE " << code << std::endl;
+      if (code) return code;
+    }
+    return 0;
+  }
+  return process_request(argc, argv, session);
+}

@@ -1,0 +1,181 @@
+// wih_twos: drop-in for the reference's compiled backend
+// (benchmarks/compiled_bunny/src/wih_twos.cpp:1799-1864) running the
+// estimator on an MI355X through the C-ABI.  Same flags, same files in and
+// out, same exit codes: usage errors exit 2, runtime errors print
+// "fatal: <what>" and exit 1, a path that hits --max-steps is fatal.
+//
+// Mode twos only (the reference's voxel, mg, diffwos and validate modes are
+// other solvers).  By default the walks replay the reference's own streams
+// (mt19937_64 per walk keyed as evaluate_site keys them, binary64), so the
+// field file matches the reference's; --streams philox selects the
+// production path (Philox streams, binary32 walks, optional distance grid and
+// adaptive budgets).
+#include <algorithm>
+#include <chrono>
+#include <cstdlib>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "wih_io.h"
+#include "wih_options.h"
+
+namespace {
+
+using wih::Options;
+
+// The problem's single side: zero Dirichlet data and the source of
+// --source-mode (wih_twos.cpp:546-567).
+twos_side make_side(const Options &o) {
+  twos_side s{};
+  if (o.source_mode == "affine") {
+    s.source_kind = 1;
+    s.src[0] = 1.0;
+    s.src[2] = o.source_y_slope;
+    s.src[5] = 0.5;
+  } else {
+    s.source_kind = 2;
+    s.src[0] = 0.08;
+    const double rows[3][8] = {{TWOS_TERM_GAUSS, -0.48, -0.18, -0.08, 0.24, 0.22, 0.16, 5.2},
+                               {TWOS_TERM_GAUSS, 0.43, 0.19, -0.07, 0.21, 0.20, 0.15, 4.6},
+                               {TWOS_TERM_GAUSS, 0.02, -0.31, -0.02, 0.30, 0.16, 0.19, 2.0}};
+    s.n_terms = 3;
+    for (int k = 0; k < 3; ++k)
+      for (int c = 0; c < 8; ++c) s.terms[k][c] = rows[k][c];
+  }
+  return s;
+}
+
+void write_field(const std::string &path, const std::vector<double> &points,
+                 const std::vector<twos_site_result> &r) {
+  std::ofstream out(path);
+  if (!out) throw std::runtime_error("cannot write field output: " + path);
+  out << "site,x,y,z,u,score,score_se,steps,limit_hits\n" << std::setprecision(17);
+  for (size_t i = 0; i < r.size(); ++i)
+    out << i << ',' << points[3 * i] << ',' << points[3 * i + 1] << ',' << points[3 * i + 2] << ',' << r[i].u << ','
+        << r[i].score << ',' << r[i].score_se << ',' << r[i].steps << ',' << r[i].caps << '\n';
+}
+
+// write_trace (wih_twos.cpp:1725-1744): trace_sites evenly spaced sites, each
+// one path keyed by seed + 900000 + site, rows (position, domain distance).
+void write_trace(const std::string &path, twos_ctx *ctx, int32_t geom, const twos_side &side,
+                 const std::vector<double> &points, const Options &o) {
+  if (path.empty() || o.trace_sites == 0) return;
+  std::ofstream out(path);
+  if (!out) throw std::runtime_error("cannot write trace output: " + path);
+  out << "path,step,x,y,z,radius\n" << std::setprecision(17);
+  const size_t n = points.size() / 3;
+  const int count = std::min<int>(o.trace_sites, int(n));
+  twos_params p = wih::make_params(o, TWOS_SQUARE, TWOS_RNG_MT_SITE, 0);
+  p.rng = o.streams == "reference" ? TWOS_RNG_MT_KEY : TWOS_RNG_PHILOX;
+  p.seed = o.streams == "reference" ? o.seed + 900000ULL : o.seed;
+  std::vector<double> rows(4 * size_t(o.max_steps + 2));
+  for (int i = 0; i < count; ++i) {
+    const size_t site = size_t(i) * n / size_t(count);
+    int32_t nrows = 0;
+    wih::check(twos_trace(ctx, geom, &side, &p, &points[3 * site], int64_t(site), 0, rows.data(),
+                          int32_t(rows.size() / 4), &nrows),
+               "trace");
+    for (int32_t k = 0; k < nrows; ++k)
+      out << i << ',' << k << ',' << rows[4 * k] << ',' << rows[4 * k + 1] << ',' << rows[4 * k + 2] << ','
+          << rows[4 * k + 3] << '\n';
+  }
+}
+
+void write_summary(const std::string &path, const wih::PlyMesh &mesh, size_t n_cavities, const Options &o,
+                   size_t sites, double build_s, const std::vector<double> &times, uint64_t steps,
+                   uint64_t limits, double walks_per_site) {
+  const double runtime = wih::median(times);
+  const double paths = double(sites) * walks_per_site;
+  std::ofstream out(path);
+  if (!out) throw std::runtime_error("cannot write summary: " + path);
+  out << std::setprecision(17) << "{\n"
+      << "  \"engine\": \"MI355X HIP LBVH\",\n"
+      << "  \"vertices\": " << mesh.vertex_count() << ",\n"
+      << "  \"faces\": " << mesh.face_count() << ",\n"
+      << "  \"source_mode\": \"" << o.source_mode << "\",\n"
+      << "  \"cavities\": " << n_cavities << ",\n"
+      << "  \"cavity_radius\": " << o.cavity_radius << ",\n"
+      << "  \"sites\": " << sites << ",\n"
+      << "  \"walks_per_site\": " << walks_per_site << ",\n"
+      << "  \"repeats\": " << o.repeats << ",\n"
+      << "  \"threads\": " << o.threads << ",\n"
+      << "  \"epsilon\": " << o.epsilon << ",\n"
+      << "  \"mesh_build_seconds\": " << build_s << ",\n"
+      << "  \"runtime_seconds_median\": " << runtime << ",\n"
+      << "  \"runtime_seconds_min\": " << *std::min_element(times.begin(), times.end()) << ",\n"
+      << "  \"paths_per_second\": " << paths / runtime << ",\n"
+      << "  \"distance_queries_per_second\": " << (double(steps) + paths) / runtime << ",\n"
+      << "  \"mean_steps\": " << double(steps) / paths << ",\n"
+      << "  \"max_step_limit_hits\": " << limits << ",\n"
+      << "  \"mesh_input_megabytes\": " << double(mesh.input_bytes()) / (1024.0 * 1024.0) << ",\n"
+      << "  \"peak_rss_megabytes\": " << wih::peak_rss_megabytes() << ",\n"
+      << "  \"streams\": \"" << o.streams << "\",\n"
+      << "  \"timings_seconds\": [";
+  for (size_t i = 0; i < times.size(); ++i) out << (i ? ", " : "") << times[i];
+  out << "]\n}\n";
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  try {
+    const Options o = wih::parse_options(argc, argv, "wih_twos");
+    if (o.mode != "twos") wih::usage("wih_twos", "mode " + o.mode + " is not provided by this build (twos only)");
+    const wih::PlyMesh mesh = wih::load_ascii_ply(o.mesh_path);
+    const std::vector<double> points = wih::read_points(o.points_path);
+    const std::vector<double> cavities = wih::read_optional_points(o.cavities_path);
+    const size_t n = points.size() / 3;
+    wih::Context ctx;
+    int32_t mesh_id = -1, geom = -1;
+    double build_ms = 0.0;
+    wih::check(twos_mesh_create(ctx.get(), mesh.vertices.data(), int64_t(mesh.vertex_count()), mesh.faces.data(),
+                                int64_t(mesh.face_count()), &mesh_id, &build_ms),
+               "mesh");
+    twos_geometry_desc d{};
+    d.container = TWOS_MESH;
+    d.mesh = mesh_id;
+    d.mesh_radius_f32 = o.binary32_radius;
+    d.n_cavities = int32_t(cavities.size() / 3);
+    d.cavities = cavities.empty() ? nullptr : cavities.data();
+    d.cavity_radius = o.cavity_radius;
+    wih::check(twos_geometry_create(ctx.get(), &d, &geom), "geometry");
+    const twos_side side = make_side(o);
+    const twos_side sides[2] = {side, side};
+    std::vector<double> times;
+    std::vector<twos_site_result> retained(n);
+    for (int repeat = 0; repeat < o.repeats; ++repeat) {
+      std::vector<twos_site_result> results(n);
+      const twos_params p = wih::make_params(o, TWOS_SQUARE, TWOS_RNG_MT_SITE, repeat);
+      const auto start = std::chrono::steady_clock::now();
+      const twos_status st = twos_estimate(ctx.get(), geom, sides, &p, points.data(), int64_t(n), results.data(),
+                                           nullptr);
+      times.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count());
+      if (st != TWOS_OK && st != TWOS_ERR_STEP_CAP) wih::check(st, "estimate");
+      if (repeat == 0) retained = std::move(results);
+    }
+    uint64_t steps = 0, limits = 0;
+    double walks = 0.0;
+    for (const auto &r : retained) {
+      steps += uint64_t(r.steps);
+      limits += uint64_t(r.caps);
+      walks += double(r.walks);
+    }
+    if (limits != 0) throw std::runtime_error("one or more paths hit --max-steps");
+    write_field(o.output_path, points, retained);
+    write_trace(o.trace_path, ctx.get(), geom, side, points, o);
+    const double walks_per_site = n ? walks / double(n) : 0.0;
+    write_summary(o.summary_path, mesh, cavities.size() / 3, o, n, build_ms * 1e-3, times, steps, limits,
+                  walks_per_site);
+    std::cout << "MI355X TWoS: " << n << " sites x " << walks_per_site << " walks, median " << wih::median(times)
+              << " s, " << std::fixed << std::setprecision(0) << (double(n) * walks_per_site / wih::median(times))
+              << " paths/s\n";
+    return 0;
+  } catch (const std::exception &e) {
+    std::cerr << "fatal: " << e.what() << '\n';
+    return 1;
+  }
+}

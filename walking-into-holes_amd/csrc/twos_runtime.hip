@@ -67,6 +67,8 @@ constexpr float kGridKappa = 4.0f;
 
 struct MeshEntry {
   MeshBuffers mb;
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};  // vertex bounds (distance-grid extent)
+  int32_t grid_cells = 0;
   int32_t depth = 0;
   int64_t bytes = 0;
   double build_ms = 0.0;
@@ -1011,6 +1013,8 @@ twos_status twos_mesh_create(twos_ctx *ctx, const double *vertices, int64_t nv, 
         hi[k] = std::max(hi[k], vertices[3 * i + k]);
       }
     e = build_distance_grid(m.mb, lo, hi, kGridCells, ctx->stream);
+    for (int k = 0; k < 3; ++k) { m.lo[k] = lo[k]; m.hi[k] = hi[k]; }
+    m.grid_cells = kGridCells;
   }
   (void)hipEventRecord(b, ctx->stream);
   (void)hipStreamSynchronize(ctx->stream);
@@ -1033,6 +1037,52 @@ twos_status twos_mesh_create(twos_ctx *ctx, const double *vertices, int64_t nv, 
   *mesh_id = id;
   if (build_ms) *build_ms = ms;
   return TWOS_OK;
+}
+
+twos_status twos_mesh_set_distance_grid(twos_ctx *ctx, int32_t mesh_id, int32_t cells, double *build_ms) {
+  if (!ctx) return fail(TWOS_ERR_INVALID_ARGUMENT, "null context");
+  if (cells < 8 || cells > 512) return fail(TWOS_ERR_INVALID_ARGUMENT, "cells must be in [8, 512]");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  auto it = ctx->meshes.find(mesh_id);
+  if (it == ctx->meshes.end()) return fail(TWOS_ERR_NOT_FOUND, "unknown mesh id");
+  MeshEntry &m = it->second;
+  HIP_OK(hipSetDevice(ctx->device));
+  // Work already queued may read the old grid: finish it before the swap.
+  HIP_OK(hipStreamSynchronize(ctx->stream));
+  MeshBuffers nb = m.mb;
+  nb.grid = nullptr;
+  hipEvent_t a, b;
+  HIP_OK(hipEventCreate(&a));
+  HIP_OK(hipEventCreate(&b));
+  HIP_OK(hipEventRecord(a, ctx->stream));
+  const hipError_t e = build_distance_grid(nb, m.lo, m.hi, cells, ctx->stream);
+  (void)hipEventRecord(b, ctx->stream);
+  (void)hipStreamSynchronize(ctx->stream);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, a, b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (e != hipSuccess) {
+    if (nb.grid) (void)hipFree(nb.grid);
+    return fail(TWOS_ERR_DEVICE, std::string("distance grid: ") + hipGetErrorString(e));
+  }
+  (void)hipFree(m.mb.grid);
+  m.mb = nb;
+  m.grid_cells = cells;
+  for (auto &kv : ctx->geoms) {
+    GeomEntry &g = kv.second;
+    if (g.desc.container != kMesh || g.desc.mesh != mesh_id) continue;
+    g.dev.grid = nb.grid;
+    for (int k = 0; k < 3; ++k) {
+      g.dev.grid_lo[k] = nb.grid_lo[k];
+      g.dev.grid_n[k] = nb.grid_n[k];
+    }
+    g.dev.grid_h = nb.grid_h;
+    g.dev.grid_inv_h = nb.grid_h > 0.f ? 1.0f / nb.grid_h : 0.f;
+  }
+  if (ctx->engine) ctx->engine->graph_key = -1;
+  if (build_ms) *build_ms = ms;
+  return upload_geom_table(ctx);
 }
 
 twos_status twos_mesh_info(twos_ctx *ctx, int32_t mesh_id, int32_t *depth, int64_t *nodes, int64_t *bytes) {

@@ -275,6 +275,14 @@ class Context:
         _check(_lib().twos_mesh_info(self._h, mid.value, C.byref(depth), C.byref(nodes), C.byref(nbytes)))
         return Mesh(mid.value, len(v), len(f), ms.value, depth.value, nbytes.value)
 
+    def set_distance_grid(self, mesh: Mesh | int, cells: int) -> float:
+        """Rebuild the mesh's distance grid with `cells` cells along its longest
+        axis (8..512); returns the build time in ms."""
+        ms = C.c_double()
+        _check(_lib().twos_mesh_set_distance_grid(self._h, mesh.id if isinstance(mesh, Mesh) else int(mesh),
+                                                  int(cells), C.byref(ms)))
+        return ms.value
+
     def geometry(self, container: int, *, mesh: Mesh | int | None = None, ball=(0.0, 0.0, 0.0, 1.0),
                  box_lo=(0.0, 0.0, 0.0), box_hi=(1.0, 1.0, 1.0), spheres=None, cylinders=None, cavities=None,
                  cavity_radius: float = 0.0, mesh_radius_f32: bool = False) -> int:

@@ -84,6 +84,7 @@ SIGNATURES = {
     "twos_destroy": (None, [_ctx_p]),
     "twos_mesh_create": (C.c_int32, [_ctx_p, _dp, C.c_int64, _u32p, C.c_int64, _i32p, _dp]),
     "twos_mesh_info": (C.c_int32, [_ctx_p, C.c_int32, _i32p, _i64p, _i64p]),
+    "twos_mesh_set_distance_grid": (C.c_int32, [_ctx_p, C.c_int32, C.c_int32, _dp]),
     "twos_mesh_release": (C.c_int32, [_ctx_p, C.c_int32]),
     "twos_geometry_create": (C.c_int32, [_ctx_p, C.POINTER(GeometryDesc), _i32p]),
     "twos_geometry_union_sphere": (C.c_int32, [_ctx_p, C.c_int32, _dp, C.c_double, _i32p]),

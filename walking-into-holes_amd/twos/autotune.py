@@ -7,8 +7,11 @@ Axes (binary32 persistent kernel; binary64 tunes the block only):
                   spill to scratch)
   lds_stack       16, 24 or 32 LDS traversal-stack entries per lane (16 and
                   32 exist for waves_per_simd 0 and 5)
-  grid_kappa      distance-grid threshold in cell sizes, 1, 1.5, 2, 3, 4 or 6
-                  (mesh geometries with the distance grid on)
+  grid_kappa      distance-grid threshold in cell sizes, 0.5 to 6 (mesh
+                  geometries with the distance grid on)
+  grid_cells      distance-grid resolution, 96 to 384 cells along
+                  the mesh's longest axis (when the mesh handle is given; the
+                  mesh keeps the chosen grid)
 
 The launch shape never changes a result: every path is keyed by (seed, site,
 side, walk) and each site's samples are folded in walk order.  grid_kappa
@@ -37,12 +40,13 @@ import numpy as np
 
 from . import _capi
 
-DEFAULTS = {"block": 256, "waves_per_simd": 0, "lds_stack": 24, "grid_kappa": 4.0}
+DEFAULTS = {"block": 256, "waves_per_simd": 0, "lds_stack": 24, "grid_kappa": 4.0, "grid_cells": 128}
 AXES = {
     "block": (64, 128, 256),
     "waves_per_simd": (0, 5, 6, 8),
     "lds_stack": (16, 24, 32),
-    "grid_kappa": (1.0, 1.5, 2.0, 3.0, 4.0, 6.0),
+    "grid_kappa": (0.5, 0.75, 1.0, 1.5, 2.0, 3.0, 4.0, 6.0),
+    "grid_cells": (96, 128, 192, 256, 320, 384),
 }
 
 _memory: dict[str, "Tuning"] = {}
@@ -78,39 +82,54 @@ def _key(ctx, geom_sig: str, precision: int, combine: int, eps: float, grid: boo
 
 
 def autotune(ctx, geom: int, sites, sides, *, geom_sig: str, combine: int, precision: int = 32,
-             eps: float = 1e-4, max_steps: int = 8192, distance_grid: bool = False, sample: int = 4096,
-             walks: int = 128, seed: int = 12345, repeats: int = 2, cache_path: str | os.PathLike | None = None,
-             log=None) -> Tuning:
+             eps: float = 1e-4, max_steps: int = 8192, distance_grid: bool = False, mesh=None,
+             sample: int = 4096, walks: int = 128, seed: int = 12345, repeats: int = 2,
+             cache_path: str | os.PathLike | None = None, log=None) -> Tuning:
     """Choose the launch shape for estimates like the one described.
 
     geom_sig names the geometry for the cache (for example the mesh's triangle
     count and kind); sites/sides are representative of the calls to tune for
     (a sample of min(len(sites), sample) sites, evenly spaced, runs `walks`
-    walks per side with no adaptivity)."""
-    key = _key(ctx, geom_sig, precision, combine, eps, distance_grid)
+    walks per side with no adaptivity).  With `mesh` (the geometry's mesh
+    handle) and the distance grid on, the grid resolution is tuned too and
+    left at the chosen value."""
+    key = _key(ctx, geom_sig, precision, combine, eps, distance_grid) + ("|m" if mesh is not None else "")
+    tune_cells = mesh is not None and distance_grid
+
+    def applied(t: "Tuning") -> "Tuning":
+        if tune_cells:
+            ctx.set_distance_grid(mesh, t.choice["grid_cells"])
+        return t
+
     if key in _memory:
-        return _memory[key]
+        return applied(_memory[key])
     if cache_path is not None and Path(cache_path).exists():
         stored = json.loads(Path(cache_path).read_text())
         if key in stored:
             d = stored[key]
             t = Tuning(d["choice"], d["ms_per_path"], d["default_ms_per_path"], [tuple(x) for x in d["trials"]])
             _memory[key] = t
-            return t
+            return applied(t)
     pts = np.asarray(sites, dtype=np.float64).reshape(-1, 3)
     stride = max(1, len(pts) // sample)
     pts = pts[::stride][:sample]
     axes = dict(AXES)
     if not distance_grid:
         axes.pop("grid_kappa")
+    if not tune_cells:
+        axes.pop("grid_cells")
     if precision == 64:
-        axes = {"block": AXES["block"]}
+        axes = {k: v for k, v in axes.items() if k in ("block", "grid_kappa", "grid_cells")}
     measured: dict[tuple, float] = {}
+    cells_now = [None]
 
     def cost(cfg: dict) -> float:
         k = tuple(sorted(cfg.items()))
         if k in measured:
             return measured[k]
+        if tune_cells and cfg["grid_cells"] != cells_now[0]:
+            ctx.set_distance_grid(mesh, cfg["grid_cells"])
+            cells_now[0] = cfg["grid_cells"]
         best = math.inf
         for _ in range(repeats):
             r = ctx.estimate(geom, pts, sides, combine=combine, precision=precision, seed=seed, walks=walks,
@@ -123,7 +142,8 @@ def autotune(ctx, geom: int, sites, sides, *, geom_sig: str, combine: int, preci
             log(f"[autotune] {cfg} -> {best * 1e6:.2f} ns/path")
         return best
 
-    cfg = dict(DEFAULTS)
+    cfg = {k: v for k, v in DEFAULTS.items() if k in axes or k in ("block", "waves_per_simd", "lds_stack",
+                                                                   "grid_kappa")}
     default_cost = cost(cfg)
     for _ in range(2):
         changed = False
@@ -138,6 +158,7 @@ def autotune(ctx, geom: int, sites, sides, *, geom_sig: str, combine: int, preci
         if not changed:
             break
     t = Tuning(cfg, cost(cfg), default_cost, [(dict(k), v) for k, v in measured.items()])
+    applied(t)
     _memory[key] = t
     if cache_path is not None:
         path = Path(cache_path)
