@@ -36,6 +36,7 @@ FP64_VALU_PEAK = 78.6e12
 FP32_VALU_PEAK = 157.3e12
 NODE_BYTES = 128  # one 4-wide node: four child boxes + four child links
 TRI_BYTES = 36    # nine binary32 coordinates per triangle test
+GRID_BYTES = 4    # one binary32 distance per distance-grid lookup
 
 
 def parse() -> argparse.Namespace:
@@ -59,6 +60,7 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--lds-stack", type=int, default=0, help="fixed launch shape (skips the autotuner)")
     ap.add_argument("--grid-kappa", type=float, default=0.0, help="fixed launch shape (skips the autotuner)")
     ap.add_argument("--grid-cells", type=int, default=0, help="fixed distance-grid resolution (skips the autotuner)")
+    ap.add_argument("--grid-corners", type=int, default=-1, help="fixed distance-grid sampling: 0 centres, 1 corners")
     ap.add_argument("--exact", action="store_true", help="exact closest-point radius at every mesh step")
     ap.add_argument("--sites", type=int, default=0, help="override the site count (profiling runs)")
     return ap.parse_args()
@@ -118,17 +120,18 @@ def main() -> int:
     fixed = {k: v for k, v in (("waves_per_simd", args.waves_per_simd), ("block", args.block),
                                ("lds_stack", args.lds_stack), ("grid_kappa", args.grid_kappa))
              if (v >= 0 if k == "waves_per_simd" else v > 0)}
-    if fixed or args.grid_cells > 0:
+    if fixed or args.grid_cells > 0 or args.grid_corners >= 0:
         shape = fixed
-        if args.grid_cells > 0 and mesh is not None:
-            ctx.set_distance_grid(mesh, args.grid_cells)
+        if (args.grid_cells > 0 or args.grid_corners >= 0) and mesh is not None:
+            ctx.set_distance_grid(mesh, args.grid_cells if args.grid_cells > 0 else 128, args.grid_corners > 0)
     elif not args.no_autotune:
         from twos import autotune
         t0 = time.perf_counter()
         tuning = autotune.autotune(ctx, geom, sites, sides, geom_sig=f"{w.name}", combine=twos.CROSS,
                                    precision=w.precision, eps=w.eps, max_steps=w.max_steps,
                                    distance_grid=grid_on, mesh=mesh, log=log)
-        shape = {k: v for k, v in tuning.choice.items() if k != "grid_cells" and (grid_on or k != "grid_kappa")}
+        shape = {k: v for k, v in tuning.choice.items()
+                 if k not in ("grid_cells", "grid_corners") and (grid_on or k != "grid_kappa")}
         log(f"autotune {time.perf_counter() - t0:.1f} s: {tuning.choice}, "
             f"{tuning.speedup:.3f}x the default shape per path")
 
@@ -172,14 +175,16 @@ def main() -> int:
         q_sub = cnt.stats["steps"] + cnt.stats["paths"]
         n_nodes = cnt.stats["nodes_visited"] / q_sub
         n_tris = cnt.stats["tris_tested"] / q_sub
-        b_query = NODE_BYTES * n_nodes + TRI_BYTES * n_tris
+        n_grid = cnt.stats["grid_lookups"] / q_sub
+        b_query = NODE_BYTES * n_nodes + TRI_BYTES * n_tris + GRID_BYTES * n_grid
         achieved = queries * b_query / (eager_ms * 1e-3) / 1e9
         traffic = measured_traffic(w.name, n, not args.exact, launches)
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "algorithmic_bytes_per_launch": round(queries * b_query / launches),
                     "bytes_per_query": round(b_query, 1), "nodes_per_query": round(n_nodes, 2),
-                    "tris_per_query": round(n_tris, 2), "walk_kernel_ms_per_launch": round(eager_ms / launches, 3),
+                    "tris_per_query": round(n_tris, 2), "grid_lookups_per_query": round(n_grid, 3),
+                    "walk_kernel_ms_per_launch": round(eager_ms / launches, 3),
                     "launches": launches}
     else:
         flops = 23 if w.precision == 64 else 177
@@ -211,10 +216,11 @@ def main() -> int:
         "mean_walks_per_side": round(float(np.mean(res.walks)), 1),
         "rounds": res.stats["rounds"],
         "launch_shape": {"block": 256, "waves_per_simd": 0, "lds_stack": 24,
-                         **({"grid_kappa": 4.0, "grid_cells": 128} if grid_on else {}), **shape,
-                         **({"grid_cells": tuning.choice["grid_cells"]}
+                         **({"grid_kappa": 4.0, "grid_cells": 128, "grid_corners": 0} if grid_on else {}), **shape,
+                         **({k: tuning.choice[k] for k in ("grid_cells", "grid_corners")}
                             if tuning is not None and "grid_cells" in tuning.choice else {}),
-                         **({"grid_cells": args.grid_cells} if grid_on and args.grid_cells > 0 else {})},
+                         **({"grid_cells": args.grid_cells} if grid_on and args.grid_cells > 0 else {}),
+                         **({"grid_corners": args.grid_corners} if grid_on and args.grid_corners >= 0 else {})},
         "roofline": roofline,
     }
     if tuning is not None:

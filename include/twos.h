@@ -190,6 +190,7 @@ typedef struct {
   int32_t block;
   int32_t waves_per_simd;
   int32_t grid;
+  int64_t grid_lookups;     /* distance-grid lookups (when counting is on) */
 } twos_stats;
 
 const char *twos_version(void);
@@ -202,10 +203,13 @@ twos_status twos_mesh_create(twos_ctx *ctx, const double *vertices, int64_t nv, 
                              int64_t nf, int32_t *mesh_id, double *build_ms);
 twos_status twos_mesh_info(twos_ctx *ctx, int32_t mesh_id, int32_t *depth, int64_t *nodes, int64_t *bytes);
 /* Rebuilds the mesh's conservative distance grid with `cells` cells along the
- * longest axis of its bounding box (8..512; meshes start at 128) and updates
- * every geometry on the mesh.  No counterpart in the reference (Embree has no
+ * longest axis of its bounding box (8..512; meshes start at 128 cells, values
+ * at cell centres) and updates every geometry on the mesh.  corners != 0
+ * samples the distance at the cell corners and bounds a step by the best of
+ * a cell's eight corners.  No counterpart in the reference (Embree has no
  * distance grid); it is a tuning control for TWOS_FLAG_DISTANCE_GRID. */
-twos_status twos_mesh_set_distance_grid(twos_ctx *ctx, int32_t mesh_id, int32_t cells, double *build_ms);
+twos_status twos_mesh_set_distance_grid(twos_ctx *ctx, int32_t mesh_id, int32_t cells, int32_t corners,
+                                        double *build_ms);
 twos_status twos_mesh_release(twos_ctx *ctx, int32_t mesh_id);
 
 twos_status twos_geometry_create(twos_ctx *ctx, const twos_geometry_desc *desc, int32_t *geom_id);

@@ -17,8 +17,9 @@ class FakeContext:
         self.cells = 128
         self.cell_history = []
 
-    def set_distance_grid(self, mesh, cells):
+    def set_distance_grid(self, mesh, cells, corners=False):
         self.cells = cells
+        self.corners = corners
         self.cell_history.append(cells)
         return 0.0
 

@@ -61,7 +61,7 @@ def test_c3_exact_walks_against_oracle(ctx, c3):
     assert np.sum(np.abs(z) > 3) <= 1 and np.max(np.abs(z)) < 5
 
 
-@pytest.mark.parametrize("kappa", [0.5, 1.0, 2.0, 4.0])
+@pytest.mark.parametrize("kappa", [0.25, 0.5, 1.0, 4.0])
 def test_c3_distance_grid_against_exact(ctx, c3, kappa):
     """The conservative-radius walk estimates the same quantity as the exact
     walk (independent seeds), at every threshold the autotuner may choose."""
@@ -75,15 +75,17 @@ def test_c3_distance_grid_against_exact(ctx, c3, kappa):
     assert abs(np.mean(z)) < 0.35  # no systematic shift (128 sites: sd of the mean ~0.09)
 
 
-def test_c3_finer_distance_grid_against_exact(ctx, c3):
-    """A rebuilt 256-cell grid changes the paths and still estimates the
-    exact walk's quantity; the mesh is restored to 128 cells afterwards."""
+@pytest.mark.parametrize("corners", [False, True])
+def test_c3_finer_distance_grid_against_exact(ctx, c3, corners):
+    """A rebuilt 256-cell grid (centre or eight-corner bound) changes the paths
+    and still estimates the exact walk's quantity; the mesh is restored to its
+    128-cell centre grid afterwards."""
     w, mesh, g = c3
     sites = w.sites[128:256]
     kw = dict(combine=twos.CROSS, precision=32, walks=4096, eps=w.eps, max_steps=w.max_steps)
     coarse = ctx.estimate(g, sites, W.sides(w, twos.side), seed=3, distance_grid=True, grid_kappa=2.0, **kw)
     try:
-        ctx.set_distance_grid(mesh, 256)
+        ctx.set_distance_grid(mesh, 256, corners)
         fine = ctx.estimate(g, sites, W.sides(w, twos.side), seed=3, distance_grid=True, grid_kappa=2.0, **kw)
     finally:
         ctx.set_distance_grid(mesh, 128)

@@ -271,6 +271,7 @@ __device__ __forceinline__ void load_tri<double>(const MeshView &m, int32_t k, V
 struct TraversalCount {
   uint32_t nodes;
   uint32_t tris;
+  uint32_t grid;  // distance-grid lookups
 };
 
 // 4-wide node, 128 bytes, eight float4: the four children's bounds as
@@ -509,25 +510,52 @@ __device__ __forceinline__ R mesh_distance(const DevGeom &g, R d2) {
   return best;
 }
 
-// Conservative radius from the distance grid: with c the centre of the cell
-// holding p (clamped to the grid), the mesh distance is 1-Lipschitz, so
-// d(p) >= d(c) - |p - c|; a small margin covers binary32 rounding of d(c).
+// Conservative radius from the distance grid.  The mesh distance d is
+// 1-Lipschitz, so d(p) >= d(c) - |p - c| for every sample point c; a small
+// margin covers the binary32 rounding of d(c) and of c itself.
+//   centre grid: c = the centre of the cell holding p (clamped to the grid);
+//   corner grid: the best of the eight corners of that cell, which is much
+//   tighter where d is smooth (the corner most aligned with the gradient
+//   loses |p - c| (1 - cos angle) instead of up to |p - c|).
+// Any sample gives a valid bound, so the cell index may round either way at a
+// cell face: multiply by 1/h instead of dividing.
 template <typename R>
 __device__ __forceinline__ R grid_lower(const DevGeom &g, V3<R> p) {
-  // Any cell centre gives a valid bound, so the cell index may round either
-  // way at a cell face: multiply by 1/h instead of dividing.
   const R h = R(g.grid_h), inv_h = R(g.grid_inv_h);
+  const int last = g.grid_corners ? 2 : 1;  // highest first index of a cell
   int i = int(floor((p.x - R(g.grid_lo[0])) * inv_h));
   int k = int(floor((p.y - R(g.grid_lo[1])) * inv_h));
   int l = int(floor((p.z - R(g.grid_lo[2])) * inv_h));
-  i = min(max(i, 0), g.grid_n[0] - 1);
-  k = min(max(k, 0), g.grid_n[1] - 1);
-  l = min(max(l, 0), g.grid_n[2] - 1);
-  const V3<R> c{R(g.grid_lo[0]) + (R(i) + R(0.5)) * h, R(g.grid_lo[1]) + (R(k) + R(0.5)) * h,
-                R(g.grid_lo[2]) + (R(l) + R(0.5)) * h};
-  const R dc = R(g.grid[(size_t(i) * g.grid_n[1] + k) * g.grid_n[2] + l]);
-  const V3<R> d = p - c;
-  return dc - sqrt(dot(d, d)) - R(2e-6) * (R(1) + dc);
+  i = min(max(i, 0), g.grid_n[0] - last);
+  k = min(max(k, 0), g.grid_n[1] - last);
+  l = min(max(l, 0), g.grid_n[2] - last);
+  if (!g.grid_corners) {
+    const V3<R> c{R(g.grid_lo[0]) + (R(i) + R(0.5)) * h, R(g.grid_lo[1]) + (R(k) + R(0.5)) * h,
+                  R(g.grid_lo[2]) + (R(l) + R(0.5)) * h};
+    const R dc = R(g.grid[(size_t(i) * g.grid_n[1] + k) * g.grid_n[2] + l]);
+    const V3<R> d = p - c;
+    return dc - sqrt(dot(d, d)) - R(2e-6) * (R(1) + dc);
+  }
+  const R x0 = p.x - (R(g.grid_lo[0]) + R(i) * h), y0 = p.y - (R(g.grid_lo[1]) + R(k) * h),
+          z0 = p.z - (R(g.grid_lo[2]) + R(l) * h);
+  const float *row = g.grid + (size_t(i) * g.grid_n[1] + k) * g.grid_n[2] + l;
+  const size_t si = size_t(g.grid_n[1]) * g.grid_n[2], sk = size_t(g.grid_n[2]);
+  R best = -Inf<R>::v();
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const float *q = row + a * si + b * sk;
+      const R dx = x0 - R(a) * h, dy = y0 - R(b) * h;
+      const R dxy = dx * dx + dy * dy;
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const R dc = R(q[c]);
+        const R dz = z0 - R(c) * h;
+        best = fmax(best, dc - sqrt(dxy + dz * dz) - R(2e-6) * (R(1) + dc));
+      }
+    }
+  return best;
 }
 
 // tau < inf: a mesh step whose grid lower bound is at least tau takes that

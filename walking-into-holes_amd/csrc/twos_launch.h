@@ -20,12 +20,15 @@ struct MeshBuffers {
   float *grid = nullptr;
   float grid_lo[3] = {0.f, 0.f, 0.f};
   float grid_h = 0.f;
-  int32_t grid_n[3] = {0, 0, 0};
+  int32_t grid_n[3] = {0, 0, 0};  // samples per axis
+  int32_t grid_corners = 0;       // samples at cell corners (1) or centres (0)
 };
 
 hipError_t lbvh_build(const double *d_vtx, const uint32_t *d_faces, int32_t n, MeshBuffers &mb, int32_t *depth,
                       hipStream_t stream);
-hipError_t build_distance_grid(MeshBuffers &mb, const double lo[3], const double hi[3], int cells,
+// cells along the longest axis; corners: sample the distance at the cell
+// corners (cells + 1 samples per axis) instead of the centres.
+hipError_t build_distance_grid(MeshBuffers &mb, const double lo[3], const double hi[3], int cells, bool corners,
                                hipStream_t stream);
 hipError_t launch_inside(const MeshBuffers &mb, const double *d_pts, int64_t n, int32_t *d_out, hipStream_t stream);
 hipError_t launch_closest(const DevGeom *d_geom, int precision, const double *d_pts, int64_t n, double *d_dist,

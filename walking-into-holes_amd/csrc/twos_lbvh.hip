@@ -456,13 +456,15 @@ __global__ void inside_kernel(MeshView m, const double *pts, int64_t n, int32_t 
 }
 
 // Distance at every cell centre of a uniform grid (binary32 exact query).
-__global__ void distance_grid(MeshView m, float lx, float ly, float lz, float h, int nx, int ny, int nz,
-                              float *out) {
+// Sample (i, k, l) sits at lo + (index + offset) h: offset 0.5 for cell
+// centres, 0 for cell corners.
+__global__ void distance_grid(MeshView m, float lx, float ly, float lz, float h, float offset, int nx, int ny,
+                              int nz, float *out) {
   const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t total = int64_t(nx) * ny * nz;
   if (idx >= total) return;
   const int l = int(idx % nz), k = int((idx / nz) % ny), i = int(idx / (int64_t(ny) * nz));
-  const V3<float> p{lx + (float(i) + 0.5f) * h, ly + (float(k) + 0.5f) * h, lz + (float(l) + 0.5f) * h};
+  const V3<float> p{lx + (float(i) + offset) * h, ly + (float(k) + offset) * h, lz + (float(l) + offset) * h};
   int32_t hint = -1;
   V3<float> pt;
   out[idx] = sqrtf(mesh_closest<float, false>(m, p, pt, hint, nullptr));
@@ -578,7 +580,7 @@ hipError_t lbvh_build(const double *d_vtx, const uint32_t *d_faces, int32_t n, M
   return hipSuccess;
 }
 
-hipError_t build_distance_grid(MeshBuffers &mb, const double lo[3], const double hi[3], int cells,
+hipError_t build_distance_grid(MeshBuffers &mb, const double lo[3], const double hi[3], int cells, bool corners,
                                hipStream_t stream) {
   double ext = 0.0;
   for (int k = 0; k < 3; ++k) ext = std::max(ext, hi[k] - lo[k]);
@@ -587,15 +589,17 @@ hipError_t build_distance_grid(MeshBuffers &mb, const double lo[3], const double
   int64_t total = 1;
   for (int k = 0; k < 3; ++k) {
     mb.grid_lo[k] = float(lo[k] - pad);
-    mb.grid_n[k] = std::max(1, int(std::ceil((hi[k] - lo[k] + 2 * pad) / h)));
+    mb.grid_n[k] = std::max(1, int(std::ceil((hi[k] - lo[k] + 2 * pad) / h))) + (corners ? 1 : 0);
     total *= mb.grid_n[k];
   }
   mb.grid_h = float(h);
+  mb.grid_corners = corners ? 1 : 0;
   hipError_t e = hipMalloc(&mb.grid, sizeof(float) * size_t(total));
   if (e != hipSuccess) return e;
   MeshView m{mb.nodes, mb.nodes4, mb.tris_f, mb.tris_d, mb.tri_ids, mb.n_tris};
   hipLaunchKernelGGL(distance_grid, dim3(blocks_for(total, 128)), dim3(128), 0, stream, m, mb.grid_lo[0],
-                     mb.grid_lo[1], mb.grid_lo[2], mb.grid_h, mb.grid_n[0], mb.grid_n[1], mb.grid_n[2], mb.grid);
+                     mb.grid_lo[1], mb.grid_lo[2], mb.grid_h, corners ? 0.0f : 0.5f, mb.grid_n[0], mb.grid_n[1],
+                     mb.grid_n[2], mb.grid);
   return hipGetLastError();
 }
 

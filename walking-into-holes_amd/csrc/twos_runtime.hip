@@ -133,7 +133,7 @@ struct Engine {
   DevSide *d_sides = nullptr;
   double *d_val = nullptr;
   int32_t *d_steps = nullptr;
-  unsigned long long *d_counters = nullptr;  // [kMaxClasses] tickets, then nodes, tris
+  unsigned long long *d_counters = nullptr;  // [kMaxClasses] tickets, then nodes, tris, grid lookups
   // Round input block (one H2D copy) and output block (one D2H copy).
   char *d_in = nullptr, *h_in = nullptr, *d_out = nullptr, *h_out = nullptr;
   // One-site work list of the plain schedule: 64 B in, 96 B out.
@@ -192,7 +192,7 @@ struct Engine {
     HIP_OK(dalloc(&d_sides, 2 * size_t(cap_problems)));
     HIP_OK(dalloc(&d_val, 2 * size_t(cap_walks)));
     HIP_OK(dalloc(&d_steps, 2 * size_t(cap_walks)));
-    HIP_OK(dalloc(&d_counters, kMaxClasses + 2));
+    HIP_OK(dalloc(&d_counters, kMaxClasses + 3));
     auto align = [](size_t v) { return (v + 255) & ~size_t(255); };
     off_slot = 0;
     off_base = align(off_slot + 4 * C);
@@ -580,7 +580,7 @@ struct Engine {
         int kind = geom_kind_of(rc->geom);
         for (const Entry &en : by_class[c])
           if (geom_kind_of(en.r->geom) != kind) kind = kGeomGeneric;
-        if (rc->prm.flags & TWOS_FLAG_COUNT_TRAVERSAL) HIP_OK(launch_zero(d_counters + kMaxClasses, 2, strm));
+        if (rc->prm.flags & TWOS_FLAG_COUNT_TRAVERSAL) HIP_OK(launch_zero(d_counters + kMaxClasses, 3, strm));
         HIP_OK(hipEventRecord(ev[0], strm));
         st = launch_walks(rc, int(c), e0s[c], totals[c], kind);
         if (st != TWOS_OK) return st;
@@ -603,10 +603,11 @@ struct Engine {
         HIP_OK(hipEventElapsedTime(&ms, ev[1], ev[2]));
         stats.reduce_kernel_ms += ms;
         if (rc->prm.flags & TWOS_FLAG_COUNT_TRAVERSAL) {
-          unsigned long long cnt[2];
+          unsigned long long cnt[3];
           HIP_OK(hipMemcpy(cnt, d_counters + kMaxClasses, sizeof(cnt), hipMemcpyDeviceToHost));
           stats.nodes_visited += int64_t(cnt[0]);
           stats.tris_tested += int64_t(cnt[1]);
+          stats.grid_lookups += int64_t(cnt[2]);
         }
       }
       HIP_OK(hipMemcpyAsync(h_out + off_compact, d_out + off_compact, sizeof(twos_site_result) * n_entries,
@@ -838,6 +839,7 @@ twos_status build_geom(twos_ctx *ctx, GeomEntry &g) {
     }
     v.grid_h = mb.grid_h;
     v.grid_inv_h = mb.grid_h > 0.f ? 1.0f / mb.grid_h : 0.f;
+    v.grid_corners = mb.grid_corners;
     v.n_tris = mb.n_tris;
     it->second.users += 1;
   }
@@ -1012,7 +1014,7 @@ twos_status twos_mesh_create(twos_ctx *ctx, const double *vertices, int64_t nv, 
         lo[k] = std::min(lo[k], vertices[3 * i + k]);
         hi[k] = std::max(hi[k], vertices[3 * i + k]);
       }
-    e = build_distance_grid(m.mb, lo, hi, kGridCells, ctx->stream);
+    e = build_distance_grid(m.mb, lo, hi, kGridCells, false, ctx->stream);
     for (int k = 0; k < 3; ++k) { m.lo[k] = lo[k]; m.hi[k] = hi[k]; }
     m.grid_cells = kGridCells;
   }
@@ -1039,7 +1041,8 @@ twos_status twos_mesh_create(twos_ctx *ctx, const double *vertices, int64_t nv, 
   return TWOS_OK;
 }
 
-twos_status twos_mesh_set_distance_grid(twos_ctx *ctx, int32_t mesh_id, int32_t cells, double *build_ms) {
+twos_status twos_mesh_set_distance_grid(twos_ctx *ctx, int32_t mesh_id, int32_t cells, int32_t corners,
+                                        double *build_ms) {
   if (!ctx) return fail(TWOS_ERR_INVALID_ARGUMENT, "null context");
   if (cells < 8 || cells > 512) return fail(TWOS_ERR_INVALID_ARGUMENT, "cells must be in [8, 512]");
   std::lock_guard<std::mutex> lock(ctx->mu);
@@ -1055,7 +1058,7 @@ twos_status twos_mesh_set_distance_grid(twos_ctx *ctx, int32_t mesh_id, int32_t 
   HIP_OK(hipEventCreate(&a));
   HIP_OK(hipEventCreate(&b));
   HIP_OK(hipEventRecord(a, ctx->stream));
-  const hipError_t e = build_distance_grid(nb, m.lo, m.hi, cells, ctx->stream);
+  const hipError_t e = build_distance_grid(nb, m.lo, m.hi, cells, corners != 0, ctx->stream);
   (void)hipEventRecord(b, ctx->stream);
   (void)hipStreamSynchronize(ctx->stream);
   float ms = 0.f;
@@ -1079,6 +1082,7 @@ twos_status twos_mesh_set_distance_grid(twos_ctx *ctx, int32_t mesh_id, int32_t 
     }
     g.dev.grid_h = nb.grid_h;
     g.dev.grid_inv_h = nb.grid_h > 0.f ? 1.0f / nb.grid_h : 0.f;
+    g.dev.grid_corners = nb.grid_corners;
   }
   if (ctx->engine) ctx->engine->graph_key = -1;
   if (build_ms) *build_ms = ms;

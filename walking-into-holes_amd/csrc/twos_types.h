@@ -23,7 +23,8 @@ using DevSide = twos_side;
 struct DevGeom {
   int32_t container, n_spheres, n_cylinders, n_cavities;
   int32_t mesh_radius_f32, n_tris;
-  int32_t pad[2];
+  int32_t grid_corners;  // 1: grid values at cell corners (8-corner bound), 0: at cell centres
+  int32_t pad;
   double ball[4];
   double box_lo[3];
   double box_hi[3];
@@ -35,7 +36,7 @@ struct DevGeom {
   const float *tris_f;
   const double *tris_d;
   const int32_t *tri_ids;
-  const float *grid;     // distance at cell centres (conservative-radius grid), or null
+  const float *grid;     // mesh distance at cell centres or corners (conservative-radius grid), or null
   float grid_lo[3];
   float grid_h;
   int32_t grid_n[3];

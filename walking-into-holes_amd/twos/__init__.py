@@ -275,12 +275,13 @@ class Context:
         _check(_lib().twos_mesh_info(self._h, mid.value, C.byref(depth), C.byref(nodes), C.byref(nbytes)))
         return Mesh(mid.value, len(v), len(f), ms.value, depth.value, nbytes.value)
 
-    def set_distance_grid(self, mesh: Mesh | int, cells: int) -> float:
+    def set_distance_grid(self, mesh: Mesh | int, cells: int, corners: bool = False) -> float:
         """Rebuild the mesh's distance grid with `cells` cells along its longest
-        axis (8..512); returns the build time in ms."""
+        axis (8..512), sampled at cell centres or, with corners, at cell corners
+        (eight-corner bound); returns the build time in ms."""
         ms = C.c_double()
         _check(_lib().twos_mesh_set_distance_grid(self._h, mesh.id if isinstance(mesh, Mesh) else int(mesh),
-                                                  int(cells), C.byref(ms)))
+                                                  int(cells), int(bool(corners)), C.byref(ms)))
         return ms.value
 
     def geometry(self, container: int, *, mesh: Mesh | int | None = None, ball=(0.0, 0.0, 0.0, 1.0),

@@ -69,7 +69,8 @@ class Stats(C.Structure):
     _fields_ = [("seconds", C.c_double), ("walk_kernel_ms", C.c_double), ("reduce_kernel_ms", C.c_double),
                 ("walk_launches", C.c_int64), ("steps", C.c_int64), ("paths", C.c_int64),
                 ("nodes_visited", C.c_int64), ("tris_tested", C.c_int64), ("rounds", C.c_int32),
-                ("block", C.c_int32), ("waves_per_simd", C.c_int32), ("grid", C.c_int32)]
+                ("block", C.c_int32), ("waves_per_simd", C.c_int32), ("grid", C.c_int32),
+                ("grid_lookups", C.c_int64)]
 
 
 _ctx_p = C.c_void_p
@@ -84,7 +85,7 @@ SIGNATURES = {
     "twos_destroy": (None, [_ctx_p]),
     "twos_mesh_create": (C.c_int32, [_ctx_p, _dp, C.c_int64, _u32p, C.c_int64, _i32p, _dp]),
     "twos_mesh_info": (C.c_int32, [_ctx_p, C.c_int32, _i32p, _i64p, _i64p]),
-    "twos_mesh_set_distance_grid": (C.c_int32, [_ctx_p, C.c_int32, C.c_int32, _dp]),
+    "twos_mesh_set_distance_grid": (C.c_int32, [_ctx_p, C.c_int32, C.c_int32, C.c_int32, _dp]),
     "twos_mesh_release": (C.c_int32, [_ctx_p, C.c_int32]),
     "twos_geometry_create": (C.c_int32, [_ctx_p, C.POINTER(GeometryDesc), _i32p]),
     "twos_geometry_union_sphere": (C.c_int32, [_ctx_p, C.c_int32, _dp, C.c_double, _i32p]),

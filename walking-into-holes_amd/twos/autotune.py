@@ -7,9 +7,11 @@ Axes (binary32 persistent kernel; binary64 tunes the block only):
                   spill to scratch)
   lds_stack       16, 24 or 32 LDS traversal-stack entries per lane (16 and
                   32 exist for waves_per_simd 0 and 5)
-  grid_kappa      distance-grid threshold in cell sizes, 0.5 to 6 (mesh
+  grid_kappa      distance-grid threshold in cell sizes, 0.25 to 6 (mesh
                   geometries with the distance grid on)
-  grid_cells      distance-grid resolution, 96 to 384 cells along
+  grid_corners    distance-grid samples at cell centres (0) or corners (1,
+                  eight-corner bound), with the mesh handle
+  grid_cells      distance-grid resolution, 96 to 512 cells along
                   the mesh's longest axis (when the mesh handle is given; the
                   mesh keeps the chosen grid)
 
@@ -40,13 +42,15 @@ import numpy as np
 
 from . import _capi
 
-DEFAULTS = {"block": 256, "waves_per_simd": 0, "lds_stack": 24, "grid_kappa": 4.0, "grid_cells": 128}
+DEFAULTS = {"block": 256, "waves_per_simd": 0, "lds_stack": 24, "grid_corners": 0, "grid_kappa": 4.0,
+            "grid_cells": 128}
 AXES = {
     "block": (64, 128, 256),
     "waves_per_simd": (0, 5, 6, 8),
     "lds_stack": (16, 24, 32),
-    "grid_kappa": (0.5, 0.75, 1.0, 1.5, 2.0, 3.0, 4.0, 6.0),
-    "grid_cells": (96, 128, 192, 256, 320, 384),
+    "grid_corners": (0, 1),
+    "grid_kappa": (0.25, 0.35, 0.5, 0.75, 1.0, 1.5, 2.0, 3.0, 4.0, 6.0),
+    "grid_cells": (96, 128, 192, 256, 320, 384, 448, 512),
 }
 
 _memory: dict[str, "Tuning"] = {}
@@ -98,7 +102,7 @@ def autotune(ctx, geom: int, sites, sides, *, geom_sig: str, combine: int, preci
 
     def applied(t: "Tuning") -> "Tuning":
         if tune_cells:
-            ctx.set_distance_grid(mesh, t.choice["grid_cells"])
+            ctx.set_distance_grid(mesh, t.choice["grid_cells"], bool(t.choice["grid_corners"]))
         return t
 
     if key in _memory:
@@ -118,18 +122,19 @@ def autotune(ctx, geom: int, sites, sides, *, geom_sig: str, combine: int, preci
         axes.pop("grid_kappa")
     if not tune_cells:
         axes.pop("grid_cells")
+        axes.pop("grid_corners")
     if precision == 64:
-        axes = {k: v for k, v in axes.items() if k in ("block", "grid_kappa", "grid_cells")}
+        axes = {k: v for k, v in axes.items() if k in ("block", "grid_kappa", "grid_cells", "grid_corners")}
     measured: dict[tuple, float] = {}
-    cells_now = [None]
+    grid_now = [None]
 
     def cost(cfg: dict) -> float:
         k = tuple(sorted(cfg.items()))
         if k in measured:
             return measured[k]
-        if tune_cells and cfg["grid_cells"] != cells_now[0]:
-            ctx.set_distance_grid(mesh, cfg["grid_cells"])
-            cells_now[0] = cfg["grid_cells"]
+        if tune_cells and (cfg["grid_cells"], cfg["grid_corners"]) != grid_now[0]:
+            ctx.set_distance_grid(mesh, cfg["grid_cells"], bool(cfg["grid_corners"]))
+            grid_now[0] = (cfg["grid_cells"], cfg["grid_corners"])
         best = math.inf
         for _ in range(repeats):
             r = ctx.estimate(geom, pts, sides, combine=combine, precision=precision, seed=seed, walks=walks,
