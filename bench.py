@@ -36,7 +36,8 @@ FP64_VALU_PEAK = 78.6e12
 FP32_VALU_PEAK = 157.3e12
 NODE_BYTES = 128  # one 4-wide node: four child boxes + four child links
 TRI_BYTES = 36    # nine binary32 coordinates per triangle test
-GRID_BYTES = 4    # one binary32 distance per distance-grid lookup
+GRID_BYTES = 4    # one binary32 distance per sample of a distance-grid lookup (1 centre or 8 corners)
+TUNING_CACHE = ROOT / "walking-into-holes_amd" / "twos" / "tuning_cache.json"
 
 
 def parse() -> argparse.Namespace:
@@ -56,6 +57,9 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--waves-per-simd", type=int, default=-1,
                     help="persistent occupancy variant; default: the autotuner's choice")
     ap.add_argument("--no-autotune", action="store_true", help="default launch shape instead of the tuner's")
+    ap.add_argument("--retune", action="store_true",
+                    help="run the autotuner's trials instead of reading its committed cache "
+                         "(writes gpurun_out/tuning_cache.json)")
     ap.add_argument("--block", type=int, default=0, help="fixed launch shape (skips the autotuner)")
     ap.add_argument("--lds-stack", type=int, default=0, help="fixed launch shape (skips the autotuner)")
     ap.add_argument("--grid-kappa", type=float, default=0.0, help="fixed launch shape (skips the autotuner)")
@@ -129,7 +133,9 @@ def main() -> int:
         t0 = time.perf_counter()
         tuning = autotune.autotune(ctx, geom, sites, sides, geom_sig=f"{w.name}", combine=twos.CROSS,
                                    precision=w.precision, eps=w.eps, max_steps=w.max_steps,
-                                   distance_grid=grid_on, mesh=mesh, log=log)
+                                   distance_grid=grid_on, mesh=mesh, log=log,
+                                   cache_path=ROOT / "gpurun_out" / "tuning_cache.json" if args.retune
+                                   else TUNING_CACHE)
         shape = {k: v for k, v in tuning.choice.items()
                  if k not in ("grid_cells", "grid_corners") and (grid_on or k != "grid_kappa")}
         log(f"autotune {time.perf_counter() - t0:.1f} s: {tuning.choice}, "
@@ -162,6 +168,12 @@ def main() -> int:
     steps_per_s = float(np.sum(steps)) / total_t
     capped = int(np.sum(res.status == 1))
 
+    out_shape = {"block": 256, "waves_per_simd": 0, "lds_stack": 24,
+                         **({"grid_kappa": 4.0, "grid_cells": 128, "grid_corners": 0} if grid_on else {}), **shape,
+                         **({k: tuning.choice[k] for k in ("grid_cells", "grid_corners")}
+                            if tuning is not None and "grid_cells" in tuning.choice else {}),
+                         **({"grid_cells": args.grid_cells} if grid_on and args.grid_cells > 0 else {}),
+                         **({"grid_corners": args.grid_corners} if grid_on and args.grid_corners >= 0 else {})}
     # Walk kernel alone (eager launches, HIP events on the library's stream).
     eager = run(twos.PERSISTENT, sites, False)
     eager_ms = eager.stats["walk_kernel_ms"]
@@ -176,9 +188,10 @@ def main() -> int:
         n_nodes = cnt.stats["nodes_visited"] / q_sub
         n_tris = cnt.stats["tris_tested"] / q_sub
         n_grid = cnt.stats["grid_lookups"] / q_sub
-        b_query = NODE_BYTES * n_nodes + TRI_BYTES * n_tris + GRID_BYTES * n_grid
+        corners = out_shape.get("grid_corners", 0) == 1
+        b_query = NODE_BYTES * n_nodes + TRI_BYTES * n_tris + GRID_BYTES * (8 if corners else 1) * n_grid
         achieved = queries * b_query / (eager_ms * 1e-3) / 1e9
-        traffic = measured_traffic(w.name, n, not args.exact, launches)
+        traffic = measured_traffic(w.name, n, out_shape, launches)
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "algorithmic_bytes_per_launch": round(queries * b_query / launches),
@@ -215,12 +228,7 @@ def main() -> int:
         "sites_capped": capped,
         "mean_walks_per_side": round(float(np.mean(res.walks)), 1),
         "rounds": res.stats["rounds"],
-        "launch_shape": {"block": 256, "waves_per_simd": 0, "lds_stack": 24,
-                         **({"grid_kappa": 4.0, "grid_cells": 128, "grid_corners": 0} if grid_on else {}), **shape,
-                         **({k: tuning.choice[k] for k in ("grid_cells", "grid_corners")}
-                            if tuning is not None and "grid_cells" in tuning.choice else {}),
-                         **({"grid_cells": args.grid_cells} if grid_on and args.grid_cells > 0 else {}),
-                         **({"grid_corners": args.grid_corners} if grid_on and args.grid_corners >= 0 else {})},
+        "launch_shape": out_shape,
         "roofline": roofline,
     }
     if tuning is not None:
@@ -293,22 +301,24 @@ def copy_probe() -> int:
     return 0
 
 
-def measured_traffic(workload: str, n_sites: int, grid: bool, launches: int):
+def measured_traffic(workload: str, n_sites: int, shape: dict, launches: int):
     """HBM bytes per walk-kernel launch from the committed rocprofv3 --pmc
     summary (FETCH_SIZE and WRITE_SIZE, each scaled by a copy kernel of known
-    size).  The estimate is deterministic, so the same workload repeats the
-    same launches; other configurations report null."""
+    size).  The estimate is deterministic, so the same workload and launch
+    shape repeat the same launches; any other configuration reports null."""
     path = ROOT / "profiles" / "walk_persistent_sh_icosphere_16384.md"
-    if workload != "sh_icosphere_81920" or n_sites != 16384 or not grid or not path.exists():
+    if workload != "sh_icosphere_81920" or n_sites != 16384 or not path.exists():
         return None
     fields = {}
     for line in path.read_text().splitlines():
-        if ":" in line and line.split(":")[0] in ("traffic_bytes_per_launch", "traffic_launches"):
+        if ":" in line and line.split(":")[0] in ("traffic_bytes_per_launch", "traffic_launches", "traffic_shape"):
             key, val = line.split(":", 1)
-            fields[key] = float(val)
-    if fields.get("traffic_launches") != launches:
+            fields[key] = val.strip()
+    if "traffic_shape" not in fields or json.loads(fields["traffic_shape"]) != shape:
         return None
-    return int(fields["traffic_bytes_per_launch"])
+    if float(fields.get("traffic_launches", -1)) != launches:
+        return None
+    return int(float(fields["traffic_bytes_per_launch"]))
 
 
 def cpu_baseline(w, sites, count: int) -> dict:
