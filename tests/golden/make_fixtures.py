@@ -206,7 +206,44 @@ def housing_fixtures(root: Path) -> None:
     (HERE / "housing_meta.json").write_text(json.dumps({"requests": requests}, indent=1) + "\n")
 
 
+def design_fixture(root: Path) -> None:
+    """The pilot greedy insertion design on the branch mesh
+    (benchmarks/submission_study/pilot/pipeline/branch/twos-1611/: design.json
+    and its raw/round{k}-scout, -rescore requests): the candidate pool (round 0's
+    scout sites), the chosen centres, and per round the scout and rescore
+    seeds, walk counts, feasible-candidate counts, the shortlist as indices
+    into that round's feasible candidates, and both rounds' score columns."""
+    run = root / "benchmarks" / "submission_study" / "pilot" / "pipeline" / "branch" / "twos-1611"
+    design = json.loads((run / "design.json").read_text())
+    pool = np.loadtxt(run / "raw" / "round0-scout" / "points.txt", ndmin=2)
+    rounds = []
+    for k in range(len(design["centers"])):
+        entry = {}
+        for kind in ("scout", "rescore"):
+            d = run / "raw" / f"round{k}-{kind}"
+            a = study_arguments(d / "request.json")
+            pts = np.loadtxt(d / "points.txt", ndmin=2)
+            with (d / "field.csv").open() as handle:
+                rows = list(csv.DictReader(handle))
+            entry[kind] = {"seed": int(a["--seed"]), "walks": int(a["--walks"]), "sites": len(pts),
+                           "score": [float(r["score"]) for r in rows]}
+            if kind == "scout":
+                scout_pts = pts
+            else:
+                entry["shortlist"] = [int(np.nonzero(np.all(scout_pts == q, axis=1))[0][0]) for q in pts]
+        entry["radius"] = float(a["--cavity-radius"])
+        entry["epsilon"] = float(a["--epsilon"])
+        entry["max_steps"] = int(a["--max-steps"])
+        rounds.append(entry)
+    out = {"source": str(run.relative_to(root)), "seed": design["seed"], "radius": design["radius"],
+           "protocol": design["protocol"], "centers": design["centers"], "pool": pool.tolist(), "rounds": rounds,
+           "config": (root / "benchmarks" / "submission_study" / "pilot" / "source-config-v1" /
+                      "branch.txt").read_text()}
+    (HERE / "study_branch_design.json").write_text(json.dumps(out) + "\n")
+
+
 if __name__ == "__main__":
     main()
     study_fixtures(Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference"))
     housing_fixtures(Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference"))
+    design_fixture(Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference"))
