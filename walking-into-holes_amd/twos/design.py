@@ -163,6 +163,36 @@ def validate_insertion(estimate_u, probes, weights, base_holes, center, radius: 
     return Validation(delta, se, delta / cap, se / cap, float(np.mean(w * u0)), float(np.mean(w * u1)))
 
 
+class GpuDesign:
+    """Estimator adapter for any primal/adjoint pair of sides on a resident
+    mesh: cavities of one radius, CROSS on Philox streams (binary32 walks by
+    default), an optional launch shape from the autotuner.  Returns the
+    cross-index score u p and its jackknife SE per point."""
+
+    def __init__(self, ctx, mesh, sides, *, radius: float, eps: float, max_steps: int, precision: int = 32,
+                 distance_grid: bool = True, shape: dict | None = None):
+        import twos
+        self.twos, self.ctx, self.mesh, self.sides = twos, ctx, mesh, sides
+        self.radius, self.eps, self.max_steps, self.precision = radius, eps, max_steps, precision
+        self.distance_grid, self.shape = distance_grid, dict(shape or {})
+        self.steps = 0
+        self.paths = 0
+
+    def __call__(self, points, holes, walks, seed):
+        t = self.twos
+        g = self.ctx.geometry(t.MESH, mesh=self.mesh, cavities=holes if len(holes) else None,
+                              cavity_radius=self.radius)
+        try:
+            r = self.ctx.estimate(g, points, self.sides, combine=t.CROSS, precision=self.precision, seed=seed,
+                                  walks=walks, eps=self.eps, max_steps=self.max_steps,
+                                  distance_grid=self.distance_grid, use_graph=True, **self.shape)
+        finally:
+            self.ctx.release_geometry(g)
+        self.steps += int(r.stats["steps"])
+        self.paths += int(r.stats["paths"])
+        return r.score, r.score_se
+
+
 class GpuStudy:
     """Estimator adapter for the study problem on the GPU: heat bumps and the
     unit-mass sensor bump at a Green point, cavities of one radius.
