@@ -162,6 +162,17 @@ typedef struct {
  * Spheres step) instead of an exact closest-point query; steps near the
  * boundary stay exact.  Unbiased, but paths differ from the exact walk. */
 #define TWOS_FLAG_DISTANCE_GRID 2
+/* Gradient channel: each path's first step also gives a gradient sample
+ * (3 / r0) (V - s0) omega0 + (r0^2 / 10) grad f, with r0, omega0 the first
+ * sphere's radius and direction, V the path value, s0 its first source term
+ * and grad f the affine source's gradient (the ball solution of -Lap w = f,
+ * w = 0 on the sphere, has gradient r0^2/10 grad f at the centre).  Per site:
+ * the means grad_u, grad_v and the cross-index product
+ * grad_dot = (A.B - C)/(N(N-1)) (A, B the sums of the two sides' samples, C the
+ * sum of their per-walk dot products), with a first-order (delta-method) SE.
+ * No counterpart in the reference (an extension of the estimator); not
+ * available with sampled (source_kind 2) sources. */
+#define TWOS_FLAG_GRADIENT 4
 
 typedef struct {
   double u;         /* mean of side-0 samples */
@@ -175,6 +186,10 @@ typedef struct {
   int64_t caps;     /* paths that hit max_steps */
   int32_t status;   /* 0 done, 1 budget exhausted before the CI target */
   int32_t pad;
+  double grad_u[3]; /* TWOS_FLAG_GRADIENT: mean gradient samples of side 0 ... */
+  double grad_v[3]; /* ... and side 1 (NaN without the flag) */
+  double grad_dot;  /* cross-index estimate of grad u . grad v */
+  double grad_dot_se;
 } twos_site_result;
 
 typedef struct {

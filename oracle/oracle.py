@@ -49,14 +49,15 @@ class Params(C.Structure):
     _fields_ = [("combine", C.c_int32), ("rng", C.c_int32), ("seed", C.c_uint64),
                 ("repeat", C.c_int32), ("max_steps", C.c_int32), ("eps", C.c_double),
                 ("walks", C.c_int32), ("walks_max", C.c_int32), ("ci_rel", C.c_double),
-                ("ci_z", C.c_double), ("threads", C.c_int32), ("pad", C.c_int32)]
+                ("ci_z", C.c_double), ("threads", C.c_int32), ("gradient", C.c_int32)]
 
 
 class Site(C.Structure):
     _fields_ = [("u", C.c_double), ("v", C.c_double), ("score", C.c_double),
                 ("score_se", C.c_double), ("td", C.c_double), ("same", C.c_double),
                 ("walks", C.c_int64), ("steps", C.c_int64), ("caps", C.c_int64),
-                ("status", C.c_int32), ("pad", C.c_int32)]
+                ("status", C.c_int32), ("pad", C.c_int32), ("grad_u", C.c_double * 3),
+                ("grad_v", C.c_double * 3), ("grad_dot", C.c_double), ("grad_dot_se", C.c_double)]
 
 
 def build() -> Path:
@@ -100,6 +101,7 @@ def lib() -> C.CDLL:
                                C.c_uint64, C.c_int64, C.c_int32, _dp, C.c_int32]
         L.og_combine.argtypes = [_dp, _dp, C.c_int64, C.c_int32, _dp]
         L.og_combine_powersums.argtypes = [_dp, C.c_int64, _dp]
+        L.og_combine_grad.argtypes = [_dp, _dp, C.c_int64, _dp]
         _lib = L
     return _lib
 
@@ -265,13 +267,14 @@ def housing_chip_terms():
 
 def estimate(geometry: Geometry, sites, sides, *, combine=SQUARE, rng=RNG_PHILOX, seed=0,
              repeat=0, walks=16, walks_max=0, ci_rel=0.02, ci_z=1.96, eps=5e-4, max_steps=512,
-             threads=0, dump: bool = False):
+             threads=0, gradient: bool = False, dump: bool = False):
     sites = _f64(np.atleast_2d(sites))
     n = len(sites)
     g = geometry.struct()
     sd = (Side * 2)(sides[0], sides[1] if len(sides) > 1 else sides[0])
     p = Params(combine=combine, rng=rng, seed=seed, repeat=repeat, max_steps=max_steps, eps=eps,
-               walks=walks, walks_max=walks_max, ci_rel=ci_rel, ci_z=ci_z, threads=threads)
+               walks=walks, walks_max=walks_max, ci_rel=ci_rel, ci_z=ci_z, threads=threads,
+               gradient=int(gradient))
     out = (Site * max(n, 1))()
     stride = max(walks, walks_max) if dump else 0
     da = np.zeros((n, stride))
@@ -283,7 +286,7 @@ def estimate(geometry: Geometry, sites, sides, *, combine=SQUARE, rng=RNG_PHILOX
     if status == 2:
         raise ValueError("invalid estimator arguments")
     fields = [f[0] for f in Site._fields_ if f[0] != "pad"]
-    res = {k: np.array([getattr(out[i], k) for i in range(n)]) for k in fields}
+    res = {k: np.array([np.array(getattr(out[i], k)) for i in range(n)]) for k in fields}
     res["status_code"] = status
     if dump:
         res["a"], res["b"], res["walk_steps"] = da, db, ds
@@ -314,6 +317,16 @@ def combine_powersums(sums, n: int) -> np.ndarray:
     s = _f64(sums)
     out = np.zeros(6)
     lib().og_combine_powersums(_ptr(s, _dp), n, _ptr(out, _dp))
+    return out
+
+
+def combine_grad(ga, gb) -> np.ndarray:
+    """Gradient channel from per-walk samples ([n][3] each): grad_u[3],
+    grad_v[3], grad_dot, grad_dot_se."""
+    a = _f64(np.asarray(ga).reshape(-1, 3))
+    b = _f64(np.asarray(gb).reshape(-1, 3))
+    out = np.zeros(8)
+    lib().og_combine_grad(_ptr(a, _dp), _ptr(b, _dp), len(a), _ptr(out, _dp))
     return out
 
 

@@ -533,7 +533,25 @@ struct WalkOut {
   double a = 0.0, b = 0.0;
   int32_t steps = 0;
   bool cap = false;
+  // First step, for the gradient channel: 3 omega0 / r0, r0 (0 if the path
+  // ended at its first query) and both sides' values after its source term.
+  V3 w0{0.0, 0.0, 0.0};
+  double r0 = 0.0, s0 = 0.0, s1 = 0.0;
+  V3 ga{0.0, 0.0, 0.0}, gb{0.0, 0.0, 0.0};  // the gradient samples
 };
+
+// Gradient sample of one side from a path's first step: (3/r0)(V - s0) omega0
+// plus (r0^2/10) grad f for an affine source (the solution of -Lap w = f in the
+// ball, w = 0 on its sphere, has gradient r0^2 grad f / 10 at the centre).
+V3 grad_sample(const WalkOut &o, double value, double first, const og_side &s) {
+  if (!(o.r0 > 0.0)) return {0.0, 0.0, 0.0};
+  V3 g = o.w0 * (value - first);
+  if (s.source_kind == 1) {
+    const double c = o.r0 * o.r0 / 10.0;
+    g = g + V3{s.src[1] * c, s.src[2] * c, s.src[3] * c};
+  }
+  return g;
+}
 
 // One Walk on Spheres path: wih_twos.cpp:635-664.  Each step queries the
 // domain distance r, stops when !(r > eps), adds the Green-ball source
@@ -569,7 +587,14 @@ WalkOut walk(const og_geom &g, const og_side &s0, const og_side *s1, V3 x, doubl
     const V3 a = sampled ? x + stream.green(step, radius) : x;
     if (s0.source_kind) out.a += green_term(s0, x, a, radius, paired_form);
     if (s1 && s1->source_kind) out.b += green_term(*s1, x, a, radius, paired_form);
-    x = x + stream.dir(step) * radius;
+    const V3 d = stream.dir(step);
+    if (step == 0) {
+      out.w0 = d * (3.0 / radius);
+      out.r0 = radius;
+      out.s0 = out.a;
+      out.s1 = out.b;
+    }
+    x = x + d * radius;
     ++out.steps;
   }
   if (trace && row < max_rows) {
@@ -602,14 +627,21 @@ WalkOut sample(const og_geom &g, const og_side sides[2], const og_params &p, con
     if (p.combine == OG_SQUARE) {
       WalkOut o = walk(g, sides[0], nullptr, z, p.eps, p.max_steps, st0, false, nullptr, 0, nullptr);
       o.b = o.a;
+      o.ga = o.gb = grad_sample(o, o.a, o.s0, sides[0]);
       return o;
     }
-    if (p.combine == OG_PAIRED)
-      return walk(g, sides[0], &sides[1], z, p.eps, p.max_steps, st0, false, nullptr, 0, nullptr);
+    if (p.combine == OG_PAIRED) {
+      WalkOut o = walk(g, sides[0], &sides[1], z, p.eps, p.max_steps, st0, false, nullptr, 0, nullptr);
+      o.ga = grad_sample(o, o.a, o.s0, sides[0]);
+      o.gb = grad_sample(o, o.b, o.s1, sides[1]);
+      return o;
+    }
     PhiloxStream st1{{0u, 0u}, uint32_t(w)};
     philox_key(p.seed, s, 1, st1.key);
     WalkOut a = walk(g, sides[0], nullptr, z, p.eps, p.max_steps, st0, false, nullptr, 0, nullptr);
     WalkOut b = walk(g, sides[1], nullptr, z, p.eps, p.max_steps, st1, false, nullptr, 0, nullptr);
+    a.ga = grad_sample(a, a.a, a.s0, sides[0]);
+    a.gb = grad_sample(b, b.a, b.s0, sides[1]);
     a.b = b.a;
     a.steps += b.steps;
     a.cap = a.cap || b.cap;
@@ -625,10 +657,15 @@ WalkOut sample(const og_geom &g, const og_side sides[2], const og_params &p, con
     st.rng.seed(splitmix64(p.seed) ^ splitmix64(uint64_t(s) + 91347) ^ splitmix64(uint64_t(w) + 571));
   }
   const bool paired_form = p.rng == OG_RNG_MT_PAIRED;
-  if (p.combine == OG_PAIRED)
-    return walk(g, sides[0], &sides[1], z, p.eps, p.max_steps, st, paired_form, nullptr, 0, nullptr);
+  if (p.combine == OG_PAIRED) {
+    WalkOut o = walk(g, sides[0], &sides[1], z, p.eps, p.max_steps, st, paired_form, nullptr, 0, nullptr);
+    o.ga = grad_sample(o, o.a, o.s0, sides[0]);
+    o.gb = grad_sample(o, o.b, o.s1, sides[1]);
+    return o;
+  }
   WalkOut o = walk(g, sides[0], nullptr, z, p.eps, p.max_steps, st, paired_form, nullptr, 0, nullptr);
   o.b = o.a;
+  o.ga = o.gb = grad_sample(o, o.a, o.s0, sides[0]);
   return o;
 }
 
@@ -785,6 +822,53 @@ void og_combine_powersums(const double s[8], int64_t count, double out[6]) {
   out[3] = std::sqrt((n - 1.0) / n * spread);
 }
 
+// Gradient channel of one site from its 28 sums (A = sum a, B = sum b,
+// C = sum a.b, sum a a^T as xx yy zz xy xz yz, sum b b^T, sum a b^T row-major):
+// grad_dot = (A.B - C)/(N(N-1)) over pairs of distinct walks, SE to first order
+// sqrt((b^T S_aa b + a^T S_bb a + 2 b^T S_ab a)/N) with a, b the means and S
+// the sample covariances.
+void grad_estimate(const double *g, int64_t n, og_site &r) {
+  const double N = double(n);
+  double a[3], b[3];
+  for (int k = 0; k < 3; ++k) {
+    a[k] = g[k] / N;
+    b[k] = g[3 + k] / N;
+    r.grad_u[k] = a[k];
+    r.grad_v[k] = b[k];
+  }
+  const double ab = g[0] * g[3] + g[1] * g[4] + g[2] * g[5];
+  r.grad_dot = n > 1 ? (ab - g[6]) / (N * (N - 1.0)) : std::nan("");
+  if (n < 4) {
+    r.grad_dot_se = std::nan("");
+    return;
+  }
+  const int ix[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
+  double v = 0.0;
+  for (int p = 0; p < 3; ++p)
+    for (int q = 0; q < 3; ++q) {
+      const double saa = (g[7 + ix[p][q]] - N * a[p] * a[q]) / (N - 1.0);
+      const double sbb = (g[13 + ix[p][q]] - N * b[p] * b[q]) / (N - 1.0);
+      const double sab = (g[19 + 3 * p + q] - N * a[p] * b[q]) / (N - 1.0);
+      v += b[p] * saa * b[q] + a[p] * sbb * a[q] + 2.0 * b[p] * sab * a[q];
+    }
+  r.grad_dot_se = std::sqrt(std::max(v, 0.0) / N);
+}
+
+void grad_accumulate(const V3 &ga, const V3 &gb, double *s) {
+  const double a[3] = {ga.x, ga.y, ga.z}, b[3] = {gb.x, gb.y, gb.z};
+  for (int q = 0; q < 3; ++q) {
+    s[q] += a[q];
+    s[3 + q] += b[q];
+  }
+  s[6] += a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+  s[7] += a[0] * a[0]; s[8] += a[1] * a[1]; s[9] += a[2] * a[2];
+  s[10] += a[0] * a[1]; s[11] += a[0] * a[2]; s[12] += a[1] * a[2];
+  s[13] += b[0] * b[0]; s[14] += b[1] * b[1]; s[15] += b[2] * b[2];
+  s[16] += b[0] * b[1]; s[17] += b[0] * b[2]; s[18] += b[1] * b[2];
+  for (int p = 0; p < 3; ++p)
+    for (int q = 0; q < 3; ++q) s[19 + 3 * p + q] += a[p] * b[q];
+}
+
 int32_t og_estimate(const og_geom *g, const og_side sides[2], const og_params *prm,
                     const double *sites, int64_t n, og_site *out, double *dump_a, double *dump_b,
                     int32_t *dump_steps, int64_t dump_stride) {
@@ -792,6 +876,7 @@ int32_t og_estimate(const og_geom *g, const og_side sides[2], const og_params *p
   if (p.walks < 2 || p.max_steps < 1 || !(p.eps > 0.0) || n < 0) return 2;
   if (p.rng != OG_RNG_PHILOX && p.combine == OG_CROSS) return 2;
   if (p.rng == OG_RNG_MT_KEY) return 2;
+  if (p.gradient && (sides[0].source_kind == 2 || sides[1].source_kind == 2)) return 2;
   const int32_t walks_max = std::max(p.walks, p.walks_max);
   int32_t status = 0;
 #ifdef _OPENMP
@@ -807,11 +892,13 @@ int32_t og_estimate(const og_geom *g, const og_side sides[2], const og_params *p
     int64_t steps = 0, caps = 0;
     int64_t done = 0, target = p.walks;
     double c[6];
+    double gs[28] = {0.0};
     while (true) {
       for (int64_t w = done; w < target; ++w) {
         const WalkOut o = sample(*g, sides, p, z, s, w);
         a.push_back(o.a);
         b.push_back(o.b);
+        if (p.gradient) grad_accumulate(o.ga, o.gb, gs);
         steps += o.steps;
         caps += o.cap ? 1 : 0;
         if (dump_a && w < dump_stride) {
@@ -836,11 +923,30 @@ int32_t og_estimate(const og_geom *g, const og_side sides[2], const og_params *p
     r.u = c[0]; r.v = c[1]; r.score = c[2]; r.score_se = c[3];
     r.td = -c[2]; r.same = c[4];
     r.walks = done; r.steps = steps; r.caps = caps;
+    if (p.gradient) {
+      grad_estimate(gs, done, r);
+    } else {
+      for (int q = 0; q < 3; ++q) r.grad_u[q] = r.grad_v[q] = std::nan("");
+      r.grad_dot = r.grad_dot_se = std::nan("");
+    }
     out[s] = r;
   }
   for (int64_t s = 0; s < n; ++s)
     if (out[s].caps) status = 3;
   return status;
+}
+
+void og_combine_grad(const double *ga, const double *gb, int64_t n, double out[8]) {
+  double s[28] = {0.0};
+  for (int64_t i = 0; i < n; ++i) grad_accumulate(load3(ga + 3 * i), load3(gb + 3 * i), s);
+  og_site r{};
+  grad_estimate(s, n, r);
+  for (int k = 0; k < 3; ++k) {
+    out[k] = r.grad_u[k];
+    out[3 + k] = r.grad_v[k];
+  }
+  out[6] = r.grad_dot;
+  out[7] = r.grad_dot_se;
 }
 
 int32_t og_trace(const og_geom *g, const og_side *side, const og_params *prm, const double start[3],

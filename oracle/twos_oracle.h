@@ -90,7 +90,7 @@ typedef struct {
   double ci_rel;
   double ci_z;
   int32_t threads;
-  int32_t pad;
+  int32_t gradient;    /* 1: first-step gradient channel (affine or no source) */
 } og_params;
 
 typedef struct {
@@ -98,6 +98,8 @@ typedef struct {
   int64_t walks, steps, caps;
   int32_t status; /* 0 finished (fixed budget or CI met), 1 budget exhausted before the CI target */
   int32_t pad;
+  double grad_u[3], grad_v[3]; /* gradient channel: mean samples of each side (NaN without it) */
+  double grad_dot, grad_dot_se; /* cross-index grad u . grad v and its delta-method SE */
 } og_site;
 
 uint64_t og_splitmix64(uint64_t x);
@@ -127,6 +129,9 @@ int32_t og_trace(const og_geom *g, const og_side *side, const og_params *prm,
 
 void og_combine(const double *a, const double *b, int64_t n, int32_t square, double out[6]);
 void og_combine_powersums(const double sums[8], int64_t n, double out[6]);
+/* Gradient channel of n walks from per-walk samples ga, gb ([n][3] each):
+ * out = {grad_u[3], grad_v[3], grad_dot, grad_dot_se}. */
+void og_combine_grad(const double *ga, const double *gb, int64_t n, double out[8]);
 
 #ifdef __cplusplus
 }

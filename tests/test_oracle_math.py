@@ -241,6 +241,67 @@ def test_green_point_moments(rng):
     assert abs(vals.mean() - wrong) > 10 * se
 
 
+def test_grad_combine_brute_force_and_jackknife():
+    """grad_dot counts every ordered pair of distinct walks once: equal to the
+    O(N^2) sum over i != j of a_i . b_j / (N(N-1)); its first-order SE tracks
+    the leave-one-out jackknife of that U-statistic for large N."""
+    rng = np.random.default_rng(5)
+    for n, corr in ((7, 0.0), (40, 0.6)):
+        a = rng.normal(size=(n, 3)) + [0.5, -1.0, 2.0]
+        b = corr * a + rng.normal(size=(n, 3)) + [1.0, 0.3, -0.2]
+        out = O.combine_grad(a, b)
+        brute = sum(a[i] @ b[j] for i in range(n) for j in range(n) if i != j) / (n * (n - 1))
+        assert abs(out[6] - brute) < 1e-12 * max(1.0, abs(brute))
+        np.testing.assert_allclose(out[:3], a.mean(0), rtol=1e-14)
+        np.testing.assert_allclose(out[3:6], b.mean(0), rtol=1e-14)
+    n = 3000
+    a = rng.normal(size=(n, 3)) + [0.5, -1.0, 2.0]
+    b = 0.4 * a + rng.normal(size=(n, 3)) + [1.0, 0.3, -0.2]
+    out = O.combine_grad(a, b)
+    A, B, Cs = a.sum(0), b.sum(0), np.einsum("ij,ij->", a, b)
+    loo = ((A - a) * (B - b)).sum(1) - (Cs - np.einsum("ij,ij->i", a, b))
+    loo /= (n - 1) * (n - 2)
+    jack = np.sqrt((n - 1) / n * np.sum((loo - loo.mean()) ** 2))
+    assert abs(out[7] / jack - 1) < 0.02
+    assert np.isnan(O.combine_grad(a[:3], b[:3])[7])  # no SE below four walks
+
+
+def test_gradient_channel_c1_exact():
+    """Configuration 1: grad g_u . grad g_v = 2xz - 2y + 0.5x and the mean
+    gradients (2x, -2y, 0.5), (z, 1, x) from the first-step weights."""
+    gu = O.side(poly=[0, 0, 0, 0.5, 1, -1, 0, 0, 0, 0], poly_classes=(0,))
+    gv = O.side(poly=[0, 0, 1, 0, 0, 0, 0, 0, 1, 0], poly_classes=(0,))
+    z = np.array([[0.1, 0.2, -0.3], [0.4, -0.2, 0.1], [0.0, 0.0, 0.0], [-0.5, 0.3, 0.2], [0.2, 0.5, 0.4]])
+    r = O.estimate(_ball_geometry(), z, [gu, gv], combine=O.CROSS, rng=O.RNG_PHILOX, seed=1, walks=20000,
+                   eps=1e-3, max_steps=4096, gradient=True, threads=8)
+    x, y, w = z.T
+    err = (r["grad_dot"] - (2 * x * w - 2 * y + 0.5 * x)) / r["grad_dot_se"]
+    assert np.max(np.abs(err)) < 4, err
+    np.testing.assert_allclose(r["grad_u"], np.c_[2 * x, -2 * y, 0.5 + 0 * x], atol=0.08)
+    np.testing.assert_allclose(r["grad_v"], np.c_[w, 1 + 0 * x, x], atol=0.08)
+    # Without the flag the channel is NaN.
+    r0 = O.estimate(_ball_geometry(), z[:1], [gu, gv], combine=O.CROSS, seed=1, walks=64, eps=1e-3)
+    assert np.isnan(r0["grad_dot"][0]) and np.all(np.isnan(r0["grad_u"][0]))
+
+
+def test_gradient_channel_affine_source():
+    """-Lap u = f, u = 0 on the unit sphere, f = f0 + g.x: u = f0 (1-|x|^2)/6 +
+    (g.x)(1-|x|^2)/10, so grad u = -f0 x/3 + g (1-|x|^2)/10 - (g.x) x/5; the
+    first-step sample needs the deterministic (r0^2/10) g term to get this."""
+    side = O.reference_affine_side(0.65)   # f = 1 + 0.65 (y + 0.5): f0 = 1.325, g = (0, 0.65, 0)
+    z = np.array([[0.0, 0.0, 0.0], [0.3, -0.2, 0.1], [-0.1, 0.4, 0.2]])
+    r = O.estimate(_ball_geometry(), z, [side], combine=O.SQUARE, rng=O.RNG_PHILOX, seed=2, walks=40000,
+                   eps=1e-4, max_steps=4096, gradient=True, threads=8)
+    f0, g = 1.325, np.array([0.0, 0.65, 0.0])
+    q = 1 - np.sum(z * z, 1)
+    exact = -f0 * z / 3 + np.outer(q, g) / 10 - np.outer(z @ g, np.ones(3)) * z / 5
+    # Sampling error per component is about 0.0015 at these sites (40000 walks).
+    np.testing.assert_allclose(r["grad_u"], exact, atol=0.008)
+    # At the centre the first sphere reaches the boundary, so the estimate is
+    # exactly (r0^2/10) g; with r0^2/6 it would be 0.108.
+    np.testing.assert_allclose(r["grad_u"][0], [0.0, 0.065, 0.0], atol=1e-12)
+
+
 def test_unit_ball_harmonic_td_c1():
     """Configuration 1: g_u = x^2 - y^2 + 0.5 z, g_v = x z + y are harmonic, so
     u = g_u, v = g_v and the Dirichlet-cavity TD is -g_u g_v

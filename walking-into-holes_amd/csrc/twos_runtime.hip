@@ -138,6 +138,7 @@ struct Engine {
   char *d_in = nullptr, *h_in = nullptr, *d_out = nullptr, *h_out = nullptr;
   // One-site work list of the plain schedule: 64 B in, 96 B out.
   char *d_plain = nullptr, *h_plain = nullptr;
+  double *d_grad = nullptr;  // [cap_walks][2][3] gradient samples, allocated on the first gradient request
   size_t in_bytes = 0, out_bytes = 0;
   size_t off_slot = 0, off_base = 0, off_prefix = 0, off_nact = 0, off_total = 0;
   size_t off_compact = 0, off_decision = 0;
@@ -167,6 +168,7 @@ struct Engine {
     dfree(s.pos); dfree(s.geom); dfree(s.problem); dfree(s.keys); dfree(s.site); dfree(s.seed);
     dfree(s.repeat); dfree(s.walks_max); dfree(s.max_steps); dfree(s.eps); dfree(s.ci_rel); dfree(s.ci_z);
     dfree(s.acc); dfree(s.acc_walks); dfree(s.acc_steps); dfree(s.acc_caps); dfree(s.result); dfree(s.hint0); dfree(s.grid_kappa);
+    dfree(s.acc_grad); dfree(d_grad);
     dfree(d_sides); dfree(d_val); dfree(d_steps); dfree(d_counters); dfree(d_in); dfree(d_out); dfree(d_plain);
     if (h_in) (void)hipHostFree(h_in);
     if (h_out) (void)hipHostFree(h_out);
@@ -189,6 +191,7 @@ struct Engine {
     HIP_OK(dalloc(&s.acc, 8 * C)); HIP_OK(dalloc(&s.acc_walks, C)); HIP_OK(dalloc(&s.acc_steps, C));
     HIP_OK(dalloc(&s.acc_caps, C)); HIP_OK(dalloc(&s.result, C)); HIP_OK(dalloc(&s.hint0, C));
     HIP_OK(dalloc(&s.grid_kappa, C));
+    HIP_OK(dalloc(&s.acc_grad, 28 * C));
     HIP_OK(dalloc(&d_sides, 2 * size_t(cap_problems)));
     HIP_OK(dalloc(&d_val, 2 * size_t(cap_walks)));
     HIP_OK(dalloc(&d_steps, 2 * size_t(cap_walks)));
@@ -207,6 +210,7 @@ struct Engine {
     HIP_OK(dalloc(&d_out, out_bytes));
     HIP_OK(hipHostMalloc(reinterpret_cast<void **>(&h_in), in_bytes, hipHostMallocDefault));
     HIP_OK(hipHostMalloc(reinterpret_cast<void **>(&h_out), out_bytes, hipHostMallocDefault));
+    static_assert(64 + sizeof(twos_site_result) + 4 <= 256, "plain-schedule block");
     HIP_OK(dalloc(&d_plain, 256));
     HIP_OK(hipHostMalloc(reinterpret_cast<void **>(&h_plain), 256, hipHostMallocDefault));
     std::memset(h_plain, 0, 256);
@@ -320,6 +324,10 @@ struct Engine {
     HIP_OK(hipMemsetAsync(s.acc_walks + s0, 0, 8 * n, st));
     HIP_OK(hipMemsetAsync(s.acc_steps + s0, 0, 8 * n, st));
     HIP_OK(hipMemsetAsync(s.acc_caps + s0, 0, 8 * n, st));
+    if (r->prm.flags & TWOS_FLAG_GRADIENT) {
+      HIP_OK(hipMemsetAsync(s.acc_grad + 28 * s0, 0, sizeof(double) * 28 * n, st));
+      if (!d_grad) HIP_OK(dalloc(&d_grad, 6 * size_t(cap_walks)));
+    }
     HIP_OK(hipMemcpyAsync(d_sides + 2 * r->problem, r->sides, sizeof(DevSide) * 2, hipMemcpyHostToDevice, st));
     // Each new site's first closest triangle seeds its paths' queries.
     if (ctx->d_geoms) HIP_OK(launch_slot_hints(tables(0, 0), s0, n, st));
@@ -361,6 +369,8 @@ struct Engine {
     return std::min<int32_t>(d, wmax - d);
   }
 
+  double *grad_of(const Request *r) const { return (r->prm.flags & TWOS_FLAG_GRADIENT) ? d_grad : nullptr; }
+
   twos_status launch_walks(const Request *r, int cls, int64_t e0, int64_t total_walks, int kind) {
     const twos_params &p = r->prm;
     Tables tb = tables(cls, e0);
@@ -372,6 +382,7 @@ struct Engine {
     wp.job_end = total_walks * wp.jobs_per_walk;
     wp.val = d_val;
     wp.steps = d_steps;
+    wp.grad = grad_of(r);
     wp.counters = d_counters + kMaxClasses;
     hipStream_t st = ctx->stream;
     const bool count = (p.flags & TWOS_FLAG_COUNT_TRAVERSAL) != 0;
@@ -426,7 +437,8 @@ struct Engine {
       // (eps, max_steps, the distance-grid threshold, sides, geometry) is
       // per slot, so a site's result is the same alone and in any mix.
       const int64_t key = int64_t(r->prm.precision) * 16 + r->prm.combine * 4 + (r->prm.rng != kRngPhilox ? 1 : 0) +
-                          ((r->prm.flags & TWOS_FLAG_COUNT_TRAVERSAL) ? 1024 : 0);
+                          ((r->prm.flags & TWOS_FLAG_COUNT_TRAVERSAL) ? 1024 : 0) +
+                          ((r->prm.flags & TWOS_FLAG_GRADIENT) ? 2048 : 0);
       int c = -1;
       for (size_t k = 0; k < class_key.size(); ++k)
         if (class_key[k] == key) c = int(k);
@@ -503,6 +515,7 @@ struct Engine {
           wp.job_end = int64_t(en.inc) * jpw;
           wp.val = d_val;
           wp.steps = d_steps;
+          wp.grad = grad_of(en.r);
           const int block = en.r->prm.block > 0 ? en.r->prm.block : 256;
           const int kind = geom_kind_of(en.r->geom);
           const hipError_t le = en.r->prm.rng != kRngPhilox ? launch_walk_simple_f64(kind, true, tb, wp, 64, strm)
@@ -534,8 +547,12 @@ struct Engine {
         }
       }
     } else if (graphable) {
-      const int64_t key = int64_t(r0->prm.precision) * 1000003 + r0->prm.combine * 1009 + kind0 * 7 +
-                          int64_t(r0->prm.waves_per_simd) * 100000007LL;
+      // Everything the captured launches depend on: precision, combine,
+      // geometry kind, launch shape and the channel flags.
+      const twos_params &q = r0->prm;
+      const int64_t key = int64_t(q.precision) | int64_t(q.combine) << 8 | int64_t(kind0) << 12 |
+                          int64_t(q.waves_per_simd) << 16 | int64_t(q.block) << 24 | int64_t(q.lds_stack) << 36 |
+                          int64_t(q.flags & 7) << 44;
       if (graph_key != key) {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -548,6 +565,7 @@ struct Engine {
         WalkParams wp{};
         wp.val = d_val;
         wp.steps = d_steps;
+        wp.grad = grad_of(r0);
         if (ls == TWOS_OK) ce = launch_reduce(tb, wp, int(cap_sites), strm);
         if (ls == TWOS_OK && ce == hipSuccess)
           ce = hipMemcpyAsync(h_out, d_out, out_bytes, hipMemcpyDeviceToHost, strm);
@@ -594,6 +612,7 @@ struct Engine {
         WalkParams wp{};
         wp.val = d_val;
         wp.steps = d_steps;
+        wp.grad = grad_of(rc);
         HIP_OK(launch_reduce(tb, wp, int(by_class[c].size()), strm));
         HIP_OK(hipEventRecord(ev[2], strm));
         HIP_OK(hipStreamSynchronize(strm));
@@ -733,6 +752,10 @@ Request *make_request(twos_ctx *ctx, int32_t geom, const twos_side sides[2], con
   }
   for (int k = 0; k < 2; ++k)
     if ((*st = check_side(sides[k])) != TWOS_OK) return nullptr;
+  if ((p->flags & TWOS_FLAG_GRADIENT) && (sides[0].source_kind == 2 || sides[1].source_kind == 2)) {
+    *st = fail(TWOS_ERR_INVALID_ARGUMENT, "the gradient channel needs an affine source or none");
+    return nullptr;
+  }
   if (n > 0 && (!sites || !out)) {
     *st = fail(TWOS_ERR_INVALID_ARGUMENT, "sites and out must be non-null");
     return nullptr;

@@ -60,6 +60,7 @@ struct SlotArrays {
   double *ci_rel;
   double *ci_z;
   double *acc;        // [C][8] binary64 power sums
+  double *acc_grad;   // [C][28] gradient-channel sums (A, B, C, second moments)
   int64_t *acc_walks;
   int64_t *acc_steps;
   int64_t *acc_caps;
@@ -90,7 +91,8 @@ struct WalkParams {
   double *trace;          // optional rows (x, y, z, r) for the trace entry point
   int32_t trace_rows;
   int32_t pad;
-  unsigned long long *counters;  // nodes, tris (when counting)
+  unsigned long long *counters;  // nodes, tris, grid lookups (when counting)
+  double *grad;                  // [total walks][2][3] gradient samples, or null
 };
 
 }  // namespace twos

@@ -51,6 +51,11 @@ struct Job {
   const DevSide *s0;
   const DevSide *s1;
   MeshView mv;  // the geometry's mesh buffers, kept per lane (unused fields fold away)
+  // Gradient channel (kernels built with kGrad only): the first sphere's
+  // 3 omega0 / r0, its radius (0 until the first step) and the path values
+  // after the first step's source term.
+  V3<R> gw;
+  R gr, gs0, gs1;
 };
 
 template <typename R>
@@ -68,6 +73,7 @@ __device__ __forceinline__ void job_init(Job<R> &j, const Tables &tb, const Walk
   j.acc0 = R(0);
   j.acc1 = R(0);
   j.steps = 0;
+  j.gr = R(0);
   j.hint = tb.slots.hint0[slot];
   j.max_steps = tb.slots.max_steps[slot];
   j.eps = R(tb.slots.eps[slot]);
@@ -83,6 +89,22 @@ __device__ __forceinline__ void job_init(Job<R> &j, const Tables &tb, const Walk
   j.s1 = tb.sides + 2 * prob + 1;
 }
 
+// Gradient sample of one side: (3/r0)(V - s0) omega0 + (r0^2/10) grad f for an
+// affine source; zero for a path that ended at its first query.
+template <typename R>
+__device__ __forceinline__ V3<R> grad_sample(const Job<R> &j, R value, R first, const DevSide &s) {
+  if (!(j.gr > R(0))) return {R(0), R(0), R(0)};
+  V3<R> g = j.gw * (value - first);
+  if (s.source_kind == 1) {
+    const R c = j.gr * j.gr / R(10);
+    g = g + V3<R>{R(s.src[1]) * c, R(s.src[2]) * c, R(s.src[3]) * c};
+  }
+  return g;
+}
+
+__device__ __forceinline__ void put3(double *o, V3<float> v) { o[0] = v.x; o[1] = v.y; o[2] = v.z; }
+__device__ __forceinline__ void put3(double *o, V3<double> v) { o[0] = v.x; o[1] = v.y; o[2] = v.z; }
+
 template <typename R>
 __device__ __forceinline__ void job_store(const Job<R> &j, const WalkParams &wp, bool capped) {
   const int32_t st = j.steps | (capped ? (1 << 30) : 0);
@@ -94,6 +116,30 @@ __device__ __forceinline__ void job_store(const Job<R> &j, const WalkParams &wp,
     wp.val[2 * j.t + 1] = double(wp.combine == kPaired ? j.acc1 : j.acc0);
     wp.steps[2 * j.t + 0] = st;
     wp.steps[2 * j.t + 1] = 0;
+  }
+  if (wp.grad) {
+    double *o = wp.grad + 6 * j.t;
+    if (wp.combine == kCross) {
+      put3(o + 3 * j.side, grad_sample(j, j.acc0, j.gs0, *j.s0));
+    } else {
+      const V3<R> a = grad_sample(j, j.acc0, j.gs0, *j.s0);
+      put3(o, a);
+      put3(o + 3, wp.combine == kPaired ? grad_sample(j, j.acc1, j.gs1, *j.s1) : a);
+    }
+  }
+}
+
+// First step of a path taken in direction d with radius r: keep what the
+// gradient sample needs (after the step's source term was added).
+template <typename R, bool kGrad>
+__device__ __forceinline__ void job_first_step(Job<R> &j, V3<R> d, R r) {
+  if constexpr (kGrad) {
+    if (j.steps == 0) {
+      j.gw = d * (R(3) / r);
+      j.gr = r;
+      j.gs0 = j.acc0;
+      j.gs1 = j.acc1;
+    }
   }
 }
 
@@ -148,7 +194,7 @@ __device__ __forceinline__ V3<R> philox_dir(const Job<R> &j) {
 // Simple kernel: one thread per job, the whole path in one go.  Used by the
 // plain schedule (one launch per site and round), by the mt19937_64 replay
 // streams and by the trace entry point.
-template <typename R, int GK, bool kMt>
+template <typename R, int GK, bool kMt, bool kGrad = false>
 __global__ __launch_bounds__(256) void walk_simple(Tables tb, WalkParams wp) {
   const int64_t u = wp.job_begin + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (u >= wp.job_end) return;
@@ -194,6 +240,7 @@ __global__ __launch_bounds__(256) void walk_simple(Tables tb, WalkParams wp) {
     } else {
       d = philox_dir(j);
     }
+    job_first_step<R, kGrad>(j, d, r);
     j.x = j.x + d * r;
     ++j.steps;
   }
@@ -222,7 +269,7 @@ constexpr size_t persistent_lds_bytes(int lds, int block) { return size_t(lds) *
 
 // Applies a finished boundary query (distance r, closest point bp of class
 // cls) to the path: 0 moved on, 1 ended in the eps-shell, 2 capped.
-template <typename R>
+template <typename R, bool kGrad = false>
 __device__ __forceinline__ int job_advance(Job<R> &j, const WalkParams &wp, R r, V3<R> bp, int cls) {
   if (!isfinite(r)) return 2;  // as in job_step: a failed query ends the path capped
   const DevSide &s0 = *j.s0;
@@ -233,6 +280,7 @@ __device__ __forceinline__ int job_advance(Job<R> &j, const WalkParams &wp, R r,
   }
   job_source(j, wp, r, job_sampled(j, wp) ? j.x + philox_green<R>(j.key, j.walk, j.steps, r) : j.x);
   const V3<R> d = philox_dir(j);
+  job_first_step<R, kGrad>(j, d, r);
   j.x = j.x + d * r;
   ++j.steps;
   return j.steps >= j.max_steps ? 2 : 0;
@@ -268,7 +316,7 @@ __device__ __forceinline__ bool query_start(Job<R> &j, MeshQuery<R> &q, Traversa
 // (amdgpu_waves_per_eu); 1 leaves the choice to the compiler.  kLds is the
 // per-lane LDS stack depth; the launch passes persistent_lds_bytes(kLds,
 // blockDim.x) of dynamic LDS.
-template <typename R, int GK, bool kCount, int kWaves, int kLds>
+template <typename R, int GK, bool kCount, int kWaves, int kLds, bool kGrad = false>
 __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWaves))) void walk_persistent(
     Tables tb, WalkParams wp) {
   extern __shared__ uint32_t lds_stack[];
@@ -329,7 +377,7 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWav
           r = container_analytic<R>(*j.g, j.x, bp, cls);
         }
         r = holes_and_cavities<R, GK>(*j.g, j.x, r, bp, cls);
-        const int st = job_advance(j, wp, r, bp, cls);
+        const int st = job_advance<R, kGrad>(j, wp, r, bp, cls);
         if (st == 0) {
           in_query = query_start<R, GK, kCount>(j, q, &cnt);
         } else {
@@ -458,6 +506,8 @@ __global__ __launch_bounds__(kBlock) void reduce_sites(Tables tb, WalkParams wp)
     tb.slots.acc_caps[slot] += tcaps;
     twos_site_result r;
     site_estimate(acc, walks, r);
+    for (int q = 0; q < 3; ++q) r.grad_u[q] = r.grad_v[q] = __builtin_nan("");
+    r.grad_dot = r.grad_dot_se = __builtin_nan("");  // filled by reduce_grad when requested
     r.walks = walks;
     r.steps = tb.slots.acc_steps[slot];
     r.caps = tb.slots.acc_caps[slot];
@@ -474,6 +524,95 @@ __global__ __launch_bounds__(kBlock) void reduce_sites(Tables tb, WalkParams wp)
     tb.slots.result[slot] = r;
     tb.round.compact[k] = r;
     tb.round.decision[k] = decision;
+  }
+}
+
+// Gradient channel of one site from its 28 sums: A = sum a (3), B = sum b (3),
+// C = sum a.b, the second moments sum a a^T (6: xx yy zz xy xz yz), sum b b^T
+// (6) and sum a b^T (9, row-major).  grad_dot = (A.B - C)/(N(N-1)) counts
+// every pair of distinct walks; its SE is the first-order (delta-method) one,
+// sqrt((b^T S_aa b + a^T S_bb a + 2 b^T S_ab a) / N) with a, b the means and S
+// the sample covariances.
+__device__ __forceinline__ void grad_estimate(const double *g, int64_t n, twos_site_result &r) {
+  const double N = double(n);
+  double a[3], b[3];
+  for (int k = 0; k < 3; ++k) {
+    a[k] = g[k] / N;
+    b[k] = g[3 + k] / N;
+    r.grad_u[k] = a[k];
+    r.grad_v[k] = b[k];
+  }
+  const double ab = g[0] * g[3] + g[1] * g[4] + g[2] * g[5];
+  r.grad_dot = n > 1 ? (ab - g[6]) / (N * (N - 1.0)) : __builtin_nan("");
+  if (n < 4) {
+    r.grad_dot_se = __builtin_nan("");
+    return;
+  }
+  const int ix[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
+  double saa[3][3], sbb[3][3], sab[3][3];
+  for (int p = 0; p < 3; ++p)
+    for (int q = 0; q < 3; ++q) {
+      saa[p][q] = (g[7 + ix[p][q]] - N * a[p] * a[q]) / (N - 1.0);
+      sbb[p][q] = (g[13 + ix[p][q]] - N * b[p] * b[q]) / (N - 1.0);
+      sab[p][q] = (g[19 + 3 * p + q] - N * a[p] * b[q]) / (N - 1.0);
+    }
+  double v = 0.0;
+  for (int p = 0; p < 3; ++p)
+    for (int q = 0; q < 3; ++q) v += b[p] * saa[p][q] * b[q] + a[p] * sbb[p][q] * a[q] + 2.0 * b[p] * sab[p][q] * a[q];
+  r.grad_dot_se = sqrt(fmax(v, 0.0) / N);
+}
+
+// Per-site fold of the gradient samples, after reduce_sites (same work list):
+// adds the round's 28 sums to the slot's and completes the result's gradient
+// fields (in walk order within each lane, then a fixed tree, like the value
+// channel).
+template <int kBlock>
+__global__ __launch_bounds__(kBlock) void reduce_grad(Tables tb, WalkParams wp) {
+  const int32_t k = blockIdx.x;
+  if (k >= *tb.round.n_active) return;
+  const int32_t slot = tb.round.act_slot[k];
+  const int64_t t0 = tb.round.act_prefix[k], t1 = tb.round.act_prefix[k + 1];
+  double s[28];
+#pragma unroll
+  for (int i = 0; i < 28; ++i) s[i] = 0.0;
+  for (int64_t t = t0 + threadIdx.x; t < t1; t += kBlock) {
+    const double *o = wp.grad + 6 * t;
+    const double a[3] = {o[0], o[1], o[2]}, b[3] = {o[3], o[4], o[5]};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      s[q] += a[q];
+      s[3 + q] += b[q];
+    }
+    s[6] += a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+    s[7] += a[0] * a[0]; s[8] += a[1] * a[1]; s[9] += a[2] * a[2];
+    s[10] += a[0] * a[1]; s[11] += a[0] * a[2]; s[12] += a[1] * a[2];
+    s[13] += b[0] * b[0]; s[14] += b[1] * b[1]; s[15] += b[2] * b[2];
+    s[16] += b[0] * b[1]; s[17] += b[0] * b[2]; s[18] += b[1] * b[2];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) s[19 + 3 * p + q] += a[p] * b[q];
+  }
+  __shared__ double red[kBlock / 64][28];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int i = 0; i < 28; ++i) s[i] = wave_sum(s[i]);
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < 28; ++i) red[wave][i] = s[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double *acc = tb.slots.acc_grad + 28 * size_t(slot);
+    for (int i = 0; i < 28; ++i) {
+      double tot = 0.0;
+      for (int w = 0; w < kBlock / 64; ++w) tot += red[w][i];
+      acc[i] += tot;
+    }
+    // reduce_sites has already counted this round's walks into acc_walks.
+    twos_site_result r = tb.slots.result[slot];
+    grad_estimate(acc, tb.slots.acc_walks[slot], r);
+    tb.slots.result[slot] = r;
+    tb.round.compact[k] = r;
   }
 }
 

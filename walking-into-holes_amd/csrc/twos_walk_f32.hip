@@ -16,6 +16,10 @@ hipError_t launch_walk_simple_f32(int geom_kind, const Tables &tb, const WalkPar
   const int64_t n = wp.job_end - wp.job_begin;
   if (n <= 0) return hipSuccess;
   const dim3 grid(blocks_for(n, block)), blk(block);
+  if (wp.grad) {  // gradient channel: one generic variant
+    hipLaunchKernelGGL((walk_simple<float, kGeomGeneric, false, true>), grid, blk, 0, stream, tb, wp);
+    return hipGetLastError();
+  }
   switch (geom_kind) {
     case kGeomMesh:
       hipLaunchKernelGGL((walk_simple<float, kGeomMesh, false>), grid, blk, 0, stream, tb, wp);
@@ -59,6 +63,16 @@ hipError_t persistent_f32(int geom_kind, bool count, const Tables &tb, const Wal
 // is one resident wave set: blocks-per-CU from the occupancy query times CUs.
 hipError_t launch_walk_persistent_f32(int geom_kind, bool count, int waves, int lds, const Tables &tb,
                                       const WalkParams &wp, int block, int cus, hipStream_t stream, int *grid_out) {
+  if (wp.grad) {  // gradient channel: the compiler's allocation, 24-entry stack, any geometry
+    void (*k)(Tables, WalkParams) = walk_persistent<float, kGeomGeneric, false, 1, kLdsDefault, true>;
+    const size_t smem = persistent_lds_bytes(kLdsDefault, block);
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, smem);
+    if (e != hipSuccess) return e;
+    *grid_out = std::max(1, per_cu) * cus;
+    hipLaunchKernelGGL(k, dim3(*grid_out), dim3(block), smem, stream, tb, wp);
+    return hipGetLastError();
+  }
   if (lds == 16) {
     if (waves == 5) return persistent_f32<5, 16>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
     return persistent_f32<1, 16>(geom_kind, count, tb, wp, block, cus, stream, grid_out);

@@ -19,7 +19,12 @@ hipError_t launch_walk_simple_f64(int geom_kind, bool mt, const Tables &tb, cons
   if (n <= 0) return hipSuccess;
   const dim3 grid(blocks_for(n, block)), blk(block);
   if (mt) {
-    hipLaunchKernelGGL((walk_simple<double, kGeomGeneric, true>), grid, blk, 0, stream, tb, wp);
+    if (wp.grad) hipLaunchKernelGGL((walk_simple<double, kGeomGeneric, true, true>), grid, blk, 0, stream, tb, wp);
+    else hipLaunchKernelGGL((walk_simple<double, kGeomGeneric, true>), grid, blk, 0, stream, tb, wp);
+    return hipGetLastError();
+  }
+  if (wp.grad) {  // gradient channel: one generic variant
+    hipLaunchKernelGGL((walk_simple<double, kGeomGeneric, false, true>), grid, blk, 0, stream, tb, wp);
     return hipGetLastError();
   }
   switch (geom_kind) {
@@ -38,11 +43,12 @@ hipError_t launch_walk_simple_f64(int geom_kind, bool mt, const Tables &tb, cons
 hipError_t launch_walk_persistent_f64(int geom_kind, bool count, const Tables &tb, const WalkParams &wp, int block,
                                       int cus, hipStream_t stream, int *grid_out) {
   void (*k)(Tables, WalkParams);
-  if (count) k = walk_persistent<double, kGeomGeneric, true, 1, kLdsDefault>;
+  if (wp.grad) k = walk_persistent<double, kGeomGeneric, false, 1, kLdsDefault, true>;
+  else if (count) k = walk_persistent<double, kGeomGeneric, true, 1, kLdsDefault>;
   else if (geom_kind == kGeomMesh) k = walk_persistent<double, kGeomMesh, false, 1, kLdsDefault>;
   else if (geom_kind == kGeomAnalytic) k = walk_persistent<double, kGeomAnalytic, false, 1, kLdsDefault>;
   else k = walk_persistent<double, kGeomGeneric, false, 1, kLdsDefault>;
-  const size_t smem = geom_kind == kGeomAnalytic && !count ? 0 : persistent_lds_bytes(kLdsDefault, block);
+  const size_t smem = geom_kind == kGeomAnalytic && !count && !wp.grad ? 0 : persistent_lds_bytes(kLdsDefault, block);
   int per_cu = 0;
   hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, smem);
   if (e != hipSuccess) return e;
@@ -64,9 +70,12 @@ hipError_t launch_zero(unsigned long long *p, int n, hipStream_t stream) {
   return hipGetLastError();
 }
 
+// The per-site fold; with gradient samples (wp.grad) a second fold of the
+// gradient channel follows on the same stream.
 hipError_t launch_reduce(const Tables &tb, const WalkParams &wp, int grid, hipStream_t stream) {
   if (grid <= 0) return hipSuccess;
   hipLaunchKernelGGL((reduce_sites<kReduceBlock>), dim3(grid), dim3(kReduceBlock), 0, stream, tb, wp);
+  if (wp.grad) hipLaunchKernelGGL((reduce_grad<kReduceBlock>), dim3(grid), dim3(kReduceBlock), 0, stream, tb, wp);
   return hipGetLastError();
 }
 

@@ -24,14 +24,14 @@ import numpy as np
 
 from . import _capi
 from ._capi import (BALL, BOX, CROSS, MESH, PAIRED, PERSISTENT, PLAIN, RNG_MT_KEY, RNG_MT_PAIRED,
-                    RNG_MT_SITE, RNG_PHILOX, SQUARE, FLAG_COUNT_TRAVERSAL, FLAG_DISTANCE_GRID, MAX_TERMS,
+                    RNG_MT_SITE, RNG_PHILOX, SQUARE, FLAG_COUNT_TRAVERSAL, FLAG_DISTANCE_GRID, FLAG_GRADIENT, MAX_TERMS,
                     TERM_BUMP, TERM_GAUSS, TERM_UNIT)
 
 __all__ = [
     "Context", "Mesh", "Side", "Result", "Server", "TwosError", "InvalidArgument", "StepCapError",
     "GeometryError", "AdmissionError", "DeviceError", "NotFoundError", "side", "reference_affine_side",
     "BALL", "BOX", "MESH", "SQUARE", "PAIRED", "CROSS", "RNG_PHILOX", "RNG_MT_SITE", "RNG_MT_PAIRED",
-    "RNG_MT_KEY", "PLAIN", "PERSISTENT", "FLAG_COUNT_TRAVERSAL", "FLAG_DISTANCE_GRID", "version",
+    "RNG_MT_KEY", "PLAIN", "PERSISTENT", "FLAG_COUNT_TRAVERSAL", "FLAG_DISTANCE_GRID", "FLAG_GRADIENT", "version",
     "TERM_BUMP", "TERM_UNIT", "TERM_GAUSS", "MAX_TERMS", "study_terms", "housing_chip_terms",
 ]
 
@@ -199,6 +199,10 @@ class Result:
     caps: np.ndarray
     status: np.ndarray
     stats: dict = field(default_factory=dict)
+    grad_u: np.ndarray | None = None    # [n][3] with gradient=True
+    grad_v: np.ndarray | None = None
+    grad_dot: np.ndarray | None = None
+    grad_dot_se: np.ndarray | None = None
     a: np.ndarray | None = None
     b: np.ndarray | None = None
     walk_steps: np.ndarray | None = None
@@ -208,15 +212,32 @@ class Result:
         return self.caps
 
 
-def _result(out, n, stats: dict) -> Result:
-    dt = np.dtype([(f[0], np.float64 if f[1] is C.c_double else (np.int64 if f[1] is C.c_int64 else np.int32))
-                   for f in _capi.SiteResult._fields_])
+def _dtype_of(ct):
+    if ct is C.c_double:
+        return np.float64
+    if ct is C.c_int64:
+        return np.int64
+    if ct is C.c_int32:
+        return np.int32
+    return (_dtype_of(ct._type_), (ct._length_,))  # fixed-size array field
+
+
+def _result(out, n, stats: dict, gradient: bool = False) -> Result:
+    fields = []
+    for name, ct in _capi.SiteResult._fields_:
+        d = _dtype_of(ct)
+        fields.append((name, *d) if isinstance(d, tuple) else (name, d))
+    dt = np.dtype(fields)
     assert dt.itemsize == C.sizeof(_capi.SiteResult)
     rec = np.frombuffer(out, dtype=dt, count=n).copy() if n else np.zeros(0, dtype=dt)
-    return Result(u=rec["u"].copy(), v=rec["v"].copy(), score=rec["score"].copy(),
-                  score_se=rec["score_se"].copy(), td=rec["td"].copy(), same=rec["same"].copy(),
-                  walks=rec["walks"].copy(), steps=rec["steps"].copy(), caps=rec["caps"].copy(),
-                  status=rec["status"].copy(), stats=stats)
+    res = Result(u=rec["u"].copy(), v=rec["v"].copy(), score=rec["score"].copy(),
+                 score_se=rec["score_se"].copy(), td=rec["td"].copy(), same=rec["same"].copy(),
+                 walks=rec["walks"].copy(), steps=rec["steps"].copy(), caps=rec["caps"].copy(),
+                 status=rec["status"].copy(), stats=stats)
+    if gradient:
+        res.grad_u, res.grad_v = rec["grad_u"].copy(), rec["grad_v"].copy()
+        res.grad_dot, res.grad_dot_se = rec["grad_dot"].copy(), rec["grad_dot_se"].copy()
+    return res
 
 
 def _stats(st: _capi.Stats) -> dict:
@@ -363,13 +384,15 @@ class Context:
     def estimate(self, geom: int, sites, sides, *, combine=SQUARE, rng=RNG_PHILOX, precision=32,
                  schedule=PERSISTENT, seed=0, repeat=0, walks=16, walks_max=0, ci_rel=0.02, ci_z=1.96,
                  eps=5e-4, max_steps=512, use_graph=False, block=0, waves_per_simd=0, site_offset=0,
-                 count_traversal=False, distance_grid=False, grid_kappa=0.0, lds_stack=0, dump: bool = False,
-                 raise_on_cap: bool = True) -> Result:
+                 count_traversal=False, distance_grid=False, grid_kappa=0.0, lds_stack=0, gradient=False,
+                 dump: bool = False, raise_on_cap: bool = True) -> Result:
         """Per-site TWoS estimate.  walks_max > walks doubles each site's walks
         until ci_z * score_se <= ci_rel * |score| or walks_max is reached.
         distance_grid: mesh steps far from the boundary take a conservative
         radius from the mesh's distance grid (unbiased; paths differ from the
-        exact walk) when it is at least grid_kappa cell sizes (0: 4)."""
+        exact walk) when it is at least grid_kappa cell sizes (0: 4).
+        gradient: also estimate grad u, grad v and their cross-index product
+        from each path's first step (Result.grad_*)."""
         p = _f64(sites).reshape(-1, 3)
         n = len(p)
         sd = (Side * 2)(sides[0], sides[1] if len(sides) > 1 else sides[0])
@@ -378,7 +401,8 @@ class Context:
                       ci_rel=ci_rel, ci_z=ci_z, use_graph=use_graph, block=block,
                       waves_per_simd=waves_per_simd, site_offset=site_offset,
                       flags=(FLAG_COUNT_TRAVERSAL if count_traversal else 0)
-                      | (FLAG_DISTANCE_GRID if distance_grid else 0), grid_kappa=grid_kappa, lds_stack=lds_stack)
+                      | (FLAG_DISTANCE_GRID if distance_grid else 0) | (FLAG_GRADIENT if gradient else 0),
+                      grid_kappa=grid_kappa, lds_stack=lds_stack)
         out = (_capi.SiteResult * max(n, 1))()
         st = _capi.Stats()
         if dump:
@@ -392,7 +416,7 @@ class Context:
         else:
             status = _lib().twos_estimate(self._h, geom, sd, C.byref(prm), _ptr(p, _capi._dp), n, out,
                                           C.byref(st))
-        res = _result(out, n, _stats(st))
+        res = _result(out, n, _stats(st), gradient)
         if dump:
             res.a, res.b, res.walk_steps = da, db, ds
         if status == _capi.ERR_STEP_CAP and not raise_on_cap:

@@ -27,6 +27,7 @@ RNG_PHILOX, RNG_MT_SITE, RNG_MT_PAIRED, RNG_MT_KEY = 0, 1, 2, 3
 PLAIN, PERSISTENT = 0, 1
 FLAG_COUNT_TRAVERSAL = 1
 FLAG_DISTANCE_GRID = 2
+FLAG_GRADIENT = 4
 TERM_BUMP, TERM_UNIT, TERM_GAUSS = 0, 1, 2
 MAX_TERMS = 8
 
@@ -62,7 +63,9 @@ class Params(C.Structure):
 class SiteResult(C.Structure):
     _fields_ = [("u", C.c_double), ("v", C.c_double), ("score", C.c_double), ("score_se", C.c_double),
                 ("td", C.c_double), ("same", C.c_double), ("walks", C.c_int64), ("steps", C.c_int64),
-                ("caps", C.c_int64), ("status", C.c_int32), ("pad", C.c_int32)]
+                ("caps", C.c_int64), ("status", C.c_int32), ("pad", C.c_int32),
+                ("grad_u", C.c_double * 3), ("grad_v", C.c_double * 3), ("grad_dot", C.c_double),
+                ("grad_dot_se", C.c_double)]
 
 
 class Stats(C.Structure):
