@@ -67,7 +67,8 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--lds-stack", type=int, default=0, help="fixed launch shape (skips the autotuner)")
     ap.add_argument("--grid-kappa", type=float, default=0.0, help="fixed launch shape (skips the autotuner)")
     ap.add_argument("--grid-cells", type=int, default=0, help="fixed distance-grid resolution (skips the autotuner)")
-    ap.add_argument("--grid-corners", type=int, default=-1, help="fixed distance-grid sampling: 0 centres, 1 corners")
+    ap.add_argument("--grid-corners", type=int, default=-1,
+                    help="fixed distance-grid mode: 0 centres, 1 corners, 2 binary16 corner records")
     ap.add_argument("--exact", action="store_true", help="exact closest-point radius at every mesh step")
     ap.add_argument("--sites", type=int, default=0, help="override the site count (profiling runs)")
     return ap.parse_args()
@@ -132,7 +133,7 @@ def main() -> int:
     if fixed or args.grid_cells > 0 or args.grid_corners >= 0:
         shape = fixed
         if (args.grid_cells > 0 or args.grid_corners >= 0) and mesh is not None:
-            ctx.set_distance_grid(mesh, args.grid_cells if args.grid_cells > 0 else 128, args.grid_corners > 0)
+            ctx.set_distance_grid(mesh, args.grid_cells if args.grid_cells > 0 else 128, max(args.grid_corners, 0))
     elif not args.no_autotune:
         from twos import autotune
         t0 = time.perf_counter()
@@ -193,8 +194,10 @@ def main() -> int:
         n_nodes = cnt.stats["nodes_visited"] / q_sub
         n_tris = cnt.stats["tris_tested"] / q_sub
         n_grid = cnt.stats["grid_lookups"] / q_sub
-        corners = out_shape.get("grid_corners", 0) == 1
-        b_query = NODE_BYTES * n_nodes + TRI_BYTES * n_tris + GRID_BYTES * (8 if corners else 1) * n_grid
+        # Samples per grid lookup: one binary32 centre, eight binary32 corners,
+        # or one 16-byte record of eight binary16 corners.
+        grid_bytes = {0: GRID_BYTES, 1: 8 * GRID_BYTES, 2: 16}[out_shape.get("grid_corners", 0)]
+        b_query = NODE_BYTES * n_nodes + TRI_BYTES * n_tris + grid_bytes * n_grid
         achieved = queries * b_query / (eager_ms * 1e-3) / 1e9
         traffic = measured_traffic(w.name, n, out_shape, launches)
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

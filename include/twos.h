@@ -219,9 +219,11 @@ twos_status twos_mesh_create(twos_ctx *ctx, const double *vertices, int64_t nv, 
 twos_status twos_mesh_info(twos_ctx *ctx, int32_t mesh_id, int32_t *depth, int64_t *nodes, int64_t *bytes);
 /* Rebuilds the mesh's conservative distance grid with `cells` cells along the
  * longest axis of its bounding box (8..512; meshes start at 128 cells, values
- * at cell centres) and updates every geometry on the mesh.  corners != 0
- * samples the distance at the cell corners and bounds a step by the best of
- * a cell's eight corners.  No counterpart in the reference (Embree has no
+ * at cell centres) and updates every geometry on the mesh.  corners 1 samples
+ * the distance at the cell corners and bounds a step by the best of a cell's
+ * eight corners; corners 2 stores those eight values per cell rounded down to
+ * binary16 in one 16-byte record (one load per lookup, four times the
+ * memory).  No counterpart in the reference (Embree has no
  * distance grid); it is a tuning control for TWOS_FLAG_DISTANCE_GRID. */
 twos_status twos_mesh_set_distance_grid(twos_ctx *ctx, int32_t mesh_id, int32_t cells, int32_t corners,
                                         double *build_ms);
@@ -245,6 +247,11 @@ twos_status twos_estimate_dump(twos_ctx *ctx, int32_t geom_id, const twos_side s
 twos_status twos_inside(twos_ctx *ctx, int32_t mesh_id, const double *points, int64_t n, int32_t *out);
 twos_status twos_closest(twos_ctx *ctx, int32_t geom_id, int32_t precision, const double *points, int64_t n,
                          double *distance, double *closest, int32_t *cls);
+/* The conservative lower bound of the mesh distance that the walk takes from
+ * the geometry's distance grid at each point (-inf without a grid).  No
+ * counterpart in the reference: a diagnostic of TWOS_FLAG_DISTANCE_GRID. */
+twos_status twos_distance_bound(twos_ctx *ctx, int32_t geom_id, int32_t precision, const double *points, int64_t n,
+                                double *bound);
 twos_status twos_trace(twos_ctx *ctx, int32_t geom_id, const twos_side *side, const twos_params *params,
                        const double start[3], int64_t site, int32_t walk, double *rows, int32_t max_rows,
                        int32_t *n_rows);

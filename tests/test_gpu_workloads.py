@@ -75,7 +75,7 @@ def test_c3_distance_grid_against_exact(ctx, c3, kappa):
     assert abs(np.mean(z)) < 0.35  # no systematic shift (128 sites: sd of the mean ~0.09)
 
 
-@pytest.mark.parametrize("corners", [False, True])
+@pytest.mark.parametrize("corners", [0, 1, 2])
 def test_c3_finer_distance_grid_against_exact(ctx, c3, corners):
     """A rebuilt 256-cell grid (centre or eight-corner bound) changes the paths
     and still estimates the exact walk's quantity; the mesh is restored to its
@@ -96,6 +96,27 @@ def test_c3_finer_distance_grid_against_exact(ctx, c3, corners):
     assert abs(np.mean(z)) < 0.35
     again = ctx.estimate(g, sites, W.sides(w, twos.side), seed=3, distance_grid=True, grid_kappa=2.0, **kw)
     np.testing.assert_array_equal(again.steps, coarse.steps)
+
+
+@pytest.mark.parametrize("mode,cells", [(0, 128), (1, 256), (2, 256), (2, 96)])
+@pytest.mark.parametrize("precision", [32, 64])
+def test_distance_grid_bound_is_conservative(ctx, c3, mode, cells, precision):
+    """At 20,000 points inside and around the mesh, the grid's lower bound
+    never exceeds the exact distance (binary64 closest point), in every grid
+    mode; and it is tight enough to be useful far from the boundary."""
+    w, mesh, g = c3
+    rng = np.random.default_rng(cells + mode)
+    lo, hi = w.vertices.min(0), w.vertices.max(0)
+    pts = rng.uniform(lo - 0.05, hi + 0.05, (20000, 3))
+    try:
+        ctx.set_distance_grid(mesh, cells, mode)
+        bound = ctx.distance_bound(g, pts, precision=precision)
+    finally:
+        ctx.set_distance_grid(mesh, 128, 0)
+    exact, _, _ = ctx.closest(g, pts, precision=64)
+    assert np.all(bound <= exact), (np.max(bound - exact), mode, cells)
+    far = exact > 0.1
+    assert np.median((exact - bound)[far]) < 1.5 * (hi - lo).max() / cells
 
 
 def test_c3_distance_grid_plain_equals_persistent(ctx, c3):

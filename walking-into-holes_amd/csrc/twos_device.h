@@ -517,12 +517,17 @@ __device__ __forceinline__ R mesh_distance(const DevGeom &g, R d2) {
 //   corner grid: the best of the eight corners of that cell, which is much
 //   tighter where d is smooth (the corner most aligned with the gradient
 //   loses |p - c| (1 - cos angle) instead of up to |p - c|).
+//   corner records: the same eight corners stored per cell in binary16,
+//   rounded down (so still lower bounds), read with one 16-byte load.
 // Any sample gives a valid bound, so the cell index may round either way at a
 // cell face: multiply by 1/h instead of dividing.
+__device__ __forceinline__ float f16_lo(uint32_t w) { return float(__builtin_bit_cast(_Float16, (unsigned short)(w & 0xffffu))); }
+__device__ __forceinline__ float f16_hi(uint32_t w) { return float(__builtin_bit_cast(_Float16, (unsigned short)(w >> 16))); }
+
 template <typename R>
 __device__ __forceinline__ R grid_lower(const DevGeom &g, V3<R> p) {
   const R h = R(g.grid_h), inv_h = R(g.grid_inv_h);
-  const int last = g.grid_corners ? 2 : 1;  // highest first index of a cell
+  const int last = g.grid_corners == 1 ? 2 : 1;  // highest first index of a cell (per sample count)
   int i = int(floor((p.x - R(g.grid_lo[0])) * inv_h));
   int k = int(floor((p.y - R(g.grid_lo[1])) * inv_h));
   int l = int(floor((p.z - R(g.grid_lo[2])) * inv_h));
@@ -538,6 +543,22 @@ __device__ __forceinline__ R grid_lower(const DevGeom &g, V3<R> p) {
   }
   const R x0 = p.x - (R(g.grid_lo[0]) + R(i) * h), y0 = p.y - (R(g.grid_lo[1]) + R(k) * h),
           z0 = p.z - (R(g.grid_lo[2]) + R(l) * h);
+  if (g.grid_corners == 2) {
+    const uint4 rec = reinterpret_cast<const uint4 *>(g.grid)[(size_t(i) * g.grid_n[1] + k) * g.grid_n[2] + l];
+    const uint32_t w[4] = {rec.x, rec.y, rec.z, rec.w};
+    R best = -Inf<R>::v();
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const R dx = x0 - R(a) * h, dy = y0 - R(b) * h;
+        const R dxy = dx * dx + dy * dy;
+        const R d0 = R(f16_lo(w[2 * a + b])), d1 = R(f16_hi(w[2 * a + b]));
+        best = fmax(best, d0 - sqrt(dxy + z0 * z0) - R(2e-6) * (R(1) + d0));
+        best = fmax(best, d1 - sqrt(dxy + (z0 - h) * (z0 - h)) - R(2e-6) * (R(1) + d1));
+      }
+    return best;
+  }
   const float *row = g.grid + (size_t(i) * g.grid_n[1] + k) * g.grid_n[2] + l;
   const size_t si = size_t(g.grid_n[1]) * g.grid_n[2], sk = size_t(g.grid_n[2]);
   R best = -Inf<R>::v();

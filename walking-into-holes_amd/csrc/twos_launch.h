@@ -20,17 +20,20 @@ struct MeshBuffers {
   float *grid = nullptr;
   float grid_lo[3] = {0.f, 0.f, 0.f};
   float grid_h = 0.f;
-  int32_t grid_n[3] = {0, 0, 0};  // samples per axis
-  int32_t grid_corners = 0;       // samples at cell corners (1) or centres (0)
+  int32_t grid_n[3] = {0, 0, 0};  // samples (modes 0, 1) or cells (mode 2) per axis
+  int32_t grid_corners = 0;       // mode: 0 centres, 1 corners, 2 binary16 corner records
 };
 
 hipError_t lbvh_build(const double *d_vtx, const uint32_t *d_faces, int32_t n, MeshBuffers &mb, int32_t *depth,
                       hipStream_t stream);
-// cells along the longest axis; corners: sample the distance at the cell
-// corners (cells + 1 samples per axis) instead of the centres.
-hipError_t build_distance_grid(MeshBuffers &mb, const double lo[3], const double hi[3], int cells, bool corners,
+// cells along the longest axis; mode 0 samples the distance at cell centres,
+// 1 at cell corners (cells + 1 samples per axis, binary32), 2 stores per cell
+// its eight corner values rounded down to binary16 in one 16-byte record.
+hipError_t build_distance_grid(MeshBuffers &mb, const double lo[3], const double hi[3], int cells, int mode,
                                hipStream_t stream);
 hipError_t launch_inside(const MeshBuffers &mb, const double *d_pts, int64_t n, int32_t *d_out, hipStream_t stream);
+hipError_t launch_bound(const DevGeom *d_geom, int precision, const double *d_pts, int64_t n, double *d_out,
+                        hipStream_t stream);
 hipError_t launch_closest(const DevGeom *d_geom, int precision, const double *d_pts, int64_t n, double *d_dist,
                           double *d_cp, int32_t *d_cls, hipStream_t stream);
 

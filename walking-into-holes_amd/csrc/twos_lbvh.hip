@@ -470,6 +470,41 @@ __global__ void distance_grid(MeshView m, float lx, float ly, float lz, float h,
   out[idx] = sqrtf(mesh_closest<float, false>(m, p, pt, hint, nullptr));
 }
 
+// binary16 rounding of a non-negative distance toward zero, so a bound built
+// from it stays a lower bound.
+__device__ __forceinline__ uint32_t f16_down(float v) {
+  _Float16 h = static_cast<_Float16>(v);
+  uint32_t bits = __builtin_bit_cast(unsigned short, h);
+  if (static_cast<float>(h) > v && bits > 0u) bits -= 1u;
+  return bits;
+}
+
+// Per cell of a corner grid (nx x ny x nz samples), its eight corner values in
+// binary16 in one 16-byte record: corner (a, b, c) in word 2a + b, half c.
+__global__ void pack_corner_records(const float *node, int nx, int ny, int nz, uint4 *out) {
+  const int cx = nx - 1, cy = ny - 1, cz = nz - 1;
+  const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= int64_t(cx) * cy * cz) return;
+  const int l = int(idx % cz), k = int((idx / cz) % cy), i = int(idx / (int64_t(cy) * cz));
+  uint32_t w[4];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {
+      const float *q = node + (size_t(i + a) * ny + (k + b)) * size_t(nz) + l;
+      w[2 * a + b] = f16_down(q[0]) | (f16_down(q[1]) << 16);
+    }
+  out[idx] = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// Lower bound of the mesh distance from the geometry's distance grid (-inf
+// without a grid).
+template <typename R>
+__global__ void bound_kernel(const DevGeom *g, const double *pts, int64_t n, double *out) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const V3<R> p{R(pts[3 * i]), R(pts[3 * i + 1]), R(pts[3 * i + 2])};
+  out[i] = g->grid ? double(grid_lower<R>(*g, p)) : -__builtin_huge_val();
+}
+
 template <typename R>
 __global__ void closest_kernel(const DevGeom *g, const double *pts, int64_t n, double *dist, double *cp, int32_t *cls) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -580,32 +615,70 @@ hipError_t lbvh_build(const double *d_vtx, const uint32_t *d_faces, int32_t n, M
   return hipSuccess;
 }
 
-hipError_t build_distance_grid(MeshBuffers &mb, const double lo[3], const double hi[3], int cells, bool corners,
+hipError_t build_distance_grid(MeshBuffers &mb, const double lo[3], const double hi[3], int cells, int mode,
                                hipStream_t stream) {
   double ext = 0.0;
   for (int k = 0; k < 3; ++k) ext = std::max(ext, hi[k] - lo[k]);
   const double pad = 1e-3 * ext + 1e-9;
   const double h = (ext + 2 * pad) / cells;
+  const bool corners = mode != 0;
   int64_t total = 1;
+  int n[3];
   for (int k = 0; k < 3; ++k) {
     mb.grid_lo[k] = float(lo[k] - pad);
-    mb.grid_n[k] = std::max(1, int(std::ceil((hi[k] - lo[k] + 2 * pad) / h))) + (corners ? 1 : 0);
-    total *= mb.grid_n[k];
+    n[k] = std::max(1, int(std::ceil((hi[k] - lo[k] + 2 * pad) / h))) + (corners ? 1 : 0);
+    total *= n[k];
   }
   mb.grid_h = float(h);
-  mb.grid_corners = corners ? 1 : 0;
-  hipError_t e = hipMalloc(&mb.grid, sizeof(float) * size_t(total));
+  mb.grid_corners = mode;
+  float *samples = nullptr;
+  hipError_t e = hipMalloc(&samples, sizeof(float) * size_t(total));
   if (e != hipSuccess) return e;
   MeshView m{mb.nodes, mb.nodes4, mb.tris_f, mb.tris_d, mb.tri_ids, mb.n_tris};
   hipLaunchKernelGGL(distance_grid, dim3(blocks_for(total, 128)), dim3(128), 0, stream, m, mb.grid_lo[0],
-                     mb.grid_lo[1], mb.grid_lo[2], mb.grid_h, corners ? 0.0f : 0.5f, mb.grid_n[0], mb.grid_n[1],
-                     mb.grid_n[2], mb.grid);
-  return hipGetLastError();
+                     mb.grid_lo[1], mb.grid_lo[2], mb.grid_h, corners ? 0.0f : 0.5f, n[0], n[1], n[2], samples);
+  if ((e = hipGetLastError()) != hipSuccess) {
+    (void)hipFree(samples);
+    return e;
+  }
+  if (mode != 2) {
+    for (int k = 0; k < 3; ++k) mb.grid_n[k] = n[k];
+    mb.grid = samples;
+    return hipSuccess;
+  }
+  // Corner records: one per cell, built from the corner samples.
+  const int64_t n_cells = int64_t(n[0] - 1) * (n[1] - 1) * (n[2] - 1);
+  uint4 *rec = nullptr;
+  if ((e = hipMalloc(&rec, sizeof(uint4) * size_t(n_cells))) != hipSuccess) {
+    (void)hipFree(samples);
+    return e;
+  }
+  hipLaunchKernelGGL(pack_corner_records, dim3(blocks_for(n_cells, 128)), dim3(128), 0, stream, samples, n[0], n[1],
+                     n[2], rec);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  (void)hipFree(samples);
+  if (e != hipSuccess) {
+    (void)hipFree(rec);
+    return e;
+  }
+  for (int k = 0; k < 3; ++k) mb.grid_n[k] = n[k] - 1;
+  mb.grid = reinterpret_cast<float *>(rec);
+  return hipSuccess;
 }
 
 hipError_t launch_inside(const MeshBuffers &mb, const double *d_pts, int64_t n, int32_t *d_out, hipStream_t stream) {
   MeshView m{mb.nodes, mb.nodes4, mb.tris_f, mb.tris_d, mb.tri_ids, mb.n_tris};
   hipLaunchKernelGGL(inside_kernel, dim3(blocks_for(n, 128)), dim3(128), 0, stream, m, d_pts, n, d_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_bound(const DevGeom *d_geom, int precision, const double *d_pts, int64_t n, double *d_out,
+                        hipStream_t stream) {
+  if (precision == 64)
+    hipLaunchKernelGGL(bound_kernel<double>, dim3(blocks_for(n, 128)), dim3(128), 0, stream, d_geom, d_pts, n, d_out);
+  else
+    hipLaunchKernelGGL(bound_kernel<float>, dim3(blocks_for(n, 128)), dim3(128), 0, stream, d_geom, d_pts, n, d_out);
   return hipGetLastError();
 }
 

@@ -1037,7 +1037,7 @@ twos_status twos_mesh_create(twos_ctx *ctx, const double *vertices, int64_t nv, 
         lo[k] = std::min(lo[k], vertices[3 * i + k]);
         hi[k] = std::max(hi[k], vertices[3 * i + k]);
       }
-    e = build_distance_grid(m.mb, lo, hi, kGridCells, false, ctx->stream);
+    e = build_distance_grid(m.mb, lo, hi, kGridCells, 0, ctx->stream);
     for (int k = 0; k < 3; ++k) { m.lo[k] = lo[k]; m.hi[k] = hi[k]; }
     m.grid_cells = kGridCells;
   }
@@ -1068,6 +1068,7 @@ twos_status twos_mesh_set_distance_grid(twos_ctx *ctx, int32_t mesh_id, int32_t 
                                         double *build_ms) {
   if (!ctx) return fail(TWOS_ERR_INVALID_ARGUMENT, "null context");
   if (cells < 8 || cells > 512) return fail(TWOS_ERR_INVALID_ARGUMENT, "cells must be in [8, 512]");
+  if (corners < 0 || corners > 2) return fail(TWOS_ERR_INVALID_ARGUMENT, "grid mode must be 0, 1 or 2");
   std::lock_guard<std::mutex> lock(ctx->mu);
   auto it = ctx->meshes.find(mesh_id);
   if (it == ctx->meshes.end()) return fail(TWOS_ERR_NOT_FOUND, "unknown mesh id");
@@ -1081,7 +1082,7 @@ twos_status twos_mesh_set_distance_grid(twos_ctx *ctx, int32_t mesh_id, int32_t 
   HIP_OK(hipEventCreate(&a));
   HIP_OK(hipEventCreate(&b));
   HIP_OK(hipEventRecord(a, ctx->stream));
-  const hipError_t e = build_distance_grid(nb, m.lo, m.hi, cells, corners != 0, ctx->stream);
+  const hipError_t e = build_distance_grid(nb, m.lo, m.hi, cells, corners, ctx->stream);
   (void)hipEventRecord(b, ctx->stream);
   (void)hipStreamSynchronize(ctx->stream);
   float ms = 0.f;
@@ -1276,6 +1277,26 @@ twos_status twos_closest(twos_ctx *ctx, int32_t geom_id, int32_t precision, cons
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   (void)hipFree(d_p); (void)hipFree(d_d); (void)hipFree(d_c); (void)hipFree(d_k);
   if (e != hipSuccess) return fail(TWOS_ERR_DEVICE, std::string("closest: ") + hipGetErrorString(e));
+  return TWOS_OK;
+}
+
+twos_status twos_distance_bound(twos_ctx *ctx, int32_t geom_id, int32_t precision, const double *points, int64_t n,
+                                double *bound) {
+  if (!ctx || (n > 0 && (!points || !bound))) return fail(TWOS_ERR_INVALID_ARGUMENT, "null argument");
+  if (precision != 32 && precision != 64) return fail(TWOS_ERR_INVALID_ARGUMENT, "precision must be 32 or 64");
+  if (n <= 0) return TWOS_OK;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (ctx->geoms.find(geom_id) == ctx->geoms.end()) return fail(TWOS_ERR_NOT_FOUND, "unknown geometry id");
+  HIP_OK(hipSetDevice(ctx->device));
+  double *d_p = nullptr, *d_b = nullptr;
+  HIP_OK(dalloc(&d_p, size_t(3 * n)));
+  HIP_OK(dalloc(&d_b, size_t(n)));
+  hipError_t e = hipMemcpyAsync(d_p, points, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = launch_bound(ctx->d_geoms + geom_id, precision, d_p, n, d_b, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(bound, d_b, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d_p); (void)hipFree(d_b);
+  if (e != hipSuccess) return fail(TWOS_ERR_DEVICE, std::string("distance bound: ") + hipGetErrorString(e));
   return TWOS_OK;
 }
 

@@ -23,7 +23,7 @@ using DevSide = twos_side;
 struct DevGeom {
   int32_t container, n_spheres, n_cylinders, n_cavities;
   int32_t mesh_radius_f32, n_tris;
-  int32_t grid_corners;  // 1: grid values at cell corners (8-corner bound), 0: at cell centres
+  int32_t grid_corners;  // grid mode: 0 cell centres, 1 cell corners, 2 binary16 corner records per cell
   int32_t pad;
   double ball[4];
   double box_lo[3];

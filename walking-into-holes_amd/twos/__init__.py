@@ -296,13 +296,14 @@ class Context:
         _check(_lib().twos_mesh_info(self._h, mid.value, C.byref(depth), C.byref(nodes), C.byref(nbytes)))
         return Mesh(mid.value, len(v), len(f), ms.value, depth.value, nbytes.value)
 
-    def set_distance_grid(self, mesh: Mesh | int, cells: int, corners: bool = False) -> float:
+    def set_distance_grid(self, mesh: Mesh | int, cells: int, corners: int = 0) -> float:
         """Rebuild the mesh's distance grid with `cells` cells along its longest
-        axis (8..512), sampled at cell centres or, with corners, at cell corners
-        (eight-corner bound); returns the build time in ms."""
+        axis (8..512), sampled at cell centres (corners=0), at cell corners
+        (1, eight-corner bound) or as per-cell binary16 corner records (2);
+        returns the build time in ms."""
         ms = C.c_double()
         _check(_lib().twos_mesh_set_distance_grid(self._h, mesh.id if isinstance(mesh, Mesh) else int(mesh),
-                                                  int(cells), int(bool(corners)), C.byref(ms)))
+                                                  int(cells), int(corners), C.byref(ms)))
         return ms.value
 
     def geometry(self, container: int, *, mesh: Mesh | int | None = None, ball=(0.0, 0.0, 0.0, 1.0),
@@ -354,6 +355,15 @@ class Context:
         mid = mesh.id if isinstance(mesh, Mesh) else int(mesh)
         _check(_lib().twos_inside(self._h, mid, _ptr(p, _capi._dp), len(p), _ptr(out, _capi._i32p)))
         return out.astype(bool)
+
+    def distance_bound(self, geom: int, points, precision: int = 32) -> np.ndarray:
+        """The distance grid's lower bound of the mesh distance at each point
+        (-inf without a grid)."""
+        p = _f64(points).reshape(-1, 3)
+        out = np.zeros(len(p))
+        _check(_lib().twos_distance_bound(self._h, geom, precision, _ptr(p, _capi._dp), len(p),
+                                          _ptr(out, _capi._dp)))
+        return out
 
     def closest(self, geom: int, points, precision: int = 64):
         """Distance to the domain boundary, closest point and boundary class."""

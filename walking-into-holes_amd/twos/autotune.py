@@ -9,8 +9,9 @@ Axes (binary32 persistent kernel; binary64 tunes the block only):
                   32 exist for waves_per_simd 0 and 5)
   grid_kappa      distance-grid threshold in cell sizes, 0.25 to 6 (mesh
                   geometries with the distance grid on)
-  grid_corners    distance-grid samples at cell centres (0) or corners (1,
-                  eight-corner bound), with the mesh handle
+  grid_corners    distance-grid samples at cell centres (0), corners (1,
+                  eight-corner bound) or binary16 corner records per cell (2),
+                  with the mesh handle
   grid_cells      distance-grid resolution, 96 to 512 cells along
                   the mesh's longest axis (when the mesh handle is given; the
                   mesh keeps the chosen grid)
@@ -48,7 +49,7 @@ AXES = {
     "block": (64, 128, 256),
     "waves_per_simd": (0, 5, 6, 8),
     "lds_stack": (16, 24, 32),
-    "grid_corners": (0, 1),
+    "grid_corners": (0, 1, 2),
     "grid_kappa": (0.25, 0.35, 0.5, 0.75, 1.0, 1.5, 2.0, 3.0, 4.0, 6.0),
     "grid_cells": (96, 128, 192, 256, 320, 384, 448, 512),
 }
@@ -102,7 +103,7 @@ def autotune(ctx, geom: int, sites, sides, *, geom_sig: str, combine: int, preci
 
     def applied(t: "Tuning") -> "Tuning":
         if tune_cells:
-            ctx.set_distance_grid(mesh, t.choice["grid_cells"], bool(t.choice["grid_corners"]))
+            ctx.set_distance_grid(mesh, t.choice["grid_cells"], int(t.choice["grid_corners"]))
         return t
 
     if key in _memory:
@@ -133,7 +134,7 @@ def autotune(ctx, geom: int, sites, sides, *, geom_sig: str, combine: int, preci
         if k in measured:
             return measured[k]
         if tune_cells and (cfg["grid_cells"], cfg["grid_corners"]) != grid_now[0]:
-            ctx.set_distance_grid(mesh, cfg["grid_cells"], bool(cfg["grid_corners"]))
+            ctx.set_distance_grid(mesh, cfg["grid_cells"], int(cfg["grid_corners"]))
             grid_now[0] = (cfg["grid_cells"], cfg["grid_corners"])
         best = math.inf
         for _ in range(repeats):
