@@ -12,7 +12,7 @@ hipGraph).  The metric is TD sites/s: sites finished at the CI target per
 second of wall time.  Beside it: walk-steps/s, the plain one-launch-per-site
 GPU schedule on a seeded 512-site sample, the walk kernel against the HBM
 roofline, and the CPU oracle (same streams, OpenMP over all host cores) on a
-seeded 192-site sample.
+seeded 320-site sample (about 13 s on 16 threads).
 
     python bench.py [--gpus 1] [--steps 2] [--warmup 1] [--workload sh_icosphere|cooling_block|unit_ball]
 """
@@ -53,7 +53,7 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--requests", type=int, default=300, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=6.0, help="serve: Poisson arrivals per second")
     ap.add_argument("--plain-sites", type=int, default=512)
-    ap.add_argument("--cpu-sites", type=int, default=192)
+    ap.add_argument("--cpu-sites", type=int, default=320)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
@@ -273,6 +273,18 @@ def serve(args) -> int:
 
     ctx = twos.Context()
     stream = serving.Stream(ctx, twos, n_requests=args.requests, rate=args.rate)
+    tuning = None
+    if not args.no_autotune:
+        # The mesh requests run at the tuned launch shape and distance grid
+        # (the committed choice for this mesh, eps and combine).
+        from twos import autotune
+        from twos import workloads as W
+        w, g = stream.geoms["sh_icosphere"]
+        tuning = autotune.autotune(ctx, g, w.sites, W.sides(w, twos.side), geom_sig=w.name, combine=twos.CROSS,
+                                   precision=w.precision, eps=w.eps, max_steps=w.max_steps, distance_grid=True,
+                                   mesh=stream.mesh, log=log, cache_path=TUNING_CACHE)
+        stream.shapes["sh_icosphere"] = {k: v for k, v in tuning.choice.items()
+                                         if k not in ("grid_cells", "grid_corners")}
     # Slots and one round's walk buffer sized to a slice of HBM (admission by memory).
     srv = ctx.server(max_sites=65536, max_walks_per_round=1 << 26)
     log(f"serving {args.requests} requests at {args.rate}/s")
@@ -288,6 +300,7 @@ def serve(args) -> int:
         "p50_ms": round(res["p50_ms"], 2), "p99_ms": round(res["p99_ms"], 2), "mean_ms": round(res["mean_ms"], 2),
         "td_sites_per_s": round(res["td_sites_per_s"], 2), "walk_steps_per_s": round(res["walk_steps_per_s"], 1),
         "rounds": res["rounds"], "validations": res["validations"], "td_sites": res["td_sites"],
+        "mesh_launch_shape": tuning.choice if tuning else None,
     }
     srv.close()
     ctx.close()
@@ -327,6 +340,19 @@ def design_bench(args) -> int:
     out = D.greedy(est, w.sites, cfg, log=log)
     total = time.perf_counter() - t0
     scored = out.candidates_scored
+    # binary64 rerun of the first and last rounds' rescoring (the same Philox
+    # streams, cavities and grid threshold): the binary32 walks' error.
+    est64 = D.GpuDesign(ctx, mesh, sides, radius=0.02, eps=w.eps, max_steps=w.max_steps, precision=64, shape=shape)
+    rerun = []
+    for k in sorted({0, len(out.rounds) - 1}):
+        r = out.rounds[k]
+        pts = w.sites[r.feasible][r.shortlist]
+        s64, se64 = est64(pts, out.centers[:k], cfg.rescore_walks, cfg.seed + cfg.seed_step * k + cfg.rescore_offset)
+        s64, se64 = np.asarray(s64), np.asarray(se64)
+        rerun.append({"round": k, "sites": len(pts),
+                      "max_abs_diff_in_se": float(np.max(np.abs(r.rescore_score - s64) / se64)),
+                      "max_rel_diff": float(np.max(np.abs(r.rescore_score - s64) / np.abs(s64)))})
+        log(f"[design] binary64 rerun of round {k}: {rerun[-1]}")
     result = {
         "metric": "design candidates scored per second (scout + rescore)", "value": round(scored / total, 2),
         "unit": "candidates/s", "n_gpus": 1, "steps": args.design_rounds, "warmup": 1,
@@ -339,6 +365,7 @@ def design_bench(args) -> int:
         "rounds": [{"feasible": len(r.feasible), "chosen": np.round(r.chosen, 6).tolist(),
                     "score": float(r.rescore_score[int(np.argmax(r.rescore_score - 1.96 * r.rescore_se))]),
                     "seconds": round(r.seconds, 3)} for r in out.rounds],
+        "binary64_rerun": rerun,
         "launch_shape": tuning.choice,
     }
     ctx.close()

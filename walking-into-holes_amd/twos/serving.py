@@ -53,6 +53,9 @@ class Stream:
                  walks: int = 256, walks_max: int = 16384):
         self.ctx, self.twos = ctx, twos_mod
         self.walks, self.walks_max = walks, walks_max
+        # Launch shape and grid threshold per geometry name (for example the
+        # autotuner's choice); neither changes which requests finish when.
+        self.shapes: dict[str, dict] = {}
         ball = W.unit_ball()
         block = W.cooling_block()
         ico = W.sh_icosphere()
@@ -109,7 +112,7 @@ class Stream:
         w, g = self.geoms[req.geom_name]
         sides = W.sides(w, self.twos.side)
         common = dict(combine=self.twos.CROSS, precision=32, seed=req.seed, eps=w.eps, max_steps=w.max_steps,
-                      distance_grid=distance_grid)
+                      distance_grid=distance_grid, **self.shapes.get(req.geom_name, {}))
         if req.kind == "td":
             req.ids = [srv.submit(g, req.sites, sides, walks=self.walks, walks_max=self.walks_max,
                                   ci_rel=req.ci_rel, **common)]
