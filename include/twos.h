@@ -60,14 +60,27 @@ extern "C" {
 #define TWOS_SQUARE 0
 #define TWOS_PAIRED 1
 #define TWOS_CROSS 2
+/* COUPLED is the coupled finite-cavity difference of study.cpp --fd
+ * (study.cpp:317-332): each walk draws a start point x uniformly in the ball
+ * of sides[1]'s single TWOS_TERM_UNIT row (the objective's density q, a
+ * unit-mass bump), weight w = (1 - rho^2)^2 35/8 (q relative to the uniform
+ * density, rho = |x - c| / R), and runs two paths of side 0 from x on the
+ * same stream: a in the geometry, b in the geometry minus a ball of the
+ * geometry's cavity_radius r at the site.  The sample is
+ * d = w (a - b) / (4 pi r): score = mean d (the change of J = integral q u per
+ * unit capacity when the cavity is cut), score_se = its standard error,
+ * u = v = score, same = mean d^2.  Both paths count in steps and caps. */
+#define TWOS_COUPLED 3
 
 /* Streams: PHILOX keyed by (seed, site, side) with counter (step, walk);
- * MT_SITE / MT_PAIRED / MT_KEY replay the reference's mt19937_64 layouts
- * (wih_twos.cpp:681-685, study.cpp:89, wih_twos.cpp:1737), binary64 only. */
+ * MT_SITE / MT_PAIRED / MT_KEY / MT_FD replay the reference's mt19937_64
+ * layouts (wih_twos.cpp:681-685, study.cpp:89, wih_twos.cpp:1737,
+ * study.cpp:322-323), binary64 only; MT_FD goes with COUPLED. */
 #define TWOS_RNG_PHILOX 0
 #define TWOS_RNG_MT_SITE 1
 #define TWOS_RNG_MT_PAIRED 2
 #define TWOS_RNG_MT_KEY 3
+#define TWOS_RNG_MT_FD 4
 
 /* Schedules: PLAIN launches one kernel per site and round, one thread per
  * walk; PERSISTENT runs every site of every admitted request in one

@@ -18,8 +18,8 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle.so"
 
 BALL, BOX, MESH = 0, 1, 2
-SQUARE, PAIRED, CROSS = 0, 1, 2
-RNG_PHILOX, RNG_MT_SITE, RNG_MT_PAIRED, RNG_MT_KEY = 0, 1, 2, 3
+SQUARE, PAIRED, CROSS, COUPLED = 0, 1, 2, 3
+RNG_PHILOX, RNG_MT_SITE, RNG_MT_PAIRED, RNG_MT_KEY, RNG_MT_FD = 0, 1, 2, 3, 4
 
 _dp = C.POINTER(C.c_double)
 _u32p = C.POINTER(C.c_uint32)

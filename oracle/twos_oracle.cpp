@@ -533,6 +533,7 @@ struct WalkOut {
   double a = 0.0, b = 0.0;
   int32_t steps = 0;
   bool cap = false;
+  int32_t caps = 0;  // capped paths of this sample (two paths for CROSS and COUPLED)
   // First step, for the gradient channel: 3 omega0 / r0, r0 (0 if the path
   // ended at its first query) and both sides' values after its source term.
   V3 w0{0.0, 0.0, 0.0};
@@ -607,6 +608,7 @@ WalkOut walk(const og_geom &g, const og_side &s0, const og_side *s1, V3 x, doubl
   }
   if (rows) *rows = row;
   out.cap = true;
+  out.caps = 1;
   return out;
 }
 
@@ -618,9 +620,75 @@ uint64_t site_key(uint64_t seed, int32_t repeat, int64_t site, int64_t sample) {
          splitmix64(uint64_t(sample) + 0x4d5a2da51de1aa47ULL);
 }
 
+// Coupled finite-cavity sample of walk `w` at candidate site `s` (centre z):
+// study.cpp:317-332.  The start point is uniform in the probe ball (centre c,
+// radius R: the single TERM_UNIT row of sides[1]), drawn as cbrt(U) for the
+// radial fraction and then random_unit_vector; both paths of side 0 start
+// there on the same stream, the second with a cavity of radius
+// g.cavity_radius at z appended after the geometry's own (`added`, :320).
+// d = w (a - b) / (4 pi r) with w = (1 - rho^2)^2 35/8.
+//   MT_FD: key = splitmix64(seed) ^ splitmix64(s + 734) ^ splitmix64(w + 8131)
+//   (:322); the probe from mt19937_64(splitmix64(key)) (:323-324), each path
+//   from walk()'s mt19937_64(splitmix64(key + 719)) (:326, wih_twos.cpp:638).
+//   PHILOX: the side-0 key of (seed, s); the probe from counter block
+//   (0xFFFFFFFF, w, 0, 0) (radial fraction, words 0 and 1) and the direction of
+//   block (0xFFFFFFFF, w, 1, 0); each path is walk w of that key.
+WalkOut coupled_sample(const og_geom &g, const og_side sides[2], const og_params &p, const V3 &z, int64_t s,
+                       int64_t w) {
+  const double *t = sides[1].terms[0];
+  const V3 c{t[1], t[2], t[3]};
+  const double R = t[4];
+  double rad;
+  V3 dir;
+  uint64_t key = 0;
+  uint32_t pkey[2] = {0u, 0u};
+  if (p.rng == OG_RNG_MT_FD) {
+    key = splitmix64(p.seed) ^ splitmix64(uint64_t(s) + 734) ^ splitmix64(uint64_t(w) + 8131);
+    std::mt19937_64 rng(splitmix64(key));
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    rad = std::cbrt(U(rng));
+    dir = random_unit_vector(rng);
+  } else {
+    philox_key(p.seed, s, 0, pkey);
+    const uint32_t ctr[4] = {0xFFFFFFFFu, uint32_t(w), 0u, 0u};
+    uint32_t b[4];
+    philox4x32_10(ctr, pkey, b);
+    rad = std::cbrt(u53(b[0], b[1]));
+    dir = philox_direction(pkey, uint32_t(w), 0xFFFFFFFFu, 1u);
+  }
+  const V3 x = c + dir * (R * rad);
+  const double weight = std::pow(1.0 - rad * rad, 2) * 35.0 / 8.0;
+  std::vector<double> cav(g.cavities, g.cavities + 3 * g.n_cavities);
+  cav.push_back(z.x);
+  cav.push_back(z.y);
+  cav.push_back(z.z);
+  og_geom g2 = g;
+  g2.n_cavities = g.n_cavities + 1;
+  g2.cavities = cav.data();
+  WalkOut a, b;
+  if (p.rng == OG_RNG_MT_FD) {
+    MtStream sa, sb;
+    sa.rng.seed(splitmix64(key + 719));
+    sb.rng.seed(splitmix64(key + 719));
+    a = walk(g, sides[0], nullptr, x, p.eps, p.max_steps, sa, false, nullptr, 0, nullptr);
+    b = walk(g2, sides[0], nullptr, x, p.eps, p.max_steps, sb, false, nullptr, 0, nullptr);
+  } else {
+    PhiloxStream sa{{pkey[0], pkey[1]}, uint32_t(w)};
+    a = walk(g, sides[0], nullptr, x, p.eps, p.max_steps, sa, false, nullptr, 0, nullptr);
+    b = walk(g2, sides[0], nullptr, x, p.eps, p.max_steps, sa, false, nullptr, 0, nullptr);
+  }
+  WalkOut o;
+  o.a = o.b = weight * (a.a - b.a) / (4 * 3.141592653589793 * g.cavity_radius);
+  o.steps = a.steps + b.steps;
+  o.cap = a.cap || b.cap;
+  o.caps = a.caps + b.caps;
+  return o;
+}
+
 // One sample (a, b) of walk `w` at site `s`.
 WalkOut sample(const og_geom &g, const og_side sides[2], const og_params &p, const V3 &z,
                int64_t s, int64_t w) {
+  if (p.combine == OG_COUPLED) return coupled_sample(g, sides, p, z, s, w);
   if (p.rng == OG_RNG_PHILOX) {
     PhiloxStream st0{{0u, 0u}, uint32_t(w)};
     philox_key(p.seed, s, 0, st0.key);
@@ -645,6 +713,7 @@ WalkOut sample(const og_geom &g, const og_side sides[2], const og_params &p, con
     a.b = b.a;
     a.steps += b.steps;
     a.cap = a.cap || b.cap;
+    a.caps += b.caps;
     return a;
   }
   MtStream st;
@@ -875,7 +944,12 @@ int32_t og_estimate(const og_geom *g, const og_side sides[2], const og_params *p
   const og_params p = *prm;
   if (p.walks < 2 || p.max_steps < 1 || !(p.eps > 0.0) || n < 0) return 2;
   if (p.rng != OG_RNG_PHILOX && p.combine == OG_CROSS) return 2;
-  if (p.rng == OG_RNG_MT_KEY) return 2;
+  if (p.rng == OG_RNG_MT_KEY || p.rng < 0 || p.rng > OG_RNG_MT_FD) return 2;
+  if ((p.rng == OG_RNG_MT_FD) != (p.combine == OG_COUPLED) && p.rng != OG_RNG_PHILOX) return 2;
+  if (p.combine == OG_COUPLED &&
+      (p.gradient || !(g->cavity_radius > 0.0) || sides[1].source_kind != 2 || sides[1].n_terms != 1 ||
+       sides[1].terms[0][0] != OG_TERM_UNIT || !(sides[1].terms[0][4] > 0.0)))
+    return 2;
   if (p.gradient && (sides[0].source_kind == 2 || sides[1].source_kind == 2)) return 2;
   const int32_t walks_max = std::max(p.walks, p.walks_max);
   int32_t status = 0;
@@ -900,7 +974,7 @@ int32_t og_estimate(const og_geom *g, const og_side sides[2], const og_params *p
         b.push_back(o.b);
         if (p.gradient) grad_accumulate(o.ga, o.gb, gs);
         steps += o.steps;
-        caps += o.cap ? 1 : 0;
+        caps += o.caps;
         if (dump_a && w < dump_stride) {
           dump_a[s * dump_stride + w] = o.a;
           dump_b[s * dump_stride + w] = o.b;
@@ -908,7 +982,22 @@ int32_t og_estimate(const og_geom *g, const og_side sides[2], const og_params *p
         }
       }
       done = target;
-      combine(a.data(), b.data(), done, p.combine == OG_SQUARE, c);
+      if (p.combine == OG_COUPLED) {
+        // v.up and v.same of study.cpp:329-331: the mean of d and its SE.
+        double up = 0.0, sum2 = 0.0;
+        for (int64_t i = 0; i < done; ++i) {
+          up += a[size_t(i)];
+          sum2 += a[size_t(i)] * a[size_t(i)];
+        }
+        const double nn = double(done);
+        up /= nn;
+        c[0] = c[1] = c[2] = up;
+        c[3] = std::sqrt(std::max(0.0, (sum2 - nn * up * up) / (nn * (nn - 1.0))));
+        c[4] = sum2 / nn;
+        c[5] = nn;
+      } else {
+        combine(a.data(), b.data(), done, p.combine == OG_SQUARE, c);
+      }
       const bool met = std::isfinite(c[3]) && p.ci_z * c[3] <= p.ci_rel * std::fabs(c[2]);
       if (walks_max <= p.walks || met) {
         r.status = 0;

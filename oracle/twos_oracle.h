@@ -32,9 +32,9 @@ extern "C" {
 /* Geometry container kinds. */
 enum { OG_BALL = 0, OG_BOX = 1, OG_MESH = 2 };
 /* Combine modes. */
-enum { OG_SQUARE = 0, OG_PAIRED = 1, OG_CROSS = 2 };
+enum { OG_SQUARE = 0, OG_PAIRED = 1, OG_CROSS = 2, OG_COUPLED = 3 };
 /* Stream modes. */
-enum { OG_RNG_PHILOX = 0, OG_RNG_MT_SITE = 1, OG_RNG_MT_PAIRED = 2, OG_RNG_MT_KEY = 3 };
+enum { OG_RNG_PHILOX = 0, OG_RNG_MT_SITE = 1, OG_RNG_MT_PAIRED = 2, OG_RNG_MT_KEY = 3, OG_RNG_MT_FD = 4 };
 
 typedef struct og_mesh og_mesh;
 

@@ -9,9 +9,13 @@
 // "DONE <code>", keeping the device context, the mesh's LBVH and its
 // geometries resident across requests; a session keeps one outer mesh.
 //
-// Mode twos only: the reference's multigrid solver (--mode mg), finite
-// differences (--fd 1) and Differential WoS (--diff 1) are other solvers; a
-// request asking for them fails with status 1.
+// --fd 1 is the coupled finite-cavity difference (study.cpp:317-332): per
+// point, walks start uniformly in the sensor ball and run with and without a
+// cavity at the point on the same stream (TWOS_COUPLED); the score column is
+// the mean of w (a - b) / (4 pi r), same_index its standard error, u and p
+// zero, and no point is skipped.  The reference's multigrid solver
+// (--mode mg) and Differential WoS (--diff 1) are other solvers; a request
+// asking for them fails with status 1.
 #include <chrono>
 #include <cmath>
 #include <fstream>
@@ -124,7 +128,6 @@ int process_request(int argc, char **argv, Session &session) {
     load_config(config, heat, sensor);
     if (o.mode != "twos" && o.mode != "mg") throw std::runtime_error("use twos or mg");
     if (o.mode == "mg") throw std::runtime_error("mode mg is not provided by this build (twos only)");
-    if (fd) throw std::runtime_error("--fd 1 is not provided by this build");
     if (diff) throw std::runtime_error("--diff 1 is not provided by this build");
     const auto start = std::chrono::steady_clock::now();
     if (!session.ctx) {
@@ -141,20 +144,22 @@ int process_request(int argc, char **argv, Session &session) {
     const std::vector<double> holes = wih::read_optional_points(o.cavities_path);
     const Geometry geometry(session, holes, o.cavity_radius);
     const int32_t geom = geometry.id();
-    // Sites inside a cavity are skipped and report zeros (study.cpp:341-342).
+    // Sites inside a cavity are skipped and report zeros (study.cpp:341-342);
+    // the finite-cavity difference evaluates every point (:319).
     const size_t n = points.size() / 3;
     std::vector<size_t> live;
     std::vector<double> live_pts;
     for (size_t i = 0; i < n; ++i)
-      if (!inside_cavity(&points[3 * i], holes, o.cavity_radius)) {
+      if (fd || !inside_cavity(&points[3 * i], holes, o.cavity_radius)) {
         live.push_back(i);
         live_pts.insert(live_pts.end(), &points[3 * i], &points[3 * i] + 3);
       }
     std::vector<twos_site_result> res(live.size());
     twos_stats stats{};
     if (!live.empty()) {
-      const twos_side sides[2] = {heat, hot ? twos_side{} : sensor};
-      const twos_params p = wih::make_params(o, TWOS_PAIRED, TWOS_RNG_MT_PAIRED, 0);
+      const twos_side sides[2] = {heat, hot && !fd ? twos_side{} : sensor};
+      const twos_params p = fd ? wih::make_params(o, TWOS_COUPLED, TWOS_RNG_MT_FD, 0)
+                               : wih::make_params(o, TWOS_PAIRED, TWOS_RNG_MT_PAIRED, 0);
       const twos_status st = twos_estimate(session.ctx->get(), geom, sides, &p, live_pts.data(),
                                            int64_t(live.size()), res.data(), &stats);
       if (st != TWOS_OK && st != TWOS_ERR_STEP_CAP) wih::check(st, "estimate");
@@ -170,7 +175,12 @@ int process_request(int argc, char **argv, Session &session) {
       const twos_site_result &r = all[i];
       steps += uint64_t(r.steps);
       caps += uint64_t(r.caps);
-      paths += uint64_t(r.walks);
+      paths += uint64_t(r.walks) * (fd ? 2 : 1);
+      if (fd) {
+        // PairValue{0, 0, up, same} of the finite-cavity difference (:329-331).
+        f << i << ",0,0," << r.score << ',' << r.score_se << ',' << r.steps << ',' << r.caps << '\n';
+        continue;
+      }
       // Mean-only mode reports the primal mean as its score (study.cpp:117).
       const double score = hot ? r.u : r.score;
       f << i << ',' << r.u << ',' << (hot ? 0.0 : r.v) << ',' << score << ',' << (hot ? 0.0 : r.same) << ','

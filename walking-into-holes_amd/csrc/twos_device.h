@@ -150,6 +150,12 @@ struct Mt64 {
   }
 };
 
+// Per-walk key of the coupled finite-cavity difference (study.cpp:322); its
+// start point comes from mt19937_64(splitmix64(key)) (:323-324).
+__device__ __forceinline__ uint64_t mt_fd_key(uint64_t seed, int64_t site, int64_t walk) {
+  return splitmix64(seed) ^ splitmix64(uint64_t(site) + 734) ^ splitmix64(uint64_t(walk) + 8131);
+}
+
 // Per-walk mt19937_64 seed of the reference's key layouts.
 __device__ __forceinline__ uint64_t mt_walk_seed(int rng, uint64_t seed, int32_t repeat, int64_t site,
                                                   int64_t walk) {
@@ -164,6 +170,10 @@ __device__ __forceinline__ uint64_t mt_walk_seed(int rng, uint64_t seed, int32_t
   if (rng == kRngMtPaired) {
     // paired_site (benchmarks/submission_study/study.cpp:89).
     return splitmix64(seed) ^ splitmix64(uint64_t(site) + 91347) ^ splitmix64(uint64_t(walk) + 571);
+  }
+  if (rng == kRngMtFd) {
+    // The coupled finite-cavity paths: walk(..., key + 719) (study.cpp:326).
+    return splitmix64(mt_fd_key(seed, site, walk) + 719);
   }
   // Explicit key (write_trace: seed + 900000 + site, wih_twos.cpp:1737).
   return splitmix64(seed + uint64_t(site));

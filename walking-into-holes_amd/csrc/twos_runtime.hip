@@ -165,7 +165,7 @@ struct Engine {
     graph_exec = nullptr;
     graph = nullptr;
     graph_key = -1;
-    dfree(s.pos); dfree(s.geom); dfree(s.problem); dfree(s.keys); dfree(s.site); dfree(s.seed);
+    dfree(s.pos); dfree(s.geom); dfree(s.problem); dfree(s.combine); dfree(s.keys); dfree(s.site); dfree(s.seed);
     dfree(s.repeat); dfree(s.walks_max); dfree(s.max_steps); dfree(s.eps); dfree(s.ci_rel); dfree(s.ci_z);
     dfree(s.acc); dfree(s.acc_walks); dfree(s.acc_steps); dfree(s.acc_caps); dfree(s.result); dfree(s.hint0); dfree(s.grid_kappa);
     dfree(s.acc_grad); dfree(d_grad);
@@ -185,6 +185,7 @@ struct Engine {
     cap_problems = std::max(problems, 1);
     const size_t C = size_t(cap_sites);
     HIP_OK(dalloc(&s.pos, 3 * C)); HIP_OK(dalloc(&s.geom, C)); HIP_OK(dalloc(&s.problem, C));
+    HIP_OK(dalloc(&s.combine, C));
     HIP_OK(dalloc(&s.keys, 2 * C)); HIP_OK(dalloc(&s.site, C)); HIP_OK(dalloc(&s.seed, C));
     HIP_OK(dalloc(&s.repeat, C)); HIP_OK(dalloc(&s.walks_max, C)); HIP_OK(dalloc(&s.max_steps, C));
     HIP_OK(dalloc(&s.eps, C)); HIP_OK(dalloc(&s.ci_rel, C)); HIP_OK(dalloc(&s.ci_z, C));
@@ -291,6 +292,7 @@ struct Engine {
     const int64_t n = r->n, s0 = r->slot0;
     hipStream_t st = ctx->stream;
     std::vector<int32_t> gi(n, r->geom), pi(n, r->problem), rep(n, r->prm.repeat), wmax(n), msteps(n, r->prm.max_steps);
+    std::vector<int32_t> comb(n, r->prm.combine);
     std::vector<uint2> keys(2 * n);
     std::vector<int64_t> site(n);
     std::vector<uint64_t> seed(n, r->prm.seed);
@@ -310,6 +312,7 @@ struct Engine {
     HIP_OK(hipMemcpyAsync(s.pos + 3 * s0, r->sites.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(s.geom + s0, gi.data(), 4 * n, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(s.problem + s0, pi.data(), 4 * n, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(s.combine + s0, comb.data(), 4 * n, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(s.keys + 2 * s0, keys.data(), sizeof(uint2) * 2 * n, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(s.site + s0, site.data(), 8 * n, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(s.seed + s0, seed.data(), 8 * n, hipMemcpyHostToDevice, st));
@@ -699,8 +702,12 @@ twos_status check_params(const twos_params *p, int64_t n) {
   if (p->walks < 2) return fail(TWOS_ERR_INVALID_ARGUMENT, "walks must be at least 2");
   if (p->max_steps < 1) return fail(TWOS_ERR_INVALID_ARGUMENT, "max_steps must be at least 1");
   if (!(p->eps > 0.0)) return fail(TWOS_ERR_INVALID_ARGUMENT, "eps must be positive");
-  if (p->combine < 0 || p->combine > 2) return fail(TWOS_ERR_INVALID_ARGUMENT, "unknown combine mode");
-  if (p->rng < 0 || p->rng > 3) return fail(TWOS_ERR_INVALID_ARGUMENT, "unknown stream mode");
+  if (p->combine < 0 || p->combine > 3) return fail(TWOS_ERR_INVALID_ARGUMENT, "unknown combine mode");
+  if (p->rng < 0 || p->rng > 4) return fail(TWOS_ERR_INVALID_ARGUMENT, "unknown stream mode");
+  if (p->rng != kRngPhilox && (p->rng == kRngMtFd) != (p->combine == kCoupled))
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "COUPLED replays the MT_FD layout, and MT_FD belongs to COUPLED");
+  if (p->combine == kCoupled && (p->flags & TWOS_FLAG_GRADIENT))
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "the gradient channel does not apply to COUPLED");
   if (p->precision != 32 && p->precision != 64) return fail(TWOS_ERR_INVALID_ARGUMENT, "precision must be 32 or 64");
   if (p->rng != kRngPhilox && p->precision != 64)
     return fail(TWOS_ERR_INVALID_ARGUMENT, "mt19937_64 replay streams run in binary64 only");
@@ -755,6 +762,17 @@ Request *make_request(twos_ctx *ctx, int32_t geom, const twos_side sides[2], con
   if ((p->flags & TWOS_FLAG_GRADIENT) && (sides[0].source_kind == 2 || sides[1].source_kind == 2)) {
     *st = fail(TWOS_ERR_INVALID_ARGUMENT, "the gradient channel needs an affine source or none");
     return nullptr;
+  }
+  if (p->combine == kCoupled) {
+    const twos_side &q = sides[1];
+    if (q.source_kind != 2 || q.n_terms != 1 || q.terms[0][0] != TWOS_TERM_UNIT) {
+      *st = fail(TWOS_ERR_INVALID_ARGUMENT, "COUPLED needs sides[1] with one TWOS_TERM_UNIT row (the probe ball)");
+      return nullptr;
+    }
+    if (!(ctx->geoms.at(geom).desc.cavity_radius > 0.0)) {
+      *st = fail(TWOS_ERR_INVALID_ARGUMENT, "COUPLED needs a geometry with a positive cavity_radius");
+      return nullptr;
+    }
   }
   if (n > 0 && (!sites || !out)) {
     *st = fail(TWOS_ERR_INVALID_ARGUMENT, "sites and out must be non-null");
@@ -947,7 +965,7 @@ twos_status run_estimate(twos_ctx *ctx, int32_t geom, const twos_side sides[2], 
   int64_t steps = 0;
   for (int64_t i = 0; i < n; ++i) {
     steps += out[i].steps;
-    e->stats.paths += out[i].walks * (p->combine == kCross ? 2 : 1);
+    e->stats.paths += out[i].walks * (p->combine == kCross || p->combine == kCoupled ? 2 : 1);
   }
   e->stats.steps = steps;
   if (stats) *stats = e->stats;

@@ -9,9 +9,12 @@
 namespace twos {
 
 enum { kBall = TWOS_BALL, kBox = TWOS_BOX, kMesh = TWOS_MESH };
-enum { kSquare = TWOS_SQUARE, kPaired = TWOS_PAIRED, kCross = TWOS_CROSS };
+enum { kSquare = TWOS_SQUARE, kPaired = TWOS_PAIRED, kCross = TWOS_CROSS, kCoupled = TWOS_COUPLED };
 enum { kRngPhilox = TWOS_RNG_PHILOX, kRngMtSite = TWOS_RNG_MT_SITE, kRngMtPaired = TWOS_RNG_MT_PAIRED,
-       kRngMtKey = TWOS_RNG_MT_KEY };
+       kRngMtKey = TWOS_RNG_MT_KEY, kRngMtFd = TWOS_RNG_MT_FD };
+// Per-path extras a walk kernel is built with: none, the first-step gradient
+// samples, or the coupled finite-cavity pair of paths.
+enum { kChanNone = 0, kChanGrad = 1, kChanCoupled = 2 };
 // Kernel specialisations by geometry: any container (runtime switch), mesh
 // container only (no analytic holes), analytic container only.
 enum { kGeomGeneric = 0, kGeomMesh = 1, kGeomAnalytic = 2 };
@@ -48,6 +51,7 @@ struct SlotArrays {
   double *pos;        // [C][3]
   int32_t *geom;      // index into the geometry table
   int32_t *problem;   // index into the side table: sides 2*problem, 2*problem+1
+  int32_t *combine;   // the request's combine mode (the reduction's estimate)
   uint2 *keys;        // [C][2] Philox keys of sides 0 and 1
   int64_t *site;      // site index used by the mt19937_64 key layouts
   uint64_t *seed;     // request seed (mt19937_64 layouts)

@@ -51,11 +51,18 @@ struct Job {
   const DevSide *s0;
   const DevSide *s1;
   MeshView mv;  // the geometry's mesh buffers, kept per lane (unused fields fold away)
-  // Gradient channel (kernels built with kGrad only): the first sphere's
+  // Gradient channel (kernels built with kChanGrad only): the first sphere's
   // 3 omega0 / r0, its radius (0 until the first step) and the path values
   // after the first step's source term.
   V3<R> gw;
   R gr, gs0, gs1;
+  // Coupled finite-cavity pair (kChanCoupled only): the walk's start point,
+  // the candidate cavity (centre = the site, radius), the start point's
+  // weight, the first path's value and step word, which path runs (0: the
+  // geometry, 1: minus the cavity) and the start point's closest triangle.
+  V3<R> x0, z;
+  R rc, wgt, a0;
+  int32_t st0, phase, hint_start;
 };
 
 template <typename R>
@@ -131,9 +138,9 @@ __device__ __forceinline__ void job_store(const Job<R> &j, const WalkParams &wp,
 
 // First step of a path taken in direction d with radius r: keep what the
 // gradient sample needs (after the step's source term was added).
-template <typename R, bool kGrad>
+template <typename R, int kChan>
 __device__ __forceinline__ void job_first_step(Job<R> &j, V3<R> d, R r) {
-  if constexpr (kGrad) {
+  if constexpr (kChan == kChanGrad) {
     if (j.steps == 0) {
       j.gw = d * (R(3) / r);
       j.gr = r;
@@ -191,16 +198,125 @@ __device__ __forceinline__ V3<R> philox_dir(const Job<R> &j) {
 }
 
 // ---------------------------------------------------------------------------
+// Coupled finite-cavity difference (study.cpp:317-332).  A walk starts at a
+// point of the probe ball (sides[1]'s TERM_UNIT row: centre, radius R),
+// x = c + omega (R rho) with rho the cube root of a uniform, weight
+// (1 - rho^2)^2 35/8; path 0 runs in the geometry, then path 1 restarts from
+// the same point on the same stream with the candidate cavity (the site, the
+// geometry's cavity radius) appended after the geometry's own cavities.
+
+template <typename R>
+__device__ __forceinline__ void coupled_start(Job<R> &j, R rho, V3<R> omega) {
+  const double *t = j.s1->terms[0];
+  j.z = j.x;
+  j.rc = R(j.g->cavity_radius);
+  const R q = R(1) - rho * rho;
+  j.wgt = q * q * R(35) / R(8);
+  j.x = V3<R>{R(t[1]), R(t[2]), R(t[3])} + omega * (R(t[4]) * rho);
+  j.x0 = j.x;
+  j.a0 = R(0);
+  j.st0 = 0;
+  j.phase = 0;
+  j.hint_start = j.hint;
+}
+
+// Philox start point: the radial uniform from words 0-1 of counter block
+// (0xFFFFFFFF, walk, 0, 0), the direction from block (0xFFFFFFFF, walk, 1, 0)
+// (no step of a path reaches counter 0xFFFFFFFF).
+__device__ __forceinline__ double probe_uniform(uint4 b, double) {
+  return double(((uint64_t(b.y) << 32) | b.x) >> 11) * 0x1.0p-53;
+}
+__device__ __forceinline__ float probe_uniform(uint4 b, float) { return float(b.y >> 8) * 0x1.0p-24f; }
+
+template <typename R>
+__device__ __forceinline__ void coupled_start_philox(Job<R> &j) {
+  const uint4 b = philox4x32_10(make_uint4(0xFFFFFFFFu, uint32_t(j.walk), 0u, 0u), j.key);
+  const R rho = cbrt(probe_uniform(b, R(0)));
+  const V3<R> omega = philox_direction(philox4x32_10(make_uint4(0xFFFFFFFFu, uint32_t(j.walk), 1u, 0u), j.key), R(0));
+  coupled_start(j, rho, omega);
+}
+
+// The candidate cavity in path 1, after the geometry's own boundary (it wins
+// only when strictly closer, as the last cavity of `added`).
+template <typename R>
+__device__ __forceinline__ R coupled_cavity(const Job<R> &j, R best, V3<R> &point, int &cls) {
+  if (j.phase == 0) return best;
+  const V3<R> d = j.x - j.z;
+  const R len = sqrt(dot(d, d));
+  const R dist = len - j.rc;
+  if (dist < best && len > R(0)) {
+    best = dist;
+    point = j.z + d * (j.rc / len);
+    cls = 7;
+  }
+  return best;
+}
+
+// Path 0 ended: keep its value and step word, restart from the start point.
+template <typename R>
+__device__ __forceinline__ void coupled_second_path(Job<R> &j, bool capped) {
+  j.a0 = j.acc0;
+  j.st0 = j.steps | (capped ? (1 << 30) : 0);
+  j.x = j.x0;
+  j.acc0 = R(0);
+  j.steps = 0;
+  j.phase = 1;
+  j.hint = j.hint_start;
+}
+
+// Both paths ended: d = w (a - b) / (4 pi r) in binary64, as study.cpp:327
+// evaluates it, in both sample slots; both step words.
+template <typename R>
+__device__ __forceinline__ void coupled_store(const Job<R> &j, const WalkParams &wp, bool capped) {
+  const double d = double(j.wgt) * (double(j.a0) - double(j.acc0)) / (4 * 3.141592653589793 * j.g->cavity_radius);
+  wp.val[2 * j.t + 0] = d;
+  wp.val[2 * j.t + 1] = d;
+  wp.steps[2 * j.t + 0] = j.st0;
+  wp.steps[2 * j.t + 1] = j.steps | (capped ? (1 << 30) : 0);
+}
+
+// job_step of a coupled pair: the same step with the candidate cavity in
+// path 1 (sampled sources of side 0 only).
+template <typename R, int GK, class Green>
+__device__ __forceinline__ int coupled_step(Job<R> &j, const WalkParams &wp, R *radius_out, Green &&green) {
+  if (j.steps >= j.max_steps) return 2;
+  V3<R> bp;
+  int cls;
+  R r = closest_boundary<R, GK, false>(*j.g, j.x, bp, cls, j.hint, nullptr, j.tau);
+  r = coupled_cavity(j, r, bp, cls);
+  if (radius_out) *radius_out = r;
+  if (!isfinite(r)) return 2;
+  if (!(r > j.eps)) {
+    j.acc0 += side_payoff<R>(*j.s0, bp, cls);
+    return 1;
+  }
+  if (j.s0->source_kind) j.acc0 += side_green_term<R>(*j.s0, j.x, j.s0->source_kind == 2 ? j.x + green(r) : j.x, r, false);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
 // Simple kernel: one thread per job, the whole path in one go.  Used by the
 // plain schedule (one launch per site and round), by the mt19937_64 replay
 // streams and by the trace entry point.
-template <typename R, int GK, bool kMt, bool kGrad = false>
+template <typename R, int GK, bool kMt, int kChan = kChanNone>
 __global__ __launch_bounds__(256) void walk_simple(Tables tb, WalkParams wp) {
   const int64_t u = wp.job_begin + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (u >= wp.job_end) return;
   Job<R> j;
   job_init(j, tb, wp, u);
   Mt64 mt;
+  if constexpr (kChan == kChanCoupled) {
+    if constexpr (kMt) {
+      // Start point from mt19937_64(splitmix64(key)): the cube root of a
+      // uniform, then random_unit_vector (study.cpp:323-324).
+      mt.seed(splitmix64(mt_fd_key(tb.slots.seed[j.slot], tb.slots.site[j.slot], j.walk)));
+      const double rho = cbrt(mt.uniform());
+      const V3<double> om = mt.direction();
+      coupled_start(j, R(rho), V3<R>{R(om.x), R(om.y), R(om.z)});
+    } else {
+      coupled_start_philox(j);
+    }
+  }
   if constexpr (kMt) {
     const int32_t slot = j.slot;
     mt.seed(mt_walk_seed(wp.rng, tb.slots.seed[slot], tb.slots.repeat[slot], tb.slots.site[slot], j.walk));
@@ -216,7 +332,22 @@ __global__ __launch_bounds__(256) void walk_simple(Tables tb, WalkParams wp) {
         return philox_green<R>(j.key, j.walk, j.steps, rr);
       }
     };
-    const int state = job_step<R, GK, false>(j, wp, nullptr, &r, green);
+    const int state = kChan == kChanCoupled ? coupled_step<R, GK>(j, wp, &r, green)
+                                            : job_step<R, GK, false>(j, wp, nullptr, &r, green);
+    if constexpr (kChan == kChanCoupled) {
+      if (state != 0) {
+        if (j.phase == 0) {
+          coupled_second_path(j, state == 2);
+          if constexpr (kMt) {
+            const int32_t slot = j.slot;
+            mt.seed(mt_walk_seed(wp.rng, tb.slots.seed[slot], tb.slots.repeat[slot], tb.slots.site[slot], j.walk));
+          }
+          continue;
+        }
+        coupled_store(j, wp, state == 2);
+        break;
+      }
+    }
     if (wp.trace && row < wp.trace_rows && state != 2) {
       double *o = wp.trace + 4 * row++;
       o[0] = double(j.x.x); o[1] = double(j.x.y); o[2] = double(j.x.z); o[3] = double(r);
@@ -240,7 +371,7 @@ __global__ __launch_bounds__(256) void walk_simple(Tables tb, WalkParams wp) {
     } else {
       d = philox_dir(j);
     }
-    job_first_step<R, kGrad>(j, d, r);
+    job_first_step<R, kChan>(j, d, r);
     j.x = j.x + d * r;
     ++j.steps;
   }
@@ -269,7 +400,7 @@ constexpr size_t persistent_lds_bytes(int lds, int block) { return size_t(lds) *
 
 // Applies a finished boundary query (distance r, closest point bp of class
 // cls) to the path: 0 moved on, 1 ended in the eps-shell, 2 capped.
-template <typename R, bool kGrad = false>
+template <typename R, int kChan = kChanNone>
 __device__ __forceinline__ int job_advance(Job<R> &j, const WalkParams &wp, R r, V3<R> bp, int cls) {
   if (!isfinite(r)) return 2;  // as in job_step: a failed query ends the path capped
   const DevSide &s0 = *j.s0;
@@ -280,7 +411,7 @@ __device__ __forceinline__ int job_advance(Job<R> &j, const WalkParams &wp, R r,
   }
   job_source(j, wp, r, job_sampled(j, wp) ? j.x + philox_green<R>(j.key, j.walk, j.steps, r) : j.x);
   const V3<R> d = philox_dir(j);
-  job_first_step<R, kGrad>(j, d, r);
+  job_first_step<R, kChan>(j, d, r);
   j.x = j.x + d * r;
   ++j.steps;
   return j.steps >= j.max_steps ? 2 : 0;
@@ -316,7 +447,7 @@ __device__ __forceinline__ bool query_start(Job<R> &j, MeshQuery<R> &q, Traversa
 // (amdgpu_waves_per_eu); 1 leaves the choice to the compiler.  kLds is the
 // per-lane LDS stack depth; the launch passes persistent_lds_bytes(kLds,
 // blockDim.x) of dynamic LDS.
-template <typename R, int GK, bool kCount, int kWaves, int kLds, bool kGrad = false>
+template <typename R, int GK, bool kCount, int kWaves, int kLds, int kChan = kChanNone>
 __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWaves))) void walk_persistent(
     Tables tb, WalkParams wp) {
   extern __shared__ uint32_t lds_stack[];
@@ -350,6 +481,7 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWav
       const int rank = __popcll(need & ((1ull << lane) - 1ull));
       if (!active && rank < take) {
         job_init(j, tb, wp, chunk_cur + rank);
+        if constexpr (kChan == kChanCoupled) coupled_start_philox(j);
         active = true;
         in_query = query_start<R, GK, kCount>(j, q, &cnt);
       }
@@ -377,11 +509,16 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWav
           r = container_analytic<R>(*j.g, j.x, bp, cls);
         }
         r = holes_and_cavities<R, GK>(*j.g, j.x, r, bp, cls);
-        const int st = job_advance<R, kGrad>(j, wp, r, bp, cls);
+        if constexpr (kChan == kChanCoupled) r = coupled_cavity(j, r, bp, cls);
+        const int st = job_advance<R, kChan>(j, wp, r, bp, cls);
         if (st == 0) {
           in_query = query_start<R, GK, kCount>(j, q, &cnt);
+        } else if (kChan == kChanCoupled && j.phase == 0) {
+          coupled_second_path(j, st == 2);
+          in_query = query_start<R, GK, kCount>(j, q, &cnt);
         } else {
-          job_store(j, wp, st == 2);
+          if constexpr (kChan == kChanCoupled) coupled_store(j, wp, st == 2);
+          else job_store(j, wp, st == 2);
           active = false;
         }
       }
@@ -455,6 +592,18 @@ __device__ __forceinline__ void site_estimate(const double *s, int64_t count, tw
   r.score_se = sqrt((n - 1.0) / n * spread);
 }
 
+// Coupled finite-cavity estimate from the same sums (both sample slots hold
+// d): the mean of d and sqrt(max(0, (sum d^2 - N mean^2) / (N (N-1)))), as
+// study.cpp:329-331 computes v.up and v.same.
+__device__ __forceinline__ void coupled_estimate(const double *s, int64_t count, twos_site_result &r) {
+  const double n = double(count);
+  const double up = s[0] / n;
+  r.u = r.v = r.score = up;
+  r.td = -up;
+  r.same = s[2] / n;
+  r.score_se = count >= 2 ? sqrt(fmax(0.0, (s[2] - n * up * up) / (n * (n - 1.0)))) : __builtin_nan("");
+}
+
 template <int kBlock>
 __global__ __launch_bounds__(kBlock) void reduce_sites(Tables tb, WalkParams wp) {
   const int32_t k = blockIdx.x;
@@ -505,7 +654,8 @@ __global__ __launch_bounds__(kBlock) void reduce_sites(Tables tb, WalkParams wp)
     tb.slots.acc_steps[slot] += tsteps;
     tb.slots.acc_caps[slot] += tcaps;
     twos_site_result r;
-    site_estimate(acc, walks, r);
+    if (tb.slots.combine[slot] == kCoupled) coupled_estimate(acc, walks, r);
+    else site_estimate(acc, walks, r);
     for (int q = 0; q < 3; ++q) r.grad_u[q] = r.grad_v[q] = __builtin_nan("");
     r.grad_dot = r.grad_dot_se = __builtin_nan("");  // filled by reduce_grad when requested
     r.walks = walks;

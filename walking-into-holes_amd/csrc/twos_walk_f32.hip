@@ -17,7 +17,11 @@ hipError_t launch_walk_simple_f32(int geom_kind, const Tables &tb, const WalkPar
   if (n <= 0) return hipSuccess;
   const dim3 grid(blocks_for(n, block)), blk(block);
   if (wp.grad) {  // gradient channel: one generic variant
-    hipLaunchKernelGGL((walk_simple<float, kGeomGeneric, false, true>), grid, blk, 0, stream, tb, wp);
+    hipLaunchKernelGGL((walk_simple<float, kGeomGeneric, false, kChanGrad>), grid, blk, 0, stream, tb, wp);
+    return hipGetLastError();
+  }
+  if (wp.combine == kCoupled) {  // coupled finite-cavity pairs: one generic variant
+    hipLaunchKernelGGL((walk_simple<float, kGeomGeneric, false, kChanCoupled>), grid, blk, 0, stream, tb, wp);
     return hipGetLastError();
   }
   switch (geom_kind) {
@@ -63,8 +67,11 @@ hipError_t persistent_f32(int geom_kind, bool count, const Tables &tb, const Wal
 // is one resident wave set: blocks-per-CU from the occupancy query times CUs.
 hipError_t launch_walk_persistent_f32(int geom_kind, bool count, int waves, int lds, const Tables &tb,
                                       const WalkParams &wp, int block, int cus, hipStream_t stream, int *grid_out) {
-  if (wp.grad) {  // gradient channel: the compiler's allocation, 24-entry stack, any geometry
-    void (*k)(Tables, WalkParams) = walk_persistent<float, kGeomGeneric, false, 1, kLdsDefault, true>;
+  if (wp.grad || wp.combine == kCoupled) {
+    // Gradient channel or coupled finite-cavity pairs: the compiler's
+    // allocation, 24-entry stack, any geometry.
+    void (*k)(Tables, WalkParams) = wp.grad ? walk_persistent<float, kGeomGeneric, false, 1, kLdsDefault, kChanGrad>
+                                            : walk_persistent<float, kGeomGeneric, false, 1, kLdsDefault, kChanCoupled>;
     const size_t smem = persistent_lds_bytes(kLdsDefault, block);
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, smem);

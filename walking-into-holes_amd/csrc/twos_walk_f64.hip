@@ -19,12 +19,18 @@ hipError_t launch_walk_simple_f64(int geom_kind, bool mt, const Tables &tb, cons
   if (n <= 0) return hipSuccess;
   const dim3 grid(blocks_for(n, block)), blk(block);
   if (mt) {
-    if (wp.grad) hipLaunchKernelGGL((walk_simple<double, kGeomGeneric, true, true>), grid, blk, 0, stream, tb, wp);
+    if (wp.grad) hipLaunchKernelGGL((walk_simple<double, kGeomGeneric, true, kChanGrad>), grid, blk, 0, stream, tb, wp);
+    else if (wp.combine == kCoupled)
+      hipLaunchKernelGGL((walk_simple<double, kGeomGeneric, true, kChanCoupled>), grid, blk, 0, stream, tb, wp);
     else hipLaunchKernelGGL((walk_simple<double, kGeomGeneric, true>), grid, blk, 0, stream, tb, wp);
     return hipGetLastError();
   }
   if (wp.grad) {  // gradient channel: one generic variant
-    hipLaunchKernelGGL((walk_simple<double, kGeomGeneric, false, true>), grid, blk, 0, stream, tb, wp);
+    hipLaunchKernelGGL((walk_simple<double, kGeomGeneric, false, kChanGrad>), grid, blk, 0, stream, tb, wp);
+    return hipGetLastError();
+  }
+  if (wp.combine == kCoupled) {  // coupled finite-cavity pairs: one generic variant
+    hipLaunchKernelGGL((walk_simple<double, kGeomGeneric, false, kChanCoupled>), grid, blk, 0, stream, tb, wp);
     return hipGetLastError();
   }
   switch (geom_kind) {
@@ -43,12 +49,14 @@ hipError_t launch_walk_simple_f64(int geom_kind, bool mt, const Tables &tb, cons
 hipError_t launch_walk_persistent_f64(int geom_kind, bool count, const Tables &tb, const WalkParams &wp, int block,
                                       int cus, hipStream_t stream, int *grid_out) {
   void (*k)(Tables, WalkParams);
-  if (wp.grad) k = walk_persistent<double, kGeomGeneric, false, 1, kLdsDefault, true>;
+  if (wp.grad) k = walk_persistent<double, kGeomGeneric, false, 1, kLdsDefault, kChanGrad>;
+  else if (wp.combine == kCoupled) k = walk_persistent<double, kGeomGeneric, false, 1, kLdsDefault, kChanCoupled>;
   else if (count) k = walk_persistent<double, kGeomGeneric, true, 1, kLdsDefault>;
   else if (geom_kind == kGeomMesh) k = walk_persistent<double, kGeomMesh, false, 1, kLdsDefault>;
   else if (geom_kind == kGeomAnalytic) k = walk_persistent<double, kGeomAnalytic, false, 1, kLdsDefault>;
   else k = walk_persistent<double, kGeomGeneric, false, 1, kLdsDefault>;
-  const size_t smem = geom_kind == kGeomAnalytic && !count && !wp.grad ? 0 : persistent_lds_bytes(kLdsDefault, block);
+  const bool stackless = geom_kind == kGeomAnalytic && !count && !wp.grad && wp.combine != kCoupled;
+  const size_t smem = stackless ? 0 : persistent_lds_bytes(kLdsDefault, block);
   int per_cu = 0;
   hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, smem);
   if (e != hipSuccess) return e;

@@ -23,15 +23,15 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _capi
-from ._capi import (BALL, BOX, CROSS, MESH, PAIRED, PERSISTENT, PLAIN, RNG_MT_KEY, RNG_MT_PAIRED,
-                    RNG_MT_SITE, RNG_PHILOX, SQUARE, FLAG_COUNT_TRAVERSAL, FLAG_DISTANCE_GRID, FLAG_GRADIENT, MAX_TERMS,
+from ._capi import (BALL, BOX, COUPLED, CROSS, MESH, PAIRED, PERSISTENT, PLAIN, RNG_MT_FD, RNG_MT_KEY,
+                    RNG_MT_PAIRED, RNG_MT_SITE, RNG_PHILOX, SQUARE, FLAG_COUNT_TRAVERSAL, FLAG_DISTANCE_GRID, FLAG_GRADIENT, MAX_TERMS,
                     TERM_BUMP, TERM_GAUSS, TERM_UNIT)
 
 __all__ = [
     "Context", "Mesh", "Side", "Result", "Server", "TwosError", "InvalidArgument", "StepCapError",
     "GeometryError", "AdmissionError", "DeviceError", "NotFoundError", "side", "reference_affine_side",
-    "BALL", "BOX", "MESH", "SQUARE", "PAIRED", "CROSS", "RNG_PHILOX", "RNG_MT_SITE", "RNG_MT_PAIRED",
-    "RNG_MT_KEY", "PLAIN", "PERSISTENT", "FLAG_COUNT_TRAVERSAL", "FLAG_DISTANCE_GRID", "FLAG_GRADIENT", "version",
+    "BALL", "BOX", "MESH", "SQUARE", "PAIRED", "CROSS", "COUPLED", "RNG_PHILOX", "RNG_MT_SITE", "RNG_MT_PAIRED",
+    "RNG_MT_KEY", "RNG_MT_FD", "probe_side", "PLAIN", "PERSISTENT", "FLAG_COUNT_TRAVERSAL", "FLAG_DISTANCE_GRID", "FLAG_GRADIENT", "version",
     "TERM_BUMP", "TERM_UNIT", "TERM_GAUSS", "MAX_TERMS", "study_terms", "housing_chip_terms",
 ]
 
@@ -144,6 +144,16 @@ def side(poly=None, cls_const=None, poly_classes=(), source=None, terms=None, of
         s.source_kind = 1
         s.src = (C.c_double * 7)(*source)
     return s
+
+
+def probe_side(center, radius: float) -> Side:
+    """The objective density of a COUPLED estimate: the unit-mass bump
+    (1 - |x - c|^2 / R^2)^2 / (32 pi R^3 / 105) on the ball B(c, R)
+    (study.cpp:44-51); COUPLED draws each walk's start point uniformly in that
+    ball with weight (1 - rho^2)^2 35/8."""
+    r = float(radius)
+    mass = 32 * 3.141592653589793 * float(np.power(r, 3.0)) / 105
+    return side(terms=[(TERM_UNIT, *[float(c) for c in center], r, mass, 0.0, 0.0)])
 
 
 def reference_affine_side(slope: float = 0.65) -> Side:

@@ -22,8 +22,8 @@ ERR_DEVICE = 6
 ERR_NOT_FOUND = 7
 
 BALL, BOX, MESH = 0, 1, 2
-SQUARE, PAIRED, CROSS = 0, 1, 2
-RNG_PHILOX, RNG_MT_SITE, RNG_MT_PAIRED, RNG_MT_KEY = 0, 1, 2, 3
+SQUARE, PAIRED, CROSS, COUPLED = 0, 1, 2, 3
+RNG_PHILOX, RNG_MT_SITE, RNG_MT_PAIRED, RNG_MT_KEY, RNG_MT_FD = 0, 1, 2, 3, 4
 PLAIN, PERSISTENT = 0, 1
 FLAG_COUNT_TRAVERSAL = 1
 FLAG_DISTANCE_GRID = 2
