@@ -113,7 +113,14 @@ typedef int32_t twos_status;
  * wih_twos.cpp:546-552); source_kind 2 adds (r^2/6) f(a) with
  * f = s0 + sum of the n_terms rows, at a Green-distributed point a of the
  * ball (a = x + r t omega, t ~ Beta(2,2); wih_twos.cpp:648-653,
- * study.cpp:106).  Both sides of a PAIRED path share one point a. */
+ * study.cpp:106).  Both sides of a PAIRED path share one point a.
+ * source_kind 3 (the reference's analytic-source variant, study.cpp:56-76,
+ * :98-103) adds per step, instead of a sampled point, the exact integral of
+ * the ball's Green function against s0 and each BUMP or UNIT row (amplitude w,
+ * or 1/p1 for UNIT): for a bump of radius R and amplitude A centred at
+ * distance d from the ball centre, Phi(d) minus the mean of Phi over the
+ * ball's sphere, Phi the bump's Newtonian potential (binary64 in every
+ * precision; no random draw). */
 typedef struct {
   double poly[10];
   double cls_const[8];

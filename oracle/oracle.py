@@ -102,6 +102,8 @@ def lib() -> C.CDLL:
         L.og_combine.argtypes = [_dp, _dp, C.c_int64, C.c_int32, _dp]
         L.og_combine_powersums.argtypes = [_dp, C.c_int64, _dp]
         L.og_combine_grad.argtypes = [_dp, _dp, C.c_int64, _dp]
+        L.og_bump_green_ball.argtypes = [_dp, C.c_double, _dp, C.c_double, C.c_double]
+        L.og_bump_green_ball.restype = C.c_double
         _lib = L
     return _lib
 
@@ -208,11 +210,14 @@ class Geometry:
         return g
 
 
-def side(poly=None, cls_const=None, poly_classes=(), source=None, terms=None, offset=0.0) -> Side:
+def side(poly=None, cls_const=None, poly_classes=(), source=None, terms=None, offset=0.0,
+         analytic: bool = False) -> Side:
     """Boundary data g = cls_const[class] + [class in poly_classes] * poly(y);
     source = (s0, sx, sy, sz, ox, oy, oz) adds (r^2/6) f(x) per step at the ball
     centre; terms = rows (type, cx, cy, cz, p0, p1, p2, w) add (r^2/6) f(a) with
-    f = offset + sum of terms at a Green-distributed point a."""
+    f = offset + sum of terms at a Green-distributed point a, or with
+    analytic=True the exact Green-ball integral of offset and each BUMP / UNIT
+    row."""
     s = Side()
     if terms is not None:
         if source is not None:
@@ -220,7 +225,7 @@ def side(poly=None, cls_const=None, poly_classes=(), source=None, terms=None, of
         terms = [list(map(float, t)) for t in terms]
         if not 0 < len(terms) <= MAX_TERMS or any(len(t) != 8 for t in terms):
             raise ValueError(f"expected 1..{MAX_TERMS} term rows of 8 numbers")
-        s.source_kind = 2
+        s.source_kind = 3 if analytic else 2
         s.src[0] = offset
         s.n_terms = len(terms)
         for k, t in enumerate(terms):
@@ -318,6 +323,12 @@ def combine_powersums(sums, n: int) -> np.ndarray:
     out = np.zeros(6)
     lib().og_combine_powersums(_ptr(s, _dp), n, _ptr(out, _dp))
     return out
+
+
+def bump_green_ball(x, radius: float, center, support: float, amplitude: float) -> float:
+    """Exact integral of the Green function of B(x, radius) against the bump
+    amplitude (1 - |y - center|^2 / support^2)^2."""
+    return lib().og_bump_green_ball(_ptr(_f64(x), _dp), radius, _ptr(_f64(center), _dp), support, amplitude)
 
 
 def combine_grad(ga, gb) -> np.ndarray:

@@ -519,10 +519,52 @@ double field(const og_side &s, const V3 &y) {
   return f;
 }
 
+// Analytic source (source_kind 3): the exact integral of the Green function
+// of B(x, r) against a compact bump A (1 - s^2/R^2)^2 whose centre is d from
+// x, as exact_bump_green computes it (study.cpp:56-76, long double):
+// Phi(d) minus the mean of Phi over the sphere |y - x| = r, Phi the bump's
+// Newtonian potential and the mean (P(d + r) - P(|d - r|)) / (2 d r) with
+// P' = s Phi.  Derivation: -Lap Phi = bump with Phi radial, so
+// Phi = A R^2 (1/6 - u^2/6 + u^4/10 - u^6/42) inside (u = s/R) and
+// M / (4 pi s) outside, M = 4 pi A R^3 8/105 the bump's mass.
+using LD = long double;
+LD bump_potential(LD s, LD R, LD A) {
+  if (s >= R) return A * R * R * R * (8.0L / 105.0L) / s;
+  const LD u = s / R, u2 = u * u;
+  return A * R * R * (1.0L / 6.0L - u2 / 6.0L + u2 * u2 / 10.0L - u2 * u2 * u2 / 42.0L);
+}
+LD bump_primitive(LD s, LD R, LD A) {
+  const LD u = std::min(s / R, 1.0L), u2 = u * u;
+  const LD inner = A * R * R * R * R * (u2 / 12.0L - u2 * u2 / 24.0L + u2 * u2 * u2 / 60.0L - u2 * u2 * u2 * u2 / 336.0L);
+  return s > R ? inner + A * R * R * R * (8.0L / 105.0L) * (s - R) : inner;
+}
+double bump_green_ball(const V3 &x, double radius, const V3 &c, double support, double amplitude) {
+  const V3 dv = x - c;
+  const LD d = std::sqrt(LD(dv.x) * dv.x + LD(dv.y) * dv.y + LD(dv.z) * dv.z);
+  const LD r = radius, R = support, A = amplitude;
+  if (d >= r + R) return 0.0;
+  LD mean;
+  if (d + R <= r) mean = A * R * R * R * (8.0L / 105.0L) / r;
+  else if (d < 1e-12L * R) mean = bump_potential(r, R, A);
+  else mean = (bump_primitive(d + r, R, A) - bump_primitive(std::fabs(d - r), R, A)) / (2 * d * r);
+  return double(std::max(0.0L, bump_potential(d, R, A) - mean));
+}
+
+double analytic_term(const og_side &s, const V3 &x, double radius) {
+  double sum = s.src[0] * radius * radius / 6.0;
+  for (int32_t k = 0; k < std::min(s.n_terms, int32_t(OG_MAX_TERMS)); ++k) {
+    const double *t = s.terms[k];
+    const double amp = int(t[0]) == OG_TERM_UNIT ? 1.0 / t[5] : t[7];
+    sum += bump_green_ball(x, radius, V3{t[1], t[2], t[3]}, t[4], amp);
+  }
+  return sum;
+}
+
 // Green-ball source term of one step for one side: (r^2/6) f.  `paired_form`
 // follows paired_site's association r*r/6*f (study.cpp:106-108), otherwise
 // walk()'s f*r*r/6 (wih_twos.cpp:648-653).
 double green_term(const og_side &s, const V3 &x, const V3 &a, double radius, bool paired_form) {
+  if (s.source_kind == 3) return analytic_term(s, x, radius);
   if (s.source_kind == 1) return source(s, x) * radius * radius / 6.0;
   if (s.source_kind == 2)
     return paired_form ? radius * radius / 6.0 * field(s, a) : field(s, a) * radius * radius / 6.0;
@@ -947,10 +989,14 @@ int32_t og_estimate(const og_geom *g, const og_side sides[2], const og_params *p
   if (p.rng == OG_RNG_MT_KEY || p.rng < 0 || p.rng > OG_RNG_MT_FD) return 2;
   if ((p.rng == OG_RNG_MT_FD) != (p.combine == OG_COUPLED) && p.rng != OG_RNG_PHILOX) return 2;
   if (p.combine == OG_COUPLED &&
-      (p.gradient || !(g->cavity_radius > 0.0) || sides[1].source_kind != 2 || sides[1].n_terms != 1 ||
+      (p.gradient || !(g->cavity_radius > 0.0) || sides[1].source_kind < 2 || sides[1].n_terms != 1 ||
        sides[1].terms[0][0] != OG_TERM_UNIT || !(sides[1].terms[0][4] > 0.0)))
     return 2;
-  if (p.gradient && (sides[0].source_kind == 2 || sides[1].source_kind == 2)) return 2;
+  if (p.gradient && (sides[0].source_kind >= 2 || sides[1].source_kind >= 2)) return 2;
+  for (int k = 0; k < 2; ++k)
+    if (sides[k].source_kind == 3)
+      for (int32_t i = 0; i < sides[k].n_terms; ++i)
+        if (sides[k].terms[i][0] == OG_TERM_GAUSS) return 2;
   const int32_t walks_max = std::max(p.walks, p.walks_max);
   int32_t status = 0;
 #ifdef _OPENMP
@@ -1023,6 +1069,10 @@ int32_t og_estimate(const og_geom *g, const og_side sides[2], const og_params *p
   for (int64_t s = 0; s < n; ++s)
     if (out[s].caps) status = 3;
   return status;
+}
+
+double og_bump_green_ball(const double x[3], double radius, const double c[3], double support, double amplitude) {
+  return bump_green_ball(load3(x), radius, load3(c), support, amplitude);
 }
 
 void og_combine_grad(const double *ga, const double *gb, int64_t n, double out[8]) {

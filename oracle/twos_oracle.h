@@ -71,7 +71,8 @@ typedef struct {
   double cls_const[8];  /* per boundary class constant */
   uint32_t cls_poly_mask; /* bit c set: class c adds poly(y) */
   int32_t source_kind;  /* 0 none; 1 affine f = ((s0 + sx (x+ox)) + sy (y+oy)) + sz (z+oz) at the
-                           ball centre; 2 f = s0 + sum of terms at a Green-distributed point */
+                           ball centre; 2 f = s0 + sum of terms at a Green-distributed point;
+                           3 the exact Green-ball integral of s0 and each BUMP / UNIT row */
   double src[7];        /* s0, sx, sy, sz, ox, oy, oz */
   int32_t n_terms;
   int32_t pad;
@@ -129,6 +130,9 @@ int32_t og_trace(const og_geom *g, const og_side *side, const og_params *prm,
 
 void og_combine(const double *a, const double *b, int64_t n, int32_t square, double out[6]);
 void og_combine_powersums(const double sums[8], int64_t n, double out[6]);
+/* Exact integral of the Green function of B(x, radius) against the bump
+ * amplitude (1 - |y - c|^2 / support^2)^2 (source_kind 3; study.cpp:56-76). */
+double og_bump_green_ball(const double x[3], double radius, const double c[3], double support, double amplitude);
 /* Gradient channel of n walks from per-walk samples ga, gb ([n][3] each):
  * out = {grad_u[3], grad_v[3], grad_dot, grad_dot_se}. */
 void og_combine_grad(const double *ga, const double *gb, int64_t n, double out[8]);

@@ -13,7 +13,12 @@
 // point, walks start uniformly in the sensor ball and run with and without a
 // cavity at the point on the same stream (TWOS_COUPLED); the score column is
 // the mean of w (a - b) / (4 pi r), same_index its standard error, u and p
-// zero, and no point is skipped.  The reference's multigrid solver
+// zero, and no point is skipped.  --analytic-source 1 runs the reference's
+// analytic-source variant (analytic_query.cpp: study.cpp built with
+// WIH_ANALYTIC_SOURCE, :98-103): each step adds the exact Green-ball integral
+// of the heat and sensor bumps (source_kind 3) instead of sampling a point;
+// like that variant it applies to the paired estimator, not to --fd.  The
+// reference's multigrid solver
 // (--mode mg) and Differential WoS (--diff 1) are other solvers; a request
 // asking for them fails with status 1.
 #include <chrono>
@@ -98,7 +103,7 @@ bool inside_cavity(const double *p, const std::vector<double> &holes, double rad
 
 int process_request(int argc, char **argv, Session &session) {
   try {
-    bool hot = false, fd = false, diff = false;
+    bool hot = false, fd = false, diff = false, analytic = false;
     std::string config;
     std::vector<char *> args{argv[0]};
     for (int i = 1; i < argc; ++i) {
@@ -109,6 +114,9 @@ int process_request(int argc, char **argv, Session &session) {
       } else if (key == "--fd") {
         if (i + 1 == argc) throw std::runtime_error("missing fd flag");
         fd = std::stoi(argv[++i]) != 0;
+      } else if (key == "--analytic-source") {
+        if (i + 1 == argc) throw std::runtime_error("missing analytic-source flag");
+        analytic = std::stoi(argv[++i]) != 0;
       } else if (key == "--diff") {
         if (i + 1 == argc) throw std::runtime_error("missing diff flag");
         diff = std::stoi(argv[++i]) != 0;
@@ -126,6 +134,7 @@ int process_request(int argc, char **argv, Session &session) {
     const wih::Options o = wih::parse_options(int(args.size()), args.data(), "wih_study");
     twos_side heat, sensor;
     load_config(config, heat, sensor);
+    if (analytic && !fd) heat.source_kind = sensor.source_kind = 3;
     if (o.mode != "twos" && o.mode != "mg") throw std::runtime_error("use twos or mg");
     if (o.mode == "mg") throw std::runtime_error("mode mg is not provided by this build (twos only)");
     if (diff) throw std::runtime_error("--diff 1 is not provided by this build");

@@ -797,6 +797,53 @@ __device__ __forceinline__ R side_field(const DevSide &s, V3<R> a) {
   return f;
 }
 
+// Exact Green-ball integral of a compact bump A (1 - s^2/R^2)^2 (s the
+// distance to its centre) over the ball B(x, r) whose centre is d from the
+// bump's: with Phi the bump's Newtonian potential (-Lap Phi = bump, Phi -> 0
+// at infinity), the integral of G_B(x, .) times the bump is Phi(d) minus the
+// mean of Phi over the sphere |y - x| = r, and that mean is
+// (P(d + r) - P(|d - r|)) / (2 d r) with P' (s) = s Phi(s) (the reference's
+// exact_bump_green, study.cpp:56-76, evaluates the same quantity in long
+// double).  In u = s / R and total mass M = 4 pi A R^3 8/105:
+//   Phi = A R^2 (1/6 - u^2/6 + u^4/10 - u^6/42)          (s < R)
+//       = A R^3 (8/105) / s                               (s >= R)
+//   P   = A R^4 u^2 (1/12 - u^2/24 + u^4/60 - u^6/336)    (s <= R)
+//       = P(R) + A R^3 (8/105) (s - R)                    (s > R)
+__device__ __forceinline__ double bump_potential(double s, double R, double A) {
+  if (s >= R) return A * R * R * R * (8.0 / 105.0) / s;
+  const double u2 = (s / R) * (s / R);
+  return A * R * R * (1.0 / 6.0 + u2 * (-1.0 / 6.0 + u2 * (1.0 / 10.0 - u2 / 42.0)));
+}
+__device__ __forceinline__ double bump_primitive(double s, double R, double A) {
+  const double u = fmin(s / R, 1.0), u2 = u * u;
+  const double inner = A * R * R * R * R * u2 * (1.0 / 12.0 + u2 * (-1.0 / 24.0 + u2 * (1.0 / 60.0 - u2 / 336.0)));
+  return s > R ? inner + A * R * R * R * (8.0 / 105.0) * (s - R) : inner;
+}
+__device__ __forceinline__ double bump_green_ball(double d, double r, double R, double A) {
+  if (d >= r + R) return 0.0;  // the bump lies outside the ball
+  double mean;
+  if (d + R <= r) mean = A * R * R * R * (8.0 / 105.0) / r;  // the sphere encloses the whole bump
+  else if (d < 1e-12 * R) mean = bump_potential(r, R, A);    // concentric: Phi is constant on the sphere
+  else mean = (bump_primitive(d + r, R, A) - bump_primitive(fabs(d - r), R, A)) / (2.0 * d * r);
+  return fmax(0.0, bump_potential(d, R, A) - mean);
+}
+
+// source_kind 3: the exact integral of the ball's Green function against s0
+// (s0 r^2 / 6) and each BUMP / UNIT row, in binary64.
+template <typename R>
+__device__ __forceinline__ double side_analytic(const DevSide &s, V3<R> x, R r) {
+  const double rr = double(r);
+  double sum = s.src[0] * rr * rr / 6.0;
+  const int n = min(s.n_terms, TWOS_MAX_TERMS);
+  for (int k = 0; k < n; ++k) {
+    const double *t = s.terms[k];
+    const double dx = double(x.x) - t[1], dy = double(x.y) - t[2], dz = double(x.z) - t[3];
+    const double amp = int(t[0]) == TWOS_TERM_UNIT ? 1.0 / t[5] : t[7];
+    sum += bump_green_ball(sqrt(dx * dx + dy * dy + dz * dz), rr, t[4], amp);
+  }
+  return sum;
+}
+
 // (r^2/6) f of one step for one side.  x: ball centre, a: Green point.
 // paired_form follows paired_site's r*r/6*f (study.cpp:106-108), otherwise
 // walk()'s f*r*r/6 (wih_twos.cpp:648-653).
@@ -804,6 +851,7 @@ template <typename R>
 __device__ __forceinline__ R side_green_term(const DevSide &s, V3<R> x, V3<R> a, R r, bool paired_form) {
   if (s.source_kind == 1) return side_source<R>(s, x) * r * r / R(6);
   if (s.source_kind == 2) return paired_form ? r * r / R(6) * side_field<R>(s, a) : side_field<R>(s, a) * r * r / R(6);
+  if (s.source_kind == 3) return R(side_analytic<R>(s, x, r));
   return R(0);
 }
 

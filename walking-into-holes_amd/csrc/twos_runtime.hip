@@ -730,8 +730,12 @@ twos_status check_params(const twos_params *p, int64_t n) {
 }
 
 twos_status check_side(const twos_side &s) {
-  if (s.source_kind < 0 || s.source_kind > 2) return fail(TWOS_ERR_INVALID_ARGUMENT, "unknown source kind");
-  if (s.source_kind == 2) {
+  if (s.source_kind < 0 || s.source_kind > 3) return fail(TWOS_ERR_INVALID_ARGUMENT, "unknown source kind");
+  if (s.source_kind == 3)
+    for (int k = 0; k < s.n_terms && k < TWOS_MAX_TERMS; ++k)
+      if (s.terms[k][0] == TWOS_TERM_GAUSS)
+        return fail(TWOS_ERR_INVALID_ARGUMENT, "the analytic source integrates BUMP and UNIT rows only");
+  if (s.source_kind >= 2) {
     if (s.n_terms < 1 || s.n_terms > TWOS_MAX_TERMS)
       return fail(TWOS_ERR_INVALID_ARGUMENT, "a sampled source needs 1..8 term rows");
     for (int k = 0; k < s.n_terms; ++k) {
@@ -759,13 +763,13 @@ Request *make_request(twos_ctx *ctx, int32_t geom, const twos_side sides[2], con
   }
   for (int k = 0; k < 2; ++k)
     if ((*st = check_side(sides[k])) != TWOS_OK) return nullptr;
-  if ((p->flags & TWOS_FLAG_GRADIENT) && (sides[0].source_kind == 2 || sides[1].source_kind == 2)) {
+  if ((p->flags & TWOS_FLAG_GRADIENT) && (sides[0].source_kind >= 2 || sides[1].source_kind >= 2)) {
     *st = fail(TWOS_ERR_INVALID_ARGUMENT, "the gradient channel needs an affine source or none");
     return nullptr;
   }
   if (p->combine == kCoupled) {
     const twos_side &q = sides[1];
-    if (q.source_kind != 2 || q.n_terms != 1 || q.terms[0][0] != TWOS_TERM_UNIT) {
+    if (q.source_kind < 2 || q.n_terms != 1 || q.terms[0][0] != TWOS_TERM_UNIT) {
       *st = fail(TWOS_ERR_INVALID_ARGUMENT, "COUPLED needs sides[1] with one TWOS_TERM_UNIT row (the probe ball)");
       return nullptr;
     }

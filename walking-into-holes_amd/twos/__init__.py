@@ -102,7 +102,8 @@ def _ptr(a: np.ndarray, t):
 Side = _capi.Side
 
 
-def side(poly=None, cls_const=None, poly_classes=(), source=None, terms=None, offset=0.0) -> Side:
+def side(poly=None, cls_const=None, poly_classes=(), source=None, terms=None, offset=0.0,
+         analytic: bool = False) -> Side:
     """Boundary data and source of one side.  The Dirichlet value at the closest
     boundary point y of class c is cls_const[c] + [c in poly_classes] * poly(y),
     poly = (c0, cx, cy, cz, cxx, cyy, czz, cxy, cxz, cyz).  Classes: 0-5 container
@@ -111,7 +112,10 @@ def side(poly=None, cls_const=None, poly_classes=(), source=None, terms=None, of
     f = ((s0 + sx (x+ox)) + sy (y+oy)) + sz (z+oz) at the ball centre.
     terms = rows (type, cx, cy, cz, p0, p1, p2, w) (TERM_BUMP, TERM_UNIT,
     TERM_GAUSS) add (r^2/6) f(a) with f = offset + sum of the rows, at a
-    Green-distributed point a of each step's ball (wih_twos.cpp:648-653)."""
+    Green-distributed point a of each step's ball (wih_twos.cpp:648-653);
+    with analytic=True each step adds instead the exact integral of the ball's
+    Green function against offset and each BUMP / UNIT row (the reference's
+    analytic-source variant, study.cpp:56-76), with no random draw."""
     s = Side()
     if terms is not None:
         if source is not None:
@@ -119,7 +123,9 @@ def side(poly=None, cls_const=None, poly_classes=(), source=None, terms=None, of
         rows = [[float(v) for v in t] for t in terms]
         if not 0 < len(rows) <= MAX_TERMS or any(len(t) != 8 for t in rows):
             raise InvalidArgument(f"terms needs 1..{MAX_TERMS} rows of 8 numbers")
-        s.source_kind = 2
+        if analytic and any(int(t[0]) == TERM_GAUSS for t in rows):
+            raise InvalidArgument("the analytic source integrates BUMP and UNIT rows only")
+        s.source_kind = 3 if analytic else 2
         s.src[0] = float(offset)
         s.n_terms = len(rows)
         for k, t in enumerate(rows):
