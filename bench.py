@@ -65,6 +65,7 @@ def parse() -> argparse.Namespace:
                          "(writes gpurun_out/tuning_cache.json)")
     ap.add_argument("--block", type=int, default=0, help="fixed launch shape (skips the autotuner)")
     ap.add_argument("--lds-stack", type=int, default=0, help="fixed launch shape (skips the autotuner)")
+    ap.add_argument("--step-batch", type=int, default=-1, help="fixed launch shape (skips the autotuner)")
     ap.add_argument("--grid-kappa", type=float, default=0.0, help="fixed launch shape (skips the autotuner)")
     ap.add_argument("--grid-cells", type=int, default=0, help="fixed distance-grid resolution (skips the autotuner)")
     ap.add_argument("--grid-corners", type=int, default=-1,
@@ -128,8 +129,9 @@ def main() -> int:
     shape = {}
     tuning = None
     fixed = {k: v for k, v in (("waves_per_simd", args.waves_per_simd), ("block", args.block),
-                               ("lds_stack", args.lds_stack), ("grid_kappa", args.grid_kappa))
-             if (v >= 0 if k == "waves_per_simd" else v > 0)}
+                               ("lds_stack", args.lds_stack), ("grid_kappa", args.grid_kappa),
+                               ("step_batch", args.step_batch))
+             if (v >= 0 if k in ("waves_per_simd", "step_batch") else v > 0)}
     if fixed or args.grid_cells > 0 or args.grid_corners >= 0:
         shape = fixed
         if (args.grid_cells > 0 or args.grid_corners >= 0) and mesh is not None:
@@ -174,7 +176,7 @@ def main() -> int:
     steps_per_s = float(np.sum(steps)) / total_t
     capped = int(np.sum(res.status == 1))
 
-    out_shape = {"block": 256, "waves_per_simd": 0, "lds_stack": 24,
+    out_shape = {"block": 256, "waves_per_simd": 0, "lds_stack": 24, "step_batch": 0,
                          **({"grid_kappa": 4.0, "grid_cells": 128, "grid_corners": 0} if grid_on else {}), **shape,
                          **({k: tuning.choice[k] for k in ("grid_cells", "grid_corners")}
                             if tuning is not None and "grid_cells" in tuning.choice else {}),

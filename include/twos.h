@@ -172,7 +172,12 @@ typedef struct {
                            least grid_kappa cell sizes (0 = 4) */
   int32_t lds_stack;    /* binary32 persistent kernel: LDS traversal-stack entries
                            per lane, 16, 24 or 32 (0 = 24) */
-  int32_t pad;
+  int32_t step_batch;   /* binary32 persistent kernel: 0, or 24, 32, 40 -- a lane
+                           whose query has ended takes its walk step once that
+                           many lanes of its wave are ready (1: at least as many
+                           as are still traversing), or none traverses;
+                           lds_stack 24 and waves_per_simd 0 or 5 only.  No
+                           result depends on it. */
 } twos_params;
 
 /* Count BVH nodes fetched and triangles tested (slower; for roofline figures). */

@@ -16,6 +16,8 @@ class FakeContext:
         self.calls = 0
         self.cells = 128
         self.cell_history = []
+        self.batches = []
+        self.batch_factor = {}
 
     def set_distance_grid(self, mesh, cells, corners=False):
         self.cells = cells
@@ -23,9 +25,11 @@ class FakeContext:
         self.cell_history.append(cells)
         return 0.0
 
-    def estimate(self, geom, pts, sides, *, block, waves_per_simd, lds_stack, grid_kappa, **kw):
+    def estimate(self, geom, pts, sides, *, block, waves_per_simd, lds_stack, grid_kappa, step_batch=0, **kw):
         self.calls += 1
+        self.batches.append((step_batch, waves_per_simd, lds_stack))
         ms = self.cost(block, waves_per_simd, lds_stack, grid_kappa) * (1.0 + 0.1 * abs(self.cells - 192) / 64)
+        ms *= self.batch_factor.get(step_batch, 1.0)
 
         class R:
             stats = {"walk_kernel_ms": ms * 1000.0, "paths": 1000}
@@ -43,15 +47,26 @@ def test_descent_finds_the_minimum():
     ctx = FakeContext(_surface)
     t = autotune.autotune(ctx, 0, np.zeros((10, 3)), None, geom_sig="fake", combine=2, distance_grid=True,
                           repeats=1)
-    assert t.choice == {"block": 128, "waves_per_simd": 5, "lds_stack": 16, "grid_kappa": 3.0}
+    assert t.choice == {"block": 128, "waves_per_simd": 5, "lds_stack": 16, "step_batch": 0, "grid_kappa": 3.0}
     assert t.speedup > 1.0
+    # Step batching exists only for the 24-entry stack with 0 or 5 waves.
+    assert all(b == 0 or (lds == 24 and w in (0, 5)) for b, w, lds in ctx.batches)
+
+
+def test_step_batch_is_chosen_when_faster():
+    autotune.clear_cache()
+    ctx = FakeContext(lambda b, w, l, k: 1.0 + 0.02 * (l != 24) + 0.05 * (w != 5))
+    ctx.batch_factor = {1: 0.85, 32: 0.9}
+    t = autotune.autotune(ctx, 0, np.zeros((10, 3)), None, geom_sig="fake6", combine=2, repeats=1)
+    assert t.choice["step_batch"] == 1 and t.choice["lds_stack"] == 24 and t.choice["waves_per_simd"] == 5
 
 
 def test_gains_below_one_percent_keep_the_default():
     autotune.clear_cache()
     t = autotune.autotune(FakeContext(lambda b, w, l, k: 1.0 - 0.005 * (b == 64)), 0, np.zeros((4, 3)), None,
                           geom_sig="fake5", combine=2, repeats=1)
-    assert t.choice == {k: autotune.DEFAULTS[k] for k in ("block", "waves_per_simd", "lds_stack", "grid_kappa")}
+    assert t.choice == {k: autotune.DEFAULTS[k] for k in ("block", "waves_per_simd", "lds_stack", "step_batch",
+                                                          "grid_kappa")}
 
 
 def test_invalid_variants_are_never_run():

@@ -710,9 +710,22 @@ __device__ __forceinline__ bool mq_visit(const MeshView &m, V3<R> p, MeshQuery<R
   R d[4] = {box_d2<R>(p, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x), box_d2<R>(p, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y),
             box_d2<R>(p, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z), box_d2<R>(p, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w)};
   int32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
+  // Leaf children (one triangle each) in slot order, one test per pass: the
+  // wave runs as many passes as its lanes have leaves to test, not one per
+  // slot that any lane uses.  A leaf whose box no longer beats the best
+  // distance is skipped; that skip never changes the result (its triangle is
+  // at least as far as its box).
+  uint32_t leaves = 0;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
-    if (c[i] < 0 && c[i] != kEmptyChild && mq_within(q, d[i])) mq_test<R, kCount>(m, p, q, ~c[i], cnt);
+    if (c[i] < 0 && c[i] != kEmptyChild && mq_within(q, d[i])) leaves |= 1u << i;
+  while (leaves) {
+    const int i = __builtin_ctz(leaves);
+    leaves &= leaves - 1;
+    const int32_t ci = i == 0 ? c[0] : (i == 1 ? c[1] : (i == 2 ? c[2] : c[3]));
+    const R di = i == 0 ? d[0] : (i == 1 ? d[1] : (i == 2 ? d[2] : d[3]));
+    if (mq_within(q, di)) mq_test<R, kCount>(m, p, q, ~ci, cnt);
+  }
   R key[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) key[i] = (c[i] >= 0 && mq_within(q, d[i])) ? d[i] : Inf<R>::v();

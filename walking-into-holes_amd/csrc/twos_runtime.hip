@@ -405,7 +405,8 @@ struct Engine {
       const int block = p.block > 0 ? p.block : 256;
       e = p.precision == 64
               ? launch_walk_persistent_f64(kind, count, tb, wp, block, cus, st, &grid)
-              : launch_walk_persistent_f32(kind, count, p.waves_per_simd, p.lds_stack, tb, wp, block, cus, st, &grid);
+              : launch_walk_persistent_f32(kind, count, p.waves_per_simd, p.lds_stack, p.step_batch, tb, wp, block,
+                                           cus, st, &grid);
       stats.grid = grid;
       stats.block = block;
       stats.waves_per_simd = p.waves_per_simd;
@@ -555,7 +556,7 @@ struct Engine {
       const twos_params &q = r0->prm;
       const int64_t key = int64_t(q.precision) | int64_t(q.combine) << 8 | int64_t(kind0) << 12 |
                           int64_t(q.waves_per_simd) << 16 | int64_t(q.block) << 24 | int64_t(q.lds_stack) << 36 |
-                          int64_t(q.flags & 7) << 44;
+                          int64_t(q.flags & 7) << 44 | int64_t(q.step_batch) << 48;
       if (graph_key != key) {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -726,6 +727,11 @@ twos_status check_params(const twos_params *p, int64_t n) {
     return fail(TWOS_ERR_INVALID_ARGUMENT, "grid_kappa must be a non-negative number of cells");
   if (p->waves_per_simd < 0 || p->waves_per_simd > 16)
     return fail(TWOS_ERR_INVALID_ARGUMENT, "waves_per_simd must be in [0, 16]");
+  if (p->step_batch != 0 && p->step_batch != 1 && p->step_batch != 24 && p->step_batch != 32 && p->step_batch != 40)
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "step_batch must be 0, 1, 24, 32 or 40");
+  if (p->step_batch != 0 && ((p->lds_stack != 0 && p->lds_stack != 24) || (p->waves_per_simd != 0 &&
+                                                                            p->waves_per_simd != 5)))
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "step_batch is built for lds_stack 24 with waves_per_simd 0 or 5");
   return TWOS_OK;
 }
 

@@ -446,8 +446,12 @@ __device__ __forceinline__ bool query_start(Job<R> &j, MeshQuery<R> &q, Traversa
 // kWaves is the occupancy the register allocator is asked to allow
 // (amdgpu_waves_per_eu); 1 leaves the choice to the compiler.  kLds is the
 // per-lane LDS stack depth; the launch passes persistent_lds_bytes(kLds,
-// blockDim.x) of dynamic LDS.
-template <typename R, int GK, bool kCount, int kWaves, int kLds, int kChan = kChanNone>
+// blockDim.x) of dynamic LDS.  kBatch > 0 batches the walk steps: a lane whose
+// query has ended waits (one node visit of the others at a time) until at
+// least kBatch lanes of its wave are ready (kBatch = 1: at least as many as are
+// still traversing) or none is traversing, so the wave runs the step code less
+// often; which lane takes which step, and every path, is unchanged.
+template <typename R, int GK, bool kCount, int kWaves, int kLds, int kChan = kChanNone, int kBatch = 0>
 __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWaves))) void walk_persistent(
     Tables tb, WalkParams wp) {
   extern __shared__ uint32_t lds_stack[];
@@ -491,7 +495,14 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWav
     if (!__any(active)) break;
     if (active) {
       if (in_query) in_query = !mq_visit<R, kCount, kLds>(j.mv, j.x, q, stack, stride, overflow, &cnt);
-      if (!in_query) {
+      bool go = !in_query;
+      if constexpr (kBatch > 0) {
+        // Wave-uniform: both masks are ballots over the active lanes.
+        const uint64_t busy = __ballot(in_query);
+        const int ready = __popcll(__ballot(!in_query));
+        go = go && (busy == 0 || ready >= (kBatch == 1 ? __popcll(busy) : kBatch));
+      }
+      if (go) {
         V3<R> bp;
         int cls;
         R r;

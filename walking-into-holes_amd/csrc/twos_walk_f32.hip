@@ -38,17 +38,18 @@ hipError_t launch_walk_simple_f32(int geom_kind, const Tables &tb, const WalkPar
 }
 
 namespace {
-template <int kWaves, int kLds>
+template <int kWaves, int kLds, int kBatch = 0>
 hipError_t persistent_f32(int geom_kind, bool count, const Tables &tb, const WalkParams &wp, int block,
                           int cus, hipStream_t stream, int *grid_out) {
   void (*k)(Tables, WalkParams);
   int lds = kLds;
   if (count) k = walk_persistent<float, kGeomGeneric, true, kWaves, kLds>;
-  else if (geom_kind == kGeomMesh) k = walk_persistent<float, kGeomMesh, false, kWaves, kLds>;
+  else if (geom_kind == kGeomMesh) k = walk_persistent<float, kGeomMesh, false, kWaves, kLds, kChanNone, kBatch>;
   else if (geom_kind == kGeomAnalytic) {
-    k = walk_persistent<float, kGeomAnalytic, false, kWaves, kLdsDefault>;  // no traversal stack in use
+    // no traversal stack in use
+    k = walk_persistent<float, kGeomAnalytic, false, kWaves, kLdsDefault, kChanNone, kBatch>;
     lds = 0;
-  } else k = walk_persistent<float, kGeomGeneric, false, kWaves, kLds>;
+  } else k = walk_persistent<float, kGeomGeneric, false, kWaves, kLds, kChanNone, kBatch>;
   const size_t smem = persistent_lds_bytes(lds, block);
   int per_cu = 0;
   hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, smem);
@@ -65,7 +66,7 @@ hipError_t persistent_f32(int geom_kind, bool count, const Tables &tb, const Wal
 // default 24-entry stack).  lds: per-lane LDS stack entries (16, 24 or 32;
 // 16 and 32 are built for the compiler's allocation and 5 waves).  The grid
 // is one resident wave set: blocks-per-CU from the occupancy query times CUs.
-hipError_t launch_walk_persistent_f32(int geom_kind, bool count, int waves, int lds, const Tables &tb,
+hipError_t launch_walk_persistent_f32(int geom_kind, bool count, int waves, int lds, int batch, const Tables &tb,
                                       const WalkParams &wp, int block, int cus, hipStream_t stream, int *grid_out) {
   if (wp.grad || wp.combine == kCoupled) {
     // Gradient channel or coupled finite-cavity pairs: the compiler's
@@ -79,6 +80,17 @@ hipError_t launch_walk_persistent_f32(int geom_kind, bool count, int waves, int 
     *grid_out = std::max(1, per_cu) * cus;
     hipLaunchKernelGGL(k, dim3(*grid_out), dim3(block), smem, stream, tb, wp);
     return hipGetLastError();
+  }
+  if (batch && !count && (lds == 24 || lds == 0)) {
+    const bool w5 = waves == 5;
+    if (batch == 1) return w5 ? persistent_f32<5, 24, 1>(geom_kind, count, tb, wp, block, cus, stream, grid_out)
+                              : persistent_f32<1, 24, 1>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
+    if (batch == 24) return w5 ? persistent_f32<5, 24, 24>(geom_kind, count, tb, wp, block, cus, stream, grid_out)
+                               : persistent_f32<1, 24, 24>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
+    if (batch == 32) return w5 ? persistent_f32<5, 24, 32>(geom_kind, count, tb, wp, block, cus, stream, grid_out)
+                               : persistent_f32<1, 24, 32>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
+    if (batch == 40) return w5 ? persistent_f32<5, 24, 40>(geom_kind, count, tb, wp, block, cus, stream, grid_out)
+                               : persistent_f32<1, 24, 40>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
   }
   if (lds == 16) {
     if (waves == 5) return persistent_f32<5, 16>(geom_kind, count, tb, wp, block, cus, stream, grid_out);

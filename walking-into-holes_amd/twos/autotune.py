@@ -7,6 +7,10 @@ Axes (binary32 persistent kernel; binary64 tunes the block only):
                   spill to scratch)
   lds_stack       16, 24 or 32 LDS traversal-stack entries per lane (16 and
                   32 exist for waves_per_simd 0 and 5)
+  step_batch      0 (a lane takes its walk step as soon as its query ends),
+                  1 (once at least as many lanes are ready as traverse) or 32
+                  (once 32 lanes are ready); built for lds_stack 24 with
+                  waves_per_simd 0 or 5
   grid_kappa      distance-grid threshold in cell sizes, 0.25 to 6 (mesh
                   geometries with the distance grid on)
   grid_corners    distance-grid samples at cell centres (0), corners (1,
@@ -43,12 +47,13 @@ import numpy as np
 
 from . import _capi
 
-DEFAULTS = {"block": 256, "waves_per_simd": 0, "lds_stack": 24, "grid_corners": 0, "grid_kappa": 4.0,
-            "grid_cells": 128}
+DEFAULTS = {"block": 256, "waves_per_simd": 0, "lds_stack": 24, "step_batch": 0, "grid_corners": 0,
+            "grid_kappa": 4.0, "grid_cells": 128}
 AXES = {
     "block": (64, 128, 256),
     "waves_per_simd": (0, 5, 6, 8),
     "lds_stack": (16, 24, 32),
+    "step_batch": (0, 1, 32),
     "grid_corners": (0, 1, 2),
     "grid_kappa": (0.25, 0.35, 0.5, 0.75, 1.0, 1.5, 2.0, 3.0, 4.0, 6.0),
     "grid_cells": (96, 128, 192, 256, 320, 384, 448, 512),
@@ -75,9 +80,12 @@ class Tuning:
 
 
 def _valid(cfg: dict, precision: int) -> bool:
+    batch = cfg.get("step_batch", 0)
     if precision == 64:
-        return cfg["waves_per_simd"] == 0 and cfg["lds_stack"] == 24
+        return cfg["waves_per_simd"] == 0 and cfg["lds_stack"] == 24 and batch == 0
     if cfg["lds_stack"] != 24 and cfg["waves_per_simd"] not in (0, 5):
+        return False
+    if batch and (cfg["lds_stack"] != 24 or cfg["waves_per_simd"] not in (0, 5)):
         return False
     return True
 
@@ -141,6 +149,7 @@ def autotune(ctx, geom: int, sites, sides, *, geom_sig: str, combine: int, preci
             r = ctx.estimate(geom, pts, sides, combine=combine, precision=precision, seed=seed, walks=walks,
                              eps=eps, max_steps=max_steps, distance_grid=distance_grid,
                              block=cfg["block"], waves_per_simd=cfg["waves_per_simd"], lds_stack=cfg["lds_stack"],
+                             step_batch=cfg.get("step_batch", 0),
                              grid_kappa=cfg["grid_kappa"] if distance_grid else 0.0, raise_on_cap=False)
             best = min(best, r.stats["walk_kernel_ms"] / max(1, r.stats["paths"]))
         measured[k] = best
