@@ -43,8 +43,11 @@ hipError_t launch_walk_simple_f32(int geom_kind, const Tables &tb, const WalkPar
 // batch: 0, or 24 / 32 / 40 lanes of a wave ready before it takes walk steps,
 // or 1: at least as many ready as traversing (built for the 24-entry stack
 // with the compiler's allocation or 5 waves).
-hipError_t launch_walk_persistent_f32(int geom_kind, bool count, int waves, int lds, int batch, const Tables &tb,
-                                      const WalkParams &wp, int block, int cus, hipStream_t stream, int *grid_out);
+// bare: no side of any request in the launch has a volume source (mesh
+// geometries take a kernel built without the source code).
+hipError_t launch_walk_persistent_f32(int geom_kind, bool count, int waves, int lds, int batch, bool bare,
+                                      const Tables &tb, const WalkParams &wp, int block, int cus, hipStream_t stream,
+                                      int *grid_out);
 hipError_t launch_slot_hints(const Tables &tb, int64_t first, int64_t n, hipStream_t stream);
 hipError_t launch_walk_simple_f64(int geom_kind, bool mt, const Tables &tb, const WalkParams &wp, int block,
                                   hipStream_t stream);

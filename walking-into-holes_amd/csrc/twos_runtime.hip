@@ -374,7 +374,15 @@ struct Engine {
 
   double *grad_of(const Request *r) const { return (r->prm.flags & TWOS_FLAG_GRADIENT) ? d_grad : nullptr; }
 
-  twos_status launch_walks(const Request *r, int cls, int64_t e0, int64_t total_walks, int kind) {
+  // No side of any request in the class has a volume source.
+  template <class E>
+  static bool class_bare(const std::vector<E> &entries) {
+    for (const E &en : entries)
+      if (en.r->sides[0].source_kind != 0 || en.r->sides[1].source_kind != 0) return false;
+    return true;
+  }
+
+  twos_status launch_walks(const Request *r, int cls, int64_t e0, int64_t total_walks, int kind, bool bare) {
     const twos_params &p = r->prm;
     Tables tb = tables(cls, e0);
     WalkParams wp{};
@@ -405,8 +413,8 @@ struct Engine {
       const int block = p.block > 0 ? p.block : 256;
       e = p.precision == 64
               ? launch_walk_persistent_f64(kind, count, tb, wp, block, cus, st, &grid)
-              : launch_walk_persistent_f32(kind, count, p.waves_per_simd, p.lds_stack, p.step_batch, tb, wp, block,
-                                           cus, st, &grid);
+              : launch_walk_persistent_f32(kind, count, p.waves_per_simd, p.lds_stack, p.step_batch, bare, tb, wp,
+                                           block, cus, st, &grid);
       stats.grid = grid;
       stats.block = block;
       stats.waves_per_simd = p.waves_per_simd;
@@ -556,7 +564,8 @@ struct Engine {
       const twos_params &q = r0->prm;
       const int64_t key = int64_t(q.precision) | int64_t(q.combine) << 8 | int64_t(kind0) << 12 |
                           int64_t(q.waves_per_simd) << 16 | int64_t(q.block) << 24 | int64_t(q.lds_stack) << 36 |
-                          int64_t(q.flags & 7) << 44 | int64_t(q.step_batch) << 48;
+                          int64_t(q.flags & 7) << 44 | int64_t(q.step_batch) << 48 |
+                          int64_t(class_bare(by_class[0])) << 56;
       if (graph_key != key) {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -564,7 +573,7 @@ struct Engine {
         graph = nullptr;
         HIP_OK(hipStreamBeginCapture(strm, hipStreamCaptureModeThreadLocal));
         hipError_t ce = hipMemcpyAsync(d_in, h_in, in_bytes, hipMemcpyHostToDevice, strm);
-        twos_status ls = ce == hipSuccess ? launch_walks(r0, 0, 0, 0, kind0) : TWOS_ERR_DEVICE;
+        twos_status ls = ce == hipSuccess ? launch_walks(r0, 0, 0, 0, kind0, class_bare(by_class[0])) : TWOS_ERR_DEVICE;
         Tables tb = tables(0, 0);
         WalkParams wp{};
         wp.val = d_val;
@@ -604,7 +613,7 @@ struct Engine {
           if (geom_kind_of(en.r->geom) != kind) kind = kGeomGeneric;
         if (rc->prm.flags & TWOS_FLAG_COUNT_TRAVERSAL) HIP_OK(launch_zero(d_counters + kMaxClasses, 3, strm));
         HIP_OK(hipEventRecord(ev[0], strm));
-        st = launch_walks(rc, int(c), e0s[c], totals[c], kind);
+        st = launch_walks(rc, int(c), e0s[c], totals[c], kind, class_bare(by_class[c]));
         if (st != TWOS_OK) return st;
         HIP_OK(hipEventRecord(ev[1], strm));
         if (dump_a) {

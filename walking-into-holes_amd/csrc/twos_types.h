@@ -13,8 +13,9 @@ enum { kSquare = TWOS_SQUARE, kPaired = TWOS_PAIRED, kCross = TWOS_CROSS, kCoupl
 enum { kRngPhilox = TWOS_RNG_PHILOX, kRngMtSite = TWOS_RNG_MT_SITE, kRngMtPaired = TWOS_RNG_MT_PAIRED,
        kRngMtKey = TWOS_RNG_MT_KEY, kRngMtFd = TWOS_RNG_MT_FD };
 // Per-path extras a walk kernel is built with: none, the first-step gradient
-// samples, or the coupled finite-cavity pair of paths.
-enum { kChanNone = 0, kChanGrad = 1, kChanCoupled = 2 };
+// samples, the coupled finite-cavity pair of paths, or none and no volume
+// source on either side (the source code is left out of the kernel).
+enum { kChanNone = 0, kChanGrad = 1, kChanCoupled = 2, kChanBare = 3 };
 // Kernel specialisations by geometry: any container (runtime switch), mesh
 // container only (no analytic holes), analytic container only.
 enum { kGeomGeneric = 0, kGeomMesh = 1, kGeomAnalytic = 2 };

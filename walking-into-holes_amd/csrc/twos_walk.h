@@ -68,7 +68,7 @@ struct Job {
 template <typename R>
 __device__ __forceinline__ void job_init(Job<R> &j, const Tables &tb, const WalkParams &wp, int64_t u) {
   const int32_t n_active = *tb.round.n_active;
-  const int64_t t = u / wp.jobs_per_walk;
+  const int64_t t = wp.jobs_per_walk == 2 ? (u >> 1) : u;  // 1 or 2 jobs per walk
   j.side = int32_t(u - t * wp.jobs_per_walk);
   const int32_t k = find_site(tb.round.act_prefix, n_active, t);
   const int32_t slot = tb.round.act_slot[k];
@@ -409,7 +409,8 @@ __device__ __forceinline__ int job_advance(Job<R> &j, const WalkParams &wp, R r,
     if (wp.combine == kPaired) j.acc1 += side_payoff<R>(*j.s1, bp, cls);
     return 1;
   }
-  job_source(j, wp, r, job_sampled(j, wp) ? j.x + philox_green<R>(j.key, j.walk, j.steps, r) : j.x);
+  if constexpr (kChan != kChanBare)
+    job_source(j, wp, r, job_sampled(j, wp) ? j.x + philox_green<R>(j.key, j.walk, j.steps, r) : j.x);
   const V3<R> d = philox_dir(j);
   job_first_step<R, kChan>(j, d, r);
   j.x = j.x + d * r;

@@ -39,11 +39,12 @@ hipError_t launch_walk_simple_f32(int geom_kind, const Tables &tb, const WalkPar
 
 namespace {
 template <int kWaves, int kLds, int kBatch = 0>
-hipError_t persistent_f32(int geom_kind, bool count, const Tables &tb, const WalkParams &wp, int block,
+hipError_t persistent_f32(int geom_kind, bool count, bool bare, const Tables &tb, const WalkParams &wp, int block,
                           int cus, hipStream_t stream, int *grid_out) {
   void (*k)(Tables, WalkParams);
   int lds = kLds;
   if (count) k = walk_persistent<float, kGeomGeneric, true, kWaves, kLds>;
+  else if (geom_kind == kGeomMesh && bare) k = walk_persistent<float, kGeomMesh, false, kWaves, kLds, kChanBare, kBatch>;
   else if (geom_kind == kGeomMesh) k = walk_persistent<float, kGeomMesh, false, kWaves, kLds, kChanNone, kBatch>;
   else if (geom_kind == kGeomAnalytic) {
     // no traversal stack in use
@@ -66,8 +67,9 @@ hipError_t persistent_f32(int geom_kind, bool count, const Tables &tb, const Wal
 // default 24-entry stack).  lds: per-lane LDS stack entries (16, 24 or 32;
 // 16 and 32 are built for the compiler's allocation and 5 waves).  The grid
 // is one resident wave set: blocks-per-CU from the occupancy query times CUs.
-hipError_t launch_walk_persistent_f32(int geom_kind, bool count, int waves, int lds, int batch, const Tables &tb,
-                                      const WalkParams &wp, int block, int cus, hipStream_t stream, int *grid_out) {
+hipError_t launch_walk_persistent_f32(int geom_kind, bool count, int waves, int lds, int batch, bool bare,
+                                      const Tables &tb, const WalkParams &wp, int block, int cus, hipStream_t stream,
+                                      int *grid_out) {
   if (wp.grad || wp.combine == kCoupled) {
     // Gradient channel or coupled finite-cavity pairs: the compiler's
     // allocation, 24-entry stack, any geometry.
@@ -83,27 +85,27 @@ hipError_t launch_walk_persistent_f32(int geom_kind, bool count, int waves, int 
   }
   if (batch && !count && (lds == 24 || lds == 0)) {
     const bool w5 = waves == 5;
-    if (batch == 1) return w5 ? persistent_f32<5, 24, 1>(geom_kind, count, tb, wp, block, cus, stream, grid_out)
-                              : persistent_f32<1, 24, 1>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
-    if (batch == 24) return w5 ? persistent_f32<5, 24, 24>(geom_kind, count, tb, wp, block, cus, stream, grid_out)
-                               : persistent_f32<1, 24, 24>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
-    if (batch == 32) return w5 ? persistent_f32<5, 24, 32>(geom_kind, count, tb, wp, block, cus, stream, grid_out)
-                               : persistent_f32<1, 24, 32>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
-    if (batch == 40) return w5 ? persistent_f32<5, 24, 40>(geom_kind, count, tb, wp, block, cus, stream, grid_out)
-                               : persistent_f32<1, 24, 40>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
+    if (batch == 1) return w5 ? persistent_f32<5, 24, 1>(geom_kind, count, bare, tb, wp, block, cus, stream, grid_out)
+                              : persistent_f32<1, 24, 1>(geom_kind, count, bare, tb, wp, block, cus, stream, grid_out);
+    if (batch == 24) return w5 ? persistent_f32<5, 24, 24>(geom_kind, count, bare, tb, wp, block, cus, stream, grid_out)
+                               : persistent_f32<1, 24, 24>(geom_kind, count, bare, tb, wp, block, cus, stream, grid_out);
+    if (batch == 32) return w5 ? persistent_f32<5, 24, 32>(geom_kind, count, bare, tb, wp, block, cus, stream, grid_out)
+                               : persistent_f32<1, 24, 32>(geom_kind, count, bare, tb, wp, block, cus, stream, grid_out);
+    if (batch == 40) return w5 ? persistent_f32<5, 24, 40>(geom_kind, count, bare, tb, wp, block, cus, stream, grid_out)
+                               : persistent_f32<1, 24, 40>(geom_kind, count, bare, tb, wp, block, cus, stream, grid_out);
   }
   if (lds == 16) {
-    if (waves == 5) return persistent_f32<5, 16>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
-    return persistent_f32<1, 16>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
+    if (waves == 5) return persistent_f32<5, 16>(geom_kind, count, bare, tb, wp, block, cus, stream, grid_out);
+    return persistent_f32<1, 16>(geom_kind, count, bare, tb, wp, block, cus, stream, grid_out);
   }
   if (lds == 32) {
-    if (waves == 5) return persistent_f32<5, 32>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
-    return persistent_f32<1, 32>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
+    if (waves == 5) return persistent_f32<5, 32>(geom_kind, count, bare, tb, wp, block, cus, stream, grid_out);
+    return persistent_f32<1, 32>(geom_kind, count, bare, tb, wp, block, cus, stream, grid_out);
   }
-  if (waves == 8) return persistent_f32<8, 24>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
-  if (waves == 6) return persistent_f32<6, 24>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
-  if (waves == 5) return persistent_f32<5, 24>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
-  return persistent_f32<1, 24>(geom_kind, count, tb, wp, block, cus, stream, grid_out);
+  if (waves == 8) return persistent_f32<8, 24>(geom_kind, count, bare, tb, wp, block, cus, stream, grid_out);
+  if (waves == 6) return persistent_f32<6, 24>(geom_kind, count, bare, tb, wp, block, cus, stream, grid_out);
+  if (waves == 5) return persistent_f32<5, 24>(geom_kind, count, bare, tb, wp, block, cus, stream, grid_out);
+  return persistent_f32<1, 24>(geom_kind, count, bare, tb, wp, block, cus, stream, grid_out);
 }
 
 hipError_t launch_slot_hints(const Tables &tb, int64_t first, int64_t n, hipStream_t stream) {
