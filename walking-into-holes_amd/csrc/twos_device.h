@@ -68,6 +68,13 @@ constexpr double kTwoPi = 6.283185307179586;  // 2 * acos(-1.0) in binary64
 // pairs (0,1) and (2,3).  binary32: the top 24 bits of the same two numbers,
 // so a binary32 walk and its binary64 rerun follow the same directions to
 // within 2^-24.  z = 1 - 2 u1, phi = 2 pi u2 (wih_twos.cpp:536-542).
+// Square root of the walk arithmetic: binary64 exact; binary32 the hardware
+// v_sqrt_f32 (within 1 ulp) instead of the correctly rounded expansion, which
+// the binary32 walk tolerates (a radius off by an ulp; the distance grid's
+// 2e-6 relative margin covers its bound).
+__device__ __forceinline__ double wsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float wsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 __device__ __forceinline__ V3<double> philox_direction(uint4 w, double) {
   const double u1 = double(((uint64_t(w.y) << 32) | w.x) >> 11) * 0x1.0p-53;
   const double u2 = double(((uint64_t(w.w) << 32) | w.z) >> 11) * 0x1.0p-53;
@@ -83,7 +90,7 @@ __device__ __forceinline__ V3<float> philox_direction(uint4 w, float) {
   const float u1 = float(w.y >> 8) * 0x1.0p-24f;
   const float u2 = float(w.w >> 8) * 0x1.0p-24f;
   const float z = 1.0f - 2.0f * u1;
-  const float radial = sqrtf(fmaxf(0.0f, (1.0f - z) * (1.0f + z)));
+  const float radial = wsqrt(fmaxf(0.0f, (1.0f - z) * (1.0f + z)));
   float s, c;
   sincospif(2.0f * u2, &s, &c);
   return {radial * c, radial * s, z};
@@ -436,7 +443,7 @@ __device__ __forceinline__ R container_analytic(const DevGeom &g, V3<R> p, V3<R>
     const V3<R> c{R(g.ball[0]), R(g.ball[1]), R(g.ball[2])};
     const R rad = R(g.ball[3]);
     const V3<R> d = p - c;
-    const R len = sqrt(dot(d, d));
+    const R len = wsqrt(dot(d, d));
     best = rad - len;
     point = len > R(0) ? c + d * (rad / len) : V3<R>{c.x + rad, c.y, c.z};
     cls = 0;
@@ -469,7 +476,7 @@ __device__ __forceinline__ R holes_and_cavities(const DevGeom &g, V3<R> p, R bes
       const Vec4<R> s = load4<R>(g, 0, i);
       const V3<R> c{s.x, s.y, s.z};
       const V3<R> d = p - c;
-      const R len = sqrt(dot(d, d));
+      const R len = wsqrt(dot(d, d));
       const R dist = len - s.w;
       if (dist < best && len > R(0)) {
         best = dist;
@@ -482,7 +489,7 @@ __device__ __forceinline__ R holes_and_cavities(const DevGeom &g, V3<R> p, R bes
       const int axis = int(s.x);
       const int ka = axis == 0 ? 1 : 0, kb = axis == 2 ? 1 : 2;
       const R da = comp(p, ka) - s.y, db = comp(p, kb) - s.z;
-      const R len = sqrt(da * da + db * db);
+      const R len = wsqrt(da * da + db * db);
       const R dist = len - s.w;
       if (dist < best && len > R(0)) {
         best = dist;
@@ -498,7 +505,7 @@ __device__ __forceinline__ R holes_and_cavities(const DevGeom &g, V3<R> p, R bes
     const Vec4<R> s = load4<R>(g, 2, i);
     const V3<R> c{s.x, s.y, s.z};
     const V3<R> d = p - c;
-    const R len = sqrt(dot(d, d));
+    const R len = wsqrt(dot(d, d));
     const R cr = R(g.cavity_radius);
     const R dist = len - cr;
     if (dist < best && len > R(0)) {
@@ -515,7 +522,7 @@ __device__ __forceinline__ R holes_and_cavities(const DevGeom &g, V3<R> p, R bes
 // of wih_twos.cpp:426).
 template <typename R>
 __device__ __forceinline__ R mesh_distance(const DevGeom &g, R d2) {
-  R best = sqrt(d2);
+  R best = wsqrt(d2);
   if (sizeof(R) == 8 && g.mesh_radius_f32) best = R(float(best));
   return best;
 }
@@ -549,7 +556,7 @@ __device__ __forceinline__ R grid_lower(const DevGeom &g, V3<R> p) {
                   R(g.grid_lo[2]) + (R(l) + R(0.5)) * h};
     const R dc = R(g.grid[(size_t(i) * g.grid_n[1] + k) * g.grid_n[2] + l]);
     const V3<R> d = p - c;
-    return dc - sqrt(dot(d, d)) - R(2e-6) * (R(1) + dc);
+    return dc - wsqrt(dot(d, d)) - R(2e-6) * (R(1) + dc);
   }
   const R x0 = p.x - (R(g.grid_lo[0]) + R(i) * h), y0 = p.y - (R(g.grid_lo[1]) + R(k) * h),
           z0 = p.z - (R(g.grid_lo[2]) + R(l) * h);
@@ -564,8 +571,8 @@ __device__ __forceinline__ R grid_lower(const DevGeom &g, V3<R> p) {
         const R dx = x0 - R(a) * h, dy = y0 - R(b) * h;
         const R dxy = dx * dx + dy * dy;
         const R d0 = R(f16_lo(w[2 * a + b])), d1 = R(f16_hi(w[2 * a + b]));
-        best = fmax(best, d0 - sqrt(dxy + z0 * z0) - R(2e-6) * (R(1) + d0));
-        best = fmax(best, d1 - sqrt(dxy + (z0 - h) * (z0 - h)) - R(2e-6) * (R(1) + d1));
+        best = fmax(best, d0 - wsqrt(dxy + z0 * z0) - R(2e-6) * (R(1) + d0));
+        best = fmax(best, d1 - wsqrt(dxy + (z0 - h) * (z0 - h)) - R(2e-6) * (R(1) + d1));
       }
     return best;
   }
@@ -583,7 +590,7 @@ __device__ __forceinline__ R grid_lower(const DevGeom &g, V3<R> p) {
       for (int c = 0; c < 2; ++c) {
         const R dc = R(q[c]);
         const R dz = z0 - R(c) * h;
-        best = fmax(best, dc - sqrt(dxy + dz * dz) - R(2e-6) * (R(1) + dc));
+        best = fmax(best, dc - wsqrt(dxy + dz * dz) - R(2e-6) * (R(1) + dc));
       }
     }
   return best;

@@ -242,7 +242,7 @@ template <typename R>
 __device__ __forceinline__ R coupled_cavity(const Job<R> &j, R best, V3<R> &point, int &cls) {
   if (j.phase == 0) return best;
   const V3<R> d = j.x - j.z;
-  const R len = sqrt(dot(d, d));
+  const R len = wsqrt(dot(d, d));
   const R dist = len - j.rc;
   if (dist < best && len > R(0)) {
     best = dist;
