@@ -196,6 +196,7 @@ def main() -> int:
         n_nodes = cnt.stats["nodes_visited"] / q_sub
         n_tris = cnt.stats["tris_tested"] / q_sub
         n_grid = cnt.stats["grid_lookups"] / q_sub
+        exact = 1.0 - cnt.stats["grid_hits"] / q_sub  # queries that traverse the BVH
         # Samples per grid lookup: one binary32 centre, eight binary32 corners,
         # or one 16-byte record of eight binary16 corners.
         grid_bytes = {0: GRID_BYTES, 1: 8 * GRID_BYTES, 2: 16}[out_shape.get("grid_corners", 0)]
@@ -207,6 +208,8 @@ def main() -> int:
                     "algorithmic_bytes_per_launch": round(queries * b_query / launches),
                     "bytes_per_query": round(b_query, 1), "nodes_per_query": round(n_nodes, 2),
                     "tris_per_query": round(n_tris, 2), "grid_lookups_per_query": round(n_grid, 3),
+                    "traversing_query_fraction": round(exact, 3),
+                    "nodes_per_traversing_query": round(n_nodes / max(exact, 1e-9), 2),
                     "walk_kernel_ms_per_launch": round(eager_ms / launches, 3),
                     "launches": launches}
     else:

@@ -231,6 +231,7 @@ typedef struct {
   int32_t waves_per_simd;
   int32_t grid;
   int64_t grid_lookups;     /* distance-grid lookups (when counting is on) */
+  int64_t grid_hits;        /* lookups whose bound was the step's radius, no traversal (counting) */
 } twos_stats;
 
 const char *twos_version(void);

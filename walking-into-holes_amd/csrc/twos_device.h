@@ -289,6 +289,7 @@ struct TraversalCount {
   uint32_t nodes;
   uint32_t tris;
   uint32_t grid;  // distance-grid lookups
+  uint32_t hits;  // lookups whose bound was taken as the radius (no traversal)
 };
 
 // 4-wide node, 128 bytes, eight float4: the four children's bounds as

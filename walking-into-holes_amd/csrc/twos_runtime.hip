@@ -196,7 +196,7 @@ struct Engine {
     HIP_OK(dalloc(&d_sides, 2 * size_t(cap_problems)));
     HIP_OK(dalloc(&d_val, 2 * size_t(cap_walks)));
     HIP_OK(dalloc(&d_steps, 2 * size_t(cap_walks)));
-    HIP_OK(dalloc(&d_counters, kMaxClasses + 3));
+    HIP_OK(dalloc(&d_counters, kMaxClasses + 4));
     auto align = [](size_t v) { return (v + 255) & ~size_t(255); };
     off_slot = 0;
     off_base = align(off_slot + 4 * C);
@@ -611,7 +611,7 @@ struct Engine {
         int kind = geom_kind_of(rc->geom);
         for (const Entry &en : by_class[c])
           if (geom_kind_of(en.r->geom) != kind) kind = kGeomGeneric;
-        if (rc->prm.flags & TWOS_FLAG_COUNT_TRAVERSAL) HIP_OK(launch_zero(d_counters + kMaxClasses, 3, strm));
+        if (rc->prm.flags & TWOS_FLAG_COUNT_TRAVERSAL) HIP_OK(launch_zero(d_counters + kMaxClasses, 4, strm));
         HIP_OK(hipEventRecord(ev[0], strm));
         st = launch_walks(rc, int(c), e0s[c], totals[c], kind, class_bare(by_class[c]));
         if (st != TWOS_OK) return st;
@@ -635,11 +635,12 @@ struct Engine {
         HIP_OK(hipEventElapsedTime(&ms, ev[1], ev[2]));
         stats.reduce_kernel_ms += ms;
         if (rc->prm.flags & TWOS_FLAG_COUNT_TRAVERSAL) {
-          unsigned long long cnt[3];
+          unsigned long long cnt[4];
           HIP_OK(hipMemcpy(cnt, d_counters + kMaxClasses, sizeof(cnt), hipMemcpyDeviceToHost));
           stats.nodes_visited += int64_t(cnt[0]);
           stats.tris_tested += int64_t(cnt[1]);
           stats.grid_lookups += int64_t(cnt[2]);
+          stats.grid_hits += int64_t(cnt[3]);
         }
       }
       HIP_OK(hipMemcpyAsync(h_out + off_compact, d_out + off_compact, sizeof(twos_site_result) * n_entries,

@@ -436,6 +436,7 @@ __device__ __forceinline__ bool query_start(Job<R> &j, MeshQuery<R> &q, Traversa
     const R lb = grid_lower<R>(*j.g, j.x);
     if (kCount) cnt->grid += 1;
     if (lb >= j.tau) {
+      if (kCount) cnt->hits += 1;
       q.grid = true;
       q.best = lb;
       return false;
@@ -466,7 +467,7 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWav
   bool active = false, in_query = false;
   int64_t chunk_cur = 0, chunk_end = 0;  // wave-uniform
   bool drained = false;                  // wave-uniform
-  TraversalCount cnt{0, 0, 0};
+  TraversalCount cnt{0, 0, 0, 0};
   while (true) {
     uint64_t need = __ballot(!active);
     while (need != 0 && !drained) {
@@ -540,6 +541,7 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWav
     atomicAdd(wp.counters + 0, (unsigned long long)cnt.nodes);
     atomicAdd(wp.counters + 1, (unsigned long long)cnt.tris);
     atomicAdd(wp.counters + 2, (unsigned long long)cnt.grid);
+    atomicAdd(wp.counters + 3, (unsigned long long)cnt.hits);
   }
 }
 
