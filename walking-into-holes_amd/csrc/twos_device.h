@@ -303,6 +303,21 @@ constexpr int kNode4Quads = 8;
 // the builder rejects trees whose 4-wide depth would overflow it.
 constexpr int kStackDepth = 48;
 
+// Start node of an exact query at p whose best squared distance so far is
+// ub2 (after the warm-start triangle): the cell's start node when the record
+// grid has them, p lies in the grid and ub2 is within their validity, else
+// the root.
+template <typename R>
+__device__ __forceinline__ int32_t grid_start_node(const DevGeom &g, V3<R> p, R ub2) {
+  if (!g.grid_start || !(ub2 <= R(g.grid_start_r2))) return 0;
+  const R inv_h = R(g.grid_inv_h);
+  const int i = int(floor((p.x - R(g.grid_lo[0])) * inv_h));
+  const int k = int(floor((p.y - R(g.grid_lo[1])) * inv_h));
+  const int l = int(floor((p.z - R(g.grid_lo[2])) * inv_h));
+  if (i < 0 || k < 0 || l < 0 || i >= g.grid_n[0] || k >= g.grid_n[1] || l >= g.grid_n[2]) return 0;
+  return g.grid_start[(size_t(i) * g.grid_n[1] + k) * g.grid_n[2] + l];
+}
+
 // Exact closest point on the mesh over the 4-wide BVH.  The query first tests
 // `hint` (the previous step's closest triangle: a walk moves by less than
 // twice its distance, so this bound is tight), then visits children
@@ -313,7 +328,7 @@ constexpr int kStackDepth = 48;
 // scan.  binary32: the same traversal in binary32.
 template <typename R, bool kCount>
 __device__ __forceinline__ R mesh_closest(const MeshView &m, V3<R> p, V3<R> &best_pt, int32_t &hint,
-                                          TraversalCount *cnt) {
+                                          TraversalCount *cnt, const DevGeom *starts = nullptr) {
   constexpr bool kExact = sizeof(R) == 8;
   R best = Inf<R>::v();
   int32_t best_id = 0x7fffffff;
@@ -354,7 +369,7 @@ __device__ __forceinline__ R mesh_closest(const MeshView &m, V3<R> p, V3<R> &bes
   int32_t stk_node[kStackDepth];
   R stk_d[kStackDepth];
   int sp = 0;
-  int32_t node = 0;
+  int32_t node = starts ? grid_start_node<R>(*starts, p, best) : 0;
   int32_t visits = m.n_tris + 8;  // 4-wide nodes < triangles; guards a corrupt tree
   while (--visits >= 0) {
     const float4 *q = m.nodes4 + size_t(kNode4Quads) * node;
@@ -612,7 +627,8 @@ __device__ __forceinline__ R closest_boundary(const DevGeom &g, V3<R> p, V3<R> &
       point = p;
     } else {
       MeshView m{g.nodes, g.nodes4, g.tris_f, g.tris_d, g.tri_ids, g.n_tris};
-      best = mesh_distance<R>(g, mesh_closest<R, kCount>(m, p, point, hint, cnt));
+      // With the grid on (tau finite) an exact query may start below the root.
+      best = mesh_distance<R>(g, mesh_closest<R, kCount>(m, p, point, hint, cnt, tau < Inf<R>::v() ? &g : nullptr));
     }
     cls = 0;
   } else {

@@ -204,7 +204,7 @@ __device__ __forceinline__ int delta(const uint32_t *codes, int32_t n, int32_t i
   return __clz(a ^ b);
 }
 
-__global__ void karras(const uint32_t *codes, int32_t n, float4 *nodes, int32_t *leaf_parent) {
+__global__ void karras(const uint32_t *codes, int32_t n, float4 *nodes, int32_t *leaf_parent, int2 *range) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   const int d = (delta(codes, n, i, i + 1) - delta(codes, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -223,6 +223,7 @@ __global__ void karras(const uint32_t *codes, int32_t n, float4 *nodes, int32_t 
   } while (step > 1);
   const int32_t gamma = i + s * d + min(d, 0);
   const int32_t lo = min(i, j), hi = max(i, j);
+  range[i] = make_int2(lo, hi);  // the node's sorted leaves
   const int32_t left = lo == gamma ? ~gamma : gamma;
   const int32_t right = hi == gamma + 1 ? ~(gamma + 1) : gamma + 1;
   float4 q3;
@@ -316,8 +317,8 @@ __global__ void even_depth(const float4 *nodes, int32_t n_internal, uint32_t *fl
   map[i] = flag[i];
 }
 
-__global__ void collapse4(const float4 *nodes, const uint32_t *flag, const uint32_t *map, int32_t n_internal,
-                          float4 *out) {
+__global__ void collapse4(const float4 *nodes, const uint32_t *flag, const uint32_t *map, const int2 *range,
+                          int32_t n_internal, float4 *out) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_internal || !flag[i]) return;
   float lo[3][4], hi[3][4];
@@ -350,7 +351,9 @@ __global__ void collapse4(const float4 *nodes, const uint32_t *flag, const uint3
   o[4] = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
   o[5] = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
   o[6] = make_float4(__int_as_float(ch[0]), __int_as_float(ch[1]), __int_as_float(ch[2]), __int_as_float(ch[3]));
-  o[7] = make_float4(0.f, 0.f, 0.f, 0.f);
+  // Row 7: the node's leaf range (first, last sorted triangle), for the
+  // start-node descent of record grids.
+  o[7] = make_float4(__int_as_float(range[i].x), __int_as_float(range[i].y), 0.f, 0.f);
 }
 
 __global__ void leaf_depth(const float4 *nodes, const int32_t *leaf_parent, int32_t n, int32_t *max_depth) {
@@ -495,6 +498,86 @@ __global__ void pack_corner_records(const float *node, int nx, int ny, int nz, u
   out[idx] = make_uint4(w[0], w[1], w[2], w[3]);
 }
 
+// Start node of exact queries per cell of a record grid.  The triangles
+// within `reach` of the cell centre c (a conservative float test: distances
+// enlarged by 0.1%) form a contiguous-or-not set of sorted leaves; with lo, hi
+// the smallest and largest, the start node is the deepest 4-wide node whose
+// leaf range holds [lo, hi] (ranges of a Morton-ordered tree nest).  A query
+// at p in the cell whose best distance so far is u <= reach - |p - c| only
+// needs triangles closer than u, which all lie within reach of c, so the
+// traversal may start there.  Cells whose record bounds the centre's
+// distance above reach, cells with no such triangle, and a range search that
+// overflows its stack keep the root (0).
+__global__ void cell_start_nodes(MeshView m, const uint4 *rec, float lx, float ly, float lz, float h, int nx,
+                                 int ny, int nz, float reach, int32_t *out) {
+  const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= int64_t(nx) * ny * nz) return;
+  const int l = int(idx % nz), k = int((idx / nz) % ny), i = int(idx / (int64_t(ny) * nz));
+  const uint4 r = rec[idx];
+  const uint32_t w[4] = {r.x, r.y, r.z, r.w};
+  float dmax = 0.f;
+  for (int a = 0; a < 4; ++a) dmax = fmaxf(dmax, fmaxf(f16_lo(w[a]), f16_hi(w[a])));
+  int32_t start = 0;
+  // Every corner is half a diagonal from the centre, and the corner values
+  // are rounded down: the centre is at least dmax - half_diag from the mesh.
+  const float half_diag = 0.8660254f * h, wide = reach * 1.001f;
+  if (m.n_tris > 1 && dmax - half_diag <= wide) {
+    const V3<float> c{lx + (float(i) + 0.5f) * h, ly + (float(k) + 0.5f) * h, lz + (float(l) + 0.5f) * h};
+    const float r2 = wide * wide;
+    int32_t lo = 0x7fffffff, hi = -1;
+    int32_t stack[kStackDepth];
+    int sp = 0;
+    int32_t node = 0;
+    bool ok = true;
+    int32_t visits = m.n_tris + 8;
+    while (ok && --visits >= 0) {
+      const float4 *q = m.nodes4 + size_t(kNode4Quads) * node;
+      const float4 bx0 = q[0], by0 = q[1], bz0 = q[2], bx1 = q[3], by1 = q[4], bz1 = q[5];
+      const int4 ch = *reinterpret_cast<const int4 *>(q + 6);
+      const float d[4] = {box_d2<float>(c, bx0.x, by0.x, bz0.x, bx1.x, by1.x, bz1.x),
+                          box_d2<float>(c, bx0.y, by0.y, bz0.y, bx1.y, by1.y, bz1.y),
+                          box_d2<float>(c, bx0.z, by0.z, bz0.z, bx1.z, by1.z, bz1.z),
+                          box_d2<float>(c, bx0.w, by0.w, bz0.w, bx1.w, by1.w, bz1.w)};
+      const int32_t cc[4] = {ch.x, ch.y, ch.z, ch.w};
+      for (int s = 0; s < 4; ++s) {
+        if (cc[s] == kEmptyChild || !(d[s] <= r2)) continue;
+        if (cc[s] < 0) {
+          V3<float> a, b, t;
+          load_tri<float>(m, ~cc[s], a, b, t);
+          const V3<float> e = c - closest_point_triangle(c, a, b, t);
+          if (dot(e, e) <= r2) {
+            lo = min(lo, ~cc[s]);
+            hi = max(hi, ~cc[s]);
+          }
+        } else if (sp < kStackDepth) {
+          stack[sp++] = cc[s];
+        } else {
+          ok = false;
+        }
+      }
+      if (sp == 0) break;
+      node = stack[--sp];
+    }
+    if (ok && visits >= 0 && lo <= hi) {
+      node = 0;
+      for (int depth = 0; depth < 64; ++depth) {
+        const int4 ch = *reinterpret_cast<const int4 *>(m.nodes4 + size_t(kNode4Quads) * node + 6);
+        const int32_t cc[4] = {ch.x, ch.y, ch.z, ch.w};
+        int32_t next = -1;
+        for (int s = 0; s < 4; ++s) {
+          if (cc[s] < 0) continue;
+          const float4 rg = m.nodes4[size_t(kNode4Quads) * cc[s] + 7];
+          if (__float_as_int(rg.x) <= lo && hi <= __float_as_int(rg.y)) next = cc[s];
+        }
+        if (next < 0) break;
+        node = next;
+      }
+      start = node;
+    }
+  }
+  out[idx] = start;
+}
+
 // Lower bound of the mesh distance from the geometry's distance grid (-inf
 // without a grid).
 template <typename R>
@@ -538,6 +621,7 @@ hipError_t lbvh_build(const double *d_vtx, const uint32_t *d_faces, int32_t n, M
   float *tri_box = nullptr;
   uint32_t *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr}, *hist = nullptr;
   int32_t *leaf_parent = nullptr, *d_depth = nullptr;
+  int2 *range = nullptr;
   unsigned int *flags = nullptr, *ticket = nullptr;
   const int red_blocks = 64;
   hipError_t err = hipSuccess;
@@ -545,6 +629,7 @@ hipError_t lbvh_build(const double *d_vtx, const uint32_t *d_faces, int32_t n, M
     (void)hipFree(centroid); (void)hipFree(bounds); (void)hipFree(partial); (void)hipFree(tri_box);
     (void)hipFree(keys[0]); (void)hipFree(keys[1]); (void)hipFree(vals[0]); (void)hipFree(vals[1]); (void)hipFree(hist);
     (void)hipFree(leaf_parent); (void)hipFree(d_depth); (void)hipFree(flags); (void)hipFree(ticket);
+    (void)hipFree(range);
   };
 #define TRY(x)                    \
   do {                            \
@@ -564,6 +649,7 @@ hipError_t lbvh_build(const double *d_vtx, const uint32_t *d_faces, int32_t n, M
   }
   TRY(hipMalloc(&hist, sizeof(uint32_t) * 256 * size_t(n_tiles)));
   TRY(hipMalloc(&leaf_parent, sizeof(int32_t) * size_t(n)));
+  TRY(hipMalloc(&range, sizeof(int2) * size_t(std::max(n - 1, 1))));
   TRY(hipMalloc(&d_depth, sizeof(int32_t)));
   TRY(hipMalloc(&flags, sizeof(unsigned int) * size_t(std::max(n - 1, 1))));
   TRY(hipMalloc(&ticket, sizeof(unsigned int)));
@@ -591,7 +677,7 @@ hipError_t lbvh_build(const double *d_vtx, const uint32_t *d_faces, int32_t n, M
   TRY(hipMemsetAsync(mb.nodes, 0, sizeof(float4) * 4 * size_t(std::max(n - 1, 1)), stream));
   if (n > 1) {
     hipLaunchKernelGGL(karras, dim3(blocks_for(n - 1, 256)), dim3(256), 0, stream, keys[cur], n, mb.nodes,
-                       leaf_parent);
+                       leaf_parent, range);
     hipLaunchKernelGGL(link_parents, dim3(blocks_for(n - 1, 256)), dim3(256), 0, stream, mb.nodes, n - 1);
     TRY(hipMemsetAsync(flags, 0, sizeof(unsigned int) * size_t(n - 1), stream));
     hipLaunchKernelGGL(refit, dim3(blocks_for(n, 256)), dim3(256), 0, stream, mb.nodes, leaf_parent, vals[cur],
@@ -600,8 +686,8 @@ hipError_t lbvh_build(const double *d_vtx, const uint32_t *d_faces, int32_t n, M
     uint32_t *parity = flags, *map = keys[cur ^ 1];
     hipLaunchKernelGGL(even_depth, dim3(blocks_for(n - 1, 256)), dim3(256), 0, stream, mb.nodes, n - 1, parity, map);
     hipLaunchKernelGGL(scan_exclusive, dim3(1), dim3(1024), 0, stream, map, n - 1);
-    hipLaunchKernelGGL(collapse4, dim3(blocks_for(n - 1, 256)), dim3(256), 0, stream, mb.nodes, parity, map, n - 1,
-                       mb.nodes4);
+    hipLaunchKernelGGL(collapse4, dim3(blocks_for(n - 1, 256)), dim3(256), 0, stream, mb.nodes, parity, map, range,
+                       n - 1, mb.nodes4);
   }
   hipLaunchKernelGGL(reorder_tris, dim3(blocks_for(n, 256)), dim3(256), 0, stream, d_vtx, d_faces, vals[cur], n,
                      mb.tris_f, mb.tris_d, mb.tri_ids);
@@ -664,6 +750,23 @@ hipError_t build_distance_grid(MeshBuffers &mb, const double lo[3], const double
   }
   for (int k = 0; k < 3; ++k) mb.grid_n[k] = n[k] - 1;
   mb.grid = reinterpret_cast<float *>(rec);
+  // Start nodes: reach three cells, so a query whose best distance is up to
+  // about 2.1 cells starts below the root.
+  int32_t *start = nullptr;
+  if (hipMalloc(&start, sizeof(int32_t) * size_t(n_cells)) == hipSuccess) {
+    const float reach = 3.0f * mb.grid_h;
+    hipLaunchKernelGGL(cell_start_nodes, dim3(blocks_for(n_cells, 128)), dim3(128), 0, stream, m, rec, mb.grid_lo[0],
+                       mb.grid_lo[1], mb.grid_lo[2], mb.grid_h, n[0] - 1, n[1] - 1, n[2] - 1, reach, start);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) {
+      (void)hipFree(start);
+      return e;
+    }
+    const float valid = 0.999f * (reach - 0.8660254f * mb.grid_h);
+    mb.grid_start = start;
+    mb.grid_start_r2 = valid * valid;
+  }
   return hipSuccess;
 }
 

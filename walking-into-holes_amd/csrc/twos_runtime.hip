@@ -901,6 +901,8 @@ twos_status build_geom(twos_ctx *ctx, GeomEntry &g) {
     v.grid_h = mb.grid_h;
     v.grid_inv_h = mb.grid_h > 0.f ? 1.0f / mb.grid_h : 0.f;
     v.grid_corners = mb.grid_corners;
+    v.grid_start = mb.grid_start;
+    v.grid_start_r2 = mb.grid_start_r2;
     v.n_tris = mb.n_tris;
     it->second.users += 1;
   }
@@ -1036,7 +1038,7 @@ void twos_destroy(twos_ctx *ctx) {
   for (auto &kv : ctx->geoms) free_geom(ctx, kv.second);
   for (auto &kv : ctx->meshes) {
     MeshBuffers &mb = kv.second.mb;
-    (void)hipFree(mb.nodes); (void)hipFree(mb.nodes4); (void)hipFree(mb.grid); (void)hipFree(mb.tris_f); (void)hipFree(mb.tris_d); (void)hipFree(mb.tri_ids);
+    (void)hipFree(mb.nodes); (void)hipFree(mb.nodes4); (void)hipFree(mb.grid); (void)hipFree(mb.grid_start); (void)hipFree(mb.tris_f); (void)hipFree(mb.tris_d); (void)hipFree(mb.tri_ids);
   }
   (void)hipFree(ctx->d_geoms);
   (void)hipStreamDestroy(ctx->stream);
@@ -1092,7 +1094,7 @@ twos_status twos_mesh_create(twos_ctx *ctx, const double *vertices, int64_t nv, 
   m.bytes = int64_t(nf > 1 ? nf - 1 : 1) * (64 + 128) + nf * (36 + 72 + 4);
   // The 4-wide traversal stacks at most three siblings per 4-wide level.
   if (3 * ((m.depth - 1 + 1) / 2) >= kStackDepth) {
-    (void)hipFree(m.mb.nodes); (void)hipFree(m.mb.nodes4); (void)hipFree(m.mb.grid); (void)hipFree(m.mb.tris_f); (void)hipFree(m.mb.tris_d); (void)hipFree(m.mb.tri_ids);
+    (void)hipFree(m.mb.nodes); (void)hipFree(m.mb.nodes4); (void)hipFree(m.mb.grid); (void)hipFree(m.mb.grid_start); (void)hipFree(m.mb.tris_f); (void)hipFree(m.mb.tris_d); (void)hipFree(m.mb.tri_ids);
     return fail(TWOS_ERR_GEOMETRY, "LBVH deeper than the traversal stack");
   }
   const int32_t id = ctx->next_mesh++;
@@ -1116,6 +1118,8 @@ twos_status twos_mesh_set_distance_grid(twos_ctx *ctx, int32_t mesh_id, int32_t 
   HIP_OK(hipStreamSynchronize(ctx->stream));
   MeshBuffers nb = m.mb;
   nb.grid = nullptr;
+  nb.grid_start = nullptr;
+  nb.grid_start_r2 = 0.f;
   hipEvent_t a, b;
   HIP_OK(hipEventCreate(&a));
   HIP_OK(hipEventCreate(&b));
@@ -1129,9 +1133,11 @@ twos_status twos_mesh_set_distance_grid(twos_ctx *ctx, int32_t mesh_id, int32_t 
   (void)hipEventDestroy(b);
   if (e != hipSuccess) {
     if (nb.grid) (void)hipFree(nb.grid);
+    if (nb.grid_start) (void)hipFree(nb.grid_start);
     return fail(TWOS_ERR_DEVICE, std::string("distance grid: ") + hipGetErrorString(e));
   }
   (void)hipFree(m.mb.grid);
+  if (m.mb.grid_start) (void)hipFree(m.mb.grid_start);
   m.mb = nb;
   m.grid_cells = cells;
   for (auto &kv : ctx->geoms) {
@@ -1145,6 +1151,8 @@ twos_status twos_mesh_set_distance_grid(twos_ctx *ctx, int32_t mesh_id, int32_t 
     g.dev.grid_h = nb.grid_h;
     g.dev.grid_inv_h = nb.grid_h > 0.f ? 1.0f / nb.grid_h : 0.f;
     g.dev.grid_corners = nb.grid_corners;
+    g.dev.grid_start = nb.grid_start;
+    g.dev.grid_start_r2 = nb.grid_start_r2;
   }
   if (ctx->engine) ctx->engine->graph_key = -1;
   if (build_ms) *build_ms = ms;
@@ -1168,7 +1176,7 @@ twos_status twos_mesh_release(twos_ctx *ctx, int32_t mesh_id) {
   if (it == ctx->meshes.end()) return fail(TWOS_ERR_NOT_FOUND, "unknown mesh id");
   if (it->second.users > 0) return fail(TWOS_ERR_INVALID_ARGUMENT, "mesh still used by a geometry");
   MeshBuffers &mb = it->second.mb;
-  (void)hipFree(mb.nodes); (void)hipFree(mb.nodes4); (void)hipFree(mb.grid); (void)hipFree(mb.tris_f); (void)hipFree(mb.tris_d); (void)hipFree(mb.tri_ids);
+  (void)hipFree(mb.nodes); (void)hipFree(mb.nodes4); (void)hipFree(mb.grid); (void)hipFree(mb.grid_start); (void)hipFree(mb.tris_f); (void)hipFree(mb.tris_d); (void)hipFree(mb.tri_ids);
   ctx->meshes.erase(it);
   return TWOS_OK;
 }

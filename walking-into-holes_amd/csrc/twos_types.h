@@ -45,6 +45,13 @@ struct DevGeom {
   float grid_h;
   int32_t grid_n[3];
   float grid_inv_h;
+  // Record grids: per cell, the 4-wide node to start an exact query at (the
+  // deepest whose leaf range holds every triangle within grid_start_reach of
+  // the cell centre), valid while the query's best distance so far is at most
+  // sqrt(grid_start_r2); null without them.
+  const int32_t *grid_start;
+  float grid_start_r2;
+  int32_t pad2;
 };
 
 // Per-site-slot state resident on the device (capacity = server max_sites).

@@ -442,7 +442,11 @@ __device__ __forceinline__ bool query_start(Job<R> &j, MeshQuery<R> &q, Traversa
       return false;
     }
   }
-  return !mq_begin<R, kCount>(j.mv, j.x, q, j.hint, cnt);
+  if (mq_begin<R, kCount>(j.mv, j.x, q, j.hint, cnt)) return false;
+  // With the grid on, the query may start below the root (after the
+  // warm-start triangle has bounded its distance).
+  if (j.tau < Inf<R>::v()) q.node = grid_start_node<R>(*j.g, j.x, q.best);
+  return true;
 }
 
 // kWaves is the occupancy the register allocator is asked to allow
