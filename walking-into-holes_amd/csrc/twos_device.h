@@ -328,9 +328,9 @@ __device__ __forceinline__ int32_t grid_start_node(const DevGeom &g, V3<R> p, R 
 // scan.  binary32: the same traversal in binary32.
 template <typename R, bool kCount>
 __device__ __forceinline__ R mesh_closest(const MeshView &m, V3<R> p, V3<R> &best_pt, int32_t &hint,
-                                          TraversalCount *cnt, const DevGeom *starts = nullptr) {
+                                          TraversalCount *cnt, R init2 = Inf<R>::v(), int32_t start = 0) {
   constexpr bool kExact = sizeof(R) == 8;
-  R best = Inf<R>::v();
+  R best = init2;
   int32_t best_id = 0x7fffffff;
   int32_t best_k = -1;
   auto test = [&](int32_t k) {
@@ -356,12 +356,13 @@ __device__ __forceinline__ R mesh_closest(const MeshView &m, V3<R> p, V3<R> &bes
   };
   // A non-finite point would pass every box test (fmax drops NaN): no query.
   if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z))) return best;
-  if (hint >= 0 && hint < m.n_tris) test(hint);
   if (m.n_tris == 1) {
-    if (best_k < 0) test(0);
+    best = Inf<R>::v();
+    test(0);
     hint = best_k;
     return best;
   }
+  if (hint >= 0 && hint < m.n_tris) test(hint);
   // Relative slack keeps binary64 rounding in a box bound from pruning a
   // triangle that ties the current best.
   const R slack = kExact ? R(1.0 - 1e-12) : R(1);
@@ -369,7 +370,7 @@ __device__ __forceinline__ R mesh_closest(const MeshView &m, V3<R> p, V3<R> &bes
   int32_t stk_node[kStackDepth];
   R stk_d[kStackDepth];
   int sp = 0;
-  int32_t node = starts ? grid_start_node<R>(*starts, p, best) : 0;
+  int32_t node = start;
   int32_t visits = m.n_tris + 8;  // 4-wide nodes < triangles; guards a corrupt tree
   while (--visits >= 0) {
     const float4 *q = m.nodes4 + size_t(kNode4Quads) * node;
@@ -415,7 +416,16 @@ __device__ __forceinline__ R mesh_closest(const MeshView &m, V3<R> p, V3<R> &bes
         break;
       }
     }
-    if (!found) break;
+    if (!found) {
+      if (best_k < 0 && best < Inf<R>::v()) {
+        // A search started from a bound found nothing below it (never
+        // expected: the bound is above the true distance); search the whole tree.
+        best = Inf<R>::v();
+        node = 0;
+        continue;
+      }
+      break;
+    }
   }
   hint = best_k;
   return best;
@@ -557,8 +567,16 @@ __device__ __forceinline__ R mesh_distance(const DevGeom &g, R d2) {
 __device__ __forceinline__ float f16_lo(uint32_t w) { return float(__builtin_bit_cast(_Float16, (unsigned short)(w & 0xffffu))); }
 __device__ __forceinline__ float f16_hi(uint32_t w) { return float(__builtin_bit_cast(_Float16, (unsigned short)(w >> 16))); }
 
+// Upper bound of the mesh distance at p from a sample value v (at most a
+// binary16 ulp below the distance at its point, itself within binary32
+// rounding) at distance s from p: the distance is 1-Lipschitz.
 template <typename R>
-__device__ __forceinline__ R grid_lower(const DevGeom &g, V3<R> p) {
+__device__ __forceinline__ R grid_upper(R v, R s) {
+  return (v * R(1.00390625) + s) * R(1.00001) + R(1e-6);
+}
+
+template <typename R>
+__device__ __forceinline__ R grid_lower(const DevGeom &g, V3<R> p, R *upper = nullptr) {
   const R h = R(g.grid_h), inv_h = R(g.grid_inv_h);
   const int last = g.grid_corners == 1 ? 2 : 1;  // highest first index of a cell (per sample count)
   int i = int(floor((p.x - R(g.grid_lo[0])) * inv_h));
@@ -572,14 +590,16 @@ __device__ __forceinline__ R grid_lower(const DevGeom &g, V3<R> p) {
                   R(g.grid_lo[2]) + (R(l) + R(0.5)) * h};
     const R dc = R(g.grid[(size_t(i) * g.grid_n[1] + k) * g.grid_n[2] + l]);
     const V3<R> d = p - c;
-    return dc - wsqrt(dot(d, d)) - R(2e-6) * (R(1) + dc);
+    const R dist = wsqrt(dot(d, d));
+    if (upper) *upper = grid_upper(dc, dist);
+    return dc - dist - R(2e-6) * (R(1) + dc);
   }
   const R x0 = p.x - (R(g.grid_lo[0]) + R(i) * h), y0 = p.y - (R(g.grid_lo[1]) + R(k) * h),
           z0 = p.z - (R(g.grid_lo[2]) + R(l) * h);
   if (g.grid_corners == 2) {
     const uint4 rec = reinterpret_cast<const uint4 *>(g.grid)[(size_t(i) * g.grid_n[1] + k) * g.grid_n[2] + l];
     const uint32_t w[4] = {rec.x, rec.y, rec.z, rec.w};
-    R best = -Inf<R>::v();
+    R best = -Inf<R>::v(), up = Inf<R>::v();
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -587,14 +607,17 @@ __device__ __forceinline__ R grid_lower(const DevGeom &g, V3<R> p) {
         const R dx = x0 - R(a) * h, dy = y0 - R(b) * h;
         const R dxy = dx * dx + dy * dy;
         const R d0 = R(f16_lo(w[2 * a + b])), d1 = R(f16_hi(w[2 * a + b]));
-        best = fmax(best, d0 - wsqrt(dxy + z0 * z0) - R(2e-6) * (R(1) + d0));
-        best = fmax(best, d1 - wsqrt(dxy + (z0 - h) * (z0 - h)) - R(2e-6) * (R(1) + d1));
+        const R s0 = wsqrt(dxy + z0 * z0), s1 = wsqrt(dxy + (z0 - h) * (z0 - h));
+        best = fmax(best, d0 - s0 - R(2e-6) * (R(1) + d0));
+        best = fmax(best, d1 - s1 - R(2e-6) * (R(1) + d1));
+        up = fmin(up, fmin(grid_upper(d0, s0), grid_upper(d1, s1)));
       }
+    if (upper) *upper = up;
     return best;
   }
   const float *row = g.grid + (size_t(i) * g.grid_n[1] + k) * g.grid_n[2] + l;
   const size_t si = size_t(g.grid_n[1]) * g.grid_n[2], sk = size_t(g.grid_n[2]);
-  R best = -Inf<R>::v();
+  R best = -Inf<R>::v(), up = Inf<R>::v();
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -606,9 +629,12 @@ __device__ __forceinline__ R grid_lower(const DevGeom &g, V3<R> p) {
       for (int c = 0; c < 2; ++c) {
         const R dc = R(q[c]);
         const R dz = z0 - R(c) * h;
-        best = fmax(best, dc - wsqrt(dxy + dz * dz) - R(2e-6) * (R(1) + dc));
+        const R s = wsqrt(dxy + dz * dz);
+        best = fmax(best, dc - s - R(2e-6) * (R(1) + dc));
+        up = fmin(up, grid_upper(dc, s));
       }
     }
+  if (upper) *upper = up;
   return best;
 }
 
@@ -621,14 +647,18 @@ __device__ __forceinline__ R closest_boundary(const DevGeom &g, V3<R> p, V3<R> &
                                               TraversalCount *cnt, R tau = Inf<R>::v()) {
   R best;
   if (kKind == kGeomMesh || (kKind == kGeomGeneric && g.container == kMesh)) {
-    const R lb = tau < Inf<R>::v() ? grid_lower<R>(g, p) : R(-1);
+    R ub = Inf<R>::v();
+    const R lb = tau < Inf<R>::v() ? grid_lower<R>(g, p, &ub) : R(-1);
     if (lb >= tau) {
       best = lb;
       point = p;
     } else {
       MeshView m{g.nodes, g.nodes4, g.tris_f, g.tris_d, g.tri_ids, g.n_tris};
-      // With the grid on (tau finite) an exact query may start below the root.
-      best = mesh_distance<R>(g, mesh_closest<R, kCount>(m, p, point, hint, cnt, tau < Inf<R>::v() ? &g : nullptr));
+      // With the grid on (tau finite) an exact query searches below the
+      // grid's upper bound and may start below the root.
+      const R init2 = ub < Inf<R>::v() ? ub * ub : ub;
+      best = mesh_distance<R>(g, mesh_closest<R, kCount>(m, p, point, hint, cnt, init2,
+                                                         grid_start_node<R>(g, p, init2)));
     }
     cls = 0;
   } else {
@@ -650,12 +680,13 @@ __device__ __forceinline__ R closest_boundary(const DevGeom &g, V3<R> p, V3<R> &
 
 template <typename R>
 struct MeshQuery {
-  R best;        // squared distance so far, or the grid radius when `grid`
+  R best;        // squared distance so far (at first the bound it started from), or the grid radius when `grid`
   V3<R> pt;
   int32_t best_id;
   int32_t best_k;
   int32_t node;
   int32_t sp;
+  int32_t pending;  // warm-start triangle still to test (first, in the first visit), or -1
   bool grid;
 };
 
@@ -705,19 +736,25 @@ __device__ __forceinline__ bool mq_within(const MeshQuery<R> &q, R d) {
   else return d < q.best;
 }
 
-// Starts a query; returns true when it is already complete.
+// Starts a query; returns true when it is already complete.  init2: a
+// squared distance above the true one (or infinity) the search starts from;
+// start: the node to start at (the root, or a grid cell's start node valid
+// for that bound).  The warm-start triangle is tested first, in the first
+// visit, where it shares a pass with other lanes' leaf tests.
 template <typename R, bool kCount>
 __device__ __forceinline__ bool mq_begin(const MeshView &m, V3<R> p, MeshQuery<R> &q, int32_t hint,
-                                         TraversalCount *cnt) {
-  q.best = Inf<R>::v();
+                                         TraversalCount *cnt, R init2 = Inf<R>::v(), int32_t start = 0) {
+  q.best = init2;
   q.best_id = 0x7fffffff;
   q.best_k = -1;
-  q.node = 0;
+  q.node = start;
   q.sp = 0;
   q.grid = false;
-  if (hint >= 0 && hint < m.n_tris) mq_test<R, kCount>(m, p, q, hint, cnt);
+  q.pending = hint >= 0 && hint < m.n_tris ? hint : -1;
   if (m.n_tris == 1) {
-    if (q.best_k < 0) mq_test<R, kCount>(m, p, q, 0, cnt);
+    q.best = Inf<R>::v();
+    q.pending = -1;
+    mq_test<R, kCount>(m, p, q, 0, cnt);
     return true;
   }
   return false;
@@ -734,21 +771,31 @@ __device__ __forceinline__ bool mq_visit(const MeshView &m, V3<R> p, MeshQuery<R
   R d[4] = {box_d2<R>(p, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x), box_d2<R>(p, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y),
             box_d2<R>(p, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z), box_d2<R>(p, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w)};
   int32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
-  // Leaf children (one triangle each) in slot order, one test per pass: the
-  // wave runs as many passes as its lanes have leaves to test, not one per
-  // slot that any lane uses.  A leaf whose box no longer beats the best
-  // distance is skipped; that skip never changes the result (its triangle is
-  // at least as far as its box).
+  // Triangle tests, one per pass: the pending warm-start triangle first,
+  // then leaf children (one triangle each) in slot order; the wave runs as
+  // many passes as its lanes have triangles to test, not one per slot that
+  // any lane uses.  A leaf whose box no longer beats the best distance is
+  // skipped; that skip never changes the result (its triangle is at least as
+  // far as its box).
   uint32_t leaves = 0;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
     if (c[i] < 0 && c[i] != kEmptyChild && mq_within(q, d[i])) leaves |= 1u << i;
-  while (leaves) {
-    const int i = __builtin_ctz(leaves);
-    leaves &= leaves - 1;
-    const int32_t ci = i == 0 ? c[0] : (i == 1 ? c[1] : (i == 2 ? c[2] : c[3]));
-    const R di = i == 0 ? d[0] : (i == 1 ? d[1] : (i == 2 ? d[2] : d[3]));
-    if (mq_within(q, di)) mq_test<R, kCount>(m, p, q, ~ci, cnt);
+  int32_t pend = q.pending;
+  q.pending = -1;
+  while (pend >= 0 || leaves) {
+    int32_t tri;
+    bool go = true;
+    if (pend >= 0) {
+      tri = pend;
+      pend = -1;
+    } else {
+      const int i = __builtin_ctz(leaves);
+      leaves &= leaves - 1;
+      tri = ~(i == 0 ? c[0] : (i == 1 ? c[1] : (i == 2 ? c[2] : c[3])));
+      go = mq_within(q, i == 0 ? d[0] : (i == 1 ? d[1] : (i == 2 ? d[2] : d[3])));
+    }
+    if (go) mq_test<R, kCount>(m, p, q, tri, cnt);
   }
   R key[4];
 #pragma unroll
@@ -781,6 +828,13 @@ __device__ __forceinline__ bool mq_visit(const MeshView &m, V3<R> p, MeshQuery<R
       q.node = stack_node(e);
       return false;
     }
+  }
+  if (q.best_k < 0 && q.best < Inf<R>::v()) {
+    // A search started from a bound found nothing below it (never expected:
+    // the bound is above the true distance); search the whole tree.
+    q.best = Inf<R>::v();
+    q.node = 0;
+    return false;
   }
   return true;
 }

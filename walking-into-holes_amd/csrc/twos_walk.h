@@ -432,8 +432,11 @@ __device__ __forceinline__ bool query_start(Job<R> &j, MeshQuery<R> &q, Traversa
     q.best = Inf<R>::v();  // no query: the path ends capped
     return false;
   }
+  R init2 = Inf<R>::v();
+  int32_t start = 0;
   if (j.tau < Inf<R>::v()) {
-    const R lb = grid_lower<R>(*j.g, j.x);
+    R ub;
+    const R lb = grid_lower<R>(*j.g, j.x, &ub);
     if (kCount) cnt->grid += 1;
     if (lb >= j.tau) {
       if (kCount) cnt->hits += 1;
@@ -441,12 +444,12 @@ __device__ __forceinline__ bool query_start(Job<R> &j, MeshQuery<R> &q, Traversa
       q.best = lb;
       return false;
     }
+    // The exact query searches below the grid's upper bound and may start
+    // below the root.
+    init2 = ub < Inf<R>::v() ? ub * ub : ub;
+    start = grid_start_node<R>(*j.g, j.x, init2);
   }
-  if (mq_begin<R, kCount>(j.mv, j.x, q, j.hint, cnt)) return false;
-  // With the grid on, the query may start below the root (after the
-  // warm-start triangle has bounded its distance).
-  if (j.tau < Inf<R>::v()) q.node = grid_start_node<R>(*j.g, j.x, q.best);
-  return true;
+  return !mq_begin<R, kCount>(j.mv, j.x, q, j.hint, cnt, init2, start);
 }
 
 // kWaves is the occupancy the register allocator is asked to allow
