@@ -23,9 +23,11 @@ struct Tables {
   RoundArrays round;
 };
 
-__device__ __forceinline__ int32_t find_site(const int64_t *prefix, int32_t n, int64_t t) {
-  // Largest k with prefix[k] <= t (prefix[0] = 0, prefix[n] = total).
-  int32_t lo = 0, hi = n;
+__device__ __forceinline__ int32_t find_site(const int64_t *prefix, int32_t n, int64_t t, int32_t lo = 0) {
+  // Largest k with prefix[k] <= t (prefix[0] = 0, prefix[n] = total), given
+  // prefix[lo] <= t: most calls land in site lo itself.
+  if (prefix[lo + 1] > t) return lo;
+  int32_t hi = n;
   while (hi - lo > 1) {
     const int32_t mid = (lo + hi) >> 1;
     if (prefix[mid] <= t) lo = mid; else hi = mid;
@@ -65,12 +67,15 @@ struct Job {
   int32_t st0, phase, hint_start;
 };
 
+// site_lo: a site whose first walk is at or before this job's walk (0, or
+// the site of the wave's ticket chunk start).
 template <typename R>
-__device__ __forceinline__ void job_init(Job<R> &j, const Tables &tb, const WalkParams &wp, int64_t u) {
+__device__ __forceinline__ void job_init(Job<R> &j, const Tables &tb, const WalkParams &wp, int64_t u,
+                                         int32_t site_lo = 0) {
   const int32_t n_active = *tb.round.n_active;
   const int64_t t = wp.jobs_per_walk == 2 ? (u >> 1) : u;  // 1 or 2 jobs per walk
   j.side = int32_t(u - t * wp.jobs_per_walk);
-  const int32_t k = find_site(tb.round.act_prefix, n_active, t);
+  const int32_t k = find_site(tb.round.act_prefix, n_active, t, site_lo);
   const int32_t slot = tb.round.act_slot[k];
   j.slot = slot;
   j.t = t;
@@ -473,6 +478,7 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWav
   MeshQuery<R> q;
   bool active = false, in_query = false;
   int64_t chunk_cur = 0, chunk_end = 0;  // wave-uniform
+  int32_t chunk_site = 0;                // wave-uniform: the site of the chunk's first ticket
   bool drained = false;                  // wave-uniform
   TraversalCount cnt{0, 0, 0, 0};
   while (true) {
@@ -488,12 +494,14 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWav
         }
         chunk_cur = int64_t(base);
         chunk_end = min(int64_t(base) + kChunk, total);
+        chunk_site = find_site(tb.round.act_prefix, *tb.round.n_active,
+                               wp.jobs_per_walk == 2 ? (chunk_cur >> 1) : chunk_cur);
       }
       const int64_t avail = chunk_end - chunk_cur;
       const int take = int(min<int64_t>(avail, __popcll(need)));
       const int rank = __popcll(need & ((1ull << lane) - 1ull));
       if (!active && rank < take) {
-        job_init(j, tb, wp, chunk_cur + rank);
+        job_init(j, tb, wp, chunk_cur + rank, chunk_site);
         if constexpr (kChan == kChanCoupled) coupled_start_philox(j);
         active = true;
         in_query = query_start<R, GK, kCount>(j, q, &cnt);
