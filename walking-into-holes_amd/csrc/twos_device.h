@@ -234,6 +234,44 @@ __device__ __forceinline__ V3<R> closest_point_triangle(V3<R> p, V3<R> a, V3<R> 
   return a + ab * v + ac * w;
 }
 
+// binary32: the same Voronoi-region decisions in the same order, without
+// branches: every lane computes the six dot products once, picks its region
+// by selects, takes one reciprocal for the region's ratio and forms
+// a + ab v + ac w.  (Ericson's early returns cost a wave every region any of
+// its lanes lands in.)  The point can differ from the branchy form by an ulp
+// of rounding (b as a + ab, for instance); binary64 keeps the branchy form.
+template <>
+__device__ __forceinline__ V3<float> closest_point_triangle<float>(V3<float> p, V3<float> a, V3<float> b,
+                                                                    V3<float> c) {
+  const V3<float> ab = b - a, ac = c - a, ap = p - a, bp = p - b, cp = p - c;
+  const float d1 = dot(ab, ap), d2 = dot(ac, ap), d3 = dot(ab, bp), d4 = dot(ac, bp), d5 = dot(ab, cp),
+              d6 = dot(ac, cp);
+  const float vc = d1 * d4 - d3 * d2, vb = d5 * d2 - d1 * d6, va = d3 * d6 - d5 * d4;
+  const float e43 = d4 - d3, e56 = d5 - d6;
+  const bool rA = d1 <= 0.f && d2 <= 0.f;
+  const bool rB = !rA && d3 >= 0.f && d4 <= d3;
+  const bool rAB = !rA && !rB && vc <= 0.f && d1 >= 0.f && d3 <= 0.f;
+  const bool done3 = rA || rB || rAB;
+  const bool rC = !done3 && d6 >= 0.f && d5 <= d6;
+  const bool rAC = !done3 && !rC && vb <= 0.f && d2 >= 0.f && d6 <= 0.f;
+  const bool rBC = !done3 && !rC && !rAC && va <= 0.f && e43 >= 0.f && e56 >= 0.f;
+  const bool inner = !done3 && !rC && !rAC && !rBC;
+  // The region's ratio num / den (vertex regions take none).
+  const float num = rAB ? d1 : (rAC ? d2 : (rBC ? e43 : 1.f));
+  const float den = rAB ? d1 - d3 : (rAC ? d2 - d6 : (rBC ? e43 + e56 : (inner ? va + vb + vc : 1.f)));
+  const float t = num * __builtin_amdgcn_rcpf(den);
+  float v = 0.f, w = 0.f;
+  v = rB ? 1.f : v;
+  v = rAB ? t : v;
+  w = rC ? 1.f : w;
+  w = rAC ? t : w;
+  v = rBC ? 1.f - t : v;
+  w = rBC ? t : w;
+  v = inner ? vb * t : v;
+  w = inner ? vc * t : w;
+  return a + ab * v + ac * w;
+}
+
 // ---------------------------------------------------------------------------
 // LBVH closest-point traversal.
 //

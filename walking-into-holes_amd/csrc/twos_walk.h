@@ -476,7 +476,9 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWav
   const int64_t total = *tb.round.total_jobs;
   Job<R> j;
   MeshQuery<R> q;
-  bool active = false, in_query = false;
+  // fresh: a new path whose first query has not started; it starts with the
+  // next walk steps, so the query-start code runs in one place.
+  bool active = false, in_query = false, fresh = false;
   int64_t chunk_cur = 0, chunk_end = 0;  // wave-uniform
   int32_t chunk_site = 0;                // wave-uniform: the site of the chunk's first ticket
   bool drained = false;                  // wave-uniform
@@ -504,7 +506,8 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWav
         job_init(j, tb, wp, chunk_cur + rank, chunk_site);
         if constexpr (kChan == kChanCoupled) coupled_start_philox(j);
         active = true;
-        in_query = query_start<R, GK, kCount>(j, q, &cnt);
+        fresh = true;
+        in_query = false;
       }
       chunk_cur += take;
       need = __ballot(!active);
@@ -520,35 +523,40 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWav
         go = go && (busy == 0 || ready >= (kBatch == 1 ? __popcll(busy) : kBatch));
       }
       if (go) {
-        V3<R> bp;
-        int cls;
-        R r;
-        if (GK != kGeomAnalytic && is_mesh<R, GK>(j)) {
-          cls = 0;
-          if (q.grid) {
-            r = q.best;
-            bp = j.x;
+        bool start = fresh;
+        if (!fresh) {
+          V3<R> bp;
+          int cls;
+          R r;
+          if (GK != kGeomAnalytic && is_mesh<R, GK>(j)) {
+            cls = 0;
+            if (q.grid) {
+              r = q.best;
+              bp = j.x;
+            } else {
+              r = mesh_distance<R>(*j.g, q.best);
+              bp = q.pt;
+              j.hint = q.best_k;
+            }
           } else {
-            r = mesh_distance<R>(*j.g, q.best);
-            bp = q.pt;
-            j.hint = q.best_k;
+            r = container_analytic<R>(*j.g, j.x, bp, cls);
           }
-        } else {
-          r = container_analytic<R>(*j.g, j.x, bp, cls);
+          r = holes_and_cavities<R, GK>(*j.g, j.x, r, bp, cls);
+          if constexpr (kChan == kChanCoupled) r = coupled_cavity(j, r, bp, cls);
+          const int st = job_advance<R, kChan>(j, wp, r, bp, cls);
+          if (st == 0) {
+            start = true;
+          } else if (kChan == kChanCoupled && j.phase == 0) {
+            coupled_second_path(j, st == 2);
+            start = true;
+          } else {
+            if constexpr (kChan == kChanCoupled) coupled_store(j, wp, st == 2);
+            else job_store(j, wp, st == 2);
+            active = false;
+          }
         }
-        r = holes_and_cavities<R, GK>(*j.g, j.x, r, bp, cls);
-        if constexpr (kChan == kChanCoupled) r = coupled_cavity(j, r, bp, cls);
-        const int st = job_advance<R, kChan>(j, wp, r, bp, cls);
-        if (st == 0) {
-          in_query = query_start<R, GK, kCount>(j, q, &cnt);
-        } else if (kChan == kChanCoupled && j.phase == 0) {
-          coupled_second_path(j, st == 2);
-          in_query = query_start<R, GK, kCount>(j, q, &cnt);
-        } else {
-          if constexpr (kChan == kChanCoupled) coupled_store(j, wp, st == 2);
-          else job_store(j, wp, st == 2);
-          active = false;
-        }
+        fresh = false;
+        if (start) in_query = query_start<R, GK, kCount>(j, q, &cnt);
       }
     }
   }
