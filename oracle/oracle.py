@@ -19,7 +19,7 @@ LIB_PATH = HERE / "liboracle.so"
 
 BALL, BOX, MESH = 0, 1, 2
 SQUARE, PAIRED, CROSS, COUPLED = 0, 1, 2, 3
-RNG_PHILOX, RNG_MT_SITE, RNG_MT_PAIRED, RNG_MT_KEY, RNG_MT_FD = 0, 1, 2, 3, 4
+RNG_PHILOX, RNG_MT_SITE, RNG_MT_PAIRED, RNG_MT_KEY, RNG_MT_FD, RNG_MT_DIFF = 0, 1, 2, 3, 4, 5
 
 _dp = C.POINTER(C.c_double)
 _u32p = C.POINTER(C.c_uint32)
@@ -103,6 +103,9 @@ def lib() -> C.CDLL:
         L.og_combine_powersums.argtypes = [_dp, C.c_int64, _dp]
         L.og_combine_grad.argtypes = [_dp, _dp, C.c_int64, _dp]
         L.og_bump_green_ball.argtypes = [_dp, C.c_double, _dp, C.c_double, C.c_double]
+        L.og_differential.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _dp, C.c_int64, C.c_double, _dp,
+                                      C.POINTER(C.c_int64)]
+        L.og_differential.restype = C.c_int32
         L.og_bump_green_ball.restype = C.c_double
         _lib = L
     return _lib
@@ -323,6 +326,30 @@ def combine_powersums(sums, n: int) -> np.ndarray:
     out = np.zeros(6)
     lib().og_combine_powersums(_ptr(s, _dp), n, _ptr(out, _dp))
     return out
+
+
+DIFF_COUNTS = ("outer_steps", "nested_steps", "cavity_terminations", "near_seam", "segment_unsafe", "unsafe",
+               "limits", "distance_queries", "parity_rays")
+
+
+def differential(geometry: Geometry, sites, side_, *, seed=0, repeat=0, walks=16, eps=1e-4, max_steps=2048,
+                 offset=0.004, threads=0):
+    """Differential WoS cavity derivatives per site on the reference's
+    streams: (values [n][n_cavities][8] = dc x y z, their SEs, dR, its SE;
+    counts [n][9] in DIFF_COUNTS order)."""
+    sites = _f64(np.atleast_2d(sites))
+    n = len(sites)
+    g = geometry.struct()
+    k = g.n_cavities
+    p = Params(rng=RNG_MT_DIFF, seed=seed, repeat=repeat, max_steps=max_steps, eps=eps, walks=walks,
+               threads=threads)
+    out = np.zeros((n, max(k, 1), 8))
+    counts = np.zeros((n, 9), dtype=np.int64)
+    st = lib().og_differential(C.byref(g), C.byref(side_), C.byref(p), _ptr(sites, _dp), n, offset,
+                               _ptr(out, _dp), counts.ctypes.data_as(C.POINTER(C.c_int64)))
+    if st == 2:
+        raise ValueError("invalid differential arguments")
+    return out[:, :k], counts
 
 
 def bump_green_ball(x, radius: float, center, support: float, amplitude: float) -> float:

@@ -380,13 +380,14 @@ struct og_mesh {
   // Ray parity along one fixed irrational direction, stepping from hit to hit:
   // EmbreeMesh::inside, wih_twos.cpp:361-394 (tnear 1e-6, tfar 4, step 2e-6,
   // at most 128 crossings).  Returns -1 when the crossing limit is exceeded.
-  int inside(const V3 &p) const {
+  int inside(const V3 &p, int64_t *rays = nullptr) const {
     V3 direction{1.0, 0.3713906763541037, 0.1570796326794897};
     direction = direction * (1.0 / norm(direction));
     V3 origin = p;
     unsigned hits = 0;
     for (; hits < 128; ++hits) {
       double t;
+      if (rays) ++*rays;  // one ray per intersection query, as note_query(1) counts them
       if (!ray_hit(origin, direction, 1.0e-6, 4.0, t)) break;
       origin = origin + direction * (t + 2.0e-6);
     }
@@ -819,9 +820,205 @@ void combine(const double *a, const double *b, int64_t count, bool square, doubl
   }
 }
 
+// ---------------------------------------------------------------------------
+// Differential WoS for cavity-shape derivatives (evaluate_differential_site,
+// wih_twos.cpp:725-928; study.cpp:303-316 runs it over every point with
+// repeat 0).  Mesh container with the reference's cavities only.
+
+struct Hit {
+  double distance = kInf;
+  V3 point{0.0, 0.0, 0.0}, normal{0.0, 0.0, 0.0};
+  int32_t cavity = -1;
+};
+
+// closest_boundary with the hit's outward normal and cavity index
+// (wih_twos.cpp:602-626): the mesh first (normal towards the mesh point), then
+// each cavity when strictly closer (normal into the cavity).
+Hit boundary_hit(const og_geom &g, const V3 &p, int64_t *queries) {
+  Hit h;
+  int64_t tri;
+  ++*queries;
+  h.distance = g.mesh->closest(p, h.point, tri, false, nullptr, nullptr);
+  if (h.distance > 0.0) h.normal = (h.point - p) * (1.0 / h.distance);
+  for (int32_t i = 0; i < g.n_cavities; ++i) {
+    const V3 c = load3(g.cavities + 3 * i);
+    const V3 radial = p - c;
+    const double len = norm(radial);
+    const double dist = len - g.cavity_radius;
+    if (dist < h.distance && len > 0.0) {
+      const V3 dir = radial * (1.0 / len);
+      h.distance = dist;
+      h.point = c + dir * g.cavity_radius;
+      h.normal = dir * -1.0;
+      h.cavity = i;
+    }
+  }
+  return h;
+}
+
+// domain_distance (wih_twos.cpp:585-593): one mesh query, then the cavities.
+double domain_distance(const og_geom &g, const V3 &p, int64_t *queries) {
+  V3 q;
+  int64_t tri;
+  ++*queries;
+  double best = g.mesh->closest(p, q, tri, false, nullptr, nullptr);
+  for (int32_t i = 0; i < g.n_cavities; ++i) best = std::min(best, norm(p - load3(g.cavities + 3 * i)) - g.cavity_radius);
+  return best;
+}
+
+struct DiffCounts {
+  int64_t outer_steps = 0, nested_steps = 0, cavity_terms = 0, near_seam = 0, segment_unsafe = 0, unsafe = 0,
+          limits = 0, queries = 0, rays = 0;
+};
+
+// classify_offset_safety (wih_twos.cpp:764-833), with the enclosure fallback
+// the study executable compiles in (study.cpp:8).  Returns (near_seam,
+// segment_unsafe).
+std::pair<bool, bool> offset_safety(const og_geom &g, int32_t hit, const V3 &bp, const V3 &off, double ell,
+                                    double eps, DiffCounts &c) {
+  bool near_seam = false, segment_unsafe = false;
+  const double r = g.cavity_radius;
+  const V3 seg = off - bp;
+  const double seg2 = dot(seg, seg);
+  const double tol = 1.0e-10 * std::max(1.0, std::fabs(r));
+  const double coll2 = (r + tol) * (r + tol);
+  const V3 hc = load3(g.cavities + 3 * hit);
+  for (int32_t i = 0; i < g.n_cavities; ++i) {
+    if (i == hit) continue;
+    const V3 ci = load3(g.cavities + 3 * i);
+    const V3 delta = ci - hc;
+    const double cd = norm(delta);
+    if (cd > tol && cd < 2.0 * r) {
+      // Distance from the hit point to the seam circle of two equal spheres.
+      const V3 axis = delta * (1.0 / cd);
+      const V3 seam_c = (hc + ci) * 0.5;
+      const double seam_r = std::sqrt(std::max(0.0, r * r - 0.25 * (cd * cd)));
+      const V3 rel = bp - seam_c;
+      const double ax = dot(rel, axis);
+      const V3 rad = rel - axis * ax;
+      const double nr = norm(rad) - seam_r;
+      near_seam = near_seam || std::sqrt(ax * ax + nr * nr) <= ell;
+    }
+    if (seg2 > 0.0) {
+      const double t = std::clamp(dot(ci - bp, seg) / seg2, 0.0, 1.0);
+      const V3 cl = bp + seg * t;
+      if (dot(cl - ci, cl - ci) <= coll2) segment_unsafe = true;
+    }
+  }
+  bool inside = g.mesh->inside(off, &c.rays) == 1;
+  if (!inside && g.mesh->inside(hc, &c.rays) == 1) {
+    V3 q;
+    int64_t tri;
+    ++c.queries;
+    const double dc = g.mesh->closest(hc, q, tri, false, nullptr, nullptr);
+    if (dc > r + ell + eps && norm(off - hc) <= r + ell + eps * 0.5) inside = true;
+  }
+  if (!inside || !(domain_distance(g, off, &c.queries) > eps)) segment_unsafe = true;
+  return {near_seam, segment_unsafe};
+}
+
+// One site: per cavity the mean and SE of the center and radius derivative
+// samples (8 values: dc x y z, dc se x y z, dR, dR se).
+void differential_site(const og_geom &g, const og_side &side, const og_params &p, const V3 &z, int64_t site,
+                       double ell, double *out, DiffCounts &c) {
+  const int32_t K = g.n_cavities;
+  std::vector<V3> sums(size_t(K), V3{0.0, 0.0, 0.0}), sq(size_t(K), V3{0.0, 0.0, 0.0});
+  std::vector<double> rs(size_t(K), 0.0), rsq(size_t(K), 0.0);
+  for (int64_t n = 0; n < p.walks; ++n) {
+    const uint64_t key = splitmix64(p.seed + 0x6a09e667f3bcc909ULL) ^
+                         splitmix64(uint64_t(p.repeat) + 0xbb67ae8584caa73bULL) ^
+                         splitmix64(uint64_t(site) + 0x3c6ef372fe94f82bULL) ^
+                         splitmix64(uint64_t(n) + 0xa54ff53a5f1d36f1ULL);
+    // walk_to_boundary (wih_twos.cpp:725-740): no source, the last hit kept.
+    std::mt19937_64 rng(splitmix64(key));
+    V3 x = z;
+    Hit h;
+    bool capped = true;
+    for (int32_t s = 0; s < p.max_steps; ++s) {
+      h = boundary_hit(g, x, &c.queries);
+      if (!(h.distance > p.eps)) {
+        capped = false;
+        break;
+      }
+      x = x + random_unit_vector(rng) * h.distance;
+      ++c.outer_steps;
+    }
+    if (capped) {
+      ++c.limits;
+      continue;
+    }
+    if (h.cavity < 0) continue;
+    ++c.cavity_terms;
+    const V3 off = h.point - h.normal * ell;
+    const auto safety = offset_safety(g, h.cavity, h.point, off, ell, p.eps, c);
+    c.near_seam += safety.first ? 1 : 0;
+    c.segment_unsafe += safety.second ? 1 : 0;
+    if (safety.first || safety.second) {
+      ++c.unsafe;
+      continue;
+    }
+    // The nested walk (walk(), wih_twos.cpp:635-664, keyed nested_key).
+    MtStream st;
+    st.rng.seed(splitmix64(splitmix64(key ^ 0x510e527fade682d1ULL)));
+    int32_t rows = 0;
+    const WalkOut w = walk(g, side, nullptr, off, p.eps, p.max_steps, st, false, nullptr, 0, &rows);
+    c.nested_steps += w.steps;
+    c.queries += w.steps + (w.cap ? 0 : 1);
+    c.limits += w.cap ? 1 : 0;
+    const double scale = w.a / ell;
+    const V3 cs = h.normal * scale;
+    const size_t k = size_t(h.cavity);
+    sums[k] = sums[k] + cs;
+    sq[k].x += cs.x * cs.x;
+    sq[k].y += cs.y * cs.y;
+    sq[k].z += cs.z * cs.z;
+    rs[k] += -scale;
+    rsq[k] += scale * scale;
+  }
+  const double N = double(p.walks);
+  for (int32_t k = 0; k < K; ++k) {
+    double *o = out + 8 * size_t(k);
+    const V3 mean = sums[size_t(k)] * (1.0 / N);
+    o[0] = mean.x; o[1] = mean.y; o[2] = mean.z;
+    o[6] = rs[size_t(k)] / N;
+    if (p.walks > 1) {
+      const double den = N * (N - 1.0);
+      const V3 &s = sums[size_t(k)], &q = sq[size_t(k)];
+      o[3] = std::sqrt(std::max(0.0, (q.x - s.x * s.x / N) / den));
+      o[4] = std::sqrt(std::max(0.0, (q.y - s.y * s.y / N) / den));
+      o[5] = std::sqrt(std::max(0.0, (q.z - s.z * s.z / N) / den));
+      o[7] = std::sqrt(std::max(0.0, (rsq[size_t(k)] - rs[size_t(k)] * rs[size_t(k)] / N) / den));
+    } else {
+      o[3] = o[4] = o[5] = o[7] = 0.0;
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
+
+int32_t og_differential(const og_geom *g, const og_side *side, const og_params *prm, const double *sites, int64_t n,
+                        double ell, double *out, int64_t *counts) {
+  if (!g || g->container != OG_MESH || !g->mesh || prm->walks < 1 || !(ell > 0.0) || prm->rng != OG_RNG_MT_DIFF)
+    return 2;
+  const og_params p = *prm;
+  std::vector<DiffCounts> per(size_t(std::max<int64_t>(n, 1)));
+#ifdef _OPENMP
+  if (p.threads > 0) omp_set_num_threads(p.threads);
+#pragma omp parallel for schedule(dynamic, 4)
+#endif
+  for (int64_t s = 0; s < n; ++s)
+    differential_site(*g, *side, p, load3(sites + 3 * s), s, ell, out + 8 * size_t(g->n_cavities) * size_t(s),
+                      per[size_t(s)]);
+  for (int64_t s = 0; s < n; ++s) {
+    const DiffCounts &c = per[size_t(s)];
+    const int64_t row[9] = {c.outer_steps, c.nested_steps, c.cavity_terms, c.near_seam, c.segment_unsafe, c.unsafe,
+                            c.limits, c.queries, c.rays};
+    for (int k = 0; k < 9; ++k) counts[9 * s + k] = row[k];
+  }
+  return 0;
+}
 
 uint64_t og_splitmix64(uint64_t x) { return splitmix64(x); }
 

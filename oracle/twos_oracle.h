@@ -19,6 +19,9 @@
  *       the reference field fixtures' score / score_se columns.
  *   harmonic Dirichlet payoff: unit-ball closed forms (u = g for harmonic g).
  *   ray-parity inside test: analytic ball / star-shaped meshes.
+ *   og_differential: the retained wih_study --diff 1 requests on the branch
+ *       mesh (request totals of steps, distance queries and parity rays
+ *       exactly; derivative columns, most sites bitwise).
  */
 #ifndef TWOS_ORACLE_H
 #define TWOS_ORACLE_H
@@ -34,7 +37,8 @@ enum { OG_BALL = 0, OG_BOX = 1, OG_MESH = 2 };
 /* Combine modes. */
 enum { OG_SQUARE = 0, OG_PAIRED = 1, OG_CROSS = 2, OG_COUPLED = 3 };
 /* Stream modes. */
-enum { OG_RNG_PHILOX = 0, OG_RNG_MT_SITE = 1, OG_RNG_MT_PAIRED = 2, OG_RNG_MT_KEY = 3, OG_RNG_MT_FD = 4 };
+enum { OG_RNG_PHILOX = 0, OG_RNG_MT_SITE = 1, OG_RNG_MT_PAIRED = 2, OG_RNG_MT_KEY = 3, OG_RNG_MT_FD = 4,
+       OG_RNG_MT_DIFF = 5 };
 
 typedef struct og_mesh og_mesh;
 
@@ -130,6 +134,14 @@ int32_t og_trace(const og_geom *g, const og_side *side, const og_params *prm,
 
 void og_combine(const double *a, const double *b, int64_t n, int32_t square, double out[6]);
 void og_combine_powersums(const double sums[8], int64_t n, double out[6]);
+/* Differential WoS cavity-shape derivatives (evaluate_differential_site,
+ * wih_twos.cpp:725-928) on a mesh with cavities, the reference's streams
+ * (rng = OG_RNG_MT_DIFF, repeat as given): out[n][n_cavities][8] = dc x y z,
+ * their SEs, dR, its SE; counts[n][9] = outer steps, nested steps, cavity
+ * terminations, near-seam, segment-unsafe, unsafe, limits, distance queries,
+ * parity rays. */
+int32_t og_differential(const og_geom *g, const og_side *side, const og_params *prm, const double *sites, int64_t n,
+                        double ell, double *out, int64_t *counts);
 /* Exact integral of the Green function of B(x, radius) against the bump
  * amplitude (1 - |y - c|^2 / support^2)^2 (source_kind 3; study.cpp:56-76). */
 double og_bump_green_ball(const double x[3], double radius, const double c[3], double support, double amplitude);

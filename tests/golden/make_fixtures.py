@@ -173,6 +173,42 @@ def study_fixtures(root: Path) -> None:
     (HERE / "study_branch_meta.json").write_text(json.dumps(meta, indent=1) + "\n")
 
 
+def diff_fixtures(root: Path) -> None:
+    """Differential WoS requests (wih_study --diff 1) on the branch mesh: all
+    twelve of validation-local-cavity (one cavity, 128 sites, offsets 0.002,
+    0.004, 0.008) and the first two of validation (six cavities, 256 sites).
+    Per request: sites, cavity centres, the field's per-cavity columns
+    dc{k}_x,y,z, dc{k}_se_x,y,z, dR{k}, dR{k}_se, and kernel.json's request
+    totals (steps = outer + nested steps, distance queries, parity rays, caps)."""
+    study = root / "benchmarks" / "submission_study"
+    picks = sorted(study.glob("validation-local-cavity/raw/gradient-*/request.json"))
+    picks += sorted(study.glob("validation/raw/gradient-*/request.json"))[:2]
+    arrays, requests = {}, []
+    for k, rq in enumerate(picks):
+        d = rq.parent
+        a = study_arguments(rq)
+        assert a["--diff"] == "1" and a["--mesh"].endswith("/branch.ply")
+        assert a["--config"].endswith("assets/branch.txt") and a["--repeats"] == "1"
+        points = np.loadtxt(d / "points.txt", ndmin=2).reshape(-1, 3)
+        holes = np.loadtxt(d / "holes.txt", ndmin=2).reshape(-1, 3)
+        with (d / "field.csv").open() as handle:
+            rows = list(csv.DictReader(handle))
+        assert [int(r["site"]) for r in rows] == list(range(len(points)))
+        cols = [[f"dc{c}_x", f"dc{c}_y", f"dc{c}_z", f"dc{c}_se_x", f"dc{c}_se_y", f"dc{c}_se_z", f"dR{c}",
+                 f"dR{c}_se"] for c in range(len(holes))]
+        arrays[f"{k}_points"] = points
+        arrays[f"{k}_holes"] = holes
+        arrays[f"{k}_values"] = np.array([[[float(r[x]) for x in cs] for cs in cols] for r in rows])
+        kernel = json.loads((d / "kernel.json").read_text())
+        requests.append({"source": str(d.relative_to(study)), "seed": int(a["--seed"]), "walks": int(a["--walks"]),
+                         "epsilon": float(a["--epsilon"]), "max_steps": int(a["--max-steps"]),
+                         "cavity_radius": float(a["--cavity-radius"]), "offset": float(a["--diff-offset"]),
+                         "config": "current",
+                         "totals": {x: int(kernel[x]) for x in ("steps", "distance_queries", "parity_rays", "caps")}})
+    np.savez_compressed(HERE / "study_branch_diff.npz", **arrays)
+    (HERE / "study_branch_diff.json").write_text(json.dumps({"requests": requests}, indent=1) + "\n")
+
+
 def housing_fixtures(root: Path) -> None:
     """Scout and validation fields of the engineering-housing design loop
     (benchmarks/engineering_housing/output/field-rep-1-*.csv with their
@@ -247,3 +283,4 @@ if __name__ == "__main__":
     study_fixtures(Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference"))
     housing_fixtures(Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference"))
     design_fixture(Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference"))
+    diff_fixtures(Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference"))

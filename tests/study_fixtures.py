@@ -115,3 +115,48 @@ def compare_housing(req: HousingRequest, u, score, score_se, steps, caps, rtol: 
         err = np.abs(ours[:, c] - ref)
         assert np.all(err <= rtol * scale), (req.tag, c, np.max(err / scale))
     return np.all(ours == req.values, axis=1)
+
+
+@dataclass
+class DiffRequest:
+    source: str
+    points: np.ndarray
+    holes: np.ndarray
+    values: np.ndarray   # [n, n_cavities, 8]: dc x y z, dc se x y z, dR, dR se
+    totals: dict         # request totals: steps, distance_queries, parity_rays, caps
+    seed: int
+    walks: int
+    epsilon: float
+    max_steps: int
+    cavity_radius: float
+    offset: float
+    config: str
+
+
+def load_diff(golden: Path):
+    """Differential WoS requests (wih_study --diff 1) on the branch mesh
+    (tests/golden/study_branch_diff.*): the mesh and the requests."""
+    meta = json.loads((golden / "study_branch_meta.json").read_text())
+    mesh = np.load(golden / "study_branch_mesh.npz")
+    arrays = np.load(golden / "study_branch_diff.npz")
+    reqs = json.loads((golden / "study_branch_diff.json").read_text())["requests"]
+    out = [DiffRequest(source=r["source"], points=arrays[f"{k}_points"], holes=arrays[f"{k}_holes"],
+                       values=arrays[f"{k}_values"], totals=r["totals"], seed=r["seed"], walks=r["walks"],
+                       epsilon=r["epsilon"], max_steps=r["max_steps"], cavity_radius=r["cavity_radius"],
+                       offset=r["offset"], config=meta["configs"][r["config"]])
+           for k, r in enumerate(reqs)]
+    return mesh["vertices"], mesh["faces"], out
+
+
+def compare_diff(req: DiffRequest, values, rtol: float) -> np.ndarray:
+    """Every derivative column within rtol, relative to the larger of
+    |reference| and 1e-3 times that column's largest |value| over the request;
+    returns per site whether all of its values are bitwise equal (near-tie
+    closest triangles move a few paths by ulps, see compare)."""
+    ref = req.values
+    assert values.shape == ref.shape, (values.shape, ref.shape)
+    scale = np.maximum(np.abs(ref), 1e-3 * np.max(np.abs(ref), axis=0, keepdims=True))
+    err = np.abs(values - ref)
+    bad = err > rtol * scale
+    assert not np.any(bad), (req.source, np.argwhere(bad)[:4], np.max(err / np.maximum(scale, 1e-300)))
+    return np.all(values == ref, axis=(1, 2))
