@@ -282,6 +282,30 @@ twos_status twos_trace(twos_ctx *ctx, int32_t geom_id, const twos_side *side, co
                        const double start[3], int64_t site, int32_t walk, double *rows, int32_t max_rows,
                        int32_t *n_rows);
 
+/* Differential Walk on Spheres: the derivative of u at each site with respect
+ * to each cavity's centre and radius (replaces evaluate_differential_site,
+ * benchmarks/compiled_bunny/src/wih_twos.cpp:725-928, over the sites as
+ * wih_study --diff 1 runs it, benchmarks/submission_study/study.cpp:303-316).
+ * geom_id: a mesh container, minus cavities only (no spheres or cylinders;
+ * without cavities every sample is zero and only the counters move).  Per
+ * site, params->walks samples: an outer path walks to the boundary without a
+ * source; one that ends on cavity k at the point h with outward normal n
+ * (into the cavity) is offset to off = h - n ell and, unless that offset is
+ * unsafe (near the seam of two overlapping cavities, its segment crossing
+ * another cavity, or off outside the domain / within eps of its boundary),
+ * runs one nested path of `side` from off with value V; the sample adds
+ * n V / ell to cavity k's centre derivative and -V / ell to its radius
+ * derivative (other cavities get zero).  Streams: the reference's
+ * mt19937_64 keys (params->seed, params->repeat, params->site_offset + site,
+ * sample); params->precision must be 64.  out: [n][n_cavities][8] = the
+ * centre derivative x y z, their standard errors, the radius derivative and
+ * its standard error (means over all walks, unsafe and non-cavity samples
+ * counting as zero).  counts (may be null): [n][9] = outer steps, nested
+ * steps, cavity terminations, near-seam, segment-unsafe, unsafe, max_steps
+ * caps, distance queries, ray-parity queries. */
+twos_status twos_differential(twos_ctx *ctx, int32_t geom_id, const twos_side *side, const twos_params *params,
+                              const double *sites, int64_t n, double ell, double *out, int64_t *counts);
+
 /* Continuous-batching server: requests are admitted between rounds as device
  * memory allows, all admitted sites share each persistent launch, and each
  * result depends only on (seed, site, side, walk), alone or in the mix. */

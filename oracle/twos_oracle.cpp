@@ -839,6 +839,7 @@ Hit boundary_hit(const og_geom &g, const V3 &p, int64_t *queries) {
   int64_t tri;
   ++*queries;
   h.distance = g.mesh->closest(p, h.point, tri, false, nullptr, nullptr);
+  if (g.mesh_radius_f32) h.distance = double(float(h.distance));
   if (h.distance > 0.0) h.normal = (h.point - p) * (1.0 / h.distance);
   for (int32_t i = 0; i < g.n_cavities; ++i) {
     const V3 c = load3(g.cavities + 3 * i);
@@ -862,6 +863,7 @@ double domain_distance(const og_geom &g, const V3 &p, int64_t *queries) {
   int64_t tri;
   ++*queries;
   double best = g.mesh->closest(p, q, tri, false, nullptr, nullptr);
+  if (g.mesh_radius_f32) best = double(float(best));
   for (int32_t i = 0; i < g.n_cavities; ++i) best = std::min(best, norm(p - load3(g.cavities + 3 * i)) - g.cavity_radius);
   return best;
 }
@@ -910,7 +912,8 @@ std::pair<bool, bool> offset_safety(const og_geom &g, int32_t hit, const V3 &bp,
     V3 q;
     int64_t tri;
     ++c.queries;
-    const double dc = g.mesh->closest(hc, q, tri, false, nullptr, nullptr);
+    double dc = g.mesh->closest(hc, q, tri, false, nullptr, nullptr);
+    if (g.mesh_radius_f32) dc = double(float(dc));
     if (dc > r + ell + eps && norm(off - hc) <= r + ell + eps * 0.5) inside = true;
   }
   if (!inside || !(domain_distance(g, off, &c.queries) > eps)) segment_unsafe = true;

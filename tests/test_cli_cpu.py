@@ -32,6 +32,7 @@ def inputs(tmp_path):
     (["--walks", "1"], "invalid numeric option"),
     (["--epsilon", "0"], "invalid numeric option"),
     (["--mode", "voxel"], "not provided by this build"),
+    (["--mode", "diffwos"], "Differential WoS requires --cavities"),
     (["--bogus", "1"], "unknown option --bogus"),
     (["--walks"], "missing value for --walks"),
     (["--walks", "x"], "invalid value for --walks"),
@@ -106,7 +107,6 @@ def test_study_rejects_other_solvers(inputs, tmp_path):
     _, base = inputs
     cfg = tmp_path / "c.txt"
     cfg.write_text("0 0 0 0.1\n0.5 0 0 0.1 3\n")
-    for extra, needle in ((["--mode", "mg"], "mode mg is not provided"),
-                          (["--diff", "1"], "--diff 1 is not provided"), (["--sensor-side", "2"], "sensor side")):
+    for extra, needle in ((["--mode", "mg"], "mode mg is not provided"), (["--sensor-side", "2"], "sensor side")):
         r = _run("wih_study", [*base, "--config", str(cfg), *extra])
         assert r.returncode == 1 and needle in r.stderr, (extra, r.stderr)

@@ -17,10 +17,15 @@
 // analytic-source variant (analytic_query.cpp: study.cpp built with
 // WIH_ANALYTIC_SOURCE, :98-103): each step adds the exact Green-ball integral
 // of the heat and sensor bumps (source_kind 3) instead of sampling a point;
-// like that variant it applies to the paired estimator, not to --fd.  The
-// reference's multigrid solver
-// (--mode mg) and Differential WoS (--diff 1) are other solvers; a request
-// asking for them fails with status 1.
+// like that variant it applies to the paired estimator, not to --fd.
+// --diff 1 is Differential WoS (study.cpp:303-316; twos_differential): per
+// point and cavity, the derivative of the heat solution with respect to the
+// cavity's centre and radius, written as site,x,y,z then per cavity
+// dc{k}_x,y,z, dc{k}_se_x,y,z, dR{k}, dR{k}_se (write_differential_field,
+// wih_twos.cpp:1576-1602); every point is evaluated, and an unsafe offset or a
+// capped path fails the request after the field is written.  The reference's
+// multigrid solver (--mode mg) is another solver; a request asking for it
+// fails with status 1.
 #include <chrono>
 #include <cmath>
 #include <fstream>
@@ -101,6 +106,50 @@ bool inside_cavity(const double *p, const std::vector<double> &holes, double rad
   return false;
 }
 
+// --diff 1: Differential WoS over every point, repeat 0, the heat sources
+// (study.cpp:303-316).
+int differential_request(Session &session, const wih::Options &o, int32_t geom, const twos_side &heat,
+                         const std::vector<double> &points, const std::vector<double> &holes,
+                         std::chrono::steady_clock::time_point start) {
+  const size_t n = points.size() / 3, k = holes.size() / 3;
+  twos_params p = wih::make_params(o, TWOS_SQUARE, TWOS_RNG_MT_SITE, 0);
+  p.precision = 64;  // the reference's mt19937_64 keys, binary64 only
+  std::vector<double> out(n * k * 8);
+  std::vector<int64_t> counts(n * 9);
+  wih::check(twos_differential(session.ctx->get(), geom, &heat, &p, points.data(), int64_t(n), o.diff_offset,
+                               out.data(), counts.data()),
+             "differential");
+  const double seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+  std::ofstream f(o.output_path);
+  if (!f) throw std::runtime_error("cannot write differential field output: " + o.output_path);
+  f << "site,x,y,z";
+  for (size_t c = 0; c < k; ++c)
+    f << ",dc" << c << "_x,dc" << c << "_y,dc" << c << "_z,dc" << c << "_se_x,dc" << c << "_se_y,dc" << c
+      << "_se_z,dR" << c << ",dR" << c << "_se";
+  f << '\n' << std::setprecision(17);
+  uint64_t steps = 0, unsafe = 0, caps = 0, queries = 0, rays = 0;
+  for (size_t i = 0; i < n; ++i) {
+    f << i << ',' << points[3 * i] << ',' << points[3 * i + 1] << ',' << points[3 * i + 2];
+    for (size_t c = 0; c < 8 * k; ++c) f << ',' << out[8 * k * i + c];
+    f << '\n';
+    const int64_t *q = &counts[9 * i];
+    steps += uint64_t(q[0] + q[1]);
+    unsafe += uint64_t(q[5]);
+    caps += uint64_t(q[6]);
+    queries += uint64_t(q[7]);
+    rays += uint64_t(q[8]);
+  }
+  f.close();
+  if (unsafe || caps)
+    throw std::runtime_error("unsafe differential path: unsafe=" + std::to_string(unsafe) +
+                             " caps=" + std::to_string(caps));
+  std::ofstream s(o.summary_path);
+  s << std::setprecision(17) << "{\"total_seconds\":" << seconds << ",\"steps\":" << steps << ",\"caps\":" << caps
+    << ",\"peak_rss_megabytes\":" << wih::peak_rss_megabytes() << ",\"distance_queries\":" << queries
+    << ",\"parity_rays\":" << rays << "}\n";
+  return 0;
+}
+
 int process_request(int argc, char **argv, Session &session) {
   try {
     bool hot = false, fd = false, diff = false, analytic = false;
@@ -137,7 +186,6 @@ int process_request(int argc, char **argv, Session &session) {
     if (analytic && !fd) heat.source_kind = sensor.source_kind = 3;
     if (o.mode != "twos" && o.mode != "mg") throw std::runtime_error("use twos or mg");
     if (o.mode == "mg") throw std::runtime_error("mode mg is not provided by this build (twos only)");
-    if (diff) throw std::runtime_error("--diff 1 is not provided by this build");
     const auto start = std::chrono::steady_clock::now();
     if (!session.ctx) {
       const wih::PlyMesh mesh = wih::load_ascii_ply(o.mesh_path);
@@ -153,6 +201,7 @@ int process_request(int argc, char **argv, Session &session) {
     const std::vector<double> holes = wih::read_optional_points(o.cavities_path);
     const Geometry geometry(session, holes, o.cavity_radius);
     const int32_t geom = geometry.id();
+    if (diff) return differential_request(session, o, geom, heat, points, holes, start);
     // Sites inside a cavity are skipped and report zeros (study.cpp:341-342);
     // the finite-cavity difference evaluates every point (:319).
     const size_t n = points.size() / 3;

@@ -4,8 +4,9 @@
 // out, same exit codes: usage errors exit 2, runtime errors print
 // "fatal: <what>" and exit 1, a path that hits --max-steps is fatal.
 //
-// Mode twos only (the reference's voxel, mg, diffwos and validate modes are
-// other solvers).  By default the walks replay the reference's own streams
+// Modes twos and diffwos (Differential WoS cavity derivatives,
+// run_differential_wos, wih_twos.cpp:1604-1705; twos_differential); the
+// reference's voxel, mg and validate modes are other solvers.  By default the walks replay the reference's own streams
 // (mt19937_64 per walk keyed as evaluate_site keys them, binary64), so the
 // field file matches the reference's; --streams philox selects the
 // production path (Philox streams, binary32 walks, optional distance grid and
@@ -119,12 +120,93 @@ void write_summary(const std::string &path, const wih::PlyMesh &mesh, size_t n_c
   out << "]\n}\n";
 }
 
+// --mode diffwos: every repeat runs the differential estimator with its
+// repeat index in the keys, the field and counters are repeat 0's
+// (run_differential_wos, wih_twos.cpp:1604-1705).
+int run_differential(const Options &o, twos_ctx *ctx, int32_t geom, const twos_side &side, const wih::PlyMesh &mesh,
+                     const std::vector<double> &points, size_t n_cavities, double build_s) {
+  const size_t n = points.size() / 3;
+  std::vector<double> times, out(n * n_cavities * 8), kept;
+  std::vector<int64_t> counts(n * 9), kept_counts;
+  for (int repeat = 0; repeat < o.repeats; ++repeat) {
+    twos_params p = wih::make_params(o, TWOS_SQUARE, TWOS_RNG_MT_SITE, repeat);
+    p.precision = 64;  // the reference's mt19937_64 keys, binary64 only
+    const auto start = std::chrono::steady_clock::now();
+    wih::check(twos_differential(ctx, geom, &side, &p, points.data(), int64_t(n), o.diff_offset, out.data(),
+                                 counts.data()),
+               "differential");
+    times.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count());
+    if (repeat == 0) {
+      kept = out;
+      kept_counts = counts;
+    }
+  }
+  uint64_t c[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (size_t i = 0; i < n; ++i)
+    for (int k = 0; k < 9; ++k) c[k] += uint64_t(kept_counts[9 * i + size_t(k)]);
+  if (c[6] != 0) throw std::runtime_error("one or more differential paths hit --max-steps");
+  std::ofstream f(o.output_path);
+  if (!f) throw std::runtime_error("cannot write differential field output: " + o.output_path);
+  f << "site,x,y,z";
+  for (size_t k = 0; k < n_cavities; ++k)
+    f << ",dc" << k << "_x,dc" << k << "_y,dc" << k << "_z,dc" << k << "_se_x,dc" << k << "_se_y,dc" << k
+      << "_se_z,dR" << k << ",dR" << k << "_se";
+  f << '\n' << std::setprecision(17);
+  for (size_t i = 0; i < n; ++i) {
+    f << i << ',' << points[3 * i] << ',' << points[3 * i + 1] << ',' << points[3 * i + 2];
+    for (size_t v = 0; v < 8 * n_cavities; ++v) f << ',' << kept[8 * n_cavities * i + v];
+    f << '\n';
+  }
+  f.close();
+  const double paths = double(n) * o.walks, runtime = wih::median(times);
+  std::ofstream s(o.summary_path);
+  if (!s) throw std::runtime_error("cannot write differential summary: " + o.summary_path);
+  s << std::setprecision(17) << "{\n"
+    << "  \"engine\": \"MI355X HIP LBVH Differential WoS\",\n"
+    << "  \"mode\": \"diffwos\",\n"
+    << "  \"vertices\": " << mesh.vertex_count() << ",\n"
+    << "  \"faces\": " << mesh.face_count() << ",\n"
+    << "  \"source_mode\": \"" << o.source_mode << "\",\n"
+    << "  \"cavities\": " << n_cavities << ",\n"
+    << "  \"gradient_parameters\": " << 4 * n_cavities << ",\n"
+    << "  \"cavity_radius\": " << o.cavity_radius << ",\n"
+    << "  \"sites\": " << n << ",\n"
+    << "  \"walks_per_site\": " << o.walks << ",\n"
+    << "  \"seed\": " << o.seed << ",\n"
+    << "  \"epsilon\": " << o.epsilon << ",\n"
+    << "  \"normal_offset\": " << o.diff_offset << ",\n"
+    << "  \"seam_strip_multiplier\": 1,\n"
+    << "  \"runtime_seconds_median\": " << runtime << ",\n"
+    << "  \"outer_mean_steps\": " << double(c[0]) / paths << ",\n"
+    << "  \"nested_mean_steps_per_outer_path\": " << double(c[1]) / paths << ",\n"
+    << "  \"cavity_termination_fraction\": " << double(c[2]) / paths << ",\n"
+    << "  \"near_seam_fraction\": " << double(c[3]) / paths << ",\n"
+    << "  \"segment_unsafe_fraction\": " << double(c[4]) / paths << ",\n"
+    << "  \"unsafe_offset_fraction\": " << double(c[5]) / paths << ",\n"
+    << "  \"unsafe_given_cavity_fraction\": " << (c[2] ? double(c[5]) / double(c[2]) : 0.0) << ",\n"
+    << "  \"near_seam_terminations\": " << c[3] << ",\n"
+    << "  \"segment_unsafe_terminations\": " << c[4] << ",\n"
+    << "  \"unsafe_offset_terminations\": " << c[5] << ",\n"
+    << "  \"distance_queries\": " << c[7] << ",\n"
+    << "  \"parity_rays\": " << c[8] << ",\n"
+    << "  \"mesh_build_seconds\": " << build_s << ",\n"
+    << "  \"peak_rss_megabytes\": " << wih::peak_rss_megabytes() << ",\n"
+    << "  \"timings_seconds\": [";
+  for (size_t i = 0; i < times.size(); ++i) s << (i ? ", " : "") << times[i];
+  s << "]\n}\n";
+  std::cout << "MI355X Differential WoS: " << n << " sites x " << o.walks << " walks, median " << runtime
+            << " s\n";
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
   try {
     const Options o = wih::parse_options(argc, argv, "wih_twos");
-    if (o.mode != "twos") wih::usage("wih_twos", "mode " + o.mode + " is not provided by this build (twos only)");
+    if (o.mode != "twos" && o.mode != "diffwos")
+      wih::usage("wih_twos", "mode " + o.mode + " is not provided by this build (twos and diffwos only)");
+    if (o.mode == "diffwos" && o.cavities_path.empty()) wih::usage("wih_twos", "Differential WoS requires --cavities");
     const wih::PlyMesh mesh = wih::load_ascii_ply(o.mesh_path);
     const std::vector<double> points = wih::read_points(o.points_path);
     const std::vector<double> cavities = wih::read_optional_points(o.cavities_path);
@@ -144,6 +226,8 @@ int main(int argc, char **argv) {
     d.cavity_radius = o.cavity_radius;
     wih::check(twos_geometry_create(ctx.get(), &d, &geom), "geometry");
     const twos_side side = make_side(o);
+    if (o.mode == "diffwos")
+      return run_differential(o, ctx.get(), geom, side, mesh, points, cavities.size() / 3, build_ms * 1e-3);
     const twos_side sides[2] = {side, side};
     std::vector<double> times;
     std::vector<twos_site_result> retained(n);

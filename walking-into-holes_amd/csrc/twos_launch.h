@@ -57,5 +57,9 @@ hipError_t launch_walk_persistent_f64(int geom_kind, bool count, const Tables &t
                                       int cus, hipStream_t stream, int *grid_out);
 hipError_t launch_reduce(const Tables &tb, const WalkParams &wp, int grid, hipStream_t stream);
 hipError_t launch_zero(unsigned long long *p, int n, hipStream_t stream);
+// Differential WoS over one chunk of sites (p.site_base, n_sites sites):
+// the per-job walks, then the per-(site, cavity) fold.
+hipError_t launch_differential(const DevGeom *d_geom, const DevSide *d_side, const DiffParams &p, int32_t n_sites,
+                               int32_t n_cav, hipStream_t stream);
 
 }  // namespace twos

@@ -1414,6 +1414,75 @@ twos_status twos_trace(twos_ctx *ctx, int32_t geom_id, const twos_side *side, co
   return TWOS_OK;
 }
 
+twos_status twos_differential(twos_ctx *ctx, int32_t geom_id, const twos_side *side, const twos_params *params,
+                              const double *sites, int64_t n, double ell, double *out, int64_t *counts) {
+  if (!ctx || !side || !params || (n > 0 && (!sites || !out))) return fail(TWOS_ERR_INVALID_ARGUMENT, "null argument");
+  const twos_params &p = *params;
+  if (n < 0 || p.walks < 1 || p.max_steps < 1 || !(p.eps > 0.0) || !(ell > 0.0) || !std::isfinite(ell))
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "invalid numeric option");
+  if (p.precision != 64) return fail(TWOS_ERR_INVALID_ARGUMENT, "the differential estimator runs in binary64");
+  if (twos_status sst = check_side(*side)) return sst;
+  for (int64_t i = 0; i < 3 * n; ++i)
+    if (!std::isfinite(sites[i])) return fail(TWOS_ERR_INVALID_ARGUMENT, "non-finite site coordinate");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  auto it = ctx->geoms.find(geom_id);
+  if (it == ctx->geoms.end()) return fail(TWOS_ERR_NOT_FOUND, "unknown geometry id");
+  const GeomEntry &ge = it->second;
+  if (ge.desc.container != kMesh || ge.dev.n_spheres != 0 || ge.dev.n_cylinders != 0)
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "the differential estimator needs a mesh container with cavities only");
+  const int32_t n_cav = ge.dev.n_cavities;
+  if (n_cav > 0 && !(ge.desc.cavity_radius > 0.0))
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "the differential estimator needs a positive cavity radius");
+  if (n == 0) return TWOS_OK;
+  HIP_OK(hipSetDevice(ctx->device));
+  // Per-job records take 56 bytes; chunks of sites keep them within 1/4 of
+  // the context's budget (and at most 2^22 jobs).
+  const int64_t job_cap = std::max<int64_t>(p.walks, std::min<int64_t>(int64_t(1) << 22, int64_t(ctx->budget / 4 / 56)));
+  const int64_t chunk = std::max<int64_t>(1, job_cap / p.walks);
+  const int64_t jobs_max = std::min(n, chunk) * p.walks;
+  struct Buffers {
+    double *sites = nullptr, *val = nullptr, *out = nullptr;
+    int32_t *rec = nullptr;
+    int64_t *counts = nullptr;
+    DevSide *side = nullptr;
+    ~Buffers() { dfree(sites); dfree(val); dfree(out); dfree(rec); dfree(counts); dfree(side); }
+  } b;
+  HIP_OK(dalloc(&b.sites, 3 * size_t(n)));
+  HIP_OK(dalloc(&b.val, 4 * size_t(jobs_max)));
+  HIP_OK(dalloc(&b.rec, 6 * size_t(jobs_max)));
+  HIP_OK(dalloc(&b.out, 8 * size_t(n) * size_t(n_cav)));
+  HIP_OK(dalloc(&b.counts, 9 * size_t(n)));
+  HIP_OK(dalloc(&b.side, 1));
+  HIP_OK(hipMemcpy(b.sites, sites, sizeof(double) * 3 * size_t(n), hipMemcpyHostToDevice));
+  const DevSide dside = *side;
+  HIP_OK(hipMemcpy(b.side, &dside, sizeof(DevSide), hipMemcpyHostToDevice));
+  DiffParams dp{};
+  dp.sites = b.sites;
+  dp.val = b.val;
+  dp.rec = b.rec;
+  dp.out = b.out;
+  dp.counts = b.counts;
+  dp.walks = p.walks;
+  dp.site_offset = p.site_offset;
+  dp.seed = p.seed;
+  dp.repeat = p.repeat;
+  dp.max_steps = p.max_steps;
+  dp.eps = p.eps;
+  dp.ell = ell;
+  for (int64_t s0 = 0; s0 < n; s0 += chunk) {
+    const int32_t cnt = int32_t(std::min(chunk, n - s0));
+    dp.site_base = s0;
+    dp.jobs = int64_t(cnt) * p.walks;
+    HIP_OK(launch_differential(ctx->d_geoms + geom_id, b.side, dp, cnt, n_cav, ctx->stream));
+  }
+  HIP_OK(hipMemcpyAsync(out, b.out, sizeof(double) * 8 * size_t(n) * size_t(n_cav), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  if (counts)
+    HIP_OK(hipMemcpyAsync(counts, b.counts, sizeof(int64_t) * 9 * size_t(n), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_OK(hipStreamSynchronize(ctx->stream));
+  return TWOS_OK;
+}
+
 twos_status twos_server_create(twos_ctx *ctx, int64_t max_sites, int64_t max_walks_per_round, twos_server **out) {
   if (!ctx || !out || max_sites < 1 || max_walks_per_round < 64)
     return fail(TWOS_ERR_INVALID_ARGUMENT, "invalid server capacity");

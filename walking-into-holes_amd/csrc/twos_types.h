@@ -107,4 +107,22 @@ struct WalkParams {
   double *grad;                  // [total walks][2][3] gradient samples, or null
 };
 
+// One launch of the differential kernels: jobs = sites of the chunk x walks.
+struct DiffParams {
+  const double *sites;   // [n][3] all sites of the request
+  double *val;           // [jobs][4] per-job normal and V / ell
+  int32_t *rec;          // [jobs][6] per-job cavity, flags and counters
+  double *out;           // [n][n_cavities][8] folded derivatives
+  int64_t *counts;       // [n][9] per-site counters
+  int64_t jobs;
+  int64_t walks;
+  int64_t site_base;     // first site of the chunk
+  int64_t site_offset;   // added to the site index in the key
+  uint64_t seed;
+  int32_t repeat;
+  int32_t max_steps;
+  double eps;
+  double ell;            // offset length
+};
+
 }  // namespace twos

@@ -280,7 +280,7 @@ class Context:
         _check(_lib().twos_create(device, hbm_budget, C.byref(h)))
         self._h = h
         self.device = device
-        self._geoms: dict[int, dict] = {}
+        self._geom_cavities: dict[int, tuple] = {}  # geometry id -> (cavity count, cavity radius)
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -348,6 +348,7 @@ class Context:
                 setattr(d, cnt, len(a))
         gid = C.c_int32()
         _check(_lib().twos_geometry_create(self._h, C.byref(d), C.byref(gid)))
+        self._geom_cavities[gid.value] = (int(d.n_cavities), float(cavity_radius))
         return gid.value
 
     def union_sphere(self, geom: int, center, radius: float) -> int:
@@ -356,10 +357,15 @@ class Context:
         c = _f64(center)
         gid = C.c_int32()
         _check(_lib().twos_geometry_union_sphere(self._h, geom, _ptr(c, _capi._dp), radius, C.byref(gid)))
+        # A sphere of the base's cavity radius (or the first one) joins its
+        # cavities; another radius becomes a sphere hole.
+        k, r = self._geom_cavities.get(geom, (0, 0.0))
+        self._geom_cavities[gid.value] = (k + 1, radius) if k == 0 or r == radius else (k, r)
         return gid.value
 
     def release_geometry(self, geom: int) -> None:
         _check(_lib().twos_geometry_release(self._h, geom))
+        self._geom_cavities.pop(geom, None)
 
     def release_mesh(self, mesh: Mesh | int) -> None:
         _check(_lib().twos_mesh_release(self._h, mesh.id if isinstance(mesh, Mesh) else int(mesh)))
@@ -406,6 +412,28 @@ class Context:
         _check(_lib().twos_trace(self._h, geom, C.byref(side_), C.byref(prm), _ptr(s, _capi._dp), site, walk,
                                  _ptr(rows, _capi._dp), len(rows), C.byref(n)))
         return rows[:n.value].copy()
+
+    # Per-site counters of differential(), in column order.
+    DIFF_COUNTS = ("outer_steps", "nested_steps", "cavity_terminations", "near_seam", "segment_unsafe", "unsafe",
+                   "limits", "distance_queries", "parity_rays")
+
+    def differential(self, geom: int, sites, side_: Side, *, seed=0, repeat=0, walks=16, eps=1e-4,
+                     max_steps=2048, offset=0.004, site_offset=0):
+        """Differential WoS cavity derivatives (evaluate_differential_site,
+        wih_twos.cpp:725-928) on the reference's mt19937_64 streams, binary64:
+        (values [n][n_cavities][8] = centre derivative x y z, their SEs, radius
+        derivative, its SE; counts [n][9] in DIFF_COUNTS order)."""
+        p = _f64(sites).reshape(-1, 3)
+        n = len(p)
+        k = self._geom_cavities.get(geom, (0, 0.0))[0]
+        out = np.zeros((n, max(k, 1), 8))
+        counts = np.zeros((n, 9), dtype=np.int64)
+        prm = _params(rng=RNG_MT_KEY, precision=64, seed=seed, repeat=repeat, eps=eps, max_steps=max_steps,
+                      walks=walks, schedule=PLAIN, site_offset=site_offset)
+        _check(_lib().twos_differential(self._h, geom, C.byref(side_), C.byref(prm), _ptr(p, _capi._dp), n,
+                                        float(offset), _ptr(out, _capi._dp),
+                                        counts.ctypes.data_as(C.POINTER(C.c_int64))))
+        return out[:, :k], counts
 
     # -- the estimator ------------------------------------------------------
     def estimate(self, geom: int, sites, sides, *, combine=SQUARE, rng=RNG_PHILOX, precision=32,

@@ -100,6 +100,8 @@ SIGNATURES = {
     "twos_inside": (C.c_int32, [_ctx_p, C.c_int32, _dp, C.c_int64, _i32p]),
     "twos_closest": (C.c_int32, [_ctx_p, C.c_int32, C.c_int32, _dp, C.c_int64, _dp, _dp, _i32p]),
     "twos_distance_bound": (C.c_int32, [_ctx_p, C.c_int32, C.c_int32, _dp, C.c_int64, _dp]),
+    "twos_differential": (C.c_int32, [_ctx_p, C.c_int32, C.POINTER(Side), C.POINTER(Params), _dp, C.c_int64,
+                                      C.c_double, _dp, C.POINTER(C.c_int64)]),
     "twos_trace": (C.c_int32, [_ctx_p, C.c_int32, C.POINTER(Side), C.POINTER(Params), _dp, C.c_int64,
                                C.c_int32, _dp, C.c_int32, _i32p]),
     "twos_server_create": (C.c_int32, [_ctx_p, C.c_int64, C.c_int64, C.POINTER(_srv_p)]),
