@@ -18,6 +18,11 @@ __device__ inline bool ray_hit(const MeshView &m, V3<double> o, V3<double> d, do
   int32_t node = 0;
   if (m.n_tris == 1) node = ~0;
   int32_t visits = 2 * m.n_tris + 8;
+  // Slab test with the reciprocal direction (one division per axis per ray,
+  // not per box).  The boxes are outward-rounded and the slab interval is
+  // padded by 1e-12, far above the reciprocal's rounding (about 1e-14 at
+  // t = 30), so no box that holds a hit is culled and hits are unchanged.
+  const V3<double> inv{d.x != 0.0 ? 1.0 / d.x : 0.0, d.y != 0.0 ? 1.0 / d.y : 0.0, d.z != 0.0 ? 1.0 / d.z : 0.0};
   while (--visits >= 0) {
     if (node >= 0) {
       const float4 q0 = m.nodes[4 * node + 0], q1 = m.nodes[4 * node + 1], q2 = m.nodes[4 * node + 2],
@@ -29,13 +34,13 @@ __device__ inline bool ray_hit(const MeshView &m, V3<double> o, V3<double> d, do
         double t0 = tnear, t1 = best;
         bool miss = false;
         for (int k = 0; k < 3 && !miss; ++k) {
-          const double ok = comp(o, k), dk = comp(d, k);
+          const double ok = comp(o, k), dk = comp(d, k), ik = comp(inv, k);
           const double lo = bx[c][k], hi = bx[c][3 + k];
           if (dk == 0.0) {
             if (ok < lo || ok > hi) miss = true;
             continue;
           }
-          double ta = (lo - ok) / dk, tb = (hi - ok) / dk;
+          double ta = (lo - ok) * ik, tb = (hi - ok) * ik;
           if (ta > tb) { const double s = ta; ta = tb; tb = s; }
           t0 = fmax(t0, ta - 1e-12);
           t1 = fmin(t1, tb + 1e-12);
