@@ -107,6 +107,11 @@ def lib() -> C.CDLL:
                                       C.POINTER(C.c_int64)]
         L.og_differential.restype = C.c_int32
         L.og_bump_green_ball.restype = C.c_double
+        L.og_certify.argtypes = [C.c_void_p, C.c_void_p, _dp, C.c_int64, _dp, C.POINTER(C.c_int32),
+                                 C.POINTER(C.c_int32)]
+        L.og_certify.restype = C.c_int32
+        L.og_cert_distance.argtypes = [C.c_void_p, _dp, C.c_int64, _dp]
+        L.og_cert_distance.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -350,6 +355,45 @@ def differential(geometry: Geometry, sites, side_, *, seed=0, repeat=0, walks=16
     if st == 2:
         raise ValueError("invalid differential arguments")
     return out[:, :k], counts
+
+
+class CertParams(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("walks", C.c_int32), ("max_steps", C.c_int32), ("eps", C.c_double),
+                ("radius_cap", C.c_double), ("value_cap", C.c_double), ("slope", C.c_double),
+                ("source", C.c_int32), ("threads", C.c_int32), ("site_offset", C.c_int64)]
+
+
+CERT_AFFINE, CERT_HOUSING = 0, 1
+
+
+def certify(geometry: Geometry, sites, *, seed=0, walks=1024, max_steps=128, eps=1e-6, radius_cap=0.1,
+            value_cap=0.1, source=CERT_AFFINE, slope=0.65, threads=0, site_offset=0):
+    """Certified minimum-benefit samples (cert_walk, minimum_benefit.hpp:122-143)
+    on the reference's streams: (value, steps, flags) as [n][walks] arrays,
+    flags 1 capped, 2 clipped, 4 containment failure."""
+    sites = _f64(np.atleast_2d(sites))
+    n = len(sites)
+    g = geometry.struct()
+    p = CertParams(seed=seed, walks=walks, max_steps=max_steps, eps=eps, radius_cap=radius_cap,
+                   value_cap=value_cap, slope=slope, source=source, threads=threads, site_offset=site_offset)
+    value = np.zeros((n, walks))
+    steps = np.zeros((n, walks), dtype=np.int32)
+    flags = np.zeros((n, walks), dtype=np.int32)
+    st = lib().og_certify(C.byref(g), C.byref(p), _ptr(sites, _dp), n, _ptr(value, _dp),
+                          steps.ctypes.data_as(C.POINTER(C.c_int32)), flags.ctypes.data_as(C.POINTER(C.c_int32)))
+    if st == 2:
+        raise ValueError("invalid certification arguments")
+    return value, steps, flags
+
+
+def cert_distance(geometry: Geometry, points) -> np.ndarray:
+    """Certified lower bound of the domain distance (cert_domain_distance)."""
+    pts = _f64(np.atleast_2d(points))
+    out = np.zeros(len(pts))
+    g = geometry.struct()
+    if lib().og_cert_distance(C.byref(g), _ptr(pts, _dp), len(pts), _ptr(out, _dp)) == 2:
+        raise ValueError("invalid geometry")
+    return out
 
 
 def bump_green_ball(x, radius: float, center, support: float, amplitude: float) -> float:

@@ -997,6 +997,139 @@ void differential_site(const og_geom &g, const og_side &side, const og_params &p
   }
 }
 
+// ---------------------------------------------------------------------------
+// Certified minimum-benefit sampling (benchmarks/compiled_bunny/src/
+// minimum_benefit.hpp): fail-closed walks whose every geometric and source
+// quantity is rounded toward the safe side.  The distance is the union of the
+// triangles' bounding boxes (a lower bound of the surface distance), each box
+// distance rounded down; a sample adds a rounded-down lower bound of the
+// source over each step's Green point and stops at the value cap.
+
+double down_(double x) { return std::nextafter(x, -kInf); }
+double up_(double x) { return std::nextafter(x, kInf); }
+double positive_down(double x) { return std::max(0.0, down_(x)); }
+
+// box_distance_lower (minimum_benefit.hpp:18-25).
+double cert_box_lower(const double lo[3], const double hi[3], const V3 &x) {
+  double sum = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const double g = std::max({0.0, down_(lo[a] - comp(x, a)), down_(comp(x, a) - hi[a])});
+    sum = positive_down(sum + positive_down(g * g));
+  }
+  return positive_down(std::sqrt(sum));
+}
+
+// point_distance_bound (:26-35).
+double cert_point_bound(const V3 &a, const V3 &b, bool upper) {
+  double sum = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    double d = std::fabs(comp(a, k) - comp(b, k));
+    d = upper ? up_(d) : positive_down(d);
+    const double q = upper ? up_(d * d) : positive_down(d * d);
+    sum = upper ? up_(sum + q) : positive_down(sum + q);
+  }
+  return upper ? up_(std::sqrt(sum)) : positive_down(std::sqrt(sum));
+}
+
+// The smallest rounded-down box distance over all triangles
+// (CertDistanceTree::distance, :39-81).  The box bound is monotone under box
+// containment, so pruning a node whose (exact union) box is already no
+// closer than the best leaves the minimum unchanged: the result does not
+// depend on the tree.
+double cert_mesh_distance(const og_mesh &m, const V3 &p) {
+  double best = kInf;
+  int32_t stack[128];
+  int sp = 0;
+  stack[sp++] = 0;
+  while (sp) {
+    const og_mesh::Node &n = m.nodes[size_t(stack[--sp])];
+    if (cert_box_lower(n.lo, n.hi, p) >= best) continue;
+    if (n.right < 0) {
+      for (int32_t i = 0; i < -n.right; ++i) {
+        double lo[3], hi[3];
+        m.bounds_of(m.order[size_t(n.left + i)], lo, hi);
+        best = std::min(best, cert_box_lower(lo, hi, p));
+      }
+      continue;
+    }
+    stack[sp++] = n.right;
+    stack[sp++] = n.left;
+  }
+  return best;
+}
+
+// cert_domain_distance (:83-88).
+double cert_domain_distance(const og_geom &g, const V3 &p) {
+  double d = cert_mesh_distance(*g.mesh, p);
+  for (int32_t i = 0; i < g.n_cavities; ++i)
+    d = std::min(d, down_(cert_point_bound(p, load3(g.cavities + 3 * i), false) - g.cavity_radius));
+  return std::max(0.0, d);
+}
+
+// (1 - t/1024)^1024 <= exp(-t), every operation rounded down (:92-96).
+double exp_negative_lower(double t_upper) {
+  double a = std::max(0.0, down_(1.0 - up_(t_upper / 1024.0)));
+  for (int i = 0; i < 10; ++i) a = positive_down(a * a);
+  return a;
+}
+
+// cert_source_lower (:97-116): the affine source 1 + slope (y + 0.5)
+// (slope >= 0) or the housing chips, as lower bounds.
+double cert_source_lower(const V3 &p, int32_t source, double slope) {
+  if (source == 0) return std::max(0.0, down_(1.0 + down_(slope * down_(p.y + 0.5))));
+  static const double centers[3][3] = {{-.48, -.18, -.08}, {.43, .19, -.07}, {.02, -.31, -.02}};
+  static const double widths[3][3] = {{.24, .22, .16}, {.21, .20, .15}, {.30, .16, .19}};
+  static const double amplitude[3] = {5.2, 4.6, 2.0};
+  double value = .08;
+  for (int j = 0; j < 3; ++j) {
+    double exponent = 0.0;
+    for (int a = 0; a < 3; ++a) {
+      const double absdiff = up_(std::fabs(comp(p, a) - centers[j][a]));
+      const double q = up_(absdiff / widths[j][a]);
+      exponent = up_(exponent + up_(q * q));
+    }
+    value = down_(value + positive_down(amplitude[j] * exp_negative_lower(exponent)));
+  }
+  return std::max(0.0, value);
+}
+
+// cert_walk (:122-143).  flags: 1 capped, 2 clipped, 4 containment failure
+// (the walk stops there; its record must not be used).
+double cert_walk(const og_geom &g, const og_cert_params &p, V3 point, uint64_t key, int32_t &steps,
+                 int32_t &flags) {
+  std::mt19937_64 rng(splitmix64(key));
+  double value = 0.0;
+  steps = 0;
+  flags = 0;
+  for (int32_t k = 0; k < p.max_steps; ++k) {
+    const double d = cert_domain_distance(g, point);
+    if (d <= p.eps) return value;
+    const double s = std::min(p.radius_cap, positive_down(0.95 * d));
+    V3 y = point;
+    if (p.source == 1) {
+      // random_unit_vector(rng) * (s * beta_22(rng)): the Beta draw first.
+      const double t = beta_22(rng);
+      y = point + random_unit_vector(rng) * (s * t);
+    }
+    const V3 next = point + random_unit_vector(rng) * s;
+    if (!(cert_point_bound(point, next, true) < d) || !(cert_point_bound(point, y, true) < d)) {
+      flags |= 4;
+      return value;
+    }
+    const double weight = positive_down(positive_down(s * s) / 6.0);
+    const double increment = positive_down(weight * cert_source_lower(y, p.source, p.slope));
+    value = positive_down(value + increment);
+    ++steps;
+    if (value >= p.value_cap) {
+      flags |= 2;
+      return p.value_cap;
+    }
+    point = next;
+  }
+  flags |= 1;
+  return value;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1020,6 +1153,34 @@ int32_t og_differential(const og_geom *g, const og_side *side, const og_params *
                             c.limits, c.queries, c.rays};
     for (int k = 0; k < 9; ++k) counts[9 * s + k] = row[k];
   }
+  return 0;
+}
+
+int32_t og_certify(const og_geom *g, const og_cert_params *prm, const double *sites, int64_t n, double *value,
+                   int32_t *steps, int32_t *flags) {
+  if (!g || g->container != OG_MESH || !g->mesh || prm->walks < 1 || prm->max_steps < 1 ||
+      !(prm->radius_cap > 0.0) || !(prm->value_cap > 0.0) || (prm->source != 0 && prm->source != 1) ||
+      (prm->source == 0 && prm->slope < 0.0))
+    return 2;
+  const og_cert_params p = *prm;
+  const int64_t jobs = n * int64_t(p.walks);
+#ifdef _OPENMP
+  if (p.threads > 0) omp_set_num_threads(p.threads);
+#pragma omp parallel for schedule(dynamic, 64)
+#endif
+  for (int64_t t = 0; t < jobs; ++t) {
+    const int64_t site = t / p.walks, j = t - site * p.walks;
+    // Key of sample j at site (minimum_benefit.hpp:168).
+    const uint64_t key = splitmix64(p.seed) ^ splitmix64(uint64_t(p.site_offset + site) + 0xa1224ULL) ^
+                         splitmix64(uint64_t(j) + 0x6242cULL);
+    value[t] = cert_walk(*g, p, load3(sites + 3 * site), key, steps[t], flags[t]);
+  }
+  return 0;
+}
+
+int32_t og_cert_distance(const og_geom *g, const double *points, int64_t n, double *out) {
+  if (!g || g->container != OG_MESH || !g->mesh) return 2;
+  for (int64_t i = 0; i < n; ++i) out[i] = cert_domain_distance(*g, load3(points + 3 * i));
   return 0;
 }
 

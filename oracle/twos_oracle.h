@@ -19,6 +19,8 @@
  *       the reference field fixtures' score / score_se columns.
  *   harmonic Dirichlet payoff: unit-ball closed forms (u = g for harmonic g).
  *   ray-parity inside test: analytic ball / star-shaped meshes.
+ *   og_certify / og_cert_distance: the retained --mode validate samples of
+ *       the minimum-benefit pilot (every value, step count and flag).
  *   og_differential: the retained wih_study --diff 1 requests on the branch
  *       mesh (request totals of steps, distance queries and parity rays
  *       exactly; derivative columns, most sites bitwise).
@@ -142,6 +144,27 @@ void og_combine_powersums(const double sums[8], int64_t n, double out[6]);
  * parity rays. */
 int32_t og_differential(const og_geom *g, const og_side *side, const og_params *prm, const double *sites, int64_t n,
                         double ell, double *out, int64_t *counts);
+/* Certified minimum-benefit sampling (run_minimum_benefit / cert_walk,
+ * benchmarks/compiled_bunny/src/minimum_benefit.hpp:122-188) on a mesh with
+ * cavities: per site and sample j the clipped value, its steps and flags
+ * (1 capped, 2 clipped, 4 containment failure), [n][walks] each. */
+typedef struct {
+  uint64_t seed;
+  int32_t walks;
+  int32_t max_steps;
+  double eps;
+  double radius_cap;   /* --validation-radius-cap */
+  double value_cap;    /* --validation-value-cap */
+  double slope;        /* affine source slope (source 0) */
+  int32_t source;      /* 0 affine 1 + slope (y + 0.5), 1 housing chips */
+  int32_t threads;
+  int64_t site_offset; /* global index of sites[0] in the keys */
+} og_cert_params;
+int32_t og_certify(const og_geom *g, const og_cert_params *prm, const double *sites, int64_t n, double *value,
+                   int32_t *steps, int32_t *flags);
+/* The certified lower bound of the domain distance (cert_domain_distance,
+ * minimum_benefit.hpp:83-88). */
+int32_t og_cert_distance(const og_geom *g, const double *points, int64_t n, double *out);
 /* Exact integral of the Green function of B(x, radius) against the bump
  * amplitude (1 - |y - c|^2 / support^2)^2 (source_kind 3; study.cpp:56-76). */
 double og_bump_green_ball(const double x[3], double radius, const double c[3], double support, double amplitude);

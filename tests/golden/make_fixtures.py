@@ -209,6 +209,50 @@ def diff_fixtures(root: Path) -> None:
     (HERE / "study_branch_diff.json").write_text(json.dumps({"requests": requests}, indent=1) + "\n")
 
 
+def cert_fixtures(root: Path) -> None:
+    """Certified minimum-benefit validations (wih_twos --mode validate) of the
+    pilot in benchmarks/minimum_benefit/pilot-floor-policy/{bunny,housing}/
+    attempt-*/: the finalist (validation-points.csv), the cavities accepted
+    before it (validation-holes.csv), the first 2,048 of its 65,536 samples
+    (validation.csv: value, steps, capped, clipped) and the per-stage summary
+    (validation.json).  Seeds follow the runner (CONFIG seed 2026091601 +
+    100 step + 2, runner_snapshot_after_arithmetic_cleanup.py:213 with
+    configuration.json), as do the caps: radius cap 0.1, value cap 0.06 (bunny,
+    the 69,674-face watertight bunny = bunny_69674.npz, affine source slope
+    0.65) or 0.18 (housing, the housing mesh, its chips), eps 1e-6, 128 steps."""
+    base = root / "benchmarks" / "minimum_benefit" / "pilot-floor-policy"
+    config = json.loads((base / "configuration.json").read_text())
+    arrays, attempts = {}, []
+    k = 0
+    for kind in ("bunny", "housing"):
+        for d in sorted((base / kind).glob("attempt-*")):
+            step = int(d.name.split("-")[1])
+            decision = json.loads((d / "decision.json").read_text())
+            holes_path = d / "validation-holes.csv"
+            holes = (np.loadtxt(holes_path, delimiter=",", ndmin=2).reshape(-1, 3) if holes_path.stat().st_size
+                     else np.zeros((0, 3)))
+            with (d / "validation.csv").open() as handle:
+                rows = list(csv.DictReader(handle))
+            assert len(rows) == 65536 and all(int(r["site"]) == 0 for r in rows)
+            head = rows[:2048]
+            summary = json.loads((d / "validation.json").read_text())
+            arrays[f"{k}_point"] = np.loadtxt(d / "validation-points.csv", delimiter=",", ndmin=2).reshape(-1, 3)
+            arrays[f"{k}_holes"] = holes
+            arrays[f"{k}_value"] = np.array([float(r["value"]) for r in head])
+            arrays[f"{k}_steps"] = np.array([int(r["steps"]) for r in head], dtype=np.int32)
+            arrays[f"{k}_capped"] = np.array([int(r["capped"]) for r in head], dtype=np.int8)
+            arrays[f"{k}_clipped"] = np.array([int(r["clipped"]) for r in head], dtype=np.int8)
+            attempts.append({"source": str(d.relative_to(root)), "kind": kind, "step": step,
+                             "seed": config["seed"] + 100 * step + 2, "radius": decision["radius"],
+                             "value_cap": config["value_caps"][kind], "radius_cap": config["radius_cap"],
+                             "epsilon": config["epsilon"], "max_steps": config["max_steps"], "walks": 65536,
+                             "slope": 0.65, "stages": summary["stages"],
+                             "containment_failures": summary["containment_failures"]})
+            k += 1
+    np.savez_compressed(HERE / "cert_requests.npz", **arrays)
+    (HERE / "cert_meta.json").write_text(json.dumps({"attempts": attempts}, indent=1) + "\n")
+
+
 def housing_fixtures(root: Path) -> None:
     """Scout and validation fields of the engineering-housing design loop
     (benchmarks/engineering_housing/output/field-rep-1-*.csv with their
@@ -284,3 +328,4 @@ if __name__ == "__main__":
     housing_fixtures(Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference"))
     design_fixture(Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference"))
     diff_fixtures(Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference"))
+    cert_fixtures(Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference"))
