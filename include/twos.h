@@ -306,6 +306,47 @@ twos_status twos_trace(twos_ctx *ctx, int32_t geom_id, const twos_side *side, co
 twos_status twos_differential(twos_ctx *ctx, int32_t geom_id, const twos_side *side, const twos_params *params,
                               const double *sites, int64_t n, double ell, double *out, int64_t *counts);
 
+/* Certified minimum-benefit sampling (replaces cert_walk / the sample loop of
+ * run_minimum_benefit, benchmarks/compiled_bunny/src/minimum_benefit.hpp:
+ * 122-188, wih_twos --mode validate).  Fail-closed walks, binary64 on the
+ * reference's mt19937_64 keys: the distance is the smallest rounded-down
+ * distance to a triangle's bounding box or a cavity, each step's radius
+ * min(radius_cap, 0.95 d rounded down), and each step adds a rounded-down
+ * lower bound of (r^2 / 6) f at the Green point; a sample stops at eps, at
+ * value_cap (clipped, value = value_cap), at max_steps (capped) or when a
+ * step's points cannot be shown inside the ball (containment failure: the
+ * record must not be used).  Sources: TWOS_CERT_AFFINE 1 + slope (y + 0.5)
+ * with slope >= 0, TWOS_CERT_HOUSING the built-in housing chips. */
+#define TWOS_CERT_AFFINE 0
+#define TWOS_CERT_HOUSING 1
+#define TWOS_CERT_CAPPED 1
+#define TWOS_CERT_CLIPPED 2
+#define TWOS_CERT_CONTAINMENT 4
+
+typedef struct {
+  uint64_t seed;
+  int32_t walks;         /* samples per site in this call */
+  int32_t max_steps;
+  double eps;
+  double radius_cap;     /* --validation-radius-cap */
+  double value_cap;      /* --validation-value-cap */
+  double slope;          /* TWOS_CERT_AFFINE slope (--source-y-slope) */
+  int32_t source;        /* TWOS_CERT_AFFINE or TWOS_CERT_HOUSING */
+  int32_t pad;
+  int64_t site_offset;   /* global index of sites[0] in the keys */
+  int64_t sample_begin;  /* index of the first sample of this call */
+} twos_cert_params;
+
+/* geom_id: a mesh container minus cavities only.  value, steps, flags:
+ * [n][walks] for samples sample_begin .. sample_begin + walks - 1 of each
+ * site (flags TWOS_CERT_*). */
+twos_status twos_certify(twos_ctx *ctx, int32_t geom_id, const twos_cert_params *params, const double *sites,
+                         int64_t n, double *value, int32_t *steps, int32_t *flags);
+/* The certified lower bound of the domain distance at each point
+ * (cert_domain_distance, minimum_benefit.hpp:83-88; the summary's
+ * center_clearance_lower). */
+twos_status twos_cert_distance(twos_ctx *ctx, int32_t geom_id, const double *points, int64_t n, double *out);
+
 /* Continuous-batching server: requests are admitted between rounds as device
  * memory allows, all admitted sites share each persistent launch, and each
  * result depends only on (seed, site, side, walk), alone or in the mix. */

@@ -1,0 +1,273 @@
+// Certified minimum-benefit sampling (run_minimum_benefit / cert_walk,
+// benchmarks/compiled_bunny/src/minimum_benefit.hpp:122-188): fail-closed
+// walks whose geometric and source quantities are rounded toward the safe
+// side, binary64 on the reference's mt19937_64 streams.
+//
+// Distance.  The certified distance is the smallest rounded-down distance
+// from the point to a triangle's bounding box (CertDistanceTree,
+// minimum_benefit.hpp:39-81), then the cavities' rounded-down distances.  The
+// per-box bound is monotone under box containment and every rounding in it
+// is monotone, so it is found here over the mesh's 4-wide BVH (whose
+// outward-rounded binary32 boxes contain the binary64 triangle boxes),
+// pruning a node whose bound is no better than the best so far: the result
+// is the minimum over all triangles, whatever the tree.  The square root is
+// monotone too, so the search compares rounded-down squared sums and takes
+// one root at the end.
+//
+// Work model.  One thread runs one (site, sample) walk and writes its value,
+// step count and flags (1 capped, 2 clipped, 4 containment failure).
+
+#include "twos_device.h"
+#include "twos_launch.h"
+
+namespace twos {
+
+namespace {
+
+// nextafter(x, -inf) and nextafter(x, +inf) on the binary64 pattern.
+__device__ __forceinline__ double dn(double x) {
+  if (x != x || x == -__builtin_huge_val()) return x;
+  if (x == 0.0) return -0x1p-1074;
+  const long long b = __double_as_longlong(x);
+  return __longlong_as_double(x > 0.0 ? b - 1 : b + 1);
+}
+__device__ __forceinline__ double upw(double x) {
+  if (x != x || x == __builtin_huge_val()) return x;
+  if (x == 0.0) return 0x1p-1074;
+  const long long b = __double_as_longlong(x);
+  return __longlong_as_double(x > 0.0 ? b + 1 : b - 1);
+}
+__device__ __forceinline__ double pdn(double x) { return fmax(0.0, dn(x)); }
+
+// The rounded-down squared sum of box_distance_lower (minimum_benefit.hpp:18-25)
+// before its root.
+__device__ __forceinline__ double box_sum_lower(double lx, double ly, double lz, double hx, double hy, double hz,
+                                                V3<double> p) {
+  const double gx = fmax(0.0, fmax(dn(lx - p.x), dn(p.x - hx)));
+  const double gy = fmax(0.0, fmax(dn(ly - p.y), dn(p.y - hy)));
+  const double gz = fmax(0.0, fmax(dn(lz - p.z), dn(p.z - hz)));
+  double sum = pdn(0.0 + pdn(gx * gx));
+  sum = pdn(sum + pdn(gy * gy));
+  return pdn(sum + pdn(gz * gz));
+}
+
+__device__ __forceinline__ double tri_sum_lower(const MeshView &m, int32_t k, V3<double> p) {
+  V3<double> a, b, c;
+  load_tri<double>(m, k, a, b, c);
+  return box_sum_lower(fmin(fmin(a.x, b.x), c.x), fmin(fmin(a.y, b.y), c.y), fmin(fmin(a.z, b.z), c.z),
+                       fmax(fmax(a.x, b.x), c.x), fmax(fmax(a.y, b.y), c.y), fmax(fmax(a.z, b.z), c.z), p);
+}
+
+// CertDistanceTree::distance: the smallest rounded-down triangle-box distance.
+__device__ double cert_mesh_distance(const MeshView &m, V3<double> p) {
+  double best = __builtin_huge_val();
+  if (m.n_tris == 1) return pdn(sqrt(tri_sum_lower(m, 0, p)));
+  int32_t stk_node[kStackDepth];
+  double stk_d[kStackDepth];
+  int sp = 0;
+  int32_t node = 0;
+  int32_t visits = m.n_tris + 8;  // 4-wide nodes < triangles; guards a corrupt tree
+  while (--visits >= 0) {
+    const float4 *q = m.nodes4 + size_t(kNode4Quads) * node;
+    const float4 lx = q[0], ly = q[1], lz = q[2], hx = q[3], hy = q[4], hz = q[5];
+    const int4 ch = *reinterpret_cast<const int4 *>(q + 6);
+    double d[4] = {box_sum_lower(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, p),
+                   box_sum_lower(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, p),
+                   box_sum_lower(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, p),
+                   box_sum_lower(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, p)};
+    int32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (c[i] < 0 && c[i] != kEmptyChild && d[i] < best) best = fmin(best, tri_sum_lower(m, ~c[i], p));
+    double key[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) key[i] = (c[i] >= 0 && d[i] < best) ? d[i] : __builtin_huge_val();
+    auto cswap = [&](int i, int j) {
+      if (key[j] < key[i]) {
+        const double tk = key[i]; key[i] = key[j]; key[j] = tk;
+        const int32_t tc = c[i]; c[i] = c[j]; c[j] = tc;
+      }
+    };
+    cswap(0, 1); cswap(2, 3); cswap(0, 2); cswap(1, 3); cswap(1, 2);
+    const double inf = __builtin_huge_val();
+    if (key[0] < inf) {
+#pragma unroll
+      for (int i = 3; i >= 1; --i)
+        if (key[i] < inf && sp < kStackDepth) {
+          stk_node[sp] = c[i];
+          stk_d[sp] = key[i];
+          ++sp;
+        }
+      node = c[0];
+      continue;
+    }
+    bool found = false;
+    while (sp > 0) {
+      --sp;
+      if (stk_d[sp] < best) {
+        node = stk_node[sp];
+        found = true;
+        break;
+      }
+    }
+    if (!found) break;
+  }
+  return pdn(sqrt(best));
+}
+
+// point_distance_bound (minimum_benefit.hpp:26-35).
+__device__ __forceinline__ double point_bound(V3<double> a, V3<double> b, bool upper) {
+  double sum = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double d = fabs(comp(a, k) - comp(b, k));
+    d = upper ? upw(d) : pdn(d);
+    const double q = upper ? upw(d * d) : pdn(d * d);
+    sum = upper ? upw(sum + q) : pdn(sum + q);
+  }
+  return upper ? upw(sqrt(sum)) : pdn(sqrt(sum));
+}
+
+// cert_domain_distance (:83-88).
+__device__ double cert_domain_distance(const DevGeom &g, V3<double> p) {
+  const MeshView m{g.nodes, g.nodes4, g.tris_f, g.tris_d, g.tri_ids, g.n_tris};
+  double d = cert_mesh_distance(m, p);
+  for (int32_t i = 0; i < g.n_cavities; ++i) {
+    const Vec4<double> s = load4<double>(g, 2, i);
+    d = fmin(d, dn(point_bound(p, V3<double>{s.x, s.y, s.z}, false) - g.cavity_radius));
+  }
+  return fmax(0.0, d);
+}
+
+// (1 - t/1024)^1024 <= exp(-t), every operation rounded down (:92-96).
+__device__ __forceinline__ double exp_negative_lower(double t_upper) {
+  double a = fmax(0.0, dn(1.0 - upw(t_upper / 1024.0)));
+  for (int i = 0; i < 10; ++i) a = pdn(a * a);
+  return a;
+}
+
+// cert_source_lower (:97-116).
+__device__ double cert_source_lower(V3<double> p, int32_t source, double slope) {
+  if (source == TWOS_CERT_AFFINE) return fmax(0.0, dn(1.0 + dn(slope * dn(p.y + 0.5))));
+  const double centers[3][3] = {{-.48, -.18, -.08}, {.43, .19, -.07}, {.02, -.31, -.02}};
+  const double widths[3][3] = {{.24, .22, .16}, {.21, .20, .15}, {.30, .16, .19}};
+  const double amplitude[3] = {5.2, 4.6, 2.0};
+  double value = .08;
+  for (int j = 0; j < 3; ++j) {
+    double exponent = 0.0;
+    for (int a = 0; a < 3; ++a) {
+      const double absdiff = upw(fabs(comp(p, a) - centers[j][a]));
+      const double q = upw(absdiff / widths[j][a]);
+      exponent = upw(exponent + upw(q * q));
+    }
+    value = dn(value + pdn(amplitude[j] * exp_negative_lower(exponent)));
+  }
+  return fmax(0.0, value);
+}
+
+struct CertLaunch {
+  const double *sites;
+  double *value;
+  int32_t *steps;
+  int32_t *flags;
+  int64_t jobs;
+  int64_t count;        // samples per site in this launch
+  int64_t sample_begin;
+  int64_t site_offset;
+  uint64_t seed;
+  int32_t max_steps;
+  int32_t source;
+  double eps, radius_cap, value_cap, slope;
+};
+
+// cert_walk (:122-143), sample j = sample_begin + job % count of site
+// job / count, key (:168).
+__global__ __launch_bounds__(64) void cert_walk(const DevGeom *geom, CertLaunch p) {
+  const int64_t job = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (job >= p.jobs) return;
+  const DevGeom &g = *geom;
+  const int64_t site = job / p.count;
+  const int64_t j = p.sample_begin + (job - site * p.count);
+  const uint64_t key = splitmix64(p.seed) ^ splitmix64(uint64_t(p.site_offset + site) + 0xa1224ULL) ^
+                       splitmix64(uint64_t(j) + 0x6242cULL);
+  Mt64 mt;
+  mt.seed(splitmix64(key));
+  V3<double> point{p.sites[3 * site], p.sites[3 * site + 1], p.sites[3 * site + 2]};
+  double value = 0.0;
+  int32_t steps = 0, flags = 1;
+  for (int32_t k = 0; k < p.max_steps; ++k) {
+    const double d = cert_domain_distance(g, point);
+    if (d <= p.eps) {
+      flags = 0;
+      break;
+    }
+    const double s = fmin(p.radius_cap, pdn(0.95 * d));
+    V3<double> y = point;
+    if (p.source == TWOS_CERT_HOUSING) {
+      // random_unit_vector(rng) * (s * beta_22(rng)): the Beta draw first.
+      const double t = mt.beta22();
+      y = point + mt.direction() * (s * t);
+    }
+    const V3<double> next = point + mt.direction() * s;
+    if (!(point_bound(point, next, true) < d) || !(point_bound(point, y, true) < d)) {
+      flags = 4;
+      break;
+    }
+    const double weight = pdn(pdn(s * s) / 6.0);
+    const double increment = pdn(weight * cert_source_lower(y, p.source, p.slope));
+    value = pdn(value + increment);
+    ++steps;
+    if (value >= p.value_cap) {
+      value = p.value_cap;
+      flags = 2;
+      break;
+    }
+    point = next;
+  }
+  p.value[job] = value;
+  p.steps[job] = steps;
+  p.flags[job] = flags;
+}
+
+__global__ void cert_distance_kernel(const DevGeom *geom, const double *pts, int64_t n, double *out) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = cert_domain_distance(*geom, V3<double>{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]});
+}
+
+inline unsigned blocks_for(int64_t n, int b) { return unsigned((n + b - 1) / b); }
+
+}  // namespace
+
+hipError_t launch_certify(const DevGeom *d_geom, const twos_cert_params &prm, const double *d_sites, int64_t n_sites,
+                          int64_t sample_begin, int64_t count, double *d_value, int32_t *d_steps, int32_t *d_flags,
+                          hipStream_t stream) {
+  CertLaunch p{};
+  p.sites = d_sites;
+  p.value = d_value;
+  p.steps = d_steps;
+  p.flags = d_flags;
+  p.jobs = n_sites * count;
+  p.count = count;
+  p.sample_begin = sample_begin;
+  p.site_offset = prm.site_offset;
+  p.seed = prm.seed;
+  p.max_steps = prm.max_steps;
+  p.source = prm.source;
+  p.eps = prm.eps;
+  p.radius_cap = prm.radius_cap;
+  p.value_cap = prm.value_cap;
+  p.slope = prm.slope;
+  if (p.jobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(cert_walk, dim3(blocks_for(p.jobs, 64)), dim3(64), 0, stream, d_geom, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_cert_distance(const DevGeom *d_geom, const double *d_pts, int64_t n, double *d_out,
+                                hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(cert_distance_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, stream, d_geom, d_pts, n, d_out);
+  return hipGetLastError();
+}
+
+}  // namespace twos

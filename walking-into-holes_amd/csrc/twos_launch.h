@@ -59,6 +59,12 @@ hipError_t launch_reduce(const Tables &tb, const WalkParams &wp, int grid, hipSt
 hipError_t launch_zero(unsigned long long *p, int n, hipStream_t stream);
 // Differential WoS over one chunk of sites (p.site_base, n_sites sites):
 // the per-job walks, then the per-(site, cavity) fold.
+// Certified sampling of n_sites sites, samples sample_begin .. + count - 1.
+hipError_t launch_certify(const DevGeom *d_geom, const twos_cert_params &prm, const double *d_sites, int64_t n_sites,
+                          int64_t sample_begin, int64_t count, double *d_value, int32_t *d_steps, int32_t *d_flags,
+                          hipStream_t stream);
+hipError_t launch_cert_distance(const DevGeom *d_geom, const double *d_pts, int64_t n, double *d_out,
+                                hipStream_t stream);
 hipError_t launch_differential(const DevGeom *d_geom, const DevSide *d_side, const DiffParams &p, int32_t n_sites,
                                int32_t n_cav, hipStream_t stream);
 

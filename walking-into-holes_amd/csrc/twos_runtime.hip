@@ -1483,6 +1483,92 @@ twos_status twos_differential(twos_ctx *ctx, int32_t geom_id, const twos_side *s
   return TWOS_OK;
 }
 
+namespace twos {
+namespace {
+
+// A mesh geometry minus cavities only (the validator's and the differential
+// estimator's domain).
+twos_status cavity_mesh_geometry(twos_ctx *ctx, int32_t geom_id, const GeomEntry **out) {
+  auto it = ctx->geoms.find(geom_id);
+  if (it == ctx->geoms.end()) return fail(TWOS_ERR_NOT_FOUND, "unknown geometry id");
+  const GeomEntry &ge = it->second;
+  if (ge.desc.container != kMesh || ge.dev.n_spheres != 0 || ge.dev.n_cylinders != 0)
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "needs a mesh container minus cavities only");
+  *out = &ge;
+  return TWOS_OK;
+}
+
+}  // namespace
+}  // namespace twos
+
+twos_status twos_certify(twos_ctx *ctx, int32_t geom_id, const twos_cert_params *params, const double *sites,
+                         int64_t n, double *value, int32_t *steps, int32_t *flags) {
+  if (!ctx || !params || (n > 0 && (!sites || !value || !steps || !flags)))
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "null argument");
+  const twos_cert_params &p = *params;
+  if (n < 0 || p.walks < 1 || p.max_steps < 1 || !(p.eps >= 0.0) || !(p.radius_cap > 0.0) || !(p.value_cap > 0.0) ||
+      p.sample_begin < 0)
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "invalid numeric option");
+  if (p.source != TWOS_CERT_AFFINE && p.source != TWOS_CERT_HOUSING)
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "unknown certified source");
+  if (p.source == TWOS_CERT_AFFINE && !(p.slope >= 0.0))
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "validator expects nonnegative affine slope");
+  for (int64_t i = 0; i < 3 * n; ++i)
+    if (!std::isfinite(sites[i])) return fail(TWOS_ERR_INVALID_ARGUMENT, "non-finite site coordinate");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  const GeomEntry *ge = nullptr;
+  if (twos_status st = cavity_mesh_geometry(ctx, geom_id, &ge)) return st;
+  if (n == 0) return TWOS_OK;
+  HIP_OK(hipSetDevice(ctx->device));
+  // Chunks of sites keep the per-sample buffers (16 B) within 2^22 samples.
+  const int64_t chunk = std::max<int64_t>(1, (int64_t(1) << 22) / p.walks);
+  const int64_t rows = std::min(n, chunk);
+  struct Buffers {
+    double *sites = nullptr, *value = nullptr;
+    int32_t *steps = nullptr, *flags = nullptr;
+    ~Buffers() { dfree(sites); dfree(value); dfree(steps); dfree(flags); }
+  } b;
+  HIP_OK(dalloc(&b.sites, 3 * size_t(n)));
+  HIP_OK(dalloc(&b.value, size_t(rows) * size_t(p.walks)));
+  HIP_OK(dalloc(&b.steps, size_t(rows) * size_t(p.walks)));
+  HIP_OK(dalloc(&b.flags, size_t(rows) * size_t(p.walks)));
+  HIP_OK(hipMemcpy(b.sites, sites, sizeof(double) * 3 * size_t(n), hipMemcpyHostToDevice));
+  for (int64_t s0 = 0; s0 < n; s0 += chunk) {
+    const int64_t cnt = std::min(chunk, n - s0);
+    twos_cert_params q = p;
+    q.site_offset = p.site_offset + s0;
+    const size_t m = size_t(cnt) * size_t(p.walks), o = size_t(s0) * size_t(p.walks);
+    HIP_OK(launch_certify(ctx->d_geoms + geom_id, q, b.sites + 3 * s0, cnt, p.sample_begin, p.walks, b.value,
+                          b.steps, b.flags, ctx->stream));
+    HIP_OK(hipMemcpyAsync(value + o, b.value, sizeof(double) * m, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipMemcpyAsync(steps + o, b.steps, sizeof(int32_t) * m, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipMemcpyAsync(flags + o, b.flags, sizeof(int32_t) * m, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+  }
+  return TWOS_OK;
+}
+
+twos_status twos_cert_distance(twos_ctx *ctx, int32_t geom_id, const double *points, int64_t n, double *out) {
+  if (!ctx || (n > 0 && (!points || !out))) return fail(TWOS_ERR_INVALID_ARGUMENT, "null argument");
+  if (n < 0) return fail(TWOS_ERR_INVALID_ARGUMENT, "invalid point count");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  const GeomEntry *ge = nullptr;
+  if (twos_status st = cavity_mesh_geometry(ctx, geom_id, &ge)) return st;
+  if (n == 0) return TWOS_OK;
+  HIP_OK(hipSetDevice(ctx->device));
+  double *d_pts = nullptr, *d_out = nullptr;
+  HIP_OK(dalloc(&d_pts, 3 * size_t(n)));
+  hipError_t e = dalloc(&d_out, size_t(n));
+  if (e == hipSuccess) e = hipMemcpy(d_pts, points, sizeof(double) * 3 * size_t(n), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_cert_distance(ctx->d_geoms + geom_id, d_pts, n, d_out, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, sizeof(double) * size_t(n), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  dfree(d_pts);
+  dfree(d_out);
+  if (e != hipSuccess) return fail(TWOS_ERR_DEVICE, std::string("cert distance: ") + hipGetErrorString(e));
+  return TWOS_OK;
+}
+
 twos_status twos_server_create(twos_ctx *ctx, int64_t max_sites, int64_t max_walks_per_round, twos_server **out) {
   if (!ctx || !out || max_sites < 1 || max_walks_per_round < 64)
     return fail(TWOS_ERR_INVALID_ARGUMENT, "invalid server capacity");

@@ -23,7 +23,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _capi
-from ._capi import (BALL, BOX, COUPLED, CROSS, MESH, PAIRED, PERSISTENT, PLAIN, RNG_MT_FD, RNG_MT_KEY,
+from ._capi import (BALL, BOX, CERT_AFFINE, CERT_CAPPED, CERT_CLIPPED, CERT_CONTAINMENT, CERT_HOUSING, COUPLED, CROSS, MESH, PAIRED, PERSISTENT, PLAIN, RNG_MT_FD, RNG_MT_KEY,
                     RNG_MT_PAIRED, RNG_MT_SITE, RNG_PHILOX, SQUARE, FLAG_COUNT_TRAVERSAL, FLAG_DISTANCE_GRID, FLAG_GRADIENT, MAX_TERMS,
                     TERM_BUMP, TERM_GAUSS, TERM_UNIT)
 
@@ -33,6 +33,7 @@ __all__ = [
     "BALL", "BOX", "MESH", "SQUARE", "PAIRED", "CROSS", "COUPLED", "RNG_PHILOX", "RNG_MT_SITE", "RNG_MT_PAIRED",
     "RNG_MT_KEY", "RNG_MT_FD", "probe_side", "PLAIN", "PERSISTENT", "FLAG_COUNT_TRAVERSAL", "FLAG_DISTANCE_GRID", "FLAG_GRADIENT", "version",
     "TERM_BUMP", "TERM_UNIT", "TERM_GAUSS", "MAX_TERMS", "study_terms", "housing_chip_terms",
+    "CERT_AFFINE", "CERT_HOUSING", "CERT_CAPPED", "CERT_CLIPPED", "CERT_CONTAINMENT",
 ]
 
 
@@ -434,6 +435,32 @@ class Context:
                                         float(offset), _ptr(out, _capi._dp),
                                         counts.ctypes.data_as(C.POINTER(C.c_int64))))
         return out[:, :k], counts
+
+    def certify(self, geom: int, sites, *, seed=0, walks=1024, max_steps=128, eps=1e-6, radius_cap=0.1,
+                value_cap=0.1, source=_capi.CERT_AFFINE, slope=0.65, site_offset=0, sample_begin=0):
+        """Certified minimum-benefit samples (cert_walk, minimum_benefit.hpp:
+        122-143) on the reference's streams: (value, steps, flags), [n][walks]
+        each, for samples sample_begin .. sample_begin + walks - 1; flags
+        CERT_CAPPED, CERT_CLIPPED, CERT_CONTAINMENT."""
+        p = _f64(sites).reshape(-1, 3)
+        n = len(p)
+        prm = _capi.CertParams(seed=seed, walks=walks, max_steps=max_steps, eps=eps, radius_cap=radius_cap,
+                               value_cap=value_cap, slope=slope, source=source, site_offset=site_offset,
+                               sample_begin=sample_begin)
+        value = np.zeros((n, walks))
+        steps = np.zeros((n, walks), dtype=np.int32)
+        flags = np.zeros((n, walks), dtype=np.int32)
+        i32 = C.POINTER(C.c_int32)
+        _check(_lib().twos_certify(self._h, geom, C.byref(prm), _ptr(p, _capi._dp), n, _ptr(value, _capi._dp),
+                                   steps.ctypes.data_as(i32), flags.ctypes.data_as(i32)))
+        return value, steps, flags
+
+    def cert_distance(self, geom: int, points) -> np.ndarray:
+        """Certified lower bound of the domain distance (cert_domain_distance)."""
+        p = _f64(points).reshape(-1, 3)
+        out = np.zeros(len(p))
+        _check(_lib().twos_cert_distance(self._h, geom, _ptr(p, _capi._dp), len(p), _ptr(out, _capi._dp)))
+        return out
 
     # -- the estimator ------------------------------------------------------
     def estimate(self, geom: int, sites, sides, *, combine=SQUARE, rng=RNG_PHILOX, precision=32,

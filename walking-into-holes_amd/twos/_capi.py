@@ -50,6 +50,17 @@ class GeometryDesc(C.Structure):
                 ("cavity_radius", C.c_double), ("spheres", _dp), ("cylinders", _dp), ("cavities", _dp)]
 
 
+class CertParams(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("walks", C.c_int32), ("max_steps", C.c_int32), ("eps", C.c_double),
+                ("radius_cap", C.c_double), ("value_cap", C.c_double), ("slope", C.c_double),
+                ("source", C.c_int32), ("pad", C.c_int32), ("site_offset", C.c_int64),
+                ("sample_begin", C.c_int64)]
+
+
+CERT_AFFINE, CERT_HOUSING = 0, 1
+CERT_CAPPED, CERT_CLIPPED, CERT_CONTAINMENT = 1, 2, 4
+
+
 class Params(C.Structure):
     _fields_ = [("combine", C.c_int32), ("rng", C.c_int32), ("precision", C.c_int32),
                 ("schedule", C.c_int32), ("seed", C.c_uint64), ("repeat", C.c_int32),
@@ -102,6 +113,8 @@ SIGNATURES = {
     "twos_distance_bound": (C.c_int32, [_ctx_p, C.c_int32, C.c_int32, _dp, C.c_int64, _dp]),
     "twos_differential": (C.c_int32, [_ctx_p, C.c_int32, C.POINTER(Side), C.POINTER(Params), _dp, C.c_int64,
                                       C.c_double, _dp, C.POINTER(C.c_int64)]),
+    "twos_certify": (C.c_int32, [_ctx_p, C.c_int32, C.POINTER(CertParams), _dp, C.c_int64, _dp, _i32p, _i32p]),
+    "twos_cert_distance": (C.c_int32, [_ctx_p, C.c_int32, _dp, C.c_int64, _dp]),
     "twos_trace": (C.c_int32, [_ctx_p, C.c_int32, C.POINTER(Side), C.POINTER(Params), _dp, C.c_int64,
                                C.c_int32, _dp, C.c_int32, _i32p]),
     "twos_server_create": (C.c_int32, [_ctx_p, C.c_int64, C.c_int64, C.POINTER(_srv_p)]),
