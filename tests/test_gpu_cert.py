@@ -99,3 +99,46 @@ def test_sample_blocks_compose(ctx, setup):
     tail = _certify(ctx, meshes, a, 512, sample_begin=1024)
     for x, y in zip(whole, tail):
         np.testing.assert_array_equal(x[:, 1024:], y)
+
+
+def test_wih_twos_validate(setup, golden, tmp_path):
+    """wih_twos --mode validate (run_minimum_benefit, minimum_benefit.hpp:
+    145-188): the samples file and the staged summary of the reference's
+    first attempt, and the fail-closed exit for a site outside the mesh."""
+    import json
+    import subprocess
+
+    from cli_files import BIN, write_ply, write_points
+
+    _, attempts = setup
+    a = attempts[0]
+    d = tmp_path
+    v, f = CF.mesh(golden, a.kind)
+    write_ply(d / "bunny.ply", v, f)
+    write_points(d / "points.csv", a.point)
+    args = ["--mode", "validate", "--mesh", str(d / "bunny.ply"), "--points", str(d / "points.csv"), "--output",
+            str(d / "validation.csv"), "--summary", str(d / "validation.json"), "--walks", "4096", "--seed",
+            str(a.seed), "--threads", "8", "--repeats", "1", "--source-mode", "affine", "--source-y-slope", ".65",
+            "--epsilon", repr(a.epsilon), "--max-steps", str(a.max_steps), "--validation-radius-cap",
+            repr(a.radius_cap), "--validation-value-cap", repr(a.value_cap)]
+    print("cli wih_twos --mode validate", a.source, flush=True)
+    r = subprocess.run([str(BIN / "wih_twos"), *args], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr
+    out = np.genfromtxt(d / "validation.csv", delimiter=",", names=True)
+    assert len(out) == 4096 and np.all(out["site"] == 0) and np.array_equal(out["sample"], np.arange(4096))
+    n = len(a.value)
+    np.testing.assert_array_equal(out["steps"][:n], a.steps)
+    np.testing.assert_array_equal(out["capped"][:n], a.capped)
+    np.testing.assert_array_equal(out["clipped"][:n], a.clipped)
+    np.testing.assert_allclose(out["value"][:n], a.value, rtol=1e-10, atol=0)
+    s = json.loads((d / "validation.json").read_text())
+    assert [st["n"] for st in s["stages"]] == [1024, 4096] and s["containment_failures"] == 0
+    for got, ref in zip(s["stages"], a.stages[:2]):
+        assert (got["path_caps"], got["clipped"]) == (ref["path_caps"], ref["clipped"])
+        assert got["mean"] == pytest.approx(ref["mean"], rel=1e-11)
+        assert got["center_clearance_lower"] == ref["center_clearance_lower"]
+        assert got["embree_center_distance"] == pytest.approx(ref["embree_center_distance"], rel=1e-6)
+    write_points(d / "outside.csv", [[5.0, 5.0, 5.0]])
+    r = subprocess.run([str(BIN / "wih_twos"), *args[:5], str(d / "outside.csv"), *args[6:]], capture_output=True,
+                       text=True, timeout=240)
+    assert r.returncode == 1 and "candidate classified outside" in r.stderr

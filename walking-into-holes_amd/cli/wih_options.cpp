@@ -13,7 +13,7 @@ void usage(const char *tool, const std::string &message) {
   std::cerr << "Usage: " << tool << " --mesh MESH.ply --points POINTS.csv --output FIELD.csv\n"
             << "                 --summary SUMMARY.json [options]\n\n"
             << "Options (as the reference's wih_twos):\n"
-            << "  --mode NAME           twos or diffwos (the estimators of this build)\n"
+            << "  --mode NAME           twos, diffwos or validate (the estimators of this build)\n"
             << "  --walks N             paths per evaluation site (default 16)\n"
             << "  --repeats N           timed repetitions; median is reported (default 3)\n"
             << "  --threads N           accepted for compatibility (the GPU sets its own)\n"
@@ -26,7 +26,10 @@ void usage(const char *tool, const std::string &message) {
             << "  --cavity-radius X     common cavity radius (default 0)\n"
             << "  --trace-output FILE   optional CSV of representative paths\n"
             << "  --trace-sites N       number of sites to trace (default 0)\n"
-            << "  --grid, --mg-*, --diff-offset, --validation-*: accepted, unused\n"
+            << "  --diff-offset X       diffwos normal offset (default 0.02)\n"
+            << "  --validation-radius-cap X  validate: contained-ball radius cap (default 0.1)\n"
+            << "  --validation-value-cap X   validate: sample clipping level (default 0.1)\n"
+            << "  --grid, --mg-*: accepted, unused\n"
             << "MI355X options:\n"
             << "  --streams NAME        reference (mt19937_64 per walk, binary64; default) or philox\n"
             << "  --precision N         philox walks in binary 32 (default) or 64\n"

@@ -1,8 +1,8 @@
 // Command-line options of the wih_twos / wih_study drop-ins: the reference's
 // flags (benchmarks/compiled_bunny/src/wih_twos.cpp:73-240) with the same
 // defaults, checks and exit status 2 on a usage error, plus MI355X options.
-// Flags of the reference's other solvers (voxel, multigrid, Differential WoS,
-// validator) are accepted and checked but have no effect here.
+// Flags of the reference's grid solvers (voxel, multigrid) are accepted and
+// checked but have no effect here.
 #pragma once
 
 #include <cstdint>
@@ -18,7 +18,8 @@ struct Options {
   int walks = 16, repeats = 3, max_steps = 512, threads = 0, trace_sites = 0;
   uint64_t seed = 20260912ULL;
   double epsilon = 5.0e-4, source_y_slope = 0.65, cavity_radius = 0.0;
-  // Accepted for compatibility (other solvers of the reference).
+  // Differential WoS and validator settings, then flags accepted for
+  // compatibility (the reference's grid solvers).
   int grid[3] = {54, 64, 48};
   int mg_cycles = 60, mg_pre_smooth = 3, mg_post_smooth = 3, mg_coarse_smooth = 80;
   double validation_radius_cap = 0.1, validation_value_cap = 0.1, diff_offset = 0.02, mg_tolerance = 1.0e-8,

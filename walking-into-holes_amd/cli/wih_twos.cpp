@@ -4,15 +4,18 @@
 // out, same exit codes: usage errors exit 2, runtime errors print
 // "fatal: <what>" and exit 1, a path that hits --max-steps is fatal.
 //
-// Modes twos and diffwos (Differential WoS cavity derivatives,
-// run_differential_wos, wih_twos.cpp:1604-1705; twos_differential); the
-// reference's voxel, mg and validate modes are other solvers.  By default the walks replay the reference's own streams
+// Modes twos, diffwos (Differential WoS cavity derivatives,
+// run_differential_wos, wih_twos.cpp:1604-1705; twos_differential) and
+// validate (certified minimum-benefit samples, run_minimum_benefit,
+// minimum_benefit.hpp:145-188; twos_certify); the reference's voxel and mg
+// modes are other solvers.  By default the walks replay the reference's own streams
 // (mt19937_64 per walk keyed as evaluate_site keys them, binary64), so the
 // field file matches the reference's; --streams philox selects the
 // production path (Philox streams, binary32 walks, optional distance grid and
 // adaptive budgets).
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdlib>
 #include <fstream>
 #include <iomanip>
@@ -199,13 +202,102 @@ int run_differential(const Options &o, twos_ctx *ctx, int32_t geom, const twos_s
   return 0;
 }
 
+// --mode validate: per site, the staged certified samples (1,024, 4,096,
+// 16,384, 65,536 below --walks, then --walks), one call per stage; the
+// samples file and the staged summary in the reference's formats
+// (run_minimum_benefit, minimum_benefit.hpp:145-188).
+int run_validation(const Options &o, twos_ctx *ctx, int32_t mesh_id, int32_t geom, const std::vector<double> &points,
+                   const std::vector<double> &cavities, double build_s) {
+  const size_t n = points.size() / 3;
+  std::vector<int32_t> inside(n);
+  std::vector<double> clearance(n), dist(n), closest(3 * n);
+  std::vector<int32_t> cls(n);
+  wih::check(twos_cert_distance(ctx, geom, points.data(), int64_t(n), clearance.data()), "certified distance");
+  wih::check(twos_closest(ctx, geom, 64, points.data(), int64_t(n), dist.data(), closest.data(), cls.data()),
+             "distance");
+  std::ofstream output(o.output_path), summary(o.summary_path);
+  if (!output) throw std::runtime_error("cannot write validation output: " + o.output_path);
+  if (!summary) throw std::runtime_error("cannot write validation summary: " + o.summary_path);
+  output << std::setprecision(17) << "site,sample,value,steps,capped,clipped\n";
+  summary << std::setprecision(17)
+          << "{\"scope\":\"conditional ideal-sampling certificate; conservative box geometry and source arithmetic\",\n"
+          << "\"distance_tree_build_seconds\":" << build_s
+          << ",\n\"distance_mode\":\"triangle_AABB_union_lower_bound\",\n"
+          << "\"sample_cap\":" << o.validation_value_cap << ",\"radius_cap\":" << o.validation_radius_cap
+          << ",\"stages\":[\n";
+  std::vector<int> stages;
+  for (int s : {1024, 4096, 16384, 65536})
+    if (s < o.walks) stages.push_back(s);
+  stages.push_back(o.walks);
+  twos_cert_params p{};
+  p.seed = o.seed;
+  p.max_steps = o.max_steps;
+  p.eps = o.epsilon;
+  p.radius_cap = o.validation_radius_cap;
+  p.value_cap = o.validation_value_cap;
+  p.slope = o.source_y_slope;
+  p.source = o.source_mode == "affine" ? TWOS_CERT_AFFINE : TWOS_CERT_HOUSING;
+  uint64_t failures = 0;
+  bool first = true;
+  for (size_t site = 0; site < n; ++site) {
+    const double *z = &points[3 * site];
+    int32_t in = 0;
+    wih::check(twos_inside(ctx, mesh_id, z, 1, &in), "inside");
+    bool in_cavity = false;
+    for (size_t k = 0; k + 2 < cavities.size(); k += 3) {
+      const double dx = z[0] - cavities[k], dy = z[1] - cavities[k + 1], dz = z[2] - cavities[k + 2];
+      in_cavity = in_cavity || std::sqrt(dx * dx + dy * dy + dz * dz) <= o.cavity_radius;
+    }
+    if (in != 1 || in_cavity)
+      throw std::runtime_error("candidate classified outside; exact driver predicate is also required");
+    std::vector<double> value(size_t(o.walks));
+    std::vector<int32_t> steps(size_t(o.walks)), flags(size_t(o.walks));
+    int prev = 0;
+    double elapsed = 0.0;
+    for (int stage : stages) {
+      p.site_offset = int64_t(site);
+      p.sample_begin = prev;
+      p.walks = stage - prev;
+      const auto start = std::chrono::steady_clock::now();
+      wih::check(twos_certify(ctx, geom, &p, z, 1, value.data() + prev, steps.data() + prev, flags.data() + prev),
+                 "certify");
+      elapsed += std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+      long double sum = 0;
+      uint64_t nsteps = 0, clipped = 0, capped = 0;
+      for (int j = 0; j < stage; ++j) {
+        sum += value[size_t(j)];
+        nsteps += uint64_t(steps[size_t(j)]);
+        clipped += (flags[size_t(j)] & TWOS_CERT_CLIPPED) ? 1 : 0;
+        capped += (flags[size_t(j)] & TWOS_CERT_CAPPED) ? 1 : 0;
+      }
+      if (!first) summary << ",\n";
+      first = false;
+      summary << "{\"site\":" << site << ",\"n\":" << stage << ",\"mean\":" << static_cast<double>(sum / stage)
+              << ",\"mean_steps\":" << double(nsteps) / stage << ",\"clipped\":" << clipped
+              << ",\"path_caps\":" << capped << ",\"elapsed_seconds\":" << elapsed
+              << ",\"center_clearance_lower\":" << clearance[site] << ",\"embree_center_distance\":" << dist[site]
+              << "}";
+      for (int j = prev; j < stage; ++j) {
+        const int32_t f = flags[size_t(j)];
+        failures += (f & TWOS_CERT_CONTAINMENT) ? 1 : 0;
+        output << site << ',' << j << ',' << value[size_t(j)] << ',' << steps[size_t(j)] << ','
+               << ((f & TWOS_CERT_CAPPED) ? 1 : 0) << ',' << ((f & TWOS_CERT_CLIPPED) ? 1 : 0) << '\n';
+      }
+      prev = stage;
+    }
+  }
+  summary << "\n],\"containment_failures\":" << failures << "}\n";
+  if (failures) throw std::runtime_error("sample containment guard failed; validation record must be rejected");
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
   try {
     const Options o = wih::parse_options(argc, argv, "wih_twos");
-    if (o.mode != "twos" && o.mode != "diffwos")
-      wih::usage("wih_twos", "mode " + o.mode + " is not provided by this build (twos and diffwos only)");
+    if (o.mode != "twos" && o.mode != "diffwos" && o.mode != "validate")
+      wih::usage("wih_twos", "mode " + o.mode + " is not provided by this build (twos, diffwos and validate only)");
     if (o.mode == "diffwos" && o.cavities_path.empty()) wih::usage("wih_twos", "Differential WoS requires --cavities");
     const wih::PlyMesh mesh = wih::load_ascii_ply(o.mesh_path);
     const std::vector<double> points = wih::read_points(o.points_path);
@@ -225,6 +317,7 @@ int main(int argc, char **argv) {
     d.cavities = cavities.empty() ? nullptr : cavities.data();
     d.cavity_radius = o.cavity_radius;
     wih::check(twos_geometry_create(ctx.get(), &d, &geom), "geometry");
+    if (o.mode == "validate") return run_validation(o, ctx.get(), mesh_id, geom, points, cavities, build_ms * 1e-3);
     const twos_side side = make_side(o);
     if (o.mode == "diffwos")
       return run_differential(o, ctx.get(), geom, side, mesh, points, cavities.size() / 3, build_ms * 1e-3);
