@@ -43,12 +43,15 @@ __host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
 }
 
 // Philox4x32-10 (Salmon et al., SC'11): two 32x32 multiplies per round, Weyl
-// key schedule, ten rounds.  Counter = (step, walk, 0, 0).
+// key schedule, ten rounds.  Counter = (step, walk, 0, 0).  Each product is
+// taken as one 64-bit result (one v_mad_u64_u32 instead of a v_mul_lo_u32 and
+// a v_mul_hi_u32: the same bits in half the multiply instructions).
 __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k) {
 #pragma unroll
   for (int round = 0; round < 10; ++round) {
-    const uint32_t lo0 = 0xD2511F53u * c.x, hi0 = __umulhi(0xD2511F53u, c.x);
-    const uint32_t lo1 = 0xCD9E8D57u * c.z, hi1 = __umulhi(0xCD9E8D57u, c.z);
+    const uint64_t p0 = uint64_t(0xD2511F53u) * c.x, p1 = uint64_t(0xCD9E8D57u) * c.z;
+    const uint32_t lo0 = uint32_t(p0), hi0 = uint32_t(p0 >> 32);
+    const uint32_t lo1 = uint32_t(p1), hi1 = uint32_t(p1 >> 32);
     c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
     k.x += 0x9E3779B9u;
     k.y += 0xBB67AE85u;
