@@ -240,6 +240,10 @@ const char *twos_last_error(void);
 twos_status twos_create(int32_t device, uint64_t hbm_budget_bytes, twos_ctx **out);
 void twos_destroy(twos_ctx *ctx);
 
+/* A mesh whose vertex and face arrays equal a resident one bit for bit
+ * (content hash, then a full comparison) returns that mesh's id, its LBVH and
+ * distance grid shared, with *build_ms = 0; every create is matched by one
+ * twos_mesh_release, and the last release frees it. */
 twos_status twos_mesh_create(twos_ctx *ctx, const double *vertices, int64_t nv, const uint32_t *faces,
                              int64_t nf, int32_t *mesh_id, double *build_ms);
 twos_status twos_mesh_info(twos_ctx *ctx, int32_t mesh_id, int32_t *depth, int64_t *nodes, int64_t *bytes);

@@ -67,6 +67,11 @@ constexpr float kGridKappa = 4.0f;
 
 struct MeshEntry {
   MeshBuffers mb;
+  // Content key: identical vertex and face arrays share one resident mesh.
+  uint64_t hash = 0;
+  std::vector<double> vertices;
+  std::vector<uint32_t> faces;
+  int32_t handles = 1;  // creates not yet matched by a release
   double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};  // vertex bounds (distance-grid extent)
   int32_t grid_cells = 0;
   int32_t depth = 0;
@@ -1056,7 +1061,28 @@ twos_status twos_mesh_create(twos_ctx *ctx, const double *vertices, int64_t nv, 
     if (faces[i] >= uint64_t(nv)) return fail(TWOS_ERR_INVALID_ARGUMENT, "expected valid triangular faces");
   for (int64_t i = 0; i < 3 * nv; ++i)
     if (!std::isfinite(vertices[i])) return fail(TWOS_ERR_INVALID_ARGUMENT, "non-finite vertex");
+  // Content hash of the exact bit patterns; a hit is confirmed by comparing
+  // the arrays themselves.
+  uint64_t hash = splitmix64(uint64_t(nv) * 0x9E3779B97F4A7C15ULL ^ uint64_t(nf));
+  for (int64_t i = 0; i < 3 * nv; ++i) {
+    uint64_t bits;
+    std::memcpy(&bits, vertices + i, sizeof bits);
+    hash = splitmix64(hash ^ bits);
+  }
+  for (int64_t i = 0; i < 3 * nf; i += 2)
+    hash = splitmix64(hash ^ (uint64_t(faces[i]) | (i + 1 < 3 * nf ? uint64_t(faces[i + 1]) << 32 : 0)));
   std::lock_guard<std::mutex> lock(ctx->mu);
+  for (auto &kv : ctx->meshes) {
+    MeshEntry &old = kv.second;
+    if (old.hash == hash && int64_t(old.vertices.size()) == 3 * nv && int64_t(old.faces.size()) == 3 * nf &&
+        std::memcmp(old.vertices.data(), vertices, sizeof(double) * 3 * size_t(nv)) == 0 &&
+        std::memcmp(old.faces.data(), faces, sizeof(uint32_t) * 3 * size_t(nf)) == 0) {
+      ++old.handles;
+      *mesh_id = kv.first;
+      if (build_ms) *build_ms = 0.0;
+      return TWOS_OK;
+    }
+  }
   HIP_OK(hipSetDevice(ctx->device));
   double *d_v = nullptr;
   uint32_t *d_f = nullptr;
@@ -1097,8 +1123,11 @@ twos_status twos_mesh_create(twos_ctx *ctx, const double *vertices, int64_t nv, 
     (void)hipFree(m.mb.nodes); (void)hipFree(m.mb.nodes4); (void)hipFree(m.mb.grid); (void)hipFree(m.mb.grid_start); (void)hipFree(m.mb.tris_f); (void)hipFree(m.mb.tris_d); (void)hipFree(m.mb.tri_ids);
     return fail(TWOS_ERR_GEOMETRY, "LBVH deeper than the traversal stack");
   }
+  m.hash = hash;
+  m.vertices.assign(vertices, vertices + 3 * nv);
+  m.faces.assign(faces, faces + 3 * nf);
   const int32_t id = ctx->next_mesh++;
-  ctx->meshes[id] = m;
+  ctx->meshes[id] = std::move(m);
   *mesh_id = id;
   if (build_ms) *build_ms = ms;
   return TWOS_OK;
@@ -1174,6 +1203,10 @@ twos_status twos_mesh_release(twos_ctx *ctx, int32_t mesh_id) {
   std::lock_guard<std::mutex> lock(ctx->mu);
   auto it = ctx->meshes.find(mesh_id);
   if (it == ctx->meshes.end()) return fail(TWOS_ERR_NOT_FOUND, "unknown mesh id");
+  if (it->second.handles > 1) {
+    --it->second.handles;  // another create of the same mesh still holds it
+    return TWOS_OK;
+  }
   if (it->second.users > 0) return fail(TWOS_ERR_INVALID_ARGUMENT, "mesh still used by a geometry");
   MeshBuffers &mb = it->second.mb;
   (void)hipFree(mb.nodes); (void)hipFree(mb.nodes4); (void)hipFree(mb.grid); (void)hipFree(mb.grid_start); (void)hipFree(mb.tris_f); (void)hipFree(mb.tris_d); (void)hipFree(mb.tri_ids);
