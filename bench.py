@@ -62,7 +62,9 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--no-autotune", action="store_true", help="default launch shape instead of the tuner's")
     ap.add_argument("--retune", action="store_true",
                     help="run the autotuner's trials instead of reading its committed cache "
-                         "(writes gpurun_out/tuning_cache.json)")
+                         "(writes the file named by --tuning-cache)")
+    ap.add_argument("--tuning-cache", default="tuning_cache.retuned.json",
+                    help="with --retune: the cache file to write (default: in the working directory)")
     ap.add_argument("--block", type=int, default=0, help="fixed launch shape (skips the autotuner)")
     ap.add_argument("--lds-stack", type=int, default=0, help="fixed launch shape (skips the autotuner)")
     ap.add_argument("--step-batch", type=int, default=-1, help="fixed launch shape (skips the autotuner)")
@@ -142,8 +144,7 @@ def main() -> int:
         tuning = autotune.autotune(ctx, geom, sites, sides, geom_sig=f"{w.name}", combine=twos.CROSS,
                                    precision=w.precision, eps=w.eps, max_steps=w.max_steps,
                                    distance_grid=grid_on, mesh=mesh, log=log,
-                                   cache_path=ROOT / "gpurun_out" / "tuning_cache.json" if args.retune
-                                   else TUNING_CACHE)
+                                   cache_path=Path(args.tuning_cache) if args.retune else TUNING_CACHE)
         shape = {k: v for k, v in tuning.choice.items()
                  if k not in ("grid_cells", "grid_corners") and (grid_on or k != "grid_kappa")}
         log(f"autotune {time.perf_counter() - t0:.1f} s: {tuning.choice}, "

@@ -261,7 +261,7 @@ def housing_fixtures(root: Path) -> None:
     run_housing_benchmark.py:826-869: replicate 1 of "TWoS + validation" /
     "TWoS equal-time" uses SEED + 100000, scout step k adds 1000 + k,
     validation step k adds 2000 + k (SEED = 20260912, :28); walk counts are
-    the driver's defaults (:1116-1120)."""
+    that script's defaults (:1116-1120)."""
     housing = root / "benchmarks" / "engineering_housing"
     v, f = read_ascii_ply(housing / "assets" / "stylized-finned-housing.ply")
     np.savez_compressed(HERE / "housing_mesh.npz", vertices=v, faces=f)
