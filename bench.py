@@ -93,7 +93,7 @@ def build_workload(name: str):
 def main() -> int:
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only",
+        print(json.dumps({"error": "bench.py runs on one GPU; multi-GPU sharding is a design note only",
                           "n_gpus": args.gpus}))
         return 2
     import twos

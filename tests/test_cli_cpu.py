@@ -31,7 +31,7 @@ def inputs(tmp_path):
 @pytest.mark.parametrize("extra,needle", [
     (["--walks", "1"], "invalid numeric option"),
     (["--epsilon", "0"], "invalid numeric option"),
-    (["--mode", "voxel"], "not provided by this build"),
+    (["--mode", "voxel"], "not provided by the MI355X backend"),
     (["--mode", "diffwos"], "Differential WoS requires --cavities"),
     (["--bogus", "1"], "unknown option --bogus"),
     (["--walks"], "missing value for --walks"),

@@ -13,7 +13,7 @@ void usage(const char *tool, const std::string &message) {
   std::cerr << "Usage: " << tool << " --mesh MESH.ply --points POINTS.csv --output FIELD.csv\n"
             << "                 --summary SUMMARY.json [options]\n\n"
             << "Options (as the reference's wih_twos):\n"
-            << "  --mode NAME           twos, diffwos or validate (the estimators of this build)\n"
+            << "  --mode NAME           twos, diffwos or validate (the MI355X estimators)\n"
             << "  --walks N             paths per evaluation site (default 16)\n"
             << "  --repeats N           timed repetitions; median is reported (default 3)\n"
             << "  --threads N           accepted for compatibility (the GPU sets its own)\n"

@@ -185,7 +185,7 @@ int process_request(int argc, char **argv, Session &session) {
     load_config(config, heat, sensor);
     if (analytic && !fd) heat.source_kind = sensor.source_kind = 3;
     if (o.mode != "twos" && o.mode != "mg") throw std::runtime_error("use twos or mg");
-    if (o.mode == "mg") throw std::runtime_error("mode mg is not provided by this build (twos only)");
+    if (o.mode == "mg") throw std::runtime_error("mode mg is not provided by the MI355X backend (twos only)");
     const auto start = std::chrono::steady_clock::now();
     if (!session.ctx) {
       const wih::PlyMesh mesh = wih::load_ascii_ply(o.mesh_path);

@@ -297,7 +297,7 @@ int main(int argc, char **argv) {
   try {
     const Options o = wih::parse_options(argc, argv, "wih_twos");
     if (o.mode != "twos" && o.mode != "diffwos" && o.mode != "validate")
-      wih::usage("wih_twos", "mode " + o.mode + " is not provided by this build (twos, diffwos and validate only)");
+      wih::usage("wih_twos", "mode " + o.mode + " is not provided by the MI355X backend (twos, diffwos and validate only)");
     if (o.mode == "diffwos" && o.cavities_path.empty()) wih::usage("wih_twos", "Differential WoS requires --cavities");
     const wih::PlyMesh mesh = wih::load_ascii_ply(o.mesh_path);
     const std::vector<double> points = wih::read_points(o.points_path);
