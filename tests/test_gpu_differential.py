@@ -3,7 +3,7 @@ benchmarks/compiled_bunny/src/wih_twos.cpp:725-928, as wih_study --diff 1
 runs it, benchmarks/submission_study/study.cpp:303-316) on an MI355X,
 through the C-ABI.
 
-- the reference's retained --diff 1 requests on the branch mesh: kernel.json's
+- all fourteen retained --diff 1 requests on the branch mesh: kernel.json's
   request totals (steps, distance queries, parity rays) exactly, the
   derivative columns to 2e-4 of scale and most sites bitwise (near-tie
   closest triangles, and the odd ulp between the device library's and
@@ -76,10 +76,11 @@ def _totals(counts):
             "parity_rays": int(c["parity_rays"]), "caps": int(c["limits"])}
 
 
-@pytest.mark.parametrize("source", ["validation-local-cavity/raw/gradient-0.002-0",
-                                    "validation-local-cavity/raw/gradient-0.004-0",
-                                    "validation-local-cavity/raw/gradient-0.008-2",
-                                    "validation/raw/gradient-0.002-0"])
+SOURCES = [f"validation-local-cavity/raw/gradient-{o}-{k}" for o in ("0.002", "0.004", "0.008") for k in range(4)]
+SOURCES += ["validation/raw/gradient-0.002-0", "validation/raw/gradient-0.002-1"]
+
+
+@pytest.mark.parametrize("source", SOURCES)
 def test_requests_match_reference(ctx, diff, source):
     req = _req(diff, source)
     print("differential", source, len(req.points), "sites", len(req.holes), "cavities", flush=True)
