@@ -46,7 +46,8 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="sh_icosphere",
-                    choices=["sh_icosphere", "cooling_block", "unit_ball", "serve", "copy", "design"])
+                    choices=["sh_icosphere", "cooling_block", "unit_ball", "serve", "copy", "design", "diffwos",
+                             "certify"])
     ap.add_argument("--design-rounds", type=int, default=2, help="design: insertion rounds")
     ap.add_argument("--design-candidates", type=int, default=65536, help="design: candidate pool size")
     ap.add_argument("--design-scout-walks", type=int, default=512, help="design: scout walks per side")
@@ -103,6 +104,8 @@ def main() -> int:
         return serve(args)
     if args.workload == "copy":
         return copy_probe()
+    if args.workload in ("diffwos", "certify"):
+        return f4_bench(args)
     if args.workload == "design":
         return design_bench(args)
     w = build_workload(args.workload)
@@ -411,6 +414,63 @@ def measured_traffic(workload: str, n_sites: int, shape: dict, launches: int):
     if float(fields.get("traffic_launches", -1)) != launches:
         return None
     return int(float(fields["traffic_bytes_per_launch"]))
+
+
+def f4_bench(args) -> int:
+    """The adjacent estimators on the SH-icosphere mesh (81,920 triangles),
+    binary64 on the reference's mt19937_64 streams: Differential WoS
+    (twos_differential; 1,024 sites x 1,024 samples, four cavities of radius
+    0.05, the primal data g = x + 0.3 y^2, offset 0.004, eps 1e-4) or
+    certified minimum-benefit sampling (twos_certify; 16 sites x 65,536
+    samples, the affine source, radius cap 0.1, value cap 0.06, eps 1e-6, 128
+    steps).  A step is one call; the metric is samples per second."""
+    import twos
+    from twos import workloads as W
+
+    w = W.sh_icosphere(n_sites=1024 + 16)
+    ctx = twos.Context()
+    mesh = ctx.mesh(w.vertices, w.faces)
+    c = w.sites[:4]
+    diff = args.workload == "diffwos"
+    geom = ctx.geometry(twos.MESH, mesh=mesh, cavities=c, cavity_radius=0.05)
+    if diff:
+        sites = w.sites[16:16 + 1024]
+        sites = sites[np.min(np.linalg.norm(sites[:, None] - c[None], axis=2), axis=1) > 0.06]
+        side = W.sides(w, twos.side)[0]
+        walks = 1024
+        call = lambda: ctx.differential(geom, sites, side, seed=11, walks=walks, eps=1e-4, max_steps=2048,
+                                        offset=0.004)
+    else:
+        sites = w.sites[4:16]
+        sites = sites[np.min(np.linalg.norm(sites[:, None] - c[None], axis=2), axis=1) > 0.06]
+        walks = 65536
+        call = lambda: ctx.certify(geom, sites, seed=11, walks=walks, max_steps=128, eps=1e-6, radius_cap=0.1,
+                                   value_cap=0.06, source=twos.CERT_AFFINE, slope=0.65)
+    for i in range(args.warmup):
+        call()
+        log(f"warmup {i}")
+    times, steps = [], 0
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        out = call()
+        times.append(time.perf_counter() - t0)
+        steps = int(out[1][:, 0].sum() + out[1][:, 1].sum()) if diff else int(out[1].sum())
+        log(f"step {i}: {times[-1]:.3f} s, {steps} steps")
+    t = float(np.mean(times))
+    samples = len(sites) * walks
+    result = {
+        "metric": ("Differential WoS samples/s" if diff else "certified samples/s"), "value": round(samples / t, 1),
+        "unit": "samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * t, 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": (f"diffwos on {w.name}: {len(sites)} sites x {walks} samples, 4 cavities r 0.05"
+                                if diff else f"certify on {w.name}: {len(sites)} sites x {walks} samples")},
+        "walk_steps_per_s": round(steps / t, 1),
+    }
+    ctx.release_geometry(geom)
+    ctx.close()
+    print(json.dumps(result))
+    return 0
 
 
 def cpu_baseline(w, sites, count: int, repeats: int = 3) -> dict:
