@@ -11,7 +11,7 @@ Axes (binary32 persistent kernel; binary64 tunes the block only):
                   1 (once at least as many lanes are ready as traverse) or 32
                   (once 32 lanes are ready); built for lds_stack 24 with
                   waves_per_simd 0 or 5
-  grid_kappa      distance-grid threshold in cell sizes, 0.25 to 6 (mesh
+  grid_kappa      distance-grid threshold in cell sizes, 0.0625 to 6 (mesh
                   geometries with the distance grid on)
   grid_corners    distance-grid samples at cell centres (0), corners (1,
                   eight-corner bound) or binary16 corner records per cell (2),
@@ -55,7 +55,7 @@ AXES = {
     "lds_stack": (16, 24, 32),
     "step_batch": (0, 1, 32),
     "grid_corners": (0, 1, 2),
-    "grid_kappa": (0.25, 0.35, 0.5, 0.75, 1.0, 1.5, 2.0, 3.0, 4.0, 6.0),
+    "grid_kappa": (0.0625, 0.125, 0.25, 0.35, 0.5, 0.75, 1.0, 1.5, 2.0, 3.0, 4.0, 6.0),
     "grid_cells": (96, 128, 192, 256, 320, 384, 448, 512),
 }
 
