@@ -248,7 +248,7 @@ twos_status twos_mesh_create(twos_ctx *ctx, const double *vertices, int64_t nv, 
                              int64_t nf, int32_t *mesh_id, double *build_ms);
 twos_status twos_mesh_info(twos_ctx *ctx, int32_t mesh_id, int32_t *depth, int64_t *nodes, int64_t *bytes);
 /* Rebuilds the mesh's conservative distance grid with `cells` cells along the
- * longest axis of its bounding box (8..512; meshes start at 128 cells, values
+ * longest axis of its bounding box (8..1024; meshes start at 128 cells, values
  * at cell centres) and updates every geometry on the mesh.  corners 1 samples
  * the distance at the cell corners and bounds a step by the best of a cell's
  * eight corners; corners 2 stores those eight values per cell rounded down to

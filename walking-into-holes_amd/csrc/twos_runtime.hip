@@ -1136,7 +1136,7 @@ twos_status twos_mesh_create(twos_ctx *ctx, const double *vertices, int64_t nv, 
 twos_status twos_mesh_set_distance_grid(twos_ctx *ctx, int32_t mesh_id, int32_t cells, int32_t corners,
                                         double *build_ms) {
   if (!ctx) return fail(TWOS_ERR_INVALID_ARGUMENT, "null context");
-  if (cells < 8 || cells > 512) return fail(TWOS_ERR_INVALID_ARGUMENT, "cells must be in [8, 512]");
+  if (cells < 8 || cells > 1024) return fail(TWOS_ERR_INVALID_ARGUMENT, "cells must be in [8, 1024]");
   if (corners < 0 || corners > 2) return fail(TWOS_ERR_INVALID_ARGUMENT, "grid mode must be 0, 1 or 2");
   std::lock_guard<std::mutex> lock(ctx->mu);
   auto it = ctx->meshes.find(mesh_id);

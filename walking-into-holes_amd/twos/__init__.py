@@ -316,7 +316,7 @@ class Context:
 
     def set_distance_grid(self, mesh: Mesh | int, cells: int, corners: int = 0) -> float:
         """Rebuild the mesh's distance grid with `cells` cells along its longest
-        axis (8..512), sampled at cell centres (corners=0), at cell corners
+        axis (8..1024), sampled at cell centres (corners=0), at cell corners
         (1, eight-corner bound) or as per-cell binary16 corner records (2);
         returns the build time in ms."""
         ms = C.c_double()

@@ -16,7 +16,7 @@ Axes (binary32 persistent kernel; binary64 tunes the block only):
   grid_corners    distance-grid samples at cell centres (0), corners (1,
                   eight-corner bound) or binary16 corner records per cell (2),
                   with the mesh handle
-  grid_cells      distance-grid resolution, 96 to 512 cells along
+  grid_cells      distance-grid resolution, 96 to 1024 cells along
                   the mesh's longest axis (when the mesh handle is given; the
                   mesh keeps the chosen grid)
 
@@ -56,7 +56,7 @@ AXES = {
     "step_batch": (0, 1, 32),
     "grid_corners": (0, 1, 2),
     "grid_kappa": (0.0625, 0.125, 0.25, 0.35, 0.5, 0.75, 1.0, 1.5, 2.0, 3.0, 4.0, 6.0),
-    "grid_cells": (96, 128, 192, 256, 320, 384, 448, 512),
+    "grid_cells": (96, 128, 192, 256, 320, 384, 448, 512, 640, 768, 896, 1024),
 }
 
 _memory: dict[str, "Tuning"] = {}
