@@ -258,6 +258,14 @@ twos_status twos_mesh_info(twos_ctx *ctx, int32_t mesh_id, int32_t *depth, int64
 twos_status twos_mesh_set_distance_grid(twos_ctx *ctx, int32_t mesh_id, int32_t cells, int32_t corners,
                                         double *build_ms);
 twos_status twos_mesh_release(twos_ctx *ctx, int32_t mesh_id);
+/* Device memory the context accounts for: resident geometry (LBVHs, distance
+ * grids with their start nodes, geometry rows, the geometry table), engines
+ * (walk buffers, round blocks and the slots of admitted requests), the HBM
+ * budget given to twos_create (0 there: a quarter of the free memory) and
+ * the bytes one admitted site costs.  No counterpart in the reference (the
+ * reference reports mesh_build_seconds and peak RSS, wih_twos.cpp:1746-1793). */
+twos_status twos_memory_info(twos_ctx *ctx, int64_t *geometry_bytes, int64_t *engine_bytes, int64_t *budget_bytes,
+                             int64_t *slot_bytes);
 
 twos_status twos_geometry_create(twos_ctx *ctx, const twos_geometry_desc *desc, int32_t *geom_id);
 twos_status twos_geometry_union_sphere(twos_ctx *ctx, int32_t base, const double center[3], double radius,
@@ -353,7 +361,12 @@ twos_status twos_cert_distance(twos_ctx *ctx, int32_t geom_id, const double *poi
 
 /* Continuous-batching server: requests are admitted between rounds as device
  * memory allows, all admitted sites share each persistent launch, and each
- * result depends only on (seed, site, side, walk), alone or in the mix. */
+ * result depends only on (seed, site, side, walk), alone or in the mix.
+ * Admission by memory: a queued request is admitted when its slots are free
+ * and its slot bytes fit the context's HBM budget beside everything resident
+ * on the context (meshes with their distance grids, geometries, every
+ * engine's buffers, the requests already admitted).  A geometry cannot be
+ * released while a queued or active request uses it. */
 typedef struct twos_server twos_server;
 twos_status twos_server_create(twos_ctx *ctx, int64_t max_sites, int64_t max_walks_per_round,
                                twos_server **out);
@@ -364,6 +377,15 @@ twos_status twos_server_submit(twos_server *srv, int32_t geom_id, const twos_sid
 /* Runs one round over every admitted request; *n_done receives how many
  * requests completed in this round, their ids written to done_ids. */
 twos_status twos_server_step(twos_server *srv, int64_t *done_ids, int32_t max_done, int32_t *n_done);
+/* As twos_server_step, also writing each completed request's status to
+ * done_status: TWOS_OK; TWOS_ERR_STEP_CAP when a path of one of its sites hit
+ * max_steps (results written; the reference's serve loop answers DONE 1 and
+ * exits, study.cpp:373-374, here the other requests go on); or the error that
+ * kept it from running (TWOS_ERR_ADMISSION: its slots do not fit the HBM
+ * budget beside the resident geometry even with nothing else admitted).  A
+ * request that ended either way has released its slots. */
+twos_status twos_server_step_status(twos_server *srv, int64_t *done_ids, int32_t *done_status, int32_t max_done,
+                                    int32_t *n_done);
 twos_status twos_server_pending(twos_server *srv, int64_t *queued, int64_t *active);
 
 #ifdef __cplusplus

@@ -186,13 +186,19 @@ def test_binary32_against_binary64_rerun(ctx):
     assert np.max(np.abs(r32.score - r64.score) / se) < 0.5
 
 
-def test_inside_matches_oracle(ctx, bunny, gpu_bunny):
+@pytest.mark.parametrize("faces", [1000, 69674])
+def test_inside_matches_oracle(ctx, bunny, gpu_bunny, faces):
+    """Ray parity (EmbreeMesh::inside, wih_twos.cpp:361-394) is a boolean: the
+    GPU's binary64 Moller-Trumbore hits along the fixed direction, with the
+    same tnear, tfar and 2e-6 advance, equal the oracle's at every point."""
     _, _, meshes = bunny
-    v, f = meshes[1000]["vertices"], meshes[1000]["faces"]
-    pts = _box_points(v, 600, 9)
-    got = ctx.inside(gpu_bunny[1000][0], pts)
+    v, f = meshes[faces]["vertices"], meshes[faces]["faces"]
+    pts = _box_points(v, 4000, 9)
+    got = ctx.inside(gpu_bunny[faces][0], pts)
     ref = O.Mesh(v, f).inside(pts)
-    assert np.mean(got == (ref == 1)) >= 0.995
+    bad = np.nonzero(got != (ref == 1))[0]
+    assert len(bad) == 0, (bad[:10].tolist(), pts[bad[:10]].tolist())
+    assert 0.1 < np.mean(got) < 0.9  # both classes occur
 
 
 def test_graph_replays_equal_eager(ctx, bunny, gpu_bunny):

@@ -126,3 +126,117 @@ def test_insertion_validation_sees_the_cavity(env):
     se = np.sqrt(np.maximum(base.same - base.u ** 2, 0) / base.walks)
     assert np.mean(change / se) > 5
     ctx.release_geometry(g2)
+
+
+def test_graph_rounds_survive_geometry_table_growth(env):
+    """A server capturing its rounds as hipGraphs keeps equal results while
+    more than 16 and 32 derived geometries are created (the device geometry
+    table is reallocated and every engine's captured rounds are replaced)."""
+    ctx, _, _, ball, g_ball = env
+    srv = ctx.server(1024, 1 << 18)
+    kw = dict(combine=twos.CROSS, precision=32, walks=256, eps=1e-3, max_steps=8192)
+    sides = W.sides(ball, twos.side)
+    made = []
+    try:
+        for i in range(40):
+            c = [0.5 * np.cos(i), 0.5 * np.sin(i), 0.0]
+            g = ctx.union_sphere(g_ball, c, 0.05)
+            made.append(g)
+            rid = srv.submit(g, ball.sites[:32], sides, seed=40 + i, use_graph=True, **kw)
+            got = _drain(srv)[rid]
+            ref = ctx.estimate(g, ball.sites[:32], sides, seed=40 + i, **kw)
+            for k in KEYS:
+                np.testing.assert_array_equal(getattr(got, k), getattr(ref, k), err_msg=f"{i} {k}")
+    finally:
+        srv.close()
+        for g in made:
+            ctx.release_geometry(g)
+
+
+def test_geometry_in_use_cannot_be_released(env):
+    ctx, _, _, ball, g_ball = env
+    g = ctx.union_sphere(g_ball, [0.3, 0.0, 0.0], 0.05)
+    srv = ctx.server(1024, 1 << 18)
+    kw = dict(combine=twos.CROSS, precision=32, walks=64, walks_max=1024, ci_rel=0.01, eps=1e-3, max_steps=8192)
+    srv.submit(g, ball.sites[:16], W.sides(ball, twos.side), seed=3, **kw)
+    with pytest.raises(twos.InvalidArgument):
+        ctx.release_geometry(g)  # queued
+    srv.step()
+    if srv.pending() != (0, 0):
+        with pytest.raises(twos.InvalidArgument):
+            ctx.release_geometry(g)  # active
+    _drain(srv)
+    ctx.release_geometry(g)
+    srv.close()
+
+
+def test_failed_request_is_reported_and_frees_its_slots(env):
+    """A request whose paths hit max_steps ends with ERR_STEP_CAP (the
+    reference's serve loop answers DONE 1 and exits, study.cpp:373-374); the
+    requests around it complete with their standalone results, and its slots
+    serve the next request."""
+    ctx, ico, g_mesh, ball, g_ball = env
+    srv = ctx.server(256, 1 << 20)
+    kw = dict(combine=twos.CROSS, precision=32, walks=128, eps=1e-4, max_steps=8192)
+    sides_m, sides_b = W.sides(ico, twos.side), W.sides(ball, twos.side)
+    ra = srv.submit(g_mesh, ico.sites[:64], sides_m, seed=1, distance_grid=True, **kw)
+    rb = srv.submit(g_mesh, ico.sites[64:128], sides_m, seed=2, **{**kw, "max_steps": 3})
+    rc = srv.submit(g_ball, ball.sites[:64], sides_b, seed=3, **kw)
+    done = _drain(srv)
+    assert done[rb].stats["status"] == twos.StepCapError.code and np.all(done[rb].caps > 0)
+    for rid, g, pts, sides, seed, extra in ((ra, g_mesh, ico.sites[:64], sides_m, 1, {"distance_grid": True}),
+                                            (rc, g_ball, ball.sites[:64], sides_b, 3, {})):
+        assert done[rid].stats["status"] == 0
+        ref = ctx.estimate(g, pts, sides, seed=seed, **kw, **extra)
+        for k in KEYS:
+            np.testing.assert_array_equal(getattr(done[rid], k), getattr(ref, k), err_msg=k)
+    assert srv.pending() == (0, 0)
+    rd = srv.submit(g_ball, ball.sites[:64].repeat(4, axis=0), sides_b, seed=4, **kw)  # every slot
+    assert _drain(srv)[rd].stats["status"] == 0
+    srv.close()
+
+
+def test_admission_by_memory_counts_resident_geometry(golden):
+    """Admission charges each request's slots against the context's HBM budget
+    beside the resident geometry: with room for 1.5 requests, the second
+    waits for the first although slots are free; once a mesh with its
+    distance grid takes the room, a request that can never fit is reported
+    with ERR_ADMISSION while the server goes on."""
+    kw = dict(combine=twos.CROSS, precision=32, walks=64, eps=1e-3, max_steps=8192)
+    ball = W.unit_ball(n_sites=64)
+    sides = W.sides(ball, twos.side)
+    with twos.Context() as probe:
+        probe.geometry(twos.BALL)
+        srv = probe.server(2048, 1 << 16)
+        m = probe.memory()
+        srv.close()
+    slot = m["slot_bytes"]
+    budget = m["geometry_bytes"] + m["engine_bytes"] + 1500 * slot
+    with twos.Context(hbm_budget=budget) as ctx:
+        g = ctx.geometry(twos.BALL)
+        srv = ctx.server(2048, 1 << 16)
+        assert ctx.memory()["engine_bytes"] == m["engine_bytes"]
+        sites = np.tile(ball.sites, (16, 1))[:1000]
+        ra = srv.submit(g, sites, sides, seed=1, **kw)
+        rb = srv.submit(g, sites, sides, seed=2, **kw)
+        assert srv.pending() == (2, 0)
+        first = srv.step()
+        assert set(first) == {ra} and srv.pending() == (1, 0)  # 2000 sites' slots are free, 1500 sites' bytes
+        second = _drain(srv)
+        assert set(second) == {rb}
+        with twos.Context() as free:  # standalone references, outside the tight budget
+            gf = free.geometry(twos.BALL)
+            for rid, res, seed in ((ra, first[ra], 1), (rb, second[rb], 2)):
+                ref = free.estimate(gf, sites, sides, seed=seed, **kw)
+                for k in KEYS:
+                    np.testing.assert_array_equal(getattr(res, k), getattr(ref, k), err_msg=k)
+        v = np.load(golden / "bunny_1000.npz")
+        mesh = ctx.mesh(v["vertices"], v["faces"])  # LBVH + 128-cell grid: megabytes
+        assert ctx.memory()["geometry_bytes"] > m["geometry_bytes"] + 1500 * slot
+        rc = srv.submit(g, sites, sides, seed=3, **kw)
+        out = srv.step()
+        assert out[rc].stats["status"] == twos.AdmissionError.code and srv.pending() == (0, 0)
+        ctx.release_mesh(mesh)
+        rd = srv.submit(g, sites, sides, seed=4, **kw)
+        assert _drain(srv)[rd].stats["status"] == 0
+        srv.close()

@@ -345,3 +345,34 @@ def test_adaptive_ladder_meets_target_or_caps():
         assert r["walks"][i] in (64, 128, 256, 512, 1024, 2048, 4096)
         if r["status"][i] == 1:
             assert r["walks"][i] == 4096
+
+
+def test_adaptive_ladder_stops_at_the_first_rung_that_meets_the_target():
+    """Minimality: each path depends only on (seed, site, side, walk), so a
+    fixed-budget run of W walks reproduces the ladder's state at rung W.  A
+    site that stopped at rung W met the target there and did not meet it at
+    W / 2; a site that ran out of budget met it at no rung.  A ladder that
+    never stops early, or one that stops a rung late, fails this."""
+    gu = O.side(poly=[0, 0, 0, 0.5, 1, -1, 0, 0, 0, 0], poly_classes=(0,))
+    gv = O.side(poly=[0, 0, 1, 0, 0, 0, 0, 0, 1, 0], poly_classes=(0,))
+    rng = np.random.default_rng(0)
+    d = rng.normal(size=(24, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    z = d * (0.8 * rng.uniform(0, 1, 24) ** (1 / 3))[:, None]
+    kw = dict(combine=O.CROSS, rng=O.RNG_PHILOX, seed=12, eps=1e-3, max_steps=8192)
+    rungs = [64, 128, 256, 512, 1024, 2048]
+    r = O.estimate(_ball_geometry(), z, [gu, gv], walks=64, walks_max=2048, ci_rel=0.3, **kw)
+    fixed = {w: O.estimate(_ball_geometry(), z, [gu, gv], walks=w, **kw) for w in rungs}
+    met = {w: 1.96 * fixed[w]["score_se"] <= 0.3 * np.abs(fixed[w]["score"]) for w in rungs}
+    stops = set()
+    for i in range(len(z)):
+        w = int(r["walks"][i])
+        stops.add(w)
+        # The ladder's result at its last rung is the fixed-budget result there.
+        assert r["score"][i] == fixed[w]["score"][i] and r["steps"][i] == fixed[w]["steps"][i]
+        if r["status"][i] == 0:
+            assert met[w][i], i
+            assert all(not met[v][i] for v in rungs if v < w), i
+        else:
+            assert w == 2048 and not any(met[v][i] for v in rungs), i
+    assert len(stops) >= 3 and 2048 in stops and min(stops) < 2048  # the sites spread over the rungs

@@ -648,6 +648,7 @@ hipError_t build_distance_grid(MeshBuffers &mb, const double lo[3], const double
   if (mode != 2) {
     for (int k = 0; k < 3; ++k) mb.grid_n[k] = n[k];
     mb.grid = samples;
+    mb.grid_bytes = int64_t(sizeof(float)) * total;
     return hipSuccess;
   }
   // Corner records: one per cell, built from the corner samples.
@@ -668,6 +669,7 @@ hipError_t build_distance_grid(MeshBuffers &mb, const double lo[3], const double
   }
   for (int k = 0; k < 3; ++k) mb.grid_n[k] = n[k] - 1;
   mb.grid = reinterpret_cast<float *>(rec);
+  mb.grid_bytes = int64_t(sizeof(uint4)) * n_cells;
   // Start nodes: reach three cells, so a query whose best distance is up to
   // about 2.1 cells starts below the root.
   int32_t *start = nullptr;
@@ -684,6 +686,7 @@ hipError_t build_distance_grid(MeshBuffers &mb, const double lo[3], const double
     const float valid = 0.999f * (reach - 0.8660254f * mb.grid_h);
     mb.grid_start = start;
     mb.grid_start_r2 = valid * valid;
+    mb.grid_bytes += int64_t(sizeof(int32_t)) * n_cells;
   }
   return hipSuccess;
 }

@@ -85,11 +85,24 @@ struct GeomEntry {
   std::vector<double> spheres, cylinders, cavities;
   DevGeom dev{};
   std::vector<void *> owned;
+  int64_t owned_bytes = 0;
   int32_t kind = kGeomGeneric;
 };
 
+// Device bytes of one site slot: the resident slot arrays plus its entries
+// in the round input and output blocks.
+constexpr int64_t kSlotBytes = 3 * 8 + 4 + 4 + 4 + 2 * 8 + 8 + 8 + 4 + 4 + 4 + 8 + 8 + 8 + 8 * 8 + 8 + 8 + 8 +
+                               int64_t(sizeof(twos_site_result)) + 4 + 4 + 28 * 8 + (4 + 4 + 8) +
+                               int64_t(sizeof(twos_site_result)) + 4;
+
 struct Engine;
 
+}  // namespace twos
+
+struct twos_ctx;
+
+namespace twos {
+int64_t ctx_device_bytes(const twos_ctx *ctx);
 }  // namespace twos
 
 struct twos_ctx {
@@ -104,6 +117,10 @@ struct twos_ctx {
   twos::DevGeom *d_geoms = nullptr;
   int32_t d_geoms_cap = 0;
   std::unique_ptr<twos::Engine> engine;
+  // Every live engine on this context (its own and each server's): graphs
+  // captured by any of them hold d_geoms, and queued or active requests hold
+  // geometry ids.
+  std::vector<twos::Engine *> engines;
 };
 
 namespace twos {
@@ -115,6 +132,7 @@ struct Request {
   int64_t id = 0;
   int32_t geom = 0;
   int32_t problem = -1;
+  int32_t status = TWOS_OK;     // server: how the request ended
   twos_params prm{};
   DevSide sides[2];
   std::vector<double> sites;
@@ -126,6 +144,7 @@ struct Request {
   int64_t remaining = 0;
   int32_t cls = 0;              // combine class
   bool admitted = false;
+  bool capped = false;          // a path of some site hit max_steps
 };
 
 struct Engine {
@@ -133,6 +152,7 @@ struct Engine {
   int64_t cap_sites = 0, cap_walks = 0;
   int32_t cap_problems = 0;
   bool growable = false;
+  bool server = false;  // admission by memory; failed requests are reported, not fatal
 
   SlotArrays s{};
   DevSide *d_sides = nullptr;
@@ -151,6 +171,7 @@ struct Engine {
 
   std::deque<Request *> queue;
   std::vector<Request *> active;
+  std::vector<Request *> failed;  // server: ended without running (reported by the next step)
   std::map<int64_t, int64_t> free_slots;  // start -> length
   std::vector<int32_t> free_problems;
   int64_t next_id = 1;
@@ -162,7 +183,34 @@ struct Engine {
 
   twos_stats stats{};
 
-  ~Engine() { release(); }
+  // Server admission by memory: device bytes charged per admitted site, and
+  // the engine's fixed buffers (walk samples, round blocks).
+  int64_t charged = 0;
+
+  ~Engine() {
+    release();
+    unbind();
+  }
+
+  void bind(twos_ctx *c) {
+    ctx = c;
+    c->engines.push_back(this);
+  }
+
+  void unbind() {
+    if (!ctx) return;
+    auto &v = ctx->engines;
+    v.erase(std::remove(v.begin(), v.end(), this), v.end());
+    ctx = nullptr;
+  }
+
+  bool uses_geometry(int32_t geom) const {
+    for (const Request *r : queue)
+      if (r->geom == geom) return true;
+    for (const Request *r : active)
+      if (r->geom == geom) return true;
+    return false;
+  }
 
   void release() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -230,6 +278,13 @@ struct Engine {
   }
 
   bool idle() const { return queue.empty() && active.empty(); }
+
+  // Device bytes of the buffers every request shares: per-walk samples and
+  // step counts (and gradient samples once allocated), round blocks, sides.
+  int64_t fixed_bytes() const {
+    return cap_walks * 2 * int64_t(sizeof(double) + sizeof(int32_t)) + (d_grad ? cap_walks * 6 * 8 : 0) +
+           int64_t(in_bytes + out_bytes) + int64_t(cap_problems) * int64_t(sizeof(DevSide)) + 512;
+  }
 
   int64_t take_slots(int64_t n) {
     for (auto it = free_slots.begin(); it != free_slots.end(); ++it) {
@@ -348,10 +403,22 @@ struct Engine {
     return TWOS_OK;
   }
 
+  // Admits queued requests in FIFO order while slots and side tables are
+  // free; a server also needs the request's slot bytes to fit the context's
+  // HBM budget beside everything resident (meshes, distance grids,
+  // geometries, every engine's buffers and admitted requests).
   twos_status admit_queued() {
     while (!queue.empty()) {
       Request *r = queue.front();
       if (free_problems.empty()) break;
+      const int64_t need = r->n * kSlotBytes;
+      if (server && ctx_device_bytes(ctx) + need > int64_t(ctx->budget)) {
+        if (!active.empty()) break;  // wait until an admitted request frees its memory
+        queue.pop_front();
+        r->status = fail(TWOS_ERR_ADMISSION, "request does not fit the HBM budget beside the resident geometry");
+        failed.push_back(r);
+        continue;
+      }
       const int64_t slot0 = take_slots(r->n);
       if (slot0 < 0) break;
       r->slot0 = slot0;
@@ -359,7 +426,15 @@ struct Engine {
       free_problems.pop_back();
       queue.pop_front();
       const twos_status st = admit(r);
-      if (st != TWOS_OK) return st;
+      if (st != TWOS_OK) {
+        if (!server) return st;
+        give_slots(r->slot0, r->n);
+        free_problems.push_back(r->problem);
+        r->status = st;
+        failed.push_back(r);
+        continue;
+      }
+      if (server) charged += need;
       active.push_back(r);
     }
     return TWOS_OK;
@@ -433,6 +508,8 @@ struct Engine {
   twos_status step(std::vector<int64_t> &done, double *dump_a = nullptr, double *dump_b = nullptr,
                    int32_t *dump_steps = nullptr, int64_t dump_stride = 0) {
     twos_status st = admit_queued();
+    for (Request *r : failed) done.push_back(r->id);
+    failed.clear();
     if (st != TWOS_OK) return st;
     if (active.empty()) {
       if (!queue.empty()) return fail(TWOS_ERR_ADMISSION, "request does not fit the engine's slots");
@@ -453,9 +530,9 @@ struct Engine {
       // family and traversal counting.  Everything else a result depends on
       // (eps, max_steps, the distance-grid threshold, sides, geometry) is
       // per slot, so a site's result is the same alone and in any mix.
-      const int64_t key = int64_t(r->prm.precision) * 16 + r->prm.combine * 4 + (r->prm.rng != kRngPhilox ? 1 : 0) +
-                          ((r->prm.flags & TWOS_FLAG_COUNT_TRAVERSAL) ? 1024 : 0) +
-                          ((r->prm.flags & TWOS_FLAG_GRADIENT) ? 2048 : 0);
+      const int64_t key = int64_t(r->prm.precision) | int64_t(r->prm.combine) << 8 | int64_t(r->prm.rng) << 12 |
+                          int64_t((r->prm.flags & TWOS_FLAG_COUNT_TRAVERSAL) ? 1 : 0) << 16 |
+                          int64_t((r->prm.flags & TWOS_FLAG_GRADIENT) ? 1 : 0) << 17;
       int c = -1;
       for (size_t k = 0; k < class_key.size(); ++k)
         if (class_key[k] == key) c = int(k);
@@ -667,6 +744,8 @@ struct Engine {
       if (r->remaining == 0) {
         give_slots(r->slot0, r->n);
         free_problems.push_back(r->problem);
+        if (server) charged -= r->n * kSlotBytes;
+        r->status = r->capped ? TWOS_ERR_STEP_CAP : TWOS_OK;
         done.push_back(r->id);
         active.erase(active.begin() + k);
       } else {
@@ -682,6 +761,7 @@ struct Engine {
       r->state[i] = 1;
       r->remaining -= 1;
       if (r->out) r->out[i] = res;
+      if (res.caps > 0) r->capped = true;
     }
   }
 
@@ -821,6 +901,28 @@ Request *make_request(twos_ctx *ctx, int32_t geom, const twos_side sides[2], con
 }
 
 // Context-level helpers.
+int64_t ctx_geometry_bytes(const twos_ctx *ctx) {
+  int64_t b = int64_t(ctx->d_geoms_cap) * int64_t(sizeof(DevGeom));
+  for (const auto &kv : ctx->meshes) b += kv.second.bytes + kv.second.mb.grid_bytes;
+  for (const auto &kv : ctx->geoms) b += kv.second.owned_bytes;
+  return b;
+}
+
+int64_t ctx_engine_bytes(const twos_ctx *ctx) {
+  int64_t b = 0;
+  for (const Engine *e : ctx->engines)
+    b += e->fixed_bytes() + (e->server ? e->charged : e->cap_sites * kSlotBytes);
+  return b;
+}
+
+int64_t ctx_device_bytes(const twos_ctx *ctx) { return ctx_geometry_bytes(ctx) + ctx_engine_bytes(ctx); }
+
+// Captured rounds hold the geometry table's address and the grids it points
+// to: every engine on the context must capture again.
+void invalidate_graphs(twos_ctx *ctx) {
+  for (Engine *e : ctx->engines) e->graph_key = -1;
+}
+
 twos_status upload_geom_table(twos_ctx *ctx) {
   int32_t need = ctx->next_geom;
   if (need > ctx->d_geoms_cap) {
@@ -828,10 +930,12 @@ twos_status upload_geom_table(twos_ctx *ctx) {
     while (cap < need) cap *= 2;
     DevGeom *p = nullptr;
     HIP_OK(dalloc(&p, size_t(cap)));
+    // Rounds of any engine already queued on the stream may read the old table.
+    HIP_OK(hipStreamSynchronize(ctx->stream));
     dfree(ctx->d_geoms);
     ctx->d_geoms = p;
     ctx->d_geoms_cap = cap;
-    if (ctx->engine) ctx->engine->graph_key = -1;  // tables() pointer changed
+    invalidate_graphs(ctx);  // tables() pointer changed
   }
   std::vector<DevGeom> table(size_t(std::max(need, 1)));
   for (auto &kv : ctx->geoms) table[size_t(kv.first)] = kv.second.dev;
@@ -845,6 +949,7 @@ twos_status upload_owned(GeomEntry &g, const T *host, size_t count, T **dev) {
   if (count == 0) return TWOS_OK;
   HIP_OK(dalloc(dev, count));
   g.owned.push_back(*dev);
+  g.owned_bytes += int64_t(sizeof(T) * count);
   HIP_OK(hipMemcpy(*dev, host, sizeof(T) * count, hipMemcpyHostToDevice));
   return TWOS_OK;
 }
@@ -934,10 +1039,10 @@ twos_status ensure_engine(twos_ctx *ctx, const twos_params *p, int64_t n) {
   Engine *e = ctx->engine.get();
   if (e && e->cap_sites >= want_sites && e->cap_walks >= want_walks) return TWOS_OK;
   auto ne = std::make_unique<Engine>();
-  ne->ctx = ctx;
   const int64_t sites = std::max<int64_t>(want_sites, e ? e->cap_sites : 0);
   const int64_t walks = std::max<int64_t>(want_walks, e ? e->cap_walks : 0);
   ctx->engine.reset();
+  ne->bind(ctx);
   const twos_status st = ne->allocate(sites, walks, 64);
   if (st != TWOS_OK) return st;
   ctx->engine = std::move(ne);
@@ -1040,6 +1145,13 @@ void twos_destroy(twos_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   ctx->engine.reset();
+  // Servers still alive lose their device buffers; their handles only accept
+  // twos_server_destroy from now on.
+  for (Engine *e : ctx->engines) {
+    e->release();
+    e->ctx = nullptr;
+  }
+  ctx->engines.clear();
   for (auto &kv : ctx->geoms) free_geom(ctx, kv.second);
   for (auto &kv : ctx->meshes) {
     MeshBuffers &mb = kv.second.mb;
@@ -1149,6 +1261,7 @@ twos_status twos_mesh_set_distance_grid(twos_ctx *ctx, int32_t mesh_id, int32_t 
   nb.grid = nullptr;
   nb.grid_start = nullptr;
   nb.grid_start_r2 = 0.f;
+  nb.grid_bytes = 0;
   hipEvent_t a, b;
   HIP_OK(hipEventCreate(&a));
   HIP_OK(hipEventCreate(&b));
@@ -1183,7 +1296,7 @@ twos_status twos_mesh_set_distance_grid(twos_ctx *ctx, int32_t mesh_id, int32_t 
     g.dev.grid_start = nb.grid_start;
     g.dev.grid_start_r2 = nb.grid_start_r2;
   }
-  if (ctx->engine) ctx->engine->graph_key = -1;
+  invalidate_graphs(ctx);
   if (build_ms) *build_ms = ms;
   return upload_geom_table(ctx);
 }
@@ -1195,6 +1308,17 @@ twos_status twos_mesh_info(twos_ctx *ctx, int32_t mesh_id, int32_t *depth, int64
   if (depth) *depth = it->second.depth;
   if (nodes) *nodes = std::max<int64_t>(it->second.mb.n_tris - 1, 1);
   if (bytes) *bytes = it->second.bytes;
+  return TWOS_OK;
+}
+
+twos_status twos_memory_info(twos_ctx *ctx, int64_t *geometry_bytes, int64_t *engine_bytes, int64_t *budget_bytes,
+                             int64_t *slot_bytes) {
+  if (!ctx) return fail(TWOS_ERR_INVALID_ARGUMENT, "null context");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (geometry_bytes) *geometry_bytes = ctx_geometry_bytes(ctx);
+  if (engine_bytes) *engine_bytes = ctx_engine_bytes(ctx);
+  if (budget_bytes) *budget_bytes = int64_t(ctx->budget);
+  if (slot_bytes) *slot_bytes = kSlotBytes;
   return TWOS_OK;
 }
 
@@ -1289,7 +1413,9 @@ twos_status twos_geometry_release(twos_ctx *ctx, int32_t geom_id) {
   std::lock_guard<std::mutex> lock(ctx->mu);
   auto it = ctx->geoms.find(geom_id);
   if (it == ctx->geoms.end()) return fail(TWOS_ERR_NOT_FOUND, "unknown geometry id");
-  if (ctx->engine && !ctx->engine->idle()) return fail(TWOS_ERR_INVALID_ARGUMENT, "engine busy");
+  for (const Engine *e : ctx->engines)
+    if (e->uses_geometry(geom_id))
+      return fail(TWOS_ERR_INVALID_ARGUMENT, "geometry in use by a queued or active request");
   free_geom(ctx, it->second);
   ctx->geoms.erase(it);
   return upload_geom_table(ctx);
@@ -1392,7 +1518,7 @@ twos_status twos_trace(twos_ctx *ctx, int32_t geom_id, const twos_side *side, co
   if (ctx->geoms.find(geom_id) == ctx->geoms.end()) return fail(TWOS_ERR_NOT_FOUND, "unknown geometry id");
   HIP_OK(hipSetDevice(ctx->device));
   Engine e;
-  e.ctx = ctx;
+  e.bind(ctx);
   twos_status st = e.allocate(1, 64, 1);
   if (st != TWOS_OK) return st;
   Request r;
@@ -1603,16 +1729,23 @@ twos_status twos_cert_distance(twos_ctx *ctx, int32_t geom_id, const double *poi
 }
 
 twos_status twos_server_create(twos_ctx *ctx, int64_t max_sites, int64_t max_walks_per_round, twos_server **out) {
-  if (!ctx || !out || max_sites < 1 || max_walks_per_round < 64)
+  if (!ctx || !out || max_sites < 1 || max_walks_per_round < 64 || max_sites > (int64_t(1) << 30) ||
+      max_walks_per_round > (int64_t(1) << 34))
     return fail(TWOS_ERR_INVALID_ARGUMENT, "invalid server capacity");
-  const uint64_t need = uint64_t(max_walks_per_round) * 24 + uint64_t(max_sites) * 256;
-  if (need > ctx->budget) return fail(TWOS_ERR_ADMISSION, "server capacity exceeds the context's HBM budget");
   std::lock_guard<std::mutex> lock(ctx->mu);
+  // The walk buffers and the slot pool must fit beside what is resident.
+  // Admission then charges each request's slots against the budget, so
+  // geometry built later (a finer distance grid, derived geometries) holds
+  // back requests instead of overcommitting the device.
+  const int64_t need = max_walks_per_round * 2 * int64_t(sizeof(double) + sizeof(int32_t)) + max_sites * kSlotBytes;
+  if (ctx_device_bytes(ctx) + need > int64_t(ctx->budget))
+    return fail(TWOS_ERR_ADMISSION, "server capacity exceeds the context's HBM budget beside the resident geometry");
   HIP_OK(hipSetDevice(ctx->device));
   auto srv = std::make_unique<twos_server>();
   srv->ctx = ctx;
   srv->engine = std::make_unique<Engine>();
-  srv->engine->ctx = ctx;
+  srv->engine->server = true;
+  srv->engine->bind(ctx);
   const twos_status st = srv->engine->allocate(max_sites, max_walks_per_round, 1024);
   if (st != TWOS_OK) return st;
   *out = srv.release();
@@ -1621,8 +1754,14 @@ twos_status twos_server_create(twos_ctx *ctx, int64_t max_sites, int64_t max_wal
 
 void twos_server_destroy(twos_server *srv) {
   if (!srv) return;
-  std::lock_guard<std::mutex> lock(srv->ctx->mu);
-  (void)hipStreamSynchronize(srv->ctx->stream);
+  twos_ctx *ctx = srv->engine ? srv->engine->ctx : nullptr;
+  if (!ctx) {  // its context is gone, and with it the device buffers
+    delete srv;
+    return;
+  }
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   srv->engine.reset();
   delete srv;
 }
@@ -1634,6 +1773,7 @@ twos_status twos_server_submit(twos_server *srv, int32_t geom_id, const twos_sid
   if (st != TWOS_OK) return st;
   if (params->schedule != TWOS_PERSISTENT) return fail(TWOS_ERR_INVALID_ARGUMENT, "the server runs the persistent schedule");
   if (n < 1) return fail(TWOS_ERR_INVALID_ARGUMENT, "a request needs at least one site");
+  if (!srv->engine->ctx) return fail(TWOS_ERR_INVALID_ARGUMENT, "the server's context was destroyed");
   std::lock_guard<std::mutex> lock(srv->ctx->mu);
   Engine *e = srv->engine.get();
   if (n > e->cap_sites) return fail(TWOS_ERR_ADMISSION, "request has more sites than the server's slots");
@@ -1648,23 +1788,33 @@ twos_status twos_server_submit(twos_server *srv, int32_t geom_id, const twos_sid
   return TWOS_OK;
 }
 
-twos_status twos_server_step(twos_server *srv, int64_t *done_ids, int32_t max_done, int32_t *n_done) {
+twos_status twos_server_step_status(twos_server *srv, int64_t *done_ids, int32_t *done_status, int32_t max_done,
+                                    int32_t *n_done) {
   if (!srv || !n_done) return fail(TWOS_ERR_INVALID_ARGUMENT, "null argument");
+  *n_done = 0;
+  if (!srv->engine->ctx) return fail(TWOS_ERR_INVALID_ARGUMENT, "the server's context was destroyed");
   std::lock_guard<std::mutex> lock(srv->ctx->mu);
   HIP_OK(hipSetDevice(srv->ctx->device));
   std::vector<int64_t> done;
   const twos_status st = srv->engine->step(done);
-  *n_done = 0;
   for (int64_t id : done) {
-    if (done_ids && *n_done < max_done) done_ids[*n_done] = id;
+    if (*n_done < max_done) {
+      if (done_ids) done_ids[*n_done] = id;
+      if (done_status) done_status[*n_done] = srv->requests.at(id)->status;
+    }
     *n_done += 1;
     srv->requests.erase(id);
   }
   return st;
 }
 
+twos_status twos_server_step(twos_server *srv, int64_t *done_ids, int32_t max_done, int32_t *n_done) {
+  return twos_server_step_status(srv, done_ids, nullptr, max_done, n_done);
+}
+
 twos_status twos_server_pending(twos_server *srv, int64_t *queued, int64_t *active) {
   if (!srv) return fail(TWOS_ERR_INVALID_ARGUMENT, "null argument");
+  if (!srv->engine->ctx) return fail(TWOS_ERR_INVALID_ARGUMENT, "the server's context was destroyed");
   std::lock_guard<std::mutex> lock(srv->ctx->mu);
   if (queued) *queued = int64_t(srv->engine->queue.size());
   if (active) *active = int64_t(srv->engine->active.size());

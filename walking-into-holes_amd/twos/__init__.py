@@ -368,6 +368,14 @@ class Context:
         _check(_lib().twos_geometry_release(self._h, geom))
         self._geom_cavities.pop(geom, None)
 
+    def memory(self) -> dict:
+        """Device bytes the context accounts for (resident geometry, engines),
+        its HBM budget and the bytes one admitted site costs; a server admits
+        a request only while geometry + engines + its sites fit the budget."""
+        g, e, b, s = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        _check(_lib().twos_memory_info(self._h, C.byref(g), C.byref(e), C.byref(b), C.byref(s)))
+        return {"geometry_bytes": g.value, "engine_bytes": e.value, "budget_bytes": b.value, "slot_bytes": s.value}
+
     def release_mesh(self, mesh: Mesh | int) -> None:
         _check(_lib().twos_mesh_release(self._h, mesh.id if isinstance(mesh, Mesh) else int(mesh)))
 
@@ -553,14 +561,19 @@ class Server:
         return rid.value
 
     def step(self) -> dict[int, Result]:
-        """Run one round; returns the requests that completed in it."""
-        ids = (C.c_int64 * 4096)()
+        """Run one round; returns the requests that ended in it.  Each
+        result's stats["status"] is 0, ERR_STEP_CAP (a path hit max_steps;
+        results written) or the error that kept the request from running
+        (ERR_ADMISSION); the other requests go on either way."""
+        cap = max(1, len(self._pending))
+        ids = (C.c_int64 * cap)()
+        codes = (C.c_int32 * cap)()
         nd = C.c_int32()
-        _check(_lib().twos_server_step(self._h, ids, 4096, C.byref(nd)))
+        _check(_lib().twos_server_step_status(self._h, ids, codes, cap, C.byref(nd)))
         done = {}
-        for k in range(min(nd.value, 4096)):
+        for k in range(min(nd.value, cap)):
             out, n, _, _ = self._pending.pop(ids[k])
-            done[ids[k]] = _result(out, n, {})
+            done[ids[k]] = _result(out, n, {"status": int(codes[k])})
         return done
 
     def pending(self) -> tuple[int, int]:

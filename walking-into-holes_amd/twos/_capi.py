@@ -101,6 +101,7 @@ SIGNATURES = {
     "twos_mesh_info": (C.c_int32, [_ctx_p, C.c_int32, _i32p, _i64p, _i64p]),
     "twos_mesh_set_distance_grid": (C.c_int32, [_ctx_p, C.c_int32, C.c_int32, C.c_int32, _dp]),
     "twos_mesh_release": (C.c_int32, [_ctx_p, C.c_int32]),
+    "twos_memory_info": (C.c_int32, [_ctx_p, _i64p, _i64p, _i64p, _i64p]),
     "twos_geometry_create": (C.c_int32, [_ctx_p, C.POINTER(GeometryDesc), _i32p]),
     "twos_geometry_union_sphere": (C.c_int32, [_ctx_p, C.c_int32, _dp, C.c_double, _i32p]),
     "twos_geometry_release": (C.c_int32, [_ctx_p, C.c_int32]),
@@ -122,6 +123,7 @@ SIGNATURES = {
     "twos_server_submit": (C.c_int32, [_srv_p, C.c_int32, C.POINTER(Side), C.POINTER(Params), _dp, C.c_int64,
                                        C.POINTER(SiteResult), _i64p]),
     "twos_server_step": (C.c_int32, [_srv_p, _i64p, C.c_int32, _i32p]),
+    "twos_server_step_status": (C.c_int32, [_srv_p, _i64p, _i32p, C.c_int32, _i32p]),
     "twos_server_pending": (C.c_int32, [_srv_p, _i64p, _i64p]),
 }
 
