@@ -75,6 +75,11 @@ def parse() -> argparse.Namespace:
                     help="fixed distance-grid mode: 0 centres, 1 corners, 2 binary16 corner records")
     ap.add_argument("--exact", action="store_true", help="exact closest-point radius at every mesh step")
     ap.add_argument("--sites", type=int, default=0, help="override the site count (profiling runs)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the live rocprofv3 --pmc traffic passes (roofline.traffic from the committed profile)")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="internal: one eager estimate of the workload at its tuned shape, for a counter pass")
+    ap.add_argument("--fp64-sites", type=int, default=256, help="binary64 rerun sample (0: skip)")
     return ap.parse_args()
 
 
@@ -104,6 +109,8 @@ def main() -> int:
         return serve(args)
     if args.workload == "copy":
         return copy_probe()
+    if args.traffic_probe:
+        return traffic_probe(args)
     if args.workload in ("diffwos", "certify"):
         return f4_bench(args)
     if args.workload == "design":
@@ -133,6 +140,7 @@ def main() -> int:
     # before warmup), unless fixed on the command line.
     shape = {}
     tuning = None
+    grid_build_ms = 0.0
     fixed = {k: v for k, v in (("waves_per_simd", args.waves_per_simd), ("block", args.block),
                                ("lds_stack", args.lds_stack), ("grid_kappa", args.grid_kappa),
                                ("step_batch", args.step_batch))
@@ -140,7 +148,8 @@ def main() -> int:
     if fixed or args.grid_cells > 0 or args.grid_corners >= 0:
         shape = fixed
         if (args.grid_cells > 0 or args.grid_corners >= 0) and mesh is not None:
-            ctx.set_distance_grid(mesh, args.grid_cells if args.grid_cells > 0 else 128, max(args.grid_corners, 0))
+            grid_build_ms = ctx.set_distance_grid(mesh, args.grid_cells if args.grid_cells > 0 else 128,
+                                                  max(args.grid_corners, 0))
     elif not args.no_autotune:
         from twos import autotune
         t0 = time.perf_counter()
@@ -150,6 +159,7 @@ def main() -> int:
                                    cache_path=Path(args.tuning_cache) if args.retune else TUNING_CACHE)
         shape = {k: v for k, v in tuning.choice.items()
                  if k not in ("grid_cells", "grid_corners") and (grid_on or k != "grid_kappa")}
+        grid_build_ms = tuning.grid_build_ms
         log(f"autotune {time.perf_counter() - t0:.1f} s: {tuning.choice}, "
             f"{tuning.speedup:.3f}x the default shape per path")
 
@@ -206,7 +216,14 @@ def main() -> int:
         grid_bytes = {0: GRID_BYTES, 1: 8 * GRID_BYTES, 2: 16}[out_shape.get("grid_corners", 0)]
         b_query = NODE_BYTES * n_nodes + TRI_BYTES * n_tris + grid_bytes * n_grid
         achieved = queries * b_query / (eager_ms * 1e-3) / 1e9
-        traffic = measured_traffic(w.name, n, out_shape, launches)
+        if args.no_traffic:
+            traffic, traffic_source = measured_traffic(w.name, n, out_shape, launches), "committed profile"
+        else:
+            traffic, traffic_source = live_traffic(w.name, n, out_shape, launches)
+            if traffic is None:
+                committed = measured_traffic(w.name, n, out_shape, launches)
+                if committed is not None:
+                    traffic, traffic_source = committed, f"committed profile ({traffic_source})"
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "algorithmic_bytes_per_launch": round(queries * b_query / launches),
@@ -215,7 +232,19 @@ def main() -> int:
                     "traversing_query_fraction": round(exact, 3),
                     "nodes_per_traversing_query": round(n_nodes / max(exact, 1e-9), 2),
                     "walk_kernel_ms_per_launch": round(eager_ms / launches, 3),
-                    "launches": launches}
+                    "launches": launches, "traffic_source": traffic_source,
+                    "traffic_x_algorithmic": (round(traffic / (queries * b_query / launches), 3)
+                                              if traffic else None)}
+        frozen = frozen_unit(w.name)
+        if frozen is not None:
+            # The same queries priced at the bytes an exact BVH closest-point
+            # query costs (the unit frozen from the oracle's BVH): how much
+            # query work per second the walk kernel delivers in the terms of
+            # the reference's Embree query, against the same 8 TB/s.
+            eq = queries * frozen["frozen_bytes_per_query"] / (eager_ms * 1e-3) / 1e9
+            roofline.update({"frozen_bytes_per_query": round(frozen["frozen_bytes_per_query"], 1),
+                             "achieved_frozen": round(eq, 1), "frac_frozen": round(eq / HBM_PEAK_GBS, 4),
+                             "frozen_unit": "tests/golden/roofline_c3.json (oracle BVH, exact-walk query mix)"})
     else:
         flops = 23 if w.precision == 64 else 177
         peak = FP64_VALU_PEAK if w.precision == 64 else FP32_VALU_PEAK
@@ -252,7 +281,17 @@ def main() -> int:
         out["autotune"] = {"choice": tuning.choice, "speedup_vs_default_shape": round(tuning.speedup, 4),
                            "trials": len(tuning.trials)}
     if mesh is not None:
+        mem = ctx.memory()
+        mean_step = total_t / args.steps
+        build_s = (mesh.build_ms + grid_build_ms) * 1e-3
         out["lbvh_build_ms"] = round(mesh.build_ms, 3)
+        out["geometry"] = {"lbvh_build_ms": round(mesh.build_ms, 3), "grid_build_ms": round(grid_build_ms, 3),
+                           "lbvh_bytes": mesh.bvh_bytes, "geometry_bytes": mem["geometry_bytes"],
+                           "grid_bytes": mem["geometry_bytes"] - mesh.bvh_bytes,
+                           "engine_bytes": mem["engine_bytes"],
+                           "sites_per_s_cold": round(float(np.mean(done)) / (mean_step + build_s), 2)}
+    if w.precision == 32 and args.fp64_sites > 0:
+        out["fp64_rerun"] = fp64_rerun(ctx, geom, sites, sides, kw, shape, args.fp64_sites)
     if not args.no_plain:
         rng = np.random.default_rng(1)
         idx = np.sort(rng.choice(n, size=min(args.plain_sites, n), replace=False))
@@ -382,18 +421,148 @@ def design_bench(args) -> int:
     return 0
 
 
+COPY_BYTES = 256 << 20
+
+
 def copy_probe() -> int:
-    """A plain device-to-device copy of 256 MiB, three times: the reference
-    reading of rocprofv3's FETCH_SIZE / WRITE_SIZE for bytes of known count."""
-    import torch
-    n = 256 << 20
-    a = torch.ones(n, dtype=torch.uint8, device="cuda")
-    b = torch.empty_like(a)
+    """A plain device-to-device copy of 256 MiB, three times (the HIP runtime's
+    copy kernel): the reading of rocprofv3's FETCH_SIZE / WRITE_SIZE for bytes
+    of known count."""
+    import ctypes as C
+    hip = None
+    for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+        try:
+            hip = C.CDLL(name)
+            break
+        except OSError:
+            continue
+    if hip is None:
+        raise SystemExit("libamdhip64.so not found")
+    a, b = C.c_void_p(), C.c_void_p()
+    n = C.c_size_t(COPY_BYTES)
+    for call in (lambda: hip.hipMalloc(C.byref(a), n), lambda: hip.hipMalloc(C.byref(b), n),
+                 lambda: hip.hipMemset(a, 1, n)):
+        if call() != 0:
+            raise SystemExit("HIP allocation failed")
     for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes": n, "copies": 3}))
+        if hip.hipMemcpy(b, a, n, 3) != 0:  # hipMemcpyDeviceToDevice
+            raise SystemExit("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": COPY_BYTES, "copies": 3}))
     return 0
+
+
+def shape_args(shape: dict) -> list:
+    """bench.py flags that fix a launch shape and distance grid."""
+    flags = {"block": "--block", "waves_per_simd": "--waves-per-simd", "lds_stack": "--lds-stack",
+             "step_batch": "--step-batch", "grid_kappa": "--grid-kappa", "grid_cells": "--grid-cells",
+             "grid_corners": "--grid-corners"}
+    out = []
+    for k, f in flags.items():
+        if k in shape:
+            out += [f, str(shape[k])]
+    return out
+
+
+def traffic_probe(args) -> int:
+    """One eager estimate of the workload at the launch shape given on the
+    command line: the walk launches a counter pass reads."""
+    import twos
+    from twos import workloads as W
+    w = build_workload(args.workload)
+    sites = w.sites if args.sites <= 0 else w.sites[:args.sites]
+    ctx = twos.Context()
+    mesh = ctx.mesh(w.vertices, w.faces)
+    geom = ctx.geometry(twos.MESH, mesh=mesh)
+    grid = args.grid_cells > 0 and not args.exact
+    if grid:
+        ctx.set_distance_grid(mesh, args.grid_cells, max(args.grid_corners, 0))
+    shape = dict(block=args.block, waves_per_simd=max(args.waves_per_simd, 0), lds_stack=args.lds_stack,
+                 step_batch=max(args.step_batch, 0), grid_kappa=args.grid_kappa if grid else 0.0)
+    res = ctx.estimate(geom, sites, W.sides(w, twos.side), combine=twos.CROSS, precision=w.precision, seed=2026,
+                       walks=w.walks, walks_max=w.walks_max, ci_rel=w.ci_rel, eps=w.eps, max_steps=w.max_steps,
+                       distance_grid=grid, **shape)
+    ctx.close()
+    print(json.dumps({"traffic_probe": True, "walk_launches": res.stats["walk_launches"]}), flush=True)
+    return 0
+
+
+def _pmc_pass(counter: str, program: list, timeout: float) -> list:
+    """(kernel name, counter value) per dispatch of one rocprofv3 --pmc pass
+    over `program` (a child process; the counter CSV is read even when the
+    profiler itself exits non-zero after the program finished)."""
+    import csv
+    import subprocess
+    import tempfile
+    out = Path(tempfile.mkdtemp(prefix="twos_pmc_", dir="/tmp"))
+    cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", str(out), "-o", "run", "--", *program]
+    env = dict(os.environ, TMPDIR="/tmp")
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, cwd="/tmp", env=env)
+    files = sorted(out.rglob("*counter_collection.csv"))
+    if not files:
+        raise RuntimeError(f"rocprofv3 --pmc {counter}: no counter file (rc {r.returncode}): {r.stderr[-300:]}")
+    rows = []
+    for f in files:
+        with f.open() as fh:
+            for row in csv.DictReader(fh):
+                if row.get("Counter_Name") == counter:
+                    rows.append((row["Kernel_Name"], float(row["Counter_Value"])))
+    return rows
+
+
+def live_traffic(workload: str, n_sites: int, shape: dict, launches: int):
+    """HBM bytes per walk-kernel launch measured in this run: two rocprofv3
+    --pmc passes (FETCH_SIZE, WRITE_SIZE, KiB per dispatch) over the same
+    estimate at the same shape, each scaled by what the counter reads for the
+    256-MiB device-to-device copy (two more passes).  Returns (bytes, source)
+    or (None, why not)."""
+    import shutil
+    if shutil.which("rocprofv3") is None:
+        return None, "rocprofv3 not on PATH"
+    probe = [sys.executable, str(ROOT / "bench.py"), "--workload", "sh_icosphere", "--traffic-probe",
+             "--sites", str(n_sites), *shape_args(shape), *([] if "grid_cells" in shape else ["--exact"])]
+    copy = [sys.executable, str(ROOT / "bench.py"), "--workload", "copy"]
+    try:
+        per = {}
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            t0 = time.perf_counter()
+            walk = [v for k, v in _pmc_pass(counter, probe, 240) if "walk_persistent<float" in k]
+            cal = [v for k, v in _pmc_pass(counter, copy, 120) if "copyBuffer" in k]
+            log(f"traffic pass {counter}: {len(walk)} walk launches, {len(cal)} copies, "
+                f"{time.perf_counter() - t0:.1f} s")
+            if len(walk) != launches or not cal:
+                return None, f"{counter}: {len(walk)} walk launches (expected {launches}), {len(cal)} copies"
+            scale = (COPY_BYTES / 1024) / float(np.median(cal))  # KiB of data per KiB counted
+            per[counter] = sum(walk) * scale * 1024 / launches
+        return int(per["FETCH_SIZE"] + per["WRITE_SIZE"]), "rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE in this run"
+    except Exception as e:  # noqa: BLE001 - a profiler failure must not fail the benchmark
+        return None, f"live measurement failed: {e}"[:300]
+
+
+def frozen_unit(workload: str):
+    path = ROOT / "tests" / "golden" / "roofline_c3.json"
+    if not path.exists():
+        return None
+    d = json.loads(path.read_text())
+    return d if d.get("workload") == workload else None
+
+
+def fp64_rerun(ctx, geom, sites, sides, kw, shape, count: int) -> dict:
+    """The binary32 walks' error stated against a binary64 GPU rerun of the
+    same Philox streams (same shape, grid and threshold): every count-th
+    site at a fixed 4096 walks per side."""
+    import twos
+    sub = sites[:: max(1, len(sites) // count)][:count]
+    fixed = {**kw, "walks": 4096, "walks_max": 0}
+    r32 = ctx.estimate(geom, sub, sides, schedule=twos.PERSISTENT, **{**fixed, "precision": 32}, **shape)
+    r64 = ctx.estimate(geom, sub, sides, schedule=twos.PERSISTENT, **{**fixed, "precision": 64}, **shape)
+    dev = np.abs(r32.score - r64.score) / r64.score_se
+    return {"sites": len(sub), "walks_per_side": 4096, "max_abs_diff_in_se": round(float(np.max(dev)), 4),
+            "median_abs_diff_in_se": round(float(np.median(dev)), 5),
+            "max_rel_diff": float(np.max(np.abs(r32.score - r64.score) / np.abs(r64.score))),
+            "steps_max_rel_diff": float(np.max(np.abs(r32.steps - r64.steps) / r64.steps))}
 
 
 def measured_traffic(workload: str, n_sites: int, shape: dict, launches: int):

@@ -78,3 +78,45 @@ def test_production_design_and_validation(env):
     # of magnitude are checked.
     assert v.delta_j > 0 and v.per_capacity > 3 * v.per_capacity_se
     assert 0.2 < v.per_capacity / predicted < 5
+
+
+def test_c3_design_loop_with_cavities_and_distance_grid():
+    """Configuration 5 in small on the 81,920-triangle SH icosphere, at the
+    launch shape and distance grid the committed tuning cache picks for it:
+    two rounds over 4096 candidates (scout 512 walks per side, the top 8
+    rescored with 4096 fresh walks, LCB acceptance, radius-0.02 cavities), so
+    round 2 walks the mesh minus the first cavity with the grid on.  Each
+    round's rescoring agrees with the oracle's exact binary64 walk on that
+    geometry within combined SEs, and a binary64 GPU rerun on the same
+    Philox streams differs from the binary32 rescoring by under 0.5 SE
+    (run_study.py:77-106; internal_cooling/run_reference_benchmark.py:196-245)."""
+    import oracle as O
+    from twos import workloads as W
+    from tuned_shape import tuned_choice, tuned_shape
+
+    choice, shape = tuned_choice(), tuned_shape()
+    w = W.sh_icosphere(n_sites=4096)
+    with twos.Context() as ctx:
+        mesh = ctx.mesh(w.vertices, w.faces)
+        ctx.set_distance_grid(mesh, choice["grid_cells"], choice["grid_corners"])
+        sides = W.sides(w, twos.side)
+        est = D.GpuDesign(ctx, mesh, sides, radius=0.02, eps=w.eps, max_steps=w.max_steps, shape=shape)
+        cfg = D.GreedyConfig(rounds=2, radius=0.02, clearance=0.025, scout_walks=512, shortlist=8,
+                             rescore_walks=4096, seed=2026, select="lcb")
+        out = D.greedy(est, w.sites, cfg)
+        assert len(out.centers) == 2 and len(out.rounds[1].feasible) < len(out.rounds[0].feasible)
+        est64 = D.GpuDesign(ctx, mesh, sides, radius=0.02, eps=w.eps, max_steps=w.max_steps, precision=64,
+                            shape=shape)
+        omesh = O.Mesh(w.vertices, w.faces)
+        for k, r in enumerate(out.rounds):
+            pts = w.sites[r.feasible][r.shortlist]
+            holes = out.centers[:k]
+            geo = O.Geometry(O.MESH, mesh=omesh, cavities=holes if k else None, cavity_radius=0.02)
+            cpu = O.estimate(geo, pts, W.sides(w, O.side), combine=O.CROSS, rng=O.RNG_PHILOX, seed=991 + k,
+                             walks=8192, eps=w.eps, max_steps=w.max_steps)
+            z = (r.rescore_score - cpu["score"]) / np.hypot(r.rescore_se, cpu["score_se"])
+            s64, se64 = est64(pts, holes, cfg.rescore_walks, cfg.seed + cfg.seed_step * k + cfg.rescore_offset)
+            dev = np.abs(r.rescore_score - np.asarray(s64)) / np.asarray(se64)
+            print("round", k, "z vs oracle", np.round(z, 2).tolist(), "binary64 rerun", float(dev.max()), flush=True)
+            assert np.all(np.isfinite(z)) and np.sum(np.abs(z) > 3) <= 1 and np.max(np.abs(z)) < 5
+            assert np.max(dev) < 0.5

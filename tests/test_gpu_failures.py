@@ -61,3 +61,16 @@ def test_unknown_geometry(env):
     ctx, ico, g = env
     with pytest.raises(twos.NotFoundError):
         ctx.estimate(12345, ico.sites[:2], W.sides(ico, twos.side), walks=16)
+
+
+def test_grid_rebuilt_before_any_geometry_exists(golden):
+    """A fresh context: rebuilding a mesh's distance grid before any geometry
+    is created succeeds, and a geometry created afterwards sees that grid."""
+    m = np.load(golden / "bunny_1000.npz")
+    with twos.Context() as ctx:
+        mesh = ctx.mesh(m["vertices"], m["faces"])
+        assert ctx.set_distance_grid(mesh, 64, 2) > 0.0
+        g = ctx.geometry(twos.MESH, mesh=mesh)
+        c = m["vertices"].mean(0)
+        assert np.isfinite(ctx.distance_bound(g, [c])[0])
+        ctx.mesh(m["vertices"], m["faces"])  # a later build on the same thread still works

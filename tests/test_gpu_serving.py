@@ -240,3 +240,43 @@ def test_admission_by_memory_counts_resident_geometry(golden):
         rd = srv.submit(g, sites, sides, seed=4, **kw)
         assert _drain(srv)[rd].stats["status"] == 0
         srv.close()
+
+
+def test_stream_run_equals_standalone_estimates():
+    """Configuration 4 in small: a Poisson stream of 40 requests over the
+    three resident geometries (unit ball, cooling block, SH icosphere),
+    validations included, replayed through serving.run.  Every request
+    finishes, each result equals its standalone estimate bit for bit, each
+    validation's derived geometry is released, and a validation's objective
+    change has the sign the maximum principle gives."""
+    from twos import serving
+
+    with twos.Context() as ctx:
+        stream = serving.Stream(ctx, twos, n_requests=40, rate=50.0, seed=5, max_sites=256, validation_frac=0.3)
+        srv = ctx.server(max_sites=8192, max_walks_per_round=1 << 22)
+        out = serving.run(stream, srv)
+        srv.close()
+        print({k: out[k] for k in ("rounds", "p50_ms", "p99_ms", "validations", "failed")}, flush=True)
+        assert out["requests"] == 40 and out["failed"] == 0
+        assert {r.geom_name for r in stream.requests if r.kind == "td"} == set(stream.geoms)
+        vals = [r for r in stream.requests if r.kind == "validation"]
+        assert len(vals) >= 3
+        for req in stream.requests:
+            assert np.isfinite(req.finish) and len(req.results) == 1
+            (_, got), = req.results
+            g = stream.derived_geometry(req) if req.kind == "validation" else stream.geoms[req.geom_name][1]
+            geom, sites, sides, kw = stream.request_args(req, g)
+            ref = ctx.estimate(geom, sites, sides, **kw)
+            for k in KEYS:
+                np.testing.assert_array_equal(getattr(got, k), getattr(ref, k), err_msg=f"{req.rid} {req.kind} {k}")
+            if req.kind == "validation":
+                ctx.release_geometry(g)
+                with pytest.raises(twos.NotFoundError):
+                    ctx.release_geometry(req.extra["derived"])  # released by serving.run
+        # The cooling block's and the icosphere's primal data are >= 0 (u = 1 on
+        # holes, 0 on faces; x + 0.3 y^2 on the mesh in the unit box) and the
+        # cavity holds u = 0, so by the maximum principle cutting it can only
+        # lower J (the unit ball's harmonic data changes sign).
+        signed = [serving.validation_change(r) for r in vals if r.geom_name != "unit_ball"]
+        for change, se in signed:
+            assert change > -3 * se, (change, se)

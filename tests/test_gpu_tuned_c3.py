@@ -15,26 +15,18 @@ The query the grid stands in for is the reference's exact closest-point query
 (benchmarks/compiled_bunny/src/wih_twos.cpp:339-359) inside walk()
 (:635-664)."""
 
-import json
-from pathlib import Path
-
 import numpy as np
 import pytest
 
 import oracle as O
+from tuned_shape import SHAPE_KEYS, tuned_choice
 
 pytestmark = pytest.mark.gpu
 
 twos = pytest.importorskip("twos")
 from twos import workloads as W  # noqa: E402
 
-CACHE = Path(__file__).resolve().parents[1] / "walking-into-holes_amd" / "twos" / "tuning_cache.json"
-KEY = "dev0|sh_icosphere_81920|p32|c2|e0.0001|g1|m"
 KEYS = ("u", "v", "score", "score_se", "td", "same", "walks", "steps", "caps", "status")
-
-
-def tuned_choice() -> dict:
-    return json.loads(CACHE.read_text())[KEY]["choice"]
 
 
 @pytest.fixture(scope="module")
@@ -46,7 +38,7 @@ def env():
     geom = ctx.geometry(twos.MESH, mesh=mesh)
     print("tuned choice", choice, flush=True)
     ctx.set_distance_grid(mesh, choice["grid_cells"], choice["grid_corners"])
-    shape = {k: choice[k] for k in ("block", "waves_per_simd", "lds_stack", "step_batch", "grid_kappa")}
+    shape = {k: choice[k] for k in SHAPE_KEYS}
     yield ctx, w, mesh, geom, choice, shape
     ctx.close()
 

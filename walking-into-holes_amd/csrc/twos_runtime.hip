@@ -34,10 +34,22 @@ twos_status fail(twos_status code, const std::string &msg) {
   return code;
 }
 
+// A device failure: also clears the thread's last-error state (see HIP_OK).
+twos_status dev_fail(const char *what, hipError_t e) {
+  (void)hipGetLastError();
+  return fail(TWOS_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// A failed runtime call also leaves the thread's last-error state set;
+// clearing it keeps the failure from being reported again by the next
+// launch check of a later, unrelated call.
 #define HIP_OK(x)                                                                             \
   do {                                                                                        \
     const hipError_t e_ = (x);                                                                \
-    if (e_ != hipSuccess) return fail(TWOS_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    if (e_ != hipSuccess) {                                                                   \
+      (void)hipGetLastError();                                                                \
+      return fail(TWOS_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_));           \
+    }                                                                                         \
   } while (0)
 
 template <typename T>
@@ -499,7 +511,7 @@ struct Engine {
       stats.block = block;
       stats.waves_per_simd = p.waves_per_simd;
     }
-    if (e != hipSuccess) return fail(TWOS_ERR_DEVICE, std::string("walk launch: ") + hipGetErrorString(e));
+    if (e != hipSuccess) return dev_fail("walk launch", e);
     stats.walk_launches += 1;
     return TWOS_OK;
   }
@@ -615,7 +627,7 @@ struct Engine {
           const hipError_t le = en.r->prm.rng != kRngPhilox ? launch_walk_simple_f64(kind, true, tb, wp, 64, strm)
                                 : en.r->prm.precision == 64 ? launch_walk_simple_f64(kind, false, tb, wp, block, strm)
                                                             : launch_walk_simple_f32(kind, tb, wp, block, strm);
-          if (le != hipSuccess) return fail(TWOS_ERR_DEVICE, std::string("walk launch: ") + hipGetErrorString(le));
+          if (le != hipSuccess) return dev_fail("walk launch", le);
           stats.walk_launches += 1;
           HIP_OK(hipEventRecord(ev[1], strm));
           if (dump_a) {
@@ -925,6 +937,7 @@ void invalidate_graphs(twos_ctx *ctx) {
 
 twos_status upload_geom_table(twos_ctx *ctx) {
   int32_t need = ctx->next_geom;
+  if (need == 0) return TWOS_OK;  // no geometry yet (a mesh's grid may be rebuilt before any)
   if (need > ctx->d_geoms_cap) {
     int32_t cap = std::max(16, ctx->d_geoms_cap * 2);
     while (cap < need) cap *= 2;
@@ -1227,7 +1240,7 @@ twos_status twos_mesh_create(twos_ctx *ctx, const double *vertices, int64_t nv, 
   (void)hipEventDestroy(b);
   (void)hipFree(d_v);
   (void)hipFree(d_f);
-  if (e != hipSuccess) return fail(TWOS_ERR_DEVICE, std::string("LBVH build: ") + hipGetErrorString(e));
+  if (e != hipSuccess) return dev_fail("LBVH build", e);
   m.build_ms = ms;
   m.bytes = int64_t(nf > 1 ? nf - 1 : 1) * (64 + 128) + nf * (36 + 72 + 4);
   // The 4-wide traversal stacks at most three siblings per 4-wide level.
@@ -1276,7 +1289,7 @@ twos_status twos_mesh_set_distance_grid(twos_ctx *ctx, int32_t mesh_id, int32_t 
   if (e != hipSuccess) {
     if (nb.grid) (void)hipFree(nb.grid);
     if (nb.grid_start) (void)hipFree(nb.grid_start);
-    return fail(TWOS_ERR_DEVICE, std::string("distance grid: ") + hipGetErrorString(e));
+    return dev_fail("distance grid", e);
   }
   (void)hipFree(m.mb.grid);
   if (m.mb.grid_start) (void)hipFree(m.mb.grid_start);
@@ -1453,7 +1466,7 @@ twos_status twos_inside(twos_ctx *ctx, int32_t mesh_id, const double *points, in
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   (void)hipFree(d_p);
   (void)hipFree(d_o);
-  if (e != hipSuccess) return fail(TWOS_ERR_DEVICE, std::string("inside: ") + hipGetErrorString(e));
+  if (e != hipSuccess) return dev_fail("inside", e);
   for (int64_t i = 0; i < n; ++i)
     if (out[i] < 0) return fail(TWOS_ERR_GEOMETRY, "ray-parity traversal exceeded 128 intersections");
   return TWOS_OK;
@@ -1481,7 +1494,7 @@ twos_status twos_closest(twos_ctx *ctx, int32_t geom_id, int32_t precision, cons
   if (e == hipSuccess) e = hipMemcpyAsync(cls, d_k, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   (void)hipFree(d_p); (void)hipFree(d_d); (void)hipFree(d_c); (void)hipFree(d_k);
-  if (e != hipSuccess) return fail(TWOS_ERR_DEVICE, std::string("closest: ") + hipGetErrorString(e));
+  if (e != hipSuccess) return dev_fail("closest", e);
   return TWOS_OK;
 }
 
@@ -1501,7 +1514,7 @@ twos_status twos_distance_bound(twos_ctx *ctx, int32_t geom_id, int32_t precisio
   if (e == hipSuccess) e = hipMemcpyAsync(bound, d_b, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   (void)hipFree(d_p); (void)hipFree(d_b);
-  if (e != hipSuccess) return fail(TWOS_ERR_DEVICE, std::string("distance bound: ") + hipGetErrorString(e));
+  if (e != hipSuccess) return dev_fail("distance bound", e);
   return TWOS_OK;
 }
 
@@ -1568,7 +1581,7 @@ twos_status twos_trace(twos_ctx *ctx, int32_t geom_id, const twos_side *side, co
   if (he == hipSuccess) he = hipMemcpyAsync(&steps, e.d_steps, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream);
   if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
   (void)hipFree(d_rows);
-  if (he != hipSuccess) return fail(TWOS_ERR_DEVICE, std::string("trace: ") + hipGetErrorString(he));
+  if (he != hipSuccess) return dev_fail("trace", he);
   *n_rows = std::min((steps & 0x3fffffff) + 1, max_rows);
   return TWOS_OK;
 }
@@ -1724,7 +1737,7 @@ twos_status twos_cert_distance(twos_ctx *ctx, int32_t geom_id, const double *poi
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   dfree(d_pts);
   dfree(d_out);
-  if (e != hipSuccess) return fail(TWOS_ERR_DEVICE, std::string("cert distance: ") + hipGetErrorString(e));
+  if (e != hipSuccess) return dev_fail("cert distance", e);
   return TWOS_OK;
 }
 

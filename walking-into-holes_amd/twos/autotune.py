@@ -68,6 +68,7 @@ class Tuning:
     ms_per_path: float
     default_ms_per_path: float
     trials: list = field(default_factory=list)  # (config, ms per path)
+    grid_build_ms: float = 0.0  # build of the chosen distance grid when it was last applied to the mesh
 
     @property
     def speedup(self) -> float:
@@ -111,7 +112,7 @@ def autotune(ctx, geom: int, sites, sides, *, geom_sig: str, combine: int, preci
 
     def applied(t: "Tuning") -> "Tuning":
         if tune_cells:
-            ctx.set_distance_grid(mesh, t.choice["grid_cells"], int(t.choice["grid_corners"]))
+            t.grid_build_ms = ctx.set_distance_grid(mesh, t.choice["grid_cells"], int(t.choice["grid_corners"]))
         return t
 
     if key in _memory:

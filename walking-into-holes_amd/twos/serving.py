@@ -3,11 +3,15 @@
 Resident geometries: the unit ball, the cooling block and the SH icosphere.
 Requests arrive by a Poisson process; each has a log-uniform number of sites
 in [1, max_sites], a CI target U[ci_lo, ci_hi] of |TD| and its own seed.  A
-fraction are fresh-seed insertion validations: a sphere of radius U[0.01,
-0.05] is unioned into the geometry at a previously scored site, and the
-change of the primal state at probe points around it is estimated with
-common streams (the same seed and walk indices on the geometry with and
-without the cavity, so paths coincide until they feel the cavity).
+fraction are fresh-seed insertion validations: a sphere of radius r ~ U[0.01,
+0.05] is cut out of the geometry at a previously scored site z, and the
+change of the objective J = integral q u, q the unit-mass sensor bump on the
+ball B(z, 3 r), is re-estimated by the coupled finite-cavity difference
+(COUPLED: each walk starts at a q-distributed point and runs, on one stream,
+once in the geometry and once in the geometry minus the sphere), as
+J(domain) - J(domain minus the sphere) over the sphere's capacity 4 pi r
+(benchmarks/submission_study/study.cpp:317-332; capacity,
+benchmarks/submission_study/README.md:27-34).
 
 Requests are admitted between rounds whenever the server has free slots, and
 every admitted site of every request shares each persistent launch.  Latency
@@ -25,7 +29,8 @@ import numpy as np
 
 from . import workloads as W
 
-VALIDATION_WALKS = 1024
+VALIDATION_WALKS = 4096
+SENSOR_RADII = 3.0  # the validation's sensor ball, in cavity radii
 
 
 @dataclass
@@ -82,14 +87,10 @@ class Stream:
             if is_val:
                 z = scored[name][int(rng.integers(len(scored[name])))]
                 r = float(rng.uniform(0.01, 0.05))
-                # Probes on a shell around the inserted sphere.
-                k = max(1, min(count, 256))
-                d = rng.normal(size=(k, 3))
-                d /= np.linalg.norm(d, axis=1)[:, None]
-                probes = z + d * (r + rng.uniform(0.5, 2.0, (k, 1)) * r)
-                probes = probes[self._interior(name, probes)]
-                if len(probes):
-                    self.requests.append(Request(i, t, "validation", name, probes, int(rng.integers(1 << 62)),
+                # Every start point of the coupled walks lies in the sensor
+                # ball, so it must sit inside the domain.
+                if self._clearance(name, z) > SENSOR_RADII * r + 1e-3:
+                    self.requests.append(Request(i, t, "validation", name, z[None, :], int(rng.integers(1 << 62)),
                                                  0.0, extra={"center": z, "radius": r}))
                     continue
             # A TD query (also the fallback when no probe lies inside).
@@ -99,31 +100,48 @@ class Stream:
                                          float(rng.uniform(ci_lo, ci_hi))))
             scored[name].append(sites[0])
 
-    def _interior(self, name: str, pts: np.ndarray) -> np.ndarray:
-        """Probes strictly inside the base domain (a walk must start inside)."""
+    def _clearance(self, name: str, z: np.ndarray) -> float:
+        """Distance from z to the base domain's boundary (0 outside a mesh)."""
         w, g = self.geoms[name]
-        d, _, _ = self.ctx.closest(g, pts, precision=64)
-        keep = d > 1e-3
-        if w.container == self.twos.MESH:
-            keep &= self.ctx.inside(self.mesh, pts)
-        return keep
+        d, _, _ = self.ctx.closest(g, z[None, :], precision=64)
+        if w.container == self.twos.MESH and not self.ctx.inside(self.mesh, z[None, :])[0]:
+            return 0.0
+        return float(d[0])
+
+    def derived_geometry(self, req: Request) -> int:
+        """The validation's geometry: the base domain with the candidate
+        sphere's radius as its cavity radius (COUPLED cuts the sphere at the
+        request's site for the second path of each walk)."""
+        w, _ = self.geoms[req.geom_name]
+        t, r = self.twos, req.extra["radius"]
+        if w.container == t.MESH:
+            return self.ctx.geometry(t.MESH, mesh=self.mesh, cavity_radius=r)
+        if w.container == t.BOX:
+            return self.ctx.geometry(t.BOX, box_lo=w.box_lo, box_hi=w.box_hi, spheres=w.spheres,
+                                     cylinders=w.cylinders, cavity_radius=r)
+        return self.ctx.geometry(t.BALL, ball=w.ball, cavity_radius=r)
+
+    def request_args(self, req: Request, geom: int, distance_grid: bool = True):
+        """(geometry, sites, sides, keyword arguments) of the request's
+        estimate; the server and a standalone twos.Context.estimate take the
+        same ones."""
+        w, _ = self.geoms[req.geom_name]
+        sides = W.sides(w, self.twos.side)
+        common = dict(precision=32, seed=req.seed, eps=w.eps, max_steps=w.max_steps, distance_grid=distance_grid,
+                      **self.shapes.get(req.geom_name, {}))
+        if req.kind == "td":
+            return geom, req.sites, sides, dict(combine=self.twos.CROSS, walks=self.walks, walks_max=self.walks_max,
+                                                ci_rel=req.ci_rel, **common)
+        probe = self.twos.probe_side(req.extra["center"], SENSOR_RADII * req.extra["radius"])
+        return geom, req.sites, [sides[0], probe], dict(combine=self.twos.COUPLED, walks=VALIDATION_WALKS, **common)
 
     def submit(self, srv, req: Request, distance_grid: bool = True) -> None:
-        w, g = self.geoms[req.geom_name]
-        sides = W.sides(w, self.twos.side)
-        common = dict(combine=self.twos.CROSS, precision=32, seed=req.seed, eps=w.eps, max_steps=w.max_steps,
-                      distance_grid=distance_grid, **self.shapes.get(req.geom_name, {}))
-        if req.kind == "td":
-            req.ids = [srv.submit(g, req.sites, sides, walks=self.walks, walks_max=self.walks_max,
-                                  ci_rel=req.ci_rel, **common)]
-            return
-        # Validation: the primal state at the same probes with the same
-        # streams, with and without the cavity (SQUARE keeps sum a^2 for the SE).
-        g2 = self.ctx.union_sphere(g, req.extra["center"], req.extra["radius"])
-        req.extra["derived"] = g2
-        common["combine"] = self.twos.SQUARE
-        req.ids = [srv.submit(g, req.sites, sides[:1], walks=VALIDATION_WALKS, **common),
-                   srv.submit(g2, req.sites, sides[:1], walks=VALIDATION_WALKS, **common)]
+        _, g = self.geoms[req.geom_name]
+        if req.kind == "validation":
+            g = self.derived_geometry(req)
+            req.extra["derived"] = g
+        geom, sites, sides, kw = self.request_args(req, g, distance_grid)
+        req.ids = [srv.submit(geom, sites, sides, **kw)]
 
 
 def run(stream: Stream, srv, time_scale: float = 1.0, distance_grid: bool = True,
@@ -138,6 +156,7 @@ def run(stream: Stream, srv, time_scale: float = 1.0, distance_grid: bool = True
     sites_at_target = 0
     sites_total = 0
     rounds = 0
+    failed = 0
     while pending or inflight:
         now = time.perf_counter() - t0
         while pending and pending[0].arrival * time_scale <= now:
@@ -157,6 +176,7 @@ def run(stream: Stream, srv, time_scale: float = 1.0, distance_grid: bool = True
             req.results.append((sid, res))
             req.done += 1
             steps_total += int(res.steps.sum())
+            failed += res.stats.get("status", 0) != 0
             if req.done == len(req.ids):
                 req.finish = t_end
                 inflight.pop(req.rid)
@@ -174,13 +194,13 @@ def run(stream: Stream, srv, time_scale: float = 1.0, distance_grid: bool = True
             "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
             "mean_ms": float(lat.mean() * 1e3), "td_sites_at_target": sites_at_target,
             "td_sites": sites_total, "td_sites_per_s": sites_at_target / wall, "walk_steps_per_s": steps_total / wall,
+            "walk_steps": steps_total, "failed": failed,
             "validations": sum(r.kind == "validation" for r in stream.requests)}
 
 
-def validation_change(req: Request) -> tuple[np.ndarray, np.ndarray]:
-    """Change of the primal state at the probes (without minus with the
-    cavity) and a conservative standard error (the two estimates share their
-    streams, which only lowers the true error)."""
-    (_, base), (_, cav) = sorted(req.results, key=lambda x: x[0])
-    se = lambda r: np.sqrt(np.maximum(r.same - r.u * r.u, 0.0) / r.walks)
-    return base.u - cav.u, np.hypot(se(base), se(cav))
+def validation_change(req: Request) -> tuple[float, float]:
+    """A validation's objective change per capacity,
+    (J(domain) - J(domain minus the sphere)) / (4 pi r), and its standard
+    error."""
+    (_, res), = req.results
+    return float(res.score[0]), float(res.score_se[0])
