@@ -69,6 +69,7 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--block", type=int, default=0, help="fixed launch shape (skips the autotuner)")
     ap.add_argument("--lds-stack", type=int, default=0, help="fixed launch shape (skips the autotuner)")
     ap.add_argument("--step-batch", type=int, default=-1, help="fixed launch shape (skips the autotuner)")
+    ap.add_argument("--refill-batch", type=int, default=0, help="fixed launch shape (skips the autotuner)")
     ap.add_argument("--grid-kappa", type=float, default=0.0, help="fixed launch shape (skips the autotuner)")
     ap.add_argument("--grid-cells", type=int, default=0, help="fixed distance-grid resolution (skips the autotuner)")
     ap.add_argument("--grid-corners", type=int, default=-1,
@@ -143,7 +144,7 @@ def main() -> int:
     grid_build_ms = 0.0
     fixed = {k: v for k, v in (("waves_per_simd", args.waves_per_simd), ("block", args.block),
                                ("lds_stack", args.lds_stack), ("grid_kappa", args.grid_kappa),
-                               ("step_batch", args.step_batch))
+                               ("step_batch", args.step_batch), ("refill_batch", args.refill_batch))
              if (v >= 0 if k in ("waves_per_simd", "step_batch") else v > 0)}
     if fixed or args.grid_cells > 0 or args.grid_corners >= 0:
         shape = fixed
@@ -190,7 +191,7 @@ def main() -> int:
     steps_per_s = float(np.sum(steps)) / total_t
     capped = int(np.sum(res.status == 1))
 
-    out_shape = {"block": 256, "waves_per_simd": 0, "lds_stack": 24, "step_batch": 0,
+    out_shape = {"block": 256, "waves_per_simd": 0, "lds_stack": 24, "step_batch": 0, "refill_batch": 8,
                          **({"grid_kappa": 4.0, "grid_cells": 128, "grid_corners": 0} if grid_on else {}), **shape,
                          **({k: tuning.choice[k] for k in ("grid_cells", "grid_corners")}
                             if tuning is not None and "grid_cells" in tuning.choice else {}),
@@ -199,6 +200,9 @@ def main() -> int:
     # Walk kernel alone (eager launches, HIP events on the library's stream).
     eager = run(twos.PERSISTENT, sites, False)
     eager_ms = eager.stats["walk_kernel_ms"]
+    import hashlib
+    checksum = {"steps": int(np.sum(eager.steps)), "walks": int(np.sum(eager.walks)),
+                "score_sha16": hashlib.sha256(np.ascontiguousarray(eager.score).tobytes()).hexdigest()[:16]}
     launches = eager.stats["walk_launches"]
     queries = eager.stats["steps"] + eager.stats["paths"]
     roofline = None
@@ -276,6 +280,7 @@ def main() -> int:
         "rounds": res.stats["rounds"],
         "launch_shape": out_shape,
         "roofline": roofline,
+        "results_checksum": checksum,
     }
     if tuning is not None:
         out["autotune"] = {"choice": tuning.choice, "speedup_vs_default_shape": round(tuning.speedup, 4),
@@ -457,7 +462,8 @@ def copy_probe() -> int:
 def shape_args(shape: dict) -> list:
     """bench.py flags that fix a launch shape and distance grid."""
     flags = {"block": "--block", "waves_per_simd": "--waves-per-simd", "lds_stack": "--lds-stack",
-             "step_batch": "--step-batch", "grid_kappa": "--grid-kappa", "grid_cells": "--grid-cells",
+             "step_batch": "--step-batch", "refill_batch": "--refill-batch", "grid_kappa": "--grid-kappa",
+             "grid_cells": "--grid-cells",
              "grid_corners": "--grid-corners"}
     out = []
     for k, f in flags.items():
@@ -480,7 +486,8 @@ def traffic_probe(args) -> int:
     if grid:
         ctx.set_distance_grid(mesh, args.grid_cells, max(args.grid_corners, 0))
     shape = dict(block=args.block, waves_per_simd=max(args.waves_per_simd, 0), lds_stack=args.lds_stack,
-                 step_batch=max(args.step_batch, 0), grid_kappa=args.grid_kappa if grid else 0.0)
+                 step_batch=max(args.step_batch, 0), refill_batch=args.refill_batch,
+                 grid_kappa=args.grid_kappa if grid else 0.0)
     res = ctx.estimate(geom, sites, W.sides(w, twos.side), combine=twos.CROSS, precision=w.precision, seed=2026,
                        walks=w.walks, walks_max=w.walks_max, ci_rel=w.ci_rel, eps=w.eps, max_steps=w.max_steps,
                        distance_grid=grid, **shape)

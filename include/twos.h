@@ -178,6 +178,9 @@ typedef struct {
                            as are still traversing), or none traverses;
                            lds_stack 24 and waves_per_simd 0 or 5 only.  No
                            result depends on it. */
+  int32_t refill_batch; /* persistent kernels: the idle lanes of a wave take new
+                           paths once at least this many are idle (or none is
+                           active), 1..64; 0 = 8.  No result depends on it. */
 } twos_params;
 
 /* Count BVH nodes fetched and triangles tested (slower; for roofline figures). */

@@ -18,6 +18,8 @@ class FakeContext:
         self.cell_history = []
         self.batches = []
         self.batch_factor = {}
+        self.refills = []
+        self.refill_factor = {}
 
     def set_distance_grid(self, mesh, cells, corners=False):
         self.cells = cells
@@ -25,11 +27,13 @@ class FakeContext:
         self.cell_history.append(cells)
         return 0.0
 
-    def estimate(self, geom, pts, sides, *, block, waves_per_simd, lds_stack, grid_kappa, step_batch=0, **kw):
+    def estimate(self, geom, pts, sides, *, block, waves_per_simd, lds_stack, grid_kappa, step_batch=0,
+                 refill_batch=0, **kw):
         self.calls += 1
         self.batches.append((step_batch, waves_per_simd, lds_stack))
+        self.refills.append(refill_batch)
         ms = self.cost(block, waves_per_simd, lds_stack, grid_kappa) * (1.0 + 0.1 * abs(self.cells - 192) / 64)
-        ms *= self.batch_factor.get(step_batch, 1.0)
+        ms *= self.batch_factor.get(step_batch, 1.0) * self.refill_factor.get(refill_batch, 1.0)
 
         class R:
             stats = {"walk_kernel_ms": ms * 1000.0, "paths": 1000}
@@ -47,7 +51,8 @@ def test_descent_finds_the_minimum():
     ctx = FakeContext(_surface)
     t = autotune.autotune(ctx, 0, np.zeros((10, 3)), None, geom_sig="fake", combine=2, distance_grid=True,
                           repeats=1)
-    assert t.choice == {"block": 128, "waves_per_simd": 5, "lds_stack": 16, "step_batch": 0, "grid_kappa": 3.0}
+    assert t.choice == {"block": 128, "waves_per_simd": 5, "lds_stack": 16, "step_batch": 0, "refill_batch": 8,
+                        "grid_kappa": 3.0}
     assert t.speedup > 1.0
     # Step batching exists only for the 24-entry stack with 0 or 5 waves.
     assert all(b == 0 or (lds == 24 and w in (0, 5)) for b, w, lds in ctx.batches)
@@ -66,7 +71,7 @@ def test_gains_below_one_percent_keep_the_default():
     t = autotune.autotune(FakeContext(lambda b, w, l, k: 1.0 - 0.005 * (b == 64)), 0, np.zeros((4, 3)), None,
                           geom_sig="fake5", combine=2, repeats=1)
     assert t.choice == {k: autotune.DEFAULTS[k] for k in ("block", "waves_per_simd", "lds_stack", "step_batch",
-                                                          "grid_kappa")}
+                                                          "refill_batch", "grid_kappa")}
 
 
 def test_invalid_variants_are_never_run():
@@ -80,6 +85,15 @@ def test_invalid_variants_are_never_run():
     autotune.autotune(FakeContext(cost), 0, np.zeros((4, 3)), None, geom_sig="fake2", combine=2, repeats=1)
     # lds 16/32 exist only for waves 0 and 5.
     assert all(lds == 24 or waves in (0, 5) for waves, lds in seen)
+
+
+def test_refill_batch_is_chosen_when_faster():
+    autotune.clear_cache()
+    ctx = FakeContext(lambda b, w, l, k: 1.0)
+    ctx.refill_factor = {16: 0.9, 32: 0.95}
+    t = autotune.autotune(ctx, 0, np.zeros((10, 3)), None, geom_sig="fake7", combine=2, repeats=1)
+    assert t.choice["refill_batch"] == 16
+    assert set(ctx.refills) == set(autotune.AXES["refill_batch"])
 
 
 def test_binary64_tunes_block_only():

@@ -38,7 +38,7 @@ def env():
     geom = ctx.geometry(twos.MESH, mesh=mesh)
     print("tuned choice", choice, flush=True)
     ctx.set_distance_grid(mesh, choice["grid_cells"], choice["grid_corners"])
-    shape = {k: choice[k] for k in SHAPE_KEYS}
+    shape = {k: choice[k] for k in SHAPE_KEYS if k in choice}
     yield ctx, w, mesh, geom, choice, shape
     ctx.close()
 
@@ -60,7 +60,9 @@ def _kw(w, shape, **over):
 @pytest.mark.parametrize("variant", [dict(step_batch=1), dict(step_batch=24), dict(step_batch=32),
                                      dict(step_batch=40), dict(step_batch=1, waves_per_simd=0),
                                      dict(step_batch=1, block=64), dict(step_batch=1, block=256),
-                                     dict(step_batch=32, block=256, waves_per_simd=0)])
+                                     dict(step_batch=32, block=256, waves_per_simd=0), dict(refill_batch=1),
+                                     dict(refill_batch=4), dict(refill_batch=16), dict(refill_batch=64),
+                                     dict(step_batch=1, refill_batch=32, block=64)])
 def test_step_batching_never_changes_results(env, variant):
     ctx, w, _, geom, _, shape = env
     sites = w.sites[::64]  # 256 sites spread over the workload

@@ -7,7 +7,7 @@ from pathlib import Path
 
 CACHE = Path(__file__).resolve().parents[1] / "walking-into-holes_amd" / "twos" / "tuning_cache.json"
 KEY = "dev0|sh_icosphere_81920|p32|c2|e0.0001|g1|m"
-SHAPE_KEYS = ("block", "waves_per_simd", "lds_stack", "step_batch", "grid_kappa")
+SHAPE_KEYS = ("block", "waves_per_simd", "lds_stack", "step_batch", "refill_batch", "grid_kappa")
 
 
 def tuned_choice() -> dict:
@@ -16,4 +16,4 @@ def tuned_choice() -> dict:
 
 def tuned_shape() -> dict:
     c = tuned_choice()
-    return {k: c[k] for k in SHAPE_KEYS}
+    return {k: c[k] for k in SHAPE_KEYS if k in c}

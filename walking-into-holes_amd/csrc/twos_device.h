@@ -538,7 +538,8 @@ __device__ __forceinline__ R container_analytic(const DevGeom &g, V3<R> p, V3<R>
 // Holes (kKind != kGeomMesh) and cavities after the container: each wins
 // only when strictly closer than the best so far.
 template <typename R, int kKind>
-__device__ __forceinline__ R holes_and_cavities(const DevGeom &g, V3<R> p, R best, V3<R> &point, int &cls) {
+__device__ __forceinline__ R holes_and_cavities(const DevGeom &g, V3<R> p, R best, V3<R> &point, int &cls,
+                                                int32_t n_cavities) {
   if (kKind != kGeomMesh) {
     for (int i = 0; i < g.n_spheres; ++i) {
       const Vec4<R> s = load4<R>(g, 0, i);
@@ -569,7 +570,7 @@ __device__ __forceinline__ R holes_and_cavities(const DevGeom &g, V3<R> p, R bes
       }
     }
   }
-  for (int i = 0; i < g.n_cavities; ++i) {
+  for (int i = 0; i < n_cavities; ++i) {
     const Vec4<R> s = load4<R>(g, 2, i);
     const V3<R> c{s.x, s.y, s.z};
     const V3<R> d = p - c;
@@ -708,7 +709,7 @@ __device__ __forceinline__ R closest_boundary(const DevGeom &g, V3<R> p, V3<R> &
   } else {
     best = container_analytic<R>(g, p, point, cls);
   }
-  return holes_and_cavities<R, kKind>(g, p, best, point, cls);
+  return holes_and_cavities<R, kKind>(g, p, best, point, cls, g.n_cavities);
 }
 
 // ---------------------------------------------------------------------------
@@ -887,12 +888,18 @@ __device__ __forceinline__ bool mq_visit(const MeshView &m, V3<R> p, MeshQuery<R
 // Boundary data g(y, class) and volume source f(x) of one side.
 template <typename R>
 __device__ __forceinline__ R side_payoff(const DevSide &s, V3<R> y, int cls) {
-  R value = R(s.cls_const[cls & 7]);
+  const R *c;
+  R value;
+  if constexpr (sizeof(R) == 4) {
+    c = s.poly_f;
+    value = s.cls_const_f[cls & 7];
+  } else {
+    c = s.poly;
+    value = s.cls_const[cls & 7];
+  }
   if ((s.cls_poly_mask >> (cls & 7)) & 1U) {
-    const double *c = s.poly;
-    value += R(c[0]) + R(c[1]) * y.x + R(c[2]) * y.y + R(c[3]) * y.z + R(c[4]) * y.x * y.x +
-             R(c[5]) * y.y * y.y + R(c[6]) * y.z * y.z + R(c[7]) * y.x * y.y + R(c[8]) * y.x * y.z +
-             R(c[9]) * y.y * y.z;
+    value += c[0] + c[1] * y.x + c[2] * y.y + c[3] * y.z + c[4] * y.x * y.x + c[5] * y.y * y.y +
+             c[6] * y.z * y.z + c[7] * y.x * y.y + c[8] * y.x * y.z + c[9] * y.y * y.z;
   }
   return value;
 }
@@ -902,8 +909,9 @@ __device__ __forceinline__ R side_payoff(const DevSide &s, V3<R> y, int cls) {
 // the same binary64 value in the contraction-free binary64 build.
 template <typename R>
 __device__ __forceinline__ R side_source(const DevSide &s, V3<R> x) {
-  const double *c = s.src;
-  return ((R(c[0]) + R(c[1]) * (x.x + R(c[4]))) + R(c[2]) * (x.y + R(c[5]))) + R(c[3]) * (x.z + R(c[6]));
+  const R *c;
+  if constexpr (sizeof(R) == 4) c = s.src_f; else c = s.src;
+  return ((c[0] + c[1] * (x.x + c[4])) + c[2] * (x.y + c[5])) + c[3] * (x.z + c[6]);
 }
 
 // Sampled source of source_kind 2: f(a) = s0 + sum of the side's term rows in

@@ -191,7 +191,7 @@ struct Engine {
   // hipGraph of a single-class persistent round.
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
-  int64_t graph_key = -1;
+  uint64_t graph_key = ~0ull;  // ~0: nothing captured (no launch shape has this key)
 
   twos_stats stats{};
 
@@ -229,7 +229,7 @@ struct Engine {
     if (graph) (void)hipGraphDestroy(graph);
     graph_exec = nullptr;
     graph = nullptr;
-    graph_key = -1;
+    graph_key = ~0ull;
     dfree(s.pos); dfree(s.geom); dfree(s.problem); dfree(s.combine); dfree(s.keys); dfree(s.site); dfree(s.seed);
     dfree(s.repeat); dfree(s.walks_max); dfree(s.max_steps); dfree(s.eps); dfree(s.ci_rel); dfree(s.ci_z);
     dfree(s.acc); dfree(s.acc_walks); dfree(s.acc_steps); dfree(s.acc_caps); dfree(s.result); dfree(s.hint0); dfree(s.grid_kappa);
@@ -487,6 +487,7 @@ struct Engine {
     wp.steps = d_steps;
     wp.grad = grad_of(r);
     wp.counters = d_counters + kMaxClasses;
+    wp.refill = p.refill_batch > 0 ? p.refill_batch : kRefillDefault;
     hipStream_t st = ctx->stream;
     const bool count = (p.flags & TWOS_FLAG_COUNT_TRAVERSAL) != 0;
     hipError_t e;
@@ -656,10 +657,10 @@ struct Engine {
       // Everything the captured launches depend on: precision, combine,
       // geometry kind, launch shape and the channel flags.
       const twos_params &q = r0->prm;
-      const int64_t key = int64_t(q.precision) | int64_t(q.combine) << 8 | int64_t(kind0) << 12 |
-                          int64_t(q.waves_per_simd) << 16 | int64_t(q.block) << 24 | int64_t(q.lds_stack) << 36 |
-                          int64_t(q.flags & 7) << 44 | int64_t(q.step_batch) << 48 |
-                          int64_t(class_bare(by_class[0])) << 56;
+      const uint64_t key = uint64_t(q.precision) | uint64_t(q.combine) << 8 | uint64_t(kind0) << 12 |
+                           uint64_t(q.waves_per_simd) << 16 | uint64_t(q.block) << 24 | uint64_t(q.lds_stack) << 36 |
+                           uint64_t(q.flags & 7) << 44 | uint64_t(q.step_batch) << 48 |
+                           uint64_t(class_bare(by_class[0])) << 56 | uint64_t(q.refill_batch) << 57;
       if (graph_key != key) {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -836,6 +837,8 @@ twos_status check_params(const twos_params *p, int64_t n) {
     return fail(TWOS_ERR_INVALID_ARGUMENT, "waves_per_simd must be in [0, 16]");
   if (p->step_batch != 0 && p->step_batch != 1 && p->step_batch != 24 && p->step_batch != 32 && p->step_batch != 40)
     return fail(TWOS_ERR_INVALID_ARGUMENT, "step_batch must be 0, 1, 24, 32 or 40");
+  if (p->refill_batch < 0 || p->refill_batch > 64)
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "refill_batch must be in [0, 64]");
   if (p->step_batch != 0 && ((p->lds_stack != 0 && p->lds_stack != 24) || (p->waves_per_simd != 0 &&
                                                                             p->waves_per_simd != 5)))
     return fail(TWOS_ERR_INVALID_ARGUMENT, "step_batch is built for lds_stack 24 with waves_per_simd 0 or 5");
@@ -932,7 +935,7 @@ int64_t ctx_device_bytes(const twos_ctx *ctx) { return ctx_geometry_bytes(ctx) +
 // Captured rounds hold the geometry table's address and the grids it points
 // to: every engine on the context must capture again.
 void invalidate_graphs(twos_ctx *ctx) {
-  for (Engine *e : ctx->engines) e->graph_key = -1;
+  for (Engine *e : ctx->engines) e->graph_key = ~0ull;
 }
 
 twos_status upload_geom_table(twos_ctx *ctx) {

@@ -20,7 +20,24 @@ enum { kChanNone = 0, kChanGrad = 1, kChanCoupled = 2, kChanBare = 3 };
 // container only (no analytic holes), analytic container only.
 enum { kGeomGeneric = 0, kGeomMesh = 1, kGeomAnalytic = 2 };
 
-using DevSide = twos_side;
+// One side's data as the kernels read it: the C-ABI struct plus binary32
+// copies of the payoff and affine-source coefficients (each rounded to
+// nearest once on the host, which is the conversion a binary32 walk applied
+// to the binary64 value at every use), so a binary32 step loads floats and
+// converts nothing.
+struct DevSide : twos_side {
+  float poly_f[10];
+  float cls_const_f[8];
+  float src_f[7];
+  float pad_f;
+  DevSide() = default;
+  DevSide(const twos_side &s) : twos_side(s) {
+    for (int k = 0; k < 10; ++k) poly_f[k] = float(s.poly[k]);
+    for (int k = 0; k < 8; ++k) cls_const_f[k] = float(s.cls_const[k]);
+    for (int k = 0; k < 7; ++k) src_f[k] = float(s.src[k]);
+    pad_f = 0.f;
+  }
+};
 
 // One resident geometry as the kernels see it.  Analytic rows are kept in
 // binary32 (float4) and binary64 (4 doubles, cavities padded) copies.
@@ -105,6 +122,8 @@ struct WalkParams {
   int32_t pad;
   unsigned long long *counters;  // nodes, tris, grid lookups (when counting)
   double *grad;                  // [total walks][2][3] gradient samples, or null
+  int32_t refill;                // persistent kernels: idle lanes refill once this many are idle
+  int32_t pad2;
 };
 
 // One launch of the differential kernels: jobs = sites of the chunk x walks.

@@ -44,6 +44,7 @@ struct Job {
   int32_t walk;
   int32_t slot;
   int32_t max_steps;
+  int32_t n_cav;  // the geometry's cavity count (read once per path, not per step)
   R tau;         // grid-radius threshold (infinite when the grid is off)
   int32_t hint;  // sorted index of the previous closest triangle, -1 before the first query
   R eps;
@@ -91,6 +92,7 @@ __device__ __forceinline__ void job_init(Job<R> &j, const Tables &tb, const Walk
   j.eps = R(tb.slots.eps[slot]);
   j.key = tb.slots.keys[2 * size_t(slot) + (wp.combine == kCross ? j.side : 0)];
   j.g = tb.geoms + tb.slots.geom[slot];
+  j.n_cav = j.g->n_cavities;
   j.mv = MeshView{j.g->nodes, j.g->nodes4, j.g->tris_f, j.g->tris_d, j.g->tri_ids, j.g->n_tris};
   // A grid step (radius = grid lower bound >= tau) must never end the path,
   // so tau stays above 2 eps whatever the threshold.
@@ -398,20 +400,34 @@ __global__ __launch_bounds__(256) void walk_simple(Tables tb, WalkParams wp) {
 // there because every path ends within max_steps steps.
 constexpr int kChunk = 64;
 constexpr int kWalkBlock = 256;  // largest block of the persistent kernel
+// Idle lanes of a wave take new paths once at least wp.refill of them are idle
+// (or none is active): the ticket and path set-up code then runs for several
+// lanes at once instead of in nearly every iteration for one or two.  Paths
+// are keyed by job, so which lane runs which path changes nothing.
+constexpr int kRefillDefault = 8;
 constexpr int kLdsDefault = 24;  // default LDS stack entries per lane
 
 // Dynamic LDS bytes of a persistent launch: the lanes' traversal stacks.
 constexpr size_t persistent_lds_bytes(int lds, int block) { return size_t(lds) * size_t(block) * 4u; }
 
 // Applies a finished boundary query (distance r, closest point bp of class
-// cls) to the path: 0 moved on, 1 ended in the eps-shell, 2 capped.
-template <typename R, int kChan = kChanNone>
+// cls) to the path: 0 moved on, 1 ended in the eps-shell, 2 capped.  With
+// kDefer a path that ends in the shell keeps its closest point in j.x and the
+// class in j.hint (neither is needed any more) and job_retire adds the payoff
+// later, so the payoff code runs in the wave's batched refill instead of in
+// the walk step.
+template <typename R, int kChan = kChanNone, bool kDefer = false>
 __device__ __forceinline__ int job_advance(Job<R> &j, const WalkParams &wp, R r, V3<R> bp, int cls) {
   if (!isfinite(r)) return 2;  // as in job_step: a failed query ends the path capped
   const DevSide &s0 = *j.s0;
   if (!(r > j.eps)) {
-    j.acc0 += side_payoff<R>(s0, bp, cls);
-    if (wp.combine == kPaired) j.acc1 += side_payoff<R>(*j.s1, bp, cls);
+    if constexpr (kDefer) {
+      j.x = bp;
+      j.hint = cls;
+    } else {
+      j.acc0 += side_payoff<R>(s0, bp, cls);
+      if (wp.combine == kPaired) j.acc1 += side_payoff<R>(*j.s1, bp, cls);
+    }
     return 1;
   }
   if constexpr (kChan != kChanBare)
@@ -421,6 +437,17 @@ __device__ __forceinline__ int job_advance(Job<R> &j, const WalkParams &wp, R r,
   j.x = j.x + d * r;
   ++j.steps;
   return j.steps >= j.max_steps ? 2 : 0;
+}
+
+// A path ended with job_advance<..., kDefer = true> state `end` (1 in the
+// eps-shell, 2 capped): its payoff, then its samples and step word.
+template <typename R>
+__device__ __forceinline__ void job_retire(Job<R> &j, const WalkParams &wp, int end) {
+  if (end == 1) {
+    j.acc0 += side_payoff<R>(*j.s0, j.x, j.hint);
+    if (wp.combine == kPaired) j.acc1 += side_payoff<R>(*j.s1, j.x, j.hint);
+  }
+  job_store(j, wp, end == 2);
 }
 
 template <typename R, int GK>
@@ -479,12 +506,21 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWav
   // fresh: a new path whose first query has not started; it starts with the
   // next walk steps, so the query-start code runs in one place.
   bool active = false, in_query = false, fresh = false;
+  // A path that ended waits, idle, for the wave's next refill to add its
+  // payoff and store its samples (coupled pairs store at once).
+  constexpr bool kDefer = kChan != kChanCoupled;
+  int end = 0;  // 1 or 2: an ended path's samples are not stored yet
   int64_t chunk_cur = 0, chunk_end = 0;  // wave-uniform
   int32_t chunk_site = 0;                // wave-uniform: the site of the chunk's first ticket
   bool drained = false;                  // wave-uniform
   TraversalCount cnt{0, 0, 0, 0};
   while (true) {
     uint64_t need = __ballot(!active);
+    if (__popcll(need) < wp.refill && need != ~0ull) need = 0;  // wave-uniform
+    if (kDefer && need != 0 && end != 0) {
+      job_retire(j, wp, end);
+      end = 0;
+    }
     while (need != 0 && !drained) {
       if (chunk_cur >= chunk_end) {
         unsigned long long base = 0;
@@ -541,9 +577,9 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWav
           } else {
             r = container_analytic<R>(*j.g, j.x, bp, cls);
           }
-          r = holes_and_cavities<R, GK>(*j.g, j.x, r, bp, cls);
+          r = holes_and_cavities<R, GK>(*j.g, j.x, r, bp, cls, j.n_cav);
           if constexpr (kChan == kChanCoupled) r = coupled_cavity(j, r, bp, cls);
-          const int st = job_advance<R, kChan>(j, wp, r, bp, cls);
+          const int st = job_advance<R, kChan, kDefer>(j, wp, r, bp, cls);
           if (st == 0) {
             start = true;
           } else if (kChan == kChanCoupled && j.phase == 0) {
@@ -551,7 +587,7 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWav
             start = true;
           } else {
             if constexpr (kChan == kChanCoupled) coupled_store(j, wp, st == 2);
-            else job_store(j, wp, st == 2);
+            else end = st;
             active = false;
           }
         }

@@ -264,12 +264,13 @@ def _stats(st: _capi.Stats) -> dict:
 def _params(combine=SQUARE, rng=RNG_PHILOX, precision=32, schedule=PERSISTENT, seed=0, repeat=0,
             max_steps=512, eps=5e-4, walks=16, walks_max=0, ci_rel=0.02, ci_z=1.96, use_graph=False,
             block=0, waves_per_simd=0, site_offset=0, flags=0, grid_kappa=0.0, lds_stack=0,
-            step_batch=0) -> _capi.Params:
+            step_batch=0, refill_batch=0) -> _capi.Params:
     return _capi.Params(combine=combine, rng=rng, precision=precision, schedule=schedule, seed=seed,
                         repeat=repeat, max_steps=max_steps, eps=eps, walks=walks, walks_max=walks_max,
                         ci_rel=ci_rel, ci_z=ci_z, use_graph=int(use_graph), block=block,
                         waves_per_simd=waves_per_simd, site_offset=site_offset, flags=flags,
-                        grid_kappa=grid_kappa, lds_stack=lds_stack, step_batch=step_batch)
+                        grid_kappa=grid_kappa, lds_stack=lds_stack, step_batch=step_batch,
+                        refill_batch=refill_batch)
 
 
 class Context:
@@ -475,7 +476,7 @@ class Context:
                  schedule=PERSISTENT, seed=0, repeat=0, walks=16, walks_max=0, ci_rel=0.02, ci_z=1.96,
                  eps=5e-4, max_steps=512, use_graph=False, block=0, waves_per_simd=0, site_offset=0,
                  count_traversal=False, distance_grid=False, grid_kappa=0.0, lds_stack=0, gradient=False,
-                 step_batch=0, dump: bool = False, raise_on_cap: bool = True) -> Result:
+                 step_batch=0, refill_batch=0, dump: bool = False, raise_on_cap: bool = True) -> Result:
         """Per-site TWoS estimate.  walks_max > walks doubles each site's walks
         until ci_z * score_se <= ci_rel * |score| or walks_max is reached.
         distance_grid: mesh steps far from the boundary take a conservative
@@ -492,7 +493,8 @@ class Context:
                       waves_per_simd=waves_per_simd, site_offset=site_offset,
                       flags=(FLAG_COUNT_TRAVERSAL if count_traversal else 0)
                       | (FLAG_DISTANCE_GRID if distance_grid else 0) | (FLAG_GRADIENT if gradient else 0),
-                      grid_kappa=grid_kappa, lds_stack=lds_stack, step_batch=step_batch)
+                      grid_kappa=grid_kappa, lds_stack=lds_stack, step_batch=step_batch,
+                      refill_batch=refill_batch)
         out = (_capi.SiteResult * max(n, 1))()
         st = _capi.Stats()
         if dump:
@@ -544,7 +546,7 @@ class Server:
     def submit(self, geom: int, sites, sides, *, combine=CROSS, precision=32, seed=0, walks=256,
                walks_max=0, ci_rel=0.02, ci_z=1.96, eps=1e-4, max_steps=4096, use_graph=False,
                waves_per_simd=0, site_offset=0, distance_grid=False, grid_kappa=0.0, block=0,
-               lds_stack=0, step_batch=0) -> int:
+               lds_stack=0, step_batch=0, refill_batch=0) -> int:
         p = _f64(sites).reshape(-1, 3)
         n = len(p)
         sd = (Side * 2)(sides[0], sides[1] if len(sides) > 1 else sides[0])
@@ -552,7 +554,7 @@ class Server:
                       walks_max=walks_max, ci_rel=ci_rel, ci_z=ci_z, eps=eps, max_steps=max_steps,
                       use_graph=use_graph, waves_per_simd=waves_per_simd, site_offset=site_offset,
                       flags=FLAG_DISTANCE_GRID if distance_grid else 0, grid_kappa=grid_kappa, block=block,
-                      lds_stack=lds_stack, step_batch=step_batch)
+                      lds_stack=lds_stack, step_batch=step_batch, refill_batch=refill_batch)
         out = (_capi.SiteResult * n)()
         rid = C.c_int64()
         _check(_lib().twos_server_submit(self._h, geom, sd, C.byref(prm), _ptr(p, _capi._dp), n, out,

@@ -68,7 +68,7 @@ class Params(C.Structure):
                 ("walks_max", C.c_int32), ("ci_rel", C.c_double), ("ci_z", C.c_double),
                 ("use_graph", C.c_int32), ("block", C.c_int32), ("waves_per_simd", C.c_int32),
                 ("site_offset", C.c_int32), ("flags", C.c_int32), ("grid_kappa", C.c_float),
-                ("lds_stack", C.c_int32), ("step_batch", C.c_int32)]
+                ("lds_stack", C.c_int32), ("step_batch", C.c_int32), ("refill_batch", C.c_int32)]
 
 
 class SiteResult(C.Structure):

@@ -1,6 +1,7 @@
 """Autotuner for the persistent walk kernel's launch choices.
 
-Axes (binary32 persistent kernel; binary64 tunes the block only):
+Axes (binary32 persistent kernel; binary64 tunes the block and the refill
+batch only):
   block           64, 128 or 256 lanes per block
   waves_per_simd  0 = the compiler's register allocation (4 waves per SIMD),
                   5, 6 or 8 (registers capped for that occupancy; 6 and 8
@@ -11,6 +12,8 @@ Axes (binary32 persistent kernel; binary64 tunes the block only):
                   1 (once at least as many lanes are ready as traverse) or 32
                   (once 32 lanes are ready); built for lds_stack 24 with
                   waves_per_simd 0 or 5
+  refill_batch    1, 4, 8, 16 or 32: a wave's idle lanes take new paths once
+                  that many are idle (binary32 and binary64)
   grid_kappa      distance-grid threshold in cell sizes, 0.0625 to 6 (mesh
                   geometries with the distance grid on)
   grid_corners    distance-grid samples at cell centres (0), corners (1,
@@ -47,13 +50,14 @@ import numpy as np
 
 from . import _capi
 
-DEFAULTS = {"block": 256, "waves_per_simd": 0, "lds_stack": 24, "step_batch": 0, "grid_corners": 0,
+DEFAULTS = {"block": 256, "waves_per_simd": 0, "lds_stack": 24, "step_batch": 0, "refill_batch": 8, "grid_corners": 0,
             "grid_kappa": 4.0, "grid_cells": 128}
 AXES = {
     "block": (64, 128, 256),
     "waves_per_simd": (0, 5, 6, 8),
     "lds_stack": (16, 24, 32),
     "step_batch": (0, 1, 32),
+    "refill_batch": (1, 4, 8, 16, 32),
     "grid_corners": (0, 1, 2),
     "grid_kappa": (0.0625, 0.125, 0.25, 0.35, 0.5, 0.75, 1.0, 1.5, 2.0, 3.0, 4.0, 6.0),
     "grid_cells": (96, 128, 192, 256, 320, 384, 448, 512, 640, 768, 896, 1024),
@@ -134,7 +138,8 @@ def autotune(ctx, geom: int, sites, sides, *, geom_sig: str, combine: int, preci
         axes.pop("grid_cells")
         axes.pop("grid_corners")
     if precision == 64:
-        axes = {k: v for k, v in axes.items() if k in ("block", "grid_kappa", "grid_cells", "grid_corners")}
+        axes = {k: v for k, v in axes.items()
+                if k in ("block", "refill_batch", "grid_kappa", "grid_cells", "grid_corners")}
     measured: dict[tuple, float] = {}
     grid_now = [None]
 
@@ -150,7 +155,7 @@ def autotune(ctx, geom: int, sites, sides, *, geom_sig: str, combine: int, preci
             r = ctx.estimate(geom, pts, sides, combine=combine, precision=precision, seed=seed, walks=walks,
                              eps=eps, max_steps=max_steps, distance_grid=distance_grid,
                              block=cfg["block"], waves_per_simd=cfg["waves_per_simd"], lds_stack=cfg["lds_stack"],
-                             step_batch=cfg.get("step_batch", 0),
+                             step_batch=cfg.get("step_batch", 0), refill_batch=cfg.get("refill_batch", 0),
                              grid_kappa=cfg["grid_kappa"] if distance_grid else 0.0, raise_on_cap=False)
             best = min(best, r.stats["walk_kernel_ms"] / max(1, r.stats["paths"]))
         measured[k] = best
