@@ -24,7 +24,8 @@ struct MeshBuffers {
   int32_t grid_corners = 0;       // mode: 0 centres, 1 corners, 2 binary16 corner records
   int32_t *grid_start = nullptr;  // mode 2: per cell, the start node of exact queries
   float grid_start_r2 = 0.f;      // squared best distance up to which a start node is valid
-  int64_t grid_bytes = 0;         // device bytes of grid + start nodes
+  int64_t grid_bytes = 0;         // device bytes of grid + start nodes + cell nodes
+  int32_t cell_nodes = 0;         // cell nodes appended to nodes4 after the tree's max(n_tris - 1, 1) nodes
 };
 
 hipError_t lbvh_build(const double *d_vtx, const uint32_t *d_faces, int32_t n, MeshBuffers &mb, int32_t *depth,

@@ -426,10 +426,95 @@ __global__ void pack_corner_records(const float *node, int nx, int ny, int nz, u
 // traversal may start there.  Cells whose record bounds the centre's
 // distance above reach, cells with no such triangle, and a range search that
 // overflows its stack keep the root (0).
+//
+// A cell with one to four such triangles gets instead a node of its own
+// (a "cell node", appended after the tree's nodes): a 4-wide node whose
+// children are exactly those triangles as leaves, so a query starting there
+// tests them in its first visit and ends, the same guarantee making the
+// result exact.  Pass 0 writes the start node, or ~start for a cell that
+// wants a cell node, and counts those; pass 1 gives each such cell a slot
+// below `capacity` (the count of pass 0), writes its node at index base +
+// slot of cell_nodes and points the cell at it (base < 0: no cell nodes).
+
+// Triangles within sqrt(r2) of c: the sorted-leaf range [lo, hi], the count
+// and the first four.  false when the search overflowed its stack.
+__device__ bool cell_candidates(const MeshView &m, V3<float> c, float r2, int32_t &lo, int32_t &hi, int32_t &count,
+                                int32_t cand[4]) {
+  lo = 0x7fffffff;
+  hi = -1;
+  count = 0;
+  int32_t stack[kStackDepth];
+  int sp = 0;
+  int32_t node = 0;
+  int32_t visits = m.n_tris + 8;
+  while (--visits >= 0) {
+    const float4 *q = m.nodes4 + size_t(kNode4Quads) * node;
+    const float4 bx0 = q[0], by0 = q[1], bz0 = q[2], bx1 = q[3], by1 = q[4], bz1 = q[5];
+    const int4 ch = *reinterpret_cast<const int4 *>(q + 6);
+    const float d[4] = {box_d2<float>(c, bx0.x, by0.x, bz0.x, bx1.x, by1.x, bz1.x),
+                        box_d2<float>(c, bx0.y, by0.y, bz0.y, bx1.y, by1.y, bz1.y),
+                        box_d2<float>(c, bx0.z, by0.z, bz0.z, bx1.z, by1.z, bz1.z),
+                        box_d2<float>(c, bx0.w, by0.w, bz0.w, bx1.w, by1.w, bz1.w)};
+    const int32_t cc[4] = {ch.x, ch.y, ch.z, ch.w};
+    for (int s = 0; s < 4; ++s) {
+      if (cc[s] == kEmptyChild || !(d[s] <= r2)) continue;
+      if (cc[s] < 0) {
+        V3<float> a, b, t;
+        load_tri<float>(m, ~cc[s], a, b, t);
+        const V3<float> e = c - closest_point_triangle(c, a, b, t);
+        if (dot(e, e) <= r2) {
+          lo = min(lo, ~cc[s]);
+          hi = max(hi, ~cc[s]);
+          if (count < 4) cand[count] = ~cc[s];
+          ++count;
+        }
+      } else if (sp < kStackDepth) {
+        stack[sp++] = cc[s];
+      } else {
+        return false;
+      }
+    }
+    if (sp == 0) return true;
+    node = stack[--sp];
+  }
+  return false;
+}
+
+// The cell node of candidates cand[0..count): leaf children with the
+// triangles' binary64 bounds rounded outward, empty slots as in collapse4.
+__device__ void write_cell_node(const MeshView &m, const int32_t cand[4], int32_t count, float4 *o) {
+  float lo[3][4], hi[3][4];
+  int32_t ch[4];
+  for (int s = 0; s < 4; ++s) {
+    for (int k = 0; k < 3; ++k) { lo[k][s] = __builtin_huge_valf(); hi[k][s] = -__builtin_huge_valf(); }
+    ch[s] = kEmptyChild;
+    if (s >= count) continue;
+    const double *t = m.td + 9 * size_t(cand[s]);
+    for (int k = 0; k < 3; ++k) {
+      const double a = fmin(fmin(t[k], t[3 + k]), t[6 + k]), b = fmax(fmax(t[k], t[3 + k]), t[6 + k]);
+      lo[k][s] = __double2float_rd(a);
+      hi[k][s] = __double2float_ru(b);
+    }
+    ch[s] = ~cand[s];
+  }
+  o[0] = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+  o[1] = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+  o[2] = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+  o[3] = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+  o[4] = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+  o[5] = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+  o[6] = make_float4(__int_as_float(ch[0]), __int_as_float(ch[1]), __int_as_float(ch[2]), __int_as_float(ch[3]));
+  // Not a tree node: an empty leaf range keeps the start-node descent out.
+  o[7] = make_float4(__int_as_float(-1), __int_as_float(-2), 0.f, 0.f);
+}
+
+template <int kPass>
 __global__ void cell_start_nodes(MeshView m, const uint4 *rec, float lx, float ly, float lz, float h, int nx,
-                                 int ny, int nz, float reach, int32_t *out) {
+                                 int ny, int nz, float reach, int32_t *out, unsigned int *counter, float4 *cell_nodes,
+                                 int32_t base, int32_t capacity) {
   const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (idx >= int64_t(nx) * ny * nz) return;
+  if (kPass == 1 && out[idx] >= 0) return;
   const int l = int(idx % nz), k = int((idx / nz) % ny), i = int(idx / (int64_t(ny) * nz));
   const uint4 r = rec[idx];
   const uint32_t w[4] = {r.x, r.y, r.z, r.w};
@@ -439,45 +524,15 @@ __global__ void cell_start_nodes(MeshView m, const uint4 *rec, float lx, float l
   // Every corner is half a diagonal from the centre, and the corner values
   // are rounded down: the centre is at least dmax - half_diag from the mesh.
   const float half_diag = 0.8660254f * h, wide = reach * 1.001f;
+  bool cell_node = false;
+  int32_t cand[4] = {-1, -1, -1, -1};
+  int32_t count = 0;
   if (m.n_tris > 1 && dmax - half_diag <= wide) {
     const V3<float> c{lx + (float(i) + 0.5f) * h, ly + (float(k) + 0.5f) * h, lz + (float(l) + 0.5f) * h};
-    const float r2 = wide * wide;
-    int32_t lo = 0x7fffffff, hi = -1;
-    int32_t stack[kStackDepth];
-    int sp = 0;
-    int32_t node = 0;
-    bool ok = true;
-    int32_t visits = m.n_tris + 8;
-    while (ok && --visits >= 0) {
-      const float4 *q = m.nodes4 + size_t(kNode4Quads) * node;
-      const float4 bx0 = q[0], by0 = q[1], bz0 = q[2], bx1 = q[3], by1 = q[4], bz1 = q[5];
-      const int4 ch = *reinterpret_cast<const int4 *>(q + 6);
-      const float d[4] = {box_d2<float>(c, bx0.x, by0.x, bz0.x, bx1.x, by1.x, bz1.x),
-                          box_d2<float>(c, bx0.y, by0.y, bz0.y, bx1.y, by1.y, bz1.y),
-                          box_d2<float>(c, bx0.z, by0.z, bz0.z, bx1.z, by1.z, bz1.z),
-                          box_d2<float>(c, bx0.w, by0.w, bz0.w, bx1.w, by1.w, bz1.w)};
-      const int32_t cc[4] = {ch.x, ch.y, ch.z, ch.w};
-      for (int s = 0; s < 4; ++s) {
-        if (cc[s] == kEmptyChild || !(d[s] <= r2)) continue;
-        if (cc[s] < 0) {
-          V3<float> a, b, t;
-          load_tri<float>(m, ~cc[s], a, b, t);
-          const V3<float> e = c - closest_point_triangle(c, a, b, t);
-          if (dot(e, e) <= r2) {
-            lo = min(lo, ~cc[s]);
-            hi = max(hi, ~cc[s]);
-          }
-        } else if (sp < kStackDepth) {
-          stack[sp++] = cc[s];
-        } else {
-          ok = false;
-        }
-      }
-      if (sp == 0) break;
-      node = stack[--sp];
-    }
-    if (ok && visits >= 0 && lo <= hi) {
-      node = 0;
+    int32_t lo, hi;
+    const bool ok = cell_candidates(m, c, wide * wide, lo, hi, count, cand);
+    if (ok && lo <= hi) {
+      int32_t node = 0;
       for (int depth = 0; depth < 64; ++depth) {
         const int4 ch = *reinterpret_cast<const int4 *>(m.nodes4 + size_t(kNode4Quads) * node + 6);
         const int32_t cc[4] = {ch.x, ch.y, ch.z, ch.w};
@@ -491,9 +546,21 @@ __global__ void cell_start_nodes(MeshView m, const uint4 *rec, float lx, float l
         node = next;
       }
       start = node;
+      cell_node = base >= 0 && count >= 1 && count <= 4;
     }
   }
-  out[idx] = start;
+  if (kPass == 0) {
+    if (cell_node) atomicAdd(counter, 1u);
+    out[idx] = cell_node ? ~start : start;
+    return;
+  }
+  const unsigned int slot = cell_node ? atomicAdd(counter, 1u) : 0xFFFFFFFFu;
+  if (slot < unsigned(capacity)) {
+    write_cell_node(m, cand, count, cell_nodes + size_t(kNode4Quads) * (size_t(base) + slot));
+    out[idx] = base + int32_t(slot);
+  } else {
+    out[idx] = ~out[idx];  // no node: back to the start node
+  }
 }
 
 // Lower bound of the mesh distance from the geometry's distance grid (-inf
@@ -670,23 +737,60 @@ hipError_t build_distance_grid(MeshBuffers &mb, const double lo[3], const double
   for (int k = 0; k < 3; ++k) mb.grid_n[k] = n[k] - 1;
   mb.grid = reinterpret_cast<float *>(rec);
   mb.grid_bytes = int64_t(sizeof(uint4)) * n_cells;
-  // Start nodes: reach three cells, so a query whose best distance is up to
-  // about 2.1 cells starts below the root.
+  // Start nodes: reach two cells, so a query whose best distance is up to
+  // about 1.1 cells starts below the root (a search from the root costs about
+  // eight visits, from a start node two; reaches of 1.5, 1.75, 2.5 and 3 cells
+  // measured slower on the SH icosphere at 1024 cells); cells with at most
+  // four triangles in reach start at a cell node of their own, appended to a copy of the
+  // 4-wide node array (mb.nodes4 then points at the copy; the caller frees
+  // the array it replaced).
   int32_t *start = nullptr;
-  if (hipMalloc(&start, sizeof(int32_t) * size_t(n_cells)) == hipSuccess) {
-    const float reach = 3.0f * mb.grid_h;
-    hipLaunchKernelGGL(cell_start_nodes, dim3(blocks_for(n_cells, 128)), dim3(128), 0, stream, m, rec, mb.grid_lo[0],
-                       mb.grid_lo[1], mb.grid_lo[2], mb.grid_h, n[0] - 1, n[1] - 1, n[2] - 1, reach, start);
-    e = hipGetLastError();
-    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  unsigned int *counter = nullptr;
+  if (hipMalloc(&start, sizeof(int32_t) * size_t(n_cells)) == hipSuccess &&
+      hipMalloc(&counter, sizeof(unsigned int)) == hipSuccess) {
+    const float reach = 2.0f * mb.grid_h;
+    const int32_t base = std::max(mb.n_tris - 1, 1);  // the tree's node capacity (lbvh_build)
+    auto pass = [&](auto kernel, float4 *cell_nodes, int32_t capacity) {
+      hipError_t pe = hipMemsetAsync(counter, 0, sizeof(unsigned int), stream);
+      if (pe != hipSuccess) return pe;
+      hipLaunchKernelGGL(kernel, dim3(blocks_for(n_cells, 128)), dim3(128), 0, stream, m, rec, mb.grid_lo[0],
+                         mb.grid_lo[1], mb.grid_lo[2], mb.grid_h, n[0] - 1, n[1] - 1, n[2] - 1, reach, start,
+                         counter, cell_nodes, base, capacity);
+      pe = hipGetLastError();
+      return pe == hipSuccess ? hipStreamSynchronize(stream) : pe;
+    };
+    e = pass(cell_start_nodes<0>, nullptr, 0);
+    unsigned int wanted = 0;
+    if (e == hipSuccess) e = hipMemcpy(&wanted, counter, sizeof(unsigned int), hipMemcpyDeviceToHost);
+    float4 *nodes = nullptr;
+    if (e == hipSuccess && wanted > 0) {
+      // Without room for the cell nodes the cells keep their start nodes.
+      if (hipMalloc(&nodes, sizeof(float4) * kNode4Quads * (size_t(base) + wanted)) != hipSuccess) {
+        (void)hipGetLastError();
+        nodes = nullptr;
+      }
+      if (nodes) e = hipMemcpyAsync(nodes, mb.nodes4, sizeof(float4) * kNode4Quads * size_t(base),
+                                    hipMemcpyDeviceToDevice, stream);
+      if (e == hipSuccess) e = pass(cell_start_nodes<1>, nodes, nodes ? int32_t(wanted) : 0);
+    }
+    (void)hipFree(counter);
     if (e != hipSuccess) {
       (void)hipFree(start);
+      if (nodes) (void)hipFree(nodes);
       return e;
     }
     const float valid = 0.999f * (reach - 0.8660254f * mb.grid_h);
     mb.grid_start = start;
     mb.grid_start_r2 = valid * valid;
     mb.grid_bytes += int64_t(sizeof(int32_t)) * n_cells;
+    if (nodes) {
+      mb.nodes4 = nodes;
+      mb.cell_nodes = int32_t(wanted);
+      mb.grid_bytes += int64_t(sizeof(float4)) * kNode4Quads * int64_t(wanted);
+    }
+  } else {
+    (void)hipGetLastError();
+    if (start) (void)hipFree(start);
   }
   return hipSuccess;
 }

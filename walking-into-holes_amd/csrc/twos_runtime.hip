@@ -1278,6 +1278,7 @@ twos_status twos_mesh_set_distance_grid(twos_ctx *ctx, int32_t mesh_id, int32_t 
   nb.grid_start = nullptr;
   nb.grid_start_r2 = 0.f;
   nb.grid_bytes = 0;
+  nb.cell_nodes = 0;
   hipEvent_t a, b;
   HIP_OK(hipEventCreate(&a));
   HIP_OK(hipEventCreate(&b));
@@ -1296,11 +1297,15 @@ twos_status twos_mesh_set_distance_grid(twos_ctx *ctx, int32_t mesh_id, int32_t 
   }
   (void)hipFree(m.mb.grid);
   if (m.mb.grid_start) (void)hipFree(m.mb.grid_start);
+  // A record grid's build copies the 4-wide nodes into an array with its cell
+  // nodes appended; the old array goes.
+  if (nb.nodes4 != m.mb.nodes4) (void)hipFree(m.mb.nodes4);
   m.mb = nb;
   m.grid_cells = cells;
   for (auto &kv : ctx->geoms) {
     GeomEntry &g = kv.second;
     if (g.desc.container != kMesh || g.desc.mesh != mesh_id) continue;
+    g.dev.nodes4 = nb.nodes4;
     g.dev.grid = nb.grid;
     for (int k = 0; k < 3; ++k) {
       g.dev.grid_lo[k] = nb.grid_lo[k];
