@@ -101,7 +101,7 @@ def _key(ctx, geom_sig: str, precision: int, combine: int, eps: float, grid: boo
 
 def autotune(ctx, geom: int, sites, sides, *, geom_sig: str, combine: int, precision: int = 32,
              eps: float = 1e-4, max_steps: int = 8192, distance_grid: bool = False, mesh=None,
-             sample: int = 4096, walks: int = 128, seed: int = 12345, repeats: int = 2,
+             sample: int = 4096, walks: int = 512, seed: int = 12345, repeats: int = 2,
              cache_path: str | os.PathLike | None = None, log=None) -> Tuning:
     """Choose the launch shape for estimates like the one described.
 
