@@ -278,6 +278,20 @@ twos_status twos_geometry_release(twos_ctx *ctx, int32_t geom_id);
 twos_status twos_estimate(twos_ctx *ctx, int32_t geom_id, const twos_side sides[2], const twos_params *params,
                           const double *sites, int64_t n, twos_site_result *out, twos_stats *stats);
 
+/* As twos_estimate with the sites and results in device memory (for example
+ * torch tensors' data_ptr(), ordered on their stream): d_sites holds n x 3
+ * binary64 positions, d_out receives n twos_site_result records, both
+ * ordered on `stream` (a hipStream_t; null for the null stream): the estimate
+ * starts after the work already queued on it, and work queued on it after
+ * the call sees the results.  Positions are not checked on the host: a
+ * non-finite one ends its paths capped (TWOS_ERR_STEP_CAP).  Replaces the same
+ * call as twos_estimate (wih_twos.cpp:1799-1864, the site loop :1828-1836), with
+ * the caller-owned device buffers of the intended compiled-kernel boundary
+ * (benchmarks/internal_cooling/README.md:75-76). */
+twos_status twos_estimate_device(twos_ctx *ctx, int32_t geom_id, const twos_side sides[2], const twos_params *params,
+                                 const double *d_sites, int64_t n, twos_site_result *d_out, twos_stats *stats,
+                                 void *stream);
+
 /* As twos_estimate, also writing each path's samples a, b (binary64) and step
  * count (bit 30 set when capped) as [n][stride] arrays, walks >= stride dropped. */
 twos_status twos_estimate_dump(twos_ctx *ctx, int32_t geom_id, const twos_side sides[2],

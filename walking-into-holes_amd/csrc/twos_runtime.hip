@@ -18,6 +18,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "twos_device.h"
@@ -33,6 +34,12 @@ twos_status fail(twos_status code, const std::string &msg) {
   g_error = msg;
   return code;
 }
+
+#define TWOS_TRY(x)                              \
+  do {                                           \
+    const twos_status st_ = (x);                 \
+    if (st_ != TWOS_OK) return st_;              \
+  } while (0)
 
 // A device failure: also clears the thread's last-error state (see HIP_OK).
 twos_status dev_fail(const char *what, hipError_t e) {
@@ -129,6 +136,8 @@ struct twos_ctx {
   twos::DevGeom *d_geoms = nullptr;
   int32_t d_geoms_cap = 0;
   std::unique_ptr<twos::Engine> engine;
+  // Orders device-pointer estimates after the caller's stream and back.
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
   // Every live engine on this context (its own and each server's): graphs
   // captured by any of them hold d_geoms, and queued or active requests hold
   // geometry ids.
@@ -157,6 +166,8 @@ struct Request {
   int32_t cls = 0;              // combine class
   bool admitted = false;
   bool capped = false;          // a path of some site hit max_steps
+  const double *d_sites = nullptr;  // device [n][3] positions (device entry points), else `sites`
+  int64_t steps_sum = 0, paths_sum = 0;  // over the request's finished sites
 };
 
 struct Engine {
@@ -262,16 +273,7 @@ struct Engine {
     HIP_OK(dalloc(&d_val, 2 * size_t(cap_walks)));
     HIP_OK(dalloc(&d_steps, 2 * size_t(cap_walks)));
     HIP_OK(dalloc(&d_counters, kMaxClasses + 4));
-    auto align = [](size_t v) { return (v + 255) & ~size_t(255); };
-    off_slot = 0;
-    off_base = align(off_slot + 4 * C);
-    off_prefix = align(off_base + 4 * C);
-    off_nact = align(off_prefix + 8 * (C + kMaxClasses));
-    off_total = align(off_nact + 4 * kMaxClasses);
-    in_bytes = align(off_total + 8 * kMaxClasses);
-    off_compact = 0;
-    off_decision = align(sizeof(twos_site_result) * C);
-    out_bytes = align(off_decision + 4 * C);
+    block_offsets();
     HIP_OK(dalloc(&d_in, in_bytes));
     HIP_OK(dalloc(&d_out, out_bytes));
     HIP_OK(hipHostMalloc(reinterpret_cast<void **>(&h_in), in_bytes, hipHostMallocDefault));
@@ -282,11 +284,49 @@ struct Engine {
     std::memset(h_plain, 0, 256);
     std::memset(h_in, 0, in_bytes);
     for (auto &e : ev) HIP_OK(hipEventCreate(&e));
+    reset_state();
+    return TWOS_OK;
+  }
+
+  // Offsets of the round input block (one H2D copy) and output block (one
+  // D2H copy) for cap_sites slots.
+  void block_offsets() {
+    const size_t C = size_t(cap_sites);
+    auto align = [](size_t v) { return (v + 255) & ~size_t(255); };
+    off_slot = 0;
+    off_base = align(off_slot + 4 * C);
+    off_prefix = align(off_base + 4 * C);
+    off_nact = align(off_prefix + 8 * (C + kMaxClasses));
+    off_total = align(off_nact + 4 * kMaxClasses);
+    in_bytes = align(off_total + 8 * kMaxClasses);
+    off_compact = 0;
+    off_decision = align(sizeof(twos_site_result) * C);
+    out_bytes = align(off_decision + 4 * C);
+  }
+
+  void reset_state() {
+    queue.clear();
+    active.clear();
+    failed.clear();
+    charged = 0;
     free_slots.clear();
     free_slots[0] = cap_sites;
     free_problems.clear();
     for (int32_t p = cap_problems - 1; p >= 0; --p) free_problems.push_back(p);
+  }
+
+  // Kernel timing with the engine's events.
+  twos_status ev_record(int k, hipStream_t st) {
+    if (ev[k]) HIP_OK(hipEventRecord(ev[k], st));
     return TWOS_OK;
+  }
+  float ev_ms(int a, int b) {
+    float ms = 0.f;
+    if (ev[a] && ev[b] && hipEventElapsedTime(&ms, ev[a], ev[b]) != hipSuccess) {
+      (void)hipGetLastError();
+      ms = 0.f;
+    }
+    return ms;
   }
 
   bool idle() const { return queue.empty() && active.empty(); }
@@ -381,7 +421,10 @@ struct Engine {
       wmax[i] = adaptive ? r->prm.walks_max : r->prm.walks;
       rel[i] = adaptive ? r->prm.ci_rel : -1.0;
     }
-    HIP_OK(hipMemcpyAsync(s.pos + 3 * s0, r->sites.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
+    if (r->d_sites)
+      HIP_OK(hipMemcpyAsync(s.pos + 3 * s0, r->d_sites, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, st));
+    else
+      HIP_OK(hipMemcpyAsync(s.pos + 3 * s0, r->sites.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(s.geom + s0, gi.data(), 4 * n, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(s.problem + s0, pi.data(), 4 * n, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(s.combine + s0, comb.data(), 4 * n, hipMemcpyHostToDevice, st));
@@ -613,7 +656,7 @@ struct Engine {
           *reinterpret_cast<int32_t *>(h_plain + 24) = 1;
           *reinterpret_cast<int64_t *>(h_plain + 32) = int64_t(en.inc) * jpw;
           HIP_OK(hipMemcpyAsync(d_plain, h_plain, 64, hipMemcpyHostToDevice, strm));
-          HIP_OK(hipEventRecord(ev[0], strm));
+          TWOS_TRY(ev_record(0, strm));
           WalkParams wp{};
           wp.combine = en.r->prm.combine;
           wp.rng = en.r->prm.rng;
@@ -630,7 +673,7 @@ struct Engine {
                                                             : launch_walk_simple_f32(kind, tb, wp, block, strm);
           if (le != hipSuccess) return dev_fail("walk launch", le);
           stats.walk_launches += 1;
-          HIP_OK(hipEventRecord(ev[1], strm));
+          TWOS_TRY(ev_record(1, strm));
           if (dump_a) {
             HIP_OK(hipStreamSynchronize(strm));
             const std::vector<Entry> one{en};
@@ -638,14 +681,11 @@ struct Engine {
             if (st != TWOS_OK) return st;
           }
           HIP_OK(launch_reduce(tb, wp, 1, strm));
-          HIP_OK(hipEventRecord(ev[2], strm));
+          TWOS_TRY(ev_record(2, strm));
           HIP_OK(hipMemcpyAsync(h_plain + 64, d_plain + 64, sizeof(twos_site_result) + 4, hipMemcpyDeviceToHost, strm));
           HIP_OK(hipStreamSynchronize(strm));
-          float ms = 0.f;
-          HIP_OK(hipEventElapsedTime(&ms, ev[0], ev[1]));
-          stats.walk_kernel_ms += ms;
-          HIP_OK(hipEventElapsedTime(&ms, ev[1], ev[2]));
-          stats.reduce_kernel_ms += ms;
+          stats.walk_kernel_ms += ev_ms(0, 1);
+          stats.reduce_kernel_ms += ev_ms(1, 2);
           twos_site_result res;
           int32_t dec;
           std::memcpy(&res, h_plain + 64, sizeof(res));
@@ -687,14 +727,12 @@ struct Engine {
         graph_key = key;
         stats.walk_launches -= 1;  // counted once per replay below
       }
-      HIP_OK(hipEventRecord(ev[0], strm));
+      TWOS_TRY(ev_record(0, strm));
       HIP_OK(hipGraphLaunch(graph_exec, strm));
-      HIP_OK(hipEventRecord(ev[1], strm));
+      TWOS_TRY(ev_record(1, strm));
       HIP_OK(hipStreamSynchronize(strm));
       stats.walk_launches += 1;
-      float ms = 0.f;
-      HIP_OK(hipEventElapsedTime(&ms, ev[0], ev[1]));
-      stats.walk_kernel_ms += ms;
+      stats.walk_kernel_ms += ev_ms(0, 1);
       const auto *res = reinterpret_cast<twos_site_result *>(h_out + off_compact);
       const auto *dec = reinterpret_cast<int32_t *>(h_out + off_decision);
       for (size_t k = 0; k < by_class[0].size(); ++k)
@@ -707,10 +745,10 @@ struct Engine {
         for (const Entry &en : by_class[c])
           if (geom_kind_of(en.r->geom) != kind) kind = kGeomGeneric;
         if (rc->prm.flags & TWOS_FLAG_COUNT_TRAVERSAL) HIP_OK(launch_zero(d_counters + kMaxClasses, 4, strm));
-        HIP_OK(hipEventRecord(ev[0], strm));
+        TWOS_TRY(ev_record(0, strm));
         st = launch_walks(rc, int(c), e0s[c], totals[c], kind, class_bare(by_class[c]));
         if (st != TWOS_OK) return st;
-        HIP_OK(hipEventRecord(ev[1], strm));
+        TWOS_TRY(ev_record(1, strm));
         if (dump_a) {
           HIP_OK(hipStreamSynchronize(strm));
           st = dump_round(by_class[c], dump_a, dump_b, dump_steps, dump_stride);
@@ -722,13 +760,10 @@ struct Engine {
         wp.steps = d_steps;
         wp.grad = grad_of(rc);
         HIP_OK(launch_reduce(tb, wp, int(by_class[c].size()), strm));
-        HIP_OK(hipEventRecord(ev[2], strm));
+        TWOS_TRY(ev_record(2, strm));
         HIP_OK(hipStreamSynchronize(strm));
-        float ms = 0.f;
-        HIP_OK(hipEventElapsedTime(&ms, ev[0], ev[1]));
-        stats.walk_kernel_ms += ms;
-        HIP_OK(hipEventElapsedTime(&ms, ev[1], ev[2]));
-        stats.reduce_kernel_ms += ms;
+        stats.walk_kernel_ms += ev_ms(0, 1);
+        stats.reduce_kernel_ms += ev_ms(1, 2);
         if (rc->prm.flags & TWOS_FLAG_COUNT_TRAVERSAL) {
           unsigned long long cnt[4];
           HIP_OK(hipMemcpy(cnt, d_counters + kMaxClasses, sizeof(cnt), hipMemcpyDeviceToHost));
@@ -775,6 +810,8 @@ struct Engine {
       r->remaining -= 1;
       if (r->out) r->out[i] = res;
       if (res.caps > 0) r->capped = true;
+      r->steps_sum += res.steps;
+      r->paths_sum += res.walks * (r->prm.combine == kCross || r->prm.combine == kCoupled ? 2 : 1);
     }
   }
 
@@ -867,7 +904,8 @@ twos_status check_side(const twos_side &s) {
 }
 
 Request *make_request(twos_ctx *ctx, int32_t geom, const twos_side sides[2], const twos_params *p,
-                      const double *sites, int64_t n, twos_site_result *out, twos_status *st) {
+                      const double *sites, int64_t n, twos_site_result *out, twos_status *st,
+                      const double *d_sites = nullptr) {
   *st = TWOS_OK;
   if (ctx->geoms.find(geom) == ctx->geoms.end()) {
     *st = fail(TWOS_ERR_NOT_FOUND, "unknown geometry id");
@@ -894,14 +932,23 @@ Request *make_request(twos_ctx *ctx, int32_t geom, const twos_side sides[2], con
       return nullptr;
     }
   }
-  if (n > 0 && (!sites || !out)) {
-    *st = fail(TWOS_ERR_INVALID_ARGUMENT, "sites and out must be non-null");
-    return nullptr;
-  }
-  for (int64_t i = 0; i < 3 * n; ++i) {
-    if (!std::isfinite(sites[i])) {
-      *st = fail(TWOS_ERR_INVALID_ARGUMENT, "non-finite site coordinate");
+  if (d_sites) {
+    // Device positions stay on the device: a non-finite one is not caught
+    // here, its paths end capped (no query from a non-finite point).
+    if (n > 0 && out) {
+      *st = fail(TWOS_ERR_INVALID_ARGUMENT, "device requests write their results on the device");
       return nullptr;
+    }
+  } else {
+    if (n > 0 && (!sites || !out)) {
+      *st = fail(TWOS_ERR_INVALID_ARGUMENT, "sites and out must be non-null");
+      return nullptr;
+    }
+    for (int64_t i = 0; i < 3 * n; ++i) {
+      if (!std::isfinite(sites[i])) {
+        *st = fail(TWOS_ERR_INVALID_ARGUMENT, "non-finite site coordinate");
+        return nullptr;
+      }
     }
   }
   auto *r = new Request;
@@ -909,7 +956,8 @@ Request *make_request(twos_ctx *ctx, int32_t geom, const twos_side sides[2], con
   r->prm = *p;
   r->sides[0] = sides[0];
   r->sides[1] = sides[1];
-  r->sites.assign(sites, sites + 3 * n);
+  if (d_sites) r->d_sites = d_sites;
+  else r->sites.assign(sites, sites + 3 * n);
   r->n = n;
   r->out = out;
   return r;
@@ -1065,11 +1113,24 @@ twos_status ensure_engine(twos_ctx *ctx, const twos_params *p, int64_t n) {
   return TWOS_OK;
 }
 
+// Sites and results in device memory, ordered on the caller's stream.
+struct DeviceIo {
+  const double *sites = nullptr;    // [n][3]
+  twos_site_result *out = nullptr;  // [n]
+  hipStream_t stream = nullptr;
+};
+
+// One estimate on the context's own engine.  With `dio`, sites and results
+// stay on the device: the
+// context's stream first waits for the work queued on dio->stream, each
+// chunk's results are copied device to device from the slots, and
+// dio->stream then waits for those copies.
 twos_status run_estimate(twos_ctx *ctx, int32_t geom, const twos_side sides[2], const twos_params *p,
                          const double *sites, int64_t n, twos_site_result *out, twos_stats *stats, double *da,
-                         double *db, int32_t *ds, int64_t stride) {
+                         double *db, int32_t *ds, int64_t stride, const DeviceIo *dio = nullptr) {
   twos_status st = check_params(p, n);
   if (st != TWOS_OK) return st;
+  if (dio && n > 0 && (!dio->sites || !dio->out)) return fail(TWOS_ERR_INVALID_ARGUMENT, "device sites and results");
   if (n == 0) {
     if (stats) *stats = twos_stats{};
     return TWOS_OK;
@@ -1082,15 +1143,22 @@ twos_status run_estimate(twos_ctx *ctx, int32_t geom, const twos_side sides[2], 
   Engine *e = ctx->engine.get();
   if (!e->idle()) return fail(TWOS_ERR_INVALID_ARGUMENT, "context engine busy");
   e->stats = twos_stats{};
+  if (dio) {
+    HIP_OK(hipEventRecord(ctx->ev_in, dio->stream));
+    HIP_OK(hipStreamWaitEvent(ctx->stream, ctx->ev_in, 0));
+  }
   // Chunk the sites so each chunk fits the engine's slots; the global site
   // index (site_offset + i) keeps every stream identical to one big call.
   const int64_t chunk = e->cap_sites;
   bool capped = false;
+  int64_t steps = 0;
+  e->stats.paths = 0;
   for (int64_t b = 0; b < n; b += chunk) {
     const int64_t m = std::min(chunk, n - b);
     twos_params q = *p;
     q.site_offset = p->site_offset + int32_t(b);
-    Request *r = make_request(ctx, geom, sides, &q, sites + 3 * b, m, out + b, &st);
+    Request *r = dio ? make_request(ctx, geom, sides, &q, nullptr, m, nullptr, &st, dio->sites + 3 * b)
+                     : make_request(ctx, geom, sides, &q, sites + 3 * b, m, out + b, &st);
     if (!r) return st;
     std::unique_ptr<Request> hold(r);
     r->id = e->next_id++;
@@ -1100,21 +1168,23 @@ twos_status run_estimate(twos_ctx *ctx, int32_t geom, const twos_side sides[2], 
       st = e->step(done, da ? da + b * stride : nullptr, db ? db + b * stride : nullptr, ds ? ds + b * stride : nullptr,
                    stride);
       if (st != TWOS_OK) {
-        e->queue.clear();
-        e->active.clear();
-        (void)e->allocate(e->cap_sites, e->cap_walks, e->cap_problems);
+        e->reset_state();
         return st;
       }
     }
-    for (int64_t i = 0; i < m; ++i) capped = capped || out[b + i].caps > 0;
+    // The slots keep each site's last result until the next chunk is admitted.
+    if (dio)
+      HIP_OK(hipMemcpyAsync(dio->out + b, e->s.result + r->slot0, sizeof(twos_site_result) * size_t(m),
+                            hipMemcpyDeviceToDevice, ctx->stream));
+    capped = capped || r->capped;
+    steps += r->steps_sum;
+    e->stats.paths += r->paths_sum;
+  }
+  if (dio) {
+    HIP_OK(hipEventRecord(ctx->ev_out, ctx->stream));
+    HIP_OK(hipStreamWaitEvent(dio->stream, ctx->ev_out, 0));
   }
   e->stats.seconds = now_s() - t0;
-  e->stats.paths = 0;
-  int64_t steps = 0;
-  for (int64_t i = 0; i < n; ++i) {
-    steps += out[i].steps;
-    e->stats.paths += out[i].walks * (p->combine == kCross || p->combine == kCoupled ? 2 : 1);
-  }
   e->stats.steps = steps;
   if (stats) *stats = e->stats;
   if (capped) return fail(TWOS_ERR_STEP_CAP, "one or more paths hit --max-steps");
@@ -1152,6 +1222,8 @@ twos_status twos_create(int32_t device, uint64_t hbm_budget_bytes, twos_ctx **ou
   size_t free_b = 0, total_b = 0;
   HIP_OK(hipMemGetInfo(&free_b, &total_b));
   ctx->budget = hbm_budget_bytes ? hbm_budget_bytes : uint64_t(free_b / 4);
+  HIP_OK(hipEventCreateWithFlags(&ctx->ev_in, hipEventDisableTiming));
+  HIP_OK(hipEventCreateWithFlags(&ctx->ev_out, hipEventDisableTiming));
   *out = ctx.release();
   return TWOS_OK;
 }
@@ -1161,6 +1233,8 @@ void twos_destroy(twos_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   ctx->engine.reset();
+  if (ctx->ev_in) (void)hipEventDestroy(ctx->ev_in);
+  if (ctx->ev_out) (void)hipEventDestroy(ctx->ev_out);
   // Servers still alive lose their device buffers; their handles only accept
   // twos_server_destroy from now on.
   for (Engine *e : ctx->engines) {
@@ -1446,6 +1520,14 @@ twos_status twos_estimate(twos_ctx *ctx, int32_t geom_id, const twos_side sides[
                           const double *sites, int64_t n, twos_site_result *out, twos_stats *stats) {
   if (!ctx) return fail(TWOS_ERR_INVALID_ARGUMENT, "null context");
   return run_estimate(ctx, geom_id, sides, params, sites, n, out, stats, nullptr, nullptr, nullptr, 0);
+}
+
+twos_status twos_estimate_device(twos_ctx *ctx, int32_t geom_id, const twos_side sides[2], const twos_params *params,
+                                 const double *d_sites, int64_t n, twos_site_result *d_out, twos_stats *stats,
+                                 void *stream) {
+  if (!ctx) return fail(TWOS_ERR_INVALID_ARGUMENT, "null context");
+  const DeviceIo dio{d_sites, d_out, static_cast<hipStream_t>(stream)};
+  return run_estimate(ctx, geom_id, sides, params, nullptr, n, nullptr, stats, nullptr, nullptr, nullptr, 0, &dio);
 }
 
 twos_status twos_estimate_dump(twos_ctx *ctx, int32_t geom_id, const twos_side sides[2], const twos_params *params,

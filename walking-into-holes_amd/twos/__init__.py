@@ -33,7 +33,8 @@ __all__ = [
     "BALL", "BOX", "MESH", "SQUARE", "PAIRED", "CROSS", "COUPLED", "RNG_PHILOX", "RNG_MT_SITE", "RNG_MT_PAIRED",
     "RNG_MT_KEY", "RNG_MT_FD", "probe_side", "PLAIN", "PERSISTENT", "FLAG_COUNT_TRAVERSAL", "FLAG_DISTANCE_GRID", "FLAG_GRADIENT", "version",
     "TERM_BUMP", "TERM_UNIT", "TERM_GAUSS", "MAX_TERMS", "study_terms", "housing_chip_terms",
-    "CERT_AFFINE", "CERT_HOUSING", "CERT_CAPPED", "CERT_CLIPPED", "CERT_CONTAINMENT",
+    "CERT_AFFINE", "CERT_HOUSING", "CERT_CAPPED", "CERT_CLIPPED", "CERT_CONTAINMENT", "RESULT_BYTES",
+    "results_from_bytes",
 ]
 
 
@@ -255,6 +256,15 @@ def _result(out, n, stats: dict, gradient: bool = False) -> Result:
         res.grad_u, res.grad_v = rec["grad_u"].copy(), rec["grad_v"].copy()
         res.grad_dot, res.grad_dot_se = rec["grad_dot"].copy(), rec["grad_dot_se"].copy()
     return res
+
+
+RESULT_BYTES = C.sizeof(_capi.SiteResult)
+
+
+def results_from_bytes(buf, n: int) -> Result:
+    """Result of n SiteResult records read back from device memory (bytes)."""
+    raw = (_capi.SiteResult * max(n, 1)).from_buffer_copy(bytes(buf)[: RESULT_BYTES * n].ljust(RESULT_BYTES, b"\0"))
+    return _result(raw, n, {})
 
 
 def _stats(st: _capi.Stats) -> dict:
@@ -515,6 +525,34 @@ class Context:
             return res
         _check(status, res)
         return res
+
+    def estimate_device(self, geom: int, d_sites: int, n: int, d_out: int, sides, *, stream: int = 0,
+                        combine=SQUARE, rng=RNG_PHILOX, precision=32, schedule=PERSISTENT,
+                        seed=0, repeat=0, walks=16, walks_max=0, ci_rel=0.02, ci_z=1.96, eps=5e-4, max_steps=512,
+                        block=0, waves_per_simd=0, site_offset=0, count_traversal=False, distance_grid=False,
+                        grid_kappa=0.0, lds_stack=0, step_batch=0, refill_batch=0, use_graph=False,
+                        gradient=False, raise_on_cap: bool = True) -> dict:
+        """estimate() on device memory: d_sites is the device address of n x 3
+        binary64 positions, d_out of n SiteResult records (RESULT_BYTES each),
+        both ordered on `stream` (a hipStream_t handle, 0 for the null
+        stream): the estimate starts after the work queued there and later
+        work there sees the results.  Returns the stats dict; decode d_out
+        with results_from_bytes."""
+        sd = (Side * 2)(sides[0], sides[1] if len(sides) > 1 else sides[0])
+        prm = _params(combine=combine, rng=rng, precision=precision, schedule=schedule, seed=seed, repeat=repeat,
+                      max_steps=max_steps, eps=eps, walks=walks, walks_max=walks_max, ci_rel=ci_rel, ci_z=ci_z,
+                      block=block, waves_per_simd=waves_per_simd, site_offset=site_offset,
+                      flags=(FLAG_COUNT_TRAVERSAL if count_traversal else 0)
+                      | (FLAG_DISTANCE_GRID if distance_grid else 0) | (FLAG_GRADIENT if gradient else 0),
+                      grid_kappa=grid_kappa, lds_stack=lds_stack, step_batch=step_batch, refill_batch=refill_batch,
+                      use_graph=use_graph)
+        st = _capi.Stats()
+        status = _lib().twos_estimate_device(self._h, geom, sd, C.byref(prm), C.c_void_p(d_sites), n, C.c_void_p(d_out), C.byref(st),
+                    C.c_void_p(stream))
+        stats = _stats(st)
+        if not (status == _capi.ERR_STEP_CAP and not raise_on_cap):
+            _check(status)
+        return stats
 
     def server(self, max_sites: int, max_walks_per_round: int) -> "Server":
         return Server(self, max_sites, max_walks_per_round)
