@@ -292,6 +292,22 @@ twos_status twos_estimate_device(twos_ctx *ctx, int32_t geom_id, const twos_side
                                  const double *d_sites, int64_t n, twos_site_result *d_out, twos_stats *stats,
                                  void *stream);
 
+/* As twos_estimate_device, allocating nothing: no hipMalloc, hipFree or other
+ * allocation of device or pinned memory in the call.  No counterpart in the
+ * reference; it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  Every
+ * device buffer of the estimate (site slots, round blocks, per-walk samples)
+ * lives in one static 32-MiB device array per GPU, laid out when the context
+ * is created; sites run in chunks of 4096, and walks or walks_max per side
+ * must fit one round of that array (TWOS_ERR_ADMISSION with the limit
+ * otherwise); use_graph and TWOS_FLAG_GRADIENT are rejected; stats carry no
+ * kernel times.  Every context on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (from two contexts or threads) race; calls on one
+ * context are serialised. */
+twos_status twos_estimate_device_static(twos_ctx *ctx, int32_t geom_id, const twos_side sides[2],
+                                        const twos_params *params, const double *d_sites, int64_t n,
+                                        twos_site_result *d_out, twos_stats *stats, void *stream);
+
 /* As twos_estimate, also writing each path's samples a, b (binary64) and step
  * count (bit 30 set when capped) as [n][stride] arrays, walks >= stride dropped. */
 twos_status twos_estimate_dump(twos_ctx *ctx, int32_t geom_id, const twos_side sides[2],

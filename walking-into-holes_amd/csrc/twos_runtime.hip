@@ -136,6 +136,9 @@ struct twos_ctx {
   twos::DevGeom *d_geoms = nullptr;
   int32_t d_geoms_cap = 0;
   std::unique_ptr<twos::Engine> engine;
+  // The allocation-free estimate's engine, laid out in the device's static
+  // scratch array at twos_create (not listed in `engines`: static memory).
+  std::unique_ptr<twos::Engine> arena_engine;
   // Orders device-pointer estimates after the caller's stream and back.
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
   // Every live engine on this context (its own and each server's): graphs
@@ -235,12 +238,27 @@ struct Engine {
     return false;
   }
 
+  // Arena engines (the allocation-free entry point) carve every device
+  // buffer from the device's static scratch array and keep their host blocks
+  // in host_arena (pageable): nothing to free, no events, no graphs.
+  bool arena = false;
+  std::vector<char> host_arena;
+
   void release() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
     graph_exec = nullptr;
     graph = nullptr;
     graph_key = ~0ull;
+    if (arena) {
+      s = SlotArrays{};
+      d_sides = nullptr; d_val = nullptr; d_steps = nullptr; d_counters = nullptr;
+      d_in = d_out = d_plain = nullptr;
+      d_grad = nullptr;
+      h_in = h_out = h_plain = nullptr;
+      host_arena.clear();
+      return;
+    }
     dfree(s.pos); dfree(s.geom); dfree(s.problem); dfree(s.combine); dfree(s.keys); dfree(s.site); dfree(s.seed);
     dfree(s.repeat); dfree(s.walks_max); dfree(s.max_steps); dfree(s.eps); dfree(s.ci_rel); dfree(s.ci_z);
     dfree(s.acc); dfree(s.acc_walks); dfree(s.acc_steps); dfree(s.acc_caps); dfree(s.result); dfree(s.hint0); dfree(s.grid_kappa);
@@ -315,7 +333,60 @@ struct Engine {
     for (int32_t p = cap_problems - 1; p >= 0; --p) free_problems.push_back(p);
   }
 
-  // Kernel timing with the engine's events.
+  // Lays the engine out in `bytes` of device memory at `dev` (256-byte
+  // aligned): `sites` slots without the gradient channel, one problem, and
+  // as many walks per round as the rest holds.  Allocates nothing on the
+  // device; the host blocks live in host_arena.
+  twos_status allocate_arena(char *dev, size_t bytes, int64_t sites) {
+    release();
+    arena = true;
+    cap_sites = std::max<int64_t>(sites, 1);
+    cap_problems = 1;
+    cap_walks = 0;
+    const size_t C = size_t(cap_sites);
+    char *cur = dev;
+    size_t left = bytes;
+    bool ok = true;
+    auto carve = [&](auto *&ptr, size_t count) {
+      const size_t need = (sizeof(*ptr) * std::max<size_t>(count, 1) + 255) & ~size_t(255);
+      if (!ok || need > left) {
+        ok = false;
+        return;
+      }
+      ptr = reinterpret_cast<std::remove_reference_t<decltype(ptr)>>(cur);
+      cur += need;
+      left -= need;
+    };
+    carve(s.pos, 3 * C); carve(s.geom, C); carve(s.problem, C); carve(s.combine, C); carve(s.keys, 2 * C);
+    carve(s.site, C); carve(s.seed, C); carve(s.repeat, C); carve(s.walks_max, C); carve(s.max_steps, C);
+    carve(s.eps, C); carve(s.ci_rel, C); carve(s.ci_z, C); carve(s.acc, 8 * C); carve(s.acc_walks, C);
+    carve(s.acc_steps, C); carve(s.acc_caps, C); carve(s.result, C); carve(s.hint0, C); carve(s.grid_kappa, C);
+    s.acc_grad = nullptr;
+    carve(d_sides, 2);
+    carve(d_counters, kMaxClasses + 4);
+    block_offsets();
+    carve(d_in, in_bytes);
+    carve(d_out, out_bytes);
+    carve(d_plain, 256);
+    // The rest: per-walk samples (two binary64) and step words (two int32).
+    const int64_t walks = ok && left > 2 * 256 ? int64_t((left - 2 * 256) / (2 * (sizeof(double) + sizeof(int32_t)))) : 0;
+    if (walks >= 64) {
+      carve(d_val, 2 * size_t(walks));
+      carve(d_steps, 2 * size_t(walks));
+      cap_walks = walks;
+    } else {
+      ok = false;
+    }
+    if (!ok) return fail(TWOS_ERR_ADMISSION, "the static scratch array is too small for the engine");
+    host_arena.assign(in_bytes + out_bytes + 256, 0);
+    h_in = host_arena.data();
+    h_out = h_in + in_bytes;
+    h_plain = h_out + out_bytes;
+    reset_state();
+    return TWOS_OK;
+  }
+
+  // Kernel timing with the engine's events (arena engines have none).
   twos_status ev_record(int k, hipStream_t st) {
     if (ev[k]) HIP_OK(hipEventRecord(ev[k], st));
     return TWOS_OK;
@@ -1120,14 +1191,16 @@ struct DeviceIo {
   hipStream_t stream = nullptr;
 };
 
-// One estimate on the context's own engine.  With `dio`, sites and results
-// stay on the device: the
+// One estimate on the context's own engine, or on its arena engine (noalloc:
+// every device buffer in the device's static scratch array, nothing
+// allocated).  With `dio`, sites and results stay on the device: the
 // context's stream first waits for the work queued on dio->stream, each
 // chunk's results are copied device to device from the slots, and
 // dio->stream then waits for those copies.
 twos_status run_estimate(twos_ctx *ctx, int32_t geom, const twos_side sides[2], const twos_params *p,
                          const double *sites, int64_t n, twos_site_result *out, twos_stats *stats, double *da,
-                         double *db, int32_t *ds, int64_t stride, const DeviceIo *dio = nullptr) {
+                         double *db, int32_t *ds, int64_t stride, const DeviceIo *dio = nullptr,
+                         bool noalloc = false) {
   twos_status st = check_params(p, n);
   if (st != TWOS_OK) return st;
   if (dio && n > 0 && (!dio->sites || !dio->out)) return fail(TWOS_ERR_INVALID_ARGUMENT, "device sites and results");
@@ -1138,9 +1211,20 @@ twos_status run_estimate(twos_ctx *ctx, int32_t geom, const twos_side sides[2], 
   const double t0 = now_s();
   std::lock_guard<std::mutex> lock(ctx->mu);
   HIP_OK(hipSetDevice(ctx->device));
-  st = ensure_engine(ctx, p, n);
-  if (st != TWOS_OK) return st;
-  Engine *e = ctx->engine.get();
+  Engine *e;
+  if (noalloc) {
+    e = ctx->arena_engine.get();
+    if (!e) return fail(TWOS_ERR_DEVICE, "no static scratch array on this device");
+    if (p->use_graph || (p->flags & TWOS_FLAG_GRADIENT))
+      return fail(TWOS_ERR_INVALID_ARGUMENT, "the allocation-free estimate runs without graphs and gradients");
+    if (std::max(p->walks, p->walks_max) > e->cap_walks)
+      return fail(TWOS_ERR_ADMISSION, "walks per side above the static scratch array's " +
+                                          std::to_string(e->cap_walks) + " walks per round");
+  } else {
+    st = ensure_engine(ctx, p, n);
+    if (st != TWOS_OK) return st;
+    e = ctx->engine.get();
+  }
   if (!e->idle()) return fail(TWOS_ERR_INVALID_ARGUMENT, "context engine busy");
   e->stats = twos_stats{};
   if (dio) {
@@ -1193,6 +1277,40 @@ twos_status run_estimate(twos_ctx *ctx, int32_t geom, const twos_side sides[2], 
 
 }  // namespace twos
 
+namespace twos {
+namespace {
+
+// Device scratch of the allocation-free estimate (twos_estimate_device_static):
+// every buffer of its engine (site slots, round blocks, per-walk samples) lives
+// here.  Each device holds its own copy from module load to process end, and
+// every context on a device shares it.  In host code `scratch` names a host
+// stand-in: only the address from hipGetSymbolAddress, kept per device in
+// g_scratch, may be used.
+constexpr size_t kScratchBytes = size_t(32) << 20;
+constexpr int64_t kScratchSites = 4096;  // sites per chunk
+__device__ __align__(256) char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[kMaxDevices] = {};
+
+// The device's scratch address; the device must be current.
+twos_status scratch_address(int device, void **out) {
+  if (device < 0 || device >= kMaxDevices) return fail(TWOS_ERR_DEVICE, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void *ptr = nullptr;
+    HIP_OK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(TWOS_ERR_DEVICE, "hipGetSymbolAddress gave no scratch address");
+    g_scratch[device] = ptr;
+  }
+  *out = g_scratch[device];
+  return TWOS_OK;
+}
+
+}  // namespace
+}  // namespace twos
+
 struct twos_server {
   twos_ctx *ctx = nullptr;
   std::unique_ptr<twos::Engine> engine;
@@ -1224,6 +1342,15 @@ twos_status twos_create(int32_t device, uint64_t hbm_budget_bytes, twos_ctx **ou
   ctx->budget = hbm_budget_bytes ? hbm_budget_bytes : uint64_t(free_b / 4);
   HIP_OK(hipEventCreateWithFlags(&ctx->ev_in, hipEventDisableTiming));
   HIP_OK(hipEventCreateWithFlags(&ctx->ev_out, hipEventDisableTiming));
+  // The allocation-free entry point's engine, laid out once in the device's
+  // static scratch array.
+  void *arena = nullptr;
+  twos_status st = scratch_address(device, &arena);
+  if (st != TWOS_OK) return st;
+  ctx->arena_engine = std::make_unique<Engine>();
+  ctx->arena_engine->ctx = ctx.get();
+  st = ctx->arena_engine->allocate_arena(static_cast<char *>(arena), kScratchBytes, kScratchSites);
+  if (st != TWOS_OK) return st;
   *out = ctx.release();
   return TWOS_OK;
 }
@@ -1233,6 +1360,7 @@ void twos_destroy(twos_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   ctx->engine.reset();
+  ctx->arena_engine.reset();
   if (ctx->ev_in) (void)hipEventDestroy(ctx->ev_in);
   if (ctx->ev_out) (void)hipEventDestroy(ctx->ev_out);
   // Servers still alive lose their device buffers; their handles only accept
@@ -1528,6 +1656,15 @@ twos_status twos_estimate_device(twos_ctx *ctx, int32_t geom_id, const twos_side
   if (!ctx) return fail(TWOS_ERR_INVALID_ARGUMENT, "null context");
   const DeviceIo dio{d_sites, d_out, static_cast<hipStream_t>(stream)};
   return run_estimate(ctx, geom_id, sides, params, nullptr, n, nullptr, stats, nullptr, nullptr, nullptr, 0, &dio);
+}
+
+twos_status twos_estimate_device_static(twos_ctx *ctx, int32_t geom_id, const twos_side sides[2],
+                                        const twos_params *params, const double *d_sites, int64_t n,
+                                        twos_site_result *d_out, twos_stats *stats, void *stream) {
+  if (!ctx) return fail(TWOS_ERR_INVALID_ARGUMENT, "null context");
+  const DeviceIo dio{d_sites, d_out, static_cast<hipStream_t>(stream)};
+  return run_estimate(ctx, geom_id, sides, params, nullptr, n, nullptr, stats, nullptr, nullptr, nullptr, 0, &dio,
+                      true);
 }
 
 twos_status twos_estimate_dump(twos_ctx *ctx, int32_t geom_id, const twos_side sides[2], const twos_params *params,

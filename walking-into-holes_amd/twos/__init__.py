@@ -527,7 +527,7 @@ class Context:
         return res
 
     def estimate_device(self, geom: int, d_sites: int, n: int, d_out: int, sides, *, stream: int = 0,
-                        combine=SQUARE, rng=RNG_PHILOX, precision=32, schedule=PERSISTENT,
+                        static: bool = False, combine=SQUARE, rng=RNG_PHILOX, precision=32, schedule=PERSISTENT,
                         seed=0, repeat=0, walks=16, walks_max=0, ci_rel=0.02, ci_z=1.96, eps=5e-4, max_steps=512,
                         block=0, waves_per_simd=0, site_offset=0, count_traversal=False, distance_grid=False,
                         grid_kappa=0.0, lds_stack=0, step_batch=0, refill_batch=0, use_graph=False,
@@ -536,8 +536,10 @@ class Context:
         binary64 positions, d_out of n SiteResult records (RESULT_BYTES each),
         both ordered on `stream` (a hipStream_t handle, 0 for the null
         stream): the estimate starts after the work queued there and later
-        work there sees the results.  Returns the stats dict; decode d_out
-        with results_from_bytes."""
+        work there sees the results.  static=True runs the allocation-free
+        variant (every buffer in the device's static scratch array; no
+        graphs or gradients, walks per side up to its per-round limit).
+        Returns the stats dict; decode d_out with results_from_bytes."""
         sd = (Side * 2)(sides[0], sides[1] if len(sides) > 1 else sides[0])
         prm = _params(combine=combine, rng=rng, precision=precision, schedule=schedule, seed=seed, repeat=repeat,
                       max_steps=max_steps, eps=eps, walks=walks, walks_max=walks_max, ci_rel=ci_rel, ci_z=ci_z,
@@ -547,7 +549,8 @@ class Context:
                       grid_kappa=grid_kappa, lds_stack=lds_stack, step_batch=step_batch, refill_batch=refill_batch,
                       use_graph=use_graph)
         st = _capi.Stats()
-        status = _lib().twos_estimate_device(self._h, geom, sd, C.byref(prm), C.c_void_p(d_sites), n, C.c_void_p(d_out), C.byref(st),
+        fn = _lib().twos_estimate_device_static if static else _lib().twos_estimate_device
+        status = fn(self._h, geom, sd, C.byref(prm), C.c_void_p(d_sites), n, C.c_void_p(d_out), C.byref(st),
                     C.c_void_p(stream))
         stats = _stats(st)
         if not (status == _capi.ERR_STEP_CAP and not raise_on_cap):

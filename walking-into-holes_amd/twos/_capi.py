@@ -109,6 +109,8 @@ SIGNATURES = {
                                   C.POINTER(SiteResult), C.POINTER(Stats)]),
     "twos_estimate_device": (C.c_int32, [_ctx_p, C.c_int32, C.POINTER(Side), C.POINTER(Params), C.c_void_p,
                                          C.c_int64, C.c_void_p, C.POINTER(Stats), C.c_void_p]),
+    "twos_estimate_device_static": (C.c_int32, [_ctx_p, C.c_int32, C.POINTER(Side), C.POINTER(Params), C.c_void_p,
+                                                C.c_int64, C.c_void_p, C.POINTER(Stats), C.c_void_p]),
     "twos_estimate_dump": (C.c_int32, [_ctx_p, C.c_int32, C.POINTER(Side), C.POINTER(Params), _dp, C.c_int64,
                                        C.POINTER(SiteResult), C.POINTER(Stats), _dp, _dp, _i32p, C.c_int64]),
     "twos_inside": (C.c_int32, [_ctx_p, C.c_int32, _dp, C.c_int64, _i32p]),
