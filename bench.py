@@ -50,9 +50,11 @@ def parse() -> argparse.Namespace:
                              "certify"])
     ap.add_argument("--design-rounds", type=int, default=2, help="design: insertion rounds")
     ap.add_argument("--design-candidates", type=int, default=65536, help="design: candidate pool size")
-    ap.add_argument("--design-scout-walks", type=int, default=512, help="design: scout walks per side")
+    ap.add_argument("--design-scout-walks", type=int, default=4096, help="design: scout walks per side")
     ap.add_argument("--requests", type=int, default=300, help="serve: number of requests")
-    ap.add_argument("--rate", type=float, default=6.0, help="serve: Poisson arrivals per second")
+    ap.add_argument("--rate", type=float, default=40.0, help="serve: Poisson arrivals per second")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the short serving stream and design loop the default line carries")
     ap.add_argument("--plain-sites", type=int, default=512)
     ap.add_argument("--cpu-sites", type=int, default=160)
     ap.add_argument("--no-cpu", action="store_true")
@@ -311,6 +313,13 @@ def main() -> int:
         out["plain_gpu"] = {"sites_per_s": round(plain_rate, 2), "sample_sites": len(idx),
                             "seconds": round(tp, 3), "batched_sample_sites_per_s": round(finished(batched) / tb, 2)}
         out["x_plain"] = round(value / plain_rate, 2) if plain_rate > 0 else None
+    if w.container == twos.MESH and not args.no_extras:
+        # Configurations 4 and 5 in short form on the same device: 200 requests
+        # of the serving stream and two rounds of the design loop.
+        t0 = time.perf_counter()
+        out["serving"] = serve_run(ctx, 200, 40.0, tuned=not args.no_autotune)
+        out["design"] = design_run(ctx, 2, 65536, 4096)
+        log(f"serving and design {time.perf_counter() - t0:.1f} s")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, sites, args.cpu_sites)
     ctx.close()
@@ -318,16 +327,16 @@ def main() -> int:
     return 0
 
 
-def serve(args) -> int:
-    """Configuration 4: a Poisson request stream over the three resident
-    geometries, continuous batching, p50/p99 latency."""
+def serve_run(ctx, n_requests: int, rate: float, tuned: bool = True) -> dict:
+    """Configuration 4 on a context: a Poisson request stream over the three
+    resident geometries (20% coupled insertion validations), continuous
+    batching, latencies from arrival to the end of the completing round."""
     import twos
     from twos import serving
 
-    ctx = twos.Context()
-    stream = serving.Stream(ctx, twos, n_requests=args.requests, rate=args.rate)
+    stream = serving.Stream(ctx, twos, n_requests=n_requests, rate=rate)
     tuning = None
-    if not args.no_autotune:
+    if tuned:
         # The mesh requests run at the tuned launch shape and distance grid
         # (the committed choice for this mesh, eps and combine).
         from twos import autotune
@@ -338,42 +347,66 @@ def serve(args) -> int:
                                    mesh=stream.mesh, log=log, cache_path=TUNING_CACHE)
         stream.shapes["sh_icosphere"] = {k: v for k, v in tuning.choice.items()
                                          if k not in ("grid_cells", "grid_corners")}
-    # Slots and one round's walk buffer sized to a slice of HBM (admission by memory).
+    # Slots and one round's walk buffer; admission charges requests against the
+    # context's HBM budget beside the resident geometry.
     srv = ctx.server(max_sites=65536, max_walks_per_round=1 << 26)
-    log(f"serving {args.requests} requests at {args.rate}/s")
+    log(f"serving {n_requests} requests at {rate}/s")
     res = serving.run(stream, srv, log=log)
+    srv.close()
+    out = {"requests": n_requests, "rate_per_s": rate, "p50_ms": round(res["p50_ms"], 2),
+           "p99_ms": round(res["p99_ms"], 2), "mean_ms": round(res["mean_ms"], 2),
+           "td_sites_per_s": round(res["td_sites_per_s"], 2), "walk_steps_per_s": round(res["walk_steps_per_s"], 1),
+           "rounds": res["rounds"], "validations": res["validations"], "td_sites": res["td_sites"],
+           "failed": res["failed"], "wall_s": round(res["wall_s"], 3),
+           "busy_fraction": round(res["busy_s"] / res["wall_s"], 4)}
+    frozen = frozen_unit("sh_icosphere_81920")
+    if frozen is not None and res["busy_s"] > 0:
+        # The mesh requests' closest-point queries priced at the frozen exact
+        # query, over the time spent in server rounds (busy) and over the wall.
+        b = res["mesh_queries"] * frozen["frozen_bytes_per_query"]
+        out["roofline"] = {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                           "achieved_frozen_busy": round(b / res["busy_s"] / 1e9, 1),
+                           "frac_frozen_busy": round(b / res["busy_s"] / 1e9 / HBM_PEAK_GBS, 4),
+                           "frac_frozen_wall": round(b / res["wall_s"] / 1e9 / HBM_PEAK_GBS, 4),
+                           "mesh_queries": res["mesh_queries"],
+                           "basis": "mesh requests' queries x the frozen exact-query bytes (tests/golden/roofline_c3.json)"}
+    if tuning is not None:
+        out["mesh_launch_shape"] = tuning.choice
+    return out
+
+
+def serve(args) -> int:
+    import twos
+    ctx = twos.Context()
+    res = serve_run(ctx, args.requests, args.rate, tuned=not args.no_autotune)
+    ctx.close()
     out = {
         "metric": "serving p50/p99 latency (TD queries + insertion validations)",
-        "value": round(res["p50_ms"], 2), "unit": "ms", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "value": res["p50_ms"], "unit": "ms", "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": round(res["wall_s"] * 1e3, 1), "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 walks, f64 combine", "data": "synthetic",
         "config": {"workload": f"serve: {args.requests} Poisson requests at {args.rate}/s over unit ball, cooling "
-                               "block and SH icosphere; sites log-uniform 1-8192, CI U[1%,5%], 20% validations",
-                   "max_sites": 65536, "max_walks_per_round": 1 << 26},
-        "p50_ms": round(res["p50_ms"], 2), "p99_ms": round(res["p99_ms"], 2), "mean_ms": round(res["mean_ms"], 2),
-        "td_sites_per_s": round(res["td_sites_per_s"], 2), "walk_steps_per_s": round(res["walk_steps_per_s"], 1),
-        "rounds": res["rounds"], "validations": res["validations"], "td_sites": res["td_sites"],
-        "mesh_launch_shape": tuning.choice if tuning else None,
+                               "block and SH icosphere; sites log-uniform 1-8192, CI U[1%,5%], 20% coupled "
+                               "insertion validations", "max_sites": 65536, "max_walks_per_round": 1 << 26},
+        **res,
     }
-    srv.close()
-    ctx.close()
     print(json.dumps(out))
     return 0
 
 
-def design_bench(args) -> int:
-    """Sequential insertion design on the SH-icosphere mesh (a reduced
-    configuration 5): every round scores the feasible candidates (scout,
-    fixed walks per side), rescores the best 8 with 16,384 fresh walks per
-    side, accepts the best lower confidence bound and inserts a radius-0.02
-    cavity there.  The metric is candidates scored per second."""
+def design_run(ctx, rounds: int, candidates: int, scout_walks: int) -> dict:
+    """Sequential insertion design on the SH-icosphere mesh (configuration 5
+    per round): every round scores the feasible candidates (scout, fixed walks
+    per side), rescores the best 8 with 16,384 fresh walks per side, accepts
+    the best lower confidence bound and inserts a radius-0.02 cavity there;
+    then the first and last rounds' rescoring is rerun in binary64 on the same
+    Philox streams."""
     import twos
     from twos import autotune
     from twos import design as D
     from twos import workloads as W
 
-    w = W.sh_icosphere(n_sites=args.design_candidates)
-    ctx = twos.Context()
+    w = W.sh_icosphere(n_sites=candidates)
     mesh = ctx.mesh(w.vertices, w.faces)
     geom = ctx.geometry(twos.MESH, mesh=mesh)
     sides = W.sides(w, twos.side)
@@ -382,9 +415,8 @@ def design_bench(args) -> int:
                                cache_path=TUNING_CACHE)
     shape = {k: v for k, v in tuning.choice.items() if k not in ("grid_cells", "grid_corners")}
     est = D.GpuDesign(ctx, mesh, sides, radius=0.02, eps=w.eps, max_steps=w.max_steps, shape=shape)
-    cfg = D.GreedyConfig(rounds=args.design_rounds, radius=0.02, clearance=0.025,
-                         scout_walks=args.design_scout_walks, shortlist=8, rescore_walks=16384, seed=2026,
-                         select="lcb")
+    cfg = D.GreedyConfig(rounds=rounds, radius=0.02, clearance=0.025, scout_walks=scout_walks, shortlist=8,
+                         rescore_walks=16384, seed=2026, select="lcb")
     # Warm the engine and graphs on one small round, untimed.
     D.greedy(est, w.sites[:2048], D.GreedyConfig(rounds=1, radius=0.02, scout_walks=64, shortlist=2,
                                                  rescore_walks=256, seed=1))
@@ -393,8 +425,6 @@ def design_bench(args) -> int:
     out = D.greedy(est, w.sites, cfg, log=log)
     total = time.perf_counter() - t0
     scored = out.candidates_scored
-    # binary64 rerun of the first and last rounds' rescoring (the same Philox
-    # streams, cavities and grid threshold): the binary32 walks' error.
     est64 = D.GpuDesign(ctx, mesh, sides, radius=0.02, eps=w.eps, max_steps=w.max_steps, precision=64, shape=shape)
     rerun = []
     for k in sorted({0, len(out.rounds) - 1}):
@@ -406,22 +436,33 @@ def design_bench(args) -> int:
                       "max_abs_diff_in_se": float(np.max(np.abs(r.rescore_score - s64) / se64)),
                       "max_rel_diff": float(np.max(np.abs(r.rescore_score - s64) / np.abs(s64)))})
         log(f"[design] binary64 rerun of round {k}: {rerun[-1]}")
-    result = {
-        "metric": "design candidates scored per second (scout + rescore)", "value": round(scored / total, 2),
-        "unit": "candidates/s", "n_gpus": 1, "steps": args.design_rounds, "warmup": 1,
-        "ms_per_step": round(1e3 * total / args.design_rounds, 1), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32 walks, f64 combine", "data": "synthetic",
-        "config": {"workload": f"design on {w.name}: {len(w.sites)} candidates, {args.design_rounds} rounds, scout "
-                               f"{args.design_scout_walks} walks/side, top 8 rescored with 16384 fresh walks/side, "
-                               "LCB acceptance, radius-0.02 cavities"},
-        "walk_steps_per_s": round(est.steps / total, 1), "paths": est.paths,
-        "rounds": [{"feasible": len(r.feasible), "chosen": np.round(r.chosen, 6).tolist(),
-                    "score": float(r.rescore_score[int(np.argmax(r.rescore_score - 1.96 * r.rescore_se))]),
-                    "seconds": round(r.seconds, 3)} for r in out.rounds],
-        "binary64_rerun": rerun,
-        "launch_shape": tuning.choice,
-    }
+    ctx.release_geometry(geom)
+    ctx.release_mesh(mesh)
+    return {"candidates_per_s": round(scored / total, 2), "rounds": rounds, "candidates": len(w.sites),
+            "scout_walks_per_side": scout_walks, "rescore_walks_per_side": 16384, "seconds": round(total, 3),
+            "ms_per_round": round(1e3 * total / rounds, 1), "walk_steps_per_s": round(est.steps / total, 1),
+            "paths": est.paths,
+            "per_round": [{"feasible": len(r.feasible), "chosen": np.round(r.chosen, 6).tolist(),
+                           "score": float(r.rescore_score[int(np.argmax(r.rescore_score - 1.96 * r.rescore_se))]),
+                           "seconds": round(r.seconds, 3)} for r in out.rounds],
+            "binary64_rerun": rerun, "launch_shape": tuning.choice}
+
+
+def design_bench(args) -> int:
+    import twos
+    ctx = twos.Context()
+    res = design_run(ctx, args.design_rounds, args.design_candidates, args.design_scout_walks)
     ctx.close()
+    result = {
+        "metric": "design candidates scored per second (scout + rescore)", "value": res["candidates_per_s"],
+        "unit": "candidates/s", "n_gpus": 1, "steps": args.design_rounds, "warmup": 1,
+        "ms_per_step": res["ms_per_round"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 walks, f64 combine", "data": "synthetic",
+        "config": {"workload": f"design on sh_icosphere_81920: {res['candidates']} candidates, {args.design_rounds} "
+                               f"rounds, scout {args.design_scout_walks} walks/side, top 8 rescored with 16384 "
+                               "fresh walks/side, LCB acceptance, radius-0.02 cavities"},
+        **res,
+    }
     print(json.dumps(result))
     return 0
 

@@ -256,8 +256,9 @@ twos_status twos_mesh_info(twos_ctx *ctx, int32_t mesh_id, int32_t *depth, int64
  * the distance at the cell corners and bounds a step by the best of a cell's
  * eight corners; corners 2 stores those eight values per cell rounded down to
  * binary16 in one 16-byte record (one load per lookup, four times the
- * memory).  No counterpart in the reference (Embree has no
- * distance grid); it is a tuning control for TWOS_FLAG_DISTANCE_GRID. */
+ * memory).  Asking for the grid the mesh already has returns at once with
+ * *build_ms = 0.  No counterpart in the reference (Embree has no distance
+ * grid); it is a tuning control for TWOS_FLAG_DISTANCE_GRID. */
 twos_status twos_mesh_set_distance_grid(twos_ctx *ctx, int32_t mesh_id, int32_t cells, int32_t corners,
                                         double *build_ms);
 twos_status twos_mesh_release(twos_ctx *ctx, int32_t mesh_id);

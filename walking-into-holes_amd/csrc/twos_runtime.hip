@@ -1472,6 +1472,10 @@ twos_status twos_mesh_set_distance_grid(twos_ctx *ctx, int32_t mesh_id, int32_t 
   auto it = ctx->meshes.find(mesh_id);
   if (it == ctx->meshes.end()) return fail(TWOS_ERR_NOT_FOUND, "unknown mesh id");
   MeshEntry &m = it->second;
+  if (m.grid_cells == cells && m.mb.grid_corners == corners && m.mb.grid) {
+    if (build_ms) *build_ms = 0.0;  // that grid is resident already
+    return TWOS_OK;
+  }
   HIP_OK(hipSetDevice(ctx->device));
   // Work already queued may read the old grid: finish it before the swap.
   HIP_OK(hipStreamSynchronize(ctx->stream));

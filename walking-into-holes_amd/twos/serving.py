@@ -157,6 +157,8 @@ def run(stream: Stream, srv, time_scale: float = 1.0, distance_grid: bool = True
     sites_total = 0
     rounds = 0
     failed = 0
+    busy_s = 0.0                      # host time inside server rounds
+    mesh_queries = 0                  # closest-point queries of mesh requests (steps + paths)
     while pending or inflight:
         now = time.perf_counter() - t0
         while pending and pending[0].arrival * time_scale <= now:
@@ -168,7 +170,9 @@ def run(stream: Stream, srv, time_scale: float = 1.0, distance_grid: bool = True
         if not inflight:
             time.sleep(max(0.0, pending[0].arrival * time_scale - now))
             continue
+        t_step = time.perf_counter()
         done = srv.step()
+        busy_s += time.perf_counter() - t_step
         rounds += 1
         t_end = time.perf_counter() - t0
         for sid, res in done.items():
@@ -177,6 +181,8 @@ def run(stream: Stream, srv, time_scale: float = 1.0, distance_grid: bool = True
             req.done += 1
             steps_total += int(res.steps.sum())
             failed += res.stats.get("status", 0) != 0
+            if stream.geoms[req.geom_name][0].container == stream.twos.MESH:
+                mesh_queries += int(res.steps.sum()) + 2 * int(res.walks.sum())
             if req.done == len(req.ids):
                 req.finish = t_end
                 inflight.pop(req.rid)
@@ -194,7 +200,7 @@ def run(stream: Stream, srv, time_scale: float = 1.0, distance_grid: bool = True
             "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
             "mean_ms": float(lat.mean() * 1e3), "td_sites_at_target": sites_at_target,
             "td_sites": sites_total, "td_sites_per_s": sites_at_target / wall, "walk_steps_per_s": steps_total / wall,
-            "walk_steps": steps_total, "failed": failed,
+            "walk_steps": steps_total, "failed": failed, "busy_s": busy_s, "mesh_queries": mesh_queries,
             "validations": sum(r.kind == "validation" for r in stream.requests)}
 
 
