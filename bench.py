@@ -237,6 +237,11 @@ def main() -> int:
                     "tris_per_query": round(n_tris, 2), "grid_lookups_per_query": round(n_grid, 3),
                     "traversing_query_fraction": round(exact, 3),
                     "nodes_per_traversing_query": round(n_nodes / max(exact, 1e-9), 2),
+                    # Visits to the top three levels of the 4-wide tree (21
+                    # nodes), the share a per-block LDS copy of them could serve.
+                    "top_level_nodes_per_query": round(cnt.stats["top_nodes_visited"] / q_sub, 4),
+                    "top_level_node_fraction": round(cnt.stats["top_nodes_visited"]
+                                                     / max(cnt.stats["nodes_visited"], 1), 4),
                     "walk_kernel_ms_per_launch": round(eager_ms / launches, 3),
                     "launches": launches, "traffic_source": traffic_source,
                     "traffic_x_algorithmic": (round(traffic / (queries * b_query / launches), 3)

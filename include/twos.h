@@ -235,6 +235,7 @@ typedef struct {
   int32_t grid;
   int64_t grid_lookups;     /* distance-grid lookups (when counting is on) */
   int64_t grid_hits;        /* lookups whose bound was the step's radius, no traversal (counting) */
+  int64_t top_nodes_visited; /* visits to the top three levels of the 4-wide BVH (counting) */
 } twos_stats;
 
 const char *twos_version(void);

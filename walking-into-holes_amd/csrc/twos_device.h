@@ -332,7 +332,13 @@ struct TraversalCount {
   uint32_t tris;
   uint32_t grid;  // distance-grid lookups
   uint32_t hits;  // lookups whose bound was taken as the radius (no traversal)
+  uint32_t top;   // visits to nodes in the top kTopLevels levels of the 4-wide tree
 };
+
+// Counting launches report visits to the top three 4-wide levels (1 + 4 + 16
+// nodes): the nodes a per-block copy in LDS could serve.
+constexpr int kTopLevels = 3;
+constexpr int32_t kCellNodeDepth = 1 << 20;  // depth recorded for per-cell nodes
 
 // 4-wide node, 128 bytes, eight float4: the four children's bounds as
 // lo.x, lo.y, lo.z, hi.x, hi.y, hi.z rows (one child per lane of the float4),
@@ -812,7 +818,10 @@ __device__ __forceinline__ bool mq_visit(const MeshView &m, V3<R> p, MeshQuery<R
   const float4 *nq = m.nodes4 + size_t(kNode4Quads) * q.node;
   const float4 lx = nq[0], ly = nq[1], lz = nq[2], hx = nq[3], hy = nq[4], hz = nq[5];
   const int4 ch = *reinterpret_cast<const int4 *>(nq + 6);
-  if (kCount) cnt->nodes += 1;
+  if (kCount) {
+    cnt->nodes += 1;
+    cnt->top += __float_as_int(nq[7].z) < kTopLevels ? 1u : 0u;
+  }
   R d[4] = {box_d2<R>(p, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x), box_d2<R>(p, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y),
             box_d2<R>(p, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z), box_d2<R>(p, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w)};
   int32_t c[4] = {ch.x, ch.y, ch.z, ch.w};

@@ -303,8 +303,9 @@ __global__ void reorder_tris(const double *vtx, const uint32_t *faces, const uin
 
 // 4-wide collapse: binary nodes at even depth become 4-wide nodes whose
 // children are the grandchildren through each internal child (a leaf child
-// stays a leaf).  flag[i] = 1 for even depth; an exclusive scan of the flags
-// numbers the 4-wide nodes, the root staying 0.
+// stays a leaf).  flag[i] = 1 + depth / 2 for even depth (the 4-wide depth
+// plus one), else 0; an exclusive scan of map[i] = (flag[i] != 0) numbers the
+// 4-wide nodes, the root staying 0.
 __global__ void even_depth(const float4 *nodes, int32_t n_internal, uint32_t *flag, uint32_t *map) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_internal) return;
@@ -314,8 +315,8 @@ __global__ void even_depth(const float4 *nodes, int32_t n_internal, uint32_t *fl
     ++depth;
     p = __float_as_int(nodes[4 * size_t(p) + 3].z);
   }
-  flag[i] = (depth & 1) == 0 ? 1u : 0u;
-  map[i] = flag[i];
+  flag[i] = (depth & 1) == 0 ? 1u + uint32_t(depth / 2) : 0u;
+  map[i] = flag[i] != 0 ? 1u : 0u;
 }
 
 __global__ void collapse4(const float4 *nodes, const uint32_t *flag, const uint32_t *map, const int2 *range,
@@ -353,8 +354,10 @@ __global__ void collapse4(const float4 *nodes, const uint32_t *flag, const uint3
   o[5] = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
   o[6] = make_float4(__int_as_float(ch[0]), __int_as_float(ch[1]), __int_as_float(ch[2]), __int_as_float(ch[3]));
   // Row 7: the node's leaf range (first, last sorted triangle), for the
-  // start-node descent of record grids.
-  o[7] = make_float4(__int_as_float(range[i].x), __int_as_float(range[i].y), 0.f, 0.f);
+  // start-node descent of record grids, and its depth in the 4-wide tree
+  // (read by counting launches only).
+  o[7] = make_float4(__int_as_float(range[i].x), __int_as_float(range[i].y), __int_as_float(int32_t(flag[i]) - 1),
+                     0.f);
 }
 
 __global__ void leaf_depth(const float4 *nodes, const int32_t *leaf_parent, int32_t n, int32_t *max_depth) {
@@ -504,8 +507,9 @@ __device__ void write_cell_node(const MeshView &m, const int32_t cand[4], int32_
   o[4] = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
   o[5] = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
   o[6] = make_float4(__int_as_float(ch[0]), __int_as_float(ch[1]), __int_as_float(ch[2]), __int_as_float(ch[3]));
-  // Not a tree node: an empty leaf range keeps the start-node descent out.
-  o[7] = make_float4(__int_as_float(-1), __int_as_float(-2), 0.f, 0.f);
+  // Not a tree node: an empty leaf range keeps the start-node descent out;
+  // its depth is below any tree level.
+  o[7] = make_float4(__int_as_float(-1), __int_as_float(-2), __int_as_float(kCellNodeDepth), 0.f);
 }
 
 template <int kPass>

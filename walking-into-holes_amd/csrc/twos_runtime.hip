@@ -77,6 +77,7 @@ double now_s() {
 }
 
 constexpr int kMaxClasses = 8;
+constexpr int kCounters = 5;  // traversal counters of a counting launch (nodes, tris, grid lookups, hits, top)
 // Distance grid: cells along the longest bounding-box axis, and the radius
 // threshold (in cell sizes) above which a mesh step takes the grid bound.
 constexpr int kGridCells = 128;
@@ -184,7 +185,7 @@ struct Engine {
   DevSide *d_sides = nullptr;
   double *d_val = nullptr;
   int32_t *d_steps = nullptr;
-  unsigned long long *d_counters = nullptr;  // [kMaxClasses] tickets, then nodes, tris, grid lookups
+  unsigned long long *d_counters = nullptr;  // [kMaxClasses] tickets, then nodes, tris, grid lookups, grid hits, top-level nodes
   // Round input block (one H2D copy) and output block (one D2H copy).
   char *d_in = nullptr, *h_in = nullptr, *d_out = nullptr, *h_out = nullptr;
   // One-site work list of the plain schedule: 64 B in, 96 B out.
@@ -290,7 +291,7 @@ struct Engine {
     HIP_OK(dalloc(&d_sides, 2 * size_t(cap_problems)));
     HIP_OK(dalloc(&d_val, 2 * size_t(cap_walks)));
     HIP_OK(dalloc(&d_steps, 2 * size_t(cap_walks)));
-    HIP_OK(dalloc(&d_counters, kMaxClasses + 4));
+    HIP_OK(dalloc(&d_counters, kMaxClasses + kCounters));
     block_offsets();
     HIP_OK(dalloc(&d_in, in_bytes));
     HIP_OK(dalloc(&d_out, out_bytes));
@@ -363,7 +364,7 @@ struct Engine {
     carve(s.acc_steps, C); carve(s.acc_caps, C); carve(s.result, C); carve(s.hint0, C); carve(s.grid_kappa, C);
     s.acc_grad = nullptr;
     carve(d_sides, 2);
-    carve(d_counters, kMaxClasses + 4);
+    carve(d_counters, kMaxClasses + kCounters);
     block_offsets();
     carve(d_in, in_bytes);
     carve(d_out, out_bytes);
@@ -815,7 +816,7 @@ struct Engine {
         int kind = geom_kind_of(rc->geom);
         for (const Entry &en : by_class[c])
           if (geom_kind_of(en.r->geom) != kind) kind = kGeomGeneric;
-        if (rc->prm.flags & TWOS_FLAG_COUNT_TRAVERSAL) HIP_OK(launch_zero(d_counters + kMaxClasses, 4, strm));
+        if (rc->prm.flags & TWOS_FLAG_COUNT_TRAVERSAL) HIP_OK(launch_zero(d_counters + kMaxClasses, kCounters, strm));
         TWOS_TRY(ev_record(0, strm));
         st = launch_walks(rc, int(c), e0s[c], totals[c], kind, class_bare(by_class[c]));
         if (st != TWOS_OK) return st;
@@ -836,12 +837,13 @@ struct Engine {
         stats.walk_kernel_ms += ev_ms(0, 1);
         stats.reduce_kernel_ms += ev_ms(1, 2);
         if (rc->prm.flags & TWOS_FLAG_COUNT_TRAVERSAL) {
-          unsigned long long cnt[4];
+          unsigned long long cnt[kCounters];
           HIP_OK(hipMemcpy(cnt, d_counters + kMaxClasses, sizeof(cnt), hipMemcpyDeviceToHost));
           stats.nodes_visited += int64_t(cnt[0]);
           stats.tris_tested += int64_t(cnt[1]);
           stats.grid_lookups += int64_t(cnt[2]);
           stats.grid_hits += int64_t(cnt[3]);
+          stats.top_nodes_visited += int64_t(cnt[4]);
         }
       }
       HIP_OK(hipMemcpyAsync(h_out + off_compact, d_out + off_compact, sizeof(twos_site_result) * n_entries,

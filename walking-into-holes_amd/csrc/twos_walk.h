@@ -513,7 +513,7 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWav
   int64_t chunk_cur = 0, chunk_end = 0;  // wave-uniform
   int32_t chunk_site = 0;                // wave-uniform: the site of the chunk's first ticket
   bool drained = false;                  // wave-uniform
-  TraversalCount cnt{0, 0, 0, 0};
+  TraversalCount cnt{0, 0, 0, 0, 0};
   while (true) {
     uint64_t need = __ballot(!active);
     if (__popcll(need) < wp.refill && need != ~0ull) need = 0;  // wave-uniform
@@ -601,6 +601,7 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWav
     atomicAdd(wp.counters + 1, (unsigned long long)cnt.tris);
     atomicAdd(wp.counters + 2, (unsigned long long)cnt.grid);
     atomicAdd(wp.counters + 3, (unsigned long long)cnt.hits);
+    atomicAdd(wp.counters + 4, (unsigned long long)cnt.top);
   }
 }
 

@@ -84,7 +84,8 @@ class Stats(C.Structure):
                 ("walk_launches", C.c_int64), ("steps", C.c_int64), ("paths", C.c_int64),
                 ("nodes_visited", C.c_int64), ("tris_tested", C.c_int64), ("rounds", C.c_int32),
                 ("block", C.c_int32), ("waves_per_simd", C.c_int32), ("grid", C.c_int32),
-                ("grid_lookups", C.c_int64), ("grid_hits", C.c_int64)]
+                ("grid_lookups", C.c_int64), ("grid_hits", C.c_int64),
+                ("top_nodes_visited", C.c_int64)]
 
 
 _ctx_p = C.c_void_p
