@@ -175,7 +175,9 @@ typedef struct {
   int32_t step_batch;   /* binary32 persistent kernel: 0, or 24, 32, 40 -- a lane
                            whose query has ended takes its walk step once that
                            many lanes of its wave are ready (1: at least as many
-                           as are still traversing), or none traverses;
+                           as are still traversing, and traversing lanes take
+                           their next node visit once they outnumber the ready
+                           ones or none is ready), or none traverses;
                            lds_stack 24 and waves_per_simd 0 or 5 only.  No
                            result depends on it. */
   int32_t refill_batch; /* persistent kernels: the idle lanes of a wave take new

@@ -491,7 +491,11 @@ __device__ __forceinline__ bool query_start(Job<R> &j, MeshQuery<R> &q, Traversa
 // query has ended waits (one node visit of the others at a time) until at
 // least kBatch lanes of its wave are ready (kBatch = 1: at least as many as are
 // still traversing) or none is traversing, so the wave runs the step code less
-// often; which lane takes which step, and every path, is unchanged.
+// often.  kBatch = 1 batches the node visits the same way: traversing lanes
+// wait while ready lanes step, until they outnumber the ready lanes (or none
+// is ready), so most iterations run either the step code or the visit code
+// with the larger group, not both with a few lanes each.  Which lane takes
+// which step, and every path, is unchanged.
 template <typename R, int GK, bool kCount, int kWaves, int kLds, int kChan = kChanNone, int kBatch = 0>
 __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWaves))) void walk_persistent(
     Tables tb, WalkParams wp) {
@@ -549,8 +553,14 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWav
       need = __ballot(!active);
     }
     if (!__any(active)) break;
+    bool visit = true;  // wave-uniform
+    if constexpr (kBatch == 1) {
+      const int busy = __popcll(__ballot(active && in_query));
+      const int ready = __popcll(__ballot(active && !in_query));
+      visit = ready == 0 || busy > ready;
+    }
     if (active) {
-      if (in_query) in_query = !mq_visit<R, kCount, kLds>(j.mv, j.x, q, stack, stride, overflow, &cnt);
+      if (in_query && visit) in_query = !mq_visit<R, kCount, kLds>(j.mv, j.x, q, stack, stride, overflow, &cnt);
       bool go = !in_query;
       if constexpr (kBatch > 0) {
         // Wave-uniform: both masks are ballots over the active lanes.

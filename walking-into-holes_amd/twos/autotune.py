@@ -9,7 +9,9 @@ batch only):
   lds_stack       16, 24 or 32 LDS traversal-stack entries per lane (16 and
                   32 exist for waves_per_simd 0 and 5)
   step_batch      0 (a lane takes its walk step as soon as its query ends),
-                  1 (once at least as many lanes are ready as traverse) or 32
+                  1 (once at least as many lanes are ready as traverse; the
+                  node visits wait until the traversing lanes outnumber the
+                  ready ones) or 32
                   (once 32 lanes are ready); built for lds_stack 24 with
                   waves_per_simd 0 or 5
   refill_batch    1, 4, 8, 16 or 32: a wave's idle lanes take new paths once
