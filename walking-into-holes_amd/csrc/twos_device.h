@@ -624,8 +624,14 @@ __device__ __forceinline__ R grid_upper(R v, R s) {
   return (v * R(1.00390625) + s) * R(1.00001) + R(1e-6);
 }
 
-template <typename R>
-__device__ __forceinline__ R grid_lower(const DevGeom &g, V3<R> p, R *upper = nullptr) {
+struct NoShadow {
+  __device__ void operator()() const {}
+};
+
+// shadow() runs after the grid sample loads are issued and before their
+// values are used, so independent work there overlaps the loads' latency.
+template <typename R, class Shadow = NoShadow>
+__device__ __forceinline__ R grid_lower(const DevGeom &g, V3<R> p, R *upper = nullptr, Shadow shadow = {}) {
   const R h = R(g.grid_h), inv_h = R(g.grid_inv_h);
   const int last = g.grid_corners == 1 ? 2 : 1;  // highest first index of a cell (per sample count)
   int i = int(floor((p.x - R(g.grid_lo[0])) * inv_h));
@@ -638,6 +644,7 @@ __device__ __forceinline__ R grid_lower(const DevGeom &g, V3<R> p, R *upper = nu
     const V3<R> c{R(g.grid_lo[0]) + (R(i) + R(0.5)) * h, R(g.grid_lo[1]) + (R(k) + R(0.5)) * h,
                   R(g.grid_lo[2]) + (R(l) + R(0.5)) * h};
     const R dc = R(g.grid[(size_t(i) * g.grid_n[1] + k) * g.grid_n[2] + l]);
+    shadow();
     const V3<R> d = p - c;
     const R dist = wsqrt(dot(d, d));
     if (upper) *upper = grid_upper(dc, dist);
@@ -647,6 +654,7 @@ __device__ __forceinline__ R grid_lower(const DevGeom &g, V3<R> p, R *upper = nu
           z0 = p.z - (R(g.grid_lo[2]) + R(l) * h);
   if (g.grid_corners == 2) {
     const uint4 rec = reinterpret_cast<const uint4 *>(g.grid)[(size_t(i) * g.grid_n[1] + k) * g.grid_n[2] + l];
+    shadow();
     const uint32_t w[4] = {rec.x, rec.y, rec.z, rec.w};
     R best = -Inf<R>::v(), up = Inf<R>::v();
 #pragma unroll
@@ -667,6 +675,7 @@ __device__ __forceinline__ R grid_lower(const DevGeom &g, V3<R> p, R *upper = nu
     return best - R(2e-6);
   }
   const float *row = g.grid + (size_t(i) * g.grid_n[1] + k) * g.grid_n[2] + l;
+  shadow();
   const size_t si = size_t(g.grid_n[1]) * g.grid_n[2], sk = size_t(g.grid_n[2]);
   R best = -Inf<R>::v(), up = Inf<R>::v();
 #pragma unroll

@@ -54,6 +54,9 @@ struct Job {
   const DevSide *s0;
   const DevSide *s1;
   MeshView mv;  // the geometry's mesh buffers, kept per lane (unused fields fold away)
+  // (kPipe kernels) the direction of the path's next step, drawn while the
+  // distance-grid sample of its current position is in flight.
+  V3<R> dn;
   // Gradient channel (kernels built with kChanGrad only): the first sphere's
   // 3 omega0 / r0, its radius (0 until the first step) and the path values
   // after the first step's source term.
@@ -416,7 +419,7 @@ constexpr size_t persistent_lds_bytes(int lds, int block) { return size_t(lds) *
 // class in j.hint (neither is needed any more) and job_retire adds the payoff
 // later, so the payoff code runs in the wave's batched refill instead of in
 // the walk step.
-template <typename R, int kChan = kChanNone, bool kDefer = false>
+template <typename R, int kChan = kChanNone, bool kDefer = false, bool kPipe = false>
 __device__ __forceinline__ int job_advance(Job<R> &j, const WalkParams &wp, R r, V3<R> bp, int cls) {
   if (!isfinite(r)) return 2;  // as in job_step: a failed query ends the path capped
   const DevSide &s0 = *j.s0;
@@ -432,7 +435,9 @@ __device__ __forceinline__ int job_advance(Job<R> &j, const WalkParams &wp, R r,
   }
   if constexpr (kChan != kChanBare)
     job_source(j, wp, r, job_sampled(j, wp) ? j.x + philox_green<R>(j.key, j.walk, j.steps, r) : j.x);
-  const V3<R> d = philox_dir(j);
+  V3<R> d;
+  if constexpr (kPipe) d = j.dn;  // drawn when this step's query started
+  else d = philox_dir(j);
   job_first_step<R, kChan>(j, d, r);
   j.x = j.x + d * r;
   ++j.steps;
@@ -456,9 +461,14 @@ __device__ __forceinline__ bool is_mesh(const Job<R> &j) {
 }
 
 // Starts the query at the path's position; true while a traversal is pending.
-template <typename R, int GK, bool kCount>
+// kPipe: also draws the direction of the path's next step (j.dn), between
+// issuing the distance-grid load and using it, so the draw overlaps the load.
+template <typename R, int GK, bool kCount, bool kPipe = false>
 __device__ __forceinline__ bool query_start(Job<R> &j, MeshQuery<R> &q, TraversalCount *cnt) {
-  if (GK == kGeomAnalytic || !is_mesh<R, GK>(j)) return false;
+  if (GK == kGeomAnalytic || !is_mesh<R, GK>(j)) {
+    if constexpr (kPipe) j.dn = philox_dir(j);
+    return false;
+  }
   if (!(isfinite(j.x.x) && isfinite(j.x.y) && isfinite(j.x.z))) {
     q.grid = false;
     q.best = Inf<R>::v();  // no query: the path ends capped
@@ -468,7 +478,9 @@ __device__ __forceinline__ bool query_start(Job<R> &j, MeshQuery<R> &q, Traversa
   int32_t start = 0;
   if (j.tau < Inf<R>::v()) {
     R ub;
-    const R lb = grid_lower<R>(*j.g, j.x, &ub);
+    R lb;
+    if constexpr (kPipe) lb = grid_lower<R>(*j.g, j.x, &ub, [&j]() { j.dn = philox_dir(j); });
+    else lb = grid_lower<R>(*j.g, j.x, &ub);
     if (kCount) cnt->grid += 1;
     if (lb >= j.tau) {
       if (kCount) cnt->hits += 1;
@@ -480,6 +492,8 @@ __device__ __forceinline__ bool query_start(Job<R> &j, MeshQuery<R> &q, Traversa
     // below the root.
     init2 = ub < Inf<R>::v() ? ub * ub : ub;
     start = grid_start_node<R>(*j.g, j.x, init2);
+  } else if constexpr (kPipe) {
+    j.dn = philox_dir(j);
   }
   return !mq_begin<R, kCount>(j.mv, j.x, q, j.hint, cnt, init2, start);
 }
@@ -502,6 +516,9 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWav
   extern __shared__ uint32_t lds_stack[];
   const uint32_t stride = blockDim.x;
   uint32_t *stack = lds_stack + threadIdx.x;
+  // binary32 walks without the gradient or coupled channels draw each step's
+  // direction in the shadow of the previous query's grid load.
+  constexpr bool kPipe = sizeof(R) == 4 && (kChan == kChanNone || kChan == kChanBare);
   uint32_t overflow[kStackDepth - kLds];
   const int lane = int(threadIdx.x & 63);
   const int64_t total = *tb.round.total_jobs;
@@ -589,7 +606,7 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWav
           }
           r = holes_and_cavities<R, GK>(*j.g, j.x, r, bp, cls, j.n_cav);
           if constexpr (kChan == kChanCoupled) r = coupled_cavity(j, r, bp, cls);
-          const int st = job_advance<R, kChan, kDefer>(j, wp, r, bp, cls);
+          const int st = job_advance<R, kChan, kDefer, kPipe>(j, wp, r, bp, cls);
           if (st == 0) {
             start = true;
           } else if (kChan == kChanCoupled && j.phase == 0) {
@@ -602,7 +619,7 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWav
           }
         }
         fresh = false;
-        if (start) in_query = query_start<R, GK, kCount>(j, q, &cnt);
+        if (start) in_query = query_start<R, GK, kCount, kPipe>(j, q, &cnt);
       }
     }
   }
