@@ -77,6 +77,7 @@ double now_s() {
 }
 
 constexpr int kMaxClasses = 8;
+constexpr int64_t kWalkBlockSites = 64;  // requests of at least twice this many sites are listed in Morton order
 constexpr int kCounters = 5;  // traversal counters of a counting launch (nodes, tris, grid lookups, hits, top)
 // Distance grid: cells along the longest bounding-box axis, and the radius
 // threshold (in cell sizes) above which a mesh step takes the grid bound.
@@ -172,7 +173,43 @@ struct Request {
   bool capped = false;          // a path of some site hit max_steps
   const double *d_sites = nullptr;  // device [n][3] positions (device entry points), else `sites`
   int64_t steps_sum = 0, paths_sum = 0;  // over the request's finished sites
+  // The order in which a round lists the sites: Morton order of their
+  // positions (empty: index order), so the walks that run at the same time
+  // start near each other and share cache lines of the distance grid and the
+  // tree.  A site's result does not depend on it.
+  std::vector<int32_t> order;
 };
+
+// Morton order of n points (10 bits per axis over their bounding box).
+std::vector<int32_t> morton_order(const double *p, int64_t n) {
+  double lo[3] = {p[0], p[1], p[2]}, hi[3] = {p[0], p[1], p[2]};
+  for (int64_t i = 1; i < n; ++i)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(lo[a], p[3 * i + a]);
+      hi[a] = std::max(hi[a], p[3 * i + a]);
+    }
+  const double ext = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-300});
+  auto spread = [](uint64_t v) {  // 10 bits -> every third bit
+    v = (v | (v << 16)) & 0x030000FFULL;
+    v = (v | (v << 8)) & 0x0300F00FULL;
+    v = (v | (v << 4)) & 0x030C30C3ULL;
+    v = (v | (v << 2)) & 0x09249249ULL;
+    return v;
+  };
+  std::vector<uint64_t> key(static_cast<size_t>(n));
+  for (int64_t i = 0; i < n; ++i) {
+    uint64_t code = 0;
+    for (int a = 0; a < 3; ++a) {
+      const double t = (p[3 * i + a] - lo[a]) / ext * 1023.0;
+      code |= spread(uint64_t(std::min(1023.0, std::max(0.0, t)))) << a;
+    }
+    key[size_t(i)] = code << 32 | uint64_t(i);
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<int32_t> order(static_cast<size_t>(n));
+  for (int64_t i = 0; i < n; ++i) order[size_t(i)] = int32_t(key[size_t(i)] & 0xFFFFFFFFULL);
+  return order;
+}
 
 struct Engine {
   twos_ctx *ctx = nullptr;
@@ -664,7 +701,8 @@ struct Engine {
       int c = -1;
       for (size_t k = 0; k < class_key.size(); ++k)
         if (class_key[k] == key) c = int(k);
-      for (int64_t i = 0; i < r->n; ++i) {
+      for (int64_t k = 0; k < r->n; ++k) {
+        const int64_t i = r->order.empty() ? k : r->order[size_t(k)];
         if (r->state[i]) continue;
         const int32_t inc = next_inc(r, i);
         if (inc > budget) { full = true; break; }
@@ -1031,6 +1069,7 @@ Request *make_request(twos_ctx *ctx, int32_t geom, const twos_side sides[2], con
   r->sides[1] = sides[1];
   if (d_sites) r->d_sites = d_sites;
   else r->sites.assign(sites, sites + 3 * n);
+  if (!d_sites && n >= 2 * kWalkBlockSites) r->order = morton_order(sites, n);
   r->n = n;
   r->out = out;
   return r;
