@@ -638,14 +638,30 @@ def measured_traffic(workload: str, n_sites: int, shape: dict, launches: int):
     return int(float(fields["traffic_bytes_per_launch"]))
 
 
+def _diff_z(a, b):
+    """z-scores of two Differential WoS results' derivative columns in their
+    combined SE (entries with a positive SE)."""
+    zs = []
+    for col, se in ((0, 3), (1, 4), (2, 5), (6, 7)):
+        s_ = np.hypot(a[..., se], b[..., se])
+        ok = s_ > 0
+        zs.append((a[..., col] - b[..., col])[ok] / s_[ok])
+    return np.concatenate(zs)
+
+
 def f4_bench(args) -> int:
-    """The adjacent estimators on the SH-icosphere mesh (81,920 triangles),
-    binary64 on the reference's mt19937_64 streams: Differential WoS
-    (twos_differential; 1,024 sites x 1,024 samples, four cavities of radius
-    0.05, the primal data g = x + 0.3 y^2, offset 0.004, eps 1e-4) or
-    certified minimum-benefit sampling (twos_certify; 16 sites x 65,536
-    samples, the affine source, radius cap 0.1, value cap 0.06, eps 1e-6, 128
-    steps).  A step is one call; the metric is samples per second."""
+    """The adjacent estimators on the SH-icosphere mesh (81,920 triangles):
+    Differential WoS (twos_differential; 1,024 sites x 1,024 samples, four
+    cavities of radius 0.05, the primal data g = x + 0.3 y^2, offset 0.004,
+    eps 1e-4) or certified minimum-benefit sampling (twos_certify; 16 sites x
+    65,536 samples, the affine source, radius cap 0.1, value cap 0.06, eps
+    1e-6, 128 steps; binary64 on the reference's mt19937_64 streams).  A step
+    is one call; the metric is samples per second.  Differential WoS is timed
+    in its production form (Philox streams, binary32 walks, the mesh's 1024-cell
+    record grid for steps at least 2 eps clear of the mesh); its replay form
+    (binary64, the reference's mt19937_64 streams) and the other production
+    variants are timed once each beside it, with the production result's
+    agreement with the exact binary64 Philox walk in combined SEs."""
     import twos
     from twos import workloads as W
 
@@ -655,13 +671,36 @@ def f4_bench(args) -> int:
     c = w.sites[:4]
     diff = args.workload == "diffwos"
     geom = ctx.geometry(twos.MESH, mesh=mesh, cavities=c, cavity_radius=0.05)
+    forms = {}
     if diff:
         sites = w.sites[16:16 + 1024]
         sites = sites[np.min(np.linalg.norm(sites[:, None] - c[None], axis=2), axis=1) > 0.06]
         side = W.sides(w, twos.side)[0]
         walks = 1024
-        call = lambda: ctx.differential(geom, sites, side, seed=11, walks=walks, eps=1e-4, max_steps=2048,
-                                        offset=0.004)
+
+        def run(**kw):
+            return ctx.differential(geom, sites, side, seed=11, walks=walks, eps=1e-4, max_steps=2048,
+                                    offset=0.004, **kw)
+
+        def once(name, **kw):
+            run(**kw)
+            t0 = time.perf_counter()
+            out = run(**kw)
+            dt = time.perf_counter() - t0
+            st = int(out[1][:, 0].sum() + out[1][:, 1].sum())
+            forms[name] = {"samples_per_s": round(len(sites) * walks / dt, 1), "walk_steps_per_s": round(st / dt, 1),
+                           "ms": round(1e3 * dt, 2)}
+            log(f"{name}: {dt:.3f} s, {st} steps")
+            return out
+
+        once("replay_f64_mt19937")
+        exact = once("philox_f64", rng=twos.RNG_PHILOX, precision=64)
+        once("philox_f32", rng=twos.RNG_PHILOX, precision=32)
+        once("philox_f32_grid128_centres", rng=twos.RNG_PHILOX, precision=32, distance_grid=True)
+        grid_ms = ctx.set_distance_grid(mesh, 1024, 2)
+        log(f"1024-cell record grid built in {grid_ms:.0f} ms")
+        prod = dict(rng=twos.RNG_PHILOX, precision=32, distance_grid=True, grid_kappa=0.0625)
+        call = lambda: run(**prod)
     else:
         sites = w.sites[4:16]
         sites = sites[np.min(np.linalg.norm(sites[:, None] - c[None], axis=2), axis=1) > 0.06]
@@ -689,6 +728,15 @@ def f4_bench(args) -> int:
                                 if diff else f"certify on {w.name}: {len(sites)} sites x {walks} samples")},
         "walk_steps_per_s": round(steps / t, 1),
     }
+    if diff:
+        z = _diff_z(out[0], exact[0])
+        result["dtype"] = "f32 walks, f64 fold"
+        result["config"]["form"] = "Philox streams, binary32, 1024-cell record grid, threshold 2 eps"
+        result["forms"] = forms
+        result["x_replay"] = round(result["value"] / forms["replay_f64_mt19937"]["samples_per_s"], 1)
+        result["vs_exact_philox_f64"] = {"entries": int(len(z)), "max_abs_z": round(float(np.max(np.abs(z))), 3),
+                                         "mean_z": round(float(np.mean(z)), 4),
+                                         "beyond_3_se": int(np.sum(np.abs(z) > 3))}
     ctx.release_geometry(geom)
     ctx.close()
     print(json.dumps(result))

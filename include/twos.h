@@ -344,9 +344,15 @@ twos_status twos_trace(twos_ctx *ctx, int32_t geom_id, const twos_side *side, co
  * another cavity, or off outside the domain / within eps of its boundary),
  * runs one nested path of `side` from off with value V; the sample adds
  * n V / ell to cavity k's centre derivative and -V / ell to its radius
- * derivative (other cavities get zero).  Streams: the reference's
- * mt19937_64 keys (params->seed, params->repeat, params->site_offset + site,
- * sample); params->precision must be 64.  out: [n][n_cavities][8] = the
+ * derivative (other cavities get zero).  Streams: params->rng
+ * TWOS_RNG_PHILOX is the production form -- Philox streams keyed by
+ * (params->seed, params->repeat, params->site_offset + site) with the sample
+ * and step in the counter, one persistent launch, params->precision 32 or 64,
+ * and with TWOS_FLAG_DISTANCE_GRID mesh steps take the mesh's grid bound when
+ * it clears max(grid_kappa cells, 2 eps) (cavity distances stay exact); any
+ * other rng replays the reference's mt19937_64 keys (params->seed,
+ * params->repeat, params->site_offset + site, sample) in binary64 (precision
+ * must be 64).  out: [n][n_cavities][8] = the
  * centre derivative x y z, their standard errors, the radius derivative and
  * its standard error (means over all walks, unsafe and non-cavity samples
  * counting as zero).  counts (may be null): [n][9] = outer steps, nested

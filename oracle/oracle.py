@@ -338,15 +338,18 @@ DIFF_COUNTS = ("outer_steps", "nested_steps", "cavity_terminations", "near_seam"
 
 
 def differential(geometry: Geometry, sites, side_, *, seed=0, repeat=0, walks=16, eps=1e-4, max_steps=2048,
-                 offset=0.004, threads=0):
+                 offset=0.004, threads=0, rng=RNG_MT_DIFF):
     """Differential WoS cavity derivatives per site on the reference's
-    streams: (values [n][n_cavities][8] = dc x y z, their SEs, dR, its SE;
-    counts [n][9] in DIFF_COUNTS order)."""
+    streams (rng RNG_MT_DIFF) or on Philox streams (RNG_PHILOX): (values
+    [n][n_cavities][8] = dc x y z, their SEs, dR, its SE; counts [n][9] in
+    DIFF_COUNTS order)."""
+    if rng not in (RNG_MT_DIFF, RNG_PHILOX):
+        raise ValueError("differential streams are RNG_MT_DIFF or RNG_PHILOX")
     sites = _f64(np.atleast_2d(sites))
     n = len(sites)
     g = geometry.struct()
     k = g.n_cavities
-    p = Params(rng=RNG_MT_DIFF, seed=seed, repeat=repeat, max_steps=max_steps, eps=eps, walks=walks,
+    p = Params(rng=rng, seed=seed, repeat=repeat, max_steps=max_steps, eps=eps, walks=walks,
                threads=threads)
     out = np.zeros((n, max(k, 1), 8))
     counts = np.zeros((n, 9), dtype=np.int64)

@@ -927,6 +927,18 @@ void differential_site(const og_geom &g, const og_side &side, const og_params &p
   const int32_t K = g.n_cavities;
   std::vector<V3> sums(size_t(K), V3{0.0, 0.0, 0.0}), sq(size_t(K), V3{0.0, 0.0, 0.0});
   std::vector<double> rs(size_t(K), 0.0), rsq(size_t(K), 0.0);
+  // Philox streams (the production form; no reference counterpart): the
+  // outer path of sample n draws direction `step` from counter (step, n, 0, 0)
+  // under the key of (dseed, site, 0), the nested path is walk n of the key
+  // of (dseed, site, 1), where dseed folds the reference's seed and repeat
+  // terms of the outer key (wih_twos.cpp:867-870).
+  const bool philox = p.rng == OG_RNG_PHILOX;
+  uint32_t outer_key[2] = {0u, 0u}, nested_key[2] = {0u, 0u};
+  if (philox) {
+    const uint64_t dseed = splitmix64(p.seed + 0x6a09e667f3bcc909ULL) ^ splitmix64(uint64_t(p.repeat) + 0xbb67ae8584caa73bULL);
+    philox_key(dseed, site, 0, outer_key);
+    philox_key(dseed, site, 1, nested_key);
+  }
   for (int64_t n = 0; n < p.walks; ++n) {
     const uint64_t key = splitmix64(p.seed + 0x6a09e667f3bcc909ULL) ^
                          splitmix64(uint64_t(p.repeat) + 0xbb67ae8584caa73bULL) ^
@@ -943,7 +955,7 @@ void differential_site(const og_geom &g, const og_side &side, const og_params &p
         capped = false;
         break;
       }
-      x = x + random_unit_vector(rng) * h.distance;
+      x = x + (philox ? philox_direction(outer_key, uint32_t(n), uint32_t(s)) : random_unit_vector(rng)) * h.distance;
       ++c.outer_steps;
     }
     if (capped) {
@@ -961,10 +973,16 @@ void differential_site(const og_geom &g, const og_side &side, const og_params &p
       continue;
     }
     // The nested walk (walk(), wih_twos.cpp:635-664, keyed nested_key).
-    MtStream st;
-    st.rng.seed(splitmix64(splitmix64(key ^ 0x510e527fade682d1ULL)));
     int32_t rows = 0;
-    const WalkOut w = walk(g, side, nullptr, off, p.eps, p.max_steps, st, false, nullptr, 0, &rows);
+    WalkOut w;
+    if (philox) {
+      PhiloxStream st{{nested_key[0], nested_key[1]}, uint32_t(n)};
+      w = walk(g, side, nullptr, off, p.eps, p.max_steps, st, false, nullptr, 0, &rows);
+    } else {
+      MtStream st;
+      st.rng.seed(splitmix64(splitmix64(key ^ 0x510e527fade682d1ULL)));
+      w = walk(g, side, nullptr, off, p.eps, p.max_steps, st, false, nullptr, 0, &rows);
+    }
     c.nested_steps += w.steps;
     c.queries += w.steps + (w.cap ? 0 : 1);
     c.limits += w.cap ? 1 : 0;
@@ -1136,7 +1154,8 @@ extern "C" {
 
 int32_t og_differential(const og_geom *g, const og_side *side, const og_params *prm, const double *sites, int64_t n,
                         double ell, double *out, int64_t *counts) {
-  if (!g || g->container != OG_MESH || !g->mesh || prm->walks < 1 || !(ell > 0.0) || prm->rng != OG_RNG_MT_DIFF)
+  if (!g || g->container != OG_MESH || !g->mesh || prm->walks < 1 || !(ell > 0.0) ||
+      (prm->rng != OG_RNG_MT_DIFF && prm->rng != OG_RNG_PHILOX))
     return 2;
   const og_params p = *prm;
   std::vector<DiffCounts> per(size_t(std::max<int64_t>(n, 1)));

@@ -138,7 +138,11 @@ void og_combine(const double *a, const double *b, int64_t n, int32_t square, dou
 void og_combine_powersums(const double sums[8], int64_t n, double out[6]);
 /* Differential WoS cavity-shape derivatives (evaluate_differential_site,
  * wih_twos.cpp:725-928) on a mesh with cavities, the reference's streams
- * (rng = OG_RNG_MT_DIFF, repeat as given): out[n][n_cavities][8] = dc x y z,
+ * (rng = OG_RNG_MT_DIFF, repeat as given) or Philox streams (rng =
+ * OG_RNG_PHILOX: sample n's outer path draws direction `step` from counter
+ * (step, n, 0, 0) under philox_key(dseed, site, 0), its nested path is walk n
+ * of philox_key(dseed, site, 1), dseed = splitmix64(seed + 0x6a09e667f3bcc909)
+ * ^ splitmix64(repeat + 0xbb67ae8584caa73b)): out[n][n_cavities][8] = dc x y z,
  * their SEs, dR, its SE; counts[n][9] = outer steps, nested steps, cavity
  * terminations, near-seam, segment-unsafe, unsafe, limits, distance queries,
  * parity rays. */

@@ -13,7 +13,12 @@ through the C-ABI.
   cavity terminations, seam / segment / unsafe classifications, caps,
   queries, rays) exactly, values as above, also with overlapping cavities;
 - a site's result depends on (seed, repeat, site index, sample) only;
-- wih_study --diff 1 writes the library's values and the reference's totals.
+- wih_study --diff 1 writes the library's values and the reference's totals;
+- the production form (Philox streams, one persistent launch): binary64
+  equal to the oracle's Philox form per site (every counter exactly, values
+  as above), binary32 within a fraction of an SE of its binary64 rerun, the
+  distance-grid walk within its combined SE of the exact walk, and results
+  keyed by site index.
 """
 
 import json
@@ -48,7 +53,7 @@ def _req(diff, source):
     return next(r for r in diff[4] if r.source == source)
 
 
-def _gpu(ctx, diff, req, *, holes=None, points=None, walks=None, seed=None, site_offset=0):
+def _gpu(ctx, diff, req, *, holes=None, points=None, walks=None, seed=None, site_offset=0, **kw):
     heat, _ = twos.study_terms(req.config)
     g = ctx.geometry(twos.MESH, mesh=diff[2], cavities=req.holes if holes is None else holes,
                      cavity_radius=req.cavity_radius)
@@ -56,18 +61,36 @@ def _gpu(ctx, diff, req, *, holes=None, points=None, walks=None, seed=None, site
         return ctx.differential(g, req.points if points is None else points, twos.side(terms=heat),
                                 seed=req.seed if seed is None else seed, walks=walks or req.walks,
                                 eps=req.epsilon, max_steps=req.max_steps, offset=req.offset,
-                                site_offset=site_offset)
+                                site_offset=site_offset, **kw)
     finally:
         ctx.release_geometry(g)
 
 
-def _oracle(diff, req, *, holes=None, points=None, walks=None, seed=None):
+def _oracle(diff, req, *, holes=None, points=None, walks=None, seed=None, rng=O.RNG_MT_DIFF):
     heat, _ = O.study_terms(req.config)
     g = O.Geometry(O.MESH, mesh=diff[3], cavities=req.holes if holes is None else holes,
                    cavity_radius=req.cavity_radius)
     return O.differential(g, req.points if points is None else points, O.side(terms=heat),
                           seed=req.seed if seed is None else seed, walks=walks or req.walks, eps=req.epsilon,
-                          max_steps=req.max_steps, offset=req.offset, threads=O.host_threads())
+                          max_steps=req.max_steps, offset=req.offset, threads=O.host_threads(), rng=rng)
+
+
+PHILOX64 = dict(rng=twos.RNG_PHILOX, precision=64)
+PHILOX32 = dict(rng=twos.RNG_PHILOX, precision=32)
+PHILOX32_GRID = dict(rng=twos.RNG_PHILOX, precision=32, distance_grid=True)
+
+
+def _z(a, b, tol_se=0.0):
+    """Per (site, cavity, column) z-scores of two results' derivative columns
+    in their combined SE, over the entries where that SE is positive, and the
+    entries where it is zero (both must then agree exactly)."""
+    zs, zero_pairs = [], []
+    for col, se in ((0, 3), (1, 4), (2, 5), (6, 7)):
+        s = np.hypot(a[..., se], b[..., se]) + tol_se
+        ok = s > 0
+        zs.append((a[..., col] - b[..., col])[ok] / s[ok])
+        zero_pairs.append((a[..., col][~ok], b[..., col][~ok]))
+    return np.concatenate(zs), zero_pairs
 
 
 def _totals(counts):
@@ -125,6 +148,85 @@ def test_results_depend_on_site_index_only(ctx, diff):
     np.testing.assert_array_equal(hc, fc[:20])
     mid, _ = _gpu(ctx, diff, req, points=req.points[40:50], walks=64, site_offset=40)
     np.testing.assert_array_equal(mid, full[40:50])
+
+
+def test_philox_binary64_matches_oracle(ctx, diff):
+    """The production streams in binary64 without the grid: every per-site
+    counter equal to the oracle's Philox form, the derivative columns to 2e-4
+    of scale with most sites bitwise (the device library's cos / sin may
+    differ from glibc's by an ulp)."""
+    req = _req(diff, "validation-local-cavity/raw/gradient-0.004-1")
+    print("differential philox binary64", req.source, flush=True)
+    values, counts = _gpu(ctx, diff, req, walks=256, seed=77, **PHILOX64)
+    ref_values, ref_counts = _oracle(diff, req, walks=256, seed=77, rng=O.RNG_PHILOX)
+    np.testing.assert_array_equal(counts, ref_counts)
+    ref = SF.DiffRequest(**{**req.__dict__, "values": ref_values})
+    exact = SF.compare_diff(ref, values, rtol=2e-4)
+    assert exact.mean() > 0.5, exact.mean()
+    assert np.count_nonzero(values) > 0
+
+
+def test_philox_binary64_overlapping_cavities_match_oracle(ctx, diff):
+    req = _req(diff, "validation-local-cavity/raw/gradient-0.002-0")
+    c = req.holes[0]
+    holes = np.array([c, c + [0.6 * req.cavity_radius, 0.0, 0.0], c + [0.0, 3.0 * req.cavity_radius, 0.0]])
+    pts = req.points[:32]
+    print("differential philox binary64 overlapping cavities", flush=True)
+    values, counts = _gpu(ctx, diff, req, holes=holes, points=pts, walks=128, seed=3, **PHILOX64)
+    ref_values, ref_counts = _oracle(diff, req, holes=holes, points=pts, walks=128, seed=3, rng=O.RNG_PHILOX)
+    np.testing.assert_array_equal(counts, ref_counts)
+    tot = dict(zip(O.DIFF_COUNTS, counts.sum(axis=0)))
+    assert tot["near_seam"] > 0 and tot["unsafe"] >= tot["near_seam"]
+    ref = SF.DiffRequest(**{**req.__dict__, "points": pts, "holes": holes, "values": ref_values})
+    SF.compare_diff(ref, values, rtol=2e-4)
+
+
+def test_philox_binary32_shadows_binary64(ctx, diff):
+    """binary32 walks draw the same Philox numbers as their binary64 rerun, so
+    most paths take the same steps: derivatives within a fraction of their SE
+    and step totals within 1%."""
+    req = _req(diff, "validation-local-cavity/raw/gradient-0.004-0")
+    print("differential philox binary32 vs binary64", flush=True)
+    v64, c64 = _gpu(ctx, diff, req, walks=1024, seed=5, **PHILOX64)
+    v32, c32 = _gpu(ctx, diff, req, walks=1024, seed=5, **PHILOX32)
+    z, zero = _z(v32, v64)
+    assert len(z) > 300
+    # A path that ends differently moves its site by about one sample / N.
+    assert np.mean(np.abs(z) > 1.0) <= 0.02 and np.max(np.abs(z)) < 3.0, np.sort(np.abs(z))[-4:]
+    for a, b in zero:
+        np.testing.assert_array_equal(a, b)
+    for k in (0, 1, 2, 7):
+        assert abs(c32[:, k].sum() - c64[:, k].sum()) <= 0.01 * c64[:, k].sum(), (O.DIFF_COUNTS[k], c32[:, k].sum(),
+                                                                                   c64[:, k].sum())
+
+
+def test_philox_distance_grid_against_exact(ctx, diff):
+    """Mesh steps far from the mesh take the distance grid's conservative
+    bound as their radius: other paths, the same derivatives -- within their
+    combined SE of the exact binary64 walk -- fewer exact queries, and the
+    same rate of cavity terminations."""
+    req = _req(diff, "validation-local-cavity/raw/gradient-0.004-0")
+    print("differential philox binary32 grid vs exact binary64", flush=True)
+    ex, ce = _gpu(ctx, diff, req, walks=2048, seed=9, **PHILOX64)
+    gr, cg = _gpu(ctx, diff, req, walks=2048, seed=9, **PHILOX32_GRID)
+    assert not np.array_equal(cg[:, 0], ce[:, 0])  # the grid was used
+    z, _ = _z(gr, ex)
+    assert len(z) > 300
+    assert np.sum(np.abs(z) > 3.5) <= 2 and abs(np.mean(z)) < 0.2, (np.sort(np.abs(z))[-4:], np.mean(z))
+    te, tg = ce.sum(axis=0), cg.sum(axis=0)
+    assert abs(tg[2] - te[2]) <= 0.05 * te[2], (tg[2], te[2])  # cavity terminations
+
+
+def test_philox_results_depend_on_site_index_only(ctx, diff):
+    req = _req(diff, "validation-local-cavity/raw/gradient-0.002-1")
+    full, fc = _gpu(ctx, diff, req, walks=64, **PHILOX32_GRID)
+    head, hc = _gpu(ctx, diff, req, points=req.points[:20], walks=64, **PHILOX32_GRID)
+    np.testing.assert_array_equal(head, full[:20])
+    np.testing.assert_array_equal(hc, fc[:20])
+    mid, _ = _gpu(ctx, diff, req, points=req.points[40:50], walks=64, site_offset=40, **PHILOX32_GRID)
+    np.testing.assert_array_equal(mid, full[40:50])
+    again, _ = _gpu(ctx, diff, req, walks=64, **PHILOX32_GRID)
+    np.testing.assert_array_equal(again, full)
 
 
 def test_rejects_non_mesh_geometry(ctx):

@@ -77,3 +77,53 @@ def test_offset_safety_rejects_overlapping_cavities(diff):
     c = dict(zip(O.DIFF_COUNTS, counts.sum(axis=0)))
     assert c["cavity_terminations"] > 0 and c["near_seam"] > 0
     assert c["unsafe"] >= c["near_seam"]
+
+
+def _run_philox(mesh, req, walks, points=None, threads=8, seed=None):
+    heat, _ = O.study_terms(req.config)
+    g = O.Geometry(O.MESH, mesh=mesh, cavities=req.holes, cavity_radius=req.cavity_radius)
+    return O.differential(g, req.points if points is None else points, O.side(terms=heat),
+                          seed=req.seed if seed is None else seed, walks=walks, eps=req.epsilon,
+                          max_steps=req.max_steps, offset=req.offset, threads=threads, rng=O.RNG_PHILOX)
+
+
+def test_philox_streams_estimate_the_same_derivatives(diff):
+    """The Philox form (the GPU's production streams) differs from the
+    reference's only in its random numbers: on a retained request it gives
+    the same derivatives as the reference-pinned mt19937_64 form within
+    their standard errors, the same step and termination rates, and
+    standard errors of the same size over the request (each sample on its
+    own stream: a counter that ignored the sample index would repeat one path
+    and leave SE 0)."""
+    mesh, requests = diff
+    req = next(r for r in requests if r.source == "validation-local-cavity/raw/gradient-0.004-0")
+    mt, cm = _run(mesh, SF.DiffRequest(**{**req.__dict__, "walks": 512}))
+    ph, cp = _run_philox(mesh, req, 512)
+    names = O.DIFF_COUNTS
+    tm, tp = dict(zip(names, cm.sum(axis=0))), dict(zip(names, cp.sum(axis=0)))
+    for k in ("outer_steps", "nested_steps", "distance_queries", "cavity_terminations", "parity_rays"):
+        assert abs(tp[k] - tm[k]) <= 0.03 * tm[k], (k, tp[k], tm[k])
+    for col, se in ((0, 3), (1, 4), (2, 5), (6, 7)):
+        s = np.hypot(mt[..., se], ph[..., se])
+        ok = s > 0
+        assert ok.sum() >= 100, ok.sum()
+        z = (mt[..., col] - ph[..., col])[ok] / s[ok]
+        assert np.max(np.abs(z)) < 4.0 and abs(np.mean(z)) < 0.35, (col, np.max(np.abs(z)), np.mean(z))
+        # Few samples per site end safely on the cavity, so per-site SEs are
+        # noisy: compare their root mean square over the request.
+        ratio = np.sqrt(np.sum(ph[..., se] ** 2) / np.sum(mt[..., se] ** 2))
+        assert 0.5 < ratio < 2.0, (col, ratio)
+
+
+def test_philox_streams_are_keyed_by_index(diff):
+    """A Philox sample depends on (seed, repeat, site, sample) only: a prefix
+    of the sites and any thread count give the same values, and another seed
+    other values."""
+    mesh, requests = diff
+    req = requests[0]
+    full, fc = _run_philox(mesh, req, 64, threads=8)
+    head, hc = _run_philox(mesh, req, 64, threads=1, points=req.points[:20])
+    np.testing.assert_array_equal(head, full[:20])
+    np.testing.assert_array_equal(hc, fc[:20])
+    other, _ = _run_philox(mesh, req, 64, points=req.points[:20], seed=req.seed + 1)
+    assert not np.array_equal(other, head)

@@ -1,48 +1,20 @@
-// Differential Walk on Spheres: cavity-shape derivatives of the estimator's
-// value (evaluate_differential_site, benchmarks/compiled_bunny/src/
-// wih_twos.cpp:725-928, as wih_study --diff 1 runs it,
-// benchmarks/submission_study/study.cpp:303-316), binary64 on the reference's
-// mt19937_64 streams.
+// Differential Walk on Spheres, binary64: the reference's mt19937_64
+// streams (diff_walk, one thread per job), the Philox binary64 form of
+// diff_persistent, and the per-(site, cavity) fold.  The binary32 Philox
+// form is built in twos_diff_f32.hip (contraction on).
 //
-// Work model.  One job is one (site, sample) pair; a thread runs it whole:
-//   1. the outer path walks to the boundary with no source (walk_to_boundary,
-//      wih_twos.cpp:725-740), keeping the last query's closest point, its
-//      outward normal and which cavity (if any) it lies on;
-//   2. a path that ended on a cavity is offset by ell into the domain,
-//      off = hit - n ell, and classified (classify_offset_safety,
-//      wih_twos.cpp:764-833): near the seam circle of an overlapping cavity,
-//      a segment hit - off that passes through another cavity, or an offset
-//      point outside the domain make the sample unsafe;
-//   3. a safe sample runs one nested path from off with the side's boundary
-//      data and source (walk(), wih_twos.cpp:635-664); its value V gives the
-//      centre derivative sample n V / ell and the radius sample -V / ell.
-// Each job writes its sample and its counters (steps, queries, ray queries,
-// flags) to a per-job record; a second kernel folds each (site, cavity) pair's
-// samples in sample order, so a result depends only on (seed, repeat, site,
-// sample), never on which lane ran the job.
-//
-// Keys (wih_twos.cpp:867-872, 895): key = splitmix64(seed + 0x6a09e667f3bcc909)
-// ^ splitmix64(repeat + 0xbb67ae8584caa73b) ^ splitmix64(site + 0x3c6ef372fe94f82b)
-// ^ splitmix64(sample + 0xa54ff53a5f1d36f1); the outer path draws from
-// mt19937_64(splitmix64(key)), the nested path from
+// Keys of the reference's streams (wih_twos.cpp:867-872, 895): key =
+// splitmix64(seed + 0x6a09e667f3bcc909) ^ splitmix64(repeat + 0xbb67ae8584caa73b)
+// ^ splitmix64(site + 0x3c6ef372fe94f82b) ^ splitmix64(sample + 0xa54ff53a5f1d36f1);
+// the outer path draws from mt19937_64(splitmix64(key)), the nested path from
 // mt19937_64(splitmix64(splitmix64(key ^ 0x510e527fade682d1))).
 
-#include "twos_device.h"
+#include "twos_diff.h"
 #include "twos_launch.h"
-#include "twos_ray.h"
 
 namespace twos {
 
 namespace {
-
-// Job record flags.
-enum : int32_t {
-  kDiffCavity = 1,    // the outer path ended on a cavity
-  kDiffSeam = 2,      // ... near the seam of two overlapping cavities
-  kDiffSegment = 4,   // ... with an unsafe offset segment or offset point
-  kDiffUnsafe = 8,    // the sample was dropped (seam or segment)
-  kDiffLimit = 16,    // a path reached max_steps
-};
 
 struct Hit {
   double distance;
@@ -75,74 +47,6 @@ __device__ __forceinline__ Hit boundary_hit(const DevGeom &g, V3<double> p, int3
     }
   }
   return h;
-}
-
-// domain_distance (wih_twos.cpp:585-593): one mesh query, then the cavities.
-__device__ __forceinline__ double domain_distance(const DevGeom &g, V3<double> p, int32_t &hint) {
-  const MeshView m{g.nodes, g.nodes4, g.tris_f, g.tris_d, g.tri_ids, g.n_tris};
-  V3<double> q;
-  double best = mesh_distance<double>(g, mesh_closest<double, false>(m, p, q, hint, nullptr));
-  for (int32_t i = 0; i < g.n_cavities; ++i) {
-    const Vec4<double> s = load4<double>(g, 2, i);
-    const V3<double> d = p - V3<double>{s.x, s.y, s.z};
-    best = fmin(best, sqrt(dot(d, d)) - g.cavity_radius);
-  }
-  return best;
-}
-
-// classify_offset_safety (wih_twos.cpp:764-833) with the enclosure fallback
-// the study executable compiles in (study.cpp:8): returns kDiffSeam and/or
-// kDiffSegment.
-__device__ __forceinline__ int32_t offset_safety(const DevGeom &g, int32_t hit, V3<double> bp, V3<double> off,
-                                                 double ell, double eps, int32_t &queries, int32_t &rays,
-                                                 int32_t &hint) {
-  bool near_seam = false, segment_unsafe = false;
-  const double r = g.cavity_radius;
-  const V3<double> seg = off - bp;
-  const double seg2 = dot(seg, seg);
-  const double tol = 1.0e-10 * fmax(1.0, fabs(r));
-  const double coll2 = (r + tol) * (r + tol);
-  const Vec4<double> h4 = load4<double>(g, 2, hit);
-  const V3<double> hc{h4.x, h4.y, h4.z};
-  for (int32_t i = 0; i < g.n_cavities; ++i) {
-    if (i == hit) continue;
-    const Vec4<double> c4 = load4<double>(g, 2, i);
-    const V3<double> ci{c4.x, c4.y, c4.z};
-    const V3<double> delta = ci - hc;
-    const double cd = sqrt(dot(delta, delta));
-    if (cd > tol && cd < 2.0 * r) {
-      // Distance from the hit point to the seam circle of two equal spheres.
-      const V3<double> axis = delta * (1.0 / cd);
-      const V3<double> seam_c = (hc + ci) * 0.5;
-      const double seam_r = sqrt(fmax(0.0, r * r - 0.25 * (cd * cd)));
-      const V3<double> rel = bp - seam_c;
-      const double ax = dot(rel, axis);
-      const V3<double> rad = rel - axis * ax;
-      const double nr = sqrt(dot(rad, rad)) - seam_r;
-      near_seam = near_seam || sqrt(ax * ax + nr * nr) <= ell;
-    }
-    if (seg2 > 0.0) {
-      const double t = fmin(fmax(dot(ci - bp, seg) / seg2, 0.0), 1.0);
-      const V3<double> cl = bp + seg * t;
-      if (dot(cl - ci, cl - ci) <= coll2) segment_unsafe = true;
-    }
-  }
-  const MeshView m{g.nodes, g.nodes4, g.tris_f, g.tris_d, g.tri_ids, g.n_tris};
-  bool inside = inside_parity(m, off, &rays) == 1;
-  if (!inside && inside_parity(m, hc, &rays) == 1) {
-    V3<double> q;
-    ++queries;
-    const double dc = mesh_distance<double>(g, mesh_closest<double, false>(m, hc, q, hint, nullptr));
-    const V3<double> oc = off - hc;
-    if (dc > r + ell + eps && sqrt(dot(oc, oc)) <= r + ell + eps * 0.5) inside = true;
-  }
-  if (!inside) {
-    segment_unsafe = true;
-  } else {
-    ++queries;
-    if (!(domain_distance(g, off, hint) > eps)) segment_unsafe = true;
-  }
-  return (near_seam ? kDiffSeam : 0) | (segment_unsafe ? kDiffSegment : 0);
 }
 
 __device__ __forceinline__ uint64_t diff_key(uint64_t seed, int32_t repeat, int64_t site, int64_t sample) {
@@ -217,18 +121,7 @@ __global__ __launch_bounds__(64) void diff_walk(const DevGeom *geom, const DevSi
       value = value / p.ell;
     }
   }
-  double *v = p.val + 4 * job;
-  v[0] = h.normal.x;
-  v[1] = h.normal.y;
-  v[2] = h.normal.z;
-  v[3] = value;
-  int32_t *r = p.rec + 6 * job;
-  r[0] = cavity;
-  r[1] = flags;
-  r[2] = outer;
-  r[3] = nested;
-  r[4] = queries;
-  r[5] = rays;
+  diff_record<double>(p, job, h.normal, value, cavity, flags, outer, nested, queries, rays);
 }
 
 // One thread per (site, cavity) of the chunk (one per site when there are no
@@ -302,9 +195,18 @@ hipError_t launch_differential(const DevGeom *d_geom, const DevSide *d_side, con
   hipLaunchKernelGGL(diff_walk, dim3(blocks_for(p.jobs, 64)), dim3(64), 0, stream, d_geom, d_side, p);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  return launch_diff_fold(p, n_sites, n_cav, stream);
+}
+
+hipError_t launch_diff_fold(const DiffParams &p, int32_t n_sites, int32_t n_cav, hipStream_t stream) {
   hipLaunchKernelGGL(diff_fold, dim3(blocks_for(int64_t(n_sites) * (n_cav > 0 ? n_cav : 1), 64)), dim3(64), 0,
                      stream, p, n_sites, n_cav);
   return hipGetLastError();
+}
+
+hipError_t launch_diff_persistent_f64(const DevGeom *d_geom, const DevSide *d_side, const DiffParams &p, int cus,
+                                      hipStream_t stream) {
+  return launch_diff_persistent<double>(d_geom, d_side, p, cus, stream);
 }
 
 }  // namespace twos

@@ -59,15 +59,22 @@ hipError_t launch_walk_persistent_f64(int geom_kind, bool count, const Tables &t
                                       int cus, hipStream_t stream, int *grid_out);
 hipError_t launch_reduce(const Tables &tb, const WalkParams &wp, int grid, hipStream_t stream);
 hipError_t launch_zero(unsigned long long *p, int n, hipStream_t stream);
-// Differential WoS over one chunk of sites (p.site_base, n_sites sites):
-// the per-job walks, then the per-(site, cavity) fold.
 // Certified sampling of n_sites sites, samples sample_begin .. + count - 1.
 hipError_t launch_certify(const DevGeom *d_geom, const twos_cert_params &prm, const double *d_sites, int64_t n_sites,
                           int64_t sample_begin, int64_t count, double *d_value, int32_t *d_steps, int32_t *d_flags,
                           hipStream_t stream);
 hipError_t launch_cert_distance(const DevGeom *d_geom, const double *d_pts, int64_t n, double *d_out,
                                 hipStream_t stream);
+// Differential WoS over one chunk of sites (p.site_base, n_sites sites) on
+// the reference's streams: the per-job walks, then the per-(site, cavity) fold.
 hipError_t launch_differential(const DevGeom *d_geom, const DevSide *d_side, const DiffParams &p, int32_t n_sites,
                                int32_t n_cav, hipStream_t stream);
+// The Philox form: one persistent launch over p.jobs tickets (p.counter zeroed
+// before it), binary64 or binary32 walks; then launch_diff_fold.
+hipError_t launch_diff_persistent_f64(const DevGeom *d_geom, const DevSide *d_side, const DiffParams &p, int cus,
+                                      hipStream_t stream);
+hipError_t launch_diff_persistent_f32(const DevGeom *d_geom, const DevSide *d_side, const DiffParams &p, int cus,
+                                      hipStream_t stream);
+hipError_t launch_diff_fold(const DiffParams &p, int32_t n_sites, int32_t n_cav, hipStream_t stream);
 
 }  // namespace twos

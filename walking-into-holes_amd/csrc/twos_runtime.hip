@@ -1864,7 +1864,12 @@ twos_status twos_differential(twos_ctx *ctx, int32_t geom_id, const twos_side *s
   const twos_params &p = *params;
   if (n < 0 || p.walks < 1 || p.max_steps < 1 || !(p.eps > 0.0) || !(ell > 0.0) || !std::isfinite(ell))
     return fail(TWOS_ERR_INVALID_ARGUMENT, "invalid numeric option");
-  if (p.precision != 64) return fail(TWOS_ERR_INVALID_ARGUMENT, "the differential estimator runs in binary64");
+  const bool philox = p.rng == TWOS_RNG_PHILOX;
+  if (!philox && p.precision != 64)
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "the differential estimator's replay streams run in binary64");
+  if (p.precision != 32 && p.precision != 64) return fail(TWOS_ERR_INVALID_ARGUMENT, "precision must be 32 or 64");
+  if (!(p.grid_kappa >= 0.f && p.grid_kappa < 1e6f))
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "grid_kappa must be a non-negative number of cells");
   if (twos_status sst = check_side(*side)) return sst;
   for (int64_t i = 0; i < 3 * n; ++i)
     if (!std::isfinite(sites[i])) return fail(TWOS_ERR_INVALID_ARGUMENT, "non-finite site coordinate");
@@ -1889,7 +1894,8 @@ twos_status twos_differential(twos_ctx *ctx, int32_t geom_id, const twos_side *s
     int32_t *rec = nullptr;
     int64_t *counts = nullptr;
     DevSide *side = nullptr;
-    ~Buffers() { dfree(sites); dfree(val); dfree(out); dfree(rec); dfree(counts); dfree(side); }
+    unsigned long long *counter = nullptr;
+    ~Buffers() { dfree(sites); dfree(val); dfree(out); dfree(rec); dfree(counts); dfree(side); dfree(counter); }
   } b;
   HIP_OK(dalloc(&b.sites, 3 * size_t(n)));
   HIP_OK(dalloc(&b.val, 4 * size_t(jobs_max)));
@@ -1897,6 +1903,7 @@ twos_status twos_differential(twos_ctx *ctx, int32_t geom_id, const twos_side *s
   HIP_OK(dalloc(&b.out, 8 * size_t(n) * size_t(n_cav)));
   HIP_OK(dalloc(&b.counts, 9 * size_t(n)));
   HIP_OK(dalloc(&b.side, 1));
+  if (philox) HIP_OK(dalloc(&b.counter, 1));
   HIP_OK(hipMemcpy(b.sites, sites, sizeof(double) * 3 * size_t(n), hipMemcpyHostToDevice));
   const DevSide dside = *side;
   HIP_OK(hipMemcpy(b.side, &dside, sizeof(DevSide), hipMemcpyHostToDevice));
@@ -1913,11 +1920,22 @@ twos_status twos_differential(twos_ctx *ctx, int32_t geom_id, const twos_side *s
   dp.max_steps = p.max_steps;
   dp.eps = p.eps;
   dp.ell = ell;
+  dp.counter = b.counter;
+  dp.pseed = splitmix64(p.seed + 0x6a09e667f3bcc909ULL) ^ splitmix64(uint64_t(p.repeat) + 0xbb67ae8584caa73bULL);
+  dp.grid_kappa = (p.flags & TWOS_FLAG_DISTANCE_GRID) ? (p.grid_kappa > 0.f ? p.grid_kappa : kGridKappa) : 0.f;
+  const int cus = ctx->prop.multiProcessorCount;
   for (int64_t s0 = 0; s0 < n; s0 += chunk) {
     const int32_t cnt = int32_t(std::min(chunk, n - s0));
     dp.site_base = s0;
     dp.jobs = int64_t(cnt) * p.walks;
-    HIP_OK(launch_differential(ctx->d_geoms + geom_id, b.side, dp, cnt, n_cav, ctx->stream));
+    if (philox) {
+      HIP_OK(hipMemsetAsync(b.counter, 0, sizeof(unsigned long long), ctx->stream));
+      HIP_OK(p.precision == 32 ? launch_diff_persistent_f32(ctx->d_geoms + geom_id, b.side, dp, cus, ctx->stream)
+                               : launch_diff_persistent_f64(ctx->d_geoms + geom_id, b.side, dp, cus, ctx->stream));
+      HIP_OK(launch_diff_fold(dp, cnt, n_cav, ctx->stream));
+    } else {
+      HIP_OK(launch_differential(ctx->d_geoms + geom_id, b.side, dp, cnt, n_cav, ctx->stream));
+    }
   }
   HIP_OK(hipMemcpyAsync(out, b.out, sizeof(double) * 8 * size_t(n) * size_t(n_cav), hipMemcpyDeviceToHost,
                         ctx->stream));

@@ -438,18 +438,23 @@ class Context:
                    "limits", "distance_queries", "parity_rays")
 
     def differential(self, geom: int, sites, side_: Side, *, seed=0, repeat=0, walks=16, eps=1e-4,
-                     max_steps=2048, offset=0.004, site_offset=0):
+                     max_steps=2048, offset=0.004, site_offset=0, rng=RNG_MT_KEY, precision=64,
+                     distance_grid=False, grid_kappa=0.0):
         """Differential WoS cavity derivatives (evaluate_differential_site,
-        wih_twos.cpp:725-928) on the reference's mt19937_64 streams, binary64:
-        (values [n][n_cavities][8] = centre derivative x y z, their SEs, radius
-        derivative, its SE; counts [n][9] in DIFF_COUNTS order)."""
+        wih_twos.cpp:725-928): (values [n][n_cavities][8] = centre derivative
+        x y z, their SEs, radius derivative, its SE; counts [n][9] in
+        DIFF_COUNTS order).  rng RNG_MT_KEY replays the reference's
+        mt19937_64 streams in binary64; RNG_PHILOX is the production form
+        (one persistent launch, precision 32 or 64, optionally the mesh's
+        distance grid for steps at least grid_kappa cells from the mesh)."""
         p = _f64(sites).reshape(-1, 3)
         n = len(p)
         k = self._geom_cavities.get(geom, (0, 0.0))[0]
         out = np.zeros((n, max(k, 1), 8))
         counts = np.zeros((n, 9), dtype=np.int64)
-        prm = _params(rng=RNG_MT_KEY, precision=64, seed=seed, repeat=repeat, eps=eps, max_steps=max_steps,
-                      walks=walks, schedule=PLAIN, site_offset=site_offset)
+        prm = _params(rng=rng, precision=precision, seed=seed, repeat=repeat, eps=eps, max_steps=max_steps,
+                      walks=walks, schedule=PLAIN, site_offset=site_offset,
+                      flags=FLAG_DISTANCE_GRID if distance_grid else 0, grid_kappa=grid_kappa)
         _check(_lib().twos_differential(self._h, geom, C.byref(side_), C.byref(prm), _ptr(p, _capi._dp), n,
                                         float(offset), _ptr(out, _capi._dp),
                                         counts.ctypes.data_as(C.POINTER(C.c_int64))))
