@@ -655,8 +655,10 @@ def f4_bench(args) -> int:
     cavities of radius 0.05, the primal data g = x + 0.3 y^2, offset 0.004,
     eps 1e-4) or certified minimum-benefit sampling (twos_certify; 16 sites x
     65,536 samples, the affine source, radius cap 0.1, value cap 0.06, eps
-    1e-6, 128 steps; binary64 on the reference's mt19937_64 streams).  A step
-    is one call; the metric is samples per second.  Differential WoS is timed
+    1e-6, 128 steps).  A step is one call; the metric is samples per second.
+    Certified sampling is timed in its production form (Philox streams, one
+    persistent launch, binary64 with directed rounding), its replay form (the
+    reference's mt19937_64 streams) once beside it.  Differential WoS is timed
     in its production form (Philox streams, binary32 walks, the mesh's 1024-cell
     record grid for steps at least 2 eps clear of the mesh); its replay form
     (binary64, the reference's mt19937_64 streams) and the other production
@@ -705,8 +707,20 @@ def f4_bench(args) -> int:
         sites = w.sites[4:16]
         sites = sites[np.min(np.linalg.norm(sites[:, None] - c[None], axis=2), axis=1) > 0.06]
         walks = 65536
-        call = lambda: ctx.certify(geom, sites, seed=11, walks=walks, max_steps=128, eps=1e-6, radius_cap=0.1,
-                                   value_cap=0.06, source=twos.CERT_AFFINE, slope=0.65)
+
+        def cert(philox):
+            return ctx.certify(geom, sites, seed=11, walks=walks, max_steps=128, eps=1e-6, radius_cap=0.1,
+                               value_cap=0.06, source=twos.CERT_AFFINE, slope=0.65, philox=philox)
+
+        cert(False)
+        t0 = time.perf_counter()
+        replay = cert(False)
+        dt = time.perf_counter() - t0
+        forms["replay_f64_mt19937"] = {"samples_per_s": round(len(sites) * walks / dt, 1),
+                                       "walk_steps_per_s": round(float(replay[1].sum()) / dt, 1),
+                                       "ms": round(1e3 * dt, 2)}
+        log(f"replay_f64_mt19937: {dt:.3f} s")
+        call = lambda: cert(True)
     for i in range(args.warmup):
         call()
         log(f"warmup {i}")
@@ -737,6 +751,18 @@ def f4_bench(args) -> int:
         result["vs_exact_philox_f64"] = {"entries": int(len(z)), "max_abs_z": round(float(np.max(np.abs(z))), 3),
                                          "mean_z": round(float(np.mean(z)), 4),
                                          "beyond_3_se": int(np.sum(np.abs(z) > 3))}
+    else:
+        # Per site: the Philox and the reference-stream means of the certified
+        # value in their combined SE, and both forms' containment failures.
+        va, vb = out[0], replay[0]
+        se = np.hypot(va.std(axis=1), vb.std(axis=1)) / np.sqrt(walks)
+        z = (va.mean(axis=1) - vb.mean(axis=1)) / np.where(se > 0, se, np.inf)
+        result["config"]["form"] = "Philox streams, binary64 with directed rounding, one persistent launch"
+        result["forms"] = forms
+        result["x_replay"] = round(result["value"] / forms["replay_f64_mt19937"]["samples_per_s"], 1)
+        result["vs_replay"] = {"sites": int(len(z)), "max_abs_z": round(float(np.max(np.abs(z))), 3),
+                               "containment_failures": int(np.sum(out[2] & twos.CERT_CONTAINMENT)
+                                                           + np.sum(replay[2] & twos.CERT_CONTAINMENT))}
     ctx.release_geometry(geom)
     ctx.close()
     print(json.dumps(result))

@@ -387,7 +387,12 @@ typedef struct {
   double value_cap;      /* --validation-value-cap */
   double slope;          /* TWOS_CERT_AFFINE slope (--source-y-slope) */
   int32_t source;        /* TWOS_CERT_AFFINE or TWOS_CERT_HOUSING */
-  int32_t pad;
+  int32_t philox;        /* 0: the reference's mt19937_64 keys (one thread per
+                            sample); 1: Philox streams, the production form
+                            (one persistent launch; sample j of site s draws
+                            step k from counter (k, j, draw, 0) under the key
+                            of (splitmix64(seed + 0xa1224), s); sample_begin +
+                            walks at most 2^32) */
   int64_t site_offset;   /* global index of sites[0] in the keys */
   int64_t sample_begin;  /* index of the first sample of this call */
 } twos_cert_params;

@@ -363,22 +363,25 @@ def differential(geometry: Geometry, sites, side_, *, seed=0, repeat=0, walks=16
 class CertParams(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("walks", C.c_int32), ("max_steps", C.c_int32), ("eps", C.c_double),
                 ("radius_cap", C.c_double), ("value_cap", C.c_double), ("slope", C.c_double),
-                ("source", C.c_int32), ("threads", C.c_int32), ("site_offset", C.c_int64)]
+                ("source", C.c_int32), ("threads", C.c_int32), ("site_offset", C.c_int64), ("rng", C.c_int32),
+                ("pad", C.c_int32)]
 
 
 CERT_AFFINE, CERT_HOUSING = 0, 1
 
 
 def certify(geometry: Geometry, sites, *, seed=0, walks=1024, max_steps=128, eps=1e-6, radius_cap=0.1,
-            value_cap=0.1, source=CERT_AFFINE, slope=0.65, threads=0, site_offset=0):
+            value_cap=0.1, source=CERT_AFFINE, slope=0.65, threads=0, site_offset=0, rng=RNG_MT_KEY):
     """Certified minimum-benefit samples (cert_walk, minimum_benefit.hpp:122-143)
-    on the reference's streams: (value, steps, flags) as [n][walks] arrays,
-    flags 1 capped, 2 clipped, 4 containment failure."""
+    on the reference's streams (rng RNG_MT_KEY) or Philox streams
+    (RNG_PHILOX): (value, steps, flags) as [n][walks] arrays, flags 1 capped,
+    2 clipped, 4 containment failure."""
     sites = _f64(np.atleast_2d(sites))
     n = len(sites)
     g = geometry.struct()
     p = CertParams(seed=seed, walks=walks, max_steps=max_steps, eps=eps, radius_cap=radius_cap,
-                   value_cap=value_cap, slope=slope, source=source, threads=threads, site_offset=site_offset)
+                   value_cap=value_cap, slope=slope, source=source, threads=threads, site_offset=site_offset,
+                   rng=rng)
     value = np.zeros((n, walks))
     steps = np.zeros((n, walks), dtype=np.int32)
     flags = np.zeros((n, walks), dtype=np.int32)

@@ -92,6 +92,8 @@ int main() {
   std::vector<double> dv(2 * 8);
   std::vector<int64_t> dc(2 * 9);
   CHECK(!(og_differential(&g, &sides[0], &pd, sites, 2, 0.01, dv.data(), dc.data()) != 0));
+  pd.rng = OG_RNG_PHILOX;
+  CHECK(!(og_differential(&g, &sides[0], &pd, sites, 2, 0.01, dv.data(), dc.data()) != 0));
   og_cert_params pc;
   std::memset(&pc, 0, sizeof pc);
   pc.seed = 5;
@@ -104,10 +106,12 @@ int main() {
   pc.threads = 2;
   std::vector<double> cv(64);
   std::vector<int32_t> cs(64), cf(64);
-  for (int source : {0, 1}) {
-    pc.source = source;
-    CHECK(!(og_certify(&g, &pc, sites, 2, cv.data(), cs.data(), cf.data()) != 0));
-  }
+  for (int rng : {int(OG_RNG_MT_KEY), int(OG_RNG_PHILOX)})
+    for (int source : {0, 1}) {
+      pc.rng = rng;
+      pc.source = source;
+      CHECK(!(og_certify(&g, &pc, sites, 2, cv.data(), cs.data(), cf.data()) != 0));
+    }
   double cd[2];
   CHECK(!(og_cert_distance(&g, sites, 2, cd) != 0));
   CHECK(cd[0] >= 0.0 && cd[0] <= d_bvh);

@@ -1111,11 +1111,25 @@ double cert_source_lower(const V3 &p, int32_t source, double slope) {
   return std::max(0.0, value);
 }
 
+// Draws of a certified walk: the reference's mt19937_64 stream (the Beta draw,
+// then the Green point's direction, then the move's direction, in that order)
+// or a Philox stream (step k of sample j: the move's direction from counter
+// (k, j, 0, 0), the Green point's direction from (k, j, 1, 0) and its Beta
+// from (k, j, 2, 0), as the estimator's walks draw them).
+struct CertMt {
+  std::mt19937_64 rng;
+  V3 green(int32_t, double s) {
+    const double t = beta_22(rng);
+    return random_unit_vector(rng) * (s * t);
+  }
+  V3 dir(int32_t) { return random_unit_vector(rng); }
+};
+
 // cert_walk (:122-143).  flags: 1 capped, 2 clipped, 4 containment failure
 // (the walk stops there; its record must not be used).
-double cert_walk(const og_geom &g, const og_cert_params &p, V3 point, uint64_t key, int32_t &steps,
+template <class Stream>
+double cert_walk(const og_geom &g, const og_cert_params &p, V3 point, Stream &stream, int32_t &steps,
                  int32_t &flags) {
-  std::mt19937_64 rng(splitmix64(key));
   double value = 0.0;
   steps = 0;
   flags = 0;
@@ -1124,12 +1138,9 @@ double cert_walk(const og_geom &g, const og_cert_params &p, V3 point, uint64_t k
     if (d <= p.eps) return value;
     const double s = std::min(p.radius_cap, positive_down(0.95 * d));
     V3 y = point;
-    if (p.source == 1) {
-      // random_unit_vector(rng) * (s * beta_22(rng)): the Beta draw first.
-      const double t = beta_22(rng);
-      y = point + random_unit_vector(rng) * (s * t);
-    }
-    const V3 next = point + random_unit_vector(rng) * s;
+    // random_unit_vector(rng) * (s * beta_22(rng)): the Beta draw first.
+    if (p.source == 1) y = point + stream.green(k, s);
+    const V3 next = point + stream.dir(k) * s;
     if (!(cert_point_bound(point, next, true) < d) || !(cert_point_bound(point, y, true) < d)) {
       flags |= 4;
       return value;
@@ -1179,7 +1190,8 @@ int32_t og_certify(const og_geom *g, const og_cert_params *prm, const double *si
                    int32_t *steps, int32_t *flags) {
   if (!g || g->container != OG_MESH || !g->mesh || prm->walks < 1 || prm->max_steps < 1 ||
       !(prm->radius_cap > 0.0) || !(prm->value_cap > 0.0) || (prm->source != 0 && prm->source != 1) ||
-      (prm->source == 0 && prm->slope < 0.0))
+      (prm->source == 0 && prm->slope < 0.0) || (prm->rng != OG_RNG_PHILOX && prm->rng != OG_RNG_MT_KEY) ||
+      (prm->rng == OG_RNG_PHILOX && prm->walks > 0x7fffffff))
     return 2;
   const og_cert_params p = *prm;
   const int64_t jobs = n * int64_t(p.walks);
@@ -1189,10 +1201,19 @@ int32_t og_certify(const og_geom *g, const og_cert_params *prm, const double *si
 #endif
   for (int64_t t = 0; t < jobs; ++t) {
     const int64_t site = t / p.walks, j = t - site * p.walks;
-    // Key of sample j at site (minimum_benefit.hpp:168).
-    const uint64_t key = splitmix64(p.seed) ^ splitmix64(uint64_t(p.site_offset + site) + 0xa1224ULL) ^
-                         splitmix64(uint64_t(j) + 0x6242cULL);
-    value[t] = cert_walk(*g, p, load3(sites + 3 * site), key, steps[t], flags[t]);
+    if (p.rng == OG_RNG_PHILOX) {
+      // Philox: the key of (splitmix64(seed + 0xa1224), site, 0), sample j
+      // in the counter.
+      PhiloxStream st{{0u, 0u}, uint32_t(j)};
+      philox_key(splitmix64(p.seed + 0xa1224ULL), p.site_offset + site, 0, st.key);
+      value[t] = cert_walk(*g, p, load3(sites + 3 * site), st, steps[t], flags[t]);
+    } else {
+      // Key of sample j at site (minimum_benefit.hpp:168).
+      const uint64_t key = splitmix64(p.seed) ^ splitmix64(uint64_t(p.site_offset + site) + 0xa1224ULL) ^
+                           splitmix64(uint64_t(j) + 0x6242cULL);
+      CertMt st{std::mt19937_64(splitmix64(key))};
+      value[t] = cert_walk(*g, p, load3(sites + 3 * site), st, steps[t], flags[t]);
+    }
   }
   return 0;
 }

@@ -163,6 +163,10 @@ typedef struct {
   int32_t source;      /* 0 affine 1 + slope (y + 0.5), 1 housing chips */
   int32_t threads;
   int64_t site_offset; /* global index of sites[0] in the keys */
+  int32_t rng;         /* OG_RNG_MT_KEY: the reference's keys (minimum_benefit.hpp:168);
+                          OG_RNG_PHILOX: key philox_key(splitmix64(seed + 0xa1224), site, 0),
+                          counter (step, sample, draw, 0) */
+  int32_t pad;
 } og_cert_params;
 int32_t og_certify(const og_geom *g, const og_cert_params *prm, const double *sites, int64_t n, double *value,
                    int32_t *steps, int32_t *flags);

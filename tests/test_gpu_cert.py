@@ -14,7 +14,10 @@ run_minimum_benefit, benchmarks/compiled_bunny/src/minimum_benefit.hpp:
   reference does);
 - the certified clearance of every finalist bitwise;
 - a later block of samples (sample_begin) equals the same samples of one
-  large call, so stages can be run one at a time.
+  large call, so stages can be run one at a time;
+- the production form (Philox streams, one persistent launch): every
+  sample's steps and flags equal to the oracle's Philox form, values to
+  1e-10; results keyed by (seed, site, sample).
 """
 
 import numpy as np
@@ -40,16 +43,64 @@ def setup(ctx, golden):
     return meshes, CF.load(golden)
 
 
-def _certify(ctx, meshes, a, walks, sample_begin=0):
+def _certify(ctx, meshes, a, walks, sample_begin=0, points=None, site_offset=0, philox=False):
     g = ctx.geometry(twos.MESH, mesh=meshes[a.kind], cavities=a.holes if len(a.holes) else None,
                      cavity_radius=a.radius)
     try:
-        return ctx.certify(g, a.point, seed=a.seed, walks=walks, max_steps=a.max_steps, eps=a.epsilon,
-                           radius_cap=a.radius_cap, value_cap=a.value_cap,
+        return ctx.certify(g, a.point if points is None else points, seed=a.seed, walks=walks,
+                           max_steps=a.max_steps, eps=a.epsilon, radius_cap=a.radius_cap, value_cap=a.value_cap,
                            source=twos.CERT_AFFINE if a.kind == "bunny" else twos.CERT_HOUSING, slope=a.slope,
-                           sample_begin=sample_begin)
+                           sample_begin=sample_begin, site_offset=site_offset, philox=philox)
     finally:
         ctx.release_geometry(g)
+
+
+@pytest.fixture(scope="module")
+def oracle_meshes(golden):
+    O = pytest.importorskip("oracle")
+    return {k: O.Mesh(*CF.mesh(golden, k)) for k in ("bunny", "housing")}
+
+
+@pytest.mark.parametrize("index", [0, 7])
+def test_philox_samples_match_oracle(ctx, setup, oracle_meshes, index):
+    """The production form (Philox streams, one persistent launch) against the
+    oracle's Philox form on a retained finalist (the bunny with the affine
+    source; the housing with its chips and cavities): every sample's step
+    count and flags exactly, values to 1e-10 and most bitwise."""
+    import oracle as O
+    meshes, attempts = setup
+    a = attempts[index]
+    n = 4096
+    print("certify philox", a.source, flush=True)
+    value, steps, flags = _certify(ctx, meshes, a, n, philox=True)
+    geo = O.Geometry(O.MESH, mesh=oracle_meshes[a.kind], cavities=a.holes if len(a.holes) else None,
+                     cavity_radius=a.radius)
+    rv, rs, rf = O.certify(geo, a.point, seed=a.seed, walks=n, max_steps=a.max_steps, eps=a.epsilon,
+                           radius_cap=a.radius_cap, value_cap=a.value_cap,
+                           source=O.CERT_AFFINE if a.kind == "bunny" else O.CERT_HOUSING, slope=a.slope,
+                           threads=O.host_threads(), rng=O.RNG_PHILOX)
+    np.testing.assert_array_equal(steps, rs)
+    np.testing.assert_array_equal(flags, rf)
+    np.testing.assert_allclose(value, rv, rtol=1e-10, atol=0)
+    assert np.mean(value == rv) > 0.5
+    assert not np.any(flags & twos.CERT_CONTAINMENT)
+
+
+def test_philox_samples_are_keyed_by_index(ctx, setup):
+    """A Philox sample depends on (seed, site, sample) only: blocks of samples
+    (sample_begin) and a site offset give the records of one large call."""
+    meshes, attempts = setup
+    a = attempts[3]
+    pts = np.vstack([a.point, a.point + [0.01, 0.0, 0.0]])
+    full = _certify(ctx, meshes, a, 8192, points=pts, philox=True)
+    block = _certify(ctx, meshes, a, 2048, sample_begin=4096, points=pts, philox=True)
+    for x, y in zip(full, block):
+        np.testing.assert_array_equal(x[:, 4096:6144], y)
+    second = _certify(ctx, meshes, a, 8192, points=pts[1:], site_offset=1, philox=True)
+    for x, y in zip(full, second):
+        np.testing.assert_array_equal(x[1:], y)
+    ref = _certify(ctx, meshes, a, 8192, points=pts)
+    assert not np.array_equal(ref[0], full[0])  # the streams differ from the reference's
 
 
 def test_first_samples_of_every_attempt(ctx, setup):

@@ -83,3 +83,48 @@ def test_distance_is_the_box_union_minimum(golden):
         assert got[i] == np.min(pos_down(np.sqrt(s))), i
     exact = np.array([m.closest(p)[0] for p in pts])
     assert np.all(got <= exact)
+
+
+@pytest.mark.parametrize("index", [0, 7])
+def test_philox_streams_estimate_the_same_benefit(attempts, meshes, index):
+    """The Philox form (the GPU's production streams) differs from the
+    reference's only in its random numbers: on a retained finalist (the
+    bunny with the affine source; the housing with its chips and cavities)
+    the mean certified value, the mean step count and the capped / clipped
+    rates agree with the reference-pinned mt19937_64 form within their
+    standard errors, and no containment check fails.  Each sample is on its
+    own stream: a counter that ignored the sample index would give one value
+    and SE 0."""
+    a = attempts[index]
+    n = 4096
+    geo = _geometry(meshes, a)
+    kw = dict(seed=a.seed, walks=n, max_steps=a.max_steps, eps=a.epsilon, radius_cap=a.radius_cap,
+              value_cap=a.value_cap, source=O.CERT_AFFINE if a.kind == "bunny" else O.CERT_HOUSING,
+              slope=a.slope, threads=O.host_threads())
+    mv, ms, mf = O.certify(geo, a.point, **kw)
+    pv, ps, pf = O.certify(geo, a.point, rng=O.RNG_PHILOX, **kw)
+    assert not np.any(pf & 4) and not np.any(mf & 4)
+    for x, y in ((mv[0], pv[0]), (ms[0].astype(float), ps[0].astype(float)),
+                 ((mf[0] & 1).astype(float), (pf[0] & 1).astype(float)),
+                 (((mf[0] >> 1) & 1).astype(float), ((pf[0] >> 1) & 1).astype(float))):
+        se = np.hypot(np.std(x), np.std(y)) / np.sqrt(n)
+        assert abs(np.mean(x) - np.mean(y)) <= 4 * se + 1e-15, (np.mean(x), np.mean(y), se)
+    assert 0.7 < np.std(pv[0]) / np.std(mv[0]) < 1.4
+    assert len(np.unique(pv[0])) > n // 2
+
+
+def test_philox_streams_are_keyed_by_index(attempts, meshes):
+    """A Philox sample depends on (seed, site, sample) only: a prefix of the
+    samples, any thread count and a site offset give the same records."""
+    a = attempts[0]
+    geo = _geometry(meshes, a)
+    kw = dict(seed=a.seed, max_steps=a.max_steps, eps=a.epsilon, radius_cap=a.radius_cap, value_cap=a.value_cap,
+              source=O.CERT_AFFINE, slope=a.slope, rng=O.RNG_PHILOX)
+    pts = np.vstack([a.point, a.point + [0.01, 0.0, 0.0]])
+    full = O.certify(geo, pts, walks=256, threads=8, **kw)
+    head = O.certify(geo, pts, walks=64, threads=1, **kw)
+    for x, y in zip(full, head):
+        np.testing.assert_array_equal(x[:, :64], y)
+    second = O.certify(geo, pts[1:], walks=64, site_offset=1, **kw)
+    for x, y in zip(head, second):
+        np.testing.assert_array_equal(x[1:], y)

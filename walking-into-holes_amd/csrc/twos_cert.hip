@@ -14,8 +14,12 @@
 // monotone too, so the search compares rounded-down squared sums and takes
 // one root at the end.
 //
-// Work model.  One thread runs one (site, sample) walk and writes its value,
-// step count and flags (1 capped, 2 clipped, 4 containment failure).
+// Work model.  The reference's streams: one thread runs one (site, sample)
+// walk (cert_walk).  Philox streams (the production form): a persistent
+// kernel (cert_persistent).  Each walk writes its value, step count and flags
+// (1 capped, 2 clipped, 4 containment failure).
+
+#include <algorithm>
 
 #include "twos_device.h"
 #include "twos_launch.h"
@@ -59,9 +63,18 @@ __device__ __forceinline__ double tri_sum_lower(const MeshView &m, int32_t k, V3
 }
 
 // CertDistanceTree::distance: the smallest rounded-down triangle-box distance.
-__device__ double cert_mesh_distance(const MeshView &m, V3<double> p) {
+// Warm start: the search starts from `hint`'s value (the previous step's
+// nearest triangle box; a walk moves by less than its distance, so this bound
+// is tight) and leaves the nearest box's triangle there.  The minimum over all
+// triangles does not depend on where the search starts.
+__device__ double cert_mesh_distance(const MeshView &m, V3<double> p, int32_t &hint) {
   double best = __builtin_huge_val();
   if (m.n_tris == 1) return pdn(sqrt(tri_sum_lower(m, 0, p)));
+  int32_t best_k = -1;
+  if (hint >= 0 && hint < m.n_tris) {
+    best = tri_sum_lower(m, hint, p);
+    best_k = hint;
+  }
   int32_t stk_node[kStackDepth];
   double stk_d[kStackDepth];
   int sp = 0;
@@ -78,7 +91,13 @@ __device__ double cert_mesh_distance(const MeshView &m, V3<double> p) {
     int32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      if (c[i] < 0 && c[i] != kEmptyChild && d[i] < best) best = fmin(best, tri_sum_lower(m, ~c[i], p));
+      if (c[i] < 0 && c[i] != kEmptyChild && d[i] < best) {
+        const double v = tri_sum_lower(m, ~c[i], p);
+        if (v < best) {
+          best = v;
+          best_k = ~c[i];
+        }
+      }
     double key[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) key[i] = (c[i] >= 0 && d[i] < best) ? d[i] : __builtin_huge_val();
@@ -112,6 +131,7 @@ __device__ double cert_mesh_distance(const MeshView &m, V3<double> p) {
     }
     if (!found) break;
   }
+  hint = best_k;
   return pdn(sqrt(best));
 }
 
@@ -129,9 +149,9 @@ __device__ __forceinline__ double point_bound(V3<double> a, V3<double> b, bool u
 }
 
 // cert_domain_distance (:83-88).
-__device__ double cert_domain_distance(const DevGeom &g, V3<double> p) {
+__device__ double cert_domain_distance(const DevGeom &g, V3<double> p, int32_t &hint) {
   const MeshView m{g.nodes, g.nodes4, g.tris_f, g.tris_d, g.tri_ids, g.n_tris};
-  double d = cert_mesh_distance(m, p);
+  double d = cert_mesh_distance(m, p, hint);
   for (int32_t i = 0; i < g.n_cavities; ++i) {
     const Vec4<double> s = load4<double>(g, 2, i);
     d = fmin(d, dn(point_bound(p, V3<double>{s.x, s.y, s.z}, false) - g.cavity_radius));
@@ -178,7 +198,27 @@ struct CertLaunch {
   int32_t max_steps;
   int32_t source;
   double eps, radius_cap, value_cap, slope;
+  unsigned long long *counter;  // cert_persistent: job tickets, zero before the launch
 };
+
+// One step of a certified walk at `point` with certified distance d > eps:
+// the step radius s, the source's Green point y (the housing chips only) and
+// the next point from the step's draws; returns false when a point cannot be
+// shown inside the ball (containment failure), else adds the step's
+// rounded-down source term to value (cert_walk, :126-141).
+__device__ __forceinline__ bool cert_step(const CertLaunch &p, V3<double> &point, double d, V3<double> green_dir,
+                                          double green_t, V3<double> dir, double &value) {
+  const double s = fmin(p.radius_cap, pdn(0.95 * d));
+  V3<double> y = point;
+  if (p.source == TWOS_CERT_HOUSING) y = point + green_dir * (s * green_t);
+  const V3<double> next = point + dir * s;
+  if (!(point_bound(point, next, true) < d) || !(point_bound(point, y, true) < d)) return false;
+  const double weight = pdn(pdn(s * s) / 6.0);
+  const double increment = pdn(weight * cert_source_lower(y, p.source, p.slope));
+  value = pdn(value + increment);
+  point = next;
+  return true;
+}
 
 // cert_walk (:122-143), sample j = sample_begin + job % count of site
 // job / count, key (:168).
@@ -194,45 +234,136 @@ __global__ __launch_bounds__(64) void cert_walk(const DevGeom *geom, CertLaunch 
   mt.seed(splitmix64(key));
   V3<double> point{p.sites[3 * site], p.sites[3 * site + 1], p.sites[3 * site + 2]};
   double value = 0.0;
-  int32_t steps = 0, flags = 1;
+  int32_t steps = 0, flags = 1, hint = -1;
   for (int32_t k = 0; k < p.max_steps; ++k) {
-    const double d = cert_domain_distance(g, point);
+    const double d = cert_domain_distance(g, point, hint);
     if (d <= p.eps) {
       flags = 0;
       break;
     }
-    const double s = fmin(p.radius_cap, pdn(0.95 * d));
-    V3<double> y = point;
+    // random_unit_vector(rng) * (s * beta_22(rng)): the Beta draw first.
+    double t = 0.0;
+    V3<double> gdir{0.0, 0.0, 0.0};
     if (p.source == TWOS_CERT_HOUSING) {
-      // random_unit_vector(rng) * (s * beta_22(rng)): the Beta draw first.
-      const double t = mt.beta22();
-      y = point + mt.direction() * (s * t);
+      t = mt.beta22();
+      gdir = mt.direction();
     }
-    const V3<double> next = point + mt.direction() * s;
-    if (!(point_bound(point, next, true) < d) || !(point_bound(point, y, true) < d)) {
+    if (!cert_step(p, point, d, gdir, t, mt.direction(), value)) {
       flags = 4;
       break;
     }
-    const double weight = pdn(pdn(s * s) / 6.0);
-    const double increment = pdn(weight * cert_source_lower(y, p.source, p.slope));
-    value = pdn(value + increment);
     ++steps;
     if (value >= p.value_cap) {
       value = p.value_cap;
       flags = 2;
       break;
     }
-    point = next;
   }
   p.value[job] = value;
   p.steps[job] = steps;
   p.flags[job] = flags;
 }
 
+// Production form on Philox streams: sample j of site s draws the move's
+// direction of step k from counter (k, j, 0, 0) under the key of
+// (splitmix64(seed + 0xa1224), s, 0), the Green point's direction from
+// (k, j, 1, 0) and its Beta(2,2) (the median of three 24-bit uniforms) from
+// (k, j, 2, 0).  Each wave takes jobs in chunks of 64 tickets and each loop
+// iteration advances every lane by one step of its walk, so lanes never wait
+// for the wave's longest sample; every walk ends within max_steps steps.
+constexpr int kCertBlock = 256;
+
+__device__ __forceinline__ double cert_beta22(uint2 key, uint32_t j, uint32_t k) {
+  const uint4 w = philox4x32_10(make_uint4(k, j, 2u, 0u), key);
+  const double a = double(w.x >> 8) * 0x1.0p-24, b = double(w.y >> 8) * 0x1.0p-24, c = double(w.z >> 8) * 0x1.0p-24;
+  return fmax(fmin(a, b), fmin(fmax(a, b), c));
+}
+
+__global__ __launch_bounds__(kCertBlock) void cert_persistent(const DevGeom *geom, CertLaunch p, uint64_t cseed) {
+  const DevGeom &g = *geom;
+  const int lane = int(threadIdx.x & 63);
+  int64_t chunk_cur = 0, chunk_end = 0;  // wave-uniform
+  bool drained = false;                  // wave-uniform
+  bool active = false;
+  int64_t job = 0;
+  uint32_t j = 0;
+  uint2 key = make_uint2(0u, 0u);
+  V3<double> point{0.0, 0.0, 0.0};
+  double value = 0.0;
+  int32_t steps = 0, hint = -1;
+  while (true) {
+    uint64_t need = __ballot(!active);
+    while (need != 0 && !drained) {
+      if (chunk_cur >= chunk_end) {
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(p.counter, 64ull);
+        base = __shfl(base, 0);
+        if (int64_t(base) >= p.jobs) {
+          drained = true;
+          break;
+        }
+        chunk_cur = int64_t(base);
+        chunk_end = min(int64_t(base) + 64, p.jobs);
+      }
+      const int take = int(min<int64_t>(chunk_end - chunk_cur, __popcll(need)));
+      const int rank = __popcll(need & ((1ull << lane) - 1ull));
+      if (!active && rank < take) {
+        job = chunk_cur + rank;
+        const int64_t site = job / p.count;
+        j = uint32_t(p.sample_begin + (job - site * p.count));
+        const uint64_t k = splitmix64(splitmix64(cseed) + (uint64_t(p.site_offset + site) << 1));
+        key = make_uint2(uint32_t(k), uint32_t(k >> 32));
+        point = V3<double>{p.sites[3 * site], p.sites[3 * site + 1], p.sites[3 * site + 2]};
+        value = 0.0;
+        steps = 0;
+        hint = -1;
+        active = true;
+      }
+      chunk_cur += take;
+      need = __ballot(!active);
+    }
+    if (!__any(active)) break;
+    if (!active) continue;
+    int32_t flags = -1;  // -1: the walk goes on
+    if (steps >= p.max_steps) {
+      flags = 1;
+    } else {
+      const double d = cert_domain_distance(g, point, hint);
+      if (d <= p.eps) {
+        flags = 0;
+      } else {
+        double t = 0.0;
+        V3<double> gdir{0.0, 0.0, 0.0};
+        if (p.source == TWOS_CERT_HOUSING) {
+          t = cert_beta22(key, j, uint32_t(steps));
+          gdir = philox_direction(philox4x32_10(make_uint4(uint32_t(steps), j, 1u, 0u), key), 0.0);
+        }
+        const V3<double> dir = philox_direction(philox4x32_10(make_uint4(uint32_t(steps), j, 0u, 0u), key), 0.0);
+        if (!cert_step(p, point, d, gdir, t, dir, value)) {
+          flags = 4;
+        } else {
+          ++steps;
+          if (value >= p.value_cap) {
+            value = p.value_cap;
+            flags = 2;
+          }
+        }
+      }
+    }
+    if (flags >= 0) {
+      p.value[job] = value;
+      p.steps[job] = steps;
+      p.flags[job] = flags;
+      active = false;
+    }
+  }
+}
+
 __global__ void cert_distance_kernel(const DevGeom *geom, const double *pts, int64_t n, double *out) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  out[i] = cert_domain_distance(*geom, V3<double>{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]});
+  int32_t hint = -1;
+  out[i] = cert_domain_distance(*geom, V3<double>{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]}, hint);
 }
 
 inline unsigned blocks_for(int64_t n, int b) { return unsigned((n + b - 1) / b); }
@@ -241,7 +372,7 @@ inline unsigned blocks_for(int64_t n, int b) { return unsigned((n + b - 1) / b);
 
 hipError_t launch_certify(const DevGeom *d_geom, const twos_cert_params &prm, const double *d_sites, int64_t n_sites,
                           int64_t sample_begin, int64_t count, double *d_value, int32_t *d_steps, int32_t *d_flags,
-                          hipStream_t stream) {
+                          unsigned long long *counter, int cus, hipStream_t stream) {
   CertLaunch p{};
   p.sites = d_sites;
   p.value = d_value;
@@ -259,6 +390,20 @@ hipError_t launch_certify(const DevGeom *d_geom, const twos_cert_params &prm, co
   p.value_cap = prm.value_cap;
   p.slope = prm.slope;
   if (p.jobs <= 0) return hipSuccess;
+  if (prm.philox) {
+    p.counter = counter;
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cert_persistent, kCertBlock, 0);
+    if (e != hipSuccess) return e;
+    const int64_t chunks = (p.jobs + 63) / 64;
+    const int64_t blocks = std::min<int64_t>(int64_t(std::max(1, per_cu)) * std::max(1, cus),
+                                             (chunks + kCertBlock / 64 - 1) / (kCertBlock / 64));
+    e = hipMemsetAsync(counter, 0, sizeof(unsigned long long), stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(cert_persistent, dim3(unsigned(blocks)), dim3(kCertBlock), 0, stream, d_geom, p,
+                       splitmix64(prm.seed + 0xa1224ULL));
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(cert_walk, dim3(blocks_for(p.jobs, 64)), dim3(64), 0, stream, d_geom, p);
   return hipGetLastError();
 }

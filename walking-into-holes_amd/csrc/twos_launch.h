@@ -60,9 +60,12 @@ hipError_t launch_walk_persistent_f64(int geom_kind, bool count, const Tables &t
 hipError_t launch_reduce(const Tables &tb, const WalkParams &wp, int grid, hipStream_t stream);
 hipError_t launch_zero(unsigned long long *p, int n, hipStream_t stream);
 // Certified sampling of n_sites sites, samples sample_begin .. + count - 1.
+// prm.philox: Philox streams in one persistent launch over the jobs (counter: one
+// device word for its tickets; cus: the device's compute units); otherwise
+// the reference's streams, one thread per job.
 hipError_t launch_certify(const DevGeom *d_geom, const twos_cert_params &prm, const double *d_sites, int64_t n_sites,
                           int64_t sample_begin, int64_t count, double *d_value, int32_t *d_steps, int32_t *d_flags,
-                          hipStream_t stream);
+                          unsigned long long *counter, int cus, hipStream_t stream);
 hipError_t launch_cert_distance(const DevGeom *d_geom, const double *d_pts, int64_t n, double *d_out,
                                 hipStream_t stream);
 // Differential WoS over one chunk of sites (p.site_base, n_sites sites) on

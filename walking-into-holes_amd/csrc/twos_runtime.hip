@@ -1975,6 +1975,9 @@ twos_status twos_certify(twos_ctx *ctx, int32_t geom_id, const twos_cert_params 
     return fail(TWOS_ERR_INVALID_ARGUMENT, "unknown certified source");
   if (p.source == TWOS_CERT_AFFINE && !(p.slope >= 0.0))
     return fail(TWOS_ERR_INVALID_ARGUMENT, "validator expects nonnegative affine slope");
+  if (p.philox != 0 && p.philox != 1) return fail(TWOS_ERR_INVALID_ARGUMENT, "philox must be 0 or 1");
+  if (p.philox && p.sample_begin + int64_t(p.walks) > (int64_t(1) << 32))
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "Philox sample indices must fit in 32 bits");
   for (int64_t i = 0; i < 3 * n; ++i)
     if (!std::isfinite(sites[i])) return fail(TWOS_ERR_INVALID_ARGUMENT, "non-finite site coordinate");
   std::lock_guard<std::mutex> lock(ctx->mu);
@@ -1988,20 +1991,23 @@ twos_status twos_certify(twos_ctx *ctx, int32_t geom_id, const twos_cert_params 
   struct Buffers {
     double *sites = nullptr, *value = nullptr;
     int32_t *steps = nullptr, *flags = nullptr;
-    ~Buffers() { dfree(sites); dfree(value); dfree(steps); dfree(flags); }
+    unsigned long long *counter = nullptr;
+    ~Buffers() { dfree(sites); dfree(value); dfree(steps); dfree(flags); dfree(counter); }
   } b;
   HIP_OK(dalloc(&b.sites, 3 * size_t(n)));
   HIP_OK(dalloc(&b.value, size_t(rows) * size_t(p.walks)));
   HIP_OK(dalloc(&b.steps, size_t(rows) * size_t(p.walks)));
   HIP_OK(dalloc(&b.flags, size_t(rows) * size_t(p.walks)));
+  HIP_OK(dalloc(&b.counter, 1));
   HIP_OK(hipMemcpy(b.sites, sites, sizeof(double) * 3 * size_t(n), hipMemcpyHostToDevice));
+  const int cus = ctx->prop.multiProcessorCount;
   for (int64_t s0 = 0; s0 < n; s0 += chunk) {
     const int64_t cnt = std::min(chunk, n - s0);
     twos_cert_params q = p;
     q.site_offset = p.site_offset + s0;
     const size_t m = size_t(cnt) * size_t(p.walks), o = size_t(s0) * size_t(p.walks);
     HIP_OK(launch_certify(ctx->d_geoms + geom_id, q, b.sites + 3 * s0, cnt, p.sample_begin, p.walks, b.value,
-                          b.steps, b.flags, ctx->stream));
+                          b.steps, b.flags, b.counter, cus, ctx->stream));
     HIP_OK(hipMemcpyAsync(value + o, b.value, sizeof(double) * m, hipMemcpyDeviceToHost, ctx->stream));
     HIP_OK(hipMemcpyAsync(steps + o, b.steps, sizeof(int32_t) * m, hipMemcpyDeviceToHost, ctx->stream));
     HIP_OK(hipMemcpyAsync(flags + o, b.flags, sizeof(int32_t) * m, hipMemcpyDeviceToHost, ctx->stream));

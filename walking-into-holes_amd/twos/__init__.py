@@ -461,16 +461,18 @@ class Context:
         return out[:, :k], counts
 
     def certify(self, geom: int, sites, *, seed=0, walks=1024, max_steps=128, eps=1e-6, radius_cap=0.1,
-                value_cap=0.1, source=_capi.CERT_AFFINE, slope=0.65, site_offset=0, sample_begin=0):
+                value_cap=0.1, source=_capi.CERT_AFFINE, slope=0.65, site_offset=0, sample_begin=0, philox=False):
         """Certified minimum-benefit samples (cert_walk, minimum_benefit.hpp:
-        122-143) on the reference's streams: (value, steps, flags), [n][walks]
-        each, for samples sample_begin .. sample_begin + walks - 1; flags
-        CERT_CAPPED, CERT_CLIPPED, CERT_CONTAINMENT."""
+        122-143): (value, steps, flags), [n][walks] each, for samples
+        sample_begin .. sample_begin + walks - 1; flags CERT_CAPPED,
+        CERT_CLIPPED, CERT_CONTAINMENT.  On the reference's mt19937_64
+        streams, or with philox=True on Philox streams in one persistent
+        launch (the production form)."""
         p = _f64(sites).reshape(-1, 3)
         n = len(p)
         prm = _capi.CertParams(seed=seed, walks=walks, max_steps=max_steps, eps=eps, radius_cap=radius_cap,
                                value_cap=value_cap, slope=slope, source=source, site_offset=site_offset,
-                               sample_begin=sample_begin)
+                               sample_begin=sample_begin, philox=int(bool(philox)))
         value = np.zeros((n, walks))
         steps = np.zeros((n, walks), dtype=np.int32)
         flags = np.zeros((n, walks), dtype=np.int32)

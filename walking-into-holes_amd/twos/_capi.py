@@ -53,7 +53,7 @@ class GeometryDesc(C.Structure):
 class CertParams(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("walks", C.c_int32), ("max_steps", C.c_int32), ("eps", C.c_double),
                 ("radius_cap", C.c_double), ("value_cap", C.c_double), ("slope", C.c_double),
-                ("source", C.c_int32), ("pad", C.c_int32), ("site_offset", C.c_int64),
+                ("source", C.c_int32), ("philox", C.c_int32), ("site_offset", C.c_int64),
                 ("sample_begin", C.c_int64)]
 
 
