@@ -657,8 +657,10 @@ def f4_bench(args) -> int:
     65,536 samples, the affine source, radius cap 0.1, value cap 0.06, eps
     1e-6, 128 steps).  A step is one call; the metric is samples per second.
     Certified sampling is timed in its production form (Philox streams, one
-    persistent launch, binary64 with directed rounding), its replay form (the
-    reference's mt19937_64 streams) once beside it.  Differential WoS is timed
+    persistent launch, binary64 with directed rounding, the mesh's 256-cell
+    certified grid for steps at least two cells from the mesh), its replay
+    form (the reference's mt19937_64 streams) and the exact Philox form once
+    beside it.  Differential WoS is timed
     in its production form (Philox streams, binary32 walks, the mesh's 1024-cell
     record grid for steps at least 2 eps clear of the mesh); its replay form
     (binary64, the reference's mt19937_64 streams) and the other production
@@ -708,19 +710,29 @@ def f4_bench(args) -> int:
         sites = sites[np.min(np.linalg.norm(sites[:, None] - c[None], axis=2), axis=1) > 0.06]
         walks = 65536
 
-        def cert(philox):
+        def cert(philox, grid_cells=0):
             return ctx.certify(geom, sites, seed=11, walks=walks, max_steps=128, eps=1e-6, radius_cap=0.1,
-                               value_cap=0.06, source=twos.CERT_AFFINE, slope=0.65, philox=philox)
+                               value_cap=0.06, source=twos.CERT_AFFINE, slope=0.65, philox=philox,
+                               grid_cells=grid_cells)
 
-        cert(False)
+        def cert_once(name, philox, grid_cells=0):
+            cert(philox, grid_cells)
+            t0 = time.perf_counter()
+            out_ = cert(philox, grid_cells)
+            dt = time.perf_counter() - t0
+            forms[name] = {"samples_per_s": round(len(sites) * walks / dt, 1),
+                           "walk_steps_per_s": round(float(out_[1].sum()) / dt, 1), "ms": round(1e3 * dt, 2)}
+            log(f"{name}: {dt:.3f} s")
+            return out_
+
+        replay = cert_once("replay_f64_mt19937", False)
+        cert_once("philox_f64_exact", True)
+        cert_once("philox_f64_grid128", True, 128)
+        cert_once("philox_f64_grid512", True, 512)
         t0 = time.perf_counter()
-        replay = cert(False)
-        dt = time.perf_counter() - t0
-        forms["replay_f64_mt19937"] = {"samples_per_s": round(len(sites) * walks / dt, 1),
-                                       "walk_steps_per_s": round(float(replay[1].sum()) / dt, 1),
-                                       "ms": round(1e3 * dt, 2)}
-        log(f"replay_f64_mt19937: {dt:.3f} s")
-        call = lambda: cert(True)
+        ctx.cert_grid_lower(mesh, 256, sites[:1])
+        forms["cert_grid_build_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
+        call = lambda: cert(True, 256)
     for i in range(args.warmup):
         call()
         log(f"warmup {i}")
@@ -757,7 +769,8 @@ def f4_bench(args) -> int:
         va, vb = out[0], replay[0]
         se = np.hypot(va.std(axis=1), vb.std(axis=1)) / np.sqrt(walks)
         z = (va.mean(axis=1) - vb.mean(axis=1)) / np.where(se > 0, se, np.inf)
-        result["config"]["form"] = "Philox streams, binary64 with directed rounding, one persistent launch"
+        result["config"]["form"] = ("Philox streams, binary64 with directed rounding, one persistent launch, "
+                                    "256-cell certified grid for steps at least 2 cells from the mesh")
         result["forms"] = forms
         result["x_replay"] = round(result["value"] / forms["replay_f64_mt19937"]["samples_per_s"], 1)
         result["vs_replay"] = {"sites": int(len(z)), "max_abs_z": round(float(np.max(np.abs(z))), 3),

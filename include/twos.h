@@ -395,6 +395,13 @@ typedef struct {
                             walks at most 2^32) */
   int64_t site_offset;   /* global index of sites[0] in the keys */
   int64_t sample_begin;  /* index of the first sample of this call */
+  int32_t grid_cells;    /* philox only: 0, or 8..512 -- a step whose certified
+                            grid bound (the mesh's corner grid with this many
+                            cells along its longest axis; built once per mesh
+                            and resolution) is at least grid_kappa cells takes
+                            it as its mesh distance instead of the exact
+                            box-union query.  No counterpart in the reference. */
+  float grid_kappa;      /* 0 = 2 cells; the threshold is at least 2 eps */
 } twos_cert_params;
 
 /* geom_id: a mesh container minus cavities only.  value, steps, flags:
@@ -406,6 +413,13 @@ twos_status twos_certify(twos_ctx *ctx, int32_t geom_id, const twos_cert_params 
  * (cert_domain_distance, minimum_benefit.hpp:83-88; the summary's
  * center_clearance_lower). */
 twos_status twos_cert_distance(twos_ctx *ctx, int32_t geom_id, const double *points, int64_t n, double *out);
+/* The certified grid's lower bound of the mesh's box-union distance at each
+ * point (the grid with `cells` cells along the mesh's longest axis, built if
+ * the mesh does not hold it yet): max over the point's cell corners c of
+ * down(v(c) - up(|p - c|)).  No counterpart in the reference: a diagnostic of
+ * twos_cert_params.grid_cells. */
+twos_status twos_cert_grid_lower(twos_ctx *ctx, int32_t mesh_id, int32_t cells, const double *points, int64_t n,
+                                 double *out);
 
 /* Continuous-batching server: requests are admitted between rounds as device
  * memory allows, all admitted sites share each persistent launch, and each

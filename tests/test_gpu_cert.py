@@ -17,7 +17,9 @@ run_minimum_benefit, benchmarks/compiled_bunny/src/minimum_benefit.hpp:
   large call, so stages can be run one at a time;
 - the production form (Philox streams, one persistent launch): every
   sample's steps and flags equal to the oracle's Philox form, values to
-  1e-10; results keyed by (seed, site, sample).
+  1e-10; results keyed by (seed, site, sample);
+- the certified grid: its bound never above the exact certified distance,
+  and walks that use it agree with the exact walk within SE.
 """
 
 import numpy as np
@@ -43,14 +45,15 @@ def setup(ctx, golden):
     return meshes, CF.load(golden)
 
 
-def _certify(ctx, meshes, a, walks, sample_begin=0, points=None, site_offset=0, philox=False):
+def _certify(ctx, meshes, a, walks, sample_begin=0, points=None, site_offset=0, philox=False, grid_cells=0):
     g = ctx.geometry(twos.MESH, mesh=meshes[a.kind], cavities=a.holes if len(a.holes) else None,
                      cavity_radius=a.radius)
     try:
         return ctx.certify(g, a.point if points is None else points, seed=a.seed, walks=walks,
                            max_steps=a.max_steps, eps=a.epsilon, radius_cap=a.radius_cap, value_cap=a.value_cap,
                            source=twos.CERT_AFFINE if a.kind == "bunny" else twos.CERT_HOUSING, slope=a.slope,
-                           sample_begin=sample_begin, site_offset=site_offset, philox=philox)
+                           sample_begin=sample_begin, site_offset=site_offset, philox=philox,
+                           grid_cells=grid_cells)
     finally:
         ctx.release_geometry(g)
 
@@ -86,6 +89,57 @@ def test_philox_samples_match_oracle(ctx, setup, oracle_meshes, index):
     assert not np.any(flags & twos.CERT_CONTAINMENT)
 
 
+@pytest.mark.parametrize("kind", ["bunny", "housing"])
+def test_cert_grid_is_conservative(ctx, setup, golden, kind):
+    """The certified grid's bound never exceeds the exact certified box-union
+    distance (twos_cert_distance on the bare mesh), in and around the mesh,
+    at two resolutions; inside the grid (the mesh's bounds plus one cell) and
+    far from the mesh it is within two cell diagonals of it (the grid is
+    used, not just safe; outside the grid the nearest cell's corners give
+    a looser bound)."""
+    meshes, _ = setup
+    v, _ = CF.mesh(golden, kind)
+    lo, hi = v.min(axis=0), v.max(axis=0)
+    rng = np.random.default_rng(11)
+    pts = lo - 0.1 * (hi - lo) + rng.random((20000, 3)) * 1.2 * (hi - lo)
+    g = ctx.geometry(twos.MESH, mesh=meshes[kind])
+    try:
+        exact = ctx.cert_distance(g, pts)
+    finally:
+        ctx.release_geometry(g)
+    for cells in (64, 256):
+        print("cert grid", kind, cells, flush=True)
+        lb = ctx.cert_grid_lower(meshes[kind], cells, pts)
+        assert np.all(lb <= exact), (cells, np.max(lb - exact))
+        h = np.max(hi - lo) / cells
+        far = (exact > 4 * h) & np.all((pts >= lo) & (pts <= hi), axis=1)
+        assert far.sum() > 1000
+        assert np.all(lb[far] >= exact[far] - 2 * np.sqrt(3) * h), cells
+
+
+@pytest.mark.parametrize("index", [0, 7])
+def test_cert_grid_walks_against_exact(ctx, setup, index):
+    """Certified walks that take the grid's bound far from the mesh: other
+    paths (more, smaller steps: the bound is below the exact certified
+    distance), the same certified benefit and clipped rate within the
+    combined SE of the exact walk, no containment failure, and no fewer walks
+    reaching max_steps (smaller steps cap more walks)."""
+    meshes, attempts = setup
+    a = attempts[index]
+    n = 65536
+    print("certify philox grid", a.source, flush=True)
+    ev, es, ef = _certify(ctx, meshes, a, n, philox=True)
+    gv, gs, gf = _certify(ctx, meshes, a, n, philox=True, grid_cells=256)
+    assert not np.array_equal(gs, es)  # the grid was used
+    assert not np.any(gf & twos.CERT_CONTAINMENT)
+    for x, y in ((ev[0], gv[0]), (((ef[0] >> 1) & 1).astype(float), ((gf[0] >> 1) & 1).astype(float))):
+        se = np.hypot(np.std(x), np.std(y)) / np.sqrt(n)
+        assert abs(np.mean(x) - np.mean(y)) <= 4 * se + 1e-15, (np.mean(x), np.mean(y), se)
+    ce, cg = (ef[0] & 1).astype(float), (gf[0] & 1).astype(float)
+    assert np.mean(cg) >= np.mean(ce) - 4 * np.hypot(np.std(ce), np.std(cg)) / np.sqrt(n)
+    assert np.mean(gs) >= np.mean(es)
+
+
 def test_philox_samples_are_keyed_by_index(ctx, setup):
     """A Philox sample depends on (seed, site, sample) only: blocks of samples
     (sample_begin) and a site offset give the records of one large call."""
@@ -101,6 +155,10 @@ def test_philox_samples_are_keyed_by_index(ctx, setup):
         np.testing.assert_array_equal(x[1:], y)
     ref = _certify(ctx, meshes, a, 8192, points=pts)
     assert not np.array_equal(ref[0], full[0])  # the streams differ from the reference's
+    gfull = _certify(ctx, meshes, a, 8192, points=pts, philox=True, grid_cells=128)
+    gblock = _certify(ctx, meshes, a, 2048, sample_begin=4096, points=pts, philox=True, grid_cells=128)
+    for x, y in zip(gfull, gblock):
+        np.testing.assert_array_equal(x[:, 4096:6144], y)
 
 
 def test_first_samples_of_every_attempt(ctx, setup):

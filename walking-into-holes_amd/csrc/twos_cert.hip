@@ -148,10 +148,43 @@ __device__ __forceinline__ double point_bound(V3<double> a, V3<double> b, bool u
   return upper ? upw(sqrt(sum)) : pdn(sqrt(sum));
 }
 
-// cert_domain_distance (:83-88).
-__device__ double cert_domain_distance(const DevGeom &g, V3<double> p, int32_t &hint) {
+// Certified grid.  The exact box-union distance B is 1-Lipschitz, and a
+// corner value v(c) = down(D(c)) <= B(c), so for any corner c,
+// B(p) >= v(c) - |p - c| >= down(v(c) - up(|p - c|)).  The bound takes the best
+// of the eight corners of p's cell (any cell gives a valid bound, so the index
+// only has to be in range).  Corners are recomputed from the same binary64
+// expression in the build and the lookup (this file has no contraction), so
+// a lookup measures from the exact point the value belongs to.
+__device__ __forceinline__ V3<double> cert_corner(const CertGrid &cg, int i, int j, int k) {
+  return {cg.lo[0] + double(i) * cg.h, cg.lo[1] + double(j) * cg.h, cg.lo[2] + double(k) * cg.h};
+}
+
+__device__ __forceinline__ int cert_cell(double x, double lo, double inv_h, int corners) {
+  return int(fmin(fmax(floor((x - lo) * inv_h), 0.0), double(corners - 2)));
+}
+
+__device__ double cert_grid_lower(const CertGrid &cg, V3<double> p) {
+  const int i = cert_cell(p.x, cg.lo[0], cg.inv_h, cg.n[0]);
+  const int j = cert_cell(p.y, cg.lo[1], cg.inv_h, cg.n[1]);
+  const int k = cert_cell(p.z, cg.lo[2], cg.inv_h, cg.n[2]);
+  double best = -__builtin_huge_val();
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const double v = double(cg.d[(size_t(i + a) * cg.n[1] + size_t(j + b)) * cg.n[2] + size_t(k + c)]);
+        best = fmax(best, dn(v - point_bound(p, cert_corner(cg, i + a, j + b, k + c), true)));
+      }
+  return best;
+}
+
+// cert_domain_distance (:83-88).  mesh_lb >= 0: a certified lower bound of the
+// mesh's box-union distance to take instead of the exact query.
+__device__ double cert_domain_distance(const DevGeom &g, V3<double> p, int32_t &hint, double mesh_lb = -1.0) {
   const MeshView m{g.nodes, g.nodes4, g.tris_f, g.tris_d, g.tri_ids, g.n_tris};
-  double d = cert_mesh_distance(m, p, hint);
+  double d = mesh_lb >= 0.0 ? mesh_lb : cert_mesh_distance(m, p, hint);
   for (int32_t i = 0; i < g.n_cavities; ++i) {
     const Vec4<double> s = load4<double>(g, 2, i);
     d = fmin(d, dn(point_bound(p, V3<double>{s.x, s.y, s.z}, false) - g.cavity_radius));
@@ -279,7 +312,11 @@ __device__ __forceinline__ double cert_beta22(uint2 key, uint32_t j, uint32_t k)
   return fmax(fmin(a, b), fmin(fmax(a, b), c));
 }
 
-__global__ __launch_bounds__(kCertBlock) void cert_persistent(const DevGeom *geom, CertLaunch p, uint64_t cseed) {
+// With a certified grid (grid.d), a step whose grid bound is at least
+// grid.tau (> eps, so such a step never ends a walk) takes it as the mesh's
+// distance; the cavities stay exact, and every walk ends on an exact query.
+__global__ __launch_bounds__(kCertBlock) void cert_persistent(const DevGeom *geom, CertLaunch p, uint64_t cseed,
+                                                              CertGrid grid) {
   const DevGeom &g = *geom;
   const int lane = int(threadIdx.x & 63);
   int64_t chunk_cur = 0, chunk_end = 0;  // wave-uniform
@@ -328,7 +365,8 @@ __global__ __launch_bounds__(kCertBlock) void cert_persistent(const DevGeom *geo
     if (steps >= p.max_steps) {
       flags = 1;
     } else {
-      const double d = cert_domain_distance(g, point, hint);
+      const double lb = grid.d ? cert_grid_lower(grid, point) : -1.0;
+      const double d = cert_domain_distance(g, point, hint, lb >= grid.tau ? lb : -1.0);
       if (d <= p.eps) {
         flags = 0;
       } else {
@@ -359,6 +397,27 @@ __global__ __launch_bounds__(kCertBlock) void cert_persistent(const DevGeom *geo
   }
 }
 
+// One thread per corner, the last index fastest (neighbouring lanes take
+// neighbouring corners and similar paths through the tree).
+__global__ void cert_grid_build(MeshView m, CertGrid cg, float *out) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t n12 = int64_t(cg.n[1]) * cg.n[2];
+  if (t >= int64_t(cg.n[0]) * n12) return;
+  const int i = int(t / n12), j = int((t - int64_t(i) * n12) / cg.n[2]), k = int(t % cg.n[2]);
+  int32_t hint = -1;
+  const double d = cert_mesh_distance(m, cert_corner(cg, i, j, k), hint);
+  // Nearest binary32, then one ulp down when that rounded up (d >= 0).
+  float f = float(d);
+  if (double(f) > d) f = __uint_as_float(__float_as_uint(f) - 1u);
+  out[t] = f;
+}
+
+__global__ void cert_grid_lower_kernel(CertGrid cg, const double *pts, int64_t n, double *out) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = cert_grid_lower(cg, V3<double>{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]});
+}
+
 __global__ void cert_distance_kernel(const DevGeom *geom, const double *pts, int64_t n, double *out) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -372,7 +431,7 @@ inline unsigned blocks_for(int64_t n, int b) { return unsigned((n + b - 1) / b);
 
 hipError_t launch_certify(const DevGeom *d_geom, const twos_cert_params &prm, const double *d_sites, int64_t n_sites,
                           int64_t sample_begin, int64_t count, double *d_value, int32_t *d_steps, int32_t *d_flags,
-                          unsigned long long *counter, int cus, hipStream_t stream) {
+                          unsigned long long *counter, int cus, const CertGrid &grid, hipStream_t stream) {
   CertLaunch p{};
   p.sites = d_sites;
   p.value = d_value;
@@ -401,10 +460,25 @@ hipError_t launch_certify(const DevGeom *d_geom, const twos_cert_params &prm, co
     e = hipMemsetAsync(counter, 0, sizeof(unsigned long long), stream);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(cert_persistent, dim3(unsigned(blocks)), dim3(kCertBlock), 0, stream, d_geom, p,
-                       splitmix64(prm.seed + 0xa1224ULL));
+                       splitmix64(prm.seed + 0xa1224ULL), grid);
     return hipGetLastError();
   }
   hipLaunchKernelGGL(cert_walk, dim3(blocks_for(p.jobs, 64)), dim3(64), 0, stream, d_geom, p);
+  return hipGetLastError();
+}
+
+hipError_t build_cert_grid(const MeshBuffers &mb, const CertGrid &grid, hipStream_t stream) {
+  const MeshView m{mb.nodes, mb.nodes4, mb.tris_f, mb.tris_d, mb.tri_ids, mb.n_tris};
+  const int64_t n = int64_t(grid.n[0]) * grid.n[1] * grid.n[2];
+  hipLaunchKernelGGL(cert_grid_build, dim3(blocks_for(n, 64)), dim3(64), 0, stream, m, grid,
+                     const_cast<float *>(grid.d));
+  return hipGetLastError();
+}
+
+hipError_t launch_cert_grid_lower(const CertGrid &grid, const double *d_pts, int64_t n, double *d_out,
+                                  hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(cert_grid_lower_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, stream, grid, d_pts, n, d_out);
   return hipGetLastError();
 }
 

@@ -26,6 +26,9 @@ struct MeshBuffers {
   float grid_start_r2 = 0.f;      // squared best distance up to which a start node is valid
   int64_t grid_bytes = 0;         // device bytes of grid + start nodes + cell nodes
   int32_t cell_nodes = 0;         // cell nodes appended to nodes4 after the tree's max(n_tris - 1, 1) nodes
+  CertGrid cert;                  // certified corner grid (null until a certified call asks for one)
+  int32_t cert_cells = 0;
+  int64_t cert_bytes = 0;
 };
 
 hipError_t lbvh_build(const double *d_vtx, const uint32_t *d_faces, int32_t n, MeshBuffers &mb, int32_t *depth,
@@ -65,7 +68,12 @@ hipError_t launch_zero(unsigned long long *p, int n, hipStream_t stream);
 // the reference's streams, one thread per job.
 hipError_t launch_certify(const DevGeom *d_geom, const twos_cert_params &prm, const double *d_sites, int64_t n_sites,
                           int64_t sample_begin, int64_t count, double *d_value, int32_t *d_steps, int32_t *d_flags,
-                          unsigned long long *counter, int cus, hipStream_t stream);
+                          unsigned long long *counter, int cus, const CertGrid &grid, hipStream_t stream);
+// Fills grid.d (corners grid.n[0] x grid.n[1] x grid.n[2], allocated by the
+// caller) with each corner's certified box-union distance rounded down.
+hipError_t build_cert_grid(const MeshBuffers &mb, const CertGrid &grid, hipStream_t stream);
+hipError_t launch_cert_grid_lower(const CertGrid &grid, const double *d_pts, int64_t n, double *d_out,
+                                  hipStream_t stream);
 hipError_t launch_cert_distance(const DevGeom *d_geom, const double *d_pts, int64_t n, double *d_out,
                                 hipStream_t stream);
 // Differential WoS over one chunk of sites (p.site_base, n_sites sites) on

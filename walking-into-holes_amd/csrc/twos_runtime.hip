@@ -1078,7 +1078,7 @@ Request *make_request(twos_ctx *ctx, int32_t geom, const twos_side sides[2], con
 // Context-level helpers.
 int64_t ctx_geometry_bytes(const twos_ctx *ctx) {
   int64_t b = int64_t(ctx->d_geoms_cap) * int64_t(sizeof(DevGeom));
-  for (const auto &kv : ctx->meshes) b += kv.second.bytes + kv.second.mb.grid_bytes;
+  for (const auto &kv : ctx->meshes) b += kv.second.bytes + kv.second.mb.grid_bytes + kv.second.mb.cert_bytes;
   for (const auto &kv : ctx->geoms) b += kv.second.owned_bytes;
   return b;
 }
@@ -1414,7 +1414,7 @@ void twos_destroy(twos_ctx *ctx) {
   for (auto &kv : ctx->geoms) free_geom(ctx, kv.second);
   for (auto &kv : ctx->meshes) {
     MeshBuffers &mb = kv.second.mb;
-    (void)hipFree(mb.nodes); (void)hipFree(mb.nodes4); (void)hipFree(mb.grid); (void)hipFree(mb.grid_start); (void)hipFree(mb.tris_f); (void)hipFree(mb.tris_d); (void)hipFree(mb.tri_ids);
+    (void)hipFree(mb.nodes); (void)hipFree(mb.nodes4); (void)hipFree(mb.grid); (void)hipFree(mb.grid_start); (void)hipFree(mb.tris_f); (void)hipFree(mb.tris_d); (void)hipFree(mb.tri_ids); (void)hipFree(const_cast<float *>(mb.cert.d));
   }
   (void)hipFree(ctx->d_geoms);
   (void)hipStreamDestroy(ctx->stream);
@@ -1491,7 +1491,7 @@ twos_status twos_mesh_create(twos_ctx *ctx, const double *vertices, int64_t nv, 
   m.bytes = int64_t(nf > 1 ? nf - 1 : 1) * (64 + 128) + nf * (36 + 72 + 4);
   // The 4-wide traversal stacks at most three siblings per 4-wide level.
   if (3 * ((m.depth - 1 + 1) / 2) >= kStackDepth) {
-    (void)hipFree(m.mb.nodes); (void)hipFree(m.mb.nodes4); (void)hipFree(m.mb.grid); (void)hipFree(m.mb.grid_start); (void)hipFree(m.mb.tris_f); (void)hipFree(m.mb.tris_d); (void)hipFree(m.mb.tri_ids);
+    (void)hipFree(m.mb.nodes); (void)hipFree(m.mb.nodes4); (void)hipFree(m.mb.grid); (void)hipFree(m.mb.grid_start); (void)hipFree(m.mb.tris_f); (void)hipFree(m.mb.tris_d); (void)hipFree(m.mb.tri_ids); (void)hipFree(const_cast<float *>(m.mb.cert.d));
     return fail(TWOS_ERR_GEOMETRY, "LBVH deeper than the traversal stack");
   }
   m.hash = hash;
@@ -1601,7 +1601,7 @@ twos_status twos_mesh_release(twos_ctx *ctx, int32_t mesh_id) {
   }
   if (it->second.users > 0) return fail(TWOS_ERR_INVALID_ARGUMENT, "mesh still used by a geometry");
   MeshBuffers &mb = it->second.mb;
-  (void)hipFree(mb.nodes); (void)hipFree(mb.nodes4); (void)hipFree(mb.grid); (void)hipFree(mb.grid_start); (void)hipFree(mb.tris_f); (void)hipFree(mb.tris_d); (void)hipFree(mb.tri_ids);
+  (void)hipFree(mb.nodes); (void)hipFree(mb.nodes4); (void)hipFree(mb.grid); (void)hipFree(mb.grid_start); (void)hipFree(mb.tris_f); (void)hipFree(mb.tris_d); (void)hipFree(mb.tri_ids); (void)hipFree(const_cast<float *>(mb.cert.d));
   ctx->meshes.erase(it);
   return TWOS_OK;
 }
@@ -1948,6 +1948,43 @@ twos_status twos_differential(twos_ctx *ctx, int32_t geom_id, const twos_side *s
 namespace twos {
 namespace {
 
+// The mesh's certified corner grid with `cells` cells along its longest axis
+// (one cell of margin around the vertex bounds), built once per resolution.
+twos_status ensure_cert_grid(twos_ctx *ctx, MeshEntry &me, int32_t cells) {
+  MeshBuffers &mb = me.mb;
+  if (mb.cert.d && mb.cert_cells == cells) return TWOS_OK;
+  (void)hipFree(const_cast<float *>(mb.cert.d));
+  mb.cert = CertGrid{};
+  mb.cert_cells = 0;
+  mb.cert_bytes = 0;
+  double ext = 0.0;
+  for (int k = 0; k < 3; ++k) ext = std::max(ext, me.hi[k] - me.lo[k]);
+  if (!(ext > 0.0)) return fail(TWOS_ERR_INVALID_ARGUMENT, "degenerate mesh bounds");
+  CertGrid cg;
+  cg.h = ext / cells;
+  cg.inv_h = 1.0 / cg.h;
+  for (int k = 0; k < 3; ++k) {
+    cg.lo[k] = me.lo[k] - cg.h;
+    cg.n[k] = int32_t(std::ceil((me.hi[k] - me.lo[k]) / cg.h)) + 3;  // cells + 2 of margin, as corners
+  }
+  const size_t count = size_t(cg.n[0]) * cg.n[1] * cg.n[2];
+  float *d = nullptr;
+  HIP_OK(dalloc(&d, count));
+  cg.d = d;
+  if (hipError_t e = build_cert_grid(mb, cg, ctx->stream); e != hipSuccess) {
+    (void)hipFree(d);
+    HIP_OK(e);
+  }
+  if (hipError_t e = hipStreamSynchronize(ctx->stream); e != hipSuccess) {
+    (void)hipFree(d);
+    HIP_OK(e);
+  }
+  mb.cert = cg;
+  mb.cert_cells = cells;
+  mb.cert_bytes = int64_t(count * sizeof(float));
+  return TWOS_OK;
+}
+
 // A mesh geometry minus cavities only (the validator's and the differential
 // estimator's domain).
 twos_status cavity_mesh_geometry(twos_ctx *ctx, int32_t geom_id, const GeomEntry **out) {
@@ -1976,6 +2013,10 @@ twos_status twos_certify(twos_ctx *ctx, int32_t geom_id, const twos_cert_params 
   if (p.source == TWOS_CERT_AFFINE && !(p.slope >= 0.0))
     return fail(TWOS_ERR_INVALID_ARGUMENT, "validator expects nonnegative affine slope");
   if (p.philox != 0 && p.philox != 1) return fail(TWOS_ERR_INVALID_ARGUMENT, "philox must be 0 or 1");
+  if (p.grid_cells != 0 && (!p.philox || p.grid_cells < 8 || p.grid_cells > 512))
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "grid_cells must be 0, or 8..512 with philox streams");
+  if (!(p.grid_kappa >= 0.f && p.grid_kappa < 1e6f))
+    return fail(TWOS_ERR_INVALID_ARGUMENT, "grid_kappa must be a non-negative number of cells");
   if (p.philox && p.sample_begin + int64_t(p.walks) > (int64_t(1) << 32))
     return fail(TWOS_ERR_INVALID_ARGUMENT, "Philox sample indices must fit in 32 bits");
   for (int64_t i = 0; i < 3 * n; ++i)
@@ -1985,6 +2026,15 @@ twos_status twos_certify(twos_ctx *ctx, int32_t geom_id, const twos_cert_params 
   if (twos_status st = cavity_mesh_geometry(ctx, geom_id, &ge)) return st;
   if (n == 0) return TWOS_OK;
   HIP_OK(hipSetDevice(ctx->device));
+  CertGrid grid;
+  if (p.grid_cells > 0) {
+    auto mit = ctx->meshes.find(ge->desc.mesh);
+    if (mit == ctx->meshes.end()) return fail(TWOS_ERR_NOT_FOUND, "unknown mesh id");
+    if (twos_status st = ensure_cert_grid(ctx, mit->second, p.grid_cells)) return st;
+    grid = mit->second.mb.cert;
+    const double kappa = p.grid_kappa > 0.f ? double(p.grid_kappa) : 2.0;
+    grid.tau = std::max(kappa * grid.h, 2.0 * p.eps);
+  }
   // Chunks of sites keep the per-sample buffers (16 B) within 2^22 samples.
   const int64_t chunk = std::max<int64_t>(1, (int64_t(1) << 22) / p.walks);
   const int64_t rows = std::min(n, chunk);
@@ -2007,7 +2057,7 @@ twos_status twos_certify(twos_ctx *ctx, int32_t geom_id, const twos_cert_params 
     q.site_offset = p.site_offset + s0;
     const size_t m = size_t(cnt) * size_t(p.walks), o = size_t(s0) * size_t(p.walks);
     HIP_OK(launch_certify(ctx->d_geoms + geom_id, q, b.sites + 3 * s0, cnt, p.sample_begin, p.walks, b.value,
-                          b.steps, b.flags, b.counter, cus, ctx->stream));
+                          b.steps, b.flags, b.counter, cus, grid, ctx->stream));
     HIP_OK(hipMemcpyAsync(value + o, b.value, sizeof(double) * m, hipMemcpyDeviceToHost, ctx->stream));
     HIP_OK(hipMemcpyAsync(steps + o, b.steps, sizeof(int32_t) * m, hipMemcpyDeviceToHost, ctx->stream));
     HIP_OK(hipMemcpyAsync(flags + o, b.flags, sizeof(int32_t) * m, hipMemcpyDeviceToHost, ctx->stream));
@@ -2034,6 +2084,31 @@ twos_status twos_cert_distance(twos_ctx *ctx, int32_t geom_id, const double *poi
   dfree(d_pts);
   dfree(d_out);
   if (e != hipSuccess) return dev_fail("cert distance", e);
+  return TWOS_OK;
+}
+
+twos_status twos_cert_grid_lower(twos_ctx *ctx, int32_t mesh_id, int32_t cells, const double *points, int64_t n,
+                                 double *out) {
+  if (!ctx || (n > 0 && (!points || !out))) return fail(TWOS_ERR_INVALID_ARGUMENT, "null argument");
+  if (n < 0 || cells < 8 || cells > 512) return fail(TWOS_ERR_INVALID_ARGUMENT, "cells must be 8..512");
+  for (int64_t i = 0; i < 3 * n; ++i)
+    if (!std::isfinite(points[i])) return fail(TWOS_ERR_INVALID_ARGUMENT, "non-finite point coordinate");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  auto it = ctx->meshes.find(mesh_id);
+  if (it == ctx->meshes.end()) return fail(TWOS_ERR_NOT_FOUND, "unknown mesh id");
+  HIP_OK(hipSetDevice(ctx->device));
+  if (twos_status st = ensure_cert_grid(ctx, it->second, cells)) return st;
+  if (n == 0) return TWOS_OK;
+  struct Buffers {
+    double *pts = nullptr, *out = nullptr;
+    ~Buffers() { dfree(pts); dfree(out); }
+  } b;
+  HIP_OK(dalloc(&b.pts, 3 * size_t(n)));
+  HIP_OK(dalloc(&b.out, size_t(n)));
+  HIP_OK(hipMemcpy(b.pts, points, sizeof(double) * 3 * size_t(n), hipMemcpyHostToDevice));
+  HIP_OK(launch_cert_grid_lower(it->second.mb.cert, b.pts, n, b.out, ctx->stream));
+  HIP_OK(hipMemcpyAsync(out, b.out, sizeof(double) * size_t(n), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_OK(hipStreamSynchronize(ctx->stream));
   return TWOS_OK;
 }
 

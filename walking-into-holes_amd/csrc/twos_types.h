@@ -126,6 +126,19 @@ struct WalkParams {
   int32_t pad2;
 };
 
+// Certified corner grid of a mesh: at corner (i, j, k) = lo + (i, j, k) h
+// (binary64), the certified distance to the union of the triangles' bounding
+// boxes rounded down to binary32.  tau: certified walks take the grid's bound
+// as their distance when it is at least tau (a launch parameter).
+struct CertGrid {
+  const float *d = nullptr;
+  double lo[3] = {0.0, 0.0, 0.0};
+  double h = 0.0, inv_h = 0.0;
+  int32_t n[3] = {0, 0, 0};  // corners per axis
+  int32_t pad = 0;
+  double tau = 0.0;
+};
+
 // One launch of the differential kernels: jobs = sites of the chunk x walks.
 struct DiffParams {
   const double *sites;   // [n][3] all sites of the request

@@ -461,18 +461,22 @@ class Context:
         return out[:, :k], counts
 
     def certify(self, geom: int, sites, *, seed=0, walks=1024, max_steps=128, eps=1e-6, radius_cap=0.1,
-                value_cap=0.1, source=_capi.CERT_AFFINE, slope=0.65, site_offset=0, sample_begin=0, philox=False):
+                value_cap=0.1, source=_capi.CERT_AFFINE, slope=0.65, site_offset=0, sample_begin=0, philox=False,
+                grid_cells=0, grid_kappa=0.0):
         """Certified minimum-benefit samples (cert_walk, minimum_benefit.hpp:
         122-143): (value, steps, flags), [n][walks] each, for samples
         sample_begin .. sample_begin + walks - 1; flags CERT_CAPPED,
         CERT_CLIPPED, CERT_CONTAINMENT.  On the reference's mt19937_64
         streams, or with philox=True on Philox streams in one persistent
-        launch (the production form)."""
+        launch (the production form), optionally with the mesh's certified
+        grid (grid_cells along its longest axis) for steps at least
+        grid_kappa cells (0: 2) from the mesh."""
         p = _f64(sites).reshape(-1, 3)
         n = len(p)
         prm = _capi.CertParams(seed=seed, walks=walks, max_steps=max_steps, eps=eps, radius_cap=radius_cap,
                                value_cap=value_cap, slope=slope, source=source, site_offset=site_offset,
-                               sample_begin=sample_begin, philox=int(bool(philox)))
+                               sample_begin=sample_begin, philox=int(bool(philox)), grid_cells=grid_cells,
+                               grid_kappa=grid_kappa)
         value = np.zeros((n, walks))
         steps = np.zeros((n, walks), dtype=np.int32)
         flags = np.zeros((n, walks), dtype=np.int32)
@@ -486,6 +490,15 @@ class Context:
         p = _f64(points).reshape(-1, 3)
         out = np.zeros(len(p))
         _check(_lib().twos_cert_distance(self._h, geom, _ptr(p, _capi._dp), len(p), _ptr(out, _capi._dp)))
+        return out
+
+    def cert_grid_lower(self, mesh, cells: int, points) -> np.ndarray:
+        """The certified grid's lower bound of the mesh's box-union distance
+        (built with `cells` cells along the mesh's longest axis if needed)."""
+        p = _f64(points).reshape(-1, 3)
+        out = np.zeros(len(p))
+        _check(_lib().twos_cert_grid_lower(self._h, mesh.id if isinstance(mesh, Mesh) else int(mesh), cells,
+                                           _ptr(p, _capi._dp), len(p), _ptr(out, _capi._dp)))
         return out
 
     # -- the estimator ------------------------------------------------------

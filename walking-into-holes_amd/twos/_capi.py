@@ -54,7 +54,7 @@ class CertParams(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("walks", C.c_int32), ("max_steps", C.c_int32), ("eps", C.c_double),
                 ("radius_cap", C.c_double), ("value_cap", C.c_double), ("slope", C.c_double),
                 ("source", C.c_int32), ("philox", C.c_int32), ("site_offset", C.c_int64),
-                ("sample_begin", C.c_int64)]
+                ("sample_begin", C.c_int64), ("grid_cells", C.c_int32), ("grid_kappa", C.c_float)]
 
 
 CERT_AFFINE, CERT_HOUSING = 0, 1
@@ -121,6 +121,7 @@ SIGNATURES = {
                                       C.c_double, _dp, C.POINTER(C.c_int64)]),
     "twos_certify": (C.c_int32, [_ctx_p, C.c_int32, C.POINTER(CertParams), _dp, C.c_int64, _dp, _i32p, _i32p]),
     "twos_cert_distance": (C.c_int32, [_ctx_p, C.c_int32, _dp, C.c_int64, _dp]),
+    "twos_cert_grid_lower": (C.c_int32, [_ctx_p, C.c_int32, C.c_int32, _dp, C.c_int64, _dp]),
     "twos_trace": (C.c_int32, [_ctx_p, C.c_int32, C.POINTER(Side), C.POINTER(Params), _dp, C.c_int64,
                                C.c_int32, _dp, C.c_int32, _i32p]),
     "twos_server_create": (C.c_int32, [_ctx_p, C.c_int64, C.c_int64, C.POINTER(_srv_p)]),
