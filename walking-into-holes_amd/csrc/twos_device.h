@@ -27,6 +27,36 @@ __device__ __forceinline__ V3<R> operator*(V3<R> a, R s) { return {a.x * s, a.y 
 template <typename R>
 __device__ __forceinline__ R dot(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 
+// Pointers to device buffers read from a table (the geometry rows) are
+// generic to the compiler, which then emits flat loads: those also count on
+// the LDS/scalar counter, so each one waits behind the traversal stack's LDS
+// traffic.  Every geometry buffer is global memory (hipMalloc or a
+// __device__ array), whose generic and global addresses are equal, so the
+// pointer is reinterpreted as a global one at the load.
+template <typename T>
+using GPtr = const __attribute__((address_space(1))) T *;
+template <typename T>
+__device__ __forceinline__ GPtr<T> gptr(const T *p) {
+  return __builtin_bit_cast(GPtr<T>, p);
+}
+// 16-byte vector types through a global pointer (their class copy cannot
+// bind a global-address-space reference, so the load goes through the
+// builtin vector type).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 gload(const float4 *p) {
+  const f32x4 v = *gptr(reinterpret_cast<const f32x4 *>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ int4 gload(const int4 *p) {
+  const i32x4 v = *gptr(reinterpret_cast<const i32x4 *>(p));
+  return make_int4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint4 gload(const uint4 *p) {
+  const i32x4 v = *gptr(reinterpret_cast<const i32x4 *>(p));
+  return make_uint4(uint32_t(v.x), uint32_t(v.y), uint32_t(v.z), uint32_t(v.w));
+}
+
 template <typename R> struct Inf;
 template <> struct Inf<float> { static __device__ __forceinline__ float v() { return __builtin_huge_valf(); } };
 template <> struct Inf<double> { static __device__ __forceinline__ double v() { return __builtin_huge_val(); } };
@@ -312,7 +342,7 @@ __device__ __forceinline__ void load_tri(const MeshView &m, int32_t k, V3<R> &a,
 template <>
 __device__ __forceinline__ void load_tri<float>(const MeshView &m, int32_t k, V3<float> &a, V3<float> &b,
                                                 V3<float> &c) {
-  const float *t = m.tf + 9 * size_t(k);
+  const GPtr<float> t = gptr(m.tf + 9 * size_t(k));
   a = {t[0], t[1], t[2]};
   b = {t[3], t[4], t[5]};
   c = {t[6], t[7], t[8]};
@@ -321,7 +351,7 @@ __device__ __forceinline__ void load_tri<float>(const MeshView &m, int32_t k, V3
 template <>
 __device__ __forceinline__ void load_tri<double>(const MeshView &m, int32_t k, V3<double> &a, V3<double> &b,
                                                  V3<double> &c) {
-  const double *t = m.td + 9 * size_t(k);
+  const GPtr<double> t = gptr(m.td + 9 * size_t(k));
   a = {t[0], t[1], t[2]};
   b = {t[3], t[4], t[5]};
   c = {t[6], t[7], t[8]};
@@ -363,7 +393,7 @@ __device__ __forceinline__ int32_t grid_start_node(const DevGeom &g, V3<R> p, R 
   const int k = int(floor((p.y - R(g.grid_lo[1])) * inv_h));
   const int l = int(floor((p.z - R(g.grid_lo[2])) * inv_h));
   if (i < 0 || k < 0 || l < 0 || i >= g.grid_n[0] || k >= g.grid_n[1] || l >= g.grid_n[2]) return 0;
-  return g.grid_start[(size_t(i) * g.grid_n[1] + k) * g.grid_n[2] + l];
+  return gptr(g.grid_start)[(size_t(i) * g.grid_n[1] + k) * g.grid_n[2] + l];
 }
 
 // Exact closest point on the mesh over the 4-wide BVH.  The query first tests
@@ -389,7 +419,7 @@ __device__ __forceinline__ R mesh_closest(const MeshView &m, V3<R> p, V3<R> &bes
     const V3<R> d = p - q;
     const R d2 = dot(d, d);
     if constexpr (kExact) {
-      const int32_t id = m.tid[k];
+      const int32_t id = gptr(m.tid)[k];
       if (d2 < best || (d2 == best && id < best_id)) {
         best = d2;
         best_pt = q;
@@ -422,8 +452,9 @@ __device__ __forceinline__ R mesh_closest(const MeshView &m, V3<R> p, V3<R> &bes
   int32_t visits = m.n_tris + 8;  // 4-wide nodes < triangles; guards a corrupt tree
   while (--visits >= 0) {
     const float4 *q = m.nodes4 + size_t(kNode4Quads) * node;
-    const float4 lx = q[0], ly = q[1], lz = q[2], hx = q[3], hy = q[4], hz = q[5];
-    const int4 ch = *reinterpret_cast<const int4 *>(q + 6);
+    const float4 lx = gload(q), ly = gload(q + 1), lz = gload(q + 2), hx = gload(q + 3), hy = gload(q + 4),
+                 hz = gload(q + 5);
+    const int4 ch = gload(reinterpret_cast<const int4 *>(q + 6));
     if (kCount) cnt->nodes += 1;
     R d[4] = {box_d2<R>(p, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x), box_d2<R>(p, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y),
               box_d2<R>(p, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z), box_d2<R>(p, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w)};
@@ -495,13 +526,13 @@ __device__ __forceinline__ Vec4<R> load4(const DevGeom &g, int kind, int i);
 
 template <>
 __device__ __forceinline__ Vec4<float> load4<float>(const DevGeom &g, int kind, int i) {
-  const float4 v = kind == 0 ? g.spheres_f[i] : (kind == 1 ? g.cylinders_f[i] : g.cavities_f[i]);
+  const float4 v = gload((kind == 0 ? g.spheres_f : (kind == 1 ? g.cylinders_f : g.cavities_f)) + i);
   return {v.x, v.y, v.z, v.w};
 }
 
 template <>
 __device__ __forceinline__ Vec4<double> load4<double>(const DevGeom &g, int kind, int i) {
-  const double *b = kind == 0 ? g.spheres_d : (kind == 1 ? g.cylinders_d : g.cavities_d);
+  const GPtr<double> b = gptr(kind == 0 ? g.spheres_d : (kind == 1 ? g.cylinders_d : g.cavities_d));
   return {b[4 * i + 0], b[4 * i + 1], b[4 * i + 2], b[4 * i + 3]};
 }
 
@@ -643,7 +674,7 @@ __device__ __forceinline__ R grid_lower(const DevGeom &g, V3<R> p, R *upper = nu
   if (!g.grid_corners) {
     const V3<R> c{R(g.grid_lo[0]) + (R(i) + R(0.5)) * h, R(g.grid_lo[1]) + (R(k) + R(0.5)) * h,
                   R(g.grid_lo[2]) + (R(l) + R(0.5)) * h};
-    const R dc = R(g.grid[(size_t(i) * g.grid_n[1] + k) * g.grid_n[2] + l]);
+    const R dc = R(gptr(g.grid)[(size_t(i) * g.grid_n[1] + k) * g.grid_n[2] + l]);
     shadow();
     const V3<R> d = p - c;
     const R dist = wsqrt(dot(d, d));
@@ -653,7 +684,7 @@ __device__ __forceinline__ R grid_lower(const DevGeom &g, V3<R> p, R *upper = nu
   const R x0 = p.x - (R(g.grid_lo[0]) + R(i) * h), y0 = p.y - (R(g.grid_lo[1]) + R(k) * h),
           z0 = p.z - (R(g.grid_lo[2]) + R(l) * h);
   if (g.grid_corners == 2) {
-    const uint4 rec = reinterpret_cast<const uint4 *>(g.grid)[(size_t(i) * g.grid_n[1] + k) * g.grid_n[2] + l];
+    const uint4 rec = gload(reinterpret_cast<const uint4 *>(g.grid) + (size_t(i) * g.grid_n[1] + k) * g.grid_n[2] + l);
     shadow();
     const uint32_t w[4] = {rec.x, rec.y, rec.z, rec.w};
     R best = -Inf<R>::v(), up = Inf<R>::v();
@@ -674,7 +705,7 @@ __device__ __forceinline__ R grid_lower(const DevGeom &g, V3<R> p, R *upper = nu
     if (upper) *upper = up * R(1.00001) + R(1e-6);
     return best - R(2e-6);
   }
-  const float *row = g.grid + (size_t(i) * g.grid_n[1] + k) * g.grid_n[2] + l;
+  const GPtr<float> row = gptr(g.grid + (size_t(i) * g.grid_n[1] + k) * g.grid_n[2] + l);
   shadow();
   const size_t si = size_t(g.grid_n[1]) * g.grid_n[2], sk = size_t(g.grid_n[2]);
   R best = -Inf<R>::v(), up = Inf<R>::v();
@@ -682,7 +713,7 @@ __device__ __forceinline__ R grid_lower(const DevGeom &g, V3<R> p, R *upper = nu
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
-      const float *q = row + a * si + b * sk;
+      const GPtr<float> q = row + a * si + b * sk;
       const R dx = x0 - R(a) * h, dy = y0 - R(b) * h;
       const R dxy = dx * dx + dy * dy;
 #pragma unroll
@@ -776,7 +807,7 @@ __device__ __forceinline__ void mq_test(const MeshView &m, V3<R> p, MeshQuery<R>
   const V3<R> d = p - cp;
   const R d2 = dot(d, d);
   if constexpr (sizeof(R) == 8) {
-    const int32_t id = m.tid[k];
+    const int32_t id = gptr(m.tid)[k];
     if (d2 < q.best || (d2 == q.best && id < q.best_id)) {
       q.best = d2;
       q.pt = cp;
@@ -825,11 +856,12 @@ template <typename R, bool kCount, int kLds>
 __device__ __forceinline__ bool mq_visit(const MeshView &m, V3<R> p, MeshQuery<R> &q, uint32_t *lds,
                                          uint32_t stride, uint32_t *overflow, TraversalCount *cnt) {
   const float4 *nq = m.nodes4 + size_t(kNode4Quads) * q.node;
-  const float4 lx = nq[0], ly = nq[1], lz = nq[2], hx = nq[3], hy = nq[4], hz = nq[5];
-  const int4 ch = *reinterpret_cast<const int4 *>(nq + 6);
+  const float4 lx = gload(nq), ly = gload(nq + 1), lz = gload(nq + 2), hx = gload(nq + 3), hy = gload(nq + 4),
+               hz = gload(nq + 5);
+  const int4 ch = gload(reinterpret_cast<const int4 *>(nq + 6));
   if (kCount) {
     cnt->nodes += 1;
-    cnt->top += __float_as_int(nq[7].z) < kTopLevels ? 1u : 0u;
+    cnt->top += __float_as_int(gload(nq + 7).z) < kTopLevels ? 1u : 0u;
   }
   R d[4] = {box_d2<R>(p, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x), box_d2<R>(p, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y),
             box_d2<R>(p, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z), box_d2<R>(p, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w)};

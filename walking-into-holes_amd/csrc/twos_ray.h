@@ -25,8 +25,8 @@ __device__ inline bool ray_hit(const MeshView &m, V3<double> o, V3<double> d, do
   const V3<double> inv{d.x != 0.0 ? 1.0 / d.x : 0.0, d.y != 0.0 ? 1.0 / d.y : 0.0, d.z != 0.0 ? 1.0 / d.z : 0.0};
   while (--visits >= 0) {
     if (node >= 0) {
-      const float4 q0 = m.nodes[4 * node + 0], q1 = m.nodes[4 * node + 1], q2 = m.nodes[4 * node + 2],
-                   q3 = m.nodes[4 * node + 3];
+      const float4 *nn = m.nodes + 4 * size_t(node);
+      const float4 q0 = gload(nn), q1 = gload(nn + 1), q2 = gload(nn + 2), q3 = gload(nn + 3);
       const float bx[2][6] = {{q0.x, q0.y, q0.z, q0.w, q1.x, q1.y}, {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w}};
       const int32_t ch[2] = {__float_as_int(q3.x), __float_as_int(q3.y)};
       bool hit[2];
