@@ -20,6 +20,7 @@ seeded 320-site sample (about 13 s on 16 threads).
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -202,7 +203,6 @@ def main() -> int:
     # Walk kernel alone (eager launches, HIP events on the library's stream).
     eager = run(twos.PERSISTENT, sites, False)
     eager_ms = eager.stats["walk_kernel_ms"]
-    import hashlib
     checksum = {"steps": int(np.sum(eager.steps)), "walks": int(np.sum(eager.walks)),
                 "score_sha16": hashlib.sha256(np.ascontiguousarray(eager.score).tobytes()).hexdigest()[:16]}
     launches = eager.stats["walk_launches"]
@@ -753,6 +753,8 @@ def f4_bench(args) -> int:
         "config": {"workload": (f"diffwos on {w.name}: {len(sites)} sites x {walks} samples, 4 cavities r 0.05"
                                 if diff else f"certify on {w.name}: {len(sites)} sites x {walks} samples")},
         "walk_steps_per_s": round(steps / t, 1),
+        # Identical across launch shapes and batching choices (A/B check).
+        "results_checksum": hashlib.sha256(b"".join(np.ascontiguousarray(a).tobytes() for a in out)).hexdigest()[:16],
     }
     if diff:
         z = _diff_z(out[0], exact[0])

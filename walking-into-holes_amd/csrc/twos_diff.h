@@ -139,26 +139,12 @@ __device__ __forceinline__ uint2 diff_philox_key(uint64_t dseed, int64_t site, i
   return make_uint2(uint32_t(k), uint32_t(k >> 32));
 }
 
-// Domain distance at p: the mesh (the grid's lower bound when it clears tau,
-// which never ends a path; otherwise the exact query, bounded by the grid's
-// upper bound and started at the cell's start node) then each cavity when
-// strictly closer, as closest_boundary orders them (wih_twos.cpp:602-626).
-// cavity: the winning cavity (-1: the mesh), rad its centre-to-p vector and
-// len that vector's length.
+// Cavities after the mesh: each wins only when strictly closer, as
+// closest_boundary orders them (wih_twos.cpp:602-626).  cavity: the winning
+// cavity (-1: the mesh), rad its centre-to-p vector and len that vector's
+// length.
 template <typename R>
-__device__ __forceinline__ R diff_distance(const DevGeom &g, const MeshView &m, V3<R> p, R tau, int32_t &hint,
-                                           V3<R> &point, int32_t &cavity, V3<R> &rad, R &len) {
-  R best;
-  R ub = Inf<R>::v();
-  const R lb = tau < Inf<R>::v() ? grid_lower<R>(g, p, &ub) : R(-1);
-  if (lb >= tau) {
-    best = lb;
-    point = p;
-  } else {
-    const R init2 = ub < Inf<R>::v() ? ub * ub : ub;
-    best = mesh_distance<R>(g, mesh_closest<R, false>(m, p, point, hint, nullptr, init2,
-                                                      grid_start_node<R>(g, p, init2)));
-  }
+__device__ __forceinline__ R diff_cavities(const DevGeom &g, V3<R> p, R best, int32_t &cavity, V3<R> &rad, R &len) {
   cavity = -1;
   const R cr = R(g.cavity_radius);
   for (int32_t i = 0; i < g.n_cavities; ++i) {
@@ -176,13 +162,32 @@ __device__ __forceinline__ R diff_distance(const DevGeom &g, const MeshView &m, 
   return best;
 }
 
+// The exact mesh distance at p (point: the closest point), bounded by the
+// grid's upper bound and started at the cell's start node when the grid is
+// on (tau finite).
+template <typename R>
+__device__ __forceinline__ R diff_mesh_exact(const DevGeom &g, const MeshView &m, V3<R> p, R tau, int32_t &hint,
+                                             V3<R> &point) {
+  R ub = Inf<R>::v();
+  if (tau < Inf<R>::v()) (void)grid_lower<R>(g, p, &ub);
+  const R init2 = ub < Inf<R>::v() ? ub * ub : ub;
+  return mesh_distance<R>(g, mesh_closest<R, false>(m, p, point, hint, nullptr, init2, grid_start_node<R>(g, p, init2)));
+}
+
 // Persistent kernel: each wave takes jobs in chunks of 64 consecutive tickets
-// (one atomic per chunk; consecutive lanes start at the same site), and each
-// loop iteration advances every lane by one step of its current path, outer
-// or nested, so lanes never wait for the wave's longest sample.  The rare
-// offset-safety test runs inside the step that ends an outer path on a
-// cavity.  A wave leaves once the tickets are exhausted and none of its lanes
-// holds a job; every job ends within 2 max_steps steps.
+// (one atomic per chunk; consecutive lanes start at the same site).  A lane
+// is ready to step, waiting for an exact closest-point query, or waiting for
+// the offset-safety test of an outer path that ended on a cavity; each loop
+// iteration the wave runs one of three code blocks for the lanes in that
+// state: the safety tests once at least p.safety_batch lanes wait for one
+// (or nothing else is left), the exact queries once at least as many lanes
+// wait for one as are ready to step, else the step (whose distance is the
+// grid's lower bound when that clears tau; a lane whose bound does not waits
+// for an exact query).  So the traversal and the binary64 safety code run for
+// many lanes at once instead of stalling the wave for one lane at a time.
+// Which code block runs when changes no path: each path's distances and draws
+// are the same.  A wave leaves once the tickets are exhausted and none of its
+// lanes holds a job; every job ends within 2 max_steps steps.
 constexpr int kDiffBlock = 256;
 
 template <typename R>
@@ -198,13 +203,15 @@ __global__ __launch_bounds__(kDiffBlock) void diff_persistent(const DevGeom *geo
   int64_t chunk_cur = 0, chunk_end = 0;  // wave-uniform
   bool drained = false;                  // wave-uniform
   bool active = false;
-  // Lane state.
+  // Lane state.  pend: 0 ready to step, 1 waiting for an exact query, 2
+  // waiting for the safety test of its outer hit (hit point in x, normal in
+  // normal, cavity in cavity).
   V3<R> x{R(0), R(0), R(0)}, normal{R(0), R(0), R(0)};
   R value = R(0);
   int64_t job = 0;
   int64_t site = 0;
   uint2 key = make_uint2(0u, 0u);
-  int32_t sample = 0, steps = 0, hint = -1, phase = 0, cavity = -1;
+  int32_t sample = 0, steps = 0, hint = -1, phase = 0, cavity = -1, pend = 0;
   int32_t flags = 0, outer = 0, nested = 0, queries = 0, rays = 0;
   while (true) {
     uint64_t need = __ballot(!active);
@@ -234,6 +241,7 @@ __global__ __launch_bounds__(kDiffBlock) void diff_persistent(const DevGeom *geo
         hint = -1;
         phase = 0;
         cavity = -1;
+        pend = 0;
         flags = outer = nested = queries = rays = 0;
         active = true;
       }
@@ -241,78 +249,106 @@ __global__ __launch_bounds__(kDiffBlock) void diff_persistent(const DevGeom *geo
       need = __ballot(!active);
     }
     if (!__any(active)) break;
+    // Wave-uniform choice of the code block.
+    const int nr = __popcll(__ballot(active && pend == 0));
+    const int nq = __popcll(__ballot(active && pend == 1));
+    const int ns = __popcll(__ballot(active && pend == 2));
+    const int mode = (ns > 0 && (ns >= p.safety_batch || nr + nq == 0)) ? 2 : ((nq > 0 && nq >= nr) ? 1 : 0);
     if (!active) continue;
     bool done = false;
-    if (steps >= p.max_steps) {
-      // The cap check precedes the query (walk_to_boundary, walk()).
-      flags |= kDiffLimit;
-      done = true;
-      if (phase == 0) cavity = -1;
-    } else {
-      V3<R> bp, rad;
-      R len = R(0);
-      int32_t cav;
-      const R r = diff_distance<R>(g, m, x, tau, hint, bp, cav, rad, len);
-      ++queries;
-      if (phase == 0) {
-        if (!(r > eps)) {
-          // The outer path ended: the hit's point and outward normal
-          // (boundary_hit's operation order, wih_twos.cpp:602-626).
+    if (mode == 2) {
+      if (pend == 2) {
+        // The outer path ended on `cavity` at x with outward normal `normal`
+        // (boundary_hit's operation order, wih_twos.cpp:602-626): classify
+        // the offset point in binary64.
+        const V3<double> bpd{double(x.x), double(x.y), double(x.z)};
+        const V3<double> nd{double(normal.x), double(normal.y), double(normal.z)};
+        const V3<double> off = bpd - nd * p.ell;
+        const int32_t safety = offset_safety(g, cavity, bpd, off, p.ell, p.eps, queries, rays, hint);
+        flags |= safety;
+        pend = 0;
+        if (safety) {
+          flags |= kDiffUnsafe;
           done = true;
-          if (cav >= 0) {
-            const V3<R> dir = rad * (R(1) / len);
-            const Vec4<R> c4 = load4<R>(g, 2, cav);
-            bp = V3<R>{c4.x, c4.y, c4.z} + dir * cr;
-            normal = dir * R(-1);
-            flags |= kDiffCavity;
-            const V3<double> bpd{double(bp.x), double(bp.y), double(bp.z)};
-            const V3<double> nd{double(normal.x), double(normal.y), double(normal.z)};
-            const V3<double> off = bpd - nd * p.ell;
-            const int32_t safety = offset_safety(g, cav, bpd, off, p.ell, p.eps, queries, rays, hint);
-            flags |= safety;
-            if (safety) {
-              flags |= kDiffUnsafe;
-            } else {
-              // The nested path from the offset point.
-              phase = 1;
-              cavity = cav;
-              x = V3<R>{R(off.x), R(off.y), R(off.z)};
-              key = diff_philox_key(p.pseed, p.site_offset + site, 1);
-              steps = 0;
-              done = false;
-            }
-          } else {
-            normal = r > R(0) ? (bp - x) * (R(1) / r) : V3<R>{R(0), R(0), R(0)};
-          }
         } else {
-          x = x + philox_direction(philox4x32_10(make_uint4(uint32_t(steps), uint32_t(sample), 0u, 0u), key), R(0)) * r;
-          ++steps;
-          ++outer;
+          // The nested path from the offset point.
+          phase = 1;
+          x = V3<R>{R(off.x), R(off.y), R(off.z)};
+          key = diff_philox_key(p.pseed, p.site_offset + site, 1);
+          steps = 0;
         }
-      } else if (!isfinite(r)) {
-        // A failed query ends the nested path as capped (wih_twos.cpp:355-357).
+      }
+    } else if (mode == 1 ? pend == 1 : pend == 0) {
+      if (mode == 0 && steps >= p.max_steps) {
+        // The cap check precedes the query (walk_to_boundary, walk()).
         flags |= kDiffLimit;
         done = true;
-      } else if (!(r > eps)) {
-        int cls = 0;
-        if (cav >= 0) {
-          const Vec4<R> c4 = load4<R>(g, 2, cav);
-          bp = V3<R>{c4.x, c4.y, c4.z} + rad * (cr / len);
-          cls = 7;
-        }
-        value += side_payoff<R>(s, bp, cls);
-        done = true;
+        if (phase == 0) cavity = -1;
       } else {
-        if (s.source_kind) value += side_green_term<R>(s, x, sampled ? x + philox_green<R>(key, sample, steps, r) : x, r, false);
-        x = x + philox_direction(philox4x32_10(make_uint4(uint32_t(steps), uint32_t(sample), 0u, 0u), key), R(0)) * r;
-        ++steps;
-        ++nested;
+        V3<R> bp = x, rad{R(0), R(0), R(0)};
+        R len = R(0);
+        int32_t cav = -1;
+        R r;
+        bool have = true;
+        if (mode == 1 || !(tau < Inf<R>::v())) {
+          // Without the grid every step is exact, so it runs in the step block.
+          r = diff_mesh_exact<R>(g, m, x, tau, hint, bp);
+          pend = 0;
+        } else {
+          const R lb = tau < Inf<R>::v() ? grid_lower<R>(g, x) : R(-1);
+          r = lb;
+          have = lb >= tau;  // a grid step never ends a path (tau > eps)
+          if (!have) pend = 1;
+        }
+        if (have) {
+          r = diff_cavities<R>(g, x, r, cav, rad, len);
+          ++queries;
+          if (phase == 0) {
+            if (!(r > eps)) {
+              // The outer path ended: the hit's point and outward normal.
+              if (cav >= 0) {
+                const V3<R> dir = rad * (R(1) / len);
+                const Vec4<R> c4 = load4<R>(g, 2, cav);
+                x = V3<R>{c4.x, c4.y, c4.z} + dir * cr;
+                normal = dir * R(-1);
+                flags |= kDiffCavity;
+                cavity = cav;
+                pend = 2;
+              } else {
+                normal = r > R(0) ? (bp - x) * (R(1) / r) : V3<R>{R(0), R(0), R(0)};
+                done = true;
+              }
+            } else {
+              x = x + philox_direction(philox4x32_10(make_uint4(uint32_t(steps), uint32_t(sample), 0u, 0u), key), R(0)) * r;
+              ++steps;
+              ++outer;
+            }
+          } else if (!isfinite(r)) {
+            // A failed query ends the nested path as capped (wih_twos.cpp:355-357).
+            flags |= kDiffLimit;
+            done = true;
+          } else if (!(r > eps)) {
+            int cls = 0;
+            if (cav >= 0) {
+              const Vec4<R> c4 = load4<R>(g, 2, cav);
+              bp = V3<R>{c4.x, c4.y, c4.z} + rad * (cr / len);
+              cls = 7;
+            }
+            value += side_payoff<R>(s, bp, cls);
+            done = true;
+          } else {
+            if (s.source_kind) value += side_green_term<R>(s, x, sampled ? x + philox_green<R>(key, sample, steps, r) : x, r, false);
+            x = x + philox_direction(philox4x32_10(make_uint4(uint32_t(steps), uint32_t(sample), 0u, 0u), key), R(0)) * r;
+            ++steps;
+            ++nested;
+          }
+        }
       }
     }
     if (done) {
       const bool has = phase == 1;
-      diff_record<R>(p, job, normal, has ? double(value) / p.ell : 0.0, has ? cavity : -1, flags, outer,
-                     nested, queries, rays);
+      diff_record<R>(p, job, normal, has ? double(value) / p.ell : 0.0, has ? cavity : -1, flags, outer, nested,
+                     queries, rays);
       active = false;
     }
   }

@@ -1923,6 +1923,7 @@ twos_status twos_differential(twos_ctx *ctx, int32_t geom_id, const twos_side *s
   dp.counter = b.counter;
   dp.pseed = splitmix64(p.seed + 0x6a09e667f3bcc909ULL) ^ splitmix64(uint64_t(p.repeat) + 0xbb67ae8584caa73bULL);
   dp.grid_kappa = (p.flags & TWOS_FLAG_DISTANCE_GRID) ? (p.grid_kappa > 0.f ? p.grid_kappa : kGridKappa) : 0.f;
+  dp.safety_batch = 8;
   const int cus = ctx->prop.multiProcessorCount;
   for (int64_t s0 = 0; s0 < n; s0 += chunk) {
     const int32_t cnt = int32_t(std::min(chunk, n - s0));

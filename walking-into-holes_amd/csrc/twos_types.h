@@ -159,7 +159,7 @@ struct DiffParams {
   unsigned long long *counter;  // job tickets, zero before the launch
   uint64_t pseed;        // splitmix64(seed + 0x6a09e667f3bcc909) ^ splitmix64(repeat + 0xbb67ae8584caa73b)
   float grid_kappa;      // > 0: mesh steps take the grid bound when it clears max(kappa h, 2 eps)
-  int32_t pad;
+  int32_t safety_batch;  // lanes waiting for an offset-safety test before the wave runs the tests
 };
 
 }  // namespace twos
