@@ -148,9 +148,24 @@ __device__ __forceinline__ double point_bound(V3<double> a, V3<double> b, bool u
   return upper ? upw(sqrt(sum)) : pdn(sqrt(sum));
 }
 
+// An upper bound of |a - b| from round-to-nearest arithmetic (the production
+// form; the replay form keeps point_bound's per-operation upward rounding).
+// With u = 2^-53, the three differences, three squares, two sums of
+// non-negative terms and the square root (within one ulp) leave the computed
+// s >= |a - b| (1 - 5u); the product s (1 + 2^-49) rounded to nearest is at
+// least s (1 + 16u)(1 - u) > |a - b|.  The absolute 2^-500 covers squares
+// below the normal range (a difference below 2^-511).  Eight operations
+// instead of about sixty.
+__device__ __forceinline__ double dist_upper_fast(V3<double> a, V3<double> b) {
+  const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+  const double sum = dx * dx + dy * dy + dz * dz;
+  return sqrt(sum) * (1.0 + 0x1p-49) + 0x1p-500;
+}
+
 // Certified grid.  The exact box-union distance B is 1-Lipschitz, and a
 // corner value v(c) = down(D(c)) <= B(c), so for any corner c,
-// B(p) >= v(c) - |p - c| >= down(v(c) - up(|p - c|)).  The bound takes the best
+// B(p) >= v(c) - |p - c| >= down(v(c) - U) for any upper bound U of |p - c|
+// (dist_upper_fast).  The bound takes the best
 // of the eight corners of p's cell (any cell gives a valid bound, so the index
 // only has to be in range).  Corners are recomputed from the same binary64
 // expression in the build and the lookup (this file has no contraction), so
@@ -175,7 +190,7 @@ __device__ double cert_grid_lower(const CertGrid &cg, V3<double> p) {
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
         const double v = double(cg.d[(size_t(i + a) * cg.n[1] + size_t(j + b)) * cg.n[2] + size_t(k + c)]);
-        best = fmax(best, dn(v - point_bound(p, cert_corner(cg, i + a, j + b, k + c), true)));
+        best = fmax(best, dn(v - dist_upper_fast(p, cert_corner(cg, i + a, j + b, k + c))));
       }
   return best;
 }
@@ -239,13 +254,19 @@ struct CertLaunch {
 // the next point from the step's draws; returns false when a point cannot be
 // shown inside the ball (containment failure), else adds the step's
 // rounded-down source term to value (cert_walk, :126-141).
+// kFast (the production form): the containment check bounds |next - point|
+// and |y - point| with dist_upper_fast; the replay form with point_bound, as
+// the reference does.
+template <bool kFast>
 __device__ __forceinline__ bool cert_step(const CertLaunch &p, V3<double> &point, double d, V3<double> green_dir,
                                           double green_t, V3<double> dir, double &value) {
   const double s = fmin(p.radius_cap, pdn(0.95 * d));
   V3<double> y = point;
   if (p.source == TWOS_CERT_HOUSING) y = point + green_dir * (s * green_t);
   const V3<double> next = point + dir * s;
-  if (!(point_bound(point, next, true) < d) || !(point_bound(point, y, true) < d)) return false;
+  const double un = kFast ? dist_upper_fast(point, next) : point_bound(point, next, true);
+  const double uy = kFast ? dist_upper_fast(point, y) : point_bound(point, y, true);
+  if (!(un < d) || !(uy < d)) return false;
   const double weight = pdn(pdn(s * s) / 6.0);
   const double increment = pdn(weight * cert_source_lower(y, p.source, p.slope));
   value = pdn(value + increment);
@@ -281,7 +302,7 @@ __global__ __launch_bounds__(64) void cert_walk(const DevGeom *geom, CertLaunch 
       t = mt.beta22();
       gdir = mt.direction();
     }
-    if (!cert_step(p, point, d, gdir, t, mt.direction(), value)) {
+    if (!cert_step<false>(p, point, d, gdir, t, mt.direction(), value)) {
       flags = 4;
       break;
     }
@@ -377,7 +398,7 @@ __global__ __launch_bounds__(kCertBlock) void cert_persistent(const DevGeom *geo
           gdir = philox_direction(philox4x32_10(make_uint4(uint32_t(steps), j, 1u, 0u), key), 0.0);
         }
         const V3<double> dir = philox_direction(philox4x32_10(make_uint4(uint32_t(steps), j, 0u, 0u), key), 0.0);
-        if (!cert_step(p, point, d, gdir, t, dir, value)) {
+        if (!cert_step<true>(p, point, d, gdir, t, dir, value)) {
           flags = 4;
         } else {
           ++steps;
