@@ -644,8 +644,19 @@ __global__ void slot_hints(Tables tb, int64_t first, int64_t n) {
   if (g.container == kMesh) {
     const double *p = tb.slots.pos + 3 * slot;
     const MeshView m{g.nodes, g.nodes4, g.tris_f, g.tris_d, g.tri_ids, g.n_tris};
+    const V3<R> x{R(p[0]), R(p[1]), R(p[2])};
+    // With a distance grid the query starts below the grid's upper bound, at
+    // the cell's start node, as the walk's own queries do.
+    R init2 = Inf<R>::v();
+    int32_t start = 0;
+    if (g.grid && tb.slots.grid_kappa[slot] > 0.f) {
+      R ub;
+      (void)grid_lower<R>(g, x, &ub);
+      init2 = ub < Inf<R>::v() ? ub * ub : ub;
+      start = grid_start_node<R>(g, x, init2);
+    }
     V3<R> pt;
-    mesh_closest<R, false>(m, V3<R>{R(p[0]), R(p[1]), R(p[2])}, pt, hint, nullptr);
+    mesh_closest<R, false>(m, x, pt, hint, nullptr, init2, start);
   }
   tb.slots.hint0[slot] = hint;
 }
