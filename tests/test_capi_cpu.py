@@ -48,7 +48,8 @@ def test_struct_layouts_match_header_sizes(tmp_path):
     import subprocess
     from twos import _capi
     structs = {"twos_side": _capi.Side, "twos_site_result": _capi.SiteResult, "twos_params": _capi.Params,
-               "twos_geometry_desc": _capi.GeometryDesc, "twos_stats": _capi.Stats}
+               "twos_geometry_desc": _capi.GeometryDesc, "twos_stats": _capi.Stats,
+               "twos_cert_params": _capi.CertParams}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{ROOT / "include" / "twos.h"}"',
              'int main(void) {']
     for cname, py in structs.items():
