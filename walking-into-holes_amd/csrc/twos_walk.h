@@ -634,6 +634,11 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWav
 
 // Closest triangle at each new site: every path of a site starts with the
 // same query, so it is answered once at admission and seeds each path's hint.
+// A site where the distance grid's bound already clears the step threshold
+// (the walk's first step takes the grid radius, no query) gets no hint: deep
+// inside a mesh that query would test nearly every triangle, and a path's
+// first exact query, far from the site, starts from the grid's own bound.
+// The hint is only a warm start, so no distance or step depends on it.
 template <typename R>
 __global__ void slot_hints(Tables tb, int64_t first, int64_t n) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -649,9 +654,15 @@ __global__ void slot_hints(Tables tb, int64_t first, int64_t n) {
     // the cell's start node, as the walk's own queries do.
     R init2 = Inf<R>::v();
     int32_t start = 0;
-    if (g.grid && tb.slots.grid_kappa[slot] > 0.f) {
+    const float kappa = tb.slots.grid_kappa[slot];
+    if (g.grid && kappa > 0.f) {
       R ub;
-      (void)grid_lower<R>(g, x, &ub);
+      const R lb = grid_lower<R>(g, x, &ub);
+      const R tau = fmax(R(kappa) * R(g.grid_h), R(2) * R(tb.slots.eps[slot]));
+      if (lb >= tau) {
+        tb.slots.hint0[slot] = -1;
+        return;
+      }
       init2 = ub < Inf<R>::v() ? ub * ub : ub;
       start = grid_start_node<R>(g, x, init2);
     }
