@@ -196,18 +196,32 @@ std::vector<int32_t> morton_order(const double *p, int64_t n) {
     v = (v | (v << 2)) & 0x09249249ULL;
     return v;
   };
-  std::vector<uint64_t> key(static_cast<size_t>(n));
-  for (int64_t i = 0; i < n; ++i) {
-    uint64_t code = 0;
+  const size_t m = static_cast<size_t>(n);
+  std::vector<uint32_t> code(m), code2(m);
+  std::vector<int32_t> order(m), order2(m);
+  for (size_t i = 0; i < m; ++i) {
+    uint64_t c = 0;
     for (int a = 0; a < 3; ++a) {
       const double t = (p[3 * i + a] - lo[a]) / ext * 1023.0;
-      code |= spread(uint64_t(std::min(1023.0, std::max(0.0, t)))) << a;
+      c |= spread(uint64_t(std::min(1023.0, std::max(0.0, t)))) << a;
     }
-    key[size_t(i)] = code << 32 | uint64_t(i);
+    code[i] = uint32_t(c);
+    order[i] = int32_t(i);
   }
-  std::sort(key.begin(), key.end());
-  std::vector<int32_t> order(static_cast<size_t>(n));
-  for (int64_t i = 0; i < n; ++i) order[size_t(i)] = int32_t(key[size_t(i)] & 0xFFFFFFFFULL);
+  // Stable LSD radix sort of the 30-bit codes, 10 bits a pass: ties keep
+  // index order, so the order is that of sorting (code, index) pairs.
+  for (int shift = 0; shift < 30; shift += 10) {
+    size_t start[1025] = {0};
+    for (size_t i = 0; i < m; ++i) ++start[((code[i] >> shift) & 1023u) + 1];
+    for (int k = 0; k < 1024; ++k) start[k + 1] += start[k];
+    for (size_t i = 0; i < m; ++i) {
+      const size_t at = start[(code[i] >> shift) & 1023u]++;
+      code2[at] = code[i];
+      order2[at] = order[i];
+    }
+    code.swap(code2);
+    order.swap(order2);
+  }
   return order;
 }
 
@@ -228,6 +242,10 @@ struct Engine {
   // One-site work list of the plain schedule: 64 B in, 96 B out.
   char *d_plain = nullptr, *h_plain = nullptr;
   double *d_grad = nullptr;  // [cap_walks][2][3] gradient samples, allocated on the first gradient request
+  // Pinned staging of a request's slot data at admission (grown on demand),
+  // so its uploads are asynchronous copies instead of staged pageable ones.
+  char *h_stage = nullptr;
+  size_t stage_bytes = 0;
   size_t in_bytes = 0, out_bytes = 0;
   size_t off_slot = 0, off_base = 0, off_prefix = 0, off_nact = 0, off_total = 0;
   size_t off_compact = 0, off_decision = 0;
@@ -305,7 +323,9 @@ struct Engine {
     if (h_in) (void)hipHostFree(h_in);
     if (h_out) (void)hipHostFree(h_out);
     if (h_plain) (void)hipHostFree(h_plain);
-    h_in = h_out = h_plain = nullptr;
+    if (h_stage) (void)hipHostFree(h_stage);
+    h_in = h_out = h_plain = h_stage = nullptr;
+    stage_bytes = 0;
     for (auto &e : ev)
       if (e) { (void)hipEventDestroy(e); e = nullptr; }
   }
@@ -511,42 +531,86 @@ struct Engine {
   // accumulators.
   twos_status admit(Request *r) {
     const int64_t n = r->n, s0 = r->slot0;
+    const size_t N = size_t(n);
     hipStream_t st = ctx->stream;
-    std::vector<int32_t> gi(n, r->geom), pi(n, r->problem), rep(n, r->prm.repeat), wmax(n), msteps(n, r->prm.max_steps);
-    std::vector<int32_t> comb(n, r->prm.combine);
-    std::vector<uint2> keys(2 * n);
-    std::vector<int64_t> site(n);
-    std::vector<uint64_t> seed(n, r->prm.seed);
-    std::vector<double> eps(n, r->prm.eps), rel(n), z(n, r->prm.ci_z);
+    // Every per-site array is written once into the staging block (pinned;
+    // pageable in an arena engine, which allocates nothing) and uploaded from
+    // there; the stream is synchronised before the block is reused.
+    const size_t need = N * 112 + 16 * 256;
+    std::vector<char> pageable;
+    char *stage;
+    if (arena) {
+      pageable.resize(need);
+      stage = pageable.data();
+    } else {
+      if (stage_bytes < need) {
+        if (h_stage) (void)hipHostFree(h_stage);
+        h_stage = nullptr;
+        stage_bytes = 0;
+        HIP_OK(hipHostMalloc(reinterpret_cast<void **>(&h_stage), need, hipHostMallocDefault));
+        stage_bytes = need;
+      }
+      stage = h_stage;
+    }
+    size_t used = 0;
+    auto carve = [&](size_t bytes) {
+      char *at = stage + used;
+      used += (bytes + 255) & ~size_t(255);
+      return at;
+    };
+    auto *gi = reinterpret_cast<int32_t *>(carve(4 * N));
+    auto *pi = reinterpret_cast<int32_t *>(carve(4 * N));
+    auto *comb = reinterpret_cast<int32_t *>(carve(4 * N));
+    auto *keys = reinterpret_cast<uint2 *>(carve(sizeof(uint2) * 2 * N));
+    auto *site = reinterpret_cast<int64_t *>(carve(8 * N));
+    auto *seed = reinterpret_cast<uint64_t *>(carve(8 * N));
+    auto *rep = reinterpret_cast<int32_t *>(carve(4 * N));
+    auto *wmax = reinterpret_cast<int32_t *>(carve(4 * N));
+    auto *msteps = reinterpret_cast<int32_t *>(carve(4 * N));
+    auto *eps = reinterpret_cast<double *>(carve(8 * N));
+    auto *rel = reinterpret_cast<double *>(carve(8 * N));
+    auto *z = reinterpret_cast<double *>(carve(8 * N));
+    auto *kap = reinterpret_cast<float *>(carve(4 * N));
     const float kappa = (r->prm.flags & TWOS_FLAG_DISTANCE_GRID)
                             ? (r->prm.grid_kappa > 0.f ? r->prm.grid_kappa : kGridKappa) : 0.f;
-    std::vector<float> kap(n, kappa);
     const bool adaptive = r->prm.walks_max > r->prm.walks;
-    for (int64_t i = 0; i < n; ++i) {
-      const int64_t gsite = int64_t(r->prm.site_offset) + i;
-      site[i] = gsite;
+    for (size_t i = 0; i < N; ++i) {
+      const int64_t gsite = int64_t(r->prm.site_offset) + int64_t(i);
+      gi[i] = r->geom;
+      pi[i] = r->problem;
+      comb[i] = r->prm.combine;
       keys[2 * i] = philox_key_host(r->prm.seed, gsite, 0);
       keys[2 * i + 1] = philox_key_host(r->prm.seed, gsite, 1);
+      site[i] = gsite;
+      seed[i] = r->prm.seed;
+      rep[i] = r->prm.repeat;
       wmax[i] = adaptive ? r->prm.walks_max : r->prm.walks;
+      msteps[i] = r->prm.max_steps;
+      eps[i] = r->prm.eps;
       rel[i] = adaptive ? r->prm.ci_rel : -1.0;
+      z[i] = r->prm.ci_z;
+      kap[i] = kappa;
     }
-    if (r->d_sites)
-      HIP_OK(hipMemcpyAsync(s.pos + 3 * s0, r->d_sites, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, st));
-    else
-      HIP_OK(hipMemcpyAsync(s.pos + 3 * s0, r->sites.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(s.geom + s0, gi.data(), 4 * n, hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(s.problem + s0, pi.data(), 4 * n, hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(s.combine + s0, comb.data(), 4 * n, hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(s.keys + 2 * s0, keys.data(), sizeof(uint2) * 2 * n, hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(s.site + s0, site.data(), 8 * n, hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(s.seed + s0, seed.data(), 8 * n, hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(s.repeat + s0, rep.data(), 4 * n, hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(s.walks_max + s0, wmax.data(), 4 * n, hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(s.max_steps + s0, msteps.data(), 4 * n, hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(s.eps + s0, eps.data(), 8 * n, hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(s.ci_rel + s0, rel.data(), 8 * n, hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(s.ci_z + s0, z.data(), 8 * n, hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(s.grid_kappa + s0, kap.data(), 4 * n, hipMemcpyHostToDevice, st));
+    if (r->d_sites) {
+      HIP_OK(hipMemcpyAsync(s.pos + 3 * s0, r->d_sites, sizeof(double) * 3 * N, hipMemcpyDeviceToDevice, st));
+    } else {
+      auto *pos = reinterpret_cast<double *>(carve(sizeof(double) * 3 * N));
+      std::memcpy(pos, r->sites.data(), sizeof(double) * 3 * N);
+      HIP_OK(hipMemcpyAsync(s.pos + 3 * s0, pos, sizeof(double) * 3 * N, hipMemcpyHostToDevice, st));
+    }
+    HIP_OK(hipMemcpyAsync(s.geom + s0, gi, 4 * N, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(s.problem + s0, pi, 4 * N, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(s.combine + s0, comb, 4 * N, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(s.keys + 2 * s0, keys, sizeof(uint2) * 2 * N, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(s.site + s0, site, 8 * N, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(s.seed + s0, seed, 8 * N, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(s.repeat + s0, rep, 4 * N, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(s.walks_max + s0, wmax, 4 * N, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(s.max_steps + s0, msteps, 4 * N, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(s.eps + s0, eps, 8 * N, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(s.ci_rel + s0, rel, 8 * N, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(s.ci_z + s0, z, 8 * N, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(s.grid_kappa + s0, kap, 4 * N, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemsetAsync(s.acc + 8 * s0, 0, sizeof(double) * 8 * n, st));
     HIP_OK(hipMemsetAsync(s.acc_walks + s0, 0, 8 * n, st));
     HIP_OK(hipMemsetAsync(s.acc_steps + s0, 0, 8 * n, st));
