@@ -379,19 +379,27 @@ __global__ void inside_kernel(MeshView m, const double *pts, int64_t n, int32_t 
   out[i] = inside_parity(m, V3<double>{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]}, &rays);
 }
 
-// Distance at every cell centre of a uniform grid (binary32 exact query).
+// Distance at every sample of a uniform grid (binary32 exact query).
 // Sample (i, k, l) sits at lo + (index + offset) h: offset 0.5 for cell
-// centres, 0 for cell corners.
+// centres, 0 for cell corners.  A thread takes kGridRun consecutive samples
+// along x, each query first testing the previous sample's closest triangle
+// (one cell away, so its distance is a tight first bound); a wave's lanes
+// hold consecutive z samples.  The query returns the minimum over the mesh
+// whatever it tests first, so every value is the unhinted query's.
+constexpr int kGridRun = 8;
+
 __global__ void distance_grid(MeshView m, float lx, float ly, float lz, float h, float offset, int nx, int ny,
                               int nz, float *out) {
   const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int64_t total = int64_t(nx) * ny * nz;
-  if (idx >= total) return;
-  const int l = int(idx % nz), k = int((idx / nz) % ny), i = int(idx / (int64_t(ny) * nz));
-  const V3<float> p{lx + (float(i) + offset) * h, ly + (float(k) + offset) * h, lz + (float(l) + offset) * h};
+  const int runs = (nx + kGridRun - 1) / kGridRun;
+  if (idx >= int64_t(runs) * ny * nz) return;
+  const int l = int(idx % nz), k = int((idx / nz) % ny), r = int(idx / (int64_t(ny) * nz));
   int32_t hint = -1;
-  V3<float> pt;
-  out[idx] = sqrtf(mesh_closest<float, false>(m, p, pt, hint, nullptr));
+  for (int i = r * kGridRun; i < min(nx, (r + 1) * kGridRun); ++i) {
+    const V3<float> p{lx + (float(i) + offset) * h, ly + (float(k) + offset) * h, lz + (float(l) + offset) * h};
+    V3<float> pt;
+    out[(size_t(i) * ny + k) * nz + l] = sqrtf(mesh_closest<float, false>(m, p, pt, hint, nullptr));
+  }
 }
 
 // binary16 rounding of a non-negative distance toward zero, so a bound built
@@ -710,7 +718,8 @@ hipError_t build_distance_grid(MeshBuffers &mb, const double lo[3], const double
   hipError_t e = hipMalloc(&samples, sizeof(float) * size_t(total));
   if (e != hipSuccess) return e;
   MeshView m{mb.nodes, mb.nodes4, mb.tris_f, mb.tris_d, mb.tri_ids, mb.n_tris};
-  hipLaunchKernelGGL(distance_grid, dim3(blocks_for(total, 128)), dim3(128), 0, stream, m, mb.grid_lo[0],
+  const int64_t threads = int64_t((n[0] + kGridRun - 1) / kGridRun) * n[1] * n[2];
+  hipLaunchKernelGGL(distance_grid, dim3(blocks_for(threads, 128)), dim3(128), 0, stream, m, mb.grid_lo[0],
                      mb.grid_lo[1], mb.grid_lo[2], mb.grid_h, corners ? 0.0f : 0.5f, n[0], n[1], n[2], samples);
   if ((e = hipGetLastError()) != hipSuccess) {
     (void)hipFree(samples);
