@@ -418,19 +418,27 @@ __global__ __launch_bounds__(kCertBlock) void cert_persistent(const DevGeom *geo
   }
 }
 
-// One thread per corner, the last index fastest (neighbouring lanes take
-// neighbouring corners and similar paths through the tree).
+// A thread takes kCertGridRun consecutive corners along the first index,
+// each query warm-started at the previous corner's nearest box (the minimum
+// does not depend on the start); the last index is fastest across threads, so
+// neighbouring lanes take neighbouring corners and similar paths through the
+// tree.
+constexpr int kCertGridRun = 8;
+
 __global__ void cert_grid_build(MeshView m, CertGrid cg, float *out) {
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t n12 = int64_t(cg.n[1]) * cg.n[2];
-  if (t >= int64_t(cg.n[0]) * n12) return;
-  const int i = int(t / n12), j = int((t - int64_t(i) * n12) / cg.n[2]), k = int(t % cg.n[2]);
+  const int runs = (cg.n[0] + kCertGridRun - 1) / kCertGridRun;
+  if (t >= int64_t(runs) * n12) return;
+  const int r = int(t / n12), j = int((t - int64_t(r) * n12) / cg.n[2]), k = int(t % cg.n[2]);
   int32_t hint = -1;
-  const double d = cert_mesh_distance(m, cert_corner(cg, i, j, k), hint);
-  // Nearest binary32, then one ulp down when that rounded up (d >= 0).
-  float f = float(d);
-  if (double(f) > d) f = __uint_as_float(__float_as_uint(f) - 1u);
-  out[t] = f;
+  for (int i = r * kCertGridRun; i < min(cg.n[0], (r + 1) * kCertGridRun); ++i) {
+    const double d = cert_mesh_distance(m, cert_corner(cg, i, j, k), hint);
+    // Nearest binary32, then one ulp down when that rounded up (d >= 0).
+    float f = float(d);
+    if (double(f) > d) f = __uint_as_float(__float_as_uint(f) - 1u);
+    out[int64_t(i) * n12 + int64_t(j) * cg.n[2] + k] = f;
+  }
 }
 
 __global__ void cert_grid_lower_kernel(CertGrid cg, const double *pts, int64_t n, double *out) {
@@ -490,7 +498,7 @@ hipError_t launch_certify(const DevGeom *d_geom, const twos_cert_params &prm, co
 
 hipError_t build_cert_grid(const MeshBuffers &mb, const CertGrid &grid, hipStream_t stream) {
   const MeshView m{mb.nodes, mb.nodes4, mb.tris_f, mb.tris_d, mb.tri_ids, mb.n_tris};
-  const int64_t n = int64_t(grid.n[0]) * grid.n[1] * grid.n[2];
+  const int64_t n = int64_t((grid.n[0] + kCertGridRun - 1) / kCertGridRun) * grid.n[1] * grid.n[2];
   hipLaunchKernelGGL(cert_grid_build, dim3(blocks_for(n, 64)), dim3(64), 0, stream, m, grid,
                      const_cast<float *>(grid.d));
   return hipGetLastError();
