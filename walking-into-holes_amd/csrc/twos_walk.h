@@ -122,7 +122,10 @@ __device__ __forceinline__ V3<R> grad_sample(const Job<R> &j, R value, R first, 
 __device__ __forceinline__ void put3(double *o, V3<float> v) { o[0] = v.x; o[1] = v.y; o[2] = v.z; }
 __device__ __forceinline__ void put3(double *o, V3<double> v) { o[0] = v.x; o[1] = v.y; o[2] = v.z; }
 
-template <typename R>
+// kGrad: the kernel is a gradient-channel variant, the only kind the
+// launchers run with wp.grad set; other variants skip the gradient store at
+// compile time, so its zero vector does not hold registers in their loops.
+template <typename R, bool kGrad>
 __device__ __forceinline__ void job_store(const Job<R> &j, const WalkParams &wp, bool capped) {
   const int32_t st = j.steps | (capped ? (1 << 30) : 0);
   if (wp.combine == kCross) {
@@ -134,7 +137,7 @@ __device__ __forceinline__ void job_store(const Job<R> &j, const WalkParams &wp,
     wp.steps[2 * j.t + 0] = st;
     wp.steps[2 * j.t + 1] = 0;
   }
-  if (wp.grad) {
+  if (kGrad && wp.grad) {
     double *o = wp.grad + 6 * j.t;
     if (wp.combine == kCross) {
       put3(o + 3 * j.side, grad_sample(j, j.acc0, j.gs0, *j.s0));
@@ -362,7 +365,7 @@ __global__ __launch_bounds__(256) void walk_simple(Tables tb, WalkParams wp) {
       double *o = wp.trace + 4 * row++;
       o[0] = double(j.x.x); o[1] = double(j.x.y); o[2] = double(j.x.z); o[3] = double(r);
     }
-    if (state == 1) { job_store(j, wp, false); break; }
+    if (state == 1) { job_store<R, kChan == kChanGrad>(j, wp, false); break; }
     if (state == 2) {
       if (wp.trace && row < wp.trace_rows) {
         V3<R> bp;
@@ -371,7 +374,7 @@ __global__ __launch_bounds__(256) void walk_simple(Tables tb, WalkParams wp) {
         double *o = wp.trace + 4 * row++;
         o[0] = double(j.x.x); o[1] = double(j.x.y); o[2] = double(j.x.z); o[3] = double(rr);
       }
-      job_store(j, wp, true);
+      job_store<R, kChan == kChanGrad>(j, wp, true);
       break;
     }
     V3<R> d;
@@ -446,13 +449,13 @@ __device__ __forceinline__ int job_advance(Job<R> &j, const WalkParams &wp, R r,
 
 // A path ended with job_advance<..., kDefer = true> state `end` (1 in the
 // eps-shell, 2 capped): its payoff, then its samples and step word.
-template <typename R>
+template <typename R, bool kGrad>
 __device__ __forceinline__ void job_retire(Job<R> &j, const WalkParams &wp, int end) {
   if (end == 1) {
     j.acc0 += side_payoff<R>(*j.s0, j.x, j.hint);
     if (wp.combine == kPaired) j.acc1 += side_payoff<R>(*j.s1, j.x, j.hint);
   }
-  job_store(j, wp, end == 2);
+  job_store<R, kGrad>(j, wp, end == 2);
 }
 
 template <typename R, int GK>
@@ -539,7 +542,7 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWav
     uint64_t need = __ballot(!active);
     if (__popcll(need) < wp.refill && need != ~0ull) need = 0;  // wave-uniform
     if (kDefer && need != 0 && end != 0) {
-      job_retire(j, wp, end);
+      job_retire<R, kChan == kChanGrad>(j, wp, end);
       end = 0;
     }
     while (need != 0 && !drained) {
