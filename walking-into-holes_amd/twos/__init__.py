@@ -240,21 +240,35 @@ def _dtype_of(ct):
     return (_dtype_of(ct._type_), (ct._length_,))  # fixed-size array field
 
 
-def _result(out, n, stats: dict, gradient: bool = False) -> Result:
+def _result_dtype() -> np.dtype:
     fields = []
     for name, ct in _capi.SiteResult._fields_:
         d = _dtype_of(ct)
         fields.append((name, *d) if isinstance(d, tuple) else (name, d))
     dt = np.dtype(fields)
     assert dt.itemsize == C.sizeof(_capi.SiteResult)
-    rec = np.frombuffer(out, dtype=dt, count=n).copy() if n else np.zeros(0, dtype=dt)
-    res = Result(u=rec["u"].copy(), v=rec["v"].copy(), score=rec["score"].copy(),
-                 score_se=rec["score_se"].copy(), td=rec["td"].copy(), same=rec["same"].copy(),
-                 walks=rec["walks"].copy(), steps=rec["steps"].copy(), caps=rec["caps"].copy(),
-                 status=rec["status"].copy(), stats=stats)
+    return dt
+
+
+# SiteResult as a numpy record (the layout twos.h declares).
+RESULT_DTYPE = _result_dtype()
+
+
+def _result_buffer(n: int) -> np.ndarray:
+    """Zeroed output records for n sites."""
+    return np.zeros(max(n, 1), dtype=RESULT_DTYPE)
+
+
+def _result(out, n, stats: dict, gradient: bool = False) -> Result:
+    """Result from n SiteResult records: a RESULT_DTYPE array or any buffer
+    holding them; each field is copied out contiguous."""
+    rec = (out if isinstance(out, np.ndarray) else np.frombuffer(out, dtype=RESULT_DTYPE, count=n))[:n]
+    f = {k: np.ascontiguousarray(rec[k]) for k in
+         ("u", "v", "score", "score_se", "td", "same", "walks", "steps", "caps", "status")}
+    res = Result(**f, stats=stats)
     if gradient:
-        res.grad_u, res.grad_v = rec["grad_u"].copy(), rec["grad_v"].copy()
-        res.grad_dot, res.grad_dot_se = rec["grad_dot"].copy(), rec["grad_dot_se"].copy()
+        res.grad_u, res.grad_v = np.ascontiguousarray(rec["grad_u"]), np.ascontiguousarray(rec["grad_v"])
+        res.grad_dot, res.grad_dot_se = np.ascontiguousarray(rec["grad_dot"]), np.ascontiguousarray(rec["grad_dot_se"])
     return res
 
 
@@ -525,18 +539,19 @@ class Context:
                       | (FLAG_DISTANCE_GRID if distance_grid else 0) | (FLAG_GRADIENT if gradient else 0),
                       grid_kappa=grid_kappa, lds_stack=lds_stack, step_batch=step_batch,
                       refill_batch=refill_batch)
-        out = (_capi.SiteResult * max(n, 1))()
+        out = _result_buffer(n)
+        out_p = out.ctypes.data_as(C.POINTER(_capi.SiteResult))
         st = _capi.Stats()
         if dump:
             stride = max(walks, walks_max)
             da = np.zeros((n, stride))
             db = np.zeros((n, stride))
             ds = np.zeros((n, stride), dtype=np.int32)
-            status = _lib().twos_estimate_dump(self._h, geom, sd, C.byref(prm), _ptr(p, _capi._dp), n, out,
+            status = _lib().twos_estimate_dump(self._h, geom, sd, C.byref(prm), _ptr(p, _capi._dp), n, out_p,
                                                C.byref(st), _ptr(da, _capi._dp), _ptr(db, _capi._dp),
                                                _ptr(ds, _capi._i32p), stride)
         else:
-            status = _lib().twos_estimate(self._h, geom, sd, C.byref(prm), _ptr(p, _capi._dp), n, out,
+            status = _lib().twos_estimate(self._h, geom, sd, C.byref(prm), _ptr(p, _capi._dp), n, out_p,
                                           C.byref(st))
         res = _result(out, n, _stats(st), gradient)
         if dump:
