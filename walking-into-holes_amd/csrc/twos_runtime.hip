@@ -246,6 +246,7 @@ struct Engine {
   // so its uploads are asynchronous copies instead of staged pageable ones.
   char *h_stage = nullptr;
   size_t stage_bytes = 0;
+  bool stage_pending = false;  // uploads from h_stage may still be in flight on the context's stream
   size_t in_bytes = 0, out_bytes = 0;
   size_t off_slot = 0, off_base = 0, off_prefix = 0, off_nact = 0, off_total = 0;
   size_t off_compact = 0, off_decision = 0;
@@ -323,6 +324,8 @@ struct Engine {
     if (h_in) (void)hipHostFree(h_in);
     if (h_out) (void)hipHostFree(h_out);
     if (h_plain) (void)hipHostFree(h_plain);
+    if (stage_pending && ctx) (void)hipStreamSynchronize(ctx->stream);
+    stage_pending = false;
     if (h_stage) (void)hipHostFree(h_stage);
     h_in = h_out = h_plain = h_stage = nullptr;
     stage_bytes = 0;
@@ -543,6 +546,9 @@ struct Engine {
       pageable.resize(need);
       stage = pageable.data();
     } else {
+      // The previous admission's uploads read this block: let them finish.
+      if (stage_pending) HIP_OK(hipStreamSynchronize(st));
+      stage_pending = false;
       if (stage_bytes < need) {
         if (h_stage) (void)hipHostFree(h_stage);
         h_stage = nullptr;
@@ -622,8 +628,10 @@ struct Engine {
     HIP_OK(hipMemcpyAsync(d_sides + 2 * r->problem, r->sides, sizeof(DevSide) * 2, hipMemcpyHostToDevice, st));
     // Each new site's first closest triangle seeds its paths' queries.
     if (ctx->d_geoms) HIP_OK(launch_slot_hints(tables(0, 0), s0, n, st));
-    // The host vectors above die at return: finish the copies first.
-    HIP_OK(hipStreamSynchronize(st));
+    // A pageable block dies at return: finish its copies first.  The pinned
+    // block stays, and the next admission waits for its copies instead.
+    if (arena) HIP_OK(hipStreamSynchronize(st));
+    else stage_pending = true;
     r->done.assign(n, 0);
     r->state.assign(n, 0);
     r->remaining = n;
@@ -774,6 +782,7 @@ struct Engine {
           if (int(class_key.size()) == kMaxClasses) { full = true; break; }
           class_key.push_back(key);
           by_class.emplace_back();
+          by_class.back().reserve(size_t(r->remaining));
           c = int(class_key.size()) - 1;
         }
         by_class[c].push_back({r, i, inc});
