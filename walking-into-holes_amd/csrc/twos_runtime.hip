@@ -822,7 +822,7 @@ struct Engine {
                            !(r0->prm.flags & TWOS_FLAG_COUNT_TRAVERSAL);
     int kind0 = geom_kind_of(r0->geom);
     for (const Entry &en : by_class[0])
-      if (geom_kind_of(en.r->geom) != kind0) kind0 = kGeomGeneric;
+      if (en.r->geom != r0->geom && geom_kind_of(en.r->geom) != kind0) kind0 = kGeomGeneric;
 
     if (plain) {
       // One launch per site: a one-entry work list, the site's walks one
@@ -926,7 +926,7 @@ struct Engine {
         Request *rc = by_class[c].front().r;
         int kind = geom_kind_of(rc->geom);
         for (const Entry &en : by_class[c])
-          if (geom_kind_of(en.r->geom) != kind) kind = kGeomGeneric;
+          if (en.r->geom != rc->geom && geom_kind_of(en.r->geom) != kind) kind = kGeomGeneric;
         if (rc->prm.flags & TWOS_FLAG_COUNT_TRAVERSAL) HIP_OK(launch_zero(d_counters + kMaxClasses, kCounters, strm));
         TWOS_TRY(ev_record(0, strm));
         st = launch_walks(rc, int(c), e0s[c], totals[c], kind, class_bare(by_class[c]));
