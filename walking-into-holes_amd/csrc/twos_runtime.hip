@@ -557,6 +557,7 @@ struct Engine {
         stage_bytes = need;
       }
       stage = h_stage;
+      stage_pending = true;  // from the first upload on, also if one fails
     }
     size_t used = 0;
     auto carve = [&](size_t bytes) {
@@ -631,7 +632,6 @@ struct Engine {
     // A pageable block dies at return: finish its copies first.  The pinned
     // block stays, and the next admission waits for its copies instead.
     if (arena) HIP_OK(hipStreamSynchronize(st));
-    else stage_pending = true;
     r->done.assign(n, 0);
     r->state.assign(n, 0);
     r->remaining = n;

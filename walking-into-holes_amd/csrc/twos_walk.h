@@ -641,7 +641,7 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(kWav
 // (the walk's first step takes the grid radius, no query) gets no hint: deep
 // inside a mesh that query would test nearly every triangle, and a path's
 // first exact query, far from the site, starts from the grid's own bound.
-// The hint is only a warm start, so no distance or step depends on it.
+// The hint only warm-starts a query: the distance found does not depend on it.
 template <typename R>
 __global__ void slot_hints(Tables tb, int64_t first, int64_t n) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
