@@ -80,6 +80,38 @@ def test_version_and_no_device_error_without_gpu():
         ctx.close()
 
 
+def test_result_records_decode_to_contiguous_fields():
+    """Estimate results arrive as SiteResult records (twos.h): a record array
+    and the same records as raw bytes (the device entry point's read-back)
+    decode to the same contiguous per-site fields, gradient fields included,
+    and zero sites to empty arrays."""
+    import twos
+    n = 37
+    rec = np.zeros(n, dtype=twos.RESULT_DTYPE)
+    assert twos.RESULT_DTYPE.itemsize == twos.RESULT_BYTES
+    rng = np.random.default_rng(4)
+    for k in ("u", "v", "score", "score_se", "td", "same", "grad_dot", "grad_dot_se"):
+        rec[k] = rng.normal(size=n)
+    rec["grad_u"] = rng.normal(size=(n, 3))
+    rec["walks"] = rng.integers(2, 1 << 40, n)
+    rec["steps"] = rng.integers(0, 1 << 50, n)
+    rec["caps"] = rng.integers(0, 9, n)
+    rec["status"] = rng.integers(0, 2, n)
+    a = twos._result(rec, n, {"x": 1}, gradient=True)
+    b = twos.results_from_bytes(rec.tobytes(), n)
+    for k in ("u", "v", "score", "score_se", "td", "same", "walks", "steps", "caps", "status"):
+        assert getattr(a, k).flags["C_CONTIGUOUS"], k
+        np.testing.assert_array_equal(getattr(a, k), rec[k], err_msg=k)
+        np.testing.assert_array_equal(getattr(b, k), rec[k], err_msg=k)
+    np.testing.assert_array_equal(a.grad_u, rec["grad_u"])
+    np.testing.assert_array_equal(a.grad_dot_se, rec["grad_dot_se"])
+    assert a.stats == {"x": 1} and b.grad_u is None
+    rec["u"][0] = 99.0  # the fields are copies, not views of the records
+    assert rec["u"][0] == 99.0 and a.u[0] != 99.0
+    empty = twos._result(twos._result_buffer(0), 0, {})
+    assert len(empty.u) == 0 and len(empty.walks) == 0
+
+
 def test_side_validation():
     import twos
     with pytest.raises(twos.InvalidArgument):
